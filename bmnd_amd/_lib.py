@@ -1,0 +1,76 @@
+"""ctypes binding of libbmnd_hip.so (declarations in include/bmnd_hip.h).
+
+torch is imported first so that the process has exactly one HIP runtime:
+the library's libamdhip64.so.7 dependency resolves to the one torch already
+loaded.  There is no fallback: if the library is missing the import fails
+loudly.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+import torch  # noqa: F401  (must precede the library load)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbmnd_hip.so")
+_LIB: ctypes.CDLL | None = None
+
+EXPORTS = (
+    "bmnd_hip_version",
+    "bmnd_hip_last_error",
+    "bmnd_hip_configs",
+    "bmnd_hip_variance_volume",
+    "bmnd_hip_blockmatch",
+    "bmnd_hip_matrix_table_size",
+    "bmnd_hip_filter_groups",
+    "bmnd_hip_reference_map",
+    "bmnd_hip_aggregate",
+)
+
+
+class HipError(RuntimeError):
+    pass
+
+
+def _declare(L: ctypes.CDLL) -> None:
+    vp, i32, i64, f32, f64 = (ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64,
+                              ctypes.c_float, ctypes.c_double)
+    L.bmnd_hip_version.restype = i32
+    L.bmnd_hip_last_error.restype = ctypes.c_char_p
+    L.bmnd_hip_configs.restype = ctypes.c_char_p
+    L.bmnd_hip_variance_volume.argtypes = [vp, vp, i64, f64, f64, vp]
+    L.bmnd_hip_variance_volume.restype = i32
+    L.bmnd_hip_blockmatch.argtypes = [vp, vp, vp, vp, vp, i64, f32, i32, i32, i32, i32, f64,
+                                      i32, vp, vp, vp, vp]
+    L.bmnd_hip_blockmatch.restype = i32
+    L.bmnd_hip_matrix_table_size.argtypes = [i32, vp]
+    L.bmnd_hip_matrix_table_size.restype = i64
+    L.bmnd_hip_filter_groups.argtypes = [i32, vp, vp, vp, vp, vp, vp, vp, vp, i64, i32, vp, f32,
+                                         i32, f64, vp, vp, vp]
+    L.bmnd_hip_filter_groups.restype = i32
+    L.bmnd_hip_reference_map.argtypes = [vp, i64, vp, i64, vp]
+    L.bmnd_hip_reference_map.restype = i32
+    L.bmnd_hip_aggregate.argtypes = [vp, vp, vp, vp, vp, i64, i32, vp, vp, vp, vp, vp, vp, vp]
+    L.bmnd_hip_aggregate.restype = i32
+
+
+def lib() -> ctypes.CDLL:
+    """Load the HIP library (once).  Raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise HipError(
+                f"{LIB_PATH} is missing: the MI355X kernels are not built. "
+                "Run `python -c 'import __graft_entry__ as g; g.build()'` first.")
+        L = ctypes.CDLL(LIB_PATH)
+        _declare(L)
+        _LIB = L
+    return _LIB
+
+
+def check(status: int) -> None:
+    if status != 0:
+        msg = lib().bmnd_hip_last_error().decode(errors="replace")
+        raise HipError(f"libbmnd_hip call failed ({status}): {msg}")

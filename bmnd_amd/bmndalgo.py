@@ -1,0 +1,121 @@
+"""bmnd(): the drop-in entry point.
+
+Same signature, argument validation order, error messages and return
+conventions as the reference's src/bmnd/bmndalgo.py:71-183; the work runs on
+the MI355X device pipeline (engine.PoissonPipeline).  Profile options
+outside the Poisson default path raise NotImplementedError after the
+reference's own validation has passed.
+"""
+
+from __future__ import annotations
+
+from collections.abc import Callable
+
+import numpy as np
+import torch
+
+from .blockmatching import BlockMatchResult, build_result
+from .enums import CacheMode, coerce_enum
+from .engine import PoissonPipeline, check_supported
+from .profilevalidation import resolve_profile
+
+
+def _validate_sigma(sigma: object) -> float:
+    msg = f"sigma must be finite and non-negative, got {sigma!r}."
+    if isinstance(sigma, (bool, np.bool_)):
+        raise ValueError(msg)
+    try:
+        value = float(sigma)
+    except (TypeError, ValueError) as exc:
+        raise ValueError(msg) from exc
+    if not np.isfinite(value) or value < 0.0:
+        raise ValueError(msg)
+    return value
+
+
+def _validate_max_cache_bytes(max_cache_bytes):
+    """Same contract as the reference (cache.py:65-83).  The device pipeline
+    keeps no covariance caches, so the value has no further effect."""
+    if isinstance(max_cache_bytes, str):
+        max_cache_bytes = coerce_enum(max_cache_bytes, CacheMode, "max_cache_bytes mode")
+    if max_cache_bytes is None or max_cache_bytes is CacheMode.AUTO:
+        return max_cache_bytes
+    if (isinstance(max_cache_bytes, bool) or not isinstance(max_cache_bytes, (int, np.integer))
+            or int(max_cache_bytes) < 0):
+        raise ValueError("max_cache_bytes must be a non-negative integer, 'auto', or None, "
+                         f"got {max_cache_bytes!r}.")
+    return int(max_cache_bytes)
+
+
+_PIPELINES: dict = {}
+
+
+def _pipeline(profile, shape, device) -> PoissonPipeline:
+    key = (repr(profile), tuple(shape), str(device))
+    pipe = _PIPELINES.get(key)
+    if pipe is None:
+        if len(_PIPELINES) > 8:
+            _PIPELINES.clear()
+        pipe = PoissonPipeline(profile, shape, device)
+        _PIPELINES[key] = pipe
+    return pipe
+
+
+def bmnd(volume, profile, sigma: float | None = None, sigma_psd=None,
+         return_sigma_map: bool = False,
+         stage_callback: Callable[[str, np.ndarray], None] | None = None,
+         blockmatch_callback: Callable[[str, BlockMatchResult], None] | None = None,
+         max_cache_bytes=CacheMode.AUTO):
+    """Denoise an N-D Poisson volume (2-D and 3-D on the MI355X path).
+
+    Returns a float32 array of the input shape.  Callbacks receive copies of
+    the HT pilot / Wiener estimate (``stage_callback``) and the block-match
+    results (``blockmatch_callback``), in the reference's order."""
+    if sigma is not None and sigma_psd is not None:
+        raise ValueError("sigma and sigma_psd are mutually exclusive.")
+    if sigma is not None:
+        _validate_sigma(sigma)
+    _validate_max_cache_bytes(max_cache_bytes)
+    resolved = resolve_profile(profile, tuple(int(s) for s in np.shape(volume)))
+    if resolved.is_poisson and sigma_psd is not None:
+        raise ValueError("sigma_psd is only supported for stationary additive noise; "
+                         "direct Poisson mode uses a variance volume instead.")
+    arr = np.asarray(volume)
+    if not np.all(np.isfinite(arr)):
+        raise ValueError("volume must contain only finite values.")
+    with np.errstate(over="ignore", invalid="ignore"):
+        vol = arr.astype(np.float32, copy=False)
+    if not np.all(np.isfinite(vol)):
+        raise ValueError("volume values must be representable as finite float32 values.")
+    if resolved.is_poisson and np.any(vol < 0.0):
+        raise ValueError("Direct Poisson input must contain only non-negative values.")
+    if not np.any(vol != 0.0):
+        if stage_callback is not None:
+            stage_callback("ht", vol.copy())
+            stage_callback("wiener", vol.copy())
+        if return_sigma_map:
+            zeros = np.zeros_like(vol, dtype=np.float32)
+            return vol, {"ht": zeros, "wiener": zeros}
+        return vol
+    check_supported(profile, resolved)
+    if return_sigma_map:
+        raise NotImplementedError("return_sigma_map is not implemented on the MI355X path yet.")
+    vol = np.ascontiguousarray(vol)
+    device = torch.device("cuda", torch.cuda.current_device())
+    pipe = _pipeline(profile, vol.shape, device)
+    vol_t = torch.from_numpy(vol).to(device)
+    ht, wi = pipe.run(vol_t)
+    if blockmatch_callback is not None:
+        counts = tuple(vol.shape[d] - pipe.plans["ht"].block[d] + 1 for d in range(vol.ndim))
+        blockmatch_callback("ht", build_result(ht.schedule, counts, ht.sel.cpu().numpy(),
+                                               ht.counts.cpu().numpy(), ht.accepted.cpu().numpy()))
+    if stage_callback is not None:
+        stage_callback("ht", ht.est.cpu().numpy())
+    if blockmatch_callback is not None:
+        counts = tuple(vol.shape[d] - pipe.plans["wiener"].block[d] + 1 for d in range(vol.ndim))
+        blockmatch_callback("wiener", build_result(wi.schedule, counts, wi.sel.cpu().numpy(),
+                                                   wi.counts.cpu().numpy(), wi.accepted.cpu().numpy()))
+    out = wi.est.cpu().numpy()
+    if stage_callback is not None:
+        stage_callback("wiener", out.copy())
+    return out

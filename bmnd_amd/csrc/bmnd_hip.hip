@@ -1,0 +1,1186 @@
+// MI355X (gfx950) kernels for the Poisson bmnd() hot path.
+//
+// Build: hipcc --offload-arch=gfx950 -O3 -ffp-contract=off -shared -fPIC
+// (-ffp-contract=off keeps every a*b+c unfused unless written as fma(), so the
+// arithmetic order is the one documented next to each kernel).
+//
+// Kernels
+//   k_variance_volume   max(scale*max(v,0), floor)                 HBM stream
+//   k_blockmatch        fused Poisson-deviance matching + top-K      FP64 VALU
+//   k_filter            per-group transforms, exact variance,
+//                       shrinkage, inverse transform, weights       FP32 VALU/LDS
+//   k_reference_map     schedule position -> schedule index          HBM
+//   k_aggregate         ordered per-voxel gather + normalisation     L2/HBM
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/bmnd_hip.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                        \
+  do {                                                                       \
+    hipError_t e_ = (expr);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      return fail(BMND_HIP_ELAUNCH, std::string(#expr) + ": " +             \
+                                        hipGetErrorString(e_));              \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// float64 natural logarithm
+// ---------------------------------------------------------------------------
+// log(x) = e*ln2 + log(1/invc_j) + log1p(m*invc_j - 1) with x = m*2^e,
+// m in [sqrt(1/2), sqrt(2)), j = round(128*(m-1)), invc_j = RN(1/(1+j/128)).
+// The table holds invc_j and -log(invc_j) split hi+lo (computed on the host
+// in extended precision).  The j = 0 entry is exact (invc = 1, log = 0), so
+// arguments near 1 keep full relative accuracy.  log1p uses its Taylor
+// series to degree 8 (|r| < 2^-7.4); the hi parts are summed with
+// error-free transformations.  Measured against the C library's log on the
+// host in the tests.
+constexpr int kLogTabLo = -38;
+constexpr int kLogTabN = 92;  // j in [-38, 53]
+constexpr int kLogTabDoubles = 3 * kLogTabN + 2;  // + ln2 hi/lo
+
+__device__ __forceinline__ double two_sum_err(double a, double b, double s) {
+  double bb = s - a;
+  return (a - (s - bb)) + (b - bb);
+}
+
+__device__ __forceinline__ double dlog(double x, const double *__restrict__ tab) {
+  int e;
+  double m = frexp(x, &e);  // m in [0.5, 1)
+  if (m < 0.70710678118654752440) {
+    m = m * 2.0;
+    e -= 1;
+  }
+  int j = (int)rint((m - 1.0) * 128.0) - kLogTabLo;
+  j = min(max(j, 0), kLogTabN - 1);  // only x <= 0 or NaN leave the range
+  double invc = tab[3 * j], thi = tab[3 * j + 1], tlo = tab[3 * j + 2];
+  double r = fma(m, invc, -1.0);
+  double p = -0.125;
+  p = fma(p, r, 1.0 / 7.0);
+  p = fma(p, r, -1.0 / 6.0);
+  p = fma(p, r, 0.2);
+  p = fma(p, r, -0.25);
+  p = fma(p, r, 1.0 / 3.0);
+  p = fma(p, r, -0.5);
+  double ed = (double)e;
+  double a = ed * tab[3 * kLogTabN];  // exact: ln2_hi has 32 significant bits
+  double s1 = a + thi;
+  double e1 = two_sum_err(a, thi, s1);
+  double s2 = s1 + r;
+  double e2 = two_sum_err(s1, r, s2);
+  double lo = ((e1 + e2) + (ed * tab[3 * kLogTabN + 1] + tlo)) + (r * r) * p;
+  return s2 + lo;
+}
+
+// tie grid of blockmatching.py:957-964: q = 32*eps32*2^floor(log2(max(|d|,1)))
+__device__ __forceinline__ float canonical_key(float d) {
+  double mag = fabs((double)d);
+  if (mag < 1.0) mag = 1.0;
+  int e;
+  frexp(mag, &e);  // mag = f*2^e, f in [0.5,1) -> floor(log2 mag) = e-1
+  double q = ldexp(0x1p-18, e - 1);
+  return (float)(rint((double)d / q) * q);
+}
+
+// ---------------------------------------------------------------------------
+// small block-level helpers
+// ---------------------------------------------------------------------------
+template <int NT>
+__device__ int block_exclusive_scan(int v, int *s_tmp, int *total) {
+  // s_tmp needs NT/64 + 1 ints
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_tmp[wid] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int w = 0; w < NT / 64; ++w) {
+      int t = s_tmp[w];
+      s_tmp[w] = run;
+      run += t;
+    }
+    s_tmp[NT / 64] = run;
+  }
+  __syncthreads();
+  int res = x - v + s_tmp[wid];
+  *total = s_tmp[NT / 64];
+  __syncthreads();
+  return res;
+}
+
+// ---------------------------------------------------------------------------
+// variance volume (variance.py:556-598)
+// ---------------------------------------------------------------------------
+__global__ void k_variance_volume(const float *__restrict__ v, float *__restrict__ out,
+                                  int64_t n, float scale, float floor_) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    float x = v[i];
+    x = x > 0.0f ? x : 0.0f;
+    x = scale * x;
+    out[i] = x > floor_ ? x : floor_;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// block matching
+// ---------------------------------------------------------------------------
+struct BMArgs {
+  const float *vol;
+  int n0, n1, n2;        // volume dims
+  int c0, c1, c2;        // patch-grid counts
+  const int32_t *ref_lin;
+  int64_t nrefs;
+  float threshold;
+  float thr_key;
+  int max_m, min_m, pow2, stabilize;
+  double inv_scale;
+  const double *logtab;
+  int32_t *sel, *counts, *accepted;
+};
+
+constexpr int kBMThreads = 512;
+constexpr int kMaxStack = 64;
+
+template <int B0, int B1, int B2, int R0, int R1, int R2, bool EXACT>
+__global__ __launch_bounds__(kBMThreads) void k_blockmatch(BMArgs a) {
+  constexpr int S0 = 2 * R0 + B0, S1 = 2 * R1 + B1, S2 = 2 * R2 + B2;
+  constexpr int SLAB = S0 * S1 * S2;
+  constexpr int BOX = (2 * R0 + 1) * (2 * R1 + 1) * (2 * R2 + 1);
+  constexpr int P = B0 * B1 * B2;
+  __shared__ float s_slab[SLAB];
+  __shared__ double s_ref[P];
+  __shared__ double s_ref_log[P];
+  __shared__ unsigned long long s_keys[BOX];
+  __shared__ double s_tab[kLogTabDoubles];
+  __shared__ int s_hist[256];
+  __shared__ unsigned long long s_small[kMaxStack];
+  __shared__ unsigned long long s_sorted[kMaxStack];
+  __shared__ unsigned long long s_prefix;
+  __shared__ int s_krem, s_cnt, s_acc;
+
+  const int tid = threadIdx.x;
+  const int64_t g = blockIdx.x;
+  if (g >= a.nrefs) return;
+  const int lin = a.ref_lin[g];
+  const int o2 = lin % a.c2, o1 = (lin / a.c2) % a.c1, o0 = lin / (a.c2 * a.c1);
+  const int lo0 = max(o0 - R0, 0), hi0 = min(o0 + R0, a.c0 - 1);
+  const int lo1 = max(o1 - R1, 0), hi1 = min(o1 + R1, a.c1 - 1);
+  const int lo2 = max(o2 - R2, 0), hi2 = min(o2 + R2, a.c2 - 1);
+  const int e0 = hi0 - lo0 + 1, e1 = hi1 - lo1 + 1, e2 = hi2 - lo2 + 1;
+  const int box = e0 * e1 * e2;
+
+  for (int i = tid; i < kLogTabDoubles; i += kBMThreads) s_tab[i] = a.logtab[i];
+  // slab: volume region [lo, hi + B - 1] (always inside the volume)
+  const int x0 = e0 + B0 - 1, x1 = e1 + B1 - 1, x2 = e2 + B2 - 1;
+  for (int i = tid; i < SLAB; i += kBMThreads) {
+    int i2 = i % S2, i1 = (i / S2) % S1, i0 = i / (S2 * S1);
+    float v = 0.0f;
+    if (i0 < x0 && i1 < x1 && i2 < x2)
+      v = a.vol[((int64_t)(lo0 + i0) * a.n1 + (lo1 + i1)) * a.n2 + (lo2 + i2)];
+    s_slab[i] = v;
+  }
+  for (int p = tid; p < P; p += kBMThreads) {
+    int p2 = p % B2, p1 = (p / B2) % B1, p0 = p / (B2 * B1);
+    float v = a.vol[((int64_t)(o0 + p0) * a.n1 + (o1 + p1)) * a.n2 + (o2 + p2)];
+    double r = (double)v * a.inv_scale;
+    if (r < 0.0) r = 0.0;
+    s_ref[p] = r;
+  }
+  if (tid == 0) s_acc = 0;
+  __syncthreads();
+  if (!EXACT) {
+    for (int p = tid; p < P; p += kBMThreads) {
+      double r = s_ref[p];
+      s_ref_log[p] = r > 0.0 ? dlog(2.0 * r, s_tab) : 0.0;
+    }
+    __syncthreads();
+  }
+
+  const int rel0 = o0 - lo0, rel1 = o1 - lo1, rel2 = o2 - lo2;
+  const int center_local = (rel0 * e1 + rel1) * e2 + rel2;
+  int my_acc = 0;
+  for (int cand = tid; cand < box; cand += kBMThreads) {
+    const int j2 = cand % e2, j1 = (cand / e2) % e1, j0 = cand / (e2 * e1);
+    const int base = (j0 * S1 + j1) * S2 + j2;
+    double d = 0.0;
+    // terms are accumulated in patch C order (p0, p1, p2), as the reference does
+#pragma unroll 1
+    for (int p01 = 0; p01 < B0 * B1; ++p01) {
+      const int p0 = p01 / B1, p1 = p01 - p0 * B1;
+      const float *row = s_slab + base + (p0 * S1 + p1) * S2;
+      const double *rrow = s_ref + p01 * B2;
+      const double *lrow = s_ref_log + p01 * B2;
+#pragma unroll
+        for (int p2 = 0; p2 < B2; ++p2) {
+          double c = (double)row[p2] * a.inv_scale;
+          if (c < 0.0) c = 0.0;
+          const double r = rrow[p2];
+          const double s = r + c;
+          double t;
+          if (EXACT) {
+            // blockmatching.py:347-355, term by term
+            const double lr = dlog((2.0 * r) / s, s_tab);
+            const double lc = dlog((2.0 * c) / s, s_tab);
+            t = 0.0;
+            if (r > 0.0) t = t + r * lr;
+            t = (c > 0.0) ? t + c * lc : t;
+          } else {
+            const double ls = dlog(s, s_tab);
+            const double lc = dlog(2.0 * c, s_tab);
+            t = 0.0;
+            if (r > 0.0) t = t + r * (lrow[p2] - ls);
+            t = (c > 0.0) ? t + c * (lc - ls) : t;
+          }
+          d = (s > 0.0) ? d + 2.0 * t : d;
+        }
+    }
+    if (d < 0.0) d = 0.0;
+    const float d32 = (float)(d / (double)P);
+    float key;
+    bool ok;
+    if (a.stabilize) {
+      key = canonical_key(d32);
+      ok = key < a.thr_key;
+    } else {
+      key = d32;
+      ok = d32 < a.threshold;
+    }
+    my_acc += ok ? 1 : 0;
+    const unsigned long long comp = ((unsigned long long)(ok ? 0u : 1u) << 63) |
+                                    ((unsigned long long)__float_as_uint(key) << 32) |
+                                    (unsigned long long)cand;
+    s_keys[cand] = comp;
+  }
+  if (my_acc) atomicAdd(&s_acc, my_acc);
+  __syncthreads();
+
+  // how many of the smallest composites the reference would keep
+  const int acc = s_acc;
+  const int req = min(a.min_m, a.max_m);
+  int K;
+  if (acc >= a.max_m)
+    K = a.max_m;
+  else if (acc >= req)
+    K = acc;
+  else
+    K = min(req, box);
+
+  // radix select of the K-th smallest composite key (keys are unique)
+  unsigned long long prefix = 0ull, mask = 0ull;
+  int krem = K;
+  for (int shift = 56; shift >= 0; shift -= 8) {
+    for (int i = tid; i < 256; i += kBMThreads) s_hist[i] = 0;
+    __syncthreads();
+    for (int i = tid; i < box; i += kBMThreads) {
+      unsigned long long k = s_keys[i];
+      if ((k & mask) == prefix) atomicAdd(&s_hist[(int)((k >> shift) & 255ull)], 1);
+    }
+    __syncthreads();
+    if (tid < 64) {
+      int c0 = s_hist[4 * tid], c1 = s_hist[4 * tid + 1], c2 = s_hist[4 * tid + 2],
+          c3 = s_hist[4 * tid + 3];
+      int tot = c0 + c1 + c2 + c3, incl = tot;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(incl, o, 64);
+        if (tid >= o) incl += y;
+      }
+      int ex = incl - tot;
+      int cs[4] = {c0, c1, c2, c3};
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        if (ex < krem && ex + cs[b] >= krem) {
+          s_prefix = prefix | ((unsigned long long)(4 * tid + b) << shift);
+          s_krem = krem - ex;
+        }
+        ex += cs[b];
+      }
+    }
+    __syncthreads();
+    prefix = s_prefix;
+    krem = s_krem;
+    mask |= 255ull << shift;
+    __syncthreads();
+  }
+  if (tid == 0) s_cnt = 0;
+  __syncthreads();
+  for (int i = tid; i < box; i += kBMThreads) {
+    unsigned long long k = s_keys[i];
+    if (k <= prefix) s_small[atomicAdd(&s_cnt, 1)] = k;
+  }
+  __syncthreads();
+  if (tid < K) {
+    unsigned long long mine = s_small[tid];
+    int rank = 0;
+    for (int j = 0; j < K; ++j) rank += s_small[j] < mine ? 1 : 0;
+    s_sorted[rank] = mine;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    // reference first, dedupe, cap, power-of-two truncation (blockmatching.py:913-954)
+    int out[kMaxStack];
+    int cnt = 0;
+    out[cnt++] = center_local;
+    for (int i = 0; i < K && cnt < a.max_m; ++i) {
+      int loc = (int)(s_sorted[i] & 0xffffffffull);
+      if (loc == center_local) continue;
+      bool seen = false;
+      for (int j = 0; j < cnt; ++j) seen |= (out[j] == loc);
+      if (!seen) out[cnt++] = loc;
+    }
+    if (a.pow2 && cnt > 0) {
+      int t = 1;
+      while ((t << 1) <= cnt) t <<= 1;
+      if (a.min_m > t) t = a.min_m;
+      if (t < cnt) cnt = t;
+    }
+    int32_t *dst = a.sel + g * a.max_m;
+    for (int i = 0; i < a.max_m; ++i) {
+      int v = -1;
+      if (i < cnt) {
+        int loc = out[i];
+        int l2 = loc % e2, l1 = (loc / e2) % e1, l0 = loc / (e2 * e1);
+        v = ((lo0 + l0) * a.c1 + (lo1 + l1)) * a.c2 + (lo2 + l2);
+      }
+      dst[i] = v;
+    }
+    a.counts[g] = cnt;
+    if (a.accepted) a.accepted[g] = acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// group filtering
+// ---------------------------------------------------------------------------
+struct FilterArgs {
+  const float *noisy, *pilot, *var;
+  int n0, n1, n2;
+  int c0, c1, c2;
+  const int32_t *sel, *counts;
+  int64_t nrefs;
+  int max_m;
+  const float *mats;
+  float shrink;
+  int k;
+  double count_scale;
+  float *filt, *weights;
+};
+
+// apply M (n x n, out-major) along one axis of an LDS array of dims (d0,d1,d2,d3)
+// as fmaf chains in input order; src -> dst
+template <int NT>
+__device__ void apply_axis(const float *__restrict__ src, float *__restrict__ dst,
+                           int d0, int d1, int d2, int d3, int axis,
+                           const float *__restrict__ M) {
+  const int total = d0 * d1 * d2 * d3;
+  const int dims[4] = {d0, d1, d2, d3};
+  int inner = 1;
+  for (int d = axis + 1; d < 4; ++d) inner *= dims[d];
+  const int n = dims[axis];
+  for (int o = threadIdx.x; o < total; o += NT) {
+    const int in_i = o % inner;
+    const int r = (o / inner) % n;
+    const int outer = o / (inner * n);
+    const float *x = src + outer * n * inner + in_i;
+    const float *m = M + r * n;
+    float acc = 0.0f;
+    for (int kk = 0; kk < n; ++kk) acc = fmaf(m[kk], x[kk * inner], acc);
+    dst[o] = acc;
+  }
+}
+
+template <int B0, int B1, int B2, int R0, int R1, int R2, int NMAX, int NT, bool WIENER>
+__global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
+  constexpr int P = B0 * B1 * B2;
+  constexpr int NP = NMAX * P;
+  constexpr int SORTN = (NP <= 512) ? 512 : (NP <= 1024) ? 1024 : (NP <= 2048) ? 2048 : 4096;
+  static_assert(NP <= 4096, "group too large");
+  static_assert(P <= 128, "patch too large for the entry encoding");
+  __shared__ float s_x[NP];      // coefficients being filtered
+  __shared__ float s_t[NP];      // transform temporary
+  __shared__ float s_sig[NP];    // noise std
+  __shared__ float s_r[WIENER ? NP : 1];
+  __shared__ unsigned s_key[SORTN];
+  __shared__ unsigned short s_seg[NP + 1];
+  __shared__ float s_segvar[NP];
+  __shared__ float s_g[NMAX * NMAX], s_gi[NMAX * NMAX], s_g2[NMAX * NMAX];
+  __shared__ float s_sf[3][64], s_si[3][64], s_sf2[3][64];
+  __shared__ int s_pos[NMAX][3];
+  __shared__ int s_min[3], s_ushape[3];
+  __shared__ int s_scan[NT / 64 + 1];
+  __shared__ double s_plane[NMAX];
+
+  const int tid = threadIdx.x;
+  const int64_t g = blockIdx.x;
+  if (g >= a.nrefs) return;
+  const int n = a.counts[g];
+  if (n <= 0) return;
+  const int planes = min(a.k, n);
+
+  // ---- matrices ----------------------------------------------------------
+  const int64_t goff = (int64_t)(n - 1) * n * (2 * n - 1) / 6;  // sum_{m<n} m^2
+  const int64_t gtot = (int64_t)a.max_m * (a.max_m + 1) * (2 * a.max_m + 1) / 6;
+  const float *gf = a.mats + goff;
+  const float *gi = a.mats + gtot + goff;
+  const float *sp = a.mats + 2 * gtot;
+  for (int i = tid; i < n * n; i += NT) {
+    float v = gf[i];
+    s_g[i] = v;
+    s_g2[i] = v * v;
+    s_gi[i] = gi[i];
+  }
+  {
+    const int bs[3] = {B0, B1, B2};
+    int o = 0;
+    for (int d = 0; d < 3; ++d) {
+      for (int i = tid; i < bs[d] * bs[d]; i += NT) {
+        float v = sp[o + i];
+        s_sf[d][i] = v;
+        s_sf2[d][i] = v * v;
+      }
+      o += bs[d] * bs[d];
+    }
+    for (int d = 0; d < 3; ++d) {
+      for (int i = tid; i < bs[d] * bs[d]; i += NT) s_si[d][i] = sp[o + i];
+      o += bs[d] * bs[d];
+    }
+  }
+  // ---- member positions ---------------------------------------------------
+  if (tid < n) {
+    int lin = a.sel[g * a.max_m + tid];
+    s_pos[tid][2] = lin % a.c2;
+    s_pos[tid][1] = (lin / a.c2) % a.c1;
+    s_pos[tid][0] = lin / (a.c2 * a.c1);
+  }
+  __syncthreads();
+  if (tid < 3) {
+    int mn = s_pos[0][tid], mx = s_pos[0][tid];
+    for (int i = 1; i < n; ++i) {
+      mn = min(mn, s_pos[i][tid]);
+      mx = max(mx, s_pos[i][tid]);
+    }
+    s_min[tid] = mn;
+    const int bs[3] = {B0, B1, B2};
+    s_ushape[tid] = mx - mn + bs[tid];
+  }
+  __syncthreads();
+  const int np = n * P;
+  // ---- gather: variance patches (approximate planes) ------------------------
+  for (int e = tid; e < np; e += NT) {
+    const int i = e / P, p = e - i * P;
+    const int p2 = p % B2, p1 = (p / B2) % B1, p0 = p / (B2 * B1);
+    const int64_t v = ((int64_t)(s_pos[i][0] + p0) * a.n1 + (s_pos[i][1] + p1)) * a.n2 +
+                      (s_pos[i][2] + p2);
+    float x = a.var[v];
+    s_x[e] = x > 0.0f ? x : 0.0f;
+  }
+  __syncthreads();
+  // approximate std: squared transforms along every axis (variance.py:613-623)
+  apply_axis<NT>(s_x, s_t, n, B0, B1, B2, 0, s_g2);
+  __syncthreads();
+  const float *cur = s_t;
+  float *nxt = s_x;
+  if (B0 > 1) {
+    apply_axis<NT>(cur, nxt, n, B0, B1, B2, 1, s_sf2[0]);
+    __syncthreads();
+    float *tmp = (float *)cur;
+    cur = nxt;
+    nxt = tmp;
+  }
+  apply_axis<NT>(cur, nxt, n, B0, B1, B2, 2, s_sf2[1]);
+  __syncthreads();
+  apply_axis<NT>(nxt, (float *)cur, n, B0, B1, B2, 3, s_sf2[2]);
+  __syncthreads();
+  for (int e = tid; e < np; e += NT) {
+    float v = cur[e];
+    s_sig[e] = sqrtf(v > 0.0f ? v : 0.0f);
+  }
+  __syncthreads();
+
+  // ---- exact overlap-aware planes (variance.py:652-975) --------------------
+  // entries sorted by (union label, entry index): a stable order in which the
+  // entries of one union voxel follow increasing member index
+  const int u0 = s_ushape[0], u1 = s_ushape[1], u2 = s_ushape[2];
+  for (int e = tid; e < SORTN; e += NT) {
+    unsigned key = 0xffffffffu;
+    if (e < np) {
+      const int i = e / P, p = e - i * P;
+      const int p2 = p % B2, p1 = (p / B2) % B1, p0 = p / (B2 * B1);
+      const int l0 = s_pos[i][0] - s_min[0] + p0;
+      const int l1 = s_pos[i][1] - s_min[1] + p1;
+      const int l2 = s_pos[i][2] - s_min[2] + p2;
+      const unsigned label = (unsigned)((l0 * u1 + l1) * u2 + l2);
+      key = (label << 12) | (unsigned)((i << 7) | p);
+    }
+    s_key[e] = key;
+  }
+  __syncthreads();
+  for (int size = 2; size <= SORTN; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int t = tid; t < SORTN / 2; t += NT) {
+        const int lo = 2 * t - (t & (stride - 1));
+        const int hi = lo + stride;
+        const bool up = ((lo & size) == 0);
+        unsigned x = s_key[lo], y = s_key[hi];
+        if ((x > y) == up) {
+          s_key[lo] = y;
+          s_key[hi] = x;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // segment starts
+  int nseg = 0;
+  {
+    int base = 0;
+    for (int chunk = 0; chunk < np; chunk += NT) {
+      const int e = chunk + tid;
+      int flag = 0;
+      if (e < np) flag = (e == 0) || ((s_key[e] >> 12) != (s_key[e - 1] >> 12));
+      int tot;
+      int ex = block_exclusive_scan<NT>(flag, s_scan, &tot);
+      if (flag) {
+        const int sidx = base + ex;
+        s_seg[sidx] = (unsigned short)e;
+        const unsigned label = s_key[e] >> 12;
+        const int l2 = label % u2, l1 = (label / u2) % u1, l0 = label / (u2 * u1);
+        const int64_t v = ((int64_t)(l0 + s_min[0]) * a.n1 + (l1 + s_min[1])) * a.n2 +
+                          (l2 + s_min[2]);
+        s_segvar[sidx] = a.var[v];
+      }
+      base += tot;
+    }
+    nseg = base;
+    if (tid == 0) s_seg[nseg] = (unsigned short)np;
+  }
+  __syncthreads();
+  // exact variance of planes q < planes; serial float32 reduction per coefficient
+  float *exact = s_t;  // planes*P values
+  for (int c = tid; c < planes * P; c += NT) {
+    const int q = c / P, s = c - q * P;
+    const int s2 = s % B2, s1 = (s / B2) % B1, s0 = s / (B2 * B1);
+    const float *grow = s_g + q * n;
+    float acc = 0.0f;
+    for (int u = 0; u < nseg; ++u) {
+      const int beg = s_seg[u], end = s_seg[u + 1];
+      float contrib = 0.0f;
+      for (int pos = beg; pos < end; ++pos) {
+        const unsigned ent = s_key[pos] & 0xfffu;
+        const int i = (int)(ent >> 7), p = (int)(ent & 127u);
+        const int p2 = p % B2, p1 = (p / B2) % B1, p0 = p / (B2 * B1);
+        const float sv = (s_sf[0][s0 * B0 + p0] * s_sf[1][s1 * B1 + p1]) * s_sf[2][s2 * B2 + p2];
+        contrib = contrib + grow[i] * sv;
+      }
+      const float vv = s_segvar[u];
+      acc = acc + (vv * contrib) * contrib;
+    }
+    exact[c] = sqrtf(acc > 0.0f ? acc : 0.0f);
+  }
+  __syncthreads();
+  // plane limiting (variance.py:1572-1631), float32 sequential plane sums
+  if (planes == 0) {
+    // k = 0: the approximate std is the answer (variance.py:1647-1653)
+  } else if (planes == n) {
+    for (int e = tid; e < np; e += NT) {
+      s_sig[e] = exact[e];  // every plane exact: the std itself (variance.py:1698)
+    }
+  } else {
+    for (int s = tid; s < P; s += NT) {
+      float tot_a = 0.0f, tot_e = 0.0f, rem_a = 0.0f;
+      for (int q = 0; q < n; ++q) {
+        const float av = s_sig[q * P + s];
+        tot_a = tot_a + av * av;
+        if (q >= planes) rem_a = rem_a + av * av;
+      }
+      for (int q = 0; q < planes; ++q) {
+        const float ev = exact[q * P + s];
+        tot_e = tot_e + ev * ev;
+      }
+      float target = tot_a - tot_e;
+      if (target < 0.0f) target = 0.0f;
+      const float scale = rem_a > 0.0f ? target / rem_a : 0.0f;
+      const float fallback = target / (float)(n - planes);
+      for (int q = 0; q < n; ++q) {
+        float v;
+        if (q < planes) {
+          const float ev = exact[q * P + s];
+          v = ev * ev;
+        } else {
+          const float av = s_sig[q * P + s];
+          v = (av * av) * scale;
+          if (!(rem_a > 0.0f)) v = fallback;
+        }
+        s_sig[q * P + s] = sqrtf(v > 0.0f ? v : 0.0f);
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- coefficients ----------------------------------------------------------
+  auto forward = [&](const float *__restrict__ vol, float *buf) {
+    for (int e = tid; e < np; e += NT) {
+      const int i = e / P, p = e - i * P;
+      const int p2 = p % B2, p1 = (p / B2) % B1, p0 = p / (B2 * B1);
+      const int64_t v = ((int64_t)(s_pos[i][0] + p0) * a.n1 + (s_pos[i][1] + p1)) * a.n2 +
+                        (s_pos[i][2] + p2);
+      buf[e] = vol[v];
+    }
+    __syncthreads();
+    apply_axis<NT>(buf, s_t, n, B0, B1, B2, 0, s_g);
+    __syncthreads();
+    if (B0 > 1) {
+      apply_axis<NT>(s_t, buf, n, B0, B1, B2, 1, s_sf[0]);
+      __syncthreads();
+      apply_axis<NT>(buf, s_t, n, B0, B1, B2, 2, s_sf[1]);
+    } else {
+      apply_axis<NT>(s_t, buf, n, B0, B1, B2, 2, s_sf[1]);
+      __syncthreads();
+      for (int e = tid; e < np; e += NT) s_t[e] = buf[e];
+    }
+    __syncthreads();
+    apply_axis<NT>(s_t, buf, n, B0, B1, B2, 3, s_sf[2]);
+    __syncthreads();
+  };
+  float *atten = s_t;  // reused after the transforms
+  forward(a.noisy, s_x);
+  if (WIENER) forward(a.pilot, s_r);
+  if (!WIENER) {
+    // Poisson hard threshold (poisson.py:30-46, bmndalgo.py:460-485)
+    const float lam = a.shrink;
+    const double cs = a.count_scale;
+    for (int e = tid; e < np; e += NT) {
+      const float thr = lam * s_sig[e];
+      const float c = s_x[e];
+      const double cc = (double)fabsf(c) / cs;
+      const double tc = (double)thr / cs;
+      double sc = cc > tc ? cc : tc;
+      if (sc < 1.0) sc = 1.0;
+      const double tol = 0x1p-16 * sc;
+      const float keep = (cc > tc + tol) ? 1.0f : 0.0f;
+      atten[e] = keep;
+      s_x[e] = c * keep;
+    }
+  } else {
+    // Poisson Wiener gain, variance-scaled (wiener.py:24-53, 78-95)
+    const float mu = a.shrink;
+    const float eps32 = 1e-10f;
+    for (int e = tid; e < np; e += NT) {
+      const float sg = s_sig[e];
+      const float nv = sg * sg;
+      const float eff = mu * nv;
+      const float rc = fabsf(s_r[e]);
+      const float pr = rc * rc;
+      float sp_ = pr - eff;
+      if (sp_ < 0.0f) sp_ = 0.0f;
+      float gain = sp_ / ((sp_ + eff) + eps32);
+      gain = gain < 0.0f ? 0.0f : (gain > 1.0f ? 1.0f : gain);
+      atten[e] = gain;
+      s_x[e] = s_x[e] * gain;
+    }
+  }
+  __syncthreads();
+  // ---- per-patch weights (aggregationweights.py:213-259, 616-653) ------------
+  if (tid < n) {
+    double risk = 0.0;
+    for (int c = 0; c < P; ++c) {
+      const double at = (double)atten[tid * P + c];
+      const float sg = s_sig[tid * P + c];
+      const float nv = sg * sg;
+      double err = at * at;
+      err = err * (double)nv;
+      risk = risk + err;
+    }
+    s_plane[tid] = risk;
+  }
+  __syncthreads();
+  if (tid < n) {
+    double risk = 0.0;
+    for (int j = 0; j < n; ++j) {
+      const double m = (double)s_gi[tid * n + j];
+      risk = risk + (m * m) * s_plane[j];
+    }
+    float w = 0.0f;
+    if (isfinite(risk) && risk >= 0.0) w = (float)(1.0 / (risk + 1e-10));
+    a.weights[g * a.max_m + tid] = w;
+  }
+  __syncthreads();
+  // ---- inverse transform and store -------------------------------------------
+  apply_axis<NT>(s_x, s_t, n, B0, B1, B2, 0, s_gi);
+  __syncthreads();
+  if (B0 > 1) {
+    apply_axis<NT>(s_t, s_x, n, B0, B1, B2, 1, s_si[0]);
+    __syncthreads();
+    apply_axis<NT>(s_x, s_t, n, B0, B1, B2, 2, s_si[1]);
+  } else {
+    apply_axis<NT>(s_t, s_x, n, B0, B1, B2, 2, s_si[1]);
+    __syncthreads();
+    for (int e = tid; e < np; e += NT) s_t[e] = s_x[e];
+  }
+  __syncthreads();
+  apply_axis<NT>(s_t, s_x, n, B0, B1, B2, 3, s_si[2]);
+  __syncthreads();
+  float *dst = a.filt + (g * a.max_m) * (int64_t)P;
+  for (int e = tid; e < np; e += NT) dst[e] = s_x[e];
+}
+
+// ---------------------------------------------------------------------------
+// aggregation
+// ---------------------------------------------------------------------------
+__global__ void k_reference_map(const int32_t *__restrict__ ref_lin, int64_t nrefs,
+                                int32_t *__restrict__ ref_index) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < nrefs) ref_index[ref_lin[g]] = (int32_t)g;
+}
+
+struct AggArgs {
+  const float *filt, *weights;
+  const int32_t *sel, *counts, *ref_index;
+  int64_t nrefs;
+  int max_m;
+  int n0, n1, n2;
+  int c0, c1, c2;
+  const float *kaiser, *fallback;
+  float *est;
+  int *overflow;
+};
+
+constexpr int kAggThreads = 512;
+constexpr int kAggGroupCap = 4096;
+constexpr int kAggEntryCap = 2048;
+
+template <int B0, int B1, int B2, int R0, int R1, int R2, int T0, int T1, int T2>
+__global__ __launch_bounds__(kAggThreads) void k_aggregate(AggArgs a) {
+  static_assert(T0 * T1 * T2 == kAggThreads, "tile must match the block");
+  constexpr int P = B0 * B1 * B2;
+  __shared__ int s_groups[kAggGroupCap];
+  __shared__ int s_ent_idx[kAggEntryCap];
+  __shared__ int s_ent_pos[kAggEntryCap];
+  __shared__ float s_k[P];
+  __shared__ int s_gcount, s_take, s_nent;
+  __shared__ int s_scan[kAggThreads / 64 + 1];
+
+  const int tid = threadIdx.x;
+  const int nt2 = (a.n2 + T2 - 1) / T2, nt1 = (a.n1 + T1 - 1) / T1;
+  const int tb = blockIdx.x;
+  const int tx2 = tb % nt2, tx1 = (tb / nt2) % nt1, tx0 = tb / (nt2 * nt1);
+  const int t_lo0 = tx0 * T0, t_lo1 = tx1 * T1, t_lo2 = tx2 * T2;
+  const int t_hi0 = min(t_lo0 + T0, a.n0) - 1, t_hi1 = min(t_lo1 + T1, a.n1) - 1,
+            t_hi2 = min(t_lo2 + T2, a.n2) - 1;
+  for (int p = tid; p < P; p += kAggThreads) s_k[p] = a.kaiser[p];
+  if (tid == 0) s_gcount = 0;
+  __syncthreads();
+  // references whose group can reach this tile
+  const int r_lo0 = max(t_lo0 - B0 + 1 - R0, 0), r_hi0 = min(t_hi0 + R0, a.c0 - 1);
+  const int r_lo1 = max(t_lo1 - B1 + 1 - R1, 0), r_hi1 = min(t_hi1 + R1, a.c1 - 1);
+  const int r_lo2 = max(t_lo2 - B2 + 1 - R2, 0), r_hi2 = min(t_hi2 + R2, a.c2 - 1);
+  const int re0 = r_hi0 - r_lo0 + 1, re1 = r_hi1 - r_lo1 + 1, re2 = r_hi2 - r_lo2 + 1;
+  const int reach = (re0 > 0 && re1 > 0 && re2 > 0) ? re0 * re1 * re2 : 0;
+  for (int i = tid; i < reach; i += kAggThreads) {
+    const int i2 = i % re2, i1 = (i / re2) % re1, i0 = i / (re2 * re1);
+    const int lin = ((r_lo0 + i0) * a.c1 + (r_lo1 + i1)) * a.c2 + (r_lo2 + i2);
+    const int gg = a.ref_index[lin];
+    if (gg >= 0) {
+      const int slot = atomicAdd(&s_gcount, 1);
+      if (slot < kAggGroupCap) s_groups[slot] = gg;
+    }
+  }
+  __syncthreads();
+  int gcount = s_gcount;
+  if (gcount > kAggGroupCap) {
+    if (tid == 0) atomicExch(a.overflow, 1);
+    gcount = kAggGroupCap;
+  }
+  int sortn = 1;
+  while (sortn < gcount) sortn <<= 1;
+  for (int i = gcount + tid; i < sortn; i += kAggThreads) s_groups[i] = 0x7fffffff;
+  __syncthreads();
+  for (int size = 2; size <= sortn; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int t = tid; t < sortn / 2; t += kAggThreads) {
+        const int lo = 2 * t - (t & (stride - 1));
+        const int hi = lo + stride;
+        const bool up = ((lo & size) == 0);
+        const int x = s_groups[lo], y = s_groups[hi];
+        if ((x > y) == up) {
+          s_groups[lo] = y;
+          s_groups[hi] = x;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // this thread's voxel
+  const int v2 = tid % T2, v1 = (tid / T2) % T1, v0 = tid / (T2 * T1);
+  const int x0 = t_lo0 + v0, x1 = t_lo1 + v1, x2 = t_lo2 + v2;
+  const bool inside = x0 <= t_hi0 && x1 <= t_hi1 && x2 <= t_hi2;
+  float num = 0.0f, den = 0.0f;
+  int j0 = 0;
+  while (j0 < gcount) {
+    // members of groups j0.. that intersect the tile, in (group, member) order
+    int mcount = 0;
+    int gsel = -1;
+    const int j = j0 + tid;
+    if (j < gcount) {
+      gsel = s_groups[j];
+      const int cnt = a.counts[gsel];
+      for (int i = 0; i < cnt; ++i) {
+        const int lin = a.sel[(int64_t)gsel * a.max_m + i];
+        const int m2 = lin % a.c2, m1 = (lin / a.c2) % a.c1, m0 = lin / (a.c2 * a.c1);
+        const bool hit = m0 <= t_hi0 && m0 + B0 - 1 >= t_lo0 && m1 <= t_hi1 &&
+                         m1 + B1 - 1 >= t_lo1 && m2 <= t_hi2 && m2 + B2 - 1 >= t_lo2;
+        mcount += hit ? 1 : 0;
+      }
+    }
+    int total;
+    const int ex = block_exclusive_scan<kAggThreads>(mcount, s_scan, &total);
+    // largest prefix of groups whose entries fit
+    const bool fits = (j < gcount) && (ex + mcount <= kAggEntryCap);
+    if (tid == 0) s_take = 0;
+    __syncthreads();
+    if (fits) atomicMax(&s_take, tid + 1);
+    __syncthreads();
+    const int take = s_take;  // groups j0 .. j0+take-1 (a prefix, since ex is monotone)
+    int nent = 0;
+    if (tid < take) {
+      const int cnt = a.counts[gsel];
+      int w = ex;
+      for (int i = 0; i < cnt; ++i) {
+        const int lin = a.sel[(int64_t)gsel * a.max_m + i];
+        const int m2 = lin % a.c2, m1 = (lin / a.c2) % a.c1, m0 = lin / (a.c2 * a.c1);
+        const bool hit = m0 <= t_hi0 && m0 + B0 - 1 >= t_lo0 && m1 <= t_hi1 &&
+                         m1 + B1 - 1 >= t_lo1 && m2 <= t_hi2 && m2 + B2 - 1 >= t_lo2;
+        if (hit) {
+          s_ent_idx[w] = gsel * a.max_m + i;
+          // member origin relative to the tile, 3 x 10-bit biased fields
+          s_ent_pos[w] = ((m0 - t_lo0 + 512) << 20) | ((m1 - t_lo1 + 512) << 10) |
+                         (m2 - t_lo2 + 512);
+          ++w;
+        }
+      }
+    }
+    if (tid == take - 1) s_nent = ex + mcount;  // entries produced
+    __syncthreads();
+    nent = s_nent;
+    if (inside) {
+      for (int e = 0; e < nent; ++e) {
+        const int pk = s_ent_pos[e];
+        const int p0 = v0 - (((pk >> 20) & 1023) - 512);
+        const int p1 = v1 - (((pk >> 10) & 1023) - 512);
+        const int p2 = v2 - ((pk & 1023) - 512);
+        if (p0 >= 0 && p0 < B0 && p1 >= 0 && p1 < B1 && p2 >= 0 && p2 < B2) {
+          const int p = (p0 * B1 + p1) * B2 + p2;
+          const int idx = s_ent_idx[e];
+          const float ww = a.weights[idx] * s_k[p];
+          num = num + a.filt[(int64_t)idx * P + p] * ww;
+          den = den + ww;
+        }
+      }
+    }
+    __syncthreads();
+    j0 += take;
+  }
+  if (inside) {
+    const int64_t v = ((int64_t)x0 * a.n1 + x1) * a.n2 + x2;
+    a.est[v] = den > 0.0f ? num / (den + 1e-10f) : a.fallback[v];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct Config {
+  int b[3], r[3];
+};
+
+bool same3(const int32_t *x, int a0, int a1, int a2) {
+  return x[0] == a0 && x[1] == a1 && x[2] == a2;
+}
+
+std::mutex g_tab_mutex;
+std::vector<double *> g_logtab;  // per device
+
+int logtab_for_device(const double **out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_tab_mutex);
+  if ((int)g_logtab.size() <= dev) g_logtab.resize(dev + 1, nullptr);
+  if (!g_logtab[dev]) {
+    std::vector<double> h(kLogTabDoubles);
+    for (int j = kLogTabLo; j < kLogTabLo + kLogTabN; ++j) {
+      const double c = 1.0 + j / 128.0;
+      const double invc = (j == 0) ? 1.0 : 1.0 / c;
+      const long double t = -logl((long double)invc);
+      const double thi = (double)t;
+      const double tlo = (double)(t - (long double)thi);
+      const int k = j - kLogTabLo;
+      h[3 * k] = invc;
+      h[3 * k + 1] = (j == 0) ? 0.0 : thi;
+      h[3 * k + 2] = (j == 0) ? 0.0 : tlo;
+    }
+    // ln2 split: 32 significant bits + remainder
+    const long double ln2 = logl(2.0L);
+    const double hi = ldexp(nearbyint(ldexp((double)ln2, 31)), -31);
+    h[3 * kLogTabN] = hi;
+    h[3 * kLogTabN + 1] = (double)(ln2 - (long double)hi);
+    double *d = nullptr;
+    HIP_TRY(hipMalloc(&d, sizeof(double) * kLogTabDoubles));
+    HIP_TRY(hipMemcpy(d, h.data(), sizeof(double) * kLogTabDoubles, hipMemcpyHostToDevice));
+    g_logtab[dev] = d;
+  }
+  *out = g_logtab[dev];
+  return 0;
+}
+
+float host_canonical_key(float d) {
+  double mag = std::fabs((double)d);
+  if (mag < 1.0) mag = 1.0;
+  int e = std::ilogb(mag);
+  double q = std::ldexp(0x1p-18, e);
+  return (float)(std::nearbyint((double)d / q) * q);
+}
+
+int check_geometry(const int64_t *shape, const int32_t *block, const int32_t *radius,
+                   int c[3]) {
+  int64_t vox = 1;
+  for (int d = 0; d < 3; ++d) {
+    if (shape[d] <= 0 || block[d] <= 0 || block[d] > shape[d] || radius[d] < 0)
+      return fail(BMND_HIP_EINVAL, "invalid shape/block/radius");
+    c[d] = (int)(shape[d] - block[d] + 1);
+    vox *= shape[d];
+    if (shape[d] > 1000000) return fail(BMND_HIP_EINVAL, "axis too long");
+  }
+  if (vox >= (int64_t)1 << 31) return fail(BMND_HIP_EINVAL, "volume exceeds 2^31 voxels");
+  return 0;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+int32_t bmnd_hip_version(void) { return 100; }
+
+const char *bmnd_hip_last_error(void) { return g_last_error.c_str(); }
+
+const char *bmnd_hip_configs(void) {
+  return "block=(1,8,8) radius=(0,19,19)\n"
+         "block=(4,4,4) radius=(7,7,7)\n"
+         "block=(5,5,5) radius=(7,7,7)\n";
+}
+
+int32_t bmnd_hip_variance_volume(const float *vol, float *out, int64_t n, double scale,
+                                 double floor_, void *stream) {
+  if (n <= 0) return 0;
+  int blocks = (int)std::min<int64_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_variance_volume, dim3(blocks), dim3(256), 0, (hipStream_t)stream, vol,
+                     out, n, (float)scale, (float)floor_);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int32_t bmnd_hip_blockmatch(const float *vol, const int64_t *shape_h, const int32_t *block_h,
+                            const int32_t *radius_h, const int32_t *ref_lin, int64_t nrefs,
+                            float threshold, int32_t max_matches, int32_t min_matches,
+                            int32_t pow2, int32_t stabilize, double count_scale,
+                            int32_t exact, int32_t *sel, int32_t *counts, int32_t *accepted,
+                            void *stream) {
+  int c[3];
+  if (int rc = check_geometry(shape_h, block_h, radius_h, c)) return rc;
+  if (max_matches < 1 || max_matches > kMaxStack || min_matches < 1)
+    return fail(BMND_HIP_EINVAL, "stack sizes must satisfy 1 <= min, 1 <= max <= 64");
+  if (!(count_scale > 0.0) || !std::isfinite(count_scale))
+    return fail(BMND_HIP_EINVAL, "count_scale must be finite and positive");
+  if (nrefs <= 0) return 0;
+  if (nrefs > INT32_MAX) return fail(BMND_HIP_EINVAL, "too many references");
+  const double *tab = nullptr;
+  if (int rc = logtab_for_device(&tab)) return rc;
+  BMArgs a;
+  a.vol = vol;
+  a.n0 = (int)shape_h[0];
+  a.n1 = (int)shape_h[1];
+  a.n2 = (int)shape_h[2];
+  a.c0 = c[0];
+  a.c1 = c[1];
+  a.c2 = c[2];
+  a.ref_lin = ref_lin;
+  a.nrefs = nrefs;
+  a.threshold = threshold;
+  a.thr_key = host_canonical_key(threshold);
+  a.max_m = max_matches;
+  a.min_m = min_matches;
+  a.pow2 = pow2;
+  a.stabilize = stabilize;
+  a.inv_scale = 1.0 / count_scale;
+  a.logtab = tab;
+  a.sel = sel;
+  a.counts = counts;
+  a.accepted = accepted;
+  hipStream_t s = (hipStream_t)stream;
+  dim3 grid((unsigned)nrefs), blk(kBMThreads);
+#define BM_CASE(b0, b1, b2, r0, r1, r2)                                                    \
+  if (same3(block_h, b0, b1, b2) && same3(radius_h, r0, r1, r2)) {                           \
+    if (exact)                                                                               \
+      hipLaunchKernelGGL((k_blockmatch<b0, b1, b2, r0, r1, r2, true>), grid, blk, 0, s, a);  \
+    else                                                                                     \
+      hipLaunchKernelGGL((k_blockmatch<b0, b1, b2, r0, r1, r2, false>), grid, blk, 0, s, a); \
+    HIP_TRY(hipGetLastError());                                                              \
+    return 0;                                                                                \
+  }
+  BM_CASE(1, 8, 8, 0, 19, 19)
+  BM_CASE(4, 4, 4, 7, 7, 7)
+  BM_CASE(5, 5, 5, 7, 7, 7)
+#undef BM_CASE
+  return fail(BMND_HIP_EINVAL, "block/radius configuration not compiled into this library");
+}
+
+int64_t bmnd_hip_matrix_table_size(int32_t max_matches, const int32_t *block_h) {
+  const int64_t m = max_matches;
+  int64_t n = 2 * (m * (m + 1) * (2 * m + 1) / 6);
+  for (int d = 0; d < 3; ++d) n += 2 * (int64_t)block_h[d] * block_h[d];
+  return n;
+}
+
+int32_t bmnd_hip_filter_groups(int32_t wiener, const float *noisy, const float *pilot,
+                               const float *var, const int64_t *shape_h, const int32_t *block_h,
+                               const int32_t *radius_h, const int32_t *sel,
+                               const int32_t *counts, int64_t nrefs, int32_t max_matches,
+                               const float *matrices, float shrink, int32_t k,
+                               double count_scale, float *filt, float *weights, void *stream) {
+  int c[3];
+  if (int rc = check_geometry(shape_h, block_h, radius_h, c)) return rc;
+  if (k < 0) return fail(BMND_HIP_EINVAL, "k must be non-negative");
+  if (nrefs <= 0) return 0;
+  if (wiener && !pilot) return fail(BMND_HIP_EINVAL, "Wiener filtering needs the pilot volume");
+  FilterArgs a;
+  a.noisy = noisy;
+  a.pilot = pilot;
+  a.var = var;
+  a.n0 = (int)shape_h[0];
+  a.n1 = (int)shape_h[1];
+  a.n2 = (int)shape_h[2];
+  a.c0 = c[0];
+  a.c1 = c[1];
+  a.c2 = c[2];
+  a.sel = sel;
+  a.counts = counts;
+  a.nrefs = nrefs;
+  a.max_m = max_matches;
+  a.mats = matrices;
+  a.shrink = shrink;
+  a.k = k;
+  a.count_scale = count_scale;
+  a.filt = filt;
+  a.weights = weights;
+  hipStream_t s = (hipStream_t)stream;
+  dim3 grid((unsigned)nrefs);
+#define F_CASE(b0, b1, b2, r0, r1, r2, nmax, nt)                                            \
+  if (same3(block_h, b0, b1, b2) && same3(radius_h, r0, r1, r2) && max_matches <= nmax) {   \
+    if (wiener)                                                                            \
+      hipLaunchKernelGGL((k_filter<b0, b1, b2, r0, r1, r2, nmax, nt, true>), grid, dim3(nt), \
+                         0, s, a);                                                         \
+    else                                                                                   \
+      hipLaunchKernelGGL((k_filter<b0, b1, b2, r0, r1, r2, nmax, nt, false>), grid,         \
+                         dim3(nt), 0, s, a);                                               \
+    HIP_TRY(hipGetLastError());                                                            \
+    return 0;                                                                              \
+  }
+  F_CASE(1, 8, 8, 0, 19, 19, 32, 256)
+  F_CASE(4, 4, 4, 7, 7, 7, 16, 256)
+  F_CASE(5, 5, 5, 7, 7, 7, 32, 512)
+#undef F_CASE
+  return fail(BMND_HIP_EINVAL, "block/radius/stack configuration not compiled into this library");
+}
+
+int32_t bmnd_hip_reference_map(const int32_t *ref_lin, int64_t nrefs, int32_t *ref_index,
+                               int64_t num_patches, void *stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (num_patches > 0) HIP_TRY(hipMemsetAsync(ref_index, 0xff, sizeof(int32_t) * num_patches, s));
+  if (nrefs <= 0) return 0;
+  hipLaunchKernelGGL(k_reference_map, dim3((unsigned)((nrefs + 255) / 256)), dim3(256), 0, s,
+                     ref_lin, nrefs, ref_index);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int32_t bmnd_hip_aggregate(const float *filt, const float *weights, const int32_t *sel,
+                           const int32_t *counts, const int32_t *ref_index, int64_t nrefs,
+                           int32_t max_matches, const int64_t *shape_h, const int32_t *block_h,
+                           const int32_t *radius_h, const float *kaiser, const float *fallback,
+                           float *est, void *stream) {
+  int c[3];
+  if (int rc = check_geometry(shape_h, block_h, radius_h, c)) return rc;
+  AggArgs a;
+  a.filt = filt;
+  a.weights = weights;
+  a.sel = sel;
+  a.counts = counts;
+  a.ref_index = ref_index;
+  a.nrefs = nrefs;
+  a.max_m = max_matches;
+  a.n0 = (int)shape_h[0];
+  a.n1 = (int)shape_h[1];
+  a.n2 = (int)shape_h[2];
+  a.c0 = c[0];
+  a.c1 = c[1];
+  a.c2 = c[2];
+  a.kaiser = kaiser;
+  a.fallback = fallback;
+  a.est = est;
+  hipStream_t s = (hipStream_t)stream;
+  int *d_over = nullptr;
+  HIP_TRY(hipMallocAsync((void **)&d_over, sizeof(int), s));
+  HIP_TRY(hipMemsetAsync(d_over, 0, sizeof(int), s));
+  a.overflow = d_over;
+  bool launched = false;
+#define A_CASE(b0, b1, b2, r0, r1, r2, t0, t1, t2)                                           \
+  if (!launched && same3(block_h, b0, b1, b2) && same3(radius_h, r0, r1, r2)) {             \
+    const int64_t tiles = ((shape_h[0] + t0 - 1) / t0) * ((shape_h[1] + t1 - 1) / t1) *     \
+                          ((shape_h[2] + t2 - 1) / t2);                                     \
+    hipLaunchKernelGGL((k_aggregate<b0, b1, b2, r0, r1, r2, t0, t1, t2>),                    \
+                       dim3((unsigned)tiles), dim3(kAggThreads), 0, s, a);                  \
+    launched = true;                                                                        \
+  }
+  A_CASE(1, 8, 8, 0, 19, 19, 1, 16, 32)
+  A_CASE(4, 4, 4, 7, 7, 7, 8, 8, 8)
+  A_CASE(5, 5, 5, 7, 7, 7, 8, 8, 8)
+#undef A_CASE
+  if (!launched) {
+    (void)hipFreeAsync(d_over, s);
+    return fail(BMND_HIP_EINVAL, "block/radius configuration not compiled into this library");
+  }
+  HIP_TRY(hipGetLastError());
+  int h_over = 0;
+  HIP_TRY(hipMemcpyAsync(&h_over, d_over, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipFreeAsync(d_over, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (h_over) return fail(BMND_HIP_ECAPACITY, "aggregation tile reached too many groups");
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,264 @@
+"""Device pipeline of the Poisson bmnd(): both stages on one GPU.
+
+Per stage: schedule (host, cached per shape) -> fused block matching ->
+group filtering -> ordered aggregation, all on the caller's HIP stream over
+torch-allocated device buffers.  2-D inputs run as (1, H, W) volumes.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib
+from .enums import (
+    AggregationWeightDomain,
+    AggregationWeightModel,
+    AggregationWeightScope,
+    BlockMatchDistance,
+    PoissonGroupMassConservation,
+    PoissonHTMatchPolicy,
+    PoissonVarianceSource,
+    PoissonWienerGainMode,
+)
+from .profiles import BMNDProfile
+from .shifts import ScheduleArrays, get_reference_schedule_arrays
+from .transforms import Transform, TransformMode, TransformType, group_matrices, spatial_matrices
+from .utils import kaiser_window_nd
+
+# (block, search radius) pairs compiled into libbmnd_hip.so, in 3-D form
+SUPPORTED = {
+    ((1, 8, 8), (0, 19, 19)),
+    ((4, 4, 4), (7, 7, 7)),
+    ((5, 5, 5), (7, 7, 7)),
+}
+
+
+def _ptr(t: torch.Tensor | None) -> ctypes.c_void_p:
+    return ctypes.c_void_p(0 if t is None else t.data_ptr())
+
+
+def _host3(values, ndim: int, pad: int) -> tuple[int, int, int]:
+    v = tuple(int(x) for x in values)
+    return (pad,) * (3 - ndim) + v
+
+
+def _spatial_kind(transform, stage: str) -> TransformType:
+    items = [transform] if isinstance(transform, Transform) else list(transform)
+    kinds = {t.mode: t.type for t in items}
+    if (len(items) == 2 and kinds.get(TransformMode.GROUP) is TransformType.HAAR
+            and kinds.get(TransformMode.SPATIAL) in (TransformType.BIOR15, TransformType.DCT)):
+        return kinds[TransformMode.SPATIAL]
+    raise NotImplementedError(
+        f"{stage}_transform must be [HAAR/GROUP, BIOR15 or DCT/SPATIAL] on the MI355X path.")
+
+
+@dataclass
+class StagePlan:
+    name: str
+    ndim: int
+    block: tuple[int, ...]
+    step: tuple[int, ...]
+    block3: tuple[int, int, int]
+    radius3: tuple[int, int, int]
+    max_m: int
+    min_m: int
+    threshold: float
+    shrink: float
+    spatial: TransformType
+    beta: float
+    kaiser: np.ndarray
+    schedule: ScheduleArrays | None = None
+    _dev: dict = field(default_factory=dict)
+
+    @property
+    def P(self) -> int:
+        return int(np.prod(self.block3))
+
+    def matrix_table(self) -> np.ndarray:
+        parts = []
+        for n in range(1, self.max_m + 1):
+            parts.append(group_matrices(n)[0].reshape(-1))
+        for n in range(1, self.max_m + 1):
+            parts.append(group_matrices(n)[1].reshape(-1))
+        fwd, inv = [], []
+        for b in self.block3:
+            if b == 1:
+                f = i = np.ones((1, 1), dtype=np.float32)
+            else:
+                f, i = spatial_matrices(self.spatial, b)
+            fwd.append(f.reshape(-1))
+            inv.append(i.reshape(-1))
+        table = np.concatenate(parts + fwd + inv).astype(np.float32)
+        blk = (ctypes.c_int32 * 3)(*self.block3)
+        assert table.size == _lib.lib().bmnd_hip_matrix_table_size(self.max_m, blk)
+        return table
+
+    def device_constants(self, device) -> tuple[torch.Tensor, torch.Tensor]:
+        key = str(device)
+        if key not in self._dev:
+            mats = torch.from_numpy(self.matrix_table()).to(device)
+            kai = torch.from_numpy(np.ascontiguousarray(self.kaiser.reshape(-1))).to(device)
+            self._dev[key] = (mats, kai)
+        return self._dev[key]
+
+
+def check_supported(profile: BMNDProfile, resolved) -> None:
+    """Raise NotImplementedError for profile options outside the MI355X path."""
+    if not resolved.is_poisson:
+        raise NotImplementedError("bmnd_amd implements the Poisson noise model only "
+                                  "(noise_model='poisson').")
+    if resolved.blockmatch_distance is not BlockMatchDistance.POISSON_DEVIANCE:
+        raise NotImplementedError("only the 'poisson_deviance' block-matching distance runs on "
+                                  "the MI355X path.")
+    checks = [
+        (profile.poisson_ht_match_policy is PoissonHTMatchPolicy.FIXED, "poisson_ht_match_policy='fixed'"),
+        (profile.poisson_wiener_gain_mode is PoissonWienerGainMode.VARIANCE_SCALED,
+         "poisson_wiener_gain_mode='variance-scaled'"),
+        (profile.poisson_group_mass_conservation is PoissonGroupMassConservation.NONE,
+         "poisson_group_mass_conservation='none'"),
+        (profile.poisson_variance_source_wiener is PoissonVarianceSource.PILOT,
+         "poisson_variance_source_wiener='pilot'"),
+        (profile.poisson_wiener_match_source is PoissonVarianceSource.PILOT,
+         "poisson_wiener_match_source='pilot'"),
+        (not profile.ht_use_soft_thresholding, "ht_use_soft_thresholding=False"),
+    ]
+    for stage in ("ht", "wiener"):
+        checks += [
+            (getattr(profile, f"{stage}_weight_model") is AggregationWeightModel.VARIANCE,
+             f"{stage}_weight_model='variance'"),
+            (getattr(profile, f"{stage}_weight_domain") is AggregationWeightDomain.COEFFICIENT,
+             f"{stage}_weight_domain='coefficient'"),
+            (getattr(profile, f"{stage}_weight_scope") is AggregationWeightScope.PATCH,
+             f"{stage}_weight_scope='patch'"),
+        ]
+    for ok, what in checks:
+        if not ok:
+            raise NotImplementedError(f"the MI355X path requires {what}.")
+
+
+def make_plan(profile: BMNDProfile, stage: str, ndim: int) -> StagePlan:
+    if ndim not in (2, 3):
+        raise NotImplementedError("the MI355X path runs 2-D images and 3-D volumes.")
+    block = tuple(int(b) for b in getattr(profile, f"{stage}_block_size"))
+    radius = tuple(int(r) for r in getattr(profile, f"{stage}_search_window"))
+    block3, radius3 = _host3(block, ndim, 1), _host3(radius, ndim, 0)
+    if (block3, radius3) not in SUPPORTED:
+        raise NotImplementedError(
+            f"{stage} block {block} with search radius {radius} is not compiled into the "
+            "MI355X kernels (supported: 8x8/19x19, 4^3/7^3, 5^3/7^3).")
+    max_m = int(getattr(profile, f"{stage}_max_stack_size"))
+    if max_m > (16 if block3 == (4, 4, 4) else 32):
+        raise NotImplementedError(f"{stage}_max_stack_size={max_m} exceeds the compiled group size.")
+    beta = float(getattr(profile, f"{stage}_kaiser_beta"))
+    return StagePlan(
+        name=stage, ndim=ndim, block=block, step=tuple(int(s) for s in getattr(profile, f"{stage}_step")),
+        block3=block3, radius3=radius3, max_m=max_m,
+        min_m=int(getattr(profile, f"{stage}_min_stack_size")),
+        threshold=float(np.float32(getattr(profile, f"{stage}_match_threshold"))),
+        shrink=float(profile.ht_lambda_threshold if stage == "ht" else profile.wiener_variance_scale),
+        spatial=_spatial_kind(getattr(profile, f"{stage}_transform"), stage),
+        beta=beta, kaiser=kaiser_window_nd(block, beta))
+
+
+@dataclass
+class StageOutput:
+    est: torch.Tensor
+    sel: torch.Tensor
+    counts: torch.Tensor
+    accepted: torch.Tensor
+    schedule: ScheduleArrays
+
+
+class PoissonPipeline:
+    """Two-stage Poisson denoiser on one device.  Buffers are torch tensors on
+    `device`; kernels run on the current torch stream."""
+
+    def __init__(self, profile: BMNDProfile, shape: tuple[int, ...], device=None,
+                 exact_deviance: bool = True):
+        self.profile = profile
+        self.shape = tuple(int(s) for s in shape)
+        self.ndim = len(self.shape)
+        self.shape3 = _host3(self.shape, self.ndim, 1)
+        self.device = torch.device(device if device is not None else "cuda")
+        self.exact = bool(exact_deviance)
+        self.count_scale = float(profile.poisson_count_scale)
+        self.floor = float(profile.poisson_variance_floor)
+        self.k = int(profile.k)
+        self.plans = {s: make_plan(profile, s, self.ndim) for s in ("ht", "wiener")}
+        for plan in self.plans.values():
+            counts = tuple(self.shape[d] - plan.block[d] + 1 for d in range(self.ndim))
+            plan.schedule = get_reference_schedule_arrays(
+                counts, plan.block, plan.step, mode=profile.reference_schedule_mode,
+                shift_density=profile.reference_shift_density,
+                schedule_density=profile.reference_schedule_density)
+        self._refs = {}
+
+    # -- helpers -----------------------------------------------------------------
+    def _stream(self) -> ctypes.c_void_p:
+        return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def _ref_lin(self, plan: StagePlan) -> torch.Tensor:
+        if plan.name not in self._refs:
+            counts = tuple(self.shape[d] - plan.block[d] + 1 for d in range(self.ndim))
+            lin = np.ravel_multi_index(tuple(plan.schedule.reference_abs.T), counts)
+            self._refs[plan.name] = torch.from_numpy(lin.astype(np.int32)).to(self.device)
+        return self._refs[plan.name]
+
+    def variance(self, vol: torch.Tensor) -> torch.Tensor:
+        out = torch.empty_like(vol)
+        _lib.check(_lib.lib().bmnd_hip_variance_volume(_ptr(vol), _ptr(out), vol.numel(),
+                                                       self.count_scale, self.floor, self._stream()))
+        return out
+
+    def stage(self, name: str, match_vol: torch.Tensor, noisy: torch.Tensor, var: torch.Tensor,
+              fallback: torch.Tensor) -> StageOutput:
+        L = _lib.lib()
+        plan = self.plans[name]
+        stream = self._stream()
+        shape = (ctypes.c_int64 * 3)(*self.shape3)
+        block = (ctypes.c_int32 * 3)(*plan.block3)
+        radius = (ctypes.c_int32 * 3)(*plan.radius3)
+        refs = self._ref_lin(plan)
+        R = refs.numel()
+        dev = self.device
+        sel = torch.empty(R * plan.max_m, dtype=torch.int32, device=dev)
+        cnt = torch.empty(R, dtype=torch.int32, device=dev)
+        acc = torch.empty(R, dtype=torch.int32, device=dev)
+        _lib.check(L.bmnd_hip_blockmatch(_ptr(match_vol), shape, block, radius, _ptr(refs), R,
+                                         plan.threshold, plan.max_m, plan.min_m, 1, 1,
+                                         self.count_scale, int(self.exact), _ptr(sel), _ptr(cnt),
+                                         _ptr(acc), stream))
+        mats, kai = plan.device_constants(dev)
+        filt = torch.empty(R * plan.max_m * plan.P, dtype=torch.float32, device=dev)
+        wts = torch.empty(R * plan.max_m, dtype=torch.float32, device=dev)
+        wiener = name == "wiener"
+        _lib.check(L.bmnd_hip_filter_groups(int(wiener), _ptr(noisy),
+                                            _ptr(match_vol if wiener else None), _ptr(var), shape,
+                                            block, radius, _ptr(sel), _ptr(cnt), R, plan.max_m,
+                                            _ptr(mats), plan.shrink, self.k, self.count_scale,
+                                            _ptr(filt), _ptr(wts), stream))
+        npatch = int(np.prod([self.shape3[d] - plan.block3[d] + 1 for d in range(3)]))
+        ref_index = torch.empty(npatch, dtype=torch.int32, device=dev)
+        _lib.check(L.bmnd_hip_reference_map(_ptr(refs), R, _ptr(ref_index), npatch, stream))
+        est = torch.empty_like(noisy)
+        _lib.check(L.bmnd_hip_aggregate(_ptr(filt), _ptr(wts), _ptr(sel), _ptr(cnt),
+                                        _ptr(ref_index), R, plan.max_m, shape, block, radius,
+                                        _ptr(kai), _ptr(fallback), _ptr(est), stream))
+        del filt, wts, ref_index
+        return StageOutput(est, sel.view(R, plan.max_m), cnt, acc, plan.schedule)
+
+    def run(self, vol: torch.Tensor) -> tuple[StageOutput, StageOutput]:
+        """vol: float32 contiguous device tensor of `shape`.  Returns the HT
+        (pilot) and Wiener stage outputs."""
+        if vol.dtype != torch.float32 or not vol.is_contiguous() or tuple(vol.shape) != self.shape:
+            raise ValueError("vol must be a contiguous float32 tensor of the planned shape")
+        var1 = self.variance(vol)
+        ht = self.stage("ht", vol, vol, var1, vol)
+        del var1
+        var2 = self.variance(ht.est)
+        wi = self.stage("wiener", ht.est, vol, var2, vol)
+        return ht, wi

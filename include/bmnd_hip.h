@@ -1,0 +1,122 @@
+/*
+ * C-ABI of libbmnd_hip.so -- the MI355X (gfx950) kernels behind bmnd_amd.
+ *
+ * All pointers are device pointers unless the name ends in _h (host).
+ * Volumes are C-contiguous float32 and always described as 3-D: a 2-D image
+ * of shape (H, W) is passed as (1, H, W) with block (1, bh, bw) and radius
+ * (0, rh, rw).  Patch origins ("lin") are C-order linear indices into the
+ * patch grid counts[d] = shape[d] - block[d] + 1, exactly as the reference's
+ * blockmatching.multi_index_to_linear (src/bmnd/blockmatching.py:458-465).
+ * `stream` is a hipStream_t (0 = the null stream).  Every entry point returns
+ * 0 on success and a negative status otherwise; bmnd_hip_last_error() then
+ * holds a message.  No exceptions cross the ABI; the caller owns every
+ * buffer.
+ */
+#ifndef BMND_HIP_H
+#define BMND_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BMND_HIP_OK 0
+#define BMND_HIP_EINVAL -1      /* argument outside what the kernels support */
+#define BMND_HIP_ELAUNCH -2     /* HIP runtime error */
+#define BMND_HIP_ECAPACITY -3   /* a per-tile capacity was exceeded */
+
+/* Library version (major*10000 + minor*100 + patch). */
+int32_t bmnd_hip_version(void);
+
+/* Thread-local message of the last failing call. */
+const char *bmnd_hip_last_error(void);
+
+/* Names of the block/radius configurations compiled into this library,
+ * one per line; for diagnostics. */
+const char *bmnd_hip_configs(void);
+
+/* Poisson variance volume: out = max(scale * max(v, 0), floor) in float32.
+ * Replaces variance.build_poisson_variance_model (variance.py:556-598). */
+int32_t bmnd_hip_variance_volume(const float *vol, float *out, int64_t n,
+                                 double scale, double floor_, void *stream);
+
+/* Fused block matching for every reference of a stage.
+ * Replaces blockmatching.blockmatch_groups -> _fused_blockmatch_groups_batch_numba
+ * (blockmatching.py:1393-1511, 1045-1205) for the Poisson deviance distance
+ * (blockmatching.py:322-371) with tie canonicalisation (957-1042) and
+ * finalisation (913-954).
+ *   ref_lin[nrefs]            patch-grid linear index of each reference
+ *   sel[nrefs*max_matches]    out: selected lins, reference first, -1 padded
+ *   counts[nrefs]             out: final group size
+ *   accepted[nrefs]           out: number of candidates under the threshold
+ * exact != 0 evaluates each deviance term as the reference writes it
+ * (two divisions, two logarithms in float64); exact == 0 uses the one-log
+ * form r*(log 2r - log s) + c*(log 2c - log s). */
+int32_t bmnd_hip_blockmatch(const float *vol, const int64_t *shape_h,
+                            const int32_t *block_h, const int32_t *radius_h,
+                            const int32_t *ref_lin, int64_t nrefs,
+                            float threshold, int32_t max_matches,
+                            int32_t min_matches, int32_t pow2,
+                            int32_t stabilize, double count_scale,
+                            int32_t exact, int32_t *sel, int32_t *counts,
+                            int32_t *accepted, void *stream);
+
+/* Matrix table consumed by bmnd_hip_filter_groups, float32, packed as
+ *   G_n (n x n, row-major, out index first) for n = 1..max_matches,
+ *   then Ginv_n for n = 1..max_matches,
+ *   then the spatial analysis matrices of axes 0,1,2 (b_d x b_d),
+ *   then the spatial synthesis matrices of axes 0,1,2.
+ * bmnd_hip_matrix_table_size() gives its length in floats. */
+int64_t bmnd_hip_matrix_table_size(int32_t max_matches, const int32_t *block_h);
+
+/* Collaborative filtering of every group of a stage: gather, separable
+ * forward transform, coefficient noise std (exact overlap-aware planes
+ * q < k plus rescaled approximate planes), shrinkage, inverse transform and
+ * per-patch aggregation weights.
+ * Replaces bmndalgo.filter_ht_group (bmndalgo.py:335-559) when wiener == 0
+ * and bmndalgo.filter_wiener_group (bmndalgo.py:760-981) when wiener != 0,
+ * with variance.get_direct_poisson_noise_std (variance.py:1634-1711),
+ * poisson._poisson_hard_threshold_mask (poisson.py:30-46),
+ * wiener.compute_poisson_wiener_gain (wiener.py:78-95) and
+ * aggregationweights.compute_aggregation_weights (aggregationweights.py:690-800).
+ *   noisy      observation volume (HT: the coefficients to shrink)
+ *   pilot      Wiener only: pilot volume whose coefficients drive the gain
+ *   var        variance volume of the stage
+ *   filt       out: (nrefs*max_matches, P) filtered patches, row g*max+i
+ *   weights    out: (nrefs*max_matches) float32 weights
+ * shrink is ht_lambda_threshold (HT) or wiener_variance_scale (Wiener). */
+int32_t bmnd_hip_filter_groups(int32_t wiener, const float *noisy,
+                               const float *pilot, const float *var,
+                               const int64_t *shape_h, const int32_t *block_h,
+                               const int32_t *radius_h, const int32_t *sel,
+                               const int32_t *counts, int64_t nrefs,
+                               int32_t max_matches, const float *matrices,
+                               float shrink, int32_t k, double count_scale,
+                               float *filt, float *weights, void *stream);
+
+/* Reference-position map used by the aggregation: ref_index[lin] = g for
+ * every scheduled reference g, -1 elsewhere (num_patches entries). */
+int32_t bmnd_hip_reference_map(const int32_t *ref_lin, int64_t nrefs,
+                               int32_t *ref_index, int64_t num_patches,
+                               void *stream);
+
+/* Ordered aggregation and normalisation: every output voxel sums its
+ * contributions num += f*(w*k), den += w*k sequentially in (schedule index,
+ * member index) order -- the order of the reference's serial scatter -- and
+ * writes est = den > 0 ? num / (den + 1e-10f) : fallback.
+ * Replaces patchaggregation._scatter_add_patches_numba
+ * (patchaggregation.py:170-199, 373-397) plus the normalisation of
+ * bmndalgo.py:626-631 / 1039-1044.  Deterministic; no atomics on floats. */
+int32_t bmnd_hip_aggregate(const float *filt, const float *weights,
+                           const int32_t *sel, const int32_t *counts,
+                           const int32_t *ref_index, int64_t nrefs,
+                           int32_t max_matches, const int64_t *shape_h,
+                           const int32_t *block_h, const int32_t *radius_h,
+                           const float *kaiser, const float *fallback,
+                           float *est, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

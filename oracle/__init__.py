@@ -1,0 +1,338 @@
+"""ctypes front end of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+The oracle (``bmnd_oracle.c``) restates the reference denoiser's Poisson
+branch on the CPU.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` import this module; the product package
+``bmnd_amd`` never does.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libbmnd_oracle.so")
+_LIB = None
+
+MAX_NDIM = 4
+DIST_SSD, DIST_DEVIANCE, DIST_PEARSON, DIST_ANSCOMBE = 0, 1, 2, 3
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc, OpenMP)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-B" if force else "-s", "-C", _HERE],
+                       check=True)
+    return _LIB_PATH
+
+
+class OracleParams(ctypes.Structure):
+    _fields_ = [
+        ("ndim", ctypes.c_int),
+        ("shape", ctypes.c_int64 * MAX_NDIM),
+        ("ht_block", ctypes.c_int * MAX_NDIM),
+        ("ht_step", ctypes.c_int * MAX_NDIM),
+        ("ht_radius", ctypes.c_int * MAX_NDIM),
+        ("wi_block", ctypes.c_int * MAX_NDIM),
+        ("wi_step", ctypes.c_int * MAX_NDIM),
+        ("wi_radius", ctypes.c_int * MAX_NDIM),
+        ("ht_max", ctypes.c_int),
+        ("ht_min", ctypes.c_int),
+        ("wi_max", ctypes.c_int),
+        ("wi_min", ctypes.c_int),
+        ("ht_threshold", ctypes.c_float),
+        ("wi_threshold", ctypes.c_float),
+        ("ht_lambda", ctypes.c_float),
+        ("wi_mu", ctypes.c_float),
+        ("ht_beta", ctypes.c_double),
+        ("wi_beta", ctypes.c_double),
+        ("shift_density", ctypes.c_double),
+        ("schedule_density", ctypes.c_int),
+        ("k", ctypes.c_int),
+        ("count_scale", ctypes.c_double),
+        ("variance_floor", ctypes.c_double),
+        ("nthreads", ctypes.c_int),
+    ]
+
+
+def lib() -> ctypes.CDLL:
+    global _LIB
+    if _LIB is None:
+        build()
+        _LIB = ctypes.CDLL(_LIB_PATH)
+        _declare(_LIB)
+    return _LIB
+
+
+def _p(a: np.ndarray | None) -> ctypes.c_void_p:
+    return ctypes.c_void_p(0 if a is None else a.ctypes.data)
+
+
+def _declare(L: ctypes.CDLL) -> None:
+    vp, i32, i64, f32, f64 = (ctypes.c_void_p, ctypes.c_int, ctypes.c_int64,
+                              ctypes.c_float, ctypes.c_double)
+    L.oracle_kaiser.argtypes = [i32, f64, vp]
+    L.oracle_kaiser_nd.argtypes = [i32, vp, f64, vp]
+    L.oracle_haar_matrix.argtypes = [i32, vp]
+    L.oracle_haar_matrix.restype = i32
+    L.oracle_dct_matrix.argtypes = [i32, vp, vp]
+    L.oracle_bior15_matrix.argtypes = [i32, vp, vp]
+    L.oracle_bior15_matrix.restype = i32
+    L.oracle_schedule_generated.argtypes = [i32, vp, vp, vp, f64, i32, vp, i64]
+    L.oracle_schedule_generated.restype = i64
+    L.oracle_canonical_key.argtypes = [f32]
+    L.oracle_canonical_key.restype = f32
+    L.oracle_patch_distance.argtypes = [vp, vp, i32, i32, f64]
+    L.oracle_patch_distance.restype = f32
+    L.oracle_blockmatch.argtypes = [vp, vp, i32, vp, vp, vp, vp, i64, vp, vp,
+                                    i32, i32, i32, i32, i32, f64, i32, vp, vp, vp]
+    L.oracle_apply_transform.argtypes = [vp, i32, vp, vp]
+    L.oracle_exact_variance.argtypes = [vp, i32, vp, vp, i32, vp, vp, vp, i32, vp]
+    L.oracle_exact_variance.restype = i32
+    L.oracle_noise_std.argtypes = [vp, i32, vp, vp, i32, vp, vp, vp, i32, vp]
+    L.oracle_noise_std.restype = i32
+    L.oracle_limit_planes.argtypes = [vp, vp, i32, i32, i32, vp]
+    L.oracle_patch_weights.argtypes = [vp, vp, i32, i32, vp, f64, vp]
+    L.oracle_bmnd_poisson.argtypes = [vp, ctypes.POINTER(OracleParams), vp, vp,
+                                      vp, vp, i64, vp, vp, i64]
+    L.oracle_bmnd_poisson.restype = i32
+
+
+# ---------------------------------------------------------------------------
+# thin numpy wrappers
+# ---------------------------------------------------------------------------
+
+def kaiser(m: int, beta: float) -> np.ndarray:
+    out = np.empty(m, dtype=np.float64)
+    lib().oracle_kaiser(m, beta, _p(out))
+    return out
+
+
+def kaiser_nd(block: tuple[int, ...], beta: float) -> np.ndarray:
+    b = np.asarray(block, dtype=np.int32)
+    out = np.empty(int(np.prod(block)), dtype=np.float32)
+    lib().oracle_kaiser_nd(len(block), _p(b), beta, _p(out))
+    return out.reshape(block)
+
+
+def haar(n: int) -> np.ndarray:
+    out = np.empty((n, n), dtype=np.float32)
+    if lib().oracle_haar_matrix(n, _p(out)) != 0:
+        raise ValueError(f"Haar size must be a power of two, got {n}")
+    return out
+
+
+def dct(n: int) -> tuple[np.ndarray, np.ndarray]:
+    f = np.empty((n, n), dtype=np.float32)
+    i = np.empty((n, n), dtype=np.float32)
+    lib().oracle_dct_matrix(n, _p(f), _p(i))
+    return f, i
+
+
+def bior15(n: int) -> tuple[np.ndarray, np.ndarray]:
+    f = np.empty((n, n), dtype=np.float32)
+    i = np.empty((n, n), dtype=np.float32)
+    if lib().oracle_bior15_matrix(n, _p(f), _p(i)) != 0:
+        raise ValueError(f"no official bior1.5 matrix of size {n}")
+    return f, i
+
+
+def schedule(counts, block, step, shift_density=2.0, schedule_density=2) -> np.ndarray:
+    ndim = len(counts)
+    c = np.asarray(counts, dtype=np.int64)
+    b = np.asarray(block, dtype=np.int32)
+    s = np.asarray(step, dtype=np.int32)
+    n = lib().oracle_schedule_generated(ndim, _p(c), _p(b), _p(s), float(shift_density),
+                                        int(schedule_density), None, 0)
+    out = np.empty((n, ndim), dtype=np.int64)
+    lib().oracle_schedule_generated(ndim, _p(c), _p(b), _p(s), float(shift_density),
+                                    int(schedule_density), _p(out), n)
+    return out
+
+
+def canonical_key(d: float) -> float:
+    return float(lib().oracle_canonical_key(float(np.float32(d))))
+
+
+def patch_distance(ref, cand, code=DIST_DEVIANCE, count_scale=1.0) -> np.float32:
+    r = np.ascontiguousarray(ref, dtype=np.float32)
+    c = np.ascontiguousarray(cand, dtype=np.float32)
+    return np.float32(lib().oracle_patch_distance(_p(r), _p(c), r.size, code, count_scale))
+
+
+@dataclass
+class MatchResult:
+    selected: np.ndarray  # (nrefs, max) int64, -1 padded
+    counts: np.ndarray  # (nrefs,) int32
+    accepted: np.ndarray  # (nrefs,) int64
+
+    def lists(self) -> list[list[int]]:
+        return [self.selected[i, : self.counts[i]].tolist() for i in range(len(self.counts))]
+
+
+def blockmatch(refs, counts, radius, max_matches, min_matches, threshold, *,
+               patches_flat=None, volume=None, block=None, pow2=True, stabilize=True,
+               code=DIST_DEVIANCE, count_scale=1.0, nthreads=0) -> MatchResult:
+    refs = np.ascontiguousarray(np.asarray(refs, dtype=np.int64).reshape(-1, len(counts)))
+    nrefs, ndim = refs.shape
+    cts = np.asarray(counts, dtype=np.int64)
+    rad = np.asarray(radius, dtype=np.int32)
+    thr = np.asarray(threshold, dtype=np.float32)
+    if thr.ndim == 0:
+        thr = np.full(nrefs, thr, dtype=np.float32)
+    thr = np.ascontiguousarray(thr)
+    if patches_flat is not None:
+        pf = np.ascontiguousarray(patches_flat, dtype=np.float32)
+        vol, vshape = None, None
+        blk = np.ones(ndim, dtype=np.int32)
+        blk[0] = pf.shape[1]
+    else:
+        pf = None
+        vol = np.ascontiguousarray(volume, dtype=np.float32)
+        vshape = np.asarray(vol.shape, dtype=np.int64)
+        blk = np.asarray(block, dtype=np.int32)
+    sel = np.empty((nrefs, max_matches), dtype=np.int64)
+    cnt = np.empty(nrefs, dtype=np.int32)
+    acc = np.empty(nrefs, dtype=np.int64)
+    lib().oracle_blockmatch(_p(pf), _p(vol), ndim, _p(vshape), _p(cts), _p(blk), _p(refs),
+                            nrefs, _p(rad), _p(thr), max_matches, min_matches, int(pow2),
+                            int(stabilize), code, float(count_scale), nthreads, _p(sel),
+                            _p(cnt), _p(acc))
+    return MatchResult(sel, cnt, acc)
+
+
+def apply_transform(data: np.ndarray, mats: list) -> np.ndarray:
+    d = np.ascontiguousarray(data, dtype=np.float32).copy()
+    dims = np.asarray(d.shape, dtype=np.int32)
+    keep = [None if m is None else np.ascontiguousarray(m, dtype=np.float32) for m in mats]
+    ptrs = (ctypes.c_void_p * len(keep))(*[None if m is None else m.ctypes.data for m in keep])
+    lib().oracle_apply_transform(_p(d), d.ndim, _p(dims), ptrs)
+    return d
+
+
+def _spatial_ptrs(spatial):
+    keep = [None if m is None else np.ascontiguousarray(m, dtype=np.float32) for m in spatial]
+    ptrs = (ctypes.c_void_p * max(len(keep), 1))(
+        *[None if m is None else m.ctypes.data for m in keep])
+    return keep, ptrs
+
+
+def exact_variance(var_volume, block, abs_pos, group_mat, spatial, planes) -> np.ndarray:
+    vv = np.ascontiguousarray(var_volume, dtype=np.float32)
+    ndim = vv.ndim
+    shp = np.asarray(vv.shape, dtype=np.int64)
+    blk = np.asarray(block, dtype=np.int32)
+    ap = np.ascontiguousarray(abs_pos, dtype=np.int64)
+    g = np.ascontiguousarray(group_mat, dtype=np.float32)
+    keep, ptrs = _spatial_ptrs(spatial)
+    P = int(np.prod(block))
+    out = np.empty(planes * P, dtype=np.float32)
+    lib().oracle_exact_variance(_p(vv), ndim, _p(shp), _p(blk), ap.shape[0], _p(ap), _p(g),
+                                ptrs, planes, _p(out))
+    return out.reshape((planes,) + tuple(block))
+
+
+def noise_std(var_volume, block, abs_pos, group_mat, spatial, k) -> np.ndarray:
+    vv = np.ascontiguousarray(var_volume, dtype=np.float32)
+    ndim = vv.ndim
+    shp = np.asarray(vv.shape, dtype=np.int64)
+    blk = np.asarray(block, dtype=np.int32)
+    ap = np.ascontiguousarray(abs_pos, dtype=np.int64)
+    n = ap.shape[0]
+    g = np.ascontiguousarray(group_mat, dtype=np.float32)
+    keep, ptrs = _spatial_ptrs(spatial)
+    P = int(np.prod(block))
+    out = np.empty(n * P, dtype=np.float32)
+    lib().oracle_noise_std(_p(vv), ndim, _p(shp), _p(blk), n, _p(ap), _p(g), ptrs, k, _p(out))
+    return out.reshape((n,) + tuple(block))
+
+
+def limit_planes(exact_std, approx_std, k) -> np.ndarray:
+    a = np.ascontiguousarray(approx_std, dtype=np.float32)
+    n = a.shape[0]
+    p = int(a.size // n)
+    e = np.zeros_like(a)
+    ex = np.ascontiguousarray(exact_std, dtype=np.float32)
+    e[: ex.shape[0]] = ex
+    out = np.empty_like(a)
+    lib().oracle_limit_planes(_p(e), _p(a), n, p, k, _p(out))
+    return out
+
+
+def patch_weights(atten, sigma, group_inv, eps=1e-10) -> np.ndarray:
+    a = np.ascontiguousarray(atten, dtype=np.float32)
+    s = np.ascontiguousarray(sigma, dtype=np.float32)
+    n = a.shape[0]
+    p = int(a.size // n)
+    gi = np.ascontiguousarray(group_inv, dtype=np.float32)
+    out = np.empty(n, dtype=np.float32)
+    lib().oracle_patch_weights(_p(a), _p(s), n, p, _p(gi), eps, _p(out))
+    return out
+
+
+def default_params(ndim: int, shape, nthreads: int = 0) -> OracleParams:
+    """Default BM3DProfile / BM4DProfile values with noise_model=POISSON
+    (profiles.py:225-352)."""
+    p = OracleParams()
+    p.ndim = ndim
+    for d in range(ndim):
+        p.shape[d] = int(shape[d])
+    if ndim == 2:
+        hb, wb, rad, wthr = (8, 8), (8, 8), (19, 19), 0.3937
+    elif ndim == 3:
+        hb, wb, rad, wthr = (4, 4, 4), (5, 5, 5), (7, 7, 7), 0.7689
+    else:
+        raise ValueError("default Poisson profiles exist for 2-D and 3-D inputs")
+    for d in range(ndim):
+        p.ht_block[d], p.ht_step[d], p.ht_radius[d] = hb[d], 3, rad[d]
+        p.wi_block[d], p.wi_step[d], p.wi_radius[d] = wb[d], 3, rad[d]
+    p.ht_max, p.ht_min, p.wi_max, p.wi_min = 16, 2, 32, 2
+    p.ht_threshold, p.wi_threshold = 2.9527, wthr
+    p.ht_lambda, p.wi_mu = 3.0, 0.4
+    p.ht_beta = p.wi_beta = 2.0
+    p.shift_density, p.schedule_density = 2.0, 2
+    p.k = 4
+    p.count_scale = 1.0
+    p.variance_floor = 1e-10
+    p.nthreads = nthreads
+    return p
+
+
+def bmnd_poisson(vol: np.ndarray, params: OracleParams | None = None, *, nthreads: int = 0,
+                 capture_matches: bool = False):
+    """Run the oracle's two-stage Poisson pipeline.
+
+    Returns ``(out, pilot)`` or, with ``capture_matches``, ``(out, pilot,
+    (ht_sel, ht_cnt), (wi_sel, wi_cnt))`` where the selections are (nrefs, max)
+    arrays of linear patch indices padded with -1."""
+    v = np.ascontiguousarray(vol, dtype=np.float32)
+    if params is None:
+        params = default_params(v.ndim, v.shape, nthreads)
+    out = np.empty_like(v)
+    pilot = np.empty_like(v)
+    ht = wi = (None, None)
+    args = [None, None, 0, None, None, 0]
+    if capture_matches:
+        nd = v.ndim
+        cnt_ht = tuple(int(v.shape[d]) - params.ht_block[d] + 1 for d in range(nd))
+        cnt_wi = tuple(int(v.shape[d]) - params.wi_block[d] + 1 for d in range(nd))
+        n_ht = len(schedule(cnt_ht, tuple(params.ht_block[:nd]), tuple(params.ht_step[:nd]),
+                            params.shift_density, params.schedule_density))
+        n_wi = len(schedule(cnt_wi, tuple(params.wi_block[:nd]), tuple(params.wi_step[:nd]),
+                            params.shift_density, params.schedule_density))
+        ht = (np.empty((n_ht, params.ht_max), np.int64), np.empty(n_ht, np.int32))
+        wi = (np.empty((n_wi, params.wi_max), np.int64), np.empty(n_wi, np.int32))
+        args = [ht[0], ht[1], n_ht, wi[0], wi[1], n_wi]
+    rc = lib().oracle_bmnd_poisson(_p(v), ctypes.byref(params), _p(out), _p(pilot),
+                                   _p(args[0]), _p(args[1]), args[2], _p(args[3]),
+                                   _p(args[4]), args[5])
+    if rc != 0:
+        raise RuntimeError(f"oracle_bmnd_poisson failed with code {rc}")
+    if capture_matches:
+        return out, pilot, ht, wi
+    return out, pilot

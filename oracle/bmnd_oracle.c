@@ -1,0 +1,1155 @@
+/*
+ * CPU oracle for the Poisson bmnd() path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of the reference denoiser's default Poisson branch
+ * (cduhme/bmnd v0.3.0, the modules under src/bmnd/).  Every function cites the reference
+ * lines it follows.  It exists to check the HIP path (bit-for-bit where the
+ * reference's arithmetic is discrete) and to be timed as the CPU baseline.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+ * it; the product package never does.
+ *
+ * Arithmetic rules (they mirror numba/numpy semantics of the reference):
+ *  - built with -ffp-contract=off so a*b+c is never fused implicitly;
+ *  - float32 dot products of the separable transforms are written as
+ *    explicit fmaf chains in index order (the reference's BLAS sgemm order is
+ *    implementation-defined, so this order is the one this project fixes and
+ *    the GPU kernels reproduce exactly);
+ *  - the Poisson deviance is evaluated in float64 (numba promotes the
+ *    expression through the float64 `1.0/poisson_count_scale`,
+ *    blockmatching.py:338-371) with the C library's log();
+ *  - risks are float64, weights float32 (aggregationweights.py:213-259,
+ *    616-630); scatter, division and variance sums are float32.
+ *
+ * Parity pins: every function here is checked in tests/test_oracle_*.py
+ * against the reference's own known answers (blockmatching, shifts,
+ * variance, weights, scatter tests) and against brute-force / closed-form
+ * references (dense T*Sigma*T^T variance, exhaustive top-K, np.kaiser).
+ * The within-segment order of the exact-variance reduction is the one
+ * documented deviation: the reference sorts with numba's unstable argsort,
+ * this oracle (and the GPU) use the stable order (increasing member index).
+ */
+#include "bmnd_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define F32_EPS 1.1920928955078125e-07 /* np.finfo(np.float32).eps */
+
+/* ======================================================================== */
+/* geometry helpers                                                          */
+/* ======================================================================== */
+
+static int64_t prod_int(const int *v, int n) {
+  int64_t p = 1;
+  for (int i = 0; i < n; ++i) p *= v[i];
+  return p;
+}
+
+static void c_strides(int ndim, const int64_t *shape, int64_t *strides) {
+  int64_t s = 1;
+  for (int d = ndim - 1; d >= 0; --d) {
+    strides[d] = s;
+    s *= shape[d];
+  }
+}
+
+/* local coordinates of the patch entries in C order (utils.py:15-31) */
+static void local_coords(int ndim, const int *block, int p, int *coords) {
+  for (int d = ndim - 1; d >= 0; --d) {
+    coords[d] = p % block[d];
+    p /= block[d];
+  }
+}
+
+/* ======================================================================== */
+/* Kaiser window: np.kaiser restated (utils.py:5-12)                          */
+/* ======================================================================== */
+
+static long double bessel_i0(long double x) {
+  /* I0(x) = sum_k ((x/2)^2)^k / (k!)^2, summed until the terms vanish */
+  long double q = (x * 0.5L) * (x * 0.5L);
+  long double term = 1.0L, sum = 1.0L;
+  for (int k = 1; k < 500; ++k) {
+    term *= q / ((long double)k * (long double)k);
+    sum += term;
+    if (term < sum * 1e-24L) break;
+  }
+  return sum;
+}
+
+void oracle_kaiser(int m, double beta, double *out) {
+  if (m == 1) {
+    out[0] = 1.0;
+    return;
+  }
+  double alpha = (m - 1) / 2.0;
+  double denom = (double)bessel_i0((long double)beta);
+  for (int n = 0; n < m; ++n) {
+    double t = ((double)n - alpha) / alpha;
+    double arg = beta * sqrt(1.0 - t * t);
+    out[n] = (double)bessel_i0((long double)arg) / denom;
+  }
+}
+
+void oracle_kaiser_nd(int ndim, const int *block, double beta, float *out) {
+  /* w = kaiser(b0); w = outer(w, kaiser(b_d)) in float64; cast to float32 */
+  int64_t total = prod_int(block, ndim);
+  double *w = (double *)malloc(sizeof(double) * (size_t)total);
+  double axis[64];
+  oracle_kaiser(block[0], beta, axis);
+  int64_t cur = block[0];
+  for (int i = 0; i < block[0]; ++i) w[i] = axis[i];
+  for (int d = 1; d < ndim; ++d) {
+    oracle_kaiser(block[d], beta, axis);
+    for (int64_t i = cur - 1; i >= 0; --i) {
+      double wi = w[i];
+      for (int j = 0; j < block[d]; ++j) w[i * block[d] + j] = wi * axis[j];
+    }
+    cur *= block[d];
+  }
+  for (int64_t i = 0; i < total; ++i) out[i] = (float)w[i];
+  free(w);
+}
+
+/* ======================================================================== */
+/* transform matrices (transforms.py:359-440, 699-856)                        */
+/* ======================================================================== */
+
+static int is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+/* hierarchical Haar, transforms.py:393-408: rows are [coarse; detail] built
+ * by float32 Kronecker products with s = float32(1/sqrt(2)). */
+int oracle_haar_matrix(int n, float *fwd) {
+  if (!is_pow2(n)) return -1;
+  float s = (float)(1.0 / sqrt(2.0));
+  float *m = (float *)malloc(sizeof(float) * (size_t)n * n);
+  float *nm = (float *)malloc(sizeof(float) * (size_t)n * n);
+  m[0] = 1.0f;
+  int cur = 1;
+  while (cur < n) {
+    int cols = 2 * cur;
+    /* coarse = kron(m, [s, s]) : cur x 2cur */
+    for (int r = 0; r < cur; ++r)
+      for (int c = 0; c < cur; ++c) {
+        nm[r * cols + 2 * c] = m[r * cur + c] * s;
+        nm[r * cols + 2 * c + 1] = m[r * cur + c] * s;
+      }
+    /* detail = kron(eye(cur), [s, -s]) : cur x 2cur (zeros keep numpy's
+     * signs: 0*s = +0, 0*(-s) = -0) */
+    for (int r = 0; r < cur; ++r)
+      for (int c = 0; c < cur; ++c) {
+        float e = (r == c) ? 1.0f : 0.0f;
+        nm[(cur + r) * cols + 2 * c] = e * s;
+        nm[(cur + r) * cols + 2 * c + 1] = e * (-s);
+      }
+    cur = cols;
+    memcpy(m, nm, sizeof(float) * (size_t)cur * cur);
+  }
+  memcpy(fwd, m, sizeof(float) * (size_t)n * n);
+  free(m);
+  free(nm);
+  return 0;
+}
+
+/* DCT-II orthonormal, transforms.py:359-386 (also the non-power-of-two group
+ * fallback, transforms.py:425-436).  fwd[f*n+s], inv[s*n+f]. */
+void oracle_dct_matrix(int n, float *fwd, float *inv) {
+  const double pi = 3.141592653589793;
+  for (int f = 0; f < n; ++f)
+    for (int s = 0; s < n; ++s) {
+      double norm = (f == 0) ? sqrt(1.0 / n) : sqrt(2.0 / n);
+      double arg = pi * ((double)s + 0.5) * (double)f / (double)n;
+      float v = (float)(norm * cos(arg));
+      if (fwd) fwd[f * n + s] = v;
+      if (inv) inv[s * n + f] = v;
+    }
+}
+
+/* LU with partial pivoting in float64 (the computation numpy.linalg.inv does
+ * for a float32 input), result rounded to float32 (transforms.py:810). */
+static int invert_f32_via_f64(int n, const float *a, float *out) {
+  double lu[64 * 64];
+  int piv[64];
+  for (int i = 0; i < n * n; ++i) lu[i] = a[i];
+  for (int i = 0; i < n; ++i) piv[i] = i;
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    double best = fabs(lu[k * n + k]);
+    for (int i = k + 1; i < n; ++i)
+      if (fabs(lu[i * n + k]) > best) {
+        best = fabs(lu[i * n + k]);
+        p = i;
+      }
+    if (best == 0.0) return -1;
+    if (p != k) {
+      for (int j = 0; j < n; ++j) {
+        double t = lu[k * n + j];
+        lu[k * n + j] = lu[p * n + j];
+        lu[p * n + j] = t;
+      }
+      int t = piv[k];
+      piv[k] = piv[p];
+      piv[p] = t;
+    }
+    for (int i = k + 1; i < n; ++i) {
+      lu[i * n + k] /= lu[k * n + k];
+      for (int j = k + 1; j < n; ++j) lu[i * n + j] -= lu[i * n + k] * lu[k * n + j];
+    }
+  }
+  for (int col = 0; col < n; ++col) {
+    double x[64];
+    for (int i = 0; i < n; ++i) x[i] = (piv[i] == col) ? 1.0 : 0.0;
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < i; ++j) x[i] -= lu[i * n + j] * x[j];
+    for (int i = n - 1; i >= 0; --i) {
+      for (int j = i + 1; j < n; ++j) x[i] -= lu[i * n + j] * x[j];
+      x[i] /= lu[i * n + i];
+    }
+    for (int i = 0; i < n; ++i) out[i * n + col] = (float)x[i];
+  }
+  return 0;
+}
+
+/* Official bior1.5 analysis matrices for lengths 4 and 8, the literal values
+ * of transforms.py:711-806 (same table; a format the reference fixes). */
+int oracle_bior15_matrix(int n, float *fwd, float *inv) {
+  if (n == 4) {
+    const float a = 0.70710677f;
+    const float m[16] = {0.5f, 0.5f, 0.5f,  0.5f,  0.5f, 0.5f, -0.5f, -0.5f,
+                         a,    -a,   0.0f,  0.0f,  0.0f, 0.0f, a,     -a};
+    memcpy(fwd, m, sizeof(m));
+  } else if (n == 8) {
+    const float c0 = (float)0.343550200747110, c1 = (float)-0.225454819240296,
+                c2 = (float)-0.461645582253923, c3 = (float)0.569359398342840,
+                c4 = (float)0.402347308162280, c5 = (float)-0.083506045090280,
+                c6 = (float)0.707106781186550;
+    const float m[64] = {
+        c0,  c0,  c0,  c0,  c0,  c0,  c0,  c0,  /* row 0 */
+        c1,  c2,  c2,  c1,  -c1, -c2, -c2, -c1, /* row 1 */
+        c3,  c4,  -c4, -c3, c5,  -c5, c5,  -c5, /* row 2 */
+        c5,  -c5, c5,  -c5, c3,  c4,  -c4, -c3, /* row 3 */
+        c6,  -c6, 0,   0,   0,   0,   0,   0,   /* row 4 */
+        0,   0,   c6,  -c6, 0,   0,   0,   0,   /* row 5 */
+        0,   0,   0,   0,   c6,  -c6, 0,   0,   /* row 6 */
+        0,   0,   0,   0,   0,   0,   c6,  -c6  /* row 7 */
+    };
+    memcpy(fwd, m, sizeof(m));
+  } else {
+    return -1;
+  }
+  return inv ? invert_f32_via_f64(n, fwd, inv) : 0;
+}
+
+/* group transform: Haar for powers of two, DCT otherwise (transforms.py:411-440) */
+static void group_matrix(int n, float *fwd, float *inv) {
+  if (is_pow2(n)) {
+    oracle_haar_matrix(n, fwd);
+  } else {
+    oracle_dct_matrix(n, fwd, NULL);
+  }
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) inv[i * n + j] = fwd[j * n + i];
+}
+
+/* ======================================================================== */
+/* GENERATED reference schedule (shifts.py:229-266, 519-576)                  */
+/* ======================================================================== */
+
+static int axis_shifts(int block, int step, int allow_multiple, double density,
+                       int *out) {
+  if (!allow_multiple || block < 3 || step < 1 || step > block) {
+    out[0] = 0;
+    return 1;
+  }
+  double safe = density > 0.0 ? density : 0.0;
+  int count = (int)ceil(safe * (double)(block - 1) / (double)step);
+  if (count < 1) count = 1;
+  if (count > block) count = block;
+  if (count == 1) {
+    out[0] = 0;
+    return 1;
+  }
+  /* np.linspace(0, block-1, count): y[i] = i*step, last = stop exactly */
+  double stop = (double)(block - 1);
+  double lstep = stop / (double)(count - 1);
+  unsigned char seen[256] = {0};
+  seen[0] = 1;
+  for (int i = 0; i < count; ++i) {
+    double pos = (i == count - 1) ? stop : (double)i * lstep;
+    int r = (int)rint(pos); /* Python round(): half to even */
+    seen[r] = 1;
+  }
+  int n = 0;
+  for (int v = 0; v < 256; ++v)
+    if (seen[v]) out[n++] = v;
+  return n;
+}
+
+int64_t oracle_schedule_generated(int ndim, const int64_t *counts,
+                                  const int *block, const int *step,
+                                  double shift_density, int schedule_density,
+                                  int64_t *out_abs, int64_t cap) {
+  if (ndim <= 0) return 0;
+  int shifts[ORACLE_MAX_NDIM][256];
+  int nshift[ORACLE_MAX_NDIM];
+  int64_t last[ORACLE_MAX_NDIM], slots[ORACLE_MAX_NDIM], strides[ORACLE_MAX_NDIM];
+  for (int d = 0; d < ndim; ++d) {
+    nshift[d] = axis_shifts(block[d], step[d], d < (ndim - 1 > 0 ? ndim - 1 : 0),
+                            shift_density, shifts[d]);
+    last[d] = counts[d] - 1 > 0 ? counts[d] - 1 : 0;
+    slots[d] = (last[d] + step[d] - 1) / step[d] + 1;
+  }
+  c_strides(ndim, counts, strides);
+  int64_t total_grid = 1;
+  for (int d = 0; d < ndim; ++d) total_grid *= counts[d];
+  unsigned char *seen = (unsigned char *)calloc((size_t)total_grid, 1);
+  int phases = schedule_density > 1 ? schedule_density : 1;
+  int64_t nslot = 1;
+  for (int d = 0; d < ndim; ++d) nslot *= slots[d];
+  int64_t n = 0;
+  int64_t slot[ORACLE_MAX_NDIM];
+  for (int64_t sl = 0; sl < nslot; ++sl) {
+    int64_t t = sl;
+    for (int d = ndim - 1; d >= 0; --d) {
+      slot[d] = t % slots[d];
+      t /= slots[d];
+    }
+    for (int ph = 0; ph < phases; ++ph) {
+      int64_t abs_pos[ORACLE_MAX_NDIM];
+      int64_t lin = 0;
+      for (int d = 0; d < ndim; ++d) {
+        int64_t base = slot[d] * step[d];
+        if (base > last[d]) base = last[d];
+        int64_t offset = 0;
+        if (d < ndim - 1) offset = shifts[d][(slot[d + 1] + ph) % nshift[d]];
+        int64_t a = base - offset;
+        if (a < 0) a = 0;
+        if (a > last[d]) a = last[d];
+        if (slot[d] == slots[d] - 1) a = last[d];
+        abs_pos[d] = a;
+        lin += a * strides[d];
+      }
+      if (seen[lin]) continue;
+      seen[lin] = 1;
+      if (out_abs && n < cap)
+        for (int d = 0; d < ndim; ++d) out_abs[n * ndim + d] = abs_pos[d];
+      ++n;
+    }
+  }
+  free(seen);
+  return n;
+}
+
+/* ======================================================================== */
+/* block matching (blockmatching.py:322-371, 913-1205)                        */
+/* ======================================================================== */
+
+/* tie grid, blockmatching.py:957-964: q = 32 * eps32 * 2^floor(log2(max(|d|,1))) */
+float oracle_canonical_key(float d) {
+  if (!isfinite(d)) return d;
+  double mag = fabs((double)d);
+  if (mag < 1.0) mag = 1.0;
+  int e = ilogb(mag);
+  double q = 32.0 * F32_EPS * ldexp(1.0, e);
+  return (float)(rint((double)d / q) * q);
+}
+
+float oracle_patch_distance(const float *ref, const float *cand, int p,
+                            int code, double count_scale) {
+  double dist = 0.0;
+  if (code == ORACLE_DIST_SSD) {
+    for (int i = 0; i < p; ++i) {
+      float diff = cand[i] - ref[i];
+      float sq = diff * diff;
+      dist += (double)sq;
+    }
+    return (float)dist;
+  }
+  double inv = 1.0 / count_scale;
+  for (int i = 0; i < p; ++i) {
+    double r = (double)ref[i] * inv;
+    if (r < 0.0) r = 0.0;
+    double c = (double)cand[i] * inv;
+    if (c < 0.0) c = 0.0;
+    if (code == ORACLE_DIST_DEVIANCE) {
+      double comb = r + c;
+      if (comb > 0.0) {
+        double t = 0.0;
+        if (r > 0.0) t += r * log(2.0 * r / comb);
+        if (c > 0.0) t += c * log(2.0 * c / comb);
+        dist += 2.0 * t;
+      }
+    } else if (code == ORACLE_DIST_PEARSON) {
+      double comb = r + c;
+      if (comb > 0.0) {
+        double diff = c - r;
+        dist += diff * diff / comb;
+      }
+    } else {
+      double rs = 2.0 * sqrt(r + 0.375);
+      double cs = 2.0 * sqrt(c + 0.375);
+      double diff = cs - rs;
+      dist += 0.5 * diff * diff;
+    }
+  }
+  if (dist < 0.0) dist = 0.0;
+  return (float)(dist / p);
+}
+
+static int precedes(int64_t cl, float cd, int64_t el, float ed, int stabilize) {
+  if (stabilize) {
+    cd = oracle_canonical_key(cd);
+    ed = oracle_canonical_key(ed);
+    if (cd == ed) return cl < el;
+  }
+  if (cd < ed) return 1;
+  if (cd > ed) return 0;
+  return cl < el;
+}
+
+/* bounded insertion, blockmatching.py:987-1029 */
+static int insert_limited(int64_t *lins, float *dists, int limit, int size,
+                          int64_t cl, float cd, int stabilize) {
+  if (limit == 0) return size;
+  int at;
+  if (size < limit) {
+    ++size;
+    at = size - 1;
+  } else {
+    if (!precedes(cl, cd, lins[limit - 1], dists[limit - 1], stabilize))
+      return size;
+    at = limit - 1;
+  }
+  while (at > 0 && precedes(cl, cd, lins[at - 1], dists[at - 1], stabilize)) {
+    lins[at] = lins[at - 1];
+    dists[at] = dists[at - 1];
+    --at;
+  }
+  lins[at] = cl;
+  dists[at] = cd;
+  return size;
+}
+
+static int largest_pow2_at_most(int v) {
+  if (v <= 0) return 0;
+  int p = 1;
+  while ((p << 1) <= v) p <<= 1;
+  return p;
+}
+
+/* reference first, dedupe, cap, power-of-two truncation (blockmatching.py:913-954) */
+static int finalize(const int64_t *sel, int sel_count, int64_t center,
+                    int max_m, int min_m, int pow2, int64_t *out) {
+  if (max_m <= 0) return 0;
+  int cnt = 0;
+  out[cnt++] = center;
+  for (int i = 0; i < sel_count; ++i) {
+    int64_t lin = sel[i];
+    if (lin < 0 || lin == center) continue;
+    int seen = 0;
+    for (int j = 0; j < cnt; ++j)
+      if (out[j] == lin) {
+        seen = 1;
+        break;
+      }
+    if (!seen) {
+      out[cnt++] = lin;
+      if (cnt >= max_m) break;
+    }
+  }
+  if (pow2 && cnt > 0) {
+    int target = largest_pow2_at_most(cnt);
+    if (min_m > target) target = min_m;
+    if (target < cnt) cnt = target;
+  }
+  return cnt;
+}
+
+void oracle_blockmatch(const float *patches_flat, const float *volume,
+                       int ndim, const int64_t *vol_shape,
+                       const int64_t *counts, const int *block,
+                       const int64_t *refs, int64_t nrefs, const int *radius,
+                       const float *thresholds, int max_matches,
+                       int min_matches, int pow2, int stabilize, int code,
+                       double count_scale, int nthreads, int64_t *sel_out,
+                       int32_t *count_out, int64_t *accepted_out) {
+  int P = (int)prod_int(block, ndim);
+  int64_t cstr[ORACLE_MAX_NDIM], vstr[ORACLE_MAX_NDIM];
+  c_strides(ndim, counts, cstr);
+  int64_t *poff = (int64_t *)malloc(sizeof(int64_t) * (size_t)P);
+  if (!patches_flat) {
+    c_strides(ndim, vol_shape, vstr);
+    for (int p = 0; p < P; ++p) {
+      int lc[ORACLE_MAX_NDIM];
+      local_coords(ndim, block, p, lc);
+      int64_t o = 0;
+      for (int d = 0; d < ndim; ++d) o += lc[d] * vstr[d];
+      poff[p] = o;
+    }
+  }
+  int stab = stabilize && code != ORACLE_DIST_SSD;
+  int required = min_matches < max_matches ? min_matches : max_matches;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel
+#endif
+  {
+    float *refbuf = (float *)malloc(sizeof(float) * (size_t)P);
+    float *candbuf = (float *)malloc(sizeof(float) * (size_t)P);
+    int64_t *acc_l = (int64_t *)malloc(sizeof(int64_t) * (size_t)(max_matches + 1));
+    float *acc_d = (float *)malloc(sizeof(float) * (size_t)(max_matches + 1));
+    int64_t *rej_l = (int64_t *)malloc(sizeof(int64_t) * (size_t)(required + 1));
+    float *rej_d = (float *)malloc(sizeof(float) * (size_t)(required + 1));
+    int64_t *selb = (int64_t *)malloc(sizeof(int64_t) * (size_t)(max_matches + 1));
+    int64_t *fin = (int64_t *)malloc(sizeof(int64_t) * (size_t)(max_matches + 1));
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 16)
+#endif
+    for (int64_t ri = 0; ri < nrefs; ++ri) {
+      const int64_t *center = refs + ri * ndim;
+      int64_t center_lin = 0;
+      for (int d = 0; d < ndim; ++d) center_lin += center[d] * cstr[d];
+      const float *refp;
+      if (patches_flat) {
+        refp = patches_flat + center_lin * P;
+      } else {
+        int64_t base = 0;
+        for (int d = 0; d < ndim; ++d) base += center[d] * vstr[d];
+        for (int p = 0; p < P; ++p) refbuf[p] = volume[base + poff[p]];
+        refp = refbuf;
+      }
+      int64_t lo[ORACLE_MAX_NDIM], ext[ORACLE_MAX_NDIM], coord[ORACLE_MAX_NDIM];
+      int64_t box = 1;
+      for (int d = 0; d < ndim; ++d) {
+        int64_t l = center[d] - radius[d];
+        if (l < 0) l = 0;
+        int64_t h = center[d] + radius[d];
+        if (h > counts[d] - 1) h = counts[d] - 1;
+        lo[d] = l;
+        ext[d] = h - l + 1;
+        coord[d] = l;
+        box *= ext[d];
+      }
+      float thr = thresholds[ri];
+      float thr_key = stab ? oracle_canonical_key(thr) : thr;
+      int acc_size = 0, rej_size = 0;
+      int64_t accepted = 0;
+      for (int64_t b = 0; b < box; ++b) {
+        int64_t lin = 0;
+        for (int d = 0; d < ndim; ++d) lin += coord[d] * cstr[d];
+        const float *cp;
+        if (patches_flat) {
+          cp = patches_flat + lin * P;
+        } else {
+          int64_t base = 0;
+          for (int d = 0; d < ndim; ++d) base += coord[d] * vstr[d];
+          for (int p = 0; p < P; ++p) candbuf[p] = volume[base + poff[p]];
+          cp = candbuf;
+        }
+        float dist = oracle_patch_distance(refp, cp, P, code, count_scale);
+        int ok = stab ? (oracle_canonical_key(dist) < thr_key) : (dist < thr);
+        if (ok) {
+          ++accepted;
+          acc_size = insert_limited(acc_l, acc_d, max_matches, acc_size, lin, dist, stab);
+        } else {
+          rej_size = insert_limited(rej_l, rej_d, required, rej_size, lin, dist, stab);
+        }
+        for (int d = ndim - 1; d >= 0; --d) {
+          coord[d] += 1;
+          if (coord[d] < lo[d] + ext[d]) break;
+          coord[d] = lo[d];
+        }
+      }
+      int sel_count = acc_size;
+      for (int i = 0; i < acc_size; ++i) selb[i] = acc_l[i];
+      if (sel_count < required) {
+        int need = required - sel_count;
+        int take = need < rej_size ? need : rej_size;
+        for (int i = 0; i < take; ++i) selb[sel_count++] = rej_l[i];
+      }
+      int fc = finalize(selb, sel_count, center_lin, max_matches, min_matches, pow2, fin);
+      for (int i = 0; i < max_matches; ++i) sel_out[ri * max_matches + i] = i < fc ? fin[i] : -1;
+      count_out[ri] = fc;
+      if (accepted_out) accepted_out[ri] = accepted;
+    }
+    free(refbuf);
+    free(candbuf);
+    free(acc_l);
+    free(acc_d);
+    free(rej_l);
+    free(rej_d);
+    free(selb);
+    free(fin);
+  }
+  free(poff);
+}
+
+/* ======================================================================== */
+/* separable transforms (transforms.py:209-255)                               */
+/* ======================================================================== */
+
+/* data has shape dims[0..ndim_total-1]; mats[a] is dims[a] x dims[a]
+ * (row-major, out index first) or NULL to skip the axis.  Axes are applied in
+ * order 0,1,...; each output is an fmaf chain over the input index. */
+void oracle_apply_transform(float *data, int ndim_total, const int *dims,
+                            const float *const *mats) {
+  int64_t total = prod_int(dims, ndim_total);
+  float *tmp = (float *)malloc(sizeof(float) * (size_t)total);
+  for (int a = 0; a < ndim_total; ++a) {
+    const float *m = mats[a];
+    if (!m) continue;
+    int n = dims[a];
+    int64_t inner = 1;
+    for (int d = a + 1; d < ndim_total; ++d) inner *= dims[d];
+    int64_t outer = total / (inner * n);
+    for (int64_t o = 0; o < outer; ++o)
+      for (int r = 0; r < n; ++r)
+        for (int64_t in = 0; in < inner; ++in) {
+          float acc = 0.0f;
+          const float *x = data + o * n * inner + in;
+          for (int k = 0; k < n; ++k) acc = fmaf(m[r * n + k], x[k * inner], acc);
+          tmp[o * n * inner + r * inner + in] = acc;
+        }
+    memcpy(data, tmp, sizeof(float) * (size_t)total);
+  }
+  free(tmp);
+}
+
+/* ======================================================================== */
+/* exact overlap-aware Poisson variance (variance.py:652-975, 1634-1711)      */
+/* ======================================================================== */
+
+/* full separable spatial matrix, kron chain in float32 (variance.py:744-755) */
+static float *spatial_kron(int ndim, const int *block, const float *const *mats) {
+  int P = (int)prod_int(block, ndim);
+  float *full = (float *)malloc(sizeof(float) * (size_t)P * P);
+  float *tmp = (float *)malloc(sizeof(float) * (size_t)P * P);
+  full[0] = 1.0f;
+  int cur = 1;
+  for (int d = 0; d < ndim; ++d) {
+    int b = block[d];
+    int nc = cur * b;
+    for (int r = 0; r < cur; ++r)
+      for (int c = 0; c < cur; ++c)
+        for (int rr = 0; rr < b; ++rr)
+          for (int cc = 0; cc < b; ++cc) {
+            float mv = mats[d] ? mats[d][rr * b + cc] : (rr == cc ? 1.0f : 0.0f);
+            tmp[(r * b + rr) * nc + (c * b + cc)] = full[r * cur + c] * mv;
+          }
+    cur = nc;
+    memcpy(full, tmp, sizeof(float) * (size_t)cur * cur);
+  }
+  free(tmp);
+  return full;
+}
+
+int oracle_exact_variance(const float *var_volume, int ndim,
+                          const int64_t *vol_shape, const int *block, int n,
+                          const int64_t *abs_pos, const float *group_mat,
+                          const float *const *spatial_mats, int planes,
+                          float *var_out) {
+  int P = (int)prod_int(block, ndim);
+  int64_t mn[ORACLE_MAX_NDIM], ushape[ORACLE_MAX_NDIM], ustr[ORACLE_MAX_NDIM],
+      vstr[ORACLE_MAX_NDIM];
+  for (int d = 0; d < ndim; ++d) {
+    mn[d] = abs_pos[d];
+    for (int i = 1; i < n; ++i)
+      if (abs_pos[i * ndim + d] < mn[d]) mn[d] = abs_pos[i * ndim + d];
+  }
+  for (int d = 0; d < ndim; ++d) {
+    int64_t mx = abs_pos[d] - mn[d];
+    for (int i = 1; i < n; ++i)
+      if (abs_pos[i * ndim + d] - mn[d] > mx) mx = abs_pos[i * ndim + d] - mn[d];
+    ushape[d] = mx + block[d];
+  }
+  c_strides(ndim, ushape, ustr);
+  c_strides(ndim, vol_shape, vstr);
+  int64_t ucells = 1;
+  for (int d = 0; d < ndim; ++d) ucells *= ushape[d];
+  int entries = n * P;
+  int64_t *label = (int64_t *)malloc(sizeof(int64_t) * (size_t)entries);
+  for (int i = 0; i < n; ++i)
+    for (int p = 0; p < P; ++p) {
+      int lc[ORACLE_MAX_NDIM];
+      local_coords(ndim, block, p, lc);
+      int64_t l = 0;
+      for (int d = 0; d < ndim; ++d) l += (abs_pos[i * ndim + d] - mn[d] + lc[d]) * ustr[d];
+      label[i * P + p] = l;
+    }
+  /* stable counting sort by label: entries of one union voxel stay in
+   * increasing entry (member) order */
+  int32_t *cnt = (int32_t *)calloc((size_t)ucells + 1, sizeof(int32_t));
+  for (int e = 0; e < entries; ++e) cnt[label[e] + 1]++;
+  for (int64_t c = 0; c < ucells; ++c) cnt[c + 1] += cnt[c];
+  int32_t *order = (int32_t *)malloc(sizeof(int32_t) * (size_t)entries);
+  int32_t *fill = (int32_t *)malloc(sizeof(int32_t) * (size_t)(ucells + 1));
+  memcpy(fill, cnt, sizeof(int32_t) * (size_t)(ucells + 1));
+  for (int e = 0; e < entries; ++e) order[fill[label[e]]++] = e;
+
+  float *S = spatial_kron(ndim, block, spatial_mats);
+  int C = planes * P;
+  float *acc = (float *)calloc((size_t)C, sizeof(float));
+  float *contrib = (float *)malloc(sizeof(float) * (size_t)C);
+  for (int64_t cell = 0; cell < ucells; ++cell) {
+    int beg = cnt[cell], end = cnt[cell + 1];
+    if (beg == end) continue;
+    for (int c = 0; c < C; ++c) contrib[c] = 0.0f;
+    for (int pos = beg; pos < end; ++pos) {
+      int e = order[pos];
+      int i = e / P, p = e % P;
+      for (int q = 0; q < planes; ++q) {
+        float g = group_mat[q * n + i];
+        for (int s = 0; s < P; ++s) contrib[q * P + s] += g * S[s * P + p];
+      }
+    }
+    /* union voxel variance at abs = union coords + group origin */
+    int64_t t = cell, off = 0;
+    for (int d = ndim - 1; d >= 0; --d) {
+      int64_t u = t % ushape[d];
+      t /= ushape[d];
+      off += (u + mn[d]) * vstr[d];
+    }
+    float vv = var_volume[off];
+    for (int c = 0; c < C; ++c) {
+      float v = contrib[c];
+      acc[c] += (vv * v) * v;
+    }
+  }
+  memcpy(var_out, acc, sizeof(float) * (size_t)C);
+  free(label);
+  free(cnt);
+  free(order);
+  free(fill);
+  free(S);
+  free(acc);
+  free(contrib);
+  return 0;
+}
+
+/* plane limiting, variance.py:1572-1631 (float32 sequential plane sums) */
+void oracle_limit_planes(const float *exact_std, const float *approx_std,
+                         int n, int p, int k, float *out) {
+  int epc = k < n ? k : n;
+  if (epc == 0) {
+    for (int i = 0; i < n * p; ++i) {
+      float a = approx_std[i] > 0.0f ? approx_std[i] : 0.0f;
+      out[i] = sqrtf(a * a);
+    }
+    return;
+  }
+  if (epc == n) {
+    for (int i = 0; i < n * p; ++i) {
+      float e = exact_std[i] > 0.0f ? exact_std[i] : 0.0f;
+      out[i] = sqrtf(e * e);
+    }
+    return;
+  }
+  for (int s = 0; s < p; ++s) {
+    float tot_a = 0.0f, tot_e = 0.0f, rem_a = 0.0f;
+    for (int q = 0; q < n; ++q) {
+      float a = approx_std[q * p + s] > 0.0f ? approx_std[q * p + s] : 0.0f;
+      tot_a += a * a;
+      if (q >= epc) rem_a += a * a;
+    }
+    for (int q = 0; q < epc; ++q) {
+      float e = exact_std[q * p + s] > 0.0f ? exact_std[q * p + s] : 0.0f;
+      tot_e += e * e;
+    }
+    float target = tot_a - tot_e;
+    if (target < 0.0f) target = 0.0f;
+    float scale = rem_a > 0.0f ? target / rem_a : 0.0f;
+    float fallback = target / (float)(n - epc);
+    for (int q = 0; q < n; ++q) {
+      float v;
+      if (q < epc) {
+        float e = exact_std[q * p + s] > 0.0f ? exact_std[q * p + s] : 0.0f;
+        v = e * e;
+      } else {
+        float a = approx_std[q * p + s] > 0.0f ? approx_std[q * p + s] : 0.0f;
+        v = (a * a) * scale;
+        if (!(rem_a > 0.0f)) v = fallback;
+      }
+      out[q * p + s] = sqrtf(v > 0.0f ? v : 0.0f);
+    }
+  }
+}
+
+int oracle_noise_std(const float *var_volume, int ndim,
+                     const int64_t *vol_shape, const int *block, int n,
+                     const int64_t *abs_pos, const float *group_mat,
+                     const float *const *spatial_mats, int k,
+                     float *sigma_out) {
+  int P = (int)prod_int(block, ndim);
+  int64_t vstr[ORACLE_MAX_NDIM];
+  c_strides(ndim, vol_shape, vstr);
+  /* approximate planes: squared transforms of the variance patches
+   * (variance.py:613-623, 1647-1653) */
+  int dims[ORACLE_MAX_NDIM + 1];
+  dims[0] = n;
+  for (int d = 0; d < ndim; ++d) dims[d + 1] = block[d];
+  float *approx = (float *)malloc(sizeof(float) * (size_t)n * P);
+  for (int i = 0; i < n; ++i) {
+    int64_t base = 0;
+    for (int d = 0; d < ndim; ++d) base += abs_pos[i * ndim + d] * vstr[d];
+    for (int p = 0; p < P; ++p) {
+      int lc[ORACLE_MAX_NDIM];
+      local_coords(ndim, block, p, lc);
+      int64_t o = base;
+      for (int d = 0; d < ndim; ++d) o += lc[d] * vstr[d];
+      float v = var_volume[o];
+      approx[i * P + p] = v > 0.0f ? v : 0.0f;
+    }
+  }
+  float *sq[ORACLE_MAX_NDIM + 1];
+  sq[0] = (float *)malloc(sizeof(float) * (size_t)n * n);
+  for (int i = 0; i < n * n; ++i) sq[0][i] = group_mat[i] * group_mat[i];
+  for (int d = 0; d < ndim; ++d) {
+    int b = block[d];
+    if (!spatial_mats[d]) {
+      sq[d + 1] = NULL;
+      continue;
+    }
+    sq[d + 1] = (float *)malloc(sizeof(float) * (size_t)b * b);
+    for (int i = 0; i < b * b; ++i) sq[d + 1][i] = spatial_mats[d][i] * spatial_mats[d][i];
+  }
+  oracle_apply_transform(approx, ndim + 1, dims, (const float *const *)sq);
+  for (int i = 0; i < n * P; ++i) approx[i] = sqrtf(approx[i] > 0.0f ? approx[i] : 0.0f);
+  for (int d = 0; d <= ndim; ++d) free(sq[d]);
+
+  if (k == 0) {
+    memcpy(sigma_out, approx, sizeof(float) * (size_t)n * P);
+    free(approx);
+    return 0;
+  }
+  int planes = k < n ? k : n;
+  float *exact = (float *)malloc(sizeof(float) * (size_t)planes * P);
+  oracle_exact_variance(var_volume, ndim, vol_shape, block, n, abs_pos, group_mat,
+                        spatial_mats, planes, exact);
+  for (int i = 0; i < planes * P; ++i) exact[i] = sqrtf(exact[i] > 0.0f ? exact[i] : 0.0f);
+  if (planes == n) {
+    memcpy(sigma_out, exact, sizeof(float) * (size_t)n * P);
+  } else {
+    oracle_limit_planes(exact, approx, n, P, k, sigma_out);
+  }
+  free(exact);
+  free(approx);
+  return 0;
+}
+
+/* per-patch inverse risk weights, VARIANCE/COEFFICIENT/PATCH
+ * (aggregationweights.py:213-259, 616-653): risk_i = sum_j Ginv[i,j]^2 *
+ * sum_c a[j,c]^2 sigma[j,c]^2 in float64; w_i = f32(1/(risk_i + eps)). */
+void oracle_patch_weights(const float *atten, const float *sigma, int n,
+                          int p, const float *group_inv, double eps,
+                          float *w_out) {
+  double plane[64];
+  for (int j = 0; j < n; ++j) {
+    double risk = 0.0;
+    for (int c = 0; c < p; ++c) {
+      double a = (double)atten[j * p + c];
+      float nv = sigma[j * p + c] * sigma[j * p + c];
+      double err = a * a;
+      err *= (double)nv;
+      risk += err;
+    }
+    plane[j] = risk;
+  }
+  for (int i = 0; i < n; ++i) {
+    double risk = 0.0;
+    for (int j = 0; j < n; ++j) {
+      double m = (double)group_inv[i * n + j];
+      risk += (m * m) * plane[j];
+    }
+    if (!isfinite(risk) || risk < 0.0)
+      w_out[i] = 0.0f;
+    else
+      w_out[i] = (float)(1.0 / (risk + eps));
+  }
+}
+
+/* ======================================================================== */
+/* full two-stage pipeline (bmndalgo.py:71-1063, Poisson branches)            */
+/* ======================================================================== */
+
+typedef struct {
+  int ndim;
+  int64_t shape[ORACLE_MAX_NDIM], vstr[ORACLE_MAX_NDIM];
+  int64_t counts[ORACLE_MAX_NDIM], cstr[ORACLE_MAX_NDIM];
+  int block[ORACLE_MAX_NDIM];
+  int P;
+  int64_t *poff;
+  float *kaiser;
+  float *sp_fwd[ORACLE_MAX_NDIM], *sp_inv[ORACLE_MAX_NDIM];
+} stage_geom;
+
+static void geom_init(stage_geom *g, const oracle_params *prm, const int *block,
+                      double beta, int wiener) {
+  g->ndim = prm->ndim;
+  for (int d = 0; d < g->ndim; ++d) {
+    g->shape[d] = prm->shape[d];
+    g->block[d] = block[d];
+    g->counts[d] = prm->shape[d] - block[d] + 1;
+  }
+  c_strides(g->ndim, g->shape, g->vstr);
+  c_strides(g->ndim, g->counts, g->cstr);
+  g->P = (int)prod_int(block, g->ndim);
+  g->poff = (int64_t *)malloc(sizeof(int64_t) * (size_t)g->P);
+  for (int p = 0; p < g->P; ++p) {
+    int lc[ORACLE_MAX_NDIM];
+    local_coords(g->ndim, block, p, lc);
+    int64_t o = 0;
+    for (int d = 0; d < g->ndim; ++d) o += lc[d] * g->vstr[d];
+    g->poff[p] = o;
+  }
+  g->kaiser = (float *)malloc(sizeof(float) * (size_t)g->P);
+  oracle_kaiser_nd(g->ndim, block, beta, g->kaiser);
+  for (int d = 0; d < g->ndim; ++d) {
+    int b = block[d];
+    g->sp_fwd[d] = (float *)malloc(sizeof(float) * (size_t)b * b);
+    g->sp_inv[d] = (float *)malloc(sizeof(float) * (size_t)b * b);
+    if (wiener) {
+      oracle_dct_matrix(b, g->sp_fwd[d], g->sp_inv[d]);
+    } else if (oracle_bior15_matrix(b, g->sp_fwd[d], g->sp_inv[d]) != 0) {
+      /* bior1.5 lengths other than 4/8 go through pywt in the reference;
+       * not part of this oracle */
+      oracle_dct_matrix(b, g->sp_fwd[d], g->sp_inv[d]);
+    }
+  }
+}
+
+static void geom_free(stage_geom *g) {
+  free(g->poff);
+  free(g->kaiser);
+  for (int d = 0; d < g->ndim; ++d) {
+    free(g->sp_fwd[d]);
+    free(g->sp_inv[d]);
+  }
+}
+
+static void lin_to_abs(const stage_geom *g, int64_t lin, int64_t *abs_pos) {
+  for (int d = g->ndim - 1; d >= 0; --d) {
+    abs_pos[d] = lin % g->counts[d];
+    lin /= g->counts[d];
+  }
+}
+
+static int64_t lin_to_vol_base(const stage_geom *g, int64_t lin) {
+  int64_t base = 0;
+  for (int d = g->ndim - 1; d >= 0; --d) {
+    base += (lin % g->counts[d]) * g->vstr[d];
+    lin /= g->counts[d];
+  }
+  return base;
+}
+
+/* variance model, variance.py:556-598: max(scale*max(v,0), floor) in float32 */
+static void variance_volume(const float *v, int64_t n, double scale, double floor_,
+                            float *out) {
+  float s = (float)scale, f = (float)floor_;
+  for (int64_t i = 0; i < n; ++i) {
+    float x = v[i] > 0.0f ? v[i] : 0.0f;
+    x = s * x;
+    out[i] = x > f ? x : f;
+  }
+}
+
+/* one stage: block matching + group filtering + sequential aggregation */
+static int run_stage(const oracle_params *prm, int wiener, const float *match_vol,
+                     const float *noisy, const float *var_vol, const float *fallback,
+                     float *est, int64_t *sel_dump, int32_t *cnt_dump, int64_t dump_cap) {
+  const int *block = wiener ? prm->wi_block : prm->ht_block;
+  const int *step = wiener ? prm->wi_step : prm->ht_step;
+  const int *radius = wiener ? prm->wi_radius : prm->ht_radius;
+  int max_m = wiener ? prm->wi_max : prm->ht_max;
+  int min_m = wiener ? prm->wi_min : prm->ht_min;
+  float thr = wiener ? prm->wi_threshold : prm->ht_threshold;
+  stage_geom g;
+  geom_init(&g, prm, block, wiener ? prm->wi_beta : prm->ht_beta, wiener);
+  int ndim = g.ndim, P = g.P;
+  int64_t nvox = 1;
+  for (int d = 0; d < ndim; ++d) nvox *= g.shape[d];
+
+  int64_t nrefs = oracle_schedule_generated(ndim, g.counts, block, step, prm->shift_density,
+                                            prm->schedule_density, NULL, 0);
+  int64_t *refs = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nrefs * ndim + 1));
+  oracle_schedule_generated(ndim, g.counts, block, step, prm->shift_density,
+                            prm->schedule_density, refs, nrefs);
+  float *thrs = (float *)malloc(sizeof(float) * (size_t)(nrefs + 1));
+  for (int64_t i = 0; i < nrefs; ++i) thrs[i] = thr;
+  int64_t *sel = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nrefs * max_m + 1));
+  int32_t *cnt = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nrefs + 1));
+  oracle_blockmatch(NULL, match_vol, ndim, g.shape, g.counts, block, refs, nrefs, radius,
+                    thrs, max_m, min_m, 1, 1, ORACLE_DIST_DEVIANCE, prm->count_scale,
+                    prm->nthreads, sel, cnt, NULL);
+  if (sel_dump && cnt_dump) {
+    int64_t lim = nrefs < dump_cap ? nrefs : dump_cap;
+    memcpy(sel_dump, sel, sizeof(int64_t) * (size_t)(lim * max_m));
+    memcpy(cnt_dump, cnt, sizeof(int32_t) * (size_t)lim);
+  }
+
+  int64_t *off = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nrefs + 1));
+  off[0] = 0;
+  for (int64_t r = 0; r < nrefs; ++r) off[r + 1] = off[r] + cnt[r];
+  float *filt = (float *)malloc(sizeof(float) * (size_t)(off[nrefs] * P + 1));
+  float *wts = (float *)malloc(sizeof(float) * (size_t)(off[nrefs] + 1));
+
+#ifdef _OPENMP
+#pragma omp parallel
+#endif
+  {
+    int nmax = max_m;
+    float *gfwd = (float *)malloc(sizeof(float) * (size_t)nmax * nmax);
+    float *ginv = (float *)malloc(sizeof(float) * (size_t)nmax * nmax);
+    float *X = (float *)malloc(sizeof(float) * (size_t)nmax * P);
+    float *Y = (float *)malloc(sizeof(float) * (size_t)nmax * P);
+    float *sig = (float *)malloc(sizeof(float) * (size_t)nmax * P);
+    float *att = (float *)malloc(sizeof(float) * (size_t)nmax * P);
+    int64_t *apos = (int64_t *)malloc(sizeof(int64_t) * (size_t)nmax * ndim);
+    int last_n = -1;
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 8)
+#endif
+    for (int64_t r = 0; r < nrefs; ++r) {
+      int n = cnt[r];
+      if (n == 0) continue;
+      if (n != last_n) {
+        group_matrix(n, gfwd, ginv);
+        last_n = n;
+      }
+      int dims[ORACLE_MAX_NDIM + 1];
+      dims[0] = n;
+      for (int d = 0; d < ndim; ++d) dims[d + 1] = block[d];
+      const float *fm[ORACLE_MAX_NDIM + 1], *im[ORACLE_MAX_NDIM + 1], *spf[ORACLE_MAX_NDIM];
+      fm[0] = gfwd;
+      im[0] = ginv;
+      for (int d = 0; d < ndim; ++d) {
+        fm[d + 1] = g.sp_fwd[d];
+        im[d + 1] = g.sp_inv[d];
+        spf[d] = g.sp_fwd[d];
+      }
+      for (int i = 0; i < n; ++i) {
+        int64_t lin = sel[r * max_m + i];
+        lin_to_abs(&g, lin, apos + i * ndim);
+        int64_t base = lin_to_vol_base(&g, lin);
+        for (int p = 0; p < P; ++p) X[i * P + p] = noisy[base + g.poff[p]];
+        if (wiener)
+          for (int p = 0; p < P; ++p) Y[i * P + p] = match_vol[base + g.poff[p]];
+      }
+      oracle_apply_transform(X, ndim + 1, dims, fm);
+      oracle_noise_std(var_vol, ndim, g.shape, block, n, apos, gfwd, spf, prm->k, sig);
+      if (!wiener) {
+        /* hard threshold, bmndalgo.py:460-485 and poisson.py:30-46 */
+        float lam = prm->ht_lambda;
+        double cs = prm->count_scale;
+        for (int e = 0; e < n * P; ++e) {
+          float t = lam * sig[e];
+          double cc = (double)fabsf(X[e]) / cs;
+          double tc = (double)t / cs;
+          double sc = cc > tc ? cc : tc;
+          if (sc < 1.0) sc = 1.0;
+          double tol = (128.0 * F32_EPS) * sc;
+          int keep = cc > tc + tol;
+          att[e] = keep ? 1.0f : 0.0f;
+          X[e] = X[e] * att[e];
+        }
+      } else {
+        /* Wiener gain, wiener.py:24-53, 78-95 and bmndalgo.py:878-891 */
+        oracle_apply_transform(Y, ndim + 1, dims, fm);
+        float mu = prm->wi_mu;
+        float eps32 = (float)1e-10;
+        for (int e = 0; e < n * P; ++e) {
+          float nv = sig[e] * sig[e];
+          float eff = mu * nv;
+          float pr = fabsf(Y[e]) * fabsf(Y[e]);
+          float spow = pr - eff;
+          if (spow < 0.0f) spow = 0.0f;
+          float gain = spow / ((spow + eff) + eps32);
+          if (gain < 0.0f) gain = 0.0f;
+          if (gain > 1.0f) gain = 1.0f;
+          att[e] = gain;
+          X[e] = X[e] * gain;
+        }
+      }
+      oracle_apply_transform(X, ndim + 1, dims, im);
+      float *fo = filt + off[r] * P;
+      memcpy(fo, X, sizeof(float) * (size_t)n * P);
+      oracle_patch_weights(att, sig, n, P, ginv, 1e-10, wts + off[r]);
+    }
+    free(gfwd);
+    free(ginv);
+    free(X);
+    free(Y);
+    free(sig);
+    free(att);
+    free(apos);
+  }
+
+  /* sequential aggregation in schedule order (patchaggregation.py:170-199) */
+  float *num = (float *)calloc((size_t)nvox, sizeof(float));
+  float *den = (float *)calloc((size_t)nvox, sizeof(float));
+  for (int64_t r = 0; r < nrefs; ++r) {
+    for (int i = 0; i < cnt[r]; ++i) {
+      int64_t base = lin_to_vol_base(&g, sel[r * max_m + i]);
+      float w = wts[off[r] + i];
+      const float *fv = filt + (off[r] + i) * P;
+      for (int p = 0; p < P; ++p) {
+        float ww = w * g.kaiser[p];
+        int64_t v = base + g.poff[p];
+        num[v] += fv[p] * ww;
+        den[v] += ww;
+      }
+    }
+  }
+  /* normalisation, bmndalgo.py:626-631 / 1039-1044 */
+  float eps32 = (float)1e-10;
+  for (int64_t v = 0; v < nvox; ++v) est[v] = den[v] > 0.0f ? num[v] / (den[v] + eps32) : fallback[v];
+
+  free(num);
+  free(den);
+  free(filt);
+  free(wts);
+  free(off);
+  free(sel);
+  free(cnt);
+  free(thrs);
+  free(refs);
+  geom_free(&g);
+  return 0;
+}
+
+int oracle_bmnd_poisson(const float *vol, const oracle_params *prm, float *out,
+                        float *pilot, int64_t *ht_sel, int32_t *ht_cnt, int64_t ht_cap,
+                        int64_t *wi_sel, int32_t *wi_cnt, int64_t wi_cap) {
+  if (prm->ndim < 1 || prm->ndim > ORACLE_MAX_NDIM) return -1;
+  int64_t nvox = 1;
+  for (int d = 0; d < prm->ndim; ++d) nvox *= prm->shape[d];
+#ifdef _OPENMP
+  if (prm->nthreads > 0) omp_set_num_threads(prm->nthreads);
+#endif
+  int any = 0;
+  for (int64_t i = 0; i < nvox; ++i)
+    if (vol[i] != 0.0f) {
+      any = 1;
+      break;
+    }
+  if (!any) { /* all-zero fast path, bmndalgo.py:160-171 */
+    memcpy(out, vol, sizeof(float) * (size_t)nvox);
+    if (pilot) memcpy(pilot, vol, sizeof(float) * (size_t)nvox);
+    return 0;
+  }
+  float *var1 = (float *)malloc(sizeof(float) * (size_t)nvox);
+  float *est1 = (float *)malloc(sizeof(float) * (size_t)nvox);
+  variance_volume(vol, nvox, prm->count_scale, prm->variance_floor, var1);
+  run_stage(prm, 0, vol, vol, var1, vol, est1, ht_sel, ht_cnt, ht_cap);
+  if (pilot) memcpy(pilot, est1, sizeof(float) * (size_t)nvox);
+  variance_volume(est1, nvox, prm->count_scale, prm->variance_floor, var1);
+  run_stage(prm, 1, est1, vol, var1, vol, out, wi_sel, wi_cnt, wi_cap);
+  free(var1);
+  free(est1);
+  return 0;
+}

@@ -1,0 +1,104 @@
+/*
+ * CPU oracle for the Poisson bmnd() path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * This library restates, in plain C, the arithmetic of the reference
+ * denoiser (cduhme/bmnd v0.3.0, src/bmnd/) for the default Poisson branch.
+ * It is the checker the GPU path is compared against and the CPU baseline
+ * timed beside it.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  The product (bmnd_amd) never does.
+ *
+ * Conventions: arrays are C-contiguous; positions and linear indices are
+ * int64; the patch grid has counts[d] = shape[d] - block[d] + 1 origins.
+ */
+#ifndef BMND_ORACLE_H
+#define BMND_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORACLE_MAX_NDIM 4
+
+/* distance codes, blockmatching.py:13-16 */
+#define ORACLE_DIST_SSD 0
+#define ORACLE_DIST_DEVIANCE 1
+#define ORACLE_DIST_PEARSON 2
+#define ORACLE_DIST_ANSCOMBE 3
+
+/* ---- host geometry ---------------------------------------------------- */
+void oracle_kaiser(int m, double beta, double *out);
+void oracle_kaiser_nd(int ndim, const int *block, double beta, float *out);
+int oracle_haar_matrix(int n, float *fwd);
+void oracle_dct_matrix(int n, float *fwd, float *inv);
+int oracle_bior15_matrix(int n, float *fwd, float *inv);
+int64_t oracle_schedule_generated(int ndim, const int64_t *counts,
+                                  const int *block, const int *step,
+                                  double shift_density, int schedule_density,
+                                  int64_t *out_abs, int64_t cap);
+
+/* ---- block matching ---------------------------------------------------- */
+float oracle_canonical_key(float d);
+float oracle_patch_distance(const float *ref, const float *cand, int p,
+                            int code, double count_scale);
+void oracle_blockmatch(const float *patches_flat, const float *volume,
+                       int ndim, const int64_t *vol_shape,
+                       const int64_t *counts, const int *block,
+                       const int64_t *refs, int64_t nrefs, const int *radius,
+                       const float *thresholds, int max_matches,
+                       int min_matches, int pow2, int stabilize, int code,
+                       double count_scale, int nthreads, int64_t *sel_out,
+                       int32_t *count_out, int64_t *accepted_out);
+
+/* ---- group statistics --------------------------------------------------- */
+void oracle_apply_transform(float *data, int ndim_total, const int *dims,
+                            const float *const *mats);
+int oracle_exact_variance(const float *var_volume, int ndim,
+                          const int64_t *vol_shape, const int *block, int n,
+                          const int64_t *abs_pos, const float *group_mat,
+                          const float *const *spatial_mats, int planes,
+                          float *var_out);
+int oracle_noise_std(const float *var_volume, int ndim,
+                     const int64_t *vol_shape, const int *block, int n,
+                     const int64_t *abs_pos, const float *group_mat,
+                     const float *const *spatial_mats, int k,
+                     float *sigma_out);
+void oracle_limit_planes(const float *exact_std, const float *approx_std,
+                         int n, int p, int k, float *out);
+void oracle_patch_weights(const float *atten, const float *sigma, int n,
+                          int p, const float *group_inv, double eps,
+                          float *w_out);
+
+/* ---- full pipeline ----------------------------------------------------- */
+typedef struct {
+  int ndim;
+  int64_t shape[ORACLE_MAX_NDIM];
+  int ht_block[ORACLE_MAX_NDIM], ht_step[ORACLE_MAX_NDIM],
+      ht_radius[ORACLE_MAX_NDIM];
+  int wi_block[ORACLE_MAX_NDIM], wi_step[ORACLE_MAX_NDIM],
+      wi_radius[ORACLE_MAX_NDIM];
+  int ht_max, ht_min, wi_max, wi_min;
+  float ht_threshold, wi_threshold;
+  float ht_lambda, wi_mu;
+  double ht_beta, wi_beta;
+  double shift_density;
+  int schedule_density;
+  int k;
+  double count_scale;
+  double variance_floor;
+  int nthreads;
+} oracle_params;
+
+/* Runs both stages.  pilot may be NULL.  The match lists of each stage can
+ * be captured: *_sel has room for nrefs*max entries, *_cnt for nrefs; pass
+ * NULL to skip.  Returns 0, or a negative code on invalid arguments. */
+int oracle_bmnd_poisson(const float *vol, const oracle_params *prm,
+                        float *out, float *pilot, int64_t *ht_sel,
+                        int32_t *ht_cnt, int64_t ht_cap, int64_t *wi_sel,
+                        int32_t *wi_cnt, int64_t wi_cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
