@@ -1,0 +1,174 @@
+#!/usr/bin/env python3
+"""Throughput of the Poisson bmnd() pipeline on one MI355X.
+
+One step = the full two-stage bmnd() (HT pilot + Wiener) over one synthetic
+input that is already resident in HBM.  Prints ONE JSON line:
+  value        input voxels per second over the timed steps
+  roofline     the dominant kernel against its MI355X roof (HIP events)
+  cpu_baseline the CPU oracle timed on a bounded sample of the same workload
+
+Workloads (--config):
+  bm4d-256     256^3 volume of 40 random ellipsoids + 0.05 floor, peak 10
+               counts, BM4DProfile(noise_model=POISSON) defaults   [default]
+  bm3d-512x16  16 images of 512^2 (Gaussian blobs + rectangles, peaks in
+               {1,5,20,100}), BM3DProfile(noise_model=POISSON) defaults
+  bm3d-64      64^2 README square image, BM3DProfile defaults
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+METRIC = "voxels/s of bmnd() on 1 MI355X, plain vs surrogate pilot, at fixed PSNR gap"
+PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+# algorithmic operations per Poisson-deviance term: 2 logarithms + 8 arithmetic
+FLOPS_PER_TERM = 10
+
+
+def parse() -> argparse.Namespace:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawTextHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="bm4d-256", choices=["bm4d-256", "bm3d-512x16", "bm3d-64"])
+    ap.add_argument("--cpu-sample", type=int, default=64,
+                    help="side of the cube (3-D) or square (2-D) the CPU oracle is timed on")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--fast-deviance", action="store_true",
+                    help="use the one-log deviance form in block matching")
+    return ap.parse_args()
+
+
+def make_inputs(config: str):
+    from bmnd_amd import BM3DProfile, BM4DProfile, NoiseModel, phantoms
+    if config == "bm4d-256":
+        truth, noisy = phantoms.ellipsoid_volume((256, 256, 256), seed=0, peak=10.0)
+        return BM4DProfile(noise_model=NoiseModel.POISSON), [noisy], [truth], {
+            "workload": "256^3 BM4D Poisson, 40 ellipsoids + 0.05 floor, peak 10 counts",
+            "shape": [256, 256, 256], "profile": "BM4DProfile(noise_model=POISSON)", "seed": 0}
+    if config == "bm3d-512x16":
+        pairs = [phantoms.blob_image(512, seed=s) for s in range(16)]
+        return BM3DProfile(noise_model=NoiseModel.POISSON), [p[1] for p in pairs], \
+            [p[0] for p in pairs], {
+                "workload": "16 x 512^2 BM3D Poisson images, Gaussian blobs + rectangles, "
+                            "peaks {1,5,20,100}",
+                "shape": [16, 512, 512], "profile": "BM3DProfile(noise_model=POISSON)",
+                "seeds": list(range(16))}
+    truth, noisy = phantoms.square_image(0)
+    return BM3DProfile(noise_model=NoiseModel.POISSON), [noisy], [truth], {
+        "workload": "64^2 README square image, BM3D Poisson", "shape": [64, 64],
+        "profile": "BM3DProfile(noise_model=POISSON)", "seed": 0}
+
+
+def cpu_baseline(config: str, noisy: np.ndarray, side: int) -> dict:
+    import oracle
+    threads = min(os.cpu_count() or 1, 16)
+    if noisy.ndim == 3:
+        c = [(s - side) // 2 for s in noisy.shape]
+        sample = np.ascontiguousarray(noisy[c[0]:c[0] + side, c[1]:c[1] + side, c[2]:c[2] + side])
+        desc = f"{side}^3 centre crop of the benchmark volume, oracle bmnd_poisson (both stages)"
+    else:
+        c = [(s - side) // 2 for s in noisy.shape]
+        sample = np.ascontiguousarray(noisy[c[0]:c[0] + side, c[1]:c[1] + side])
+        desc = f"{side}^2 centre crop of the first image, oracle bmnd_poisson (both stages)"
+    oracle.build()
+    t0 = time.perf_counter()
+    oracle.bmnd_poisson(sample, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": float(sample.size / dt), "unit": "voxels/s", "cores": threads,
+            "kind": "port", "sample": desc, "seconds": round(dt, 3)}
+
+
+def main() -> int:
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(every run of this build uses a single MI355X).", file=sys.stderr)
+        return 2
+    import torch
+
+    from bmnd_amd.engine import KernelTimer, PoissonPipeline, blockmatch_work
+
+    profile, noisy_list, truth_list, cfg = make_inputs(args.config)
+    dev = torch.device("cuda", 0)
+    vols = [torch.from_numpy(np.ascontiguousarray(v)).to(dev) for v in noisy_list]
+    pipe = PoissonPipeline(profile, noisy_list[0].shape, dev, exact_deviance=not args.fast_deviance)
+    voxels = sum(int(v.numel()) for v in vols)
+
+    def step(timer=None):
+        outs = []
+        for v in vols:
+            ht, wi = pipe.run(v, timer)
+            outs.append(wi.est)
+        return outs
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    timer = KernelTimer()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        outs = step(timer)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = 1e3 * wall / args.steps
+
+    spans = timer.totals_ms()
+    per_kernel = {k: float(np.mean(v)) for k, v in spans.items()}
+    # algorithmic work of the block-matching launches
+    work = {s: blockmatch_work(pipe.plans[s], noisy_list[0].shape) for s in ("ht", "wiener")}
+    dominant = max(per_kernel, key=lambda k: per_kernel[k])
+    roof = None
+    if dominant.endswith(".blockmatch"):
+        st = dominant.split(".")[0]
+        flops = FLOPS_PER_TERM * work[st][1]
+        achieved = flops / (per_kernel[dominant] * 1e-3) / 1e12
+        roof = {"kernel": f"k_blockmatch ({st})", "bound": "mfma", "achieved": round(achieved, 3),
+                "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                "algorithmic": f"{FLOPS_PER_TERM} ops x {work[st][1]} deviance terms per launch",
+                "avg_ms": round(per_kernel[dominant], 3)}
+    else:
+        roof = {"kernel": dominant, "bound": "mfma", "achieved": None, "peak": PEAK_FP32_TFLOPS,
+                "unit": "TFLOP/s", "frac": None, "traffic": None,
+                "avg_ms": round(per_kernel[dominant], 3)}
+
+    psnr = None
+    if truth_list is not None:
+        from bmnd_amd.phantoms import psnr as _psnr
+        out0 = outs[0].cpu().numpy()
+        psnr = {"noisy_db": round(_psnr(noisy_list[0], truth_list[0]), 3),
+                "out_db": round(_psnr(out0, truth_list[0]), 3)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(args.config, noisy_list[0], args.cpu_sample)
+
+    value = voxels / (ms_per_step * 1e-3)
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "voxels/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64+f32", "data": "synthetic",
+        "config": dict(cfg, pilot="plain", deviance="one-log" if args.fast_deviance else "exact"),
+        "roofline": roof, "cpu_baseline": cpu,
+        "kernels_ms": {k: round(v, 3) for k, v in sorted(per_kernel.items())},
+        "blockmatch_terms": {s: work[s][1] for s in work},
+        "psnr": psnr,
+    }
+    if cpu:
+        line["gpu_over_cpu"] = round(value / cpu["value"], 1)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

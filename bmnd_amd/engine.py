@@ -37,6 +37,36 @@ SUPPORTED = {
 }
 
 
+class KernelTimer:
+    """Records HIP events around each library call on the current stream so
+    per-kernel durations can be read after a synchronize."""
+
+    def __init__(self):
+        self.spans: dict[str, list] = {}
+
+    def start(self, name: str):
+        ev = torch.cuda.Event(enable_timing=True)
+        ev.record()
+        return (name, ev)
+
+    def stop(self, token) -> None:
+        name, ev0 = token
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev1.record()
+        self.spans.setdefault(name, []).append((ev0, ev1))
+
+    def totals_ms(self) -> dict[str, list[float]]:
+        return {k: [a.elapsed_time(b) for a, b in v] for k, v in self.spans.items()}
+
+
+class _NoTimer:
+    def start(self, name):
+        return None
+
+    def stop(self, token):
+        return None
+
+
 def _ptr(t: torch.Tensor | None) -> ctypes.c_void_p:
     return ctypes.c_void_p(0 if t is None else t.data_ptr())
 
@@ -173,6 +203,19 @@ class StageOutput:
     schedule: ScheduleArrays
 
 
+def blockmatch_work(plan: StagePlan, shape: tuple[int, ...]) -> tuple[int, int]:
+    """(candidate pairs, deviance terms) of a stage: sum over references of
+    the clipped search-box size, times the patch volume for terms."""
+    ndim = len(shape)
+    counts = np.array([shape[d] - plan.block[d] + 1 for d in range(ndim)], dtype=np.int64)
+    radius = np.array(plan.radius3[3 - ndim:], dtype=np.int64)
+    ref = plan.schedule.reference_abs
+    lo = np.maximum(ref - radius, 0)
+    hi = np.minimum(ref + radius, counts - 1)
+    pairs = int(np.prod(hi - lo + 1, axis=1).sum())
+    return pairs, pairs * int(np.prod(plan.block))
+
+
 class PoissonPipeline:
     """Two-stage Poisson denoiser on one device.  Buffers are torch tensors on
     `device`; kernels run on the current torch stream."""
@@ -215,8 +258,9 @@ class PoissonPipeline:
         return out
 
     def stage(self, name: str, match_vol: torch.Tensor, noisy: torch.Tensor, var: torch.Tensor,
-              fallback: torch.Tensor) -> StageOutput:
+              fallback: torch.Tensor, timer=None) -> StageOutput:
         L = _lib.lib()
+        timer = timer or _NoTimer()
         plan = self.plans[name]
         stream = self._stream()
         shape = (ctypes.c_int64 * 3)(*self.shape3)
@@ -228,37 +272,48 @@ class PoissonPipeline:
         sel = torch.empty(R * plan.max_m, dtype=torch.int32, device=dev)
         cnt = torch.empty(R, dtype=torch.int32, device=dev)
         acc = torch.empty(R, dtype=torch.int32, device=dev)
+        tok = timer.start(f"{name}.blockmatch")
         _lib.check(L.bmnd_hip_blockmatch(_ptr(match_vol), shape, block, radius, _ptr(refs), R,
                                          plan.threshold, plan.max_m, plan.min_m, 1, 1,
                                          self.count_scale, int(self.exact), _ptr(sel), _ptr(cnt),
                                          _ptr(acc), stream))
+        timer.stop(tok)
         mats, kai = plan.device_constants(dev)
         filt = torch.empty(R * plan.max_m * plan.P, dtype=torch.float32, device=dev)
         wts = torch.empty(R * plan.max_m, dtype=torch.float32, device=dev)
         wiener = name == "wiener"
+        tok = timer.start(f"{name}.filter")
         _lib.check(L.bmnd_hip_filter_groups(int(wiener), _ptr(noisy),
                                             _ptr(match_vol if wiener else None), _ptr(var), shape,
                                             block, radius, _ptr(sel), _ptr(cnt), R, plan.max_m,
                                             _ptr(mats), plan.shrink, self.k, self.count_scale,
                                             _ptr(filt), _ptr(wts), stream))
+        timer.stop(tok)
         npatch = int(np.prod([self.shape3[d] - plan.block3[d] + 1 for d in range(3)]))
         ref_index = torch.empty(npatch, dtype=torch.int32, device=dev)
         _lib.check(L.bmnd_hip_reference_map(_ptr(refs), R, _ptr(ref_index), npatch, stream))
         est = torch.empty_like(noisy)
+        tok = timer.start(f"{name}.aggregate")
         _lib.check(L.bmnd_hip_aggregate(_ptr(filt), _ptr(wts), _ptr(sel), _ptr(cnt),
                                         _ptr(ref_index), R, plan.max_m, shape, block, radius,
                                         _ptr(kai), _ptr(fallback), _ptr(est), stream))
+        timer.stop(tok)
         del filt, wts, ref_index
         return StageOutput(est, sel.view(R, plan.max_m), cnt, acc, plan.schedule)
 
-    def run(self, vol: torch.Tensor) -> tuple[StageOutput, StageOutput]:
+    def run(self, vol: torch.Tensor, timer=None) -> tuple[StageOutput, StageOutput]:
         """vol: float32 contiguous device tensor of `shape`.  Returns the HT
         (pilot) and Wiener stage outputs."""
         if vol.dtype != torch.float32 or not vol.is_contiguous() or tuple(vol.shape) != self.shape:
             raise ValueError("vol must be a contiguous float32 tensor of the planned shape")
+        t = timer or _NoTimer()
+        tok = t.start("ht.variance")
         var1 = self.variance(vol)
-        ht = self.stage("ht", vol, vol, var1, vol)
+        t.stop(tok)
+        ht = self.stage("ht", vol, vol, var1, vol, timer)
         del var1
+        tok = t.start("wiener.variance")
         var2 = self.variance(ht.est)
-        wi = self.stage("wiener", ht.est, vol, var2, vol)
+        t.stop(tok)
+        wi = self.stage("wiener", ht.est, vol, var2, vol, timer)
         return ht, wi
