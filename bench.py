@@ -42,8 +42,10 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--cpu-sample", type=int, default=64,
                     help="side of the cube (3-D) or square (2-D) the CPU oracle is timed on")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--fast-deviance", action="store_true",
-                    help="use the one-log deviance form in block matching")
+    ap.add_argument("--deviance", default="auto",
+                    choices=["auto", "exact", "one-log", "auto-one-log"],
+                    help="block-matching deviance evaluation (auto: integer-count table, "
+                         "else the reference's two-log form)")
     return ap.parse_args()
 
 
@@ -100,7 +102,7 @@ def main() -> int:
     profile, noisy_list, truth_list, cfg = make_inputs(args.config)
     dev = torch.device("cuda", 0)
     vols = [torch.from_numpy(np.ascontiguousarray(v)).to(dev) for v in noisy_list]
-    pipe = PoissonPipeline(profile, noisy_list[0].shape, dev, exact_deviance=not args.fast_deviance)
+    pipe = PoissonPipeline(profile, noisy_list[0].shape, dev, deviance=args.deviance)
     voxels = sum(int(v.numel()) for v in vols)
 
     def step(timer=None):
@@ -158,7 +160,8 @@ def main() -> int:
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64+f32", "data": "synthetic",
-        "config": dict(cfg, pilot="plain", deviance="one-log" if args.fast_deviance else "exact"),
+        "config": dict(cfg, pilot="plain", deviance=args.deviance,
+                       blockmatch_modes=dict(pipe.last_modes)),
         "roofline": roof, "cpu_baseline": cpu,
         "kernels_ms": {k: round(v, 3) for k, v in sorted(per_kernel.items())},
         "blockmatch_terms": {s: work[s][1] for s in work},

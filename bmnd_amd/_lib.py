@@ -22,8 +22,10 @@ EXPORTS = (
     "bmnd_hip_last_error",
     "bmnd_hip_configs",
     "bmnd_hip_variance_volume",
+    "bmnd_hip_count_range",
     "bmnd_hip_blockmatch",
     "bmnd_hip_matrix_table_size",
+    "bmnd_hip_exact_std",
     "bmnd_hip_filter_groups",
     "bmnd_hip_reference_map",
     "bmnd_hip_aggregate",
@@ -42,13 +44,17 @@ def _declare(L: ctypes.CDLL) -> None:
     L.bmnd_hip_configs.restype = ctypes.c_char_p
     L.bmnd_hip_variance_volume.argtypes = [vp, vp, i64, f64, f64, vp]
     L.bmnd_hip_variance_volume.restype = i32
+    L.bmnd_hip_count_range.argtypes = [vp, i64, vp, vp]
+    L.bmnd_hip_count_range.restype = i32
     L.bmnd_hip_blockmatch.argtypes = [vp, vp, vp, vp, vp, i64, f32, i32, i32, i32, i32, f64,
-                                      i32, vp, vp, vp, vp]
+                                      i32, i32, vp, vp, vp, vp]
     L.bmnd_hip_blockmatch.restype = i32
     L.bmnd_hip_matrix_table_size.argtypes = [i32, vp]
     L.bmnd_hip_matrix_table_size.restype = i64
-    L.bmnd_hip_filter_groups.argtypes = [i32, vp, vp, vp, vp, vp, vp, vp, vp, i64, i32, vp, f32,
-                                         i32, f64, vp, vp, vp]
+    L.bmnd_hip_exact_std.argtypes = [vp, vp, vp, vp, vp, vp, i64, i32, vp, i32, vp, vp]
+    L.bmnd_hip_exact_std.restype = i32
+    L.bmnd_hip_filter_groups.argtypes = [i32, vp, vp, vp, vp, vp, vp, vp, vp, i64, i32, vp, vp,
+                                         f32, i32, f64, vp, vp, vp]
     L.bmnd_hip_filter_groups.restype = i32
     L.bmnd_hip_reference_map.argtypes = [vp, i64, vp, i64, vp]
     L.bmnd_hip_reference_map.restype = i32

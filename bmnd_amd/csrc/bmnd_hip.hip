@@ -145,6 +145,30 @@ __global__ void k_variance_volume(const float *__restrict__ v, float *__restrict
 }
 
 // ---------------------------------------------------------------------------
+// count check: are all voxels non-negative integers, and what is the largest
+// ---------------------------------------------------------------------------
+__global__ void k_count_range(const float *__restrict__ v, int64_t n, int32_t *flags) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  int ok = 1, mx = 0;
+  for (; i < n; i += stride) {
+    const float x = v[i];
+    const bool isint = x >= 0.0f && x <= 16777216.0f && x == rintf(x);
+    ok &= isint ? 1 : 0;
+    if (isint) mx = max(mx, (int)x);
+  }
+  // wave reductions, then one atomic per wave
+  for (int o = 32; o > 0; o >>= 1) {
+    ok &= __shfl_xor(ok, o, 64);
+    mx = max(mx, __shfl_xor(mx, o, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (!ok) atomicAnd(&flags[0], 0);
+    atomicMax(&flags[1], mx);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // block matching
 // ---------------------------------------------------------------------------
 struct BMArgs {
@@ -158,13 +182,20 @@ struct BMArgs {
   int max_m, min_m, pow2, stabilize;
   double inv_scale;
   const double *logtab;
+  const double *term_table;  // MODE 2: (tmax+1)^2 terms, row = reference count
+  int tmax;
   int32_t *sel, *counts, *accepted;
 };
 
 constexpr int kBMThreads = 512;
 constexpr int kMaxStack = 64;
+constexpr int kMaxTableCount = 100;
 
-template <int B0, int B1, int B2, int R0, int R1, int R2, bool EXACT>
+// MODE 0: each term as the reference writes it (two divisions, two logs);
+// MODE 1: one log per term, r*(log 2r - log s) + c*(log 2c - log s);
+// MODE 2: integer counts; the term is read from a float64 table built on the
+//         host with the same expression and the C library's log.
+template <int B0, int B1, int B2, int R0, int R1, int R2, int MODE>
 __global__ __launch_bounds__(kBMThreads) void k_blockmatch(BMArgs a) {
   constexpr int S0 = 2 * R0 + B0, S1 = 2 * R1 + B1, S2 = 2 * R2 + B2;
   constexpr int SLAB = S0 * S1 * S2;
@@ -180,6 +211,8 @@ __global__ __launch_bounds__(kBMThreads) void k_blockmatch(BMArgs a) {
   __shared__ unsigned long long s_sorted[kMaxStack];
   __shared__ unsigned long long s_prefix;
   __shared__ int s_krem, s_cnt, s_acc;
+  __shared__ int s_refrow[P];
+  extern __shared__ double s_terms[];  // MODE 2 only
 
   const int tid = threadIdx.x;
   const int64_t g = blockIdx.x;
@@ -208,10 +241,15 @@ __global__ __launch_bounds__(kBMThreads) void k_blockmatch(BMArgs a) {
     double r = (double)v * a.inv_scale;
     if (r < 0.0) r = 0.0;
     s_ref[p] = r;
+    if (MODE == 2) s_refrow[p] = (int)v * (a.tmax + 1);
+  }
+  if (MODE == 2) {
+    const int tn = (a.tmax + 1) * (a.tmax + 1);
+    for (int i = tid; i < tn; i += kBMThreads) s_terms[i] = a.term_table[i];
   }
   if (tid == 0) s_acc = 0;
   __syncthreads();
-  if (!EXACT) {
+  if (MODE == 1) {
     for (int p = tid; p < P; p += kBMThreads) {
       double r = s_ref[p];
       s_ref_log[p] = r > 0.0 ? dlog(2.0 * r, s_tab) : 0.0;
@@ -233,29 +271,35 @@ __global__ __launch_bounds__(kBMThreads) void k_blockmatch(BMArgs a) {
       const float *row = s_slab + base + (p0 * S1 + p1) * S2;
       const double *rrow = s_ref + p01 * B2;
       const double *lrow = s_ref_log + p01 * B2;
+      const int *trow = s_refrow + p01 * B2;
 #pragma unroll
-        for (int p2 = 0; p2 < B2; ++p2) {
-          double c = (double)row[p2] * a.inv_scale;
-          if (c < 0.0) c = 0.0;
-          const double r = rrow[p2];
-          const double s = r + c;
-          double t;
-          if (EXACT) {
-            // blockmatching.py:347-355, term by term
-            const double lr = dlog((2.0 * r) / s, s_tab);
-            const double lc = dlog((2.0 * c) / s, s_tab);
-            t = 0.0;
-            if (r > 0.0) t = t + r * lr;
-            t = (c > 0.0) ? t + c * lc : t;
-          } else {
-            const double ls = dlog(s, s_tab);
-            const double lc = dlog(2.0 * c, s_tab);
-            t = 0.0;
-            if (r > 0.0) t = t + r * (lrow[p2] - ls);
-            t = (c > 0.0) ? t + c * (lc - ls) : t;
-          }
-          d = (s > 0.0) ? d + 2.0 * t : d;
+      for (int p2 = 0; p2 < B2; ++p2) {
+        if (MODE == 2) {
+          // table entry = 2*(r log(2r/s) + c log(2c/s)), 0 when r = c = 0
+          d = d + s_terms[trow[p2] + (int)row[p2]];
+          continue;
         }
+        double c = (double)row[p2] * a.inv_scale;
+        if (c < 0.0) c = 0.0;
+        const double r = rrow[p2];
+        const double s = r + c;
+        double t;
+        if (MODE == 0) {
+          // blockmatching.py:347-355, term by term
+          const double lr = dlog((2.0 * r) / s, s_tab);
+          const double lc = dlog((2.0 * c) / s, s_tab);
+          t = 0.0;
+          if (r > 0.0) t = t + r * lr;
+          t = (c > 0.0) ? t + c * lc : t;
+        } else {
+          const double ls = dlog(s, s_tab);
+          const double lc = dlog(2.0 * c, s_tab);
+          t = 0.0;
+          if (r > 0.0) t = t + r * (lrow[p2] - ls);
+          t = (c > 0.0) ? t + c * (lc - ls) : t;
+        }
+        d = (s > 0.0) ? d + 2.0 * t : d;
+      }
     }
     if (d < 0.0) d = 0.0;
     const float d32 = (float)(d / (double)P);
@@ -268,6 +312,7 @@ __global__ __launch_bounds__(kBMThreads) void k_blockmatch(BMArgs a) {
       key = d32;
       ok = d32 < a.threshold;
     }
+    if (key == 0.0f) key = 0.0f;  // a signed zero sorts as +0
     my_acc += ok ? 1 : 0;
     const unsigned long long comp = ((unsigned long long)(ok ? 0u : 1u) << 63) |
                                     ((unsigned long long)__float_as_uint(key) << 32) |
@@ -386,8 +431,177 @@ struct FilterArgs {
   float shrink;
   int k;
   double count_scale;
+  const float *exact_std;  // (nrefs, kcap, P) exact-plane std from k_exact_std
+  int kcap;
   float *filt, *weights;
 };
+
+// ---------------------------------------------------------------------------
+// exact overlap-aware noise std of the first planes (variance.py:652-975)
+// ---------------------------------------------------------------------------
+// For every spatial coefficient s (one lane each) and group plane q < k:
+//   var[q,s] = sum_v var_v * (sum_{(i,p): x_i+p = v} G[q,i]*S[s,p])^2
+// with v running over the union voxels of the group in C order and the
+// entries of one voxel in increasing member order; every product and sum is
+// a separately rounded float32 operation, the reference's serial order.
+// S[s,p] = (A0[s0,p0]*A1[s1,p1])*A2[s2,p2] is the float32 Kronecker chain of
+// the spatial analysis matrices (variance.py:744-755); the first product is
+// tabulated (s_t01), so each lane does one multiply per entry.  Terms with
+// G[q,i] == 0 are skipped: they add a signed zero, which cannot change the
+// squared contribution.
+struct ExactArgs {
+  const float *var;
+  int n0, n1, n2;
+  int c0, c1, c2;
+  const int32_t *sel, *counts;
+  int64_t nrefs;
+  int max_m;
+  const float *mats;
+  int k;
+  int kcap;
+  float *exact_std;
+};
+
+template <int B0, int B1, int B2, int NMAX, int NT>
+__global__ __launch_bounds__(NT) void k_exact_std(ExactArgs a) {
+  constexpr int P = B0 * B1 * B2;
+  constexpr int B01 = B0 * B1;
+  constexpr int NP = NMAX * P;
+  constexpr int SORTN = (NP <= 512) ? 512 : (NP <= 1024) ? 1024 : (NP <= 2048) ? 2048 : 4096;
+  static_assert(NP <= 4096, "group too large");
+  static_assert(P <= 128 && P <= NT, "one lane per spatial coefficient");
+  __shared__ unsigned s_key[SORTN];
+  __shared__ float s_t01[B01 * B01];  // [p01][s01]
+  __shared__ float s_a2[B2 * B2];     // [p2][s2]
+  __shared__ float s_grow[4 * NMAX];
+  __shared__ int s_pos[NMAX][3];
+  __shared__ int s_min[3], s_ushape[3];
+
+  const int tid = threadIdx.x;
+  const int64_t g = blockIdx.x;
+  if (g >= a.nrefs) return;
+  const int n = a.counts[g];
+  const int planes = min(a.k, n);
+  if (n <= 0 || planes <= 0) return;
+
+  const int64_t gtot = (int64_t)a.max_m * (a.max_m + 1) * (2 * a.max_m + 1) / 6;
+  const float *gf = a.mats + (int64_t)(n - 1) * n * (2 * n - 1) / 6;
+  const float *A0 = a.mats + 2 * gtot;
+  const float *A1 = A0 + B0 * B0;
+  const float *A2 = A1 + B1 * B1;
+  for (int t = tid; t < B01 * B01; t += NT) {
+    const int p01 = t / B01, s01 = t - p01 * B01;
+    const int p0 = p01 / B1, p1 = p01 - p0 * B1;
+    const int s0 = s01 / B1, s1 = s01 - s0 * B1;
+    s_t01[t] = A0[s0 * B0 + p0] * A1[s1 * B1 + p1];
+  }
+  for (int t = tid; t < B2 * B2; t += NT) {
+    const int p2 = t / B2, s2 = t - p2 * B2;
+    s_a2[t] = A2[s2 * B2 + p2];
+  }
+  if (tid < n) {
+    const int lin = a.sel[g * a.max_m + tid];
+    s_pos[tid][2] = lin % a.c2;
+    s_pos[tid][1] = (lin / a.c2) % a.c1;
+    s_pos[tid][0] = lin / (a.c2 * a.c1);
+  }
+  __syncthreads();
+  if (tid < 3) {
+    int mn = s_pos[0][tid], mx = s_pos[0][tid];
+    for (int i = 1; i < n; ++i) {
+      mn = min(mn, s_pos[i][tid]);
+      mx = max(mx, s_pos[i][tid]);
+    }
+    const int bs[3] = {B0, B1, B2};
+    s_min[tid] = mn;
+    s_ushape[tid] = mx - mn + bs[tid];
+  }
+  __syncthreads();
+  const int np = n * P;
+  const int u1 = s_ushape[1], u2 = s_ushape[2];
+  // entries keyed by (union label, member, patch offset), sorted: C order over
+  // union voxels, increasing member index inside a voxel
+  for (int e = tid; e < SORTN; e += NT) {
+    unsigned key = 0xffffffffu;
+    if (e < np) {
+      const int i = e / P, p = e - i * P;
+      const int p2 = p % B2, p1 = (p / B2) % B1, p0 = p / (B2 * B1);
+      const int l0 = s_pos[i][0] - s_min[0] + p0;
+      const int l1 = s_pos[i][1] - s_min[1] + p1;
+      const int l2 = s_pos[i][2] - s_min[2] + p2;
+      const unsigned label = (unsigned)((l0 * u1 + l1) * u2 + l2);
+      key = (label << 12) | (unsigned)((i << 7) | p);
+    }
+    s_key[e] = key;
+  }
+  __syncthreads();
+  for (int size = 2; size <= SORTN; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int t = tid; t < SORTN / 2; t += NT) {
+        const int lo = 2 * t - (t & (stride - 1));
+        const int hi = lo + stride;
+        const bool up = ((lo & size) == 0);
+        const unsigned x = s_key[lo], y = s_key[hi];
+        if ((x > y) == up) {
+          s_key[lo] = y;
+          s_key[hi] = x;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  const int s = tid;
+  const bool active = s < P;
+  const int s01 = active ? s / B2 : 0, s2 = active ? s - (s / B2) * B2 : 0;
+  const int m0 = s_min[0], m1 = s_min[1], m2 = s_min[2];
+  for (int q0 = 0; q0 < planes; q0 += 4) {
+    for (int t = tid; t < 4 * n; t += NT) {
+      const int j = t / n, i = t - j * n;
+      s_grow[j * NMAX + i] = (q0 + j < planes) ? gf[(q0 + j) * n + i] : 0.0f;
+    }
+    __syncthreads();
+    float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f, acc3 = 0.0f;
+    float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f, c3 = 0.0f;
+    unsigned prev = __builtin_amdgcn_readfirstlane(s_key[0]) >> 12;
+    auto finish_voxel = [&](unsigned label) {
+      const int l2 = (int)(label % (unsigned)u2);
+      const int l1 = (int)((label / (unsigned)u2) % (unsigned)u1);
+      const int l0 = (int)(label / (unsigned)(u2 * u1));
+      const float vv = a.var[((int64_t)(l0 + m0) * a.n1 + (l1 + m1)) * a.n2 + (l2 + m2)];
+      acc0 = acc0 + (vv * c0) * c0;
+      acc1 = acc1 + (vv * c1) * c1;
+      acc2 = acc2 + (vv * c2) * c2;
+      acc3 = acc3 + (vv * c3) * c3;
+      c0 = c1 = c2 = c3 = 0.0f;
+    };
+    for (int pos = 0; pos < np; ++pos) {
+      const unsigned key = __builtin_amdgcn_readfirstlane(s_key[pos]);
+      const unsigned label = key >> 12;
+      if (label != prev) {
+        finish_voxel(prev);
+        prev = label;
+      }
+      const int i = (int)((key >> 7) & 31u), p = (int)(key & 127u);
+      const int p01 = p / B2, p2 = p - p01 * B2;
+      const float sv = s_t01[p01 * B01 + s01] * s_a2[p2 * B2 + s2];
+      const float g0 = s_grow[i], g1 = s_grow[NMAX + i], g2 = s_grow[2 * NMAX + i],
+                  g3 = s_grow[3 * NMAX + i];
+      if (g0 != 0.0f) c0 = c0 + g0 * sv;
+      if (g1 != 0.0f) c1 = c1 + g1 * sv;
+      if (g2 != 0.0f) c2 = c2 + g2 * sv;
+      if (g3 != 0.0f) c3 = c3 + g3 * sv;
+    }
+    finish_voxel(prev);
+    if (active) {
+      float *dst = a.exact_std + (g * a.kcap + q0) * P + s;
+      const float accs[4] = {acc0, acc1, acc2, acc3};
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (q0 + j < planes) dst[j * P] = sqrtf(accs[j] > 0.0f ? accs[j] : 0.0f);
+    }
+    __syncthreads();
+  }
+}
 
 // apply M (n x n, out-major) along one axis of an LDS array of dims (d0,d1,d2,d3)
 // as fmaf chains in input order; src -> dst
@@ -416,21 +630,14 @@ template <int B0, int B1, int B2, int R0, int R1, int R2, int NMAX, int NT, bool
 __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
   constexpr int P = B0 * B1 * B2;
   constexpr int NP = NMAX * P;
-  constexpr int SORTN = (NP <= 512) ? 512 : (NP <= 1024) ? 1024 : (NP <= 2048) ? 2048 : 4096;
   static_assert(NP <= 4096, "group too large");
-  static_assert(P <= 128, "patch too large for the entry encoding");
   __shared__ float s_x[NP];      // coefficients being filtered
   __shared__ float s_t[NP];      // transform temporary
   __shared__ float s_sig[NP];    // noise std
   __shared__ float s_r[WIENER ? NP : 1];
-  __shared__ unsigned s_key[SORTN];
-  __shared__ unsigned short s_seg[NP + 1];
-  __shared__ float s_segvar[NP];
   __shared__ float s_g[NMAX * NMAX], s_gi[NMAX * NMAX], s_g2[NMAX * NMAX];
   __shared__ float s_sf[3][64], s_si[3][64], s_sf2[3][64];
   __shared__ int s_pos[NMAX][3];
-  __shared__ int s_min[3], s_ushape[3];
-  __shared__ int s_scan[NT / 64 + 1];
   __shared__ double s_plane[NMAX];
 
   const int tid = threadIdx.x;
@@ -476,17 +683,6 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
     s_pos[tid][0] = lin / (a.c2 * a.c1);
   }
   __syncthreads();
-  if (tid < 3) {
-    int mn = s_pos[0][tid], mx = s_pos[0][tid];
-    for (int i = 1; i < n; ++i) {
-      mn = min(mn, s_pos[i][tid]);
-      mx = max(mx, s_pos[i][tid]);
-    }
-    s_min[tid] = mn;
-    const int bs[3] = {B0, B1, B2};
-    s_ushape[tid] = mx - mn + bs[tid];
-  }
-  __syncthreads();
   const int np = n * P;
   // ---- gather: variance patches (approximate planes) ------------------------
   for (int e = tid; e < np; e += NT) {
@@ -520,86 +716,9 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
   }
   __syncthreads();
 
-  // ---- exact overlap-aware planes (variance.py:652-975) --------------------
-  // entries sorted by (union label, entry index): a stable order in which the
-  // entries of one union voxel follow increasing member index
-  const int u0 = s_ushape[0], u1 = s_ushape[1], u2 = s_ushape[2];
-  for (int e = tid; e < SORTN; e += NT) {
-    unsigned key = 0xffffffffu;
-    if (e < np) {
-      const int i = e / P, p = e - i * P;
-      const int p2 = p % B2, p1 = (p / B2) % B1, p0 = p / (B2 * B1);
-      const int l0 = s_pos[i][0] - s_min[0] + p0;
-      const int l1 = s_pos[i][1] - s_min[1] + p1;
-      const int l2 = s_pos[i][2] - s_min[2] + p2;
-      const unsigned label = (unsigned)((l0 * u1 + l1) * u2 + l2);
-      key = (label << 12) | (unsigned)((i << 7) | p);
-    }
-    s_key[e] = key;
-  }
-  __syncthreads();
-  for (int size = 2; size <= SORTN; size <<= 1) {
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int t = tid; t < SORTN / 2; t += NT) {
-        const int lo = 2 * t - (t & (stride - 1));
-        const int hi = lo + stride;
-        const bool up = ((lo & size) == 0);
-        unsigned x = s_key[lo], y = s_key[hi];
-        if ((x > y) == up) {
-          s_key[lo] = y;
-          s_key[hi] = x;
-        }
-      }
-      __syncthreads();
-    }
-  }
-  // segment starts
-  int nseg = 0;
-  {
-    int base = 0;
-    for (int chunk = 0; chunk < np; chunk += NT) {
-      const int e = chunk + tid;
-      int flag = 0;
-      if (e < np) flag = (e == 0) || ((s_key[e] >> 12) != (s_key[e - 1] >> 12));
-      int tot;
-      int ex = block_exclusive_scan<NT>(flag, s_scan, &tot);
-      if (flag) {
-        const int sidx = base + ex;
-        s_seg[sidx] = (unsigned short)e;
-        const unsigned label = s_key[e] >> 12;
-        const int l2 = label % u2, l1 = (label / u2) % u1, l0 = label / (u2 * u1);
-        const int64_t v = ((int64_t)(l0 + s_min[0]) * a.n1 + (l1 + s_min[1])) * a.n2 +
-                          (l2 + s_min[2]);
-        s_segvar[sidx] = a.var[v];
-      }
-      base += tot;
-    }
-    nseg = base;
-    if (tid == 0) s_seg[nseg] = (unsigned short)np;
-  }
-  __syncthreads();
-  // exact variance of planes q < planes; serial float32 reduction per coefficient
+  // ---- exact overlap-aware planes, computed by k_exact_std ------------------
   float *exact = s_t;  // planes*P values
-  for (int c = tid; c < planes * P; c += NT) {
-    const int q = c / P, s = c - q * P;
-    const int s2 = s % B2, s1 = (s / B2) % B1, s0 = s / (B2 * B1);
-    const float *grow = s_g + q * n;
-    float acc = 0.0f;
-    for (int u = 0; u < nseg; ++u) {
-      const int beg = s_seg[u], end = s_seg[u + 1];
-      float contrib = 0.0f;
-      for (int pos = beg; pos < end; ++pos) {
-        const unsigned ent = s_key[pos] & 0xfffu;
-        const int i = (int)(ent >> 7), p = (int)(ent & 127u);
-        const int p2 = p % B2, p1 = (p / B2) % B1, p0 = p / (B2 * B1);
-        const float sv = (s_sf[0][s0 * B0 + p0] * s_sf[1][s1 * B1 + p1]) * s_sf[2][s2 * B2 + p2];
-        contrib = contrib + grow[i] * sv;
-      }
-      const float vv = s_segvar[u];
-      acc = acc + (vv * contrib) * contrib;
-    }
-    exact[c] = sqrtf(acc > 0.0f ? acc : 0.0f);
-  }
+  for (int c = tid; c < planes * P; c += NT) exact[c] = a.exact_std[(g * a.kcap) * P + c];
   __syncthreads();
   // plane limiting (variance.py:1572-1631), float32 sequential plane sums
   if (planes == 0) {
@@ -955,6 +1074,47 @@ int logtab_for_device(const double **out) {
   return 0;
 }
 
+struct TermTable {
+  int device, tmax;
+  double *ptr;
+};
+std::vector<TermTable> g_terms;
+
+// 2*(r log(2r/(r+c)) + c log(2c/(r+c))) for integer counts r, c in [0, tmax],
+// the reference's expression (blockmatching.py:347-355) with the C library's
+// log; 0 where r = c = 0 (no term is added there).
+int term_table_for_device(int tmax, const double **out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_tab_mutex);
+  for (const auto &t : g_terms)
+    if (t.device == dev && t.tmax == tmax) {
+      *out = t.ptr;
+      return 0;
+    }
+  const int w = tmax + 1;
+  std::vector<double> h((size_t)w * w);
+  for (int ri = 0; ri < w; ++ri)
+    for (int ci = 0; ci < w; ++ci) {
+      const double r = ri, c = ci;
+      const double comb = r + c;
+      double v = 0.0;
+      if (comb > 0.0) {
+        double t = 0.0;
+        if (r > 0.0) t += r * std::log(2.0 * r / comb);
+        if (c > 0.0) t += c * std::log(2.0 * c / comb);
+        v = 2.0 * t;
+      }
+      h[(size_t)ri * w + ci] = v;
+    }
+  double *d = nullptr;
+  HIP_TRY(hipMalloc(&d, sizeof(double) * h.size()));
+  HIP_TRY(hipMemcpy(d, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+  g_terms.push_back({dev, tmax, d});
+  *out = d;
+  return 0;
+}
+
 float host_canonical_key(float d) {
   double mag = std::fabs((double)d);
   if (mag < 1.0) mag = 1.0;
@@ -1004,12 +1164,23 @@ int32_t bmnd_hip_variance_volume(const float *vol, float *out, int64_t n, double
   return 0;
 }
 
+int32_t bmnd_hip_count_range(const float *vol, int64_t n, int32_t *flags, void *stream) {
+  hipStream_t s = (hipStream_t)stream;
+  const int32_t init[2] = {1, 0};
+  HIP_TRY(hipMemcpyAsync(flags, init, sizeof(init), hipMemcpyHostToDevice, s));
+  if (n <= 0) return 0;
+  int blocks = (int)std::min<int64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_count_range, dim3(blocks), dim3(256), 0, s, vol, n, flags);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
 int32_t bmnd_hip_blockmatch(const float *vol, const int64_t *shape_h, const int32_t *block_h,
                             const int32_t *radius_h, const int32_t *ref_lin, int64_t nrefs,
                             float threshold, int32_t max_matches, int32_t min_matches,
                             int32_t pow2, int32_t stabilize, double count_scale,
-                            int32_t exact, int32_t *sel, int32_t *counts, int32_t *accepted,
-                            void *stream) {
+                            int32_t mode, int32_t table_max, int32_t *sel, int32_t *counts,
+                            int32_t *accepted, void *stream) {
   int c[3];
   if (int rc = check_geometry(shape_h, block_h, radius_h, c)) return rc;
   if (max_matches < 1 || max_matches > kMaxStack || min_matches < 1)
@@ -1018,8 +1189,14 @@ int32_t bmnd_hip_blockmatch(const float *vol, const int64_t *shape_h, const int3
     return fail(BMND_HIP_EINVAL, "count_scale must be finite and positive");
   if (nrefs <= 0) return 0;
   if (nrefs > INT32_MAX) return fail(BMND_HIP_EINVAL, "too many references");
+  if (mode < 0 || mode > 2) return fail(BMND_HIP_EINVAL, "mode must be 0, 1 or 2");
+  if (mode == 2 && (count_scale != 1.0 || table_max < 0 || table_max > kMaxTableCount))
+    return fail(BMND_HIP_EINVAL, "table mode needs count_scale == 1 and 0 <= table_max <= 100");
   const double *tab = nullptr;
   if (int rc = logtab_for_device(&tab)) return rc;
+  const double *terms = nullptr;
+  if (mode == 2)
+    if (int rc = term_table_for_device(table_max, &terms)) return rc;
   BMArgs a;
   a.vol = vol;
   a.n0 = (int)shape_h[0];
@@ -1038,19 +1215,24 @@ int32_t bmnd_hip_blockmatch(const float *vol, const int64_t *shape_h, const int3
   a.stabilize = stabilize;
   a.inv_scale = 1.0 / count_scale;
   a.logtab = tab;
+  a.term_table = terms;
+  a.tmax = table_max;
   a.sel = sel;
   a.counts = counts;
   a.accepted = accepted;
   hipStream_t s = (hipStream_t)stream;
   dim3 grid((unsigned)nrefs), blk(kBMThreads);
-#define BM_CASE(b0, b1, b2, r0, r1, r2)                                                    \
-  if (same3(block_h, b0, b1, b2) && same3(radius_h, r0, r1, r2)) {                           \
-    if (exact)                                                                               \
-      hipLaunchKernelGGL((k_blockmatch<b0, b1, b2, r0, r1, r2, true>), grid, blk, 0, s, a);  \
-    else                                                                                     \
-      hipLaunchKernelGGL((k_blockmatch<b0, b1, b2, r0, r1, r2, false>), grid, blk, 0, s, a); \
-    HIP_TRY(hipGetLastError());                                                              \
-    return 0;                                                                                \
+  const size_t dyn = mode == 2 ? sizeof(double) * (size_t)(table_max + 1) * (table_max + 1) : 0;
+#define BM_CASE(b0, b1, b2, r0, r1, r2)                                                        \
+  if (same3(block_h, b0, b1, b2) && same3(radius_h, r0, r1, r2)) {                               \
+    if (mode == 0)                                                                               \
+      hipLaunchKernelGGL((k_blockmatch<b0, b1, b2, r0, r1, r2, 0>), grid, blk, 0, s, a);         \
+    else if (mode == 1)                                                                          \
+      hipLaunchKernelGGL((k_blockmatch<b0, b1, b2, r0, r1, r2, 1>), grid, blk, 0, s, a);         \
+    else                                                                                         \
+      hipLaunchKernelGGL((k_blockmatch<b0, b1, b2, r0, r1, r2, 2>), grid, blk, dyn, s, a);       \
+    HIP_TRY(hipGetLastError());                                                                  \
+    return 0;                                                                                    \
   }
   BM_CASE(1, 8, 8, 0, 19, 19)
   BM_CASE(4, 4, 4, 7, 7, 7)
@@ -1066,17 +1248,58 @@ int64_t bmnd_hip_matrix_table_size(int32_t max_matches, const int32_t *block_h) 
   return n;
 }
 
+int32_t bmnd_hip_exact_std(const float *var, const int64_t *shape_h, const int32_t *block_h,
+                           const int32_t *radius_h, const int32_t *sel, const int32_t *counts,
+                           int64_t nrefs, int32_t max_matches, const float *matrices, int32_t k,
+                           float *exact_std, void *stream) {
+  int c[3];
+  if (int rc = check_geometry(shape_h, block_h, radius_h, c)) return rc;
+  if (k < 0) return fail(BMND_HIP_EINVAL, "k must be non-negative");
+  if (nrefs <= 0 || k == 0) return 0;
+  ExactArgs a;
+  a.var = var;
+  a.n0 = (int)shape_h[0];
+  a.n1 = (int)shape_h[1];
+  a.n2 = (int)shape_h[2];
+  a.c0 = c[0];
+  a.c1 = c[1];
+  a.c2 = c[2];
+  a.sel = sel;
+  a.counts = counts;
+  a.nrefs = nrefs;
+  a.max_m = max_matches;
+  a.mats = matrices;
+  a.k = k;
+  a.kcap = std::min(k, max_matches);
+  a.exact_std = exact_std;
+  hipStream_t s = (hipStream_t)stream;
+  dim3 grid((unsigned)nrefs);
+#define E_CASE(b0, b1, b2, r0, r1, r2, nmax, nt)                                             \
+  if (same3(block_h, b0, b1, b2) && same3(radius_h, r0, r1, r2) && max_matches <= nmax) {    \
+    hipLaunchKernelGGL((k_exact_std<b0, b1, b2, nmax, nt>), grid, dim3(nt), 0, s, a);       \
+    HIP_TRY(hipGetLastError());                                                             \
+    return 0;                                                                               \
+  }
+  E_CASE(1, 8, 8, 0, 19, 19, 32, 64)
+  E_CASE(4, 4, 4, 7, 7, 7, 16, 64)
+  E_CASE(5, 5, 5, 7, 7, 7, 32, 128)
+#undef E_CASE
+  return fail(BMND_HIP_EINVAL, "block/radius/stack configuration not compiled into this library");
+}
+
 int32_t bmnd_hip_filter_groups(int32_t wiener, const float *noisy, const float *pilot,
                                const float *var, const int64_t *shape_h, const int32_t *block_h,
                                const int32_t *radius_h, const int32_t *sel,
                                const int32_t *counts, int64_t nrefs, int32_t max_matches,
-                               const float *matrices, float shrink, int32_t k,
-                               double count_scale, float *filt, float *weights, void *stream) {
+                               const float *matrices, const float *exact_std, float shrink,
+                               int32_t k, double count_scale, float *filt, float *weights,
+                               void *stream) {
   int c[3];
   if (int rc = check_geometry(shape_h, block_h, radius_h, c)) return rc;
   if (k < 0) return fail(BMND_HIP_EINVAL, "k must be non-negative");
   if (nrefs <= 0) return 0;
   if (wiener && !pilot) return fail(BMND_HIP_EINVAL, "Wiener filtering needs the pilot volume");
+  if (k > 0 && !exact_std) return fail(BMND_HIP_EINVAL, "k > 0 needs the exact-plane std buffer");
   FilterArgs a;
   a.noisy = noisy;
   a.pilot = pilot;
@@ -1095,6 +1318,8 @@ int32_t bmnd_hip_filter_groups(int32_t wiener, const float *noisy, const float *
   a.shrink = shrink;
   a.k = k;
   a.count_scale = count_scale;
+  a.exact_std = exact_std;
+  a.kcap = std::min(k, max_matches);
   a.filt = filt;
   a.weights = weights;
   hipStream_t s = (hipStream_t)stream;

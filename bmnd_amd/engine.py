@@ -220,14 +220,22 @@ class PoissonPipeline:
     """Two-stage Poisson denoiser on one device.  Buffers are torch tensors on
     `device`; kernels run on the current torch stream."""
 
+    DEVIANCE_MODES = ("auto", "exact", "one-log", "auto-one-log")
+
     def __init__(self, profile: BMNDProfile, shape: tuple[int, ...], device=None,
-                 exact_deviance: bool = True):
+                 deviance: str = "auto"):
         self.profile = profile
         self.shape = tuple(int(s) for s in shape)
         self.ndim = len(self.shape)
         self.shape3 = _host3(self.shape, self.ndim, 1)
         self.device = torch.device(device if device is not None else "cuda")
-        self.exact = bool(exact_deviance)
+        if deviance not in self.DEVIANCE_MODES:
+            raise ValueError(f"deviance must be one of {self.DEVIANCE_MODES}, got {deviance!r}")
+        # "auto": integer counts <= 100 use the float64 term table (bit-identical
+        # to the reference expression), anything else the two-log form; the
+        # "one-log" variants trade last-bit agreement of the distance for speed.
+        self.deviance = deviance
+        self.last_modes: dict[str, int] = {}
         self.count_scale = float(profile.poisson_count_scale)
         self.floor = float(profile.poisson_variance_floor)
         self.k = int(profile.k)
@@ -251,6 +259,23 @@ class PoissonPipeline:
             self._refs[plan.name] = torch.from_numpy(lin.astype(np.int32)).to(self.device)
         return self._refs[plan.name]
 
+    def blockmatch_mode(self, vol: torch.Tensor) -> tuple[int, int]:
+        """(kernel mode, table max) for matching on `vol`."""
+        if self.deviance == "exact":
+            return 0, 0
+        if self.deviance == "one-log":
+            return 1, 0
+        fallback = 0 if self.deviance == "auto" else 1
+        if self.count_scale != 1.0:
+            return fallback, 0
+        flags = torch.empty(2, dtype=torch.int32, device=self.device)
+        _lib.check(_lib.lib().bmnd_hip_count_range(_ptr(vol), vol.numel(), _ptr(flags),
+                                                   self._stream()))
+        is_int, vmax = (int(x) for x in flags.cpu().tolist())
+        if is_int and vmax <= 100:
+            return 2, vmax
+        return fallback, 0
+
     def variance(self, vol: torch.Tensor) -> torch.Tensor:
         out = torch.empty_like(vol)
         _lib.check(_lib.lib().bmnd_hip_variance_volume(_ptr(vol), _ptr(out), vol.numel(),
@@ -272,13 +297,23 @@ class PoissonPipeline:
         sel = torch.empty(R * plan.max_m, dtype=torch.int32, device=dev)
         cnt = torch.empty(R, dtype=torch.int32, device=dev)
         acc = torch.empty(R, dtype=torch.int32, device=dev)
+        mode, tmax = self.blockmatch_mode(match_vol)
+        self.last_modes[name] = mode
         tok = timer.start(f"{name}.blockmatch")
         _lib.check(L.bmnd_hip_blockmatch(_ptr(match_vol), shape, block, radius, _ptr(refs), R,
                                          plan.threshold, plan.max_m, plan.min_m, 1, 1,
-                                         self.count_scale, int(self.exact), _ptr(sel), _ptr(cnt),
+                                         self.count_scale, mode, tmax, _ptr(sel), _ptr(cnt),
                                          _ptr(acc), stream))
         timer.stop(tok)
         mats, kai = plan.device_constants(dev)
+        kcap = min(self.k, plan.max_m)
+        exact = None
+        if kcap > 0:
+            exact = torch.empty(R * kcap * plan.P, dtype=torch.float32, device=dev)
+            tok = timer.start(f"{name}.exact_std")
+            _lib.check(L.bmnd_hip_exact_std(_ptr(var), shape, block, radius, _ptr(sel), _ptr(cnt),
+                                            R, plan.max_m, _ptr(mats), self.k, _ptr(exact), stream))
+            timer.stop(tok)
         filt = torch.empty(R * plan.max_m * plan.P, dtype=torch.float32, device=dev)
         wts = torch.empty(R * plan.max_m, dtype=torch.float32, device=dev)
         wiener = name == "wiener"
@@ -286,9 +321,10 @@ class PoissonPipeline:
         _lib.check(L.bmnd_hip_filter_groups(int(wiener), _ptr(noisy),
                                             _ptr(match_vol if wiener else None), _ptr(var), shape,
                                             block, radius, _ptr(sel), _ptr(cnt), R, plan.max_m,
-                                            _ptr(mats), plan.shrink, self.k, self.count_scale,
-                                            _ptr(filt), _ptr(wts), stream))
+                                            _ptr(mats), _ptr(exact), plan.shrink, self.k,
+                                            self.count_scale, _ptr(filt), _ptr(wts), stream))
         timer.stop(tok)
+        del exact
         npatch = int(np.prod([self.shape3[d] - plan.block3[d] + 1 for d in range(3)]))
         ref_index = torch.empty(npatch, dtype=torch.int32, device=dev)
         _lib.check(L.bmnd_hip_reference_map(_ptr(refs), R, _ptr(ref_index), npatch, stream))

@@ -41,6 +41,13 @@ const char *bmnd_hip_configs(void);
 int32_t bmnd_hip_variance_volume(const float *vol, float *out, int64_t n,
                                  double scale, double floor_, void *stream);
 
+/* Count check used to pick the block-matching mode: flags[0] = 1 when every
+ * voxel is a non-negative integer <= 2^24, flags[1] = the largest such value
+ * (flags is a device int32[2]).  No reference counterpart; it guards the
+ * integer-count table of bmnd_hip_blockmatch mode 2. */
+int32_t bmnd_hip_count_range(const float *vol, int64_t n, int32_t *flags,
+                             void *stream);
+
 /* Fused block matching for every reference of a stage.
  * Replaces blockmatching.blockmatch_groups -> _fused_blockmatch_groups_batch_numba
  * (blockmatching.py:1393-1511, 1045-1205) for the Poisson deviance distance
@@ -50,17 +57,19 @@ int32_t bmnd_hip_variance_volume(const float *vol, float *out, int64_t n,
  *   sel[nrefs*max_matches]    out: selected lins, reference first, -1 padded
  *   counts[nrefs]             out: final group size
  *   accepted[nrefs]           out: number of candidates under the threshold
- * exact != 0 evaluates each deviance term as the reference writes it
- * (two divisions, two logarithms in float64); exact == 0 uses the one-log
- * form r*(log 2r - log s) + c*(log 2c - log s). */
+ * mode 0 evaluates each deviance term as the reference writes it (two
+ * divisions, two logarithms in float64); mode 1 uses the one-log form
+ * r*(log 2r - log s) + c*(log 2c - log s); mode 2 requires integer counts in
+ * [0, table_max] (table_max <= 100, count_scale == 1) and reads each term
+ * from a float64 table built on the host with the reference's expression. */
 int32_t bmnd_hip_blockmatch(const float *vol, const int64_t *shape_h,
                             const int32_t *block_h, const int32_t *radius_h,
                             const int32_t *ref_lin, int64_t nrefs,
                             float threshold, int32_t max_matches,
                             int32_t min_matches, int32_t pow2,
                             int32_t stabilize, double count_scale,
-                            int32_t exact, int32_t *sel, int32_t *counts,
-                            int32_t *accepted, void *stream);
+                            int32_t mode, int32_t table_max, int32_t *sel,
+                            int32_t *counts, int32_t *accepted, void *stream);
 
 /* Matrix table consumed by bmnd_hip_filter_groups, float32, packed as
  *   G_n (n x n, row-major, out index first) for n = 1..max_matches,
@@ -70,10 +79,26 @@ int32_t bmnd_hip_blockmatch(const float *vol, const int64_t *shape_h,
  * bmnd_hip_matrix_table_size() gives its length in floats. */
 int64_t bmnd_hip_matrix_table_size(int32_t max_matches, const int32_t *block_h);
 
+/* Exact overlap-aware noise std of the first min(k, N) group planes of every
+ * group: var[q,s] = sum over union voxels v (C order) of
+ * var_v * (sum over entries (i,p) with x_i + p = v of G[q,i]*S[s,p])^2,
+ * every product and sum a float32 operation in that serial order; out is
+ * sqrt(max(var, 0)).  exact_std holds (nrefs, min(k, max_matches), P).
+ * Replaces variance._build_poisson_overlap_structure_numba and
+ * _accumulate_exact_poisson_variance_from_entry_transform_numba
+ * (variance.py:652-851, 920-975) as called by get_direct_poisson_noise_std
+ * (variance.py:1634-1697). */
+int32_t bmnd_hip_exact_std(const float *var, const int64_t *shape_h,
+                           const int32_t *block_h, const int32_t *radius_h,
+                           const int32_t *sel, const int32_t *counts,
+                           int64_t nrefs, int32_t max_matches,
+                           const float *matrices, int32_t k, float *exact_std,
+                           void *stream);
+
 /* Collaborative filtering of every group of a stage: gather, separable
- * forward transform, coefficient noise std (exact overlap-aware planes
- * q < k plus rescaled approximate planes), shrinkage, inverse transform and
- * per-patch aggregation weights.
+ * forward transform, coefficient noise std (the exact planes q < k from
+ * bmnd_hip_exact_std plus the rescaled approximate planes), shrinkage,
+ * inverse transform and per-patch aggregation weights.
  * Replaces bmndalgo.filter_ht_group (bmndalgo.py:335-559) when wiener == 0
  * and bmndalgo.filter_wiener_group (bmndalgo.py:760-981) when wiener != 0,
  * with variance.get_direct_poisson_noise_std (variance.py:1634-1711),
@@ -92,8 +117,9 @@ int32_t bmnd_hip_filter_groups(int32_t wiener, const float *noisy,
                                const int32_t *radius_h, const int32_t *sel,
                                const int32_t *counts, int64_t nrefs,
                                int32_t max_matches, const float *matrices,
-                               float shrink, int32_t k, double count_scale,
-                               float *filt, float *weights, void *stream);
+                               const float *exact_std, float shrink, int32_t k,
+                               double count_scale, float *filt, float *weights,
+                               void *stream);
 
 /* Reference-position map used by the aggregation: ref_index[lin] = g for
  * every scheduled reference g, -1 elsewhere (num_patches entries). */
