@@ -57,6 +57,8 @@ class OracleParams(ctypes.Structure):
         ("count_scale", ctypes.c_double),
         ("variance_floor", ctypes.c_double),
         ("nthreads", ctypes.c_int),
+        ("ht_spatial", ctypes.c_void_p),
+        ("wi_spatial", ctypes.c_void_p),
     ]
 
 
@@ -98,6 +100,9 @@ def _declare(L: ctypes.CDLL) -> None:
     L.oracle_noise_std.restype = i32
     L.oracle_limit_planes.argtypes = [vp, vp, i32, i32, i32, vp]
     L.oracle_patch_weights.argtypes = [vp, vp, i32, i32, vp, f64, vp]
+    L.oracle_ht_shrink.argtypes = [vp, vp, i64, f32, f64, vp]
+    L.oracle_wiener_shrink.argtypes = [vp, vp, vp, i64, f32, vp]
+    L.oracle_scatter_add.argtypes = [vp, vp, i32, vp, vp, vp, vp, vp, vp, i32]
     L.oracle_bmnd_poisson.argtypes = [vp, ctypes.POINTER(OracleParams), vp, vp,
                                       vp, vp, i64, vp, vp, i64]
     L.oracle_bmnd_poisson.restype = i32
@@ -275,6 +280,63 @@ def patch_weights(atten, sigma, group_inv, eps=1e-10) -> np.ndarray:
     return out
 
 
+def ht_shrink(coef, sigma, lam=3.0, count_scale=1.0):
+    c = np.ascontiguousarray(coef, dtype=np.float32).copy()
+    s = np.ascontiguousarray(sigma, dtype=np.float32)
+    att = np.empty_like(c)
+    lib().oracle_ht_shrink(_p(c), _p(s), c.size, lam, count_scale, _p(att))
+    return c, att
+
+
+def wiener_shrink(coef, ref_coef, sigma, mu=0.4):
+    c = np.ascontiguousarray(coef, dtype=np.float32).copy()
+    r = np.ascontiguousarray(ref_coef, dtype=np.float32)
+    s = np.ascontiguousarray(sigma, dtype=np.float32)
+    att = np.empty_like(c)
+    lib().oracle_wiener_shrink(_p(c), _p(r), _p(s), c.size, mu, _p(att))
+    return c, att
+
+
+def scatter_add(num, den, block, kaiser, filt, weights, lins):
+    """In-place sequential scatter of one group into num/den."""
+    shp = np.asarray(num.shape, dtype=np.int64)
+    blk = np.asarray(block, dtype=np.int32)
+    k = np.ascontiguousarray(kaiser, dtype=np.float32)
+    f = np.ascontiguousarray(filt, dtype=np.float32)
+    w = np.ascontiguousarray(weights, dtype=np.float32)
+    ln = np.ascontiguousarray(lins, dtype=np.int64)
+    assert num.dtype == np.float32 and num.flags.c_contiguous and den.flags.c_contiguous
+    lib().oracle_scatter_add(_p(num), _p(den), num.ndim, _p(shp), _p(blk), _p(k), _p(f), _p(w),
+                             _p(ln), ln.size)
+
+
+# the official bior1.5 analysis tables of transforms.py:711-806 (data)
+_B8 = (0.343550200747110, -0.225454819240296, -0.461645582253923,
+       0.569359398342840, 0.402347308162280, -0.083506045090280, 0.707106781186550)
+
+
+def reference_spatial_matrices(kind: str, block) -> np.ndarray:
+    """Analysis then synthesis matrices of each axis, flattened, built the way
+    the reference builds them with numpy: the DCT-II from float64 cosines and
+    the bior1.5 synthesis as numpy.linalg.inv of the float32 table
+    (transforms.py:359-386, 699-811)."""
+    fwd, inv = [], []
+    for b in block:
+        if kind == "dct":
+            m = np.zeros((b, b), np.float32)
+            for k in range(b):
+                norm = np.sqrt(1 / b) if k == 0 else np.sqrt(2 / b)
+                for n in range(b):
+                    m[k, n] = norm * np.cos(np.pi * (n + 0.5) * k / b)
+            fwd.append(m)
+            inv.append(np.ascontiguousarray(m.T))
+        else:
+            f, _ = bior15(b)  # the C table (checked bitwise against the literal)
+            fwd.append(f)
+            inv.append(np.asarray(np.linalg.inv(f), dtype=np.float32))
+    return np.concatenate([m.reshape(-1) for m in fwd + inv]).astype(np.float32)
+
+
 def default_params(ndim: int, shape, nthreads: int = 0) -> OracleParams:
     """Default BM3DProfile / BM4DProfile values with noise_model=POISSON
     (profiles.py:225-352)."""
@@ -300,6 +362,9 @@ def default_params(ndim: int, shape, nthreads: int = 0) -> OracleParams:
     p.count_scale = 1.0
     p.variance_floor = 1e-10
     p.nthreads = nthreads
+    p._keep = (reference_spatial_matrices("bior", hb), reference_spatial_matrices("dct", wb))
+    p.ht_spatial = p._keep[0].ctypes.data
+    p.wi_spatial = p._keep[1].ctypes.data
     return p
 
 

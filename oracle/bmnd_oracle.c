@@ -873,6 +873,76 @@ void oracle_patch_weights(const float *atten, const float *sigma, int n,
 }
 
 /* ======================================================================== */
+/* shrinkage and scatter components                                          */
+/* ======================================================================== */
+
+/* Poisson hard threshold (poisson.py:30-46 with bmndalgo.py:460-485): keep a
+ * coefficient iff |C|/cs > thr/cs + 128*eps32*max(|C|/cs, thr/cs, 1), compared
+ * in float64, with thr = float32(lambda * sigma). atten gets 1.0f / 0.0f and
+ * coef is multiplied by it. */
+void oracle_ht_shrink(float *coef, const float *sigma, int64_t n, float lambda_,
+                      double count_scale, float *atten) {
+  for (int64_t e = 0; e < n; ++e) {
+    float t = lambda_ * sigma[e];
+    double cc = (double)fabsf(coef[e]) / count_scale;
+    double tc = (double)t / count_scale;
+    double sc = cc > tc ? cc : tc;
+    if (sc < 1.0) sc = 1.0;
+    double tol = (128.0 * F32_EPS) * sc;
+    float keep = (cc > tc + tol) ? 1.0f : 0.0f;
+    atten[e] = keep;
+    coef[e] = coef[e] * keep;
+  }
+}
+
+/* variance-scaled Poisson Wiener gain (wiener.py:24-53, 78-95):
+ * S = max(R^2 - mu*sigma^2, 0), g = clip(S / (S + mu*sigma^2 + 1e-10), 0, 1),
+ * all float32; coef (the noisy coefficients) is multiplied by g. */
+void oracle_wiener_shrink(float *coef, const float *ref_coef, const float *sigma,
+                          int64_t n, float mu, float *atten) {
+  const float eps32 = (float)1e-10;
+  for (int64_t e = 0; e < n; ++e) {
+    float nv = sigma[e] * sigma[e];
+    float eff = mu * nv;
+    float pr = fabsf(ref_coef[e]) * fabsf(ref_coef[e]);
+    float spow = pr - eff;
+    if (spow < 0.0f) spow = 0.0f;
+    float gain = spow / ((spow + eff) + eps32);
+    if (gain < 0.0f) gain = 0.0f;
+    if (gain > 1.0f) gain = 1.0f;
+    atten[e] = gain;
+    coef[e] = coef[e] * gain;
+  }
+}
+
+/* sequential scatter of one group (patchaggregation.py:170-199):
+ * num[x_i + p] += filt[i,p] * (w_i * k[p]); den[x_i + p] += w_i * k[p]. */
+void oracle_scatter_add(float *num, float *den, int ndim, const int64_t *vol_shape,
+                        const int *block, const float *kaiser, const float *filt,
+                        const float *weights, const int64_t *lins, int n) {
+  int64_t counts[ORACLE_MAX_NDIM], vstr[ORACLE_MAX_NDIM];
+  for (int d = 0; d < ndim; ++d) counts[d] = vol_shape[d] - block[d] + 1;
+  c_strides(ndim, vol_shape, vstr);
+  int P = (int)prod_int(block, ndim);
+  for (int i = 0; i < n; ++i) {
+    int64_t lin = lins[i], base = 0;
+    for (int d = ndim - 1; d >= 0; --d) {
+      base += (lin % counts[d]) * vstr[d];
+      lin /= counts[d];
+    }
+    for (int p = 0; p < P; ++p) {
+      int lc[ORACLE_MAX_NDIM];
+      local_coords(ndim, block, p, lc);
+      int64_t v = base;
+      for (int d = 0; d < ndim; ++d) v += lc[d] * vstr[d];
+      float ww = weights[i] * kaiser[p];
+      num[v] += filt[(int64_t)i * P + p] * ww;
+      den[v] += ww;
+    }
+  }
+}
+
+/* ======================================================================== */
 /* full two-stage pipeline (bmndalgo.py:71-1063, Poisson branches)            */
 /* ======================================================================== */
 
@@ -908,11 +978,16 @@ static void geom_init(stage_geom *g, const oracle_params *prm, const int *block,
   }
   g->kaiser = (float *)malloc(sizeof(float) * (size_t)g->P);
   oracle_kaiser_nd(g->ndim, block, beta, g->kaiser);
+  const float *given = wiener ? prm->wi_spatial : prm->ht_spatial;
+  int64_t off = 0;
   for (int d = 0; d < g->ndim; ++d) {
     int b = block[d];
     g->sp_fwd[d] = (float *)malloc(sizeof(float) * (size_t)b * b);
     g->sp_inv[d] = (float *)malloc(sizeof(float) * (size_t)b * b);
-    if (wiener) {
+    if (given) {
+      memcpy(g->sp_fwd[d], given + off, sizeof(float) * (size_t)b * b);
+      off += (int64_t)b * b;
+    } else if (wiener) {
       oracle_dct_matrix(b, g->sp_fwd[d], g->sp_inv[d]);
     } else if (oracle_bior15_matrix(b, g->sp_fwd[d], g->sp_inv[d]) != 0) {
       /* bior1.5 lengths other than 4/8 go through pywt in the reference;
@@ -920,6 +995,12 @@ static void geom_init(stage_geom *g, const oracle_params *prm, const int *block,
       oracle_dct_matrix(b, g->sp_fwd[d], g->sp_inv[d]);
     }
   }
+  if (given)
+    for (int d = 0; d < g->ndim; ++d) {
+      int b = block[d];
+      memcpy(g->sp_inv[d], given + off, sizeof(float) * (size_t)b * b);
+      off += (int64_t)b * b;
+    }
 }
 
 static void geom_free(stage_geom *g) {
@@ -1043,37 +1124,10 @@ static int run_stage(const oracle_params *prm, int wiener, const float *match_vo
       oracle_apply_transform(X, ndim + 1, dims, fm);
       oracle_noise_std(var_vol, ndim, g.shape, block, n, apos, gfwd, spf, prm->k, sig);
       if (!wiener) {
-        /* hard threshold, bmndalgo.py:460-485 and poisson.py:30-46 */
-        float lam = prm->ht_lambda;
-        double cs = prm->count_scale;
-        for (int e = 0; e < n * P; ++e) {
-          float t = lam * sig[e];
-          double cc = (double)fabsf(X[e]) / cs;
-          double tc = (double)t / cs;
-          double sc = cc > tc ? cc : tc;
-          if (sc < 1.0) sc = 1.0;
-          double tol = (128.0 * F32_EPS) * sc;
-          int keep = cc > tc + tol;
-          att[e] = keep ? 1.0f : 0.0f;
-          X[e] = X[e] * att[e];
-        }
+        oracle_ht_shrink(X, sig, (int64_t)n * P, prm->ht_lambda, prm->count_scale, att);
       } else {
-        /* Wiener gain, wiener.py:24-53, 78-95 and bmndalgo.py:878-891 */
         oracle_apply_transform(Y, ndim + 1, dims, fm);
-        float mu = prm->wi_mu;
-        float eps32 = (float)1e-10;
-        for (int e = 0; e < n * P; ++e) {
-          float nv = sig[e] * sig[e];
-          float eff = mu * nv;
-          float pr = fabsf(Y[e]) * fabsf(Y[e]);
-          float spow = pr - eff;
-          if (spow < 0.0f) spow = 0.0f;
-          float gain = spow / ((spow + eff) + eps32);
-          if (gain < 0.0f) gain = 0.0f;
-          if (gain > 1.0f) gain = 1.0f;
-          att[e] = gain;
-          X[e] = X[e] * gain;
-        }
+        oracle_wiener_shrink(X, Y, sig, (int64_t)n * P, prm->wi_mu, att);
       }
       oracle_apply_transform(X, ndim + 1, dims, im);
       float *fo = filt + off[r] * P;
@@ -1092,19 +1146,9 @@ static int run_stage(const oracle_params *prm, int wiener, const float *match_vo
   /* sequential aggregation in schedule order (patchaggregation.py:170-199) */
   float *num = (float *)calloc((size_t)nvox, sizeof(float));
   float *den = (float *)calloc((size_t)nvox, sizeof(float));
-  for (int64_t r = 0; r < nrefs; ++r) {
-    for (int i = 0; i < cnt[r]; ++i) {
-      int64_t base = lin_to_vol_base(&g, sel[r * max_m + i]);
-      float w = wts[off[r] + i];
-      const float *fv = filt + (off[r] + i) * P;
-      for (int p = 0; p < P; ++p) {
-        float ww = w * g.kaiser[p];
-        int64_t v = base + g.poff[p];
-        num[v] += fv[p] * ww;
-        den[v] += ww;
-      }
-    }
-  }
+  for (int64_t r = 0; r < nrefs; ++r)
+    oracle_scatter_add(num, den, ndim, g.shape, block, g.kaiser, filt + off[r] * P, wts + off[r],
+                       sel + r * max_m, cnt[r]);
   /* normalisation, bmndalgo.py:626-631 / 1039-1044 */
   float eps32 = (float)1e-10;
   for (int64_t v = 0; v < nvox; ++v) est[v] = den[v] > 0.0f ? num[v] / (den[v] + eps32) : fallback[v];

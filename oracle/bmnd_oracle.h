@@ -70,6 +70,14 @@ void oracle_patch_weights(const float *atten, const float *sigma, int n,
                           int p, const float *group_inv, double eps,
                           float *w_out);
 
+void oracle_ht_shrink(float *coef, const float *sigma, int64_t n, float lambda_,
+                      double count_scale, float *atten);
+void oracle_wiener_shrink(float *coef, const float *ref_coef, const float *sigma,
+                          int64_t n, float mu, float *atten);
+void oracle_scatter_add(float *num, float *den, int ndim, const int64_t *vol_shape,
+                        const int *block, const float *kaiser, const float *filt,
+                        const float *weights, const int64_t *lins, int n);
+
 /* ---- full pipeline ----------------------------------------------------- */
 typedef struct {
   int ndim;
@@ -88,6 +96,11 @@ typedef struct {
   double count_scale;
   double variance_floor;
   int nthreads;
+  /* optional spatial matrices per stage: for each axis d the analysis
+   * matrix (b_d x b_d), then for each axis the synthesis matrix; NULL builds
+   * them here (bior1.5 for HT, DCT for Wiener) */
+  const float *ht_spatial;
+  const float *wi_spatial;
 } oracle_params;
 
 /* Runs both stages.  pilot may be NULL.  The match lists of each stage can

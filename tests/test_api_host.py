@@ -1,0 +1,105 @@
+"""Host-side behaviour of the drop-in API that needs no GPU: profile
+defaults, validation order and messages (tests/test_bmnd_pipeline_unit.py of
+the reference), enum coercion and the all-zero fast path."""
+
+import numpy as np
+import pytest
+
+from bmnd_amd import (BM3DProfile, BM4DProfile, BlockMatchDistance, NoiseModel,
+                      PoissonWienerGainMode, ReferenceScheduleMode, bmnd)
+from bmnd_amd.profilevalidation import resolve_profile
+
+
+def _poisson(profile_cls=BM3DProfile, **kw):
+    return profile_cls(noise_model=NoiseModel.POISSON, **kw)
+
+
+def test_profile_defaults():
+    p3, p4 = BM3DProfile(), BM4DProfile()
+    assert (p3.ht_block_size, p3.wiener_block_size, p3.ht_search_window) == ((8, 8), (8, 8), (19, 19))
+    assert (p4.ht_block_size, p4.wiener_block_size, p4.ht_search_window) == ((4, 4, 4), (5, 5, 5), (7, 7, 7))
+    for p in (p3, p4):
+        assert (p.ht_step, p.ht_max_stack_size, p.wiener_max_stack_size) == ((3,) * len(p.ht_step), 16, 32)
+        assert p.ht_match_threshold == 2.9527 and p.ht_lambda_threshold == 3.0
+        assert p.wiener_variance_scale == 0.4 and p.k == 4
+        assert p.reference_schedule_mode is ReferenceScheduleMode.GENERATED
+    assert p3.wiener_match_threshold == 0.3937 and p4.wiener_match_threshold == 0.7689
+
+
+def test_distance_resolves_to_deviance_and_strings_are_coerced():
+    p = BM3DProfile(noise_model="poisson", poisson_wiener_gain_mode="variance-scaled")
+    r = resolve_profile(p, (64, 64))
+    assert r.is_poisson and r.blockmatch_distance is BlockMatchDistance.POISSON_DEVIANCE
+    assert p.noise_model is NoiseModel.POISSON
+    assert p.poisson_wiener_gain_mode is PoissonWienerGainMode.VARIANCE_SCALED
+    assert r.variance_k == 4 and p.k == 4 and p.gamma == 3.0
+
+
+def test_sigma_and_sigma_psd_are_exclusive():
+    with pytest.raises(ValueError, match="mutually exclusive"):
+        bmnd(np.ones((16, 16)), _poisson(), sigma=1.0, sigma_psd=1.0)
+
+
+def test_poisson_rejects_sigma_psd():
+    with pytest.raises(ValueError, match="sigma_psd is only supported"):
+        bmnd(np.ones((16, 16)), _poisson(), sigma_psd=1.0)
+
+
+def test_rejects_negative_and_non_finite_input():
+    with pytest.raises(ValueError, match="non-negative"):
+        bmnd(-np.ones((16, 16)), _poisson())
+    x = np.ones((16, 16))
+    x[3, 3] = np.nan
+    with pytest.raises(ValueError, match="finite values"):
+        bmnd(x, _poisson())
+    with pytest.raises(ValueError, match="representable as finite float32"):
+        bmnd(np.full((16, 16), 1e300), _poisson())
+
+
+@pytest.mark.parametrize("field,value,msg", [
+    ("ht_block_size", (8,), "must contain 2 values"),
+    ("ht_step", (0, 3), "must be positive"),
+    ("ht_min_stack_size", 40, "cannot exceed"),
+    ("ht_match_threshold", -1.0, "finite and non-negative"),
+    ("k", -1, "non-negative integer"),
+    ("poisson_count_scale", 0.0, "finite and positive"),
+    ("ref_batch_size", 0, "positive integer"),
+])
+def test_profile_validation_messages(field, value, msg):
+    p = _poisson()
+    setattr(p, field, value)
+    with pytest.raises(ValueError, match=msg):
+        bmnd(np.ones((32, 32)), p)
+
+
+def test_cache_argument_validation():
+    with pytest.raises(ValueError, match="max_cache_bytes"):
+        bmnd(np.zeros((16, 16)), _poisson(), max_cache_bytes=-3)
+
+
+def test_all_zero_volume_fast_path_runs_no_kernel():
+    calls = []
+    z = np.zeros((20, 20), np.float32)
+    out = bmnd(z, _poisson(), stage_callback=lambda name, est: calls.append((name, est.shape)))
+    assert out is z
+    assert calls == [("ht", (20, 20)), ("wiener", (20, 20))]
+    out, maps = bmnd(z, _poisson(), return_sigma_map=True)
+    assert set(maps) == {"ht", "wiener"}
+
+
+@pytest.mark.parametrize("field,value", [
+    ("poisson_wiener_gain_mode", "classic"),
+    ("ht_weight_scope", "group"),
+    ("poisson_group_mass_conservation", "ht"),
+    ("blockmatch_distance", "pearson"),
+])
+def test_unsupported_options_raise_before_any_gpu_work(field, value):
+    p = _poisson()
+    setattr(p, field, value)
+    with pytest.raises(NotImplementedError):
+        bmnd(np.ones((32, 32)), p)
+
+
+def test_gaussian_profiles_are_not_run():
+    with pytest.raises(NotImplementedError, match="Poisson"):
+        bmnd(np.ones((32, 32)), BM3DProfile())
