@@ -88,6 +88,33 @@ __device__ __forceinline__ double dlog(double x, const double *__restrict__ tab)
   return s2 + lo;
 }
 
+// Same decomposition without the error-free sums (about 1 ulp); used by the
+// one-log block-matching mode, whose terms are not required to be
+// bit-identical to the reference expression.
+__device__ __forceinline__ double dlog_fast(double x, const double *__restrict__ tab) {
+  int e;
+  double m = frexp(x, &e);
+  if (m < 0.70710678118654752440) {
+    m = m * 2.0;
+    e -= 1;
+  }
+  int j = (int)rint((m - 1.0) * 128.0) - kLogTabLo;
+  j = min(max(j, 0), kLogTabN - 1);
+  const double invc = tab[3 * j], thi = tab[3 * j + 1], tlo = tab[3 * j + 2];
+  const double r = fma(m, invc, -1.0);
+  double p = -0.125;
+  p = fma(p, r, 1.0 / 7.0);
+  p = fma(p, r, -1.0 / 6.0);
+  p = fma(p, r, 0.2);
+  p = fma(p, r, -0.25);
+  p = fma(p, r, 1.0 / 3.0);
+  p = fma(p, r, -0.5);
+  const double ed = (double)e;
+  const double hi = fma(ed, tab[3 * kLogTabN], thi);
+  const double lo = fma(ed, tab[3 * kLogTabN + 1], tlo) + (r * r) * p;
+  return hi + (r + lo);
+}
+
 // tie grid of blockmatching.py:957-964: q = 32*eps32*2^floor(log2(max(|d|,1)))
 __device__ __forceinline__ float canonical_key(float d) {
   double mag = fabs((double)d);
@@ -212,6 +239,7 @@ __global__ __launch_bounds__(kBMThreads) void k_blockmatch(BMArgs a) {
   __shared__ unsigned long long s_prefix;
   __shared__ int s_krem, s_cnt, s_acc;
   __shared__ int s_refrow[P];
+  __shared__ double s_lc[MODE == 1 ? SLAB : 1];  // MODE 1: c*log(2c) per slab voxel
   extern __shared__ double s_terms[];  // MODE 2 only
 
   const int tid = threadIdx.x;
@@ -250,9 +278,15 @@ __global__ __launch_bounds__(kBMThreads) void k_blockmatch(BMArgs a) {
   if (tid == 0) s_acc = 0;
   __syncthreads();
   if (MODE == 1) {
+    // x*log(2x) of every slab voxel and reference voxel (0 at x = 0)
     for (int p = tid; p < P; p += kBMThreads) {
-      double r = s_ref[p];
-      s_ref_log[p] = r > 0.0 ? dlog(2.0 * r, s_tab) : 0.0;
+      const double r = s_ref[p];
+      s_ref_log[p] = r > 0.0 ? r * dlog_fast(2.0 * r, s_tab) : 0.0;
+    }
+    for (int i = tid; i < SLAB; i += kBMThreads) {
+      double c = (double)s_slab[i] * a.inv_scale;
+      if (c < 0.0) c = 0.0;
+      s_lc[i] = c > 0.0 ? c * dlog_fast(2.0 * c, s_tab) : 0.0;
     }
     __syncthreads();
   }
@@ -269,6 +303,7 @@ __global__ __launch_bounds__(kBMThreads) void k_blockmatch(BMArgs a) {
     for (int p01 = 0; p01 < B0 * B1; ++p01) {
       const int p0 = p01 / B1, p1 = p01 - p0 * B1;
       const float *row = s_slab + base + (p0 * S1 + p1) * S2;
+      const double *lcrow = s_lc + (MODE == 1 ? base + (p0 * S1 + p1) * S2 : 0);
       const double *rrow = s_ref + p01 * B2;
       const double *lrow = s_ref_log + p01 * B2;
       const int *trow = s_refrow + p01 * B2;
@@ -292,11 +327,9 @@ __global__ __launch_bounds__(kBMThreads) void k_blockmatch(BMArgs a) {
           if (r > 0.0) t = t + r * lr;
           t = (c > 0.0) ? t + c * lc : t;
         } else {
-          const double ls = dlog(s, s_tab);
-          const double lc = dlog(2.0 * c, s_tab);
-          t = 0.0;
-          if (r > 0.0) t = t + r * (lrow[p2] - ls);
-          t = (c > 0.0) ? t + c * (lc - ls) : t;
+          // r log(2r/s) + c log(2c/s) = r log 2r + c log 2c - s log s
+          const double ls = dlog_fast(s, s_tab);
+          t = fma(-s, ls, lrow[p2] + lcrow[p2]);
         }
         d = (s > 0.0) ? d + 2.0 * t : d;
       }
@@ -473,7 +506,7 @@ __global__ __launch_bounds__(NT) void k_exact_std(ExactArgs a) {
   __shared__ unsigned s_key[SORTN];
   __shared__ float s_t01[B01 * B01];  // [p01][s01]
   __shared__ float s_a2[B2 * B2];     // [p2][s2]
-  __shared__ float s_grow[4 * NMAX];
+  __shared__ float4 s_g4[NMAX];
   __shared__ int s_pos[NMAX][3];
   __shared__ int s_min[3], s_ushape[3];
 
@@ -554,44 +587,63 @@ __global__ __launch_bounds__(NT) void k_exact_std(ExactArgs a) {
   const bool active = s < P;
   const int s01 = active ? s / B2 : 0, s2 = active ? s - (s / B2) * B2 : 0;
   const int m0 = s_min[0], m1 = s_min[1], m2 = s_min[2];
+  const int lane = tid & 63;
   for (int q0 = 0; q0 < planes; q0 += 4) {
-    for (int t = tid; t < 4 * n; t += NT) {
-      const int j = t / n, i = t - j * n;
-      s_grow[j * NMAX + i] = (q0 + j < planes) ? gf[(q0 + j) * n + i] : 0.0f;
+    for (int i = tid; i < n; i += NT) {
+      float4 gv;
+      gv.x = gf[q0 * n + i];
+      gv.y = (q0 + 1 < planes) ? gf[(q0 + 1) * n + i] : 0.0f;
+      gv.z = (q0 + 2 < planes) ? gf[(q0 + 2) * n + i] : 0.0f;
+      gv.w = (q0 + 3 < planes) ? gf[(q0 + 3) * n + i] : 0.0f;
+      s_g4[i] = gv;
     }
     __syncthreads();
     float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f, acc3 = 0.0f;
     float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f, c3 = 0.0f;
-    unsigned prev = __builtin_amdgcn_readfirstlane(s_key[0]) >> 12;
-    auto finish_voxel = [&](unsigned label) {
-      const int l2 = (int)(label % (unsigned)u2);
-      const int l1 = (int)((label / (unsigned)u2) % (unsigned)u1);
-      const int l0 = (int)(label / (unsigned)(u2 * u1));
-      const float vv = a.var[((int64_t)(l0 + m0) * a.n1 + (l1 + m1)) * a.n2 + (l2 + m2)];
-      acc0 = acc0 + (vv * c0) * c0;
-      acc1 = acc1 + (vv * c1) * c1;
-      acc2 = acc2 + (vv * c2) * c2;
-      acc3 = acc3 + (vv * c3) * c3;
-      c0 = c1 = c2 = c3 = 0.0f;
-    };
-    for (int pos = 0; pos < np; ++pos) {
-      const unsigned key = __builtin_amdgcn_readfirstlane(s_key[pos]);
+    float cur_vv = 0.0f;
+    unsigned prev_label = 0xffffffffu;
+    for (int base = 0; base < np; base += 64) {
+      // each lane decodes one entry of the chunk; the walk below reads them
+      // back with readlane, so the serial path does no dependent LDS loads
+      const int e = base + lane;
+      const unsigned key = e < np ? s_key[e] : 0xffffffffu;
       const unsigned label = key >> 12;
-      if (label != prev) {
-        finish_voxel(prev);
-        prev = label;
+      const int ei = (int)((key >> 7) & 31u), ep = (int)(key & 127u);
+      const int ep01 = ep / B2, ep2 = ep - ep01 * B2;
+      const int code = (ep01 * B01) | ((ep2 * B2) << 12) | (ei << 20);
+      float vv = 0.0f;
+      if (e < np) {
+        const int l2 = (int)(label % (unsigned)u2);
+        const int l1 = (int)((label / (unsigned)u2) % (unsigned)u1);
+        const int l0 = (int)(label / (unsigned)(u2 * u1));
+        vv = a.var[((int64_t)(l0 + m0) * a.n1 + (l1 + m1)) * a.n2 + (l2 + m2)];
       }
-      const int i = (int)((key >> 7) & 31u), p = (int)(key & 127u);
-      const int p01 = p / B2, p2 = p - p01 * B2;
-      const float sv = s_t01[p01 * B01 + s01] * s_a2[p2 * B2 + s2];
-      const float g0 = s_grow[i], g1 = s_grow[NMAX + i], g2 = s_grow[2 * NMAX + i],
-                  g3 = s_grow[3 * NMAX + i];
-      if (g0 != 0.0f) c0 = c0 + g0 * sv;
-      if (g1 != 0.0f) c1 = c1 + g1 * sv;
-      if (g2 != 0.0f) c2 = c2 + g2 * sv;
-      if (g3 != 0.0f) c3 = c3 + g3 * sv;
+      const int cnt = min(64, np - base);
+      for (int j = 0; j < cnt; ++j) {
+        const unsigned lab = (unsigned)__builtin_amdgcn_readlane((int)label, j);
+        if (lab != prev_label) {
+          // close the previous union voxel (a no-op before the first one)
+          acc0 = acc0 + (cur_vv * c0) * c0;
+          acc1 = acc1 + (cur_vv * c1) * c1;
+          acc2 = acc2 + (cur_vv * c2) * c2;
+          acc3 = acc3 + (cur_vv * c3) * c3;
+          c0 = c1 = c2 = c3 = 0.0f;
+          cur_vv = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, vv), j));
+          prev_label = lab;
+        }
+        const int cd = __builtin_amdgcn_readlane(code, j);
+        const float sv = s_t01[(cd & 4095) + s01] * s_a2[((cd >> 12) & 255) + s2];
+        const float4 gq = s_g4[cd >> 20];
+        if (gq.x != 0.0f) c0 = c0 + gq.x * sv;
+        if (gq.y != 0.0f) c1 = c1 + gq.y * sv;
+        if (gq.z != 0.0f) c2 = c2 + gq.z * sv;
+        if (gq.w != 0.0f) c3 = c3 + gq.w * sv;
+      }
     }
-    finish_voxel(prev);
+    acc0 = acc0 + (cur_vv * c0) * c0;
+    acc1 = acc1 + (cur_vv * c1) * c1;
+    acc2 = acc2 + (cur_vv * c2) * c2;
+    acc3 = acc3 + (cur_vv * c3) * c3;
     if (active) {
       float *dst = a.exact_std + (g * a.kcap + q0) * P + s;
       const float accs[4] = {acc0, acc1, acc2, acc3};
