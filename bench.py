@@ -42,10 +42,14 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--cpu-sample", type=int, default=64,
                     help="side of the cube (3-D) or square (2-D) the CPU oracle is timed on")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--deviance", default="auto",
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB mebibytes (for scaling "
+                         "rocprofv3 FETCH_SIZE/WRITE_SIZE readings) and exit")
+    ap.add_argument("--deviance", default="auto-one-log",
                     choices=["auto", "exact", "one-log", "auto-one-log"],
-                    help="block-matching deviance evaluation (auto: integer-count table, "
-                         "else the reference's two-log form)")
+                    help="block-matching deviance evaluation (integer counts use the "
+                         "term table; other inputs the one-log form, or with 'auto' the "
+                         "reference's two-log form)")
     return ap.parse_args()
 
 
@@ -68,6 +72,18 @@ def make_inputs(config: str):
     return BM3DProfile(noise_model=NoiseModel.POISSON), [noisy], [truth], {
         "workload": "64^2 README square image, BM3D Poisson", "shape": [64, 64],
         "profile": "BM3DProfile(noise_model=POISSON)", "seed": 0}
+
+
+def measured_traffic(config: str, deviance: str, kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic.json")
+    try:
+        data = json.load(open(path))
+        k = data[f"{config}/{deviance}"]["kernels"][kernel]
+        return int(k["fetch_bytes"] + k["write_bytes"])
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def cpu_baseline(config: str, noisy: np.ndarray, side: int) -> dict:
@@ -96,6 +112,16 @@ def main() -> int:
               "(every run of this build uses a single MI355X).", file=sys.stderr)
         return 2
     import torch
+
+    if args.calibrate_copy:
+        n = args.calibrate_copy * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4}))
+        return 0
 
     from bmnd_amd.engine import KernelTimer, PoissonPipeline, blockmatch_work
 
@@ -131,11 +157,15 @@ def main() -> int:
     roof = None
     if dominant.endswith(".blockmatch"):
         st = dominant.split(".")[0]
+        plan = pipe.plans[st]
+        kname = "k_blockmatch<{}, {}, {}, {}, {}, {}, {}>(BMArgs)".format(
+            *plan.block3, *plan.radius3, pipe.last_modes.get(st, 0))
         flops = FLOPS_PER_TERM * work[st][1]
         achieved = flops / (per_kernel[dominant] * 1e-3) / 1e12
-        roof = {"kernel": f"k_blockmatch ({st})", "bound": "mfma", "achieved": round(achieved, 3),
+        roof = {"kernel": kname, "bound": "mfma", "achieved": round(achieved, 3),
                 "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
+                "traffic": measured_traffic(args.config, args.deviance, kname),
                 "algorithmic": f"{FLOPS_PER_TERM} ops x {work[st][1]} deviance terms per launch",
                 "avg_ms": round(per_kernel[dominant], 3)}
     else:

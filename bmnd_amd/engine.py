@@ -223,7 +223,7 @@ class PoissonPipeline:
     DEVIANCE_MODES = ("auto", "exact", "one-log", "auto-one-log")
 
     def __init__(self, profile: BMNDProfile, shape: tuple[int, ...], device=None,
-                 deviance: str = "auto"):
+                 deviance: str = "auto-one-log"):
         self.profile = profile
         self.shape = tuple(int(s) for s in shape)
         self.ndim = len(self.shape)
@@ -231,9 +231,13 @@ class PoissonPipeline:
         self.device = torch.device(device if device is not None else "cuda")
         if deviance not in self.DEVIANCE_MODES:
             raise ValueError(f"deviance must be one of {self.DEVIANCE_MODES}, got {deviance!r}")
-        # "auto": integer counts <= 100 use the float64 term table (bit-identical
-        # to the reference expression), anything else the two-log form; the
-        # "one-log" variants trade last-bit agreement of the distance for speed.
+        # Integer counts <= 100 always use the float64 term table (the reference
+        # expression, evaluated on the host with the C library's log).  Other
+        # inputs: "auto" evaluates the reference's two-log expression term by
+        # term; "auto-one-log" (default) uses r log 2r + c log 2c - s log s
+        # with one log per term, which can move a distance by ~1e-14 and so a
+        # key across the 32-ulp tie grid only in rare cases (none observed on
+        # the 256^3 benchmark volume).
         self.deviance = deviance
         self.last_modes: dict[str, int] = {}
         self.count_scale = float(profile.poisson_count_scale)

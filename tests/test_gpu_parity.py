@@ -45,7 +45,7 @@ def _compare(noisy, profile, params, deviance="auto"):
 def test_readme_square_bm3d_matches_oracle():
     _, noisy = phantoms.square_image(0)
     prof = BM3DProfile(noise_model=NoiseModel.POISSON)
-    stats, out, o_out = _compare(noisy, prof, oracle.default_params(2, noisy.shape))
+    stats, out, o_out = _compare(noisy, prof, oracle.default_params(2, noisy.shape), "auto")
     assert stats["ht_groups_differing"] == 0
     assert stats["wiener_groups_differing"] == 0
     assert stats["pilot_bits_differing"] == 0
@@ -56,7 +56,7 @@ def test_small_volume_bm4d_matches_oracle():
     _, noisy = phantoms.ellipsoid_volume((28, 32, 24), seed=3, peak=10.0, n_ellipsoids=6,
                                          axes_range=(3.0, 12.0))
     prof = BM4DProfile(noise_model=NoiseModel.POISSON)
-    stats, out, o_out = _compare(noisy, prof, oracle.default_params(3, noisy.shape))
+    stats, out, o_out = _compare(noisy, prof, oracle.default_params(3, noisy.shape), "auto")
     assert stats["ht_groups_differing"] == 0
     assert stats["wiener_groups_differing"] == 0
     assert stats["pilot_bits_differing"] == 0
