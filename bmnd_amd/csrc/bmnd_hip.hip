@@ -115,6 +115,33 @@ __device__ __forceinline__ double dlog_fast(double x, const double *__restrict__
   return hi + (r + lo);
 }
 
+// Leaner float64 log for the one-log mode: x = 2^e * m, m in [1, 2) taken
+// from the IEEE bits, j = top 7 mantissa bits, c_j = 1 + (j + 1/2)/128,
+// log x = e ln2 + log c_j + log1p(m/c_j - 1) with |m/c_j - 1| < 2^-8 and a
+// degree-7 Taylor series.  Table entries {1/c_j, log c_j} are 16-byte pairs
+// (computed on the host in extended precision).  Error about 1 ulp; x must be
+// a positive normal number or 0 (returns a finite value for 0).
+constexpr int kFastLogN = 128;
+__device__ __forceinline__ double dlog_bits(double x, const double2 *__restrict__ tab,
+                                            double ln2) {
+  const unsigned long long bits = __double_as_longlong(x);
+  const unsigned hi = (unsigned)(bits >> 32);
+  const int e = (int)(hi >> 20) - 1023;
+  const int j = (int)((hi >> 13) & 127u);
+  const double m = __longlong_as_double((long long)((bits & 0x000fffffffffffffull) |
+                                                    0x3ff0000000000000ull));
+  const double2 t = tab[j];
+  const double r = fma(m, t.x, -1.0);
+  double q = 1.0 / 7.0;
+  q = fma(q, r, -1.0 / 6.0);
+  q = fma(q, r, 0.2);
+  q = fma(q, r, -0.25);
+  q = fma(q, r, 1.0 / 3.0);
+  q = fma(q, r, -0.5);
+  const double l1p = fma(r * r, q, r);
+  return fma((double)e, ln2, t.y) + l1p;
+}
+
 // tie grid of blockmatching.py:957-964: q = 32*eps32*2^floor(log2(max(|d|,1)))
 __device__ __forceinline__ float canonical_key(float d) {
   double mag = fabs((double)d);
@@ -209,6 +236,7 @@ struct BMArgs {
   int max_m, min_m, pow2, stabilize;
   double inv_scale;
   const double *logtab;
+  const double2 *fastlog;    // MODE 1: {1/c_j, log c_j}, j < kFastLogN
   const double *term_table;  // MODE 2: (tmax+1)^2 terms, row = reference count
   int tmax;
   int32_t *sel, *counts, *accepted;
@@ -219,7 +247,8 @@ constexpr int kMaxStack = 64;
 constexpr int kMaxTableCount = 100;
 
 // MODE 0: each term as the reference writes it (two divisions, two logs);
-// MODE 1: one log per term, r*(log 2r - log s) + c*(log 2c - log s);
+// MODE 1: one log per term, r log 2r + c log 2c - s log s (values clamped
+//         and scaled when the slab is staged);
 // MODE 2: integer counts; the term is read from a float64 table built on the
 //         host with the same expression and the C library's log.
 template <int B0, int B1, int B2, int R0, int R1, int R2, int MODE>
@@ -240,6 +269,7 @@ __global__ __launch_bounds__(kBMThreads) void k_blockmatch(BMArgs a) {
   __shared__ int s_krem, s_cnt, s_acc;
   __shared__ int s_refrow[P];
   __shared__ double s_lc[MODE == 1 ? SLAB : 1];  // MODE 1: c*log(2c) per slab voxel
+  __shared__ double2 s_flog[MODE == 1 ? kFastLogN : 1];
   extern __shared__ double s_terms[];  // MODE 2 only
 
   const int tid = threadIdx.x;
@@ -278,19 +308,25 @@ __global__ __launch_bounds__(kBMThreads) void k_blockmatch(BMArgs a) {
   if (tid == 0) s_acc = 0;
   __syncthreads();
   if (MODE == 1) {
-    // x*log(2x) of every slab voxel and reference voxel (0 at x = 0)
+    // x log 2x of every slab voxel and reference voxel (0 at x = 0); the slab
+    // keeps max(x, 0) (count_scale is 1 in this mode)
+    for (int i = tid; i < kFastLogN; i += kBMThreads) s_flog[i] = a.fastlog[i];
+    __syncthreads();
+    const double ln2 = s_tab[3 * kLogTabN] + s_tab[3 * kLogTabN + 1];
     for (int p = tid; p < P; p += kBMThreads) {
       const double r = s_ref[p];
-      s_ref_log[p] = r > 0.0 ? r * dlog_fast(2.0 * r, s_tab) : 0.0;
+      s_ref_log[p] = r > 0.0 ? r * dlog_bits(2.0 * r, s_flog, ln2) : 0.0;
     }
     for (int i = tid; i < SLAB; i += kBMThreads) {
-      double c = (double)s_slab[i] * a.inv_scale;
-      if (c < 0.0) c = 0.0;
-      s_lc[i] = c > 0.0 ? c * dlog_fast(2.0 * c, s_tab) : 0.0;
+      const float cf = s_slab[i] > 0.0f ? s_slab[i] : 0.0f;
+      const double c = (double)cf;
+      s_lc[i] = cf > 0.0f ? c * dlog_bits(2.0 * c, s_flog, ln2) : 0.0;
+      s_slab[i] = cf;
     }
     __syncthreads();
   }
 
+  const double ln2k = s_tab[3 * kLogTabN] + s_tab[3 * kLogTabN + 1];
   const int rel0 = o0 - lo0, rel1 = o1 - lo1, rel2 = o2 - lo2;
   const int center_local = (rel0 * e1 + rel1) * e2 + rel2;
   int my_acc = 0;
@@ -314,6 +350,13 @@ __global__ __launch_bounds__(kBMThreads) void k_blockmatch(BMArgs a) {
           d = d + s_terms[trow[p2] + (int)row[p2]];
           continue;
         }
+        if (MODE == 1) {
+          // the term is exactly 0 at r = c = 0 (the log of 0 stays finite)
+          const double s = rrow[p2] + (double)row[p2];
+          const double ls = dlog_bits(s, s_flog, ln2k);
+          d = d + fma(-s, ls, lrow[p2] + lcrow[p2]);
+          continue;
+        }
         double c = (double)row[p2] * a.inv_scale;
         if (c < 0.0) c = 0.0;
         const double r = rrow[p2];
@@ -334,6 +377,7 @@ __global__ __launch_bounds__(kBMThreads) void k_blockmatch(BMArgs a) {
         d = (s > 0.0) ? d + 2.0 * t : d;
       }
     }
+    if (MODE == 1) d = 2.0 * d;  // terms were accumulated without the factor 2
     if (d < 0.0) d = 0.0;
     const float d32 = (float)(d / (double)P);
     float key;
@@ -1167,6 +1211,30 @@ int term_table_for_device(int tmax, const double **out) {
   return 0;
 }
 
+std::vector<double *> g_fastlog;  // per device
+
+int fastlog_for_device(const double2 **out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_tab_mutex);
+  if ((int)g_fastlog.size() <= dev) g_fastlog.resize(dev + 1, nullptr);
+  if (!g_fastlog[dev]) {
+    std::vector<double> h(2 * kFastLogN);
+    for (int j = 0; j < kFastLogN; ++j) {
+      const long double c = 1.0L + ((long double)j + 0.5L) / 128.0L;
+      const double invc = (double)(1.0L / c);
+      h[2 * j] = invc;
+      h[2 * j + 1] = (double)(-logl((long double)invc));  // log of 1/invc exactly
+    }
+    double *d = nullptr;
+    HIP_TRY(hipMalloc(&d, sizeof(double) * h.size()));
+    HIP_TRY(hipMemcpy(d, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+    g_fastlog[dev] = d;
+  }
+  *out = reinterpret_cast<const double2 *>(g_fastlog[dev]);
+  return 0;
+}
+
 float host_canonical_key(float d) {
   double mag = std::fabs((double)d);
   if (mag < 1.0) mag = 1.0;
@@ -1246,9 +1314,13 @@ int32_t bmnd_hip_blockmatch(const float *vol, const int64_t *shape_h, const int3
     return fail(BMND_HIP_EINVAL, "table mode needs count_scale == 1 and 0 <= table_max <= 100");
   const double *tab = nullptr;
   if (int rc = logtab_for_device(&tab)) return rc;
+  if (mode == 1 && count_scale != 1.0) mode = 0;  // the one-log slab assumes unit scale
   const double *terms = nullptr;
   if (mode == 2)
     if (int rc = term_table_for_device(table_max, &terms)) return rc;
+  const double2 *flog = nullptr;
+  if (mode == 1)
+    if (int rc = fastlog_for_device(&flog)) return rc;
   BMArgs a;
   a.vol = vol;
   a.n0 = (int)shape_h[0];
@@ -1267,6 +1339,7 @@ int32_t bmnd_hip_blockmatch(const float *vol, const int64_t *shape_h, const int3
   a.stabilize = stabilize;
   a.inv_scale = 1.0 / count_scale;
   a.logtab = tab;
+  a.fastlog = flog;
   a.term_table = terms;
   a.tmax = table_max;
   a.sel = sel;
