@@ -31,6 +31,9 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 # algorithmic operations per Poisson-deviance term: 2 logarithms + 8 arithmetic
 FLOPS_PER_TERM = 10
+# workgroup size of the block-matching kernel per deviance mode (its last
+# template argument, part of the kernel name rocprofv3 reports)
+BM_THREADS = {0: 512, 1: 1024, 2: 512}
 
 
 def parse() -> argparse.Namespace:
@@ -158,8 +161,9 @@ def main() -> int:
     if dominant.endswith(".blockmatch"):
         st = dominant.split(".")[0]
         plan = pipe.plans[st]
-        kname = "k_blockmatch<{}, {}, {}, {}, {}, {}, {}>(BMArgs)".format(
-            *plan.block3, *plan.radius3, pipe.last_modes.get(st, 0))
+        mode = pipe.last_modes.get(st, 0)
+        kname = "k_blockmatch<{}, {}, {}, {}, {}, {}, {}, {}>(BMArgs)".format(
+            *plan.block3, *plan.radius3, mode, BM_THREADS[mode])
         flops = FLOPS_PER_TERM * work[st][1]
         achieved = flops / (per_kernel[dominant] * 1e-3) / 1e12
         roof = {"kernel": kname, "bound": "mfma", "achieved": round(achieved, 3),

@@ -243,6 +243,7 @@ struct BMArgs {
 };
 
 constexpr int kBMThreads = 512;
+constexpr int kBMThreadsOneLog = 1024;  // one workgroup per CU fits its LDS
 constexpr int kMaxStack = 64;
 constexpr int kMaxTableCount = 100;
 
@@ -251,8 +252,8 @@ constexpr int kMaxTableCount = 100;
 //         and scaled when the slab is staged);
 // MODE 2: integer counts; the term is read from a float64 table built on the
 //         host with the same expression and the C library's log.
-template <int B0, int B1, int B2, int R0, int R1, int R2, int MODE>
-__global__ __launch_bounds__(kBMThreads) void k_blockmatch(BMArgs a) {
+template <int B0, int B1, int B2, int R0, int R1, int R2, int MODE, int NT>
+__global__ __launch_bounds__(NT) void k_blockmatch(BMArgs a) {
   constexpr int S0 = 2 * R0 + B0, S1 = 2 * R1 + B1, S2 = 2 * R2 + B2;
   constexpr int SLAB = S0 * S1 * S2;
   constexpr int BOX = (2 * R0 + 1) * (2 * R1 + 1) * (2 * R2 + 1);
@@ -283,17 +284,17 @@ __global__ __launch_bounds__(kBMThreads) void k_blockmatch(BMArgs a) {
   const int e0 = hi0 - lo0 + 1, e1 = hi1 - lo1 + 1, e2 = hi2 - lo2 + 1;
   const int box = e0 * e1 * e2;
 
-  for (int i = tid; i < kLogTabDoubles; i += kBMThreads) s_tab[i] = a.logtab[i];
+  for (int i = tid; i < kLogTabDoubles; i += NT) s_tab[i] = a.logtab[i];
   // slab: volume region [lo, hi + B - 1] (always inside the volume)
   const int x0 = e0 + B0 - 1, x1 = e1 + B1 - 1, x2 = e2 + B2 - 1;
-  for (int i = tid; i < SLAB; i += kBMThreads) {
+  for (int i = tid; i < SLAB; i += NT) {
     int i2 = i % S2, i1 = (i / S2) % S1, i0 = i / (S2 * S1);
     float v = 0.0f;
     if (i0 < x0 && i1 < x1 && i2 < x2)
       v = a.vol[((int64_t)(lo0 + i0) * a.n1 + (lo1 + i1)) * a.n2 + (lo2 + i2)];
     s_slab[i] = v;
   }
-  for (int p = tid; p < P; p += kBMThreads) {
+  for (int p = tid; p < P; p += NT) {
     int p2 = p % B2, p1 = (p / B2) % B1, p0 = p / (B2 * B1);
     float v = a.vol[((int64_t)(o0 + p0) * a.n1 + (o1 + p1)) * a.n2 + (o2 + p2)];
     double r = (double)v * a.inv_scale;
@@ -303,21 +304,21 @@ __global__ __launch_bounds__(kBMThreads) void k_blockmatch(BMArgs a) {
   }
   if (MODE == 2) {
     const int tn = (a.tmax + 1) * (a.tmax + 1);
-    for (int i = tid; i < tn; i += kBMThreads) s_terms[i] = a.term_table[i];
+    for (int i = tid; i < tn; i += NT) s_terms[i] = a.term_table[i];
   }
   if (tid == 0) s_acc = 0;
   __syncthreads();
   if (MODE == 1) {
     // x log 2x of every slab voxel and reference voxel (0 at x = 0); the slab
     // keeps max(x, 0) (count_scale is 1 in this mode)
-    for (int i = tid; i < kFastLogN; i += kBMThreads) s_flog[i] = a.fastlog[i];
+    for (int i = tid; i < kFastLogN; i += NT) s_flog[i] = a.fastlog[i];
     __syncthreads();
     const double ln2 = s_tab[3 * kLogTabN] + s_tab[3 * kLogTabN + 1];
-    for (int p = tid; p < P; p += kBMThreads) {
+    for (int p = tid; p < P; p += NT) {
       const double r = s_ref[p];
       s_ref_log[p] = r > 0.0 ? r * dlog_bits(2.0 * r, s_flog, ln2) : 0.0;
     }
-    for (int i = tid; i < SLAB; i += kBMThreads) {
+    for (int i = tid; i < SLAB; i += NT) {
       const float cf = s_slab[i] > 0.0f ? s_slab[i] : 0.0f;
       const double c = (double)cf;
       s_lc[i] = cf > 0.0f ? c * dlog_bits(2.0 * c, s_flog, ln2) : 0.0;
@@ -330,7 +331,7 @@ __global__ __launch_bounds__(kBMThreads) void k_blockmatch(BMArgs a) {
   const int rel0 = o0 - lo0, rel1 = o1 - lo1, rel2 = o2 - lo2;
   const int center_local = (rel0 * e1 + rel1) * e2 + rel2;
   int my_acc = 0;
-  for (int cand = tid; cand < box; cand += kBMThreads) {
+  for (int cand = tid; cand < box; cand += NT) {
     const int j2 = cand % e2, j1 = (cand / e2) % e1, j0 = cand / (e2 * e1);
     const int base = (j0 * S1 + j1) * S2 + j2;
     double d = 0.0;
@@ -414,9 +415,9 @@ __global__ __launch_bounds__(kBMThreads) void k_blockmatch(BMArgs a) {
   unsigned long long prefix = 0ull, mask = 0ull;
   int krem = K;
   for (int shift = 56; shift >= 0; shift -= 8) {
-    for (int i = tid; i < 256; i += kBMThreads) s_hist[i] = 0;
+    for (int i = tid; i < 256; i += NT) s_hist[i] = 0;
     __syncthreads();
-    for (int i = tid; i < box; i += kBMThreads) {
+    for (int i = tid; i < box; i += NT) {
       unsigned long long k = s_keys[i];
       if ((k & mask) == prefix) atomicAdd(&s_hist[(int)((k >> shift) & 255ull)], 1);
     }
@@ -449,7 +450,7 @@ __global__ __launch_bounds__(kBMThreads) void k_blockmatch(BMArgs a) {
   }
   if (tid == 0) s_cnt = 0;
   __syncthreads();
-  for (int i = tid; i < box; i += kBMThreads) {
+  for (int i = tid; i < box; i += NT) {
     unsigned long long k = s_keys[i];
     if (k <= prefix) s_small[atomicAdd(&s_cnt, 1)] = k;
   }
@@ -524,8 +525,8 @@ struct FilterArgs {
 // S[s,p] = (A0[s0,p0]*A1[s1,p1])*A2[s2,p2] is the float32 Kronecker chain of
 // the spatial analysis matrices (variance.py:744-755); the first product is
 // tabulated (s_t01), so each lane does one multiply per entry.  Terms with
-// G[q,i] == 0 are skipped: they add a signed zero, which cannot change the
-// squared contribution.
+// G[q,i] == 0 add a signed zero: that leaves a nonzero partial sum unchanged
+// and a zero one zero, so no test is needed.
 struct ExactArgs {
   const float *var;
   int n0, n1, n2;
@@ -678,10 +679,10 @@ __global__ __launch_bounds__(NT) void k_exact_std(ExactArgs a) {
         const int cd = __builtin_amdgcn_readlane(code, j);
         const float sv = s_t01[(cd & 4095) + s01] * s_a2[((cd >> 12) & 255) + s2];
         const float4 gq = s_g4[cd >> 20];
-        if (gq.x != 0.0f) c0 = c0 + gq.x * sv;
-        if (gq.y != 0.0f) c1 = c1 + gq.y * sv;
-        if (gq.z != 0.0f) c2 = c2 + gq.z * sv;
-        if (gq.w != 0.0f) c3 = c3 + gq.w * sv;
+        c0 = c0 + gq.x * sv;
+        c1 = c1 + gq.y * sv;
+        c2 = c2 + gq.z * sv;
+        c3 = c3 + gq.w * sv;
       }
     }
     acc0 = acc0 + (cur_vv * c0) * c0;
@@ -1351,11 +1352,14 @@ int32_t bmnd_hip_blockmatch(const float *vol, const int64_t *shape_h, const int3
 #define BM_CASE(b0, b1, b2, r0, r1, r2)                                                        \
   if (same3(block_h, b0, b1, b2) && same3(radius_h, r0, r1, r2)) {                               \
     if (mode == 0)                                                                               \
-      hipLaunchKernelGGL((k_blockmatch<b0, b1, b2, r0, r1, r2, 0>), grid, blk, 0, s, a);         \
+      hipLaunchKernelGGL((k_blockmatch<b0, b1, b2, r0, r1, r2, 0, kBMThreads>), grid, blk, 0, s, \
+                         a);                                                                     \
     else if (mode == 1)                                                                          \
-      hipLaunchKernelGGL((k_blockmatch<b0, b1, b2, r0, r1, r2, 1>), grid, blk, 0, s, a);         \
+      hipLaunchKernelGGL((k_blockmatch<b0, b1, b2, r0, r1, r2, 1, kBMThreadsOneLog>), grid,      \
+                         dim3(kBMThreadsOneLog), 0, s, a);                                       \
     else                                                                                         \
-      hipLaunchKernelGGL((k_blockmatch<b0, b1, b2, r0, r1, r2, 2>), grid, blk, dyn, s, a);       \
+      hipLaunchKernelGGL((k_blockmatch<b0, b1, b2, r0, r1, r2, 2, kBMThreads>), grid, blk, dyn,  \
+                         s, a);                                                                  \
     HIP_TRY(hipGetLastError());                                                                  \
     return 0;                                                                                    \
   }
