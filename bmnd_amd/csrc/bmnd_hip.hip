@@ -700,27 +700,97 @@ __global__ __launch_bounds__(NT) void k_exact_std(ExactArgs a) {
   }
 }
 
-// apply M (n x n, out-major) along one axis of an LDS array of dims (d0,d1,d2,d3)
-// as fmaf chains in input order; src -> dst
-template <int NT>
-__device__ void apply_axis(const float *__restrict__ src, float *__restrict__ dst,
-                           int d0, int d1, int d2, int d3, int axis,
-                           const float *__restrict__ M) {
-  const int total = d0 * d1 * d2 * d3;
-  const int dims[4] = {d0, d1, d2, d3};
-  int inner = 1;
-  for (int d = axis + 1; d < 4; ++d) inner *= dims[d];
-  const int n = dims[axis];
-  for (int o = threadIdx.x; o < total; o += NT) {
-    const int in_i = o % inner;
-    const int r = (o / inner) % n;
-    const int outer = o / (inner * n);
-    const float *x = src + outer * n * inner + in_i;
-    const float *m = M + r * n;
-    float acc = 0.0f;
-    for (int kk = 0; kk < n; ++kk) acc = fmaf(m[kk], x[kk * inner], acc);
-    dst[o] = acc;
+// In-place transform along the member axis of an LDS array (n, P):
+//   data[r][c] <- sum_k M[r][k] * data[k][c], one fmaf chain over k = 0..n-1
+// (M out-major, rows stored with stride NMAX).  Lanes own columns (P padded to
+// whole waves) and each wave owns a block of rows, so every matrix read is a
+// same-address broadcast and every input is read from LDS once per row block.
+// All inputs are in registers before the barrier, so the writes can go back
+// into the same array.
+template <int NT, int P, int NMAX, int N>
+__device__ __forceinline__ void member_transform_n(float *__restrict__ data, int n,
+                                                   const float *__restrict__ M) {
+  constexpr int PP = (P + 63) / 64 * 64;
+  constexpr int RB = NT / PP;
+  static_assert(NT % PP == 0 && RB >= 1, "workgroup must hold whole column sets");
+  constexpr int NN = N > 0 ? N : NMAX;
+  constexpr int ROWS = (NN + RB - 1) / RB;
+  const int c = threadIdx.x % PP;
+  const int rb = __builtin_amdgcn_readfirstlane(threadIdx.x / PP);
+  const bool live = c < P;
+  float x[NN];
+#pragma unroll
+  for (int k = 0; k < NN; ++k) x[k] = (live && (N > 0 || k < n)) ? data[k * P + c] : 0.0f;
+  __syncthreads();
+  const int rpb = N > 0 ? ROWS : (n + RB - 1) / RB;
+#pragma unroll
+  for (int i = 0; i < ROWS; ++i) {
+    const int r = rb * rpb + i;
+    if ((N > 0 || i < rpb) && r < n) {
+      const float *m = M + r * NMAX;
+      float acc = 0.0f;
+#pragma unroll
+      for (int k = 0; k < NN; ++k)
+        if (N > 0 || k < n) acc = fmaf(m[k], x[k], acc);
+      if (live) data[r * P + c] = acc;
+    }
   }
+  __syncthreads();
+}
+
+template <int NT, int P, int NMAX>
+__device__ __forceinline__ void member_transform(float *__restrict__ data, int n,
+                                                 const float *__restrict__ M) {
+  if constexpr (NMAX >= 32) {
+    if (n == 32) return member_transform_n<NT, P, NMAX, 32>(data, n, M);
+  }
+  if constexpr (NMAX >= 16) {
+    if (n == 16) return member_transform_n<NT, P, NMAX, 16>(data, n, M);
+  }
+  if (n == 8) return member_transform_n<NT, P, NMAX, 8>(data, n, M);
+  if (n == 4) return member_transform_n<NT, P, NMAX, 4>(data, n, M);
+  if (n == 2) return member_transform_n<NT, P, NMAX, 2>(data, n, M);
+  if (n == 1) return member_transform_n<NT, P, NMAX, 1>(data, n, M);
+  member_transform_n<NT, P, NMAX, 0>(data, n, M);
+}
+
+// In-place transform along one spatial axis of an LDS array viewed as
+// (nouter, B, INNER): each lane owns whole lines, reads the B inputs of a line
+// into registers and then writes its B outputs (lines are disjoint).
+template <int NT, int B, int INNER>
+__device__ __forceinline__ void line_transform(float *__restrict__ data, int nouter,
+                                               const float *__restrict__ M) {
+  float m[B * B];
+#pragma unroll
+  for (int i = 0; i < B * B; ++i) m[i] = M[i];
+  const int nl = nouter * INNER;
+  for (int l = threadIdx.x; l < nl; l += NT) {
+    const int o = l / INNER, in_i = l - o * INNER;
+    float *base = data + o * B * INNER + in_i;
+    float x[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) x[k] = base[k * INNER];
+#pragma unroll
+    for (int r = 0; r < B; ++r) {
+      float acc = 0.0f;
+#pragma unroll
+      for (int k = 0; k < B; ++k) acc = fmaf(m[r * B + k], x[k], acc);
+      base[r * INNER] = acc;
+    }
+  }
+  __syncthreads();
+}
+
+// separable transform of a group (n, B0, B1, B2) in LDS: member axis, then the
+// spatial axes in order (an axis of length 1 has the matrix [1] and is skipped)
+template <int NT, int B0, int B1, int B2, int NMAX>
+__device__ __forceinline__ void group_transform(float *__restrict__ data, int n,
+                                                const float *__restrict__ G,
+                                                const float (*__restrict__ S)[64]) {
+  member_transform<NT, B0 * B1 * B2, NMAX>(data, n, G);
+  if constexpr (B0 > 1) line_transform<NT, B0, B1 * B2>(data, n, S[0]);
+  if constexpr (B1 > 1) line_transform<NT, B1, B2>(data, n * B0, S[1]);
+  if constexpr (B2 > 1) line_transform<NT, B2, 1>(data, n * B0 * B1, S[2]);
 }
 
 template <int B0, int B1, int B2, int R0, int R1, int R2, int NMAX, int NT, bool WIENER>
@@ -728,12 +798,15 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
   constexpr int P = B0 * B1 * B2;
   constexpr int NP = NMAX * P;
   static_assert(NP <= 4096, "group too large");
-  __shared__ float s_x[NP];      // coefficients being filtered
-  __shared__ float s_t[NP];      // transform temporary
-  __shared__ float s_sig[NP];    // noise std
-  __shared__ float s_r[WIENER ? NP : 1];
-  __shared__ float s_g[NMAX * NMAX], s_gi[NMAX * NMAX], s_g2[NMAX * NMAX];
+  static_assert(B0 * B0 <= 64 && B1 * B1 <= 64 && B2 * B2 <= 64, "block too large");
+  __shared__ float s_x[NP];    // coefficients being filtered
+  __shared__ float s_sig[NP];  // noise std of every coefficient
+  __shared__ float s_r[NP];    // Wiener: pilot coefficients, then the gain; HT: keep mask
+  // member matrices, rows with stride NMAX: G, G squared elementwise, G^-1
+  __shared__ __align__(16) float s_g[NMAX * NMAX], s_g2[NMAX * NMAX], s_gi[NMAX * NMAX];
   __shared__ float s_sf[3][64], s_si[3][64], s_sf2[3][64];
+  constexpr int KEX = NMAX < 4 ? NMAX : 4;  // exact planes staged in LDS
+  __shared__ float s_ex[KEX * P];
   __shared__ int s_pos[NMAX][3];
   __shared__ double s_plane[NMAX];
 
@@ -744,106 +817,148 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
   if (n <= 0) return;
   const int planes = min(a.k, n);
 
-  // ---- matrices ----------------------------------------------------------
+  // ---- set-up: member matrices, spatial matrices, member positions and the
+  // exact planes; every load is issued before the first wait ---------------
   const int64_t goff = (int64_t)(n - 1) * n * (2 * n - 1) / 6;  // sum_{m<n} m^2
   const int64_t gtot = (int64_t)a.max_m * (a.max_m + 1) * (2 * a.max_m + 1) / 6;
   const float *gf = a.mats + goff;
   const float *gi = a.mats + gtot + goff;
   const float *sp = a.mats + 2 * gtot;
-  for (int i = tid; i < n * n; i += NT) {
-    float v = gf[i];
-    s_g[i] = v;
-    s_g2[i] = v * v;
-    s_gi[i] = gi[i];
-  }
+  const float *exact = a.exact_std + (g * a.kcap) * P;
+  const int nn = n * n;
+  const int nex = min(planes, KEX) * P;
   {
-    const int bs[3] = {B0, B1, B2};
-    int o = 0;
-    for (int d = 0; d < 3; ++d) {
-      for (int i = tid; i < bs[d] * bs[d]; i += NT) {
-        float v = sp[o + i];
-        s_sf[d][i] = v;
-        s_sf2[d][i] = v * v;
+    constexpr int IG = (NMAX * NMAX + NT - 1) / NT;
+    constexpr int SF = B0 * B0 + B1 * B1 + B2 * B2;  // floats of the analysis matrices
+    constexpr int IS = (2 * SF + NT - 1) / NT;
+    constexpr int IE = (KEX * P + NT - 1) / NT;
+    float vg[IG], vi[IG], vs[IS], ve[IE];
+#pragma unroll
+    for (int it = 0; it < IG; ++it) {
+      const int i = tid + it * NT;
+      vg[it] = i < nn ? gf[i] : 0.0f;
+      vi[it] = i < nn ? gi[i] : 0.0f;
+    }
+#pragma unroll
+    for (int it = 0; it < IS; ++it) {
+      const int j = tid + it * NT;
+      vs[it] = j < 2 * SF ? sp[j] : 0.0f;
+    }
+#pragma unroll
+    for (int it = 0; it < IE; ++it) {
+      const int c = tid + it * NT;
+      ve[it] = c < nex ? exact[c] : 0.0f;
+    }
+    const int lin = tid < n ? a.sel[g * a.max_m + tid] : 0;
+#pragma unroll
+    for (int it = 0; it < IG; ++it) {
+      const int i = tid + it * NT;
+      if (i < nn) {
+        const int r = i / n, kk = i - r * n;
+        s_g[r * NMAX + kk] = vg[it];
+        s_g2[r * NMAX + kk] = vg[it] * vg[it];
+        s_gi[r * NMAX + kk] = vi[it];
       }
-      o += bs[d] * bs[d];
     }
-    for (int d = 0; d < 3; ++d) {
-      for (int i = tid; i < bs[d] * bs[d]; i += NT) s_si[d][i] = sp[o + i];
-      o += bs[d] * bs[d];
+#pragma unroll
+    for (int it = 0; it < IS; ++it) {
+      const int j = tid + it * NT;
+      if (j < 2 * SF) {
+        const bool inv = j >= SF;
+        const int jj = inv ? j - SF : j;
+        const int d = jj < B0 * B0 ? 0 : (jj < B0 * B0 + B1 * B1 ? 1 : 2);
+        const int e = jj - (d > 0 ? B0 * B0 : 0) - (d > 1 ? B1 * B1 : 0);
+        if (inv) {
+          s_si[d][e] = vs[it];
+        } else {
+          s_sf[d][e] = vs[it];
+          s_sf2[d][e] = vs[it] * vs[it];
+        }
+      }
     }
-  }
-  // ---- member positions ---------------------------------------------------
-  if (tid < n) {
-    int lin = a.sel[g * a.max_m + tid];
-    s_pos[tid][2] = lin % a.c2;
-    s_pos[tid][1] = (lin / a.c2) % a.c1;
-    s_pos[tid][0] = lin / (a.c2 * a.c1);
+#pragma unroll
+    for (int it = 0; it < IE; ++it) {
+      const int c = tid + it * NT;
+      if (c < nex) s_ex[c] = ve[it];
+    }
+    if (tid < n) {
+      s_pos[tid][2] = lin % a.c2;
+      s_pos[tid][1] = (lin / a.c2) % a.c1;
+      s_pos[tid][0] = lin / (a.c2 * a.c1);
+    }
   }
   __syncthreads();
   const int np = n * P;
-  // ---- gather: variance patches (approximate planes) ------------------------
-  for (int e = tid; e < np; e += NT) {
-    const int i = e / P, p = e - i * P;
-    const int p2 = p % B2, p1 = (p / B2) % B1, p0 = p / (B2 * B1);
-    const int64_t v = ((int64_t)(s_pos[i][0] + p0) * a.n1 + (s_pos[i][1] + p1)) * a.n2 +
-                      (s_pos[i][2] + p2);
-    float x = a.var[v];
-    s_x[e] = x > 0.0f ? x : 0.0f;
+  // ---- gather: variance, observation and (Wiener) pilot patches, all loads
+  // in flight at once -------------------------------------------------------
+  {
+    constexpr int IT = (NP + NT - 1) / NT;
+    float xv[IT], xn[IT], xp[WIENER ? IT : 1];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int e = tid + it * NT;
+      if (e < np) {
+        const int i = e / P, p = e - i * P;
+        const int p2 = p % B2, p1 = (p / B2) % B1, p0 = p / (B2 * B1);
+        const int64_t v = ((int64_t)(s_pos[i][0] + p0) * a.n1 + (s_pos[i][1] + p1)) * a.n2 +
+                          (s_pos[i][2] + p2);
+        xv[it] = a.var[v];
+        xn[it] = a.noisy[v];
+        if constexpr (WIENER) xp[it] = a.pilot[v];
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int e = tid + it * NT;
+      if (e < np) {
+        s_sig[e] = xv[it] > 0.0f ? xv[it] : 0.0f;
+        s_x[e] = xn[it];
+        if constexpr (WIENER) s_r[e] = xp[it];
+      }
+    }
   }
   __syncthreads();
-  // approximate std: squared transforms along every axis (variance.py:613-623)
-  apply_axis<NT>(s_x, s_t, n, B0, B1, B2, 0, s_g2);
-  __syncthreads();
-  const float *cur = s_t;
-  float *nxt = s_x;
-  if (B0 > 1) {
-    apply_axis<NT>(cur, nxt, n, B0, B1, B2, 1, s_sf2[0]);
-    __syncthreads();
-    float *tmp = (float *)cur;
-    cur = nxt;
-    nxt = tmp;
-  }
-  apply_axis<NT>(cur, nxt, n, B0, B1, B2, 2, s_sf2[1]);
-  __syncthreads();
-  apply_axis<NT>(nxt, (float *)cur, n, B0, B1, B2, 3, s_sf2[2]);
-  __syncthreads();
+  // ---- approximate std: squared transforms along every axis (variance.py:613-623)
+  group_transform<NT, B0, B1, B2, NMAX>(s_sig, n, s_g2, s_sf2);
   for (int e = tid; e < np; e += NT) {
-    float v = cur[e];
+    const float v = s_sig[e];
     s_sig[e] = sqrtf(v > 0.0f ? v : 0.0f);
   }
   __syncthreads();
 
-  // ---- exact overlap-aware planes, computed by k_exact_std ------------------
-  float *exact = s_t;  // planes*P values
-  for (int c = tid; c < planes * P; c += NT) exact[c] = a.exact_std[(g * a.kcap) * P + c];
-  __syncthreads();
-  // plane limiting (variance.py:1572-1631), float32 sequential plane sums
+  // ---- exact overlap-aware planes (k_exact_std) and plane limiting ----------
+  // the first KEX planes are in LDS; further ones are read from the buffer
+  auto exv = [&](int q, int s) { return q < KEX ? s_ex[q * P + s] : exact[q * P + s]; };
   if (planes == 0) {
     // k = 0: the approximate std is the answer (variance.py:1647-1653)
   } else if (planes == n) {
     for (int e = tid; e < np; e += NT) {
-      s_sig[e] = exact[e];  // every plane exact: the std itself (variance.py:1698)
+      const int q = e / P;
+      s_sig[e] = exv(q, e - q * P);  // every plane exact: the std itself (variance.py:1698)
     }
   } else {
+    // variance.py:1572-1631, float32 sequential plane sums
     for (int s = tid; s < P; s += NT) {
       float tot_a = 0.0f, tot_e = 0.0f, rem_a = 0.0f;
+#pragma unroll 8
       for (int q = 0; q < n; ++q) {
         const float av = s_sig[q * P + s];
         tot_a = tot_a + av * av;
         if (q >= planes) rem_a = rem_a + av * av;
       }
       for (int q = 0; q < planes; ++q) {
-        const float ev = exact[q * P + s];
+        const float ev = exv(q, s);
         tot_e = tot_e + ev * ev;
       }
       float target = tot_a - tot_e;
       if (target < 0.0f) target = 0.0f;
       const float scale = rem_a > 0.0f ? target / rem_a : 0.0f;
       const float fallback = target / (float)(n - planes);
+#pragma unroll 8
       for (int q = 0; q < n; ++q) {
         float v;
         if (q < planes) {
-          const float ev = exact[q * P + s];
+          const float ev = exv(q, s);
           v = ev * ev;
         } else {
           const float av = s_sig[q * P + s];
@@ -857,33 +972,9 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
   __syncthreads();
 
   // ---- coefficients ----------------------------------------------------------
-  auto forward = [&](const float *__restrict__ vol, float *buf) {
-    for (int e = tid; e < np; e += NT) {
-      const int i = e / P, p = e - i * P;
-      const int p2 = p % B2, p1 = (p / B2) % B1, p0 = p / (B2 * B1);
-      const int64_t v = ((int64_t)(s_pos[i][0] + p0) * a.n1 + (s_pos[i][1] + p1)) * a.n2 +
-                        (s_pos[i][2] + p2);
-      buf[e] = vol[v];
-    }
-    __syncthreads();
-    apply_axis<NT>(buf, s_t, n, B0, B1, B2, 0, s_g);
-    __syncthreads();
-    if (B0 > 1) {
-      apply_axis<NT>(s_t, buf, n, B0, B1, B2, 1, s_sf[0]);
-      __syncthreads();
-      apply_axis<NT>(buf, s_t, n, B0, B1, B2, 2, s_sf[1]);
-    } else {
-      apply_axis<NT>(s_t, buf, n, B0, B1, B2, 2, s_sf[1]);
-      __syncthreads();
-      for (int e = tid; e < np; e += NT) s_t[e] = buf[e];
-    }
-    __syncthreads();
-    apply_axis<NT>(s_t, buf, n, B0, B1, B2, 3, s_sf[2]);
-    __syncthreads();
-  };
-  float *atten = s_t;  // reused after the transforms
-  forward(a.noisy, s_x);
-  if (WIENER) forward(a.pilot, s_r);
+  group_transform<NT, B0, B1, B2, NMAX>(s_x, n, s_g, s_sf);
+  if (WIENER) group_transform<NT, B0, B1, B2, NMAX>(s_r, n, s_g, s_sf);
+  float *atten = s_r;
   if (!WIENER) {
     // Poisson hard threshold (poisson.py:30-46, bmndalgo.py:460-485)
     const float lam = a.shrink;
@@ -936,29 +1027,15 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
   if (tid < n) {
     double risk = 0.0;
     for (int j = 0; j < n; ++j) {
-      const double m = (double)s_gi[tid * n + j];
+      const double m = (double)s_gi[tid * NMAX + j];
       risk = risk + (m * m) * s_plane[j];
     }
     float w = 0.0f;
     if (isfinite(risk) && risk >= 0.0) w = (float)(1.0 / (risk + 1e-10));
     a.weights[g * a.max_m + tid] = w;
   }
-  __syncthreads();
   // ---- inverse transform and store -------------------------------------------
-  apply_axis<NT>(s_x, s_t, n, B0, B1, B2, 0, s_gi);
-  __syncthreads();
-  if (B0 > 1) {
-    apply_axis<NT>(s_t, s_x, n, B0, B1, B2, 1, s_si[0]);
-    __syncthreads();
-    apply_axis<NT>(s_x, s_t, n, B0, B1, B2, 2, s_si[1]);
-  } else {
-    apply_axis<NT>(s_t, s_x, n, B0, B1, B2, 2, s_si[1]);
-    __syncthreads();
-    for (int e = tid; e < np; e += NT) s_t[e] = s_x[e];
-  }
-  __syncthreads();
-  apply_axis<NT>(s_t, s_x, n, B0, B1, B2, 3, s_si[2]);
-  __syncthreads();
+  group_transform<NT, B0, B1, B2, NMAX>(s_x, n, s_gi, s_si);
   float *dst = a.filt + (g * a.max_m) * (int64_t)P;
   for (int e = tid; e < np; e += NT) dst[e] = s_x[e];
 }
