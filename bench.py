@@ -33,7 +33,7 @@ PEAK_HBM_GBS = 8000.0
 FLOPS_PER_TERM = 10
 # workgroup size of the block-matching kernel per deviance mode (its last
 # template argument, part of the kernel name rocprofv3 reports)
-BM_THREADS = {0: 512, 1: 1024, 2: 512}
+BM_THREADS = {0: 512, 1: 1024, 2: 512, 3: 1024}
 
 
 def parse() -> argparse.Namespace:
@@ -48,11 +48,13 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB mebibytes (for scaling "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE readings) and exit")
-    ap.add_argument("--deviance", default="auto-one-log",
-                    choices=["auto", "exact", "one-log", "auto-one-log"],
+    ap.add_argument("--deviance", default="auto-certified",
+                    choices=["auto", "exact", "one-log", "auto-one-log", "certified",
+                             "auto-certified"],
                     help="block-matching deviance evaluation (integer counts use the "
-                         "term table; other inputs the one-log form, or with 'auto' the "
-                         "reference's two-log form)")
+                         "term table; other inputs the certified one-log form, whose keys "
+                         "equal the reference expression's, or with 'auto' the reference's "
+                         "two-log form)")
     return ap.parse_args()
 
 

@@ -142,6 +142,24 @@ __device__ __forceinline__ double dlog_bits(double x, const double2 *__restrict_
   return fma((double)e, ln2, t.y) + l1p;
 }
 
+// Degree-4 float64 log used by the certified block-matching mode:
+// x = m 2^e with m in [0.5, 1) from the hardware frexp, j = top 7 mantissa
+// bits, c_j = 1 + (j + 1/2)/128, table entry {2/c_j, log c_j - ln2} (host,
+// extended precision):  log x = e ln2 + (log c_j - ln2) + log1p(r) with
+// r = 2m/c_j - 1, |r| <= 2^-8, and log1p(r) ~ r - r^2/2 + r^3/3 - r^4/4, whose
+// truncation error is below |r|^5/5 / (1 - |r|) < 1.9e-13.  Returns a finite
+// value for x = 0.
+__device__ __forceinline__ double log_deg4(double x, const double2 *__restrict__ tab, double ln2) {
+  const double m = __builtin_amdgcn_frexp_mant(x);
+  const int e = __builtin_amdgcn_frexp_exp(x);
+  const unsigned hi = (unsigned)((unsigned long long)__double_as_longlong(m) >> 32);
+  const double2 t = tab[(hi >> 13) & 127u];
+  const double r = fma(m, t.x, -1.0);
+  double q = fma(r, -0.25, 1.0 / 3.0);
+  q = fma(q, r, -0.5);
+  return fma((double)e, ln2, t.y) + fma(r * r, q, r);
+}
+
 // tie grid of blockmatching.py:957-964: q = 32*eps32*2^floor(log2(max(|d|,1)))
 __device__ __forceinline__ float canonical_key(float d) {
   double mag = fabs((double)d);
@@ -236,7 +254,8 @@ struct BMArgs {
   int max_m, min_m, pow2, stabilize;
   double inv_scale;
   const double *logtab;
-  const double2 *fastlog;    // MODE 1: {1/c_j, log c_j}, j < kFastLogN
+  const double2 *fastlog;    // MODES 1, 3: {1/c_j, log c_j}, j < kFastLogN
+  const double2 *certlog;    // MODE 3: {2/c_j, log c_j - ln2}, j < kFastLogN
   const double *term_table;  // MODE 2: (tmax+1)^2 terms, row = reference count
   int tmax;
   int32_t *sel, *counts, *accepted;
@@ -258,7 +277,8 @@ __global__ __launch_bounds__(NT) void k_blockmatch(BMArgs a) {
   constexpr int SLAB = S0 * S1 * S2;
   constexpr int BOX = (2 * R0 + 1) * (2 * R1 + 1) * (2 * R2 + 1);
   constexpr int P = B0 * B1 * B2;
-  __shared__ float s_slab[SLAB];
+  constexpr bool CERT = MODE == 3;
+  __shared__ float s_slab[CERT ? 1 : SLAB];
   __shared__ double s_ref[P];
   __shared__ double s_ref_log[P];
   __shared__ unsigned long long s_keys[BOX];
@@ -270,7 +290,13 @@ __global__ __launch_bounds__(NT) void k_blockmatch(BMArgs a) {
   __shared__ int s_krem, s_cnt, s_acc;
   __shared__ int s_refrow[P];
   __shared__ double s_lc[MODE == 1 ? SLAB : 1];  // MODE 1: c*log(2c) per slab voxel
-  __shared__ double2 s_flog[MODE == 1 ? kFastLogN : 1];
+  __shared__ double2 s_flog[(MODE == 1 || CERT) ? kFastLogN : 1];
+  // MODE 3: per slab voxel (max(c, 0), c log 2c); degree-4 log table; the sum
+  // of r log 2r over the reference; the distance error bound; value maxima
+  __shared__ double2 s_cl[CERT ? SLAB : 1];
+  __shared__ double2 s_clog[CERT ? kFastLogN : 1];
+  __shared__ double s_rsum, s_err;
+  __shared__ unsigned s_vmax[2];
   extern __shared__ double s_terms[];  // MODE 2 only
 
   const int tid = threadIdx.x;
@@ -285,22 +311,38 @@ __global__ __launch_bounds__(NT) void k_blockmatch(BMArgs a) {
   const int box = e0 * e1 * e2;
 
   for (int i = tid; i < kLogTabDoubles; i += NT) s_tab[i] = a.logtab[i];
+  if (CERT && tid < 2) s_vmax[tid] = 0u;
+  if (CERT) __syncthreads();
   // slab: volume region [lo, hi + B - 1] (always inside the volume)
   const int x0 = e0 + B0 - 1, x1 = e1 + B1 - 1, x2 = e2 + B2 - 1;
+  float vmax = 0.0f;
   for (int i = tid; i < SLAB; i += NT) {
     int i2 = i % S2, i1 = (i / S2) % S1, i0 = i / (S2 * S1);
     float v = 0.0f;
     if (i0 < x0 && i1 < x1 && i2 < x2)
       v = a.vol[((int64_t)(lo0 + i0) * a.n1 + (lo1 + i1)) * a.n2 + (lo2 + i2)];
-    s_slab[i] = v;
+    if (CERT) {
+      const float cf = v > 0.0f ? v : 0.0f;
+      vmax = fmaxf(vmax, cf);
+      s_cl[i].x = (double)cf;
+    } else {
+      s_slab[i] = v;
+    }
   }
+  float rmax = 0.0f;
   for (int p = tid; p < P; p += NT) {
     int p2 = p % B2, p1 = (p / B2) % B1, p0 = p / (B2 * B1);
     float v = a.vol[((int64_t)(o0 + p0) * a.n1 + (o1 + p1)) * a.n2 + (o2 + p2)];
     double r = (double)v * a.inv_scale;
     if (r < 0.0) r = 0.0;
     s_ref[p] = r;
+    rmax = fmaxf(rmax, v > 0.0f ? v : 0.0f);
     if (MODE == 2) s_refrow[p] = (int)v * (a.tmax + 1);
+  }
+  if (CERT) {
+    // non-negative floats order like their bit patterns
+    if (vmax > 0.0f) atomicMax(&s_vmax[1], __float_as_uint(vmax));
+    if (rmax > 0.0f) atomicMax(&s_vmax[0], __float_as_uint(rmax));
   }
   if (MODE == 2) {
     const int tn = (a.tmax + 1) * (a.tmax + 1);
@@ -308,10 +350,13 @@ __global__ __launch_bounds__(NT) void k_blockmatch(BMArgs a) {
   }
   if (tid == 0) s_acc = 0;
   __syncthreads();
-  if (MODE == 1) {
+  if (MODE == 1 || CERT) {
     // x log 2x of every slab voxel and reference voxel (0 at x = 0); the slab
-    // keeps max(x, 0) (count_scale is 1 in this mode)
-    for (int i = tid; i < kFastLogN; i += NT) s_flog[i] = a.fastlog[i];
+    // keeps max(x, 0) (count_scale is 1 in these modes)
+    for (int i = tid; i < kFastLogN; i += NT) {
+      s_flog[i] = a.fastlog[i];
+      if (CERT) s_clog[i] = a.certlog[i];
+    }
     __syncthreads();
     const double ln2 = s_tab[3 * kLogTabN] + s_tab[3 * kLogTabN + 1];
     for (int p = tid; p < P; p += NT) {
@@ -319,10 +364,31 @@ __global__ __launch_bounds__(NT) void k_blockmatch(BMArgs a) {
       s_ref_log[p] = r > 0.0 ? r * dlog_bits(2.0 * r, s_flog, ln2) : 0.0;
     }
     for (int i = tid; i < SLAB; i += NT) {
-      const float cf = s_slab[i] > 0.0f ? s_slab[i] : 0.0f;
-      const double c = (double)cf;
-      s_lc[i] = cf > 0.0f ? c * dlog_bits(2.0 * c, s_flog, ln2) : 0.0;
-      s_slab[i] = cf;
+      if (CERT) {
+        const double c = s_cl[i].x;
+        s_cl[i].y = c > 0.0 ? c * dlog_bits(2.0 * c, s_flog, ln2) : 0.0;
+      } else {
+        const float cf = s_slab[i] > 0.0f ? s_slab[i] : 0.0f;
+        const double c = (double)cf;
+        s_lc[i] = cf > 0.0f ? c * dlog_bits(2.0 * c, s_flog, ln2) : 0.0;
+        s_slab[i] = cf;
+      }
+    }
+    __syncthreads();
+    if (CERT && tid == 0) {
+      double rs = 0.0;
+      for (int p = 0; p < P; ++p) rs = rs + s_ref_log[p];
+      s_rsum = rs;
+      // Bound on |d_fast - d_ref| for every candidate of this reference, with
+      // S >= r + c for every term and W >= s (|log s| + 1) + 2 for s <= S:
+      // each path's rounding error is at most a few hundred units of 2^-53
+      // times P*W (per-term log and product errors plus the (P-1)-fold
+      // accumulation bound), and the degree-4 log adds s*1.9e-13 per term;
+      // the constants below carry a margin of about 7x and 2x on those.
+      const double S = (double)__uint_as_float(s_vmax[0]) + (double)__uint_as_float(s_vmax[1]);
+      const double lS = S > 0.0 ? fabs(dlog(S, s_tab)) : 0.0;
+      const double W = S * (lS + 1.0) + 2.0;
+      s_err = 2.0 * (double)P * (S * 4e-13 + 4096.0 * 0x1p-53 * W);
     }
     __syncthreads();
   }
@@ -330,65 +396,101 @@ __global__ __launch_bounds__(NT) void k_blockmatch(BMArgs a) {
   const double ln2k = s_tab[3 * kLogTabN] + s_tab[3 * kLogTabN + 1];
   const int rel0 = o0 - lo0, rel1 = o1 - lo1, rel2 = o2 - lo2;
   const int center_local = (rel0 * e1 + rel1) * e2 + rel2;
-  int my_acc = 0;
-  for (int cand = tid; cand < box; cand += NT) {
-    const int j2 = cand % e2, j1 = (cand / e2) % e1, j0 = cand / (e2 * e1);
-    const int base = (j0 * S1 + j1) * S2 + j2;
-    double d = 0.0;
-    // terms are accumulated in patch C order (p0, p1, p2), as the reference does
-#pragma unroll 1
-    for (int p01 = 0; p01 < B0 * B1; ++p01) {
-      const int p0 = p01 / B1, p1 = p01 - p0 * B1;
-      const float *row = s_slab + base + (p0 * S1 + p1) * S2;
-      const double *lcrow = s_lc + (MODE == 1 ? base + (p0 * S1 + p1) * S2 : 0);
-      const double *rrow = s_ref + p01 * B2;
-      const double *lrow = s_ref_log + p01 * B2;
-      const int *trow = s_refrow + p01 * B2;
-#pragma unroll
-      for (int p2 = 0; p2 < B2; ++p2) {
-        if (MODE == 2) {
-          // table entry = 2*(r log(2r/s) + c log(2c/s)), 0 when r = c = 0
-          d = d + s_terms[trow[p2] + (int)row[p2]];
-          continue;
-        }
-        if (MODE == 1) {
-          // the term is exactly 0 at r = c = 0 (the log of 0 stays finite)
-          const double s = rrow[p2] + (double)row[p2];
-          const double ls = dlog_bits(s, s_flog, ln2k);
-          d = d + fma(-s, ls, lrow[p2] + lcrow[p2]);
-          continue;
-        }
-        double c = (double)row[p2] * a.inv_scale;
-        if (c < 0.0) c = 0.0;
-        const double r = rrow[p2];
-        const double s = r + c;
-        double t;
-        if (MODE == 0) {
-          // blockmatching.py:347-355, term by term
-          const double lr = dlog((2.0 * r) / s, s_tab);
-          const double lc = dlog((2.0 * c) / s, s_tab);
-          t = 0.0;
-          if (r > 0.0) t = t + r * lr;
-          t = (c > 0.0) ? t + c * lc : t;
-        } else {
-          // r log(2r/s) + c log(2c/s) = r log 2r + c log 2c - s log s
-          const double ls = dlog_fast(s, s_tab);
-          t = fma(-s, ls, lrow[p2] + lcrow[p2]);
-        }
-        d = (s > 0.0) ? d + 2.0 * t : d;
-      }
-    }
-    if (MODE == 1) d = 2.0 * d;  // terms were accumulated without the factor 2
+  // distance -> (tie-grid key, accepted); non-decreasing in d (blockmatching.py:957-1042)
+  auto make_key = [&](double d, bool &ok) {
     if (d < 0.0) d = 0.0;
     const float d32 = (float)(d / (double)P);
     float key;
-    bool ok;
     if (a.stabilize) {
       key = canonical_key(d32);
       ok = key < a.thr_key;
     } else {
       key = d32;
       ok = d32 < a.threshold;
+    }
+    return key;
+  };
+  // the reference expression term by term (blockmatching.py:347-355); the
+  // observation c is max(x, 0) of the slab (unit count scale in MODE 3)
+  auto reference_term = [&](double r, double c, double d) {
+    const double s = r + c;
+    const double lr = dlog((2.0 * r) / s, s_tab);
+    const double lc = dlog((2.0 * c) / s, s_tab);
+    double t = 0.0;
+    if (r > 0.0) t = t + r * lr;
+    t = (c > 0.0) ? t + c * lc : t;
+    return (s > 0.0) ? d + 2.0 * t : d;
+  };
+  int my_acc = 0;
+  for (int cand = tid; cand < box; cand += NT) {
+    const int j2 = cand % e2, j1 = (cand / e2) % e1, j0 = cand / (e2 * e1);
+    const int base = (j0 * S1 + j1) * S2 + j2;
+    double d = 0.0;
+    float key;
+    bool ok;
+    if (CERT) {
+      // one-log distance with the degree-4 log, in any summation order
+      const double ln2c = ln2k;
+      double acc = 0.0;
+#pragma unroll 1
+      for (int p01 = 0; p01 < B0 * B1; ++p01) {
+        const int p0 = p01 / B1, p1 = p01 - p0 * B1;
+        const double2 *row = s_cl + base + (p0 * S1 + p1) * S2;
+        const double *rrow = s_ref + p01 * B2;
+#pragma unroll
+        for (int p2 = 0; p2 < B2; ++p2) {
+          const double2 cl = row[p2];
+          const double s = rrow[p2] + cl.x;
+          acc = acc + fma(-s, log_deg4(s, s_clog, ln2c), cl.y);
+        }
+      }
+      const double df = 2.0 * (s_rsum + acc);
+      bool ok_lo, ok_hi;
+      const float k_lo = make_key(df - s_err, ok_lo);
+      const float k_hi = make_key(df + s_err, ok_hi);
+      if (k_lo == k_hi) {
+        // the reference value lies in [df - err, df + err], where the key is constant
+        key = k_lo;
+        ok = ok_lo;
+      } else {
+        for (int p01 = 0; p01 < B0 * B1; ++p01) {
+          const int p0 = p01 / B1, p1 = p01 - p0 * B1;
+          const double2 *row = s_cl + base + (p0 * S1 + p1) * S2;
+          for (int p2 = 0; p2 < B2; ++p2) d = reference_term(s_ref[p01 * B2 + p2], row[p2].x, d);
+        }
+        key = make_key(d, ok);
+      }
+    } else {
+      // terms are accumulated in patch C order (p0, p1, p2), as the reference does
+#pragma unroll 1
+      for (int p01 = 0; p01 < B0 * B1; ++p01) {
+        const int p0 = p01 / B1, p1 = p01 - p0 * B1;
+        const float *row = s_slab + base + (p0 * S1 + p1) * S2;
+        const double *lcrow = s_lc + (MODE == 1 ? base + (p0 * S1 + p1) * S2 : 0);
+        const double *rrow = s_ref + p01 * B2;
+        const double *lrow = s_ref_log + p01 * B2;
+        const int *trow = s_refrow + p01 * B2;
+#pragma unroll
+        for (int p2 = 0; p2 < B2; ++p2) {
+          if (MODE == 2) {
+            // table entry = 2*(r log(2r/s) + c log(2c/s)), 0 when r = c = 0
+            d = d + s_terms[trow[p2] + (int)row[p2]];
+            continue;
+          }
+          if (MODE == 1) {
+            // the term is exactly 0 at r = c = 0 (the log of 0 stays finite)
+            const double s = rrow[p2] + (double)row[p2];
+            const double ls = dlog_bits(s, s_flog, ln2k);
+            d = d + fma(-s, ls, lrow[p2] + lcrow[p2]);
+            continue;
+          }
+          double c = (double)row[p2] * a.inv_scale;
+          if (c < 0.0) c = 0.0;
+          d = reference_term(rrow[p2], c, d);
+        }
+      }
+      if (MODE == 1) d = 2.0 * d;  // terms were accumulated without the factor 2
+      key = make_key(d, ok);
     }
     if (key == 0.0f) key = 0.0f;  // a signed zero sorts as +0
     my_acc += ok ? 1 : 0;
@@ -1289,27 +1391,32 @@ int term_table_for_device(int tmax, const double **out) {
   return 0;
 }
 
-std::vector<double *> g_fastlog;  // per device
+std::vector<double *> g_fastlog, g_certlog;  // per device
 
-int fastlog_for_device(const double2 **out) {
+// {1/c_j, log c_j} (certified == 0) or {2/c_j, log c_j - ln2} (certified != 0)
+// for c_j = 1 + (j + 1/2)/128; the logarithm is that of the stored reciprocal,
+// evaluated in extended precision
+int fastlog_for_device(const double2 **out, int certified = 0) {
   int dev = 0;
   HIP_TRY(hipGetDevice(&dev));
   std::lock_guard<std::mutex> lock(g_tab_mutex);
-  if ((int)g_fastlog.size() <= dev) g_fastlog.resize(dev + 1, nullptr);
-  if (!g_fastlog[dev]) {
+  std::vector<double *> &cache = certified ? g_certlog : g_fastlog;
+  if ((int)cache.size() <= dev) cache.resize(dev + 1, nullptr);
+  if (!cache[dev]) {
     std::vector<double> h(2 * kFastLogN);
     for (int j = 0; j < kFastLogN; ++j) {
       const long double c = 1.0L + ((long double)j + 0.5L) / 128.0L;
       const double invc = (double)(1.0L / c);
-      h[2 * j] = invc;
-      h[2 * j + 1] = (double)(-logl((long double)invc));  // log of 1/invc exactly
+      const long double lc = -logl((long double)invc);  // log of 1/invc exactly
+      h[2 * j] = certified ? 2.0 * invc : invc;
+      h[2 * j + 1] = certified ? (double)(lc - logl(2.0L)) : (double)lc;
     }
     double *d = nullptr;
     HIP_TRY(hipMalloc(&d, sizeof(double) * h.size()));
     HIP_TRY(hipMemcpy(d, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
-    g_fastlog[dev] = d;
+    cache[dev] = d;
   }
-  *out = reinterpret_cast<const double2 *>(g_fastlog[dev]);
+  *out = reinterpret_cast<const double2 *>(cache[dev]);
   return 0;
 }
 
@@ -1387,18 +1494,21 @@ int32_t bmnd_hip_blockmatch(const float *vol, const int64_t *shape_h, const int3
     return fail(BMND_HIP_EINVAL, "count_scale must be finite and positive");
   if (nrefs <= 0) return 0;
   if (nrefs > INT32_MAX) return fail(BMND_HIP_EINVAL, "too many references");
-  if (mode < 0 || mode > 2) return fail(BMND_HIP_EINVAL, "mode must be 0, 1 or 2");
+  if (mode < 0 || mode > 3) return fail(BMND_HIP_EINVAL, "mode must be 0, 1, 2 or 3");
   if (mode == 2 && (count_scale != 1.0 || table_max < 0 || table_max > kMaxTableCount))
     return fail(BMND_HIP_EINVAL, "table mode needs count_scale == 1 and 0 <= table_max <= 100");
   const double *tab = nullptr;
   if (int rc = logtab_for_device(&tab)) return rc;
-  if (mode == 1 && count_scale != 1.0) mode = 0;  // the one-log slab assumes unit scale
+  // the one-log slabs assume unit count scale
+  if ((mode == 1 || mode == 3) && count_scale != 1.0) mode = 0;
   const double *terms = nullptr;
   if (mode == 2)
     if (int rc = term_table_for_device(table_max, &terms)) return rc;
-  const double2 *flog = nullptr;
-  if (mode == 1)
+  const double2 *flog = nullptr, *clog = nullptr;
+  if (mode == 1 || mode == 3)
     if (int rc = fastlog_for_device(&flog)) return rc;
+  if (mode == 3)
+    if (int rc = fastlog_for_device(&clog, 1)) return rc;
   BMArgs a;
   a.vol = vol;
   a.n0 = (int)shape_h[0];
@@ -1418,6 +1528,7 @@ int32_t bmnd_hip_blockmatch(const float *vol, const int64_t *shape_h, const int3
   a.inv_scale = 1.0 / count_scale;
   a.logtab = tab;
   a.fastlog = flog;
+  a.certlog = clog;
   a.term_table = terms;
   a.tmax = table_max;
   a.sel = sel;
@@ -1433,6 +1544,9 @@ int32_t bmnd_hip_blockmatch(const float *vol, const int64_t *shape_h, const int3
                          a);                                                                     \
     else if (mode == 1)                                                                          \
       hipLaunchKernelGGL((k_blockmatch<b0, b1, b2, r0, r1, r2, 1, kBMThreadsOneLog>), grid,      \
+                         dim3(kBMThreadsOneLog), 0, s, a);                                       \
+    else if (mode == 3)                                                                          \
+      hipLaunchKernelGGL((k_blockmatch<b0, b1, b2, r0, r1, r2, 3, kBMThreadsOneLog>), grid,      \
                          dim3(kBMThreadsOneLog), 0, s, a);                                       \
     else                                                                                         \
       hipLaunchKernelGGL((k_blockmatch<b0, b1, b2, r0, r1, r2, 2, kBMThreads>), grid, blk, dyn,  \

@@ -220,10 +220,10 @@ class PoissonPipeline:
     """Two-stage Poisson denoiser on one device.  Buffers are torch tensors on
     `device`; kernels run on the current torch stream."""
 
-    DEVIANCE_MODES = ("auto", "exact", "one-log", "auto-one-log")
+    DEVIANCE_MODES = ("auto", "exact", "one-log", "auto-one-log", "certified", "auto-certified")
 
     def __init__(self, profile: BMNDProfile, shape: tuple[int, ...], device=None,
-                 deviance: str = "auto-one-log"):
+                 deviance: str = "auto-certified"):
         self.profile = profile
         self.shape = tuple(int(s) for s in shape)
         self.ndim = len(self.shape)
@@ -234,10 +234,14 @@ class PoissonPipeline:
         # Integer counts <= 100 always use the float64 term table (the reference
         # expression, evaluated on the host with the C library's log).  Other
         # inputs: "auto" evaluates the reference's two-log expression term by
-        # term; "auto-one-log" (default) uses r log 2r + c log 2c - s log s
-        # with one log per term, which can move a distance by ~1e-14 and so a
-        # key across the 32-ulp tie grid only in rare cases (none observed on
-        # the 256^3 benchmark volume).
+        # term; "auto-certified" (default) evaluates the one-log form
+        # r log 2r + c log 2c - s log s with a degree-4 log together with a
+        # bound on its distance from the reference expression, and re-evaluates
+        # the reference expression for the rare candidates whose tie-grid key
+        # the bound does not pin, so its keys equal those of "auto";
+        # "auto-one-log" uses the one-log form alone, which can move a
+        # distance by ~1e-14 and so a key across the 32-ulp tie grid in rare
+        # cases.
         self.deviance = deviance
         self.last_modes: dict[str, int] = {}
         self.count_scale = float(profile.poisson_count_scale)
@@ -269,7 +273,9 @@ class PoissonPipeline:
             return 0, 0
         if self.deviance == "one-log":
             return 1, 0
-        fallback = 0 if self.deviance == "auto" else 1
+        if self.deviance == "certified":
+            return 3, 0
+        fallback = {"auto": 0, "auto-one-log": 1, "auto-certified": 3}[self.deviance]
         if self.count_scale != 1.0:
             return fallback, 0
         flags = torch.empty(2, dtype=torch.int32, device=self.device)

@@ -1,0 +1,76 @@
+"""Block-matching modes through the C-ABI, against each other and the oracle.
+
+Float-valued inputs (a smoothed Poisson volume, like the Wiener stage's pilot)
+take the evaluated-deviance paths.  The certified mode (3) must return the
+same match lists and accepted counts as the reference expression (mode 0) and
+as the oracle.  With unstabilised keys (plain float32 distances, a grid much
+finer than the 32-ulp tie grid) a few candidates in a thousand are not pinned
+by the error bound and are re-evaluated with the reference expression, so the
+second case exercises that path."""
+
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from bmnd_amd import BM4DProfile, NoiseModel, _lib, phantoms
+from bmnd_amd.engine import make_plan
+
+pytestmark = pytest.mark.gpu
+
+
+def _pilot_like(shape, seed):
+    _, noisy = phantoms.ellipsoid_volume(shape, seed=seed, peak=10.0, n_ellipsoids=8,
+                                         axes_range=(4.0, 14.0))
+    v = noisy.astype(np.float64)
+    for ax in range(3):  # 3-tap box blur along every axis
+        v = (np.roll(v, 1, ax) + v + np.roll(v, -1, ax)) / 3.0
+    return np.ascontiguousarray(v.astype(np.float32))
+
+
+def _gpu_match(vol_t, shape, block, radius, refs_t, thr, max_m, min_m, stabilize, mode):
+    L = _lib.lib()
+    R = refs_t.numel()
+    dev = vol_t.device
+    sel = torch.empty(R * max_m, dtype=torch.int32, device=dev)
+    cnt = torch.empty(R, dtype=torch.int32, device=dev)
+    acc = torch.empty(R, dtype=torch.int32, device=dev)
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    _lib.check(L.bmnd_hip_blockmatch(
+        ctypes.c_void_p(vol_t.data_ptr()), (ctypes.c_int64 * 3)(*shape),
+        (ctypes.c_int32 * 3)(*block), (ctypes.c_int32 * 3)(*radius),
+        ctypes.c_void_p(refs_t.data_ptr()), R, float(thr), max_m, min_m, 1, int(stabilize),
+        1.0, mode, 0, ctypes.c_void_p(sel.data_ptr()), ctypes.c_void_p(cnt.data_ptr()),
+        ctypes.c_void_p(acc.data_ptr()), stream))
+    torch.cuda.synchronize()
+    return sel.view(R, max_m).cpu().numpy(), cnt.cpu().numpy(), acc.cpu().numpy()
+
+
+@pytest.mark.parametrize("stabilize", [1, 0])
+def test_certified_matches_reference_expression(stabilize):
+    shape = (34, 34, 34)
+    vol = _pilot_like(shape, seed=11)
+    plan = make_plan(BM4DProfile(noise_model=NoiseModel.POISSON), "wiener", 3)
+    block, radius = tuple(plan.block3), tuple(plan.radius3)
+    counts = [shape[d] - block[d] + 1 for d in range(3)]
+    grid = np.stack(np.meshgrid(*[np.arange(0, c, 2) for c in counts], indexing="ij"), -1)
+    refs = grid.reshape(-1, 3)
+    lin = (refs[:, 0] * counts[1] + refs[:, 1]) * counts[2] + refs[:, 2]
+    dev = torch.device("cuda", 0)
+    vol_t = torch.from_numpy(vol).to(dev)
+    refs_t = torch.from_numpy(lin.astype(np.int32)).to(dev)
+    print(f"case: stabilize={stabilize} refs={len(lin)} block={block} radius={radius}", flush=True)
+    args = (vol_t, shape, block, radius, refs_t, plan.threshold, plan.max_m, plan.min_m, stabilize)
+    s0, c0, a0 = _gpu_match(*args, mode=0)
+    s3, c3, a3 = _gpu_match(*args, mode=3)
+    o = oracle.blockmatch(refs, counts, radius, plan.max_m, plan.min_m, plan.threshold,
+                          volume=vol, block=block, stabilize=bool(stabilize))
+    np.testing.assert_array_equal(c3, c0)
+    np.testing.assert_array_equal(a3, a0)
+    np.testing.assert_array_equal(s3, s0)
+    np.testing.assert_array_equal(c3, o.counts)
+    np.testing.assert_array_equal(a3, o.accepted)
+    for r in range(len(lin)):
+        np.testing.assert_array_equal(s3[r, : c3[r]], o.selected[r, : o.counts[r]])
