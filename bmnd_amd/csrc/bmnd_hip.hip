@@ -287,7 +287,7 @@ __global__ __launch_bounds__(NT) void k_blockmatch(BMArgs a) {
   __shared__ unsigned long long s_small[kMaxStack];
   __shared__ unsigned long long s_sorted[kMaxStack];
   __shared__ unsigned long long s_prefix;
-  __shared__ int s_krem, s_cnt, s_acc;
+  __shared__ int s_krem, s_cnt, s_acc, s_done;
   __shared__ int s_refrow[P];
   __shared__ double s_lc[MODE == 1 ? SLAB : 1];  // MODE 1: c*log(2c) per slab voxel
   __shared__ double2 s_flog[(MODE == 1 || CERT) ? kFastLogN : 1];
@@ -316,17 +316,30 @@ __global__ __launch_bounds__(NT) void k_blockmatch(BMArgs a) {
   // slab: volume region [lo, hi + B - 1] (always inside the volume)
   const int x0 = e0 + B0 - 1, x1 = e1 + B1 - 1, x2 = e2 + B2 - 1;
   float vmax = 0.0f;
-  for (int i = tid; i < SLAB; i += NT) {
-    int i2 = i % S2, i1 = (i / S2) % S1, i0 = i / (S2 * S1);
-    float v = 0.0f;
-    if (i0 < x0 && i1 < x1 && i2 < x2)
-      v = a.vol[((int64_t)(lo0 + i0) * a.n1 + (lo1 + i1)) * a.n2 + (lo2 + i2)];
-    if (CERT) {
-      const float cf = v > 0.0f ? v : 0.0f;
-      vmax = fmaxf(vmax, cf);
-      s_cl[i].x = (double)cf;
-    } else {
-      s_slab[i] = v;
+  {
+    // every load in flight before the first store
+    constexpr int IT = (SLAB + NT - 1) / NT;
+    float vv[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int i = tid + it * NT;
+      const int i2 = i % S2, i1 = (i / S2) % S1, i0 = i / (S2 * S1);
+      vv[it] = 0.0f;
+      if (i < SLAB && i0 < x0 && i1 < x1 && i2 < x2)
+        vv[it] = a.vol[((int64_t)(lo0 + i0) * a.n1 + (lo1 + i1)) * a.n2 + (lo2 + i2)];
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int i = tid + it * NT;
+      if (i < SLAB) {
+        if (CERT) {
+          const float cf = vv[it] > 0.0f ? vv[it] : 0.0f;
+          vmax = fmaxf(vmax, cf);
+          s_cl[i].x = (double)cf;
+        } else {
+          s_slab[i] = vv[it];
+        }
+      }
     }
   }
   float rmax = 0.0f;
@@ -513,10 +526,18 @@ __global__ __launch_bounds__(NT) void k_blockmatch(BMArgs a) {
   else
     K = min(req, box);
 
-  // radix select of the K-th smallest composite key (keys are unique)
+  // Radix select of the K smallest composite keys (keys are unique).  The
+  // candidate index is below 2^16, so bits 16..31 are zero in every key and
+  // their passes are skipped; once the bin holding the K-th key is needed in
+  // full, every key whose resolved bits are <= the prefix is selected.
+  static_assert(BOX < 65536, "candidate index must fit 16 bits");
   unsigned long long prefix = 0ull, mask = 0ull;
   int krem = K;
   for (int shift = 56; shift >= 0; shift -= 8) {
+    if (shift == 24 || shift == 16) {
+      mask |= 255ull << shift;
+      continue;
+    }
     for (int i = tid; i < 256; i += NT) s_hist[i] = 0;
     __syncthreads();
     for (int i = tid; i < box; i += NT) {
@@ -540,6 +561,7 @@ __global__ __launch_bounds__(NT) void k_blockmatch(BMArgs a) {
         if (ex < krem && ex + cs[b] >= krem) {
           s_prefix = prefix | ((unsigned long long)(4 * tid + b) << shift);
           s_krem = krem - ex;
+          s_done = (ex + cs[b] == krem) ? 1 : 0;
         }
         ex += cs[b];
       }
@@ -548,13 +570,15 @@ __global__ __launch_bounds__(NT) void k_blockmatch(BMArgs a) {
     prefix = s_prefix;
     krem = s_krem;
     mask |= 255ull << shift;
+    const int done = s_done;
     __syncthreads();
+    if (done) break;
   }
   if (tid == 0) s_cnt = 0;
   __syncthreads();
   for (int i = tid; i < box; i += NT) {
     unsigned long long k = s_keys[i];
-    if (k <= prefix) s_small[atomicAdd(&s_cnt, 1)] = k;
+    if ((k & mask) <= prefix) s_small[atomicAdd(&s_cnt, 1)] = k;
   }
   __syncthreads();
   if (tid < K) {
@@ -564,36 +588,36 @@ __global__ __launch_bounds__(NT) void k_blockmatch(BMArgs a) {
     s_sorted[rank] = mine;
   }
   __syncthreads();
-  if (tid == 0) {
-    // reference first, dedupe, cap, power-of-two truncation (blockmatching.py:913-954)
-    int out[kMaxStack];
-    int cnt = 0;
-    out[cnt++] = center_local;
-    for (int i = 0; i < K && cnt < a.max_m; ++i) {
-      int loc = (int)(s_sorted[i] & 0xffffffffull);
-      if (loc == center_local) continue;
-      bool seen = false;
-      for (int j = 0; j < cnt; ++j) seen |= (out[j] == loc);
-      if (!seen) out[cnt++] = loc;
-    }
-    if (a.pow2 && cnt > 0) {
+  if (tid < 64) {
+    // reference first, then the sorted candidates without it (composites are
+    // unique, so the reference patch is the only possible duplicate), capped
+    // at max, then power-of-two truncation (blockmatching.py:913-954); one
+    // wave, lane i holding the i-th smallest composite
+    const int lane = tid;
+    const bool valid = lane < K;
+    const int loc = valid ? (int)(s_sorted[lane] & 0xffffffffull) : -1;
+    const bool keep = valid && loc != center_local;
+    const unsigned long long kept = __ballot(keep);
+    const int pos = 1 + __popcll(kept & ((1ull << lane) - 1ull));
+    int cnt = min(1 + __popcll(kept), a.max_m);
+    if (a.pow2) {
       int t = 1;
       while ((t << 1) <= cnt) t <<= 1;
       if (a.min_m > t) t = a.min_m;
       if (t < cnt) cnt = t;
     }
+    auto to_lin = [&](int l) {
+      const int l2 = l % e2, l1 = (l / e2) % e1, l0 = l / (e2 * e1);
+      return ((lo0 + l0) * a.c1 + (lo1 + l1)) * a.c2 + (lo2 + l2);
+    };
     int32_t *dst = a.sel + g * a.max_m;
-    for (int i = 0; i < a.max_m; ++i) {
-      int v = -1;
-      if (i < cnt) {
-        int loc = out[i];
-        int l2 = loc % e2, l1 = (loc / e2) % e1, l0 = loc / (e2 * e1);
-        v = ((lo0 + l0) * a.c1 + (lo1 + l1)) * a.c2 + (lo2 + l2);
-      }
-      dst[i] = v;
+    if (keep && pos < cnt) dst[pos] = to_lin(loc);
+    if (lane == 0) dst[0] = to_lin(center_local);
+    for (int j = cnt + lane; j < a.max_m; j += 64) dst[j] = -1;
+    if (lane == 0) {
+      a.counts[g] = cnt;
+      if (a.accepted) a.accepted[g] = acc;
     }
-    a.counts[g] = cnt;
-    if (a.accepted) a.accepted[g] = acc;
   }
 }
 
