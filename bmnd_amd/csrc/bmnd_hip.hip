@@ -774,14 +774,21 @@ __global__ __launch_bounds__(NT) void k_exact_std(ExactArgs a) {
     float cur_vv = 0.0f;
     unsigned prev_label = 0xffffffffu;
     for (int base = 0; base < np; base += 64) {
-      // each lane decodes one entry of the chunk; the walk below reads them
-      // back with readlane, so the serial path does no dependent LDS loads
+      // Each lane decodes one entry of the chunk: its union label, whether it
+      // opens a new union voxel (bit 31 of the code), the LDS offsets of its
+      // spatial factors and its member; the walk below reads them back with
+      // readlane, so the serial path does no dependent LDS loads and tests
+      // the voxel change on the scalar unit.
       const int e = base + lane;
       const unsigned key = e < np ? s_key[e] : 0xffffffffu;
       const unsigned label = key >> 12;
+      unsigned before = __shfl_up(label, 1, 64);
+      if (lane == 0) before = prev_label;
+      prev_label = (unsigned)__builtin_amdgcn_readlane((int)label, 63);
       const int ei = (int)((key >> 7) & 31u), ep = (int)(key & 127u);
       const int ep01 = ep / B2, ep2 = ep - ep01 * B2;
-      const int code = (ep01 * B01) | ((ep2 * B2) << 12) | (ei << 20);
+      const unsigned code = (unsigned)(ep01 * B01) | ((unsigned)(ep2 * B2) << 12) |
+                            ((unsigned)ei << 20) | (label != before ? 0x80000000u : 0u);
       float vv = 0.0f;
       if (e < np) {
         const int l2 = (int)(label % (unsigned)u2);
@@ -789,22 +796,20 @@ __global__ __launch_bounds__(NT) void k_exact_std(ExactArgs a) {
         const int l0 = (int)(label / (unsigned)(u2 * u1));
         vv = a.var[((int64_t)(l0 + m0) * a.n1 + (l1 + m1)) * a.n2 + (l2 + m2)];
       }
-      const int cnt = min(64, np - base);
+      const int cnt = __builtin_amdgcn_readfirstlane(min(64, np - base));
       for (int j = 0; j < cnt; ++j) {
-        const unsigned lab = (unsigned)__builtin_amdgcn_readlane((int)label, j);
-        if (lab != prev_label) {
-          // close the previous union voxel (a no-op before the first one)
+        const unsigned cd = (unsigned)__builtin_amdgcn_readlane((int)code, j);
+        const float sv = s_t01[(cd & 4095u) + s01] * s_a2[((cd >> 12) & 255u) + s2];
+        const float4 gq = s_g4[(cd >> 20) & 31u];
+        if (cd & 0x80000000u) {
+          // close the previous union voxel (adds +0 before the first one)
           acc0 = acc0 + (cur_vv * c0) * c0;
           acc1 = acc1 + (cur_vv * c1) * c1;
           acc2 = acc2 + (cur_vv * c2) * c2;
           acc3 = acc3 + (cur_vv * c3) * c3;
-          c0 = c1 = c2 = c3 = 0.0f;
           cur_vv = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, vv), j));
-          prev_label = lab;
+          c0 = c1 = c2 = c3 = 0.0f;
         }
-        const int cd = __builtin_amdgcn_readlane(code, j);
-        const float sv = s_t01[(cd & 4095) + s01] * s_a2[((cd >> 12) & 255) + s2];
-        const float4 gq = s_g4[cd >> 20];
         c0 = c0 + gq.x * sv;
         c1 = c1 + gq.y * sv;
         c2 = c2 + gq.z * sv;
