@@ -263,260 +263,61 @@ struct BMArgs {
 
 constexpr int kBMThreads = 512;
 constexpr int kBMThreadsOneLog = 1024;  // one workgroup per CU fits its LDS
+constexpr int kBMThreadsCert = 512;     // two workgroups per CU
 constexpr int kMaxStack = 64;
 constexpr int kMaxTableCount = 100;
 
-// MODE 0: each term as the reference writes it (two divisions, two logs);
-// MODE 1: one log per term, r log 2r + c log 2c - s log s (values clamped
-//         and scaled when the slab is staged);
-// MODE 2: integer counts; the term is read from a float64 table built on the
-//         host with the same expression and the C library's log.
-template <int B0, int B1, int B2, int R0, int R1, int R2, int MODE, int NT>
-__global__ __launch_bounds__(NT) void k_blockmatch(BMArgs a) {
-  constexpr int S0 = 2 * R0 + B0, S1 = 2 * R1 + B1, S2 = 2 * R2 + B2;
-  constexpr int SLAB = S0 * S1 * S2;
-  constexpr int BOX = (2 * R0 + 1) * (2 * R1 + 1) * (2 * R2 + 1);
-  constexpr int P = B0 * B1 * B2;
-  constexpr bool CERT = MODE == 3;
-  __shared__ float s_slab[CERT ? 1 : SLAB];
-  __shared__ double s_ref[P];
-  __shared__ double s_ref_log[P];
-  __shared__ unsigned long long s_keys[BOX];
-  __shared__ double s_tab[kLogTabDoubles];
+// distance -> (tie-grid key, accepted), non-decreasing in d
+// (blockmatching.py:957-1042; the 32-ulp grid when stabilising)
+template <int P>
+__device__ __forceinline__ float bm_key(double d, const BMArgs &a, bool &ok) {
+  if (d < 0.0) d = 0.0;
+  const float d32 = (float)(d / (double)P);
+  float key;
+  if (a.stabilize) {
+    key = canonical_key(d32);
+    ok = key < a.thr_key;
+  } else {
+    key = d32;
+    ok = d32 < a.threshold;
+  }
+  return key;
+}
+
+// one term of the reference expression (blockmatching.py:347-355) added to d
+__device__ __forceinline__ double bm_reference_term(double r, double c, double d,
+                                                    const double *__restrict__ tab) {
+  const double s = r + c;
+  const double lr = dlog((2.0 * r) / s, tab);
+  const double lc = dlog((2.0 * c) / s, tab);
+  double t = 0.0;
+  if (r > 0.0) t = t + r * lr;
+  t = (c > 0.0) ? t + c * lc : t;
+  return (s > 0.0) ? d + 2.0 * t : d;
+}
+
+// composite sort key: accepted candidates first, then by key, then by index
+__device__ __forceinline__ unsigned long long bm_composite(float key, bool ok, int cand) {
+  if (key == 0.0f) key = 0.0f;  // a signed zero sorts as +0
+  return ((unsigned long long)(ok ? 0u : 1u) << 63) |
+         ((unsigned long long)__float_as_uint(key) << 32) | (unsigned long long)cand;
+}
+
+// Selection shared by the block-matching kernels: from the box composites
+// (unique) and the accepted count, keep the K smallest the reference keeps,
+// order them and write the group of reference g: reference first, dedupe,
+// cap, power-of-two truncation (blockmatching.py:913-954, 1045-1205).
+template <int NT, int BOX>
+__device__ __forceinline__ void bm_select_write(const unsigned long long *__restrict__ keys,
+                                                int box, int acc, const BMArgs &a, int64_t g,
+                                                int lo0, int lo1, int lo2, int e1, int e2,
+                                                int center_local) {
   __shared__ int s_hist[256];
   __shared__ unsigned long long s_small[kMaxStack];
   __shared__ unsigned long long s_sorted[kMaxStack];
   __shared__ unsigned long long s_prefix;
-  __shared__ int s_krem, s_cnt, s_acc, s_done;
-  __shared__ int s_refrow[P];
-  __shared__ double s_lc[MODE == 1 ? SLAB : 1];  // MODE 1: c*log(2c) per slab voxel
-  __shared__ double2 s_flog[(MODE == 1 || CERT) ? kFastLogN : 1];
-  // MODE 3: per slab voxel (max(c, 0), c log 2c); degree-4 log table; the sum
-  // of r log 2r over the reference; the distance error bound; value maxima
-  __shared__ double2 s_cl[CERT ? SLAB : 1];
-  __shared__ double2 s_clog[CERT ? kFastLogN : 1];
-  __shared__ double s_rsum, s_err;
-  __shared__ unsigned s_vmax[2];
-  extern __shared__ double s_terms[];  // MODE 2 only
-
+  __shared__ int s_krem, s_cnt, s_done;
   const int tid = threadIdx.x;
-  const int64_t g = blockIdx.x;
-  if (g >= a.nrefs) return;
-  const int lin = a.ref_lin[g];
-  const int o2 = lin % a.c2, o1 = (lin / a.c2) % a.c1, o0 = lin / (a.c2 * a.c1);
-  const int lo0 = max(o0 - R0, 0), hi0 = min(o0 + R0, a.c0 - 1);
-  const int lo1 = max(o1 - R1, 0), hi1 = min(o1 + R1, a.c1 - 1);
-  const int lo2 = max(o2 - R2, 0), hi2 = min(o2 + R2, a.c2 - 1);
-  const int e0 = hi0 - lo0 + 1, e1 = hi1 - lo1 + 1, e2 = hi2 - lo2 + 1;
-  const int box = e0 * e1 * e2;
-
-  for (int i = tid; i < kLogTabDoubles; i += NT) s_tab[i] = a.logtab[i];
-  if (CERT && tid < 2) s_vmax[tid] = 0u;
-  if (CERT) __syncthreads();
-  // slab: volume region [lo, hi + B - 1] (always inside the volume)
-  const int x0 = e0 + B0 - 1, x1 = e1 + B1 - 1, x2 = e2 + B2 - 1;
-  float vmax = 0.0f;
-  {
-    // every load in flight before the first store
-    constexpr int IT = (SLAB + NT - 1) / NT;
-    float vv[IT];
-#pragma unroll
-    for (int it = 0; it < IT; ++it) {
-      const int i = tid + it * NT;
-      const int i2 = i % S2, i1 = (i / S2) % S1, i0 = i / (S2 * S1);
-      vv[it] = 0.0f;
-      if (i < SLAB && i0 < x0 && i1 < x1 && i2 < x2)
-        vv[it] = a.vol[((int64_t)(lo0 + i0) * a.n1 + (lo1 + i1)) * a.n2 + (lo2 + i2)];
-    }
-#pragma unroll
-    for (int it = 0; it < IT; ++it) {
-      const int i = tid + it * NT;
-      if (i < SLAB) {
-        if (CERT) {
-          const float cf = vv[it] > 0.0f ? vv[it] : 0.0f;
-          vmax = fmaxf(vmax, cf);
-          s_cl[i].x = (double)cf;
-        } else {
-          s_slab[i] = vv[it];
-        }
-      }
-    }
-  }
-  float rmax = 0.0f;
-  for (int p = tid; p < P; p += NT) {
-    int p2 = p % B2, p1 = (p / B2) % B1, p0 = p / (B2 * B1);
-    float v = a.vol[((int64_t)(o0 + p0) * a.n1 + (o1 + p1)) * a.n2 + (o2 + p2)];
-    double r = (double)v * a.inv_scale;
-    if (r < 0.0) r = 0.0;
-    s_ref[p] = r;
-    rmax = fmaxf(rmax, v > 0.0f ? v : 0.0f);
-    if (MODE == 2) s_refrow[p] = (int)v * (a.tmax + 1);
-  }
-  if (CERT) {
-    // non-negative floats order like their bit patterns
-    if (vmax > 0.0f) atomicMax(&s_vmax[1], __float_as_uint(vmax));
-    if (rmax > 0.0f) atomicMax(&s_vmax[0], __float_as_uint(rmax));
-  }
-  if (MODE == 2) {
-    const int tn = (a.tmax + 1) * (a.tmax + 1);
-    for (int i = tid; i < tn; i += NT) s_terms[i] = a.term_table[i];
-  }
-  if (tid == 0) s_acc = 0;
-  __syncthreads();
-  if (MODE == 1 || CERT) {
-    // x log 2x of every slab voxel and reference voxel (0 at x = 0); the slab
-    // keeps max(x, 0) (count_scale is 1 in these modes)
-    for (int i = tid; i < kFastLogN; i += NT) {
-      s_flog[i] = a.fastlog[i];
-      if (CERT) s_clog[i] = a.certlog[i];
-    }
-    __syncthreads();
-    const double ln2 = s_tab[3 * kLogTabN] + s_tab[3 * kLogTabN + 1];
-    for (int p = tid; p < P; p += NT) {
-      const double r = s_ref[p];
-      s_ref_log[p] = r > 0.0 ? r * dlog_bits(2.0 * r, s_flog, ln2) : 0.0;
-    }
-    for (int i = tid; i < SLAB; i += NT) {
-      if (CERT) {
-        const double c = s_cl[i].x;
-        s_cl[i].y = c > 0.0 ? c * dlog_bits(2.0 * c, s_flog, ln2) : 0.0;
-      } else {
-        const float cf = s_slab[i] > 0.0f ? s_slab[i] : 0.0f;
-        const double c = (double)cf;
-        s_lc[i] = cf > 0.0f ? c * dlog_bits(2.0 * c, s_flog, ln2) : 0.0;
-        s_slab[i] = cf;
-      }
-    }
-    __syncthreads();
-    if (CERT && tid == 0) {
-      double rs = 0.0;
-      for (int p = 0; p < P; ++p) rs = rs + s_ref_log[p];
-      s_rsum = rs;
-      // Bound on |d_fast - d_ref| for every candidate of this reference, with
-      // S >= r + c for every term and W >= s (|log s| + 1) + 2 for s <= S:
-      // each path's rounding error is at most a few hundred units of 2^-53
-      // times P*W (per-term log and product errors plus the (P-1)-fold
-      // accumulation bound), and the degree-4 log adds s*1.9e-13 per term;
-      // the constants below carry a margin of about 7x and 2x on those.
-      const double S = (double)__uint_as_float(s_vmax[0]) + (double)__uint_as_float(s_vmax[1]);
-      const double lS = S > 0.0 ? fabs(dlog(S, s_tab)) : 0.0;
-      const double W = S * (lS + 1.0) + 2.0;
-      s_err = 2.0 * (double)P * (S * 4e-13 + 4096.0 * 0x1p-53 * W);
-    }
-    __syncthreads();
-  }
-
-  const double ln2k = s_tab[3 * kLogTabN] + s_tab[3 * kLogTabN + 1];
-  const int rel0 = o0 - lo0, rel1 = o1 - lo1, rel2 = o2 - lo2;
-  const int center_local = (rel0 * e1 + rel1) * e2 + rel2;
-  // distance -> (tie-grid key, accepted); non-decreasing in d (blockmatching.py:957-1042)
-  auto make_key = [&](double d, bool &ok) {
-    if (d < 0.0) d = 0.0;
-    const float d32 = (float)(d / (double)P);
-    float key;
-    if (a.stabilize) {
-      key = canonical_key(d32);
-      ok = key < a.thr_key;
-    } else {
-      key = d32;
-      ok = d32 < a.threshold;
-    }
-    return key;
-  };
-  // the reference expression term by term (blockmatching.py:347-355); the
-  // observation c is max(x, 0) of the slab (unit count scale in MODE 3)
-  auto reference_term = [&](double r, double c, double d) {
-    const double s = r + c;
-    const double lr = dlog((2.0 * r) / s, s_tab);
-    const double lc = dlog((2.0 * c) / s, s_tab);
-    double t = 0.0;
-    if (r > 0.0) t = t + r * lr;
-    t = (c > 0.0) ? t + c * lc : t;
-    return (s > 0.0) ? d + 2.0 * t : d;
-  };
-  int my_acc = 0;
-  for (int cand = tid; cand < box; cand += NT) {
-    const int j2 = cand % e2, j1 = (cand / e2) % e1, j0 = cand / (e2 * e1);
-    const int base = (j0 * S1 + j1) * S2 + j2;
-    double d = 0.0;
-    float key;
-    bool ok;
-    if (CERT) {
-      // one-log distance with the degree-4 log, in any summation order
-      const double ln2c = ln2k;
-      double acc = 0.0;
-#pragma unroll 1
-      for (int p01 = 0; p01 < B0 * B1; ++p01) {
-        const int p0 = p01 / B1, p1 = p01 - p0 * B1;
-        const double2 *row = s_cl + base + (p0 * S1 + p1) * S2;
-        const double *rrow = s_ref + p01 * B2;
-#pragma unroll
-        for (int p2 = 0; p2 < B2; ++p2) {
-          const double2 cl = row[p2];
-          const double s = rrow[p2] + cl.x;
-          acc = acc + fma(-s, log_deg4(s, s_clog, ln2c), cl.y);
-        }
-      }
-      const double df = 2.0 * (s_rsum + acc);
-      bool ok_lo, ok_hi;
-      const float k_lo = make_key(df - s_err, ok_lo);
-      const float k_hi = make_key(df + s_err, ok_hi);
-      if (k_lo == k_hi) {
-        // the reference value lies in [df - err, df + err], where the key is constant
-        key = k_lo;
-        ok = ok_lo;
-      } else {
-        for (int p01 = 0; p01 < B0 * B1; ++p01) {
-          const int p0 = p01 / B1, p1 = p01 - p0 * B1;
-          const double2 *row = s_cl + base + (p0 * S1 + p1) * S2;
-          for (int p2 = 0; p2 < B2; ++p2) d = reference_term(s_ref[p01 * B2 + p2], row[p2].x, d);
-        }
-        key = make_key(d, ok);
-      }
-    } else {
-      // terms are accumulated in patch C order (p0, p1, p2), as the reference does
-#pragma unroll 1
-      for (int p01 = 0; p01 < B0 * B1; ++p01) {
-        const int p0 = p01 / B1, p1 = p01 - p0 * B1;
-        const float *row = s_slab + base + (p0 * S1 + p1) * S2;
-        const double *lcrow = s_lc + (MODE == 1 ? base + (p0 * S1 + p1) * S2 : 0);
-        const double *rrow = s_ref + p01 * B2;
-        const double *lrow = s_ref_log + p01 * B2;
-        const int *trow = s_refrow + p01 * B2;
-#pragma unroll
-        for (int p2 = 0; p2 < B2; ++p2) {
-          if (MODE == 2) {
-            // table entry = 2*(r log(2r/s) + c log(2c/s)), 0 when r = c = 0
-            d = d + s_terms[trow[p2] + (int)row[p2]];
-            continue;
-          }
-          if (MODE == 1) {
-            // the term is exactly 0 at r = c = 0 (the log of 0 stays finite)
-            const double s = rrow[p2] + (double)row[p2];
-            const double ls = dlog_bits(s, s_flog, ln2k);
-            d = d + fma(-s, ls, lrow[p2] + lcrow[p2]);
-            continue;
-          }
-          double c = (double)row[p2] * a.inv_scale;
-          if (c < 0.0) c = 0.0;
-          d = reference_term(rrow[p2], c, d);
-        }
-      }
-      if (MODE == 1) d = 2.0 * d;  // terms were accumulated without the factor 2
-      key = make_key(d, ok);
-    }
-    if (key == 0.0f) key = 0.0f;  // a signed zero sorts as +0
-    my_acc += ok ? 1 : 0;
-    const unsigned long long comp = ((unsigned long long)(ok ? 0u : 1u) << 63) |
-                                    ((unsigned long long)__float_as_uint(key) << 32) |
-                                    (unsigned long long)cand;
-    s_keys[cand] = comp;
-  }
-  if (my_acc) atomicAdd(&s_acc, my_acc);
-  __syncthreads();
-
-  // how many of the smallest composites the reference would keep
-  const int acc = s_acc;
   const int req = min(a.min_m, a.max_m);
   int K;
   if (acc >= a.max_m)
@@ -526,10 +327,10 @@ __global__ __launch_bounds__(NT) void k_blockmatch(BMArgs a) {
   else
     K = min(req, box);
 
-  // Radix select of the K smallest composite keys (keys are unique).  The
-  // candidate index is below 2^16, so bits 16..31 are zero in every key and
-  // their passes are skipped; once the bin holding the K-th key is needed in
-  // full, every key whose resolved bits are <= the prefix is selected.
+  // Radix select of the K smallest composites.  The candidate index is below
+  // 2^16, so bits 16..31 are zero in every key and their passes are skipped;
+  // once the bin holding the K-th key is needed in full, every key whose
+  // resolved bits are <= the prefix is selected.
   static_assert(BOX < 65536, "candidate index must fit 16 bits");
   unsigned long long prefix = 0ull, mask = 0ull;
   int krem = K;
@@ -541,21 +342,22 @@ __global__ __launch_bounds__(NT) void k_blockmatch(BMArgs a) {
     for (int i = tid; i < 256; i += NT) s_hist[i] = 0;
     __syncthreads();
     for (int i = tid; i < box; i += NT) {
-      unsigned long long k = s_keys[i];
+      const unsigned long long k = keys[i];
       if ((k & mask) == prefix) atomicAdd(&s_hist[(int)((k >> shift) & 255ull)], 1);
     }
     __syncthreads();
     if (tid < 64) {
-      int c0 = s_hist[4 * tid], c1 = s_hist[4 * tid + 1], c2 = s_hist[4 * tid + 2],
-          c3 = s_hist[4 * tid + 3];
-      int tot = c0 + c1 + c2 + c3, incl = tot;
+      const int c0 = s_hist[4 * tid], c1 = s_hist[4 * tid + 1], c2 = s_hist[4 * tid + 2],
+                c3 = s_hist[4 * tid + 3];
+      const int tot = c0 + c1 + c2 + c3;
+      int incl = tot;
 #pragma unroll
       for (int o = 1; o < 64; o <<= 1) {
-        int y = __shfl_up(incl, o, 64);
+        const int y = __shfl_up(incl, o, 64);
         if (tid >= o) incl += y;
       }
       int ex = incl - tot;
-      int cs[4] = {c0, c1, c2, c3};
+      const int cs[4] = {c0, c1, c2, c3};
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
         if (ex < krem && ex + cs[b] >= krem) {
@@ -577,12 +379,12 @@ __global__ __launch_bounds__(NT) void k_blockmatch(BMArgs a) {
   if (tid == 0) s_cnt = 0;
   __syncthreads();
   for (int i = tid; i < box; i += NT) {
-    unsigned long long k = s_keys[i];
+    const unsigned long long k = keys[i];
     if ((k & mask) <= prefix) s_small[atomicAdd(&s_cnt, 1)] = k;
   }
   __syncthreads();
   if (tid < K) {
-    unsigned long long mine = s_small[tid];
+    const unsigned long long mine = s_small[tid];
     int rank = 0;
     for (int j = 0; j < K; ++j) rank += s_small[j] < mine ? 1 : 0;
     s_sorted[rank] = mine;
@@ -591,8 +393,7 @@ __global__ __launch_bounds__(NT) void k_blockmatch(BMArgs a) {
   if (tid < 64) {
     // reference first, then the sorted candidates without it (composites are
     // unique, so the reference patch is the only possible duplicate), capped
-    // at max, then power-of-two truncation (blockmatching.py:913-954); one
-    // wave, lane i holding the i-th smallest composite
+    // at max, then power-of-two truncation; lane i holds the i-th smallest
     const int lane = tid;
     const bool valid = lane < K;
     const int loc = valid ? (int)(s_sorted[lane] & 0xffffffffull) : -1;
@@ -619,6 +420,343 @@ __global__ __launch_bounds__(NT) void k_blockmatch(BMArgs a) {
       if (a.accepted) a.accepted[g] = acc;
     }
   }
+}
+
+// MODE 0: each term as the reference writes it (two divisions, two logs);
+// MODE 1: one log per term, r log 2r + c log 2c - s log s (values clamped
+//         and scaled when the slab is staged);
+// MODE 2: integer counts; the term is read from a float64 table built on the
+//         host with the same expression and the C library's log.
+template <int B0, int B1, int B2, int R0, int R1, int R2, int MODE, int NT>
+__global__ __launch_bounds__(NT) void k_blockmatch(BMArgs a) {
+  constexpr int S0 = 2 * R0 + B0, S1 = 2 * R1 + B1, S2 = 2 * R2 + B2;
+  constexpr int SLAB = S0 * S1 * S2;
+  constexpr int BOX = (2 * R0 + 1) * (2 * R1 + 1) * (2 * R2 + 1);
+  constexpr int P = B0 * B1 * B2;
+  __shared__ float s_slab[SLAB];
+  __shared__ double s_ref[P];
+  __shared__ double s_ref_log[P];
+  __shared__ unsigned long long s_keys[BOX];
+  __shared__ double s_tab[kLogTabDoubles];
+  __shared__ int s_acc;
+  __shared__ int s_refrow[P];
+  __shared__ double s_lc[MODE == 1 ? SLAB : 1];  // MODE 1: c*log(2c) per slab voxel
+  __shared__ double2 s_flog[MODE == 1 ? kFastLogN : 1];
+  extern __shared__ double s_terms[];  // MODE 2 only
+
+  const int tid = threadIdx.x;
+  const int64_t g = blockIdx.x;
+  if (g >= a.nrefs) return;
+  const int lin = a.ref_lin[g];
+  const int o2 = lin % a.c2, o1 = (lin / a.c2) % a.c1, o0 = lin / (a.c2 * a.c1);
+  const int lo0 = max(o0 - R0, 0), hi0 = min(o0 + R0, a.c0 - 1);
+  const int lo1 = max(o1 - R1, 0), hi1 = min(o1 + R1, a.c1 - 1);
+  const int lo2 = max(o2 - R2, 0), hi2 = min(o2 + R2, a.c2 - 1);
+  const int e0 = hi0 - lo0 + 1, e1 = hi1 - lo1 + 1, e2 = hi2 - lo2 + 1;
+  const int box = e0 * e1 * e2;
+
+  for (int i = tid; i < kLogTabDoubles; i += NT) s_tab[i] = a.logtab[i];
+  // slab: volume region [lo, hi + B - 1] (always inside the volume); every
+  // load in flight before the first store
+  const int x0 = e0 + B0 - 1, x1 = e1 + B1 - 1, x2 = e2 + B2 - 1;
+  {
+    constexpr int IT = (SLAB + NT - 1) / NT;
+    float vv[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int i = tid + it * NT;
+      const int i2 = i % S2, i1 = (i / S2) % S1, i0 = i / (S2 * S1);
+      vv[it] = 0.0f;
+      if (i < SLAB && i0 < x0 && i1 < x1 && i2 < x2)
+        vv[it] = a.vol[((int64_t)(lo0 + i0) * a.n1 + (lo1 + i1)) * a.n2 + (lo2 + i2)];
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int i = tid + it * NT;
+      if (i < SLAB) s_slab[i] = vv[it];
+    }
+  }
+  for (int p = tid; p < P; p += NT) {
+    int p2 = p % B2, p1 = (p / B2) % B1, p0 = p / (B2 * B1);
+    float v = a.vol[((int64_t)(o0 + p0) * a.n1 + (o1 + p1)) * a.n2 + (o2 + p2)];
+    double r = (double)v * a.inv_scale;
+    if (r < 0.0) r = 0.0;
+    s_ref[p] = r;
+    if (MODE == 2) s_refrow[p] = (int)v * (a.tmax + 1);
+  }
+  if (MODE == 2) {
+    const int tn = (a.tmax + 1) * (a.tmax + 1);
+    for (int i = tid; i < tn; i += NT) s_terms[i] = a.term_table[i];
+  }
+  if (tid == 0) s_acc = 0;
+  __syncthreads();
+  if (MODE == 1) {
+    // x log 2x of every slab voxel and reference voxel (0 at x = 0); the slab
+    // keeps max(x, 0) (count_scale is 1 in this mode)
+    for (int i = tid; i < kFastLogN; i += NT) s_flog[i] = a.fastlog[i];
+    __syncthreads();
+    const double ln2 = s_tab[3 * kLogTabN] + s_tab[3 * kLogTabN + 1];
+    for (int p = tid; p < P; p += NT) {
+      const double r = s_ref[p];
+      s_ref_log[p] = r > 0.0 ? r * dlog_bits(2.0 * r, s_flog, ln2) : 0.0;
+    }
+    for (int i = tid; i < SLAB; i += NT) {
+      const float cf = s_slab[i] > 0.0f ? s_slab[i] : 0.0f;
+      const double c = (double)cf;
+      s_lc[i] = cf > 0.0f ? c * dlog_bits(2.0 * c, s_flog, ln2) : 0.0;
+      s_slab[i] = cf;
+    }
+    __syncthreads();
+  }
+
+  const double ln2k = s_tab[3 * kLogTabN] + s_tab[3 * kLogTabN + 1];
+  const int rel0 = o0 - lo0, rel1 = o1 - lo1, rel2 = o2 - lo2;
+  const int center_local = (rel0 * e1 + rel1) * e2 + rel2;
+  int my_acc = 0;
+  for (int cand = tid; cand < box; cand += NT) {
+    const int j2 = cand % e2, j1 = (cand / e2) % e1, j0 = cand / (e2 * e1);
+    const int base = (j0 * S1 + j1) * S2 + j2;
+    double d = 0.0;
+    // terms are accumulated in patch C order (p0, p1, p2), as the reference does
+#pragma unroll 1
+    for (int p01 = 0; p01 < B0 * B1; ++p01) {
+      const int p0 = p01 / B1, p1 = p01 - p0 * B1;
+      const float *row = s_slab + base + (p0 * S1 + p1) * S2;
+      const double *lcrow = s_lc + (MODE == 1 ? base + (p0 * S1 + p1) * S2 : 0);
+      const double *rrow = s_ref + p01 * B2;
+      const double *lrow = s_ref_log + p01 * B2;
+      const int *trow = s_refrow + p01 * B2;
+#pragma unroll
+      for (int p2 = 0; p2 < B2; ++p2) {
+        if (MODE == 2) {
+          // table entry = 2*(r log(2r/s) + c log(2c/s)), 0 when r = c = 0
+          d = d + s_terms[trow[p2] + (int)row[p2]];
+        } else if (MODE == 1) {
+          // the term is exactly 0 at r = c = 0 (the log of 0 stays finite)
+          const double s = rrow[p2] + (double)row[p2];
+          const double ls = dlog_bits(s, s_flog, ln2k);
+          d = d + fma(-s, ls, lrow[p2] + lcrow[p2]);
+        } else {
+          double c = (double)row[p2] * a.inv_scale;
+          if (c < 0.0) c = 0.0;
+          d = bm_reference_term(rrow[p2], c, d, s_tab);
+        }
+      }
+    }
+    if (MODE == 1) d = 2.0 * d;  // terms were accumulated without the factor 2
+    bool ok;
+    const float key = bm_key<P>(d, a, ok);
+    my_acc += ok ? 1 : 0;
+    s_keys[cand] = bm_composite(key, ok, cand);
+  }
+  if (my_acc) atomicAdd(&s_acc, my_acc);
+  __syncthreads();
+  bm_select_write<NT, BOX>(s_keys, box, s_acc, a, g, lo0, lo1, lo2, e1, e2, center_local);
+}
+
+// Certified block matching (keys equal to MODE 0's).  The distance is
+// evaluated in the one-log form
+//   d = 2 (sum_p r log 2r + sum_p c log 2c - sum_p s log s),   s = r + c,
+// with log s from the degree-4 log_deg4, the patch sums of c log 2c taken
+// from separable box sums over the slab, and a bound E on |d - d_ref| valid
+// for every candidate of the reference (see s_err).  The tie-grid key is a
+// non-decreasing function of the distance, so where key(d - E) == key(d + E)
+// the reference expression's key is that value; elsewhere the reference
+// expression is evaluated term by term.  Observations are max(x, 0) at unit
+// count scale.
+template <int B0, int B1, int B2, int R0, int R1, int R2, int NT>
+__global__ __launch_bounds__(NT) void k_blockmatch_cert(BMArgs a) {
+  constexpr int S0 = 2 * R0 + B0, S1 = 2 * R1 + B1, S2 = 2 * R2 + B2;
+  constexpr int SLAB = S0 * S1 * S2;
+  constexpr int BOX = (2 * R0 + 1) * (2 * R1 + 1) * (2 * R2 + 1);
+  constexpr int P = B0 * B1 * B2;
+  constexpr int W2 = S2 - B2 + 1;         // window positions along axis 2
+  constexpr int XS = S0 * S1 * W2;        // axis-2 window sums of c log 2c
+  constexpr int KC = (BOX + NT - 1) / NT;  // candidates per thread
+  static_assert(BOX <= XS, "the keys reuse the window-sum buffer");
+  __shared__ float s_slab[SLAB];           // max(x, 0)
+  __shared__ double s_xsum[XS];            // window sums, then per-candidate sums, then keys
+  __shared__ double s_ref[P];
+  __shared__ double s_tab[kLogTabDoubles];
+  __shared__ double2 s_flog[kFastLogN];    // staging log (about 1 ulp)
+  __shared__ double2 s_clog[kFastLogN];    // degree-4 log
+  __shared__ double s_rpart[(P + 63) / 64];
+  __shared__ double s_rsum, s_err;
+  __shared__ unsigned s_vmax[2];
+  __shared__ int s_acc;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int64_t g = blockIdx.x;
+  if (g >= a.nrefs) return;
+  const int lin = a.ref_lin[g];
+  const int o2 = lin % a.c2, o1 = (lin / a.c2) % a.c1, o0 = lin / (a.c2 * a.c1);
+  const int lo0 = max(o0 - R0, 0), hi0 = min(o0 + R0, a.c0 - 1);
+  const int lo1 = max(o1 - R1, 0), hi1 = min(o1 + R1, a.c1 - 1);
+  const int lo2 = max(o2 - R2, 0), hi2 = min(o2 + R2, a.c2 - 1);
+  const int e0 = hi0 - lo0 + 1, e1 = hi1 - lo1 + 1, e2 = hi2 - lo2 + 1;
+  const int box = e0 * e1 * e2;
+
+  for (int i = tid; i < kLogTabDoubles; i += NT) s_tab[i] = a.logtab[i];
+  for (int i = tid; i < kFastLogN; i += NT) {
+    s_flog[i] = a.fastlog[i];
+    s_clog[i] = a.certlog[i];
+  }
+  if (tid < 2) s_vmax[tid] = 0u;
+  if (tid == 0) s_acc = 0;
+  __syncthreads();
+  const int x0 = e0 + B0 - 1, x1 = e1 + B1 - 1, x2 = e2 + B2 - 1;
+  float vmax = 0.0f;
+  {
+    constexpr int IT = (SLAB + NT - 1) / NT;
+    float vv[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int i = tid + it * NT;
+      const int i2 = i % S2, i1 = (i / S2) % S1, i0 = i / (S2 * S1);
+      vv[it] = 0.0f;
+      if (i < SLAB && i0 < x0 && i1 < x1 && i2 < x2)
+        vv[it] = a.vol[((int64_t)(lo0 + i0) * a.n1 + (lo1 + i1)) * a.n2 + (lo2 + i2)];
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int i = tid + it * NT;
+      const float cf = vv[it] > 0.0f ? vv[it] : 0.0f;
+      vmax = fmaxf(vmax, cf);
+      if (i < SLAB) s_slab[i] = cf;
+    }
+  }
+  float rmax = 0.0f;
+  for (int p = tid; p < P; p += NT) {
+    const int p2 = p % B2, p1 = (p / B2) % B1, p0 = p / (B2 * B1);
+    const float v = a.vol[((int64_t)(o0 + p0) * a.n1 + (o1 + p1)) * a.n2 + (o2 + p2)];
+    const float rf = v > 0.0f ? v : 0.0f;
+    rmax = fmaxf(rmax, rf);
+    s_ref[p] = (double)rf;
+  }
+  // non-negative floats order like their bit patterns
+  if (vmax > 0.0f) atomicMax(&s_vmax[1], __float_as_uint(vmax));
+  if (rmax > 0.0f) atomicMax(&s_vmax[0], __float_as_uint(rmax));
+  __syncthreads();
+
+  const double ln2 = s_tab[3 * kLogTabN] + s_tab[3 * kLogTabN + 1];
+  // sum of r log 2r over the reference patch (fixed-order wave reduction)
+  if (tid < 64 * ((P + 63) / 64)) {
+    double rl = 0.0;
+    if (tid < P) {
+      const double r = s_ref[tid];
+      rl = r > 0.0 ? r * dlog_bits(2.0 * r, s_flog, ln2) : 0.0;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) rl = rl + __shfl_xor(rl, o, 64);
+    if (lane == 0) s_rpart[tid >> 6] = rl;
+  }
+  // axis-2 window sums of c log 2c, one slab row per thread
+  for (int row = tid; row < S0 * S1; row += NT) {
+    const float *src = s_slab + row * S2;
+    double w[B2];
+#pragma unroll
+    for (int t = 0; t < B2; ++t) w[t] = 0.0;
+#pragma unroll
+    for (int i2 = 0; i2 < S2; ++i2) {
+#pragma unroll
+      for (int t = 0; t + 1 < B2; ++t) w[t] = w[t + 1];
+      const double c = (double)src[i2];
+      w[B2 - 1] = c > 0.0 ? c * dlog_bits(2.0 * c, s_flog, ln2) : 0.0;
+      if (i2 >= B2 - 1) {
+        double sum = w[0];
+#pragma unroll
+        for (int t = 1; t < B2; ++t) sum = sum + w[t];
+        s_xsum[row * W2 + (i2 - (B2 - 1))] = sum;
+      }
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double rs = 0.0;
+    for (int w = 0; w < (P + 63) / 64; ++w) rs = rs + s_rpart[w];
+    s_rsum = rs;
+    // Bound on |d_fast - d_ref| for every candidate of this reference, with
+    // S >= s = r + c for every term and W >= s (|log s| + 1) + 2 for s <= S.
+    // Each path's rounding error is at most a few hundred units of 2^-53
+    // times P*W (per-term log and product errors plus the (P-1)-fold
+    // accumulation bound, which holds for any summation order), and the
+    // degree-4 log adds at most 1.9e-13*s per term; the constants carry a
+    // margin of about 7x and 2x on those.
+    const double S = (double)__uint_as_float(s_vmax[0]) + (double)__uint_as_float(s_vmax[1]);
+    const double lS = S > 0.0 ? fabs(dlog(S, s_tab)) : 0.0;
+    const double W = S * (lS + 1.0) + 2.0;
+    s_err = 2.0 * (double)P * (S * 4e-13 + 4096.0 * 0x1p-53 * W);
+  }
+  // patch sums of c log 2c per candidate; each is parked in the slot its key
+  // will take (slot cand of the reused buffer belongs to one thread)
+  double zc[KC];
+#pragma unroll
+  for (int k = 0; k < KC; ++k) {
+    const int cand = tid + k * NT;
+    zc[k] = 0.0;
+    if (cand < box) {
+      const int j2 = cand % e2, j1 = (cand / e2) % e1, j0 = cand / (e2 * e1);
+      double z = 0.0;
+#pragma unroll
+      for (int b0 = 0; b0 < B0; ++b0)
+#pragma unroll
+        for (int b1 = 0; b1 < B1; ++b1) z = z + s_xsum[((j0 + b0) * S1 + (j1 + b1)) * W2 + j2];
+      zc[k] = z;
+    }
+  }
+  __syncthreads();
+  unsigned long long *keys = reinterpret_cast<unsigned long long *>(s_xsum);
+#pragma unroll
+  for (int k = 0; k < KC; ++k) {
+    const int cand = tid + k * NT;
+    if (cand < box) keys[cand] = (unsigned long long)__double_as_longlong(zc[k]);
+  }
+  const double rsum = s_rsum, err = s_err;
+  const int rel0 = o0 - lo0, rel1 = o1 - lo1, rel2 = o2 - lo2;
+  const int center_local = (rel0 * e1 + rel1) * e2 + rel2;
+  int my_acc = 0;
+  for (int cand = tid; cand < box; cand += NT) {
+    const int j2 = cand % e2, j1 = (cand / e2) % e1, j0 = cand / (e2 * e1);
+    const int base = (j0 * S1 + j1) * S2 + j2;
+    double acc = 0.0;
+#pragma unroll 1
+    for (int p01 = 0; p01 < B0 * B1; ++p01) {
+      const int p0 = p01 / B1, p1 = p01 - p0 * B1;
+      const float *row = s_slab + base + (p0 * S1 + p1) * S2;
+      const double *rrow = s_ref + p01 * B2;
+#pragma unroll
+      for (int p2 = 0; p2 < B2; ++p2) {
+        const double s = rrow[p2] + (double)row[p2];
+        acc = fma(-s, log_deg4(s, s_clog, ln2), acc);
+      }
+    }
+    const double zsum = __longlong_as_double((long long)keys[cand]);
+    const double df = 2.0 * (rsum + (zsum + acc));
+    bool ok_lo, ok_hi, ok;
+    const float k_lo = bm_key<P>(df - err, a, ok_lo);
+    const float k_hi = bm_key<P>(df + err, a, ok_hi);
+    float key;
+    if (k_lo == k_hi) {
+      // the reference value lies in [df - err, df + err], where the key is constant
+      key = k_lo;
+      ok = ok_lo;
+    } else {
+      double d = 0.0;
+      for (int p01 = 0; p01 < B0 * B1; ++p01) {
+        const int p0 = p01 / B1, p1 = p01 - p0 * B1;
+        const float *row = s_slab + base + (p0 * S1 + p1) * S2;
+        for (int p2 = 0; p2 < B2; ++p2)
+          d = bm_reference_term(s_ref[p01 * B2 + p2], (double)row[p2], d, s_tab);
+      }
+      key = bm_key<P>(d, a, ok);
+    }
+    my_acc += ok ? 1 : 0;
+    keys[cand] = bm_composite(key, ok, cand);
+  }
+  if (my_acc) atomicAdd(&s_acc, my_acc);
+  __syncthreads();
+  bm_select_write<NT, BOX>(keys, box, s_acc, a, g, lo0, lo1, lo2, e1, e2, center_local);
 }
 
 // ---------------------------------------------------------------------------
@@ -1575,8 +1713,8 @@ int32_t bmnd_hip_blockmatch(const float *vol, const int64_t *shape_h, const int3
       hipLaunchKernelGGL((k_blockmatch<b0, b1, b2, r0, r1, r2, 1, kBMThreadsOneLog>), grid,      \
                          dim3(kBMThreadsOneLog), 0, s, a);                                       \
     else if (mode == 3)                                                                          \
-      hipLaunchKernelGGL((k_blockmatch<b0, b1, b2, r0, r1, r2, 3, kBMThreadsOneLog>), grid,      \
-                         dim3(kBMThreadsOneLog), 0, s, a);                                       \
+      hipLaunchKernelGGL((k_blockmatch_cert<b0, b1, b2, r0, r1, r2, kBMThreadsCert>), grid,      \
+                         dim3(kBMThreadsCert), 0, s, a);                                         \
     else                                                                                         \
       hipLaunchKernelGGL((k_blockmatch<b0, b1, b2, r0, r1, r2, 2, kBMThreads>), grid, blk, dyn,  \
                          s, a);                                                                  \
