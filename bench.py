@@ -33,7 +33,9 @@ PEAK_HBM_GBS = 8000.0
 FLOPS_PER_TERM = 10
 # workgroup size of the block-matching kernel per deviance mode (its last
 # template argument, part of the kernel name rocprofv3 reports)
-BM_THREADS = {0: 512, 1: 1024, 2: 512, 3: 1024}
+BM_THREADS = {0: 512, 1: 1024, 2: 512, 3: 512}
+# (stack capacity, workgroup size) of the exact-variance kernel per block
+EXACT_CONFIG = {(1, 8, 8): (32, 64), (4, 4, 4): (16, 64), (5, 5, 5): (32, 128)}
 
 
 def parse() -> argparse.Namespace:
@@ -128,7 +130,7 @@ def main() -> int:
         print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4}))
         return 0
 
-    from bmnd_amd.engine import KernelTimer, PoissonPipeline, blockmatch_work
+    from bmnd_amd.engine import KernelTimer, PoissonPipeline, blockmatch_work, exact_std_work
 
     profile, noisy_list, truth_list, cfg = make_inputs(args.config)
     dev = torch.device("cuda", 0)
@@ -136,10 +138,13 @@ def main() -> int:
     pipe = PoissonPipeline(profile, noisy_list[0].shape, dev, deviance=args.deviance)
     voxels = sum(int(v.numel()) for v in vols)
 
+    last_outs = {}
+
     def step(timer=None):
         outs = []
         for v in vols:
             ht, wi = pipe.run(v, timer)
+            last_outs["ht"], last_outs["wiener"] = ht, wi
             outs.append(wi.est)
         return outs
 
@@ -156,28 +161,49 @@ def main() -> int:
 
     spans = timer.totals_ms()
     per_kernel = {k: float(np.mean(v)) for k, v in spans.items()}
-    # algorithmic work of the block-matching launches
-    work = {s: blockmatch_work(pipe.plans[s], noisy_list[0].shape) for s in ("ht", "wiener")}
-    dominant = max(per_kernel, key=lambda k: per_kernel[k])
-    roof = None
-    if dominant.endswith(".blockmatch"):
-        st = dominant.split(".")[0]
+    shape3 = pipe.shape3
+    rooflines = {}
+    for st in ("ht", "wiener"):
         plan = pipe.plans[st]
+        # block matching: the reference's deviance expression per term
+        pairs, terms = blockmatch_work(plan, noisy_list[0].shape)
         mode = pipe.last_modes.get(st, 0)
-        kname = "k_blockmatch<{}, {}, {}, {}, {}, {}, {}, {}>(BMArgs)".format(
-            *plan.block3, *plan.radius3, mode, BM_THREADS[mode])
-        flops = FLOPS_PER_TERM * work[st][1]
-        achieved = flops / (per_kernel[dominant] * 1e-3) / 1e12
-        roof = {"kernel": kname, "bound": "mfma", "achieved": round(achieved, 3),
-                "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
+        if mode == 3:
+            kname = "k_blockmatch_cert<{}, {}, {}, {}, {}, {}, {}>(BMArgs)".format(
+                *plan.block3, *plan.radius3, BM_THREADS[3])
+        else:
+            kname = "k_blockmatch<{}, {}, {}, {}, {}, {}, {}, {}>(BMArgs)".format(
+                *plan.block3, *plan.radius3, mode, BM_THREADS[mode])
+        ms = per_kernel[f"{st}.blockmatch"]
+        ach = FLOPS_PER_TERM * terms / (ms * 1e-3) / 1e12
+        rooflines[f"{st}.blockmatch"] = {
+            "kernel": kname, "bound": "mfma", "achieved": round(ach, 3),
+            "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / PEAK_FP64_TFLOPS, 4),
+            "traffic": measured_traffic(args.config, args.deviance, kname),
+            "algorithmic": f"{FLOPS_PER_TERM} float64 ops x {terms} deviance terms per launch",
+            "avg_ms": round(ms, 3)}
+        # exact overlap-aware variance: float32 product-sums per entry and union voxel
+        if f"{st}.exact_std" in per_kernel:
+            out = last_outs[st]
+            w = exact_std_work(plan, shape3, out.sel, out.counts, pipe.k)
+            nmax, nt = EXACT_CONFIG[tuple(plan.block3)]
+            kname = "k_exact_std<{}, {}, {}, {}, {}>(ExactArgs)".format(*plan.block3, nmax, nt)
+            ms = per_kernel[f"{st}.exact_std"]
+            ach = w["flops"] / (ms * 1e-3) / 1e12
+            rooflines[f"{st}.exact_std"] = {
+                "kernel": kname, "bound": "mfma", "achieved": round(ach, 3),
+                "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(ach / PEAK_FP32_TFLOPS, 4),
                 "traffic": measured_traffic(args.config, args.deviance, kname),
-                "algorithmic": f"{FLOPS_PER_TERM} ops x {work[st][1]} deviance terms per launch",
-                "avg_ms": round(per_kernel[dominant], 3)}
-    else:
-        roof = {"kernel": dominant, "bound": "mfma", "achieved": None, "peak": PEAK_FP32_TFLOPS,
-                "unit": "TFLOP/s", "frac": None, "traffic": None,
-                "avg_ms": round(per_kernel[dominant], 3)}
+                "algorithmic": "float32 q*P*(2*n*P + 3*U) per group: {} entries, {} union "
+                               "voxels, {} ops per launch".format(
+                                   w["entries"], w["union_voxels"], w["flops"]),
+                "avg_ms": round(ms, 3)}
+    dominant = max(per_kernel, key=lambda k: per_kernel[k])
+    roof = rooflines.get(dominant) or {
+        "kernel": dominant, "bound": "mfma", "achieved": None, "peak": PEAK_FP32_TFLOPS,
+        "unit": "TFLOP/s", "frac": None, "traffic": None, "avg_ms": round(per_kernel[dominant], 3)}
+    work = {s: blockmatch_work(pipe.plans[s], noisy_list[0].shape) for s in ("ht", "wiener")}
 
     psnr = None
     if truth_list is not None:
@@ -198,7 +224,7 @@ def main() -> int:
         "dtype": "f64+f32", "data": "synthetic",
         "config": dict(cfg, pilot="plain", deviance=args.deviance,
                        blockmatch_modes=dict(pipe.last_modes)),
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "rooflines": rooflines, "cpu_baseline": cpu,
         "kernels_ms": {k: round(v, 3) for k, v in sorted(per_kernel.items())},
         "blockmatch_terms": {s: work[s][1] for s in work},
         "psnr": psnr,

@@ -216,6 +216,47 @@ def blockmatch_work(plan: StagePlan, shape: tuple[int, ...]) -> tuple[int, int]:
     return pairs, pairs * int(np.prod(plan.block))
 
 
+def exact_std_work(plan: StagePlan, shape3: tuple[int, int, int], sel: torch.Tensor,
+                   counts: torch.Tensor, k: int, chunk: int = 16384) -> dict:
+    """Algorithmic work of a stage's exact-variance launch, counted on the
+    device: for every group of n members, P = patch size, U = voxels covered
+    by the union of its patches and q = min(k, n) exact planes, the
+    overlap-aware variance sum_v var_v (sum_(i,p) G[q,i] S[s,p])^2 costs
+    q*P*(2*n*P + 3*U) floating-point operations (a multiply and an add per
+    entry and coefficient, two multiplies and an add per union voxel)."""
+    b0, b1, b2 = plan.block3
+    P = b0 * b1 * b2
+    n0, n1, n2 = shape3
+    c1, c2 = n1 - b1 + 1, n2 - b2 + 1
+    dev = sel.device
+    max_m = plan.max_m
+    sel2 = sel.view(-1, max_m).to(torch.int64)
+    cnt = counts.to(torch.int64)
+    off = torch.arange(P, device=dev)
+    p2, p1, p0 = off % b2, (off // b2) % b1, off // (b2 * b1)
+    poff = (p0 * n1 + p1) * n2 + p2  # voxel offset of each patch element
+    entries = 0
+    union = 0
+    flops = 0
+    for a in range(0, sel2.shape[0], chunk):
+        s = sel2[a:a + chunk]
+        n = cnt[a:a + chunk]
+        x2, x1, x0 = s % c2, (s // c2) % c1, s // (c2 * c1)
+        base = (x0 * n1 + x1) * n2 + x2  # voxel index of each member's origin
+        vox = base[:, :, None] + poff[None, None, :]
+        valid = (torch.arange(max_m, device=dev)[None, :] < n[:, None])[:, :, None]
+        vox = torch.where(valid, vox, torch.full_like(vox, -1)).view(s.shape[0], -1)
+        vs, _ = torch.sort(vox, dim=1)
+        new = torch.ones_like(vs, dtype=torch.bool)
+        new[:, 1:] = vs[:, 1:] != vs[:, :-1]
+        u = (new & (vs >= 0)).sum(dim=1)
+        q = torch.clamp(n, max=k)
+        entries += int((n * P).sum())
+        union += int(u.sum())
+        flops += int((q * P * (2 * n * P + 3 * u)).sum())
+    return {"entries": entries, "union_voxels": union, "flops": flops}
+
+
 class PoissonPipeline:
     """Two-stage Poisson denoiser on one device.  Buffers are torch tensors on
     `device`; kernels run on the current torch stream."""

@@ -59,9 +59,13 @@ int32_t bmnd_hip_count_range(const float *vol, int64_t n, int32_t *flags,
  *   accepted[nrefs]           out: number of candidates under the threshold
  * mode 0 evaluates each deviance term as the reference writes it (two
  * divisions, two logarithms in float64); mode 1 uses the one-log form
- * r*(log 2r - log s) + c*(log 2c - log s); mode 2 requires integer counts in
+ * r log 2r + c log 2c - s log s; mode 2 requires integer counts in
  * [0, table_max] (table_max <= 100, count_scale == 1) and reads each term
- * from a float64 table built on the host with the reference's expression. */
+ * from a float64 table built on the host with the reference's expression;
+ * mode 3 (certified) evaluates the one-log form with a degree-4 logarithm
+ * together with an error bound per reference and re-evaluates the reference
+ * expression wherever the bound leaves the tie-grid key open, so its output
+ * equals mode 0's.  Modes 1 and 3 fall back to mode 0 when count_scale != 1. */
 int32_t bmnd_hip_blockmatch(const float *vol, const int64_t *shape_h,
                             const int32_t *block_h, const int32_t *radius_h,
                             const int32_t *ref_lin, int64_t nrefs,

@@ -103,3 +103,36 @@ def test_unsupported_options_raise_before_any_gpu_work(field, value):
 def test_gaussian_profiles_are_not_run():
     with pytest.raises(NotImplementedError, match="Poisson"):
         bmnd(np.ones((32, 32)), BM3DProfile())
+
+
+def test_exact_std_work_counts_union_voxels():
+    """The exact-variance work counter (used for the roofline) against a
+    brute-force union of the member patches."""
+    import torch
+    from bmnd_amd.engine import exact_std_work, make_plan
+    plan = make_plan(BM4DProfile(noise_model=NoiseModel.POISSON), "wiener", 3)
+    shape3 = (14, 15, 16)
+    b = plan.block3
+    c = [shape3[d] - b[d] + 1 for d in range(3)]
+    rng = np.random.default_rng(1)
+    R = 12
+    cnt = rng.integers(1, plan.max_m + 1, R)
+    sel = np.full((R, plan.max_m), -1, np.int64)
+    for r in range(R):
+        sel[r, : cnt[r]] = rng.choice(int(np.prod(c)), cnt[r], replace=False)
+    w = exact_std_work(plan, shape3, torch.from_numpy(sel.astype(np.int32).ravel()),
+                       torch.from_numpy(cnt.astype(np.int32)), 4, chunk=5)
+    P = int(np.prod(b))
+    flops = union = 0
+    for r in range(R):
+        vox = set()
+        for lin in sel[r, : cnt[r]]:
+            x2, x1, x0 = lin % c[2], (lin // c[2]) % c[1], lin // (c[2] * c[1])
+            for p in range(P):
+                p2, p1, p0 = p % b[2], (p // b[2]) % b[1], p // (b[2] * b[1])
+                vox.add(((x0 + p0) * shape3[1] + x1 + p1) * shape3[2] + x2 + p2)
+        union += len(vox)
+        flops += min(4, cnt[r]) * P * (2 * cnt[r] * P + 3 * len(vox))
+    assert w["entries"] == int(cnt.sum()) * P
+    assert w["union_voxels"] == union
+    assert w["flops"] == flops
