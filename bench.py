@@ -159,6 +159,15 @@ def main() -> int:
     wall = time.perf_counter() - t0
     ms_per_step = 1e3 * wall / args.steps
 
+    # host-to-host: one further call from host arrays, upload and download included
+    t0 = time.perf_counter()
+    for v_np in noisy_list:
+        v_dev = torch.from_numpy(np.ascontiguousarray(v_np)).to(dev)
+        _, wi_h = pipe.run(v_dev)
+        _ = wi_h.est.cpu().numpy()
+    torch.cuda.synchronize()
+    h2h_ms = 1e3 * (time.perf_counter() - t0)
+
     spans = timer.totals_ms()
     per_kernel = {k: float(np.mean(v)) for k, v in spans.items()}
     shape3 = pipe.shape3
@@ -225,6 +234,8 @@ def main() -> int:
         "config": dict(cfg, pilot="plain", deviance=args.deviance,
                        blockmatch_modes=dict(pipe.last_modes)),
         "roofline": roof, "rooflines": rooflines, "cpu_baseline": cpu,
+        "host_to_host": {"ms": round(h2h_ms, 3), "value": round(voxels / (h2h_ms * 1e-3), 1),
+                         "unit": "voxels/s"},
         "kernels_ms": {k: round(v, 3) for k, v in sorted(per_kernel.items())},
         "blockmatch_terms": {s: work[s][1] for s in work},
         "psnr": psnr,

@@ -80,3 +80,74 @@ def test_deviance_modes_on_crop(deviance):
         assert stats["out_bits_differing"] == 0
     else:
         assert stats["out_max_rel"] < 1e-3
+
+
+@pytest.mark.parametrize("peak", [1.0, 100.0])
+def test_blob_image_bm3d_matches_oracle(peak):
+    """The 2-D workload type of the 512^2 benchmark set at unclipped search
+    boxes: peak 1 (mostly zero counts) and peak 100 (counts above the
+    integer-table range, so stage 1 takes the certified matcher too)."""
+    _, noisy = phantoms.blob_image(144, seed=7, peak=peak)
+    prof = BM3DProfile(noise_model=NoiseModel.POISSON)
+    stats, out, o_out = _compare(noisy, prof, oracle.default_params(2, noisy.shape),
+                                 "auto-certified")
+    assert stats["ht_groups_differing"] == 0
+    assert stats["wiener_groups_differing"] == 0
+    assert stats["pilot_bits_differing"] == 0
+    assert stats["out_bits_differing"] == 0
+
+
+def _check_groups(out, plan, shape, radius):
+    """Structural invariants of one stage's match lists: reference first,
+    power-of-two sizes within [1, max], distinct members inside the patch
+    grid and inside the reference's search box."""
+    sel = out.sel.view(-1, plan.max_m).cpu().numpy().astype(np.int64)
+    cnt = out.counts.cpu().numpy()
+    ref = np.asarray(out.schedule.reference_abs, dtype=np.int64)
+    ndim = ref.shape[1]
+    counts = np.array([shape[d] - plan.block[d] + 1 for d in range(ndim)], dtype=np.int64)
+    lin = np.zeros(len(ref), dtype=np.int64)
+    for d in range(ndim):
+        lin = lin * counts[d] + ref[:, d]
+    assert np.all(cnt >= 1) and np.all(cnt <= plan.max_m)
+    assert np.all((cnt & (cnt - 1)) == 0), "group sizes must be powers of two"
+    assert np.array_equal(sel[:, 0], lin), "the reference patch comes first"
+    valid = np.arange(plan.max_m)[None, :] < cnt[:, None]
+    assert np.all(sel[~valid] == -1)
+    assert np.all((sel[valid] >= 0) & (sel[valid] < np.prod(counts)))
+    srt = np.sort(np.where(valid, sel, np.iinfo(np.int64).max), axis=1)
+    dup = (srt[:, 1:] == srt[:, :-1]) & (srt[:, 1:] != np.iinfo(np.int64).max)
+    assert not dup.any(), "members must be distinct"
+    pos = np.empty(sel.shape + (ndim,), dtype=np.int64)
+    rem = np.where(valid, sel, lin[:, None])
+    for d in range(ndim - 1, -1, -1):
+        pos[..., d] = rem % counts[d]
+        rem = rem // counts[d]
+    assert np.all(np.abs(pos - ref[:, None, :]) <= np.asarray(radius)[None, None, :])
+
+
+@pytest.mark.parametrize("case", ["bm4d-256", "bm3d-512"])
+def test_full_size_determinism_and_group_invariants(case):
+    """Benchmark sizes, where the oracle would take minutes: two calls give
+    bit-identical outputs and match lists, every group satisfies the
+    reference's structural rules and the output is finite."""
+    dev = torch.device("cuda", 0)
+    if case == "bm4d-256":
+        _, noisy = phantoms.ellipsoid_volume((256, 256, 256), seed=0, peak=10.0)
+        prof = BM4DProfile(noise_model=NoiseModel.POISSON)
+    else:
+        _, noisy = phantoms.blob_image(512, seed=3)
+        prof = BM3DProfile(noise_model=NoiseModel.POISSON)
+    print(f"case: {case} shape={noisy.shape}", flush=True)
+    pipe = PoissonPipeline(prof, noisy.shape, dev)
+    v = torch.from_numpy(np.ascontiguousarray(noisy)).to(dev)
+    ht1, wi1 = pipe.run(v)
+    ht2, wi2 = pipe.run(v)
+    torch.cuda.synchronize()
+    assert torch.equal(wi1.est.view(torch.int32), wi2.est.view(torch.int32))
+    assert torch.equal(ht1.est.view(torch.int32), ht2.est.view(torch.int32))
+    assert torch.equal(wi1.sel, wi2.sel) and torch.equal(ht1.sel, ht2.sel)
+    assert bool(torch.isfinite(wi1.est).all())
+    for out, st in ((ht1, "ht"), (wi1, "wiener")):
+        plan = pipe.plans[st]
+        _check_groups(out, plan, noisy.shape, plan.radius3[3 - noisy.ndim:])
