@@ -1341,6 +1341,7 @@ __global__ __launch_bounds__(kAggThreads) void k_aggregate(AggArgs a) {
   __shared__ int s_groups[kAggGroupCap];
   __shared__ int s_ent_idx[kAggEntryCap];
   __shared__ int s_ent_pos[kAggEntryCap];
+  __shared__ float s_ent_w[kAggEntryCap];
   __shared__ float s_k[P];
   __shared__ int s_gcount, s_take, s_nent;
   __shared__ int s_scan[kAggThreads / 64 + 1];
@@ -1437,6 +1438,7 @@ __global__ __launch_bounds__(kAggThreads) void k_aggregate(AggArgs a) {
                          m1 + B1 - 1 >= t_lo1 && m2 <= t_hi2 && m2 + B2 - 1 >= t_lo2;
         if (hit) {
           s_ent_idx[w] = gsel * a.max_m + i;
+          s_ent_w[w] = a.weights[(int64_t)gsel * a.max_m + i];
           // member origin relative to the tile, 3 x 10-bit biased fields
           s_ent_pos[w] = ((m0 - t_lo0 + 512) << 20) | ((m1 - t_lo1 + 512) << 10) |
                          (m2 - t_lo2 + 512);
@@ -1448,17 +1450,36 @@ __global__ __launch_bounds__(kAggThreads) void k_aggregate(AggArgs a) {
     __syncthreads();
     nent = s_nent;
     if (inside) {
-      for (int e = 0; e < nent; ++e) {
-        const int pk = s_ent_pos[e];
-        const int p0 = v0 - (((pk >> 20) & 1023) - 512);
-        const int p1 = v1 - (((pk >> 10) & 1023) - 512);
-        const int p2 = v2 - ((pk & 1023) - 512);
-        if (p0 >= 0 && p0 < B0 && p1 >= 0 && p1 < B1 && p2 >= 0 && p2 < B2) {
-          const int p = (p0 * B1 + p1) * B2 + p2;
-          const int idx = s_ent_idx[e];
-          const float ww = a.weights[idx] * s_k[p];
-          num = num + a.filt[(int64_t)idx * P + p] * ww;
-          den = den + ww;
+      // chunks of entries: every load of a chunk is issued before the
+      // contributions are added, in entry order
+      constexpr int CH = 16;
+      for (int e0 = 0; e0 < nent; e0 += CH) {
+        float fv[CH], wv[CH];
+        unsigned hits = 0u;
+#pragma unroll
+        for (int k = 0; k < CH; ++k) {
+          const int e = e0 + k;
+          fv[k] = 0.0f;
+          wv[k] = 0.0f;
+          if (e < nent) {
+            const int pk = s_ent_pos[e];
+            const int p0 = v0 - (((pk >> 20) & 1023) - 512);
+            const int p1 = v1 - (((pk >> 10) & 1023) - 512);
+            const int p2 = v2 - ((pk & 1023) - 512);
+            if (p0 >= 0 && p0 < B0 && p1 >= 0 && p1 < B1 && p2 >= 0 && p2 < B2) {
+              const int p = (p0 * B1 + p1) * B2 + p2;
+              wv[k] = s_ent_w[e] * s_k[p];
+              fv[k] = a.filt[(int64_t)s_ent_idx[e] * P + p];
+              hits |= 1u << k;
+            }
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < CH; ++k) {
+          if (hits & (1u << k)) {
+            num = num + fv[k] * wv[k];
+            den = den + wv[k];
+          }
         }
       }
     }
