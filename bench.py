@@ -34,8 +34,8 @@ FLOPS_PER_TERM = 10
 # workgroup size of the block-matching kernel per deviance mode (its last
 # template argument, part of the kernel name rocprofv3 reports)
 BM_THREADS = {0: 512, 1: 1024, 2: 512, 3: 512}
-# (stack capacity, workgroup size) of the exact-variance kernel per block
-EXACT_CONFIG = {(1, 8, 8): (32, 64), (4, 4, 4): (16, 64), (5, 5, 5): (32, 128)}
+# stack capacity of the exact-variance kernel per block
+EXACT_CONFIG = {(1, 8, 8): 32, (4, 4, 4): 16, (5, 5, 5): 32}
 
 
 def parse() -> argparse.Namespace:
@@ -195,8 +195,9 @@ def main() -> int:
         if f"{st}.exact_std" in per_kernel:
             out = last_outs[st]
             w = exact_std_work(plan, shape3, out.sel, out.counts, pipe.k)
-            nmax, nt = EXACT_CONFIG[tuple(plan.block3)]
-            kname = "k_exact_std<{}, {}, {}, {}, {}>(ExactArgs)".format(*plan.block3, nmax, nt)
+            nmax = EXACT_CONFIG[tuple(plan.block3)]
+            kname = "k_exact_std<{}, {}, {}, {}, {}, {}, {}>(ExactArgs)".format(
+                *plan.block3, *plan.radius3, nmax)
             ms = per_kernel[f"{st}.exact_std"]
             ach = w["flops"] / (ms * 1e-3) / 1e12
             rooflines[f"{st}.exact_std"] = {

@@ -149,13 +149,17 @@ __device__ __forceinline__ double dlog_bits(double x, const double2 *__restrict_
 // r = 2m/c_j - 1, |r| <= 2^-8, and log1p(r) ~ r - r^2/2 + r^3/3 - r^4/4, whose
 // truncation error is below |r|^5/5 / (1 - |r|) < 1.9e-13.  Returns a finite
 // value for x = 0.
-__device__ __forceinline__ double log_deg4(double x, const double2 *__restrict__ tab, double ln2) {
+// c3 = 1/3 and c4 = -1/4 are passed in (as registers) rather than written as
+// literals: a float64 literal that is not an inline constant turns each fma
+// into a register copy plus an fmac.
+__device__ __forceinline__ double log_deg4(double x, const double2 *__restrict__ tab, double ln2,
+                                           double c3, double c4) {
   const double m = __builtin_amdgcn_frexp_mant(x);
   const int e = __builtin_amdgcn_frexp_exp(x);
   const unsigned hi = (unsigned)((unsigned long long)__double_as_longlong(m) >> 32);
   const double2 t = tab[(hi >> 13) & 127u];
   const double r = fma(m, t.x, -1.0);
-  double q = fma(r, -0.25, 1.0 / 3.0);
+  double q = fma(r, c4, c3);
   q = fma(q, r, -0.5);
   return fma((double)e, ln2, t.y) + fma(r * r, q, r);
 }
@@ -256,6 +260,7 @@ struct BMArgs {
   const double *logtab;
   const double2 *fastlog;    // MODES 1, 3: {1/c_j, log c_j}, j < kFastLogN
   const double2 *certlog;    // MODE 3: {2/c_j, log c_j - ln2}, j < kFastLogN
+  double cert_c3, cert_c4;   // MODE 3: 1/3 and -1/4, the degree-4 log's coefficients
   const double *term_table;  // MODE 2: (tmax+1)^2 terms, row = reference count
   int tmax;
   int32_t *sel, *counts, *accepted;
@@ -713,6 +718,7 @@ __global__ __launch_bounds__(NT) void k_blockmatch_cert(BMArgs a) {
     if (cand < box) keys[cand] = (unsigned long long)__double_as_longlong(zc[k]);
   }
   const double rsum = s_rsum, err = s_err;
+  const double c3 = a.cert_c3, c4 = a.cert_c4;
   const int rel0 = o0 - lo0, rel1 = o1 - lo1, rel2 = o2 - lo2;
   const int center_local = (rel0 * e1 + rel1) * e2 + rel2;
   int my_acc = 0;
@@ -728,7 +734,7 @@ __global__ __launch_bounds__(NT) void k_blockmatch_cert(BMArgs a) {
 #pragma unroll
       for (int p2 = 0; p2 < B2; ++p2) {
         const double s = rrow[p2] + (double)row[p2];
-        acc = fma(-s, log_deg4(s, s_clog, ln2), acc);
+        acc = fma(-s, log_deg4(s, s_clog, ln2, c3, c4), acc);
       }
     }
     const double zsum = __longlong_as_double((long long)keys[cand]);
@@ -804,22 +810,33 @@ struct ExactArgs {
   float *exact_std;
 };
 
-template <int B0, int B1, int B2, int NMAX, int NT>
-__global__ __launch_bounds__(NT) void k_exact_std(ExactArgs a) {
+// Exact overlap-aware variance of the first min(k, N) group planes
+// (variance.py:652-975).  One wave per group; lane l owns the spatial
+// coefficients l and l + 64 (when P > 64) and four planes at a time.  The union
+// voxels are visited in C order over the union box 64 at a time: each lane
+// takes one box voxel, finds the members covering it from three per-axis
+// member masks, and a wave scan places the (voxel, member) entries, members in
+// increasing order, in a chunk list; the serial walk then reads that list
+// with readlane, so every sum runs in the order of the sorted union labels
+// with the stable member order inside a voxel.
+template <int B0, int B1, int B2, int R0, int R1, int R2, int NMAX>
+__global__ __launch_bounds__(64) void k_exact_std(ExactArgs a) {
   constexpr int P = B0 * B1 * B2;
   constexpr int B01 = B0 * B1;
-  constexpr int NP = NMAX * P;
-  constexpr int SORTN = (NP <= 512) ? 512 : (NP <= 1024) ? 1024 : (NP <= 2048) ? 2048 : 4096;
-  static_assert(NP <= 4096, "group too large");
-  static_assert(P <= 128 && P <= NT, "one lane per spatial coefficient");
-  __shared__ unsigned s_key[SORTN];
+  constexpr int SPL = (P + 63) / 64;  // spatial coefficients per lane
+  constexpr int U0 = 2 * R0 + B0, U1 = 2 * R1 + B1, U2 = 2 * R2 + B2;  // union box bound
+  static_assert(SPL <= 2, "at most 128 spatial coefficients");
+  static_assert(NMAX <= 32, "member masks are 32-bit");
+  static_assert((B01 - 1) * B01 + B01 <= 1024 && (B2 - 1) * B2 + B2 <= 64,
+                "entry code fields");
   __shared__ float s_t01[B01 * B01];  // [p01][s01]
   __shared__ float s_a2[B2 * B2];     // [p2][s2]
   __shared__ float4 s_g4[NMAX];
-  __shared__ int s_pos[NMAX][3];
-  __shared__ int s_min[3], s_ushape[3];
+  __shared__ int s_pos[NMAX][3];      // member origins in union-box coordinates
+  __shared__ unsigned s_mask0[U0], s_mask1[U1], s_mask2[U2];
+  __shared__ unsigned s_list[64 * NMAX];
 
-  const int tid = threadIdx.x;
+  const int lane = threadIdx.x;
   const int64_t g = blockIdx.x;
   if (g >= a.nrefs) return;
   const int n = a.counts[g];
@@ -831,139 +848,166 @@ __global__ __launch_bounds__(NT) void k_exact_std(ExactArgs a) {
   const float *A0 = a.mats + 2 * gtot;
   const float *A1 = A0 + B0 * B0;
   const float *A2 = A1 + B1 * B1;
-  for (int t = tid; t < B01 * B01; t += NT) {
+  for (int t = lane; t < B01 * B01; t += 64) {
     const int p01 = t / B01, s01 = t - p01 * B01;
     const int p0 = p01 / B1, p1 = p01 - p0 * B1;
     const int s0 = s01 / B1, s1 = s01 - s0 * B1;
     s_t01[t] = A0[s0 * B0 + p0] * A1[s1 * B1 + p1];
   }
-  for (int t = tid; t < B2 * B2; t += NT) {
+  for (int t = lane; t < B2 * B2; t += 64) {
     const int p2 = t / B2, s2 = t - p2 * B2;
     s_a2[t] = A2[s2 * B2 + p2];
   }
-  if (tid < n) {
-    const int lin = a.sel[g * a.max_m + tid];
-    s_pos[tid][2] = lin % a.c2;
-    s_pos[tid][1] = (lin / a.c2) % a.c1;
-    s_pos[tid][0] = lin / (a.c2 * a.c1);
+  // member origins and the union box (min over members, extent max - min + b)
+  int o0 = 0x7fffffff, o1 = 0x7fffffff, o2 = 0x7fffffff;
+  int x0 = -1, x1 = -1, x2 = -1;
+  if (lane < n) {
+    const int lin = a.sel[g * a.max_m + lane];
+    o2 = x2 = lin % a.c2;
+    o1 = x1 = (lin / a.c2) % a.c1;
+    o0 = x0 = lin / (a.c2 * a.c1);
+  }
+  int mn0 = o0, mn1 = o1, mn2 = o2, mx0 = x0, mx1 = x1, mx2 = x2;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    mn0 = min(mn0, __shfl_xor(mn0, o, 64));
+    mn1 = min(mn1, __shfl_xor(mn1, o, 64));
+    mn2 = min(mn2, __shfl_xor(mn2, o, 64));
+    mx0 = max(mx0, __shfl_xor(mx0, o, 64));
+    mx1 = max(mx1, __shfl_xor(mx1, o, 64));
+    mx2 = max(mx2, __shfl_xor(mx2, o, 64));
+  }
+  const int m0 = __builtin_amdgcn_readfirstlane(mn0), m1 = __builtin_amdgcn_readfirstlane(mn1),
+            m2 = __builtin_amdgcn_readfirstlane(mn2);
+  const int ue0 = __builtin_amdgcn_readfirstlane(mx0) - m0 + B0;
+  const int ue1 = __builtin_amdgcn_readfirstlane(mx1) - m1 + B1;
+  const int ue2 = __builtin_amdgcn_readfirstlane(mx2) - m2 + B2;
+  if (lane < n) {
+    s_pos[lane][0] = x0 - m0;
+    s_pos[lane][1] = x1 - m1;
+    s_pos[lane][2] = x2 - m2;
   }
   __syncthreads();
-  if (tid < 3) {
-    int mn = s_pos[0][tid], mx = s_pos[0][tid];
-    for (int i = 1; i < n; ++i) {
-      mn = min(mn, s_pos[i][tid]);
-      mx = max(mx, s_pos[i][tid]);
+  // per-axis member masks: bit i of mask_d[t] is set when member i covers t
+  for (int t = lane; t < ue0 + ue1 + ue2; t += 64) {
+    const int d = t < ue0 ? 0 : (t < ue0 + ue1 ? 1 : 2);
+    const int tt = t - (d > 0 ? ue0 : 0) - (d > 1 ? ue1 : 0);
+    const int bd = d == 0 ? B0 : (d == 1 ? B1 : B2);
+    unsigned m = 0u;
+    for (int i = 0; i < n; ++i) {
+      const int lo = s_pos[i][d];
+      if (tt >= lo && tt < lo + bd) m |= 1u << i;
     }
-    const int bs[3] = {B0, B1, B2};
-    s_min[tid] = mn;
-    s_ushape[tid] = mx - mn + bs[tid];
+    if (d == 0) s_mask0[tt] = m;
+    else if (d == 1) s_mask1[tt] = m;
+    else s_mask2[tt] = m;
   }
   __syncthreads();
-  const int np = n * P;
-  const int u1 = s_ushape[1], u2 = s_ushape[2];
-  // entries keyed by (union label, member, patch offset), sorted: C order over
-  // union voxels, increasing member index inside a voxel
-  for (int e = tid; e < SORTN; e += NT) {
-    unsigned key = 0xffffffffu;
-    if (e < np) {
-      const int i = e / P, p = e - i * P;
-      const int p2 = p % B2, p1 = (p / B2) % B1, p0 = p / (B2 * B1);
-      const int l0 = s_pos[i][0] - s_min[0] + p0;
-      const int l1 = s_pos[i][1] - s_min[1] + p1;
-      const int l2 = s_pos[i][2] - s_min[2] + p2;
-      const unsigned label = (unsigned)((l0 * u1 + l1) * u2 + l2);
-      key = (label << 12) | (unsigned)((i << 7) | p);
+
+  int s01[SPL], s2[SPL];
+  bool act[SPL];
+#pragma unroll
+  for (int k = 0; k < SPL; ++k) {
+    const int s = lane + 64 * k;
+    act[k] = s < P;
+    s01[k] = act[k] ? s / B2 : 0;
+    s2[k] = act[k] ? s - (s / B2) * B2 : 0;
+  }
+  const int ub = ue0 * ue1 * ue2;
+  for (int q0 = 0; q0 < planes; q0 += 4) {
+    if (lane < n) {
+      float4 gv;
+      gv.x = gf[q0 * n + lane];
+      gv.y = (q0 + 1 < planes) ? gf[(q0 + 1) * n + lane] : 0.0f;
+      gv.z = (q0 + 2 < planes) ? gf[(q0 + 2) * n + lane] : 0.0f;
+      gv.w = (q0 + 3 < planes) ? gf[(q0 + 3) * n + lane] : 0.0f;
+      s_g4[lane] = gv;
     }
-    s_key[e] = key;
-  }
-  __syncthreads();
-  for (int size = 2; size <= SORTN; size <<= 1) {
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int t = tid; t < SORTN / 2; t += NT) {
-        const int lo = 2 * t - (t & (stride - 1));
-        const int hi = lo + stride;
-        const bool up = ((lo & size) == 0);
-        const unsigned x = s_key[lo], y = s_key[hi];
-        if ((x > y) == up) {
-          s_key[lo] = y;
-          s_key[hi] = x;
+    __syncthreads();
+    float acc[SPL][4], c[SPL][4];
+#pragma unroll
+    for (int k = 0; k < SPL; ++k)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[k][q] = c[k][q] = 0.0f;
+    float cur_vv = 0.0f;
+    for (int vb = 0; vb < ub; vb += 64) {
+      const int v = vb + lane;
+      unsigned m = 0u;
+      float vv = 0.0f;
+      int v0 = 0, v1 = 0, v2 = 0;
+      if (v < ub) {
+        v2 = v % ue2;
+        v1 = (v / ue2) % ue1;
+        v0 = v / (ue2 * ue1);
+        m = s_mask0[v0] & s_mask1[v1] & s_mask2[v2];
+        if (m) vv = a.var[((int64_t)(v0 + m0) * a.n1 + (v1 + m1)) * a.n2 + (v2 + m2)];
+      }
+      const int cnt = __popc(m);
+      int incl = cnt;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+      }
+      const int total = __builtin_amdgcn_readlane(incl, 63);
+      // this voxel's entries, members in increasing order; bit 31 marks the
+      // voxel's first entry, bits 21..26 its lane (for its variance)
+      int w = incl - cnt;
+      unsigned first = 0x80000000u;
+      unsigned mm = m;
+      while (mm) {
+        const int i = __ffs(mm) - 1;
+        mm &= mm - 1u;
+        const int q01 = (v0 - s_pos[i][0]) * B1 + (v1 - s_pos[i][1]);
+        const int q2 = v2 - s_pos[i][2];
+        s_list[w++] = (unsigned)(q01 * B01) | ((unsigned)(q2 * B2) << 10) | ((unsigned)i << 16) |
+                      ((unsigned)lane << 21) | first;
+        first = 0u;
+      }
+      __syncthreads();
+      for (int e0 = 0; e0 < total; e0 += 64) {
+        const unsigned code = (e0 + lane < total) ? s_list[e0 + lane] : 0u;
+        const int cn = min(64, total - e0);
+        for (int j = 0; j < cn; ++j) {
+          const unsigned cd = (unsigned)__builtin_amdgcn_readlane((int)code, j);
+          const unsigned ot = cd & 1023u, oa = (cd >> 10) & 63u;
+          const float4 gq = s_g4[(cd >> 16) & 31u];
+          float sv[SPL];
+#pragma unroll
+          for (int k = 0; k < SPL; ++k) sv[k] = s_t01[ot + s01[k]] * s_a2[oa + s2[k]];
+          if (cd & 0x80000000u) {
+            // close the previous union voxel (adds +0 before the first one)
+#pragma unroll
+            for (int k = 0; k < SPL; ++k)
+#pragma unroll
+              for (int q = 0; q < 4; ++q) {
+                acc[k][q] = acc[k][q] + (cur_vv * c[k][q]) * c[k][q];
+                c[k][q] = 0.0f;
+              }
+            cur_vv = __builtin_bit_cast(
+                float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, vv), (cd >> 21) & 63u));
+          }
+#pragma unroll
+          for (int k = 0; k < SPL; ++k) {
+            c[k][0] = c[k][0] + gq.x * sv[k];
+            c[k][1] = c[k][1] + gq.y * sv[k];
+            c[k][2] = c[k][2] + gq.z * sv[k];
+            c[k][3] = c[k][3] + gq.w * sv[k];
+          }
         }
       }
       __syncthreads();
     }
-  }
-  const int s = tid;
-  const bool active = s < P;
-  const int s01 = active ? s / B2 : 0, s2 = active ? s - (s / B2) * B2 : 0;
-  const int m0 = s_min[0], m1 = s_min[1], m2 = s_min[2];
-  const int lane = tid & 63;
-  for (int q0 = 0; q0 < planes; q0 += 4) {
-    for (int i = tid; i < n; i += NT) {
-      float4 gv;
-      gv.x = gf[q0 * n + i];
-      gv.y = (q0 + 1 < planes) ? gf[(q0 + 1) * n + i] : 0.0f;
-      gv.z = (q0 + 2 < planes) ? gf[(q0 + 2) * n + i] : 0.0f;
-      gv.w = (q0 + 3 < planes) ? gf[(q0 + 3) * n + i] : 0.0f;
-      s_g4[i] = gv;
-    }
-    __syncthreads();
-    float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f, acc3 = 0.0f;
-    float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f, c3 = 0.0f;
-    float cur_vv = 0.0f;
-    unsigned prev_label = 0xffffffffu;
-    for (int base = 0; base < np; base += 64) {
-      // Each lane decodes one entry of the chunk: its union label, whether it
-      // opens a new union voxel (bit 31 of the code), the LDS offsets of its
-      // spatial factors and its member; the walk below reads them back with
-      // readlane, so the serial path does no dependent LDS loads and tests
-      // the voxel change on the scalar unit.
-      const int e = base + lane;
-      const unsigned key = e < np ? s_key[e] : 0xffffffffu;
-      const unsigned label = key >> 12;
-      unsigned before = __shfl_up(label, 1, 64);
-      if (lane == 0) before = prev_label;
-      prev_label = (unsigned)__builtin_amdgcn_readlane((int)label, 63);
-      const int ei = (int)((key >> 7) & 31u), ep = (int)(key & 127u);
-      const int ep01 = ep / B2, ep2 = ep - ep01 * B2;
-      const unsigned code = (unsigned)(ep01 * B01) | ((unsigned)(ep2 * B2) << 12) |
-                            ((unsigned)ei << 20) | (label != before ? 0x80000000u : 0u);
-      float vv = 0.0f;
-      if (e < np) {
-        const int l2 = (int)(label % (unsigned)u2);
-        const int l1 = (int)((label / (unsigned)u2) % (unsigned)u1);
-        const int l0 = (int)(label / (unsigned)(u2 * u1));
-        vv = a.var[((int64_t)(l0 + m0) * a.n1 + (l1 + m1)) * a.n2 + (l2 + m2)];
-      }
-      const int cnt = __builtin_amdgcn_readfirstlane(min(64, np - base));
-      for (int j = 0; j < cnt; ++j) {
-        const unsigned cd = (unsigned)__builtin_amdgcn_readlane((int)code, j);
-        const float sv = s_t01[(cd & 4095u) + s01] * s_a2[((cd >> 12) & 255u) + s2];
-        const float4 gq = s_g4[(cd >> 20) & 31u];
-        if (cd & 0x80000000u) {
-          // close the previous union voxel (adds +0 before the first one)
-          acc0 = acc0 + (cur_vv * c0) * c0;
-          acc1 = acc1 + (cur_vv * c1) * c1;
-          acc2 = acc2 + (cur_vv * c2) * c2;
-          acc3 = acc3 + (cur_vv * c3) * c3;
-          cur_vv = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, vv), j));
-          c0 = c1 = c2 = c3 = 0.0f;
-        }
-        c0 = c0 + gq.x * sv;
-        c1 = c1 + gq.y * sv;
-        c2 = c2 + gq.z * sv;
-        c3 = c3 + gq.w * sv;
-      }
-    }
-    acc0 = acc0 + (cur_vv * c0) * c0;
-    acc1 = acc1 + (cur_vv * c1) * c1;
-    acc2 = acc2 + (cur_vv * c2) * c2;
-    acc3 = acc3 + (cur_vv * c3) * c3;
-    if (active) {
-      float *dst = a.exact_std + (g * a.kcap + q0) * P + s;
-      const float accs[4] = {acc0, acc1, acc2, acc3};
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if (q0 + j < planes) dst[j * P] = sqrtf(accs[j] > 0.0f ? accs[j] : 0.0f);
+    for (int k = 0; k < SPL; ++k) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[k][q] = acc[k][q] + (cur_vv * c[k][q]) * c[k][q];
+      if (act[k]) {
+        float *dst = a.exact_std + (g * a.kcap + q0) * P + lane + 64 * k;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (q0 + q < planes) dst[q * P] = sqrtf(acc[k][q] > 0.0f ? acc[k][q] : 0.0f);
+      }
     }
     __syncthreads();
   }
@@ -1717,6 +1761,8 @@ int32_t bmnd_hip_blockmatch(const float *vol, const int64_t *shape_h, const int3
   a.logtab = tab;
   a.fastlog = flog;
   a.certlog = clog;
+  a.cert_c3 = 1.0 / 3.0;
+  a.cert_c4 = -0.25;
   a.term_table = terms;
   a.tmax = table_max;
   a.sel = sel;
@@ -1782,15 +1828,15 @@ int32_t bmnd_hip_exact_std(const float *var, const int64_t *shape_h, const int32
   a.exact_std = exact_std;
   hipStream_t s = (hipStream_t)stream;
   dim3 grid((unsigned)nrefs);
-#define E_CASE(b0, b1, b2, r0, r1, r2, nmax, nt)                                             \
+#define E_CASE(b0, b1, b2, r0, r1, r2, nmax)                                                 \
   if (same3(block_h, b0, b1, b2) && same3(radius_h, r0, r1, r2) && max_matches <= nmax) {    \
-    hipLaunchKernelGGL((k_exact_std<b0, b1, b2, nmax, nt>), grid, dim3(nt), 0, s, a);       \
+    hipLaunchKernelGGL((k_exact_std<b0, b1, b2, r0, r1, r2, nmax>), grid, dim3(64), 0, s, a); \
     HIP_TRY(hipGetLastError());                                                             \
     return 0;                                                                               \
   }
-  E_CASE(1, 8, 8, 0, 19, 19, 32, 64)
-  E_CASE(4, 4, 4, 7, 7, 7, 16, 64)
-  E_CASE(5, 5, 5, 7, 7, 7, 32, 128)
+  E_CASE(1, 8, 8, 0, 19, 19, 32)
+  E_CASE(4, 4, 4, 7, 7, 7, 16)
+  E_CASE(5, 5, 5, 7, 7, 7, 32)
 #undef E_CASE
   return fail(BMND_HIP_EINVAL, "block/radius/stack configuration not compiled into this library");
 }
