@@ -33,7 +33,7 @@ PEAK_HBM_GBS = 8000.0
 FLOPS_PER_TERM = 10
 # workgroup size of the block-matching kernel per deviance mode (its last
 # template argument, part of the kernel name rocprofv3 reports)
-BM_THREADS = {0: 512, 1: 1024, 2: 512, 3: 512}
+BM_THREADS = {0: 512, 1: 1024, 2: 512, 3: 512, 4: 512}
 # stack capacity of the exact-variance kernel per block
 EXACT_CONFIG = {(1, 8, 8): 32, (4, 4, 4): 16, (5, 5, 5): 32}
 
@@ -50,9 +50,9 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB mebibytes (for scaling "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE readings) and exit")
-    ap.add_argument("--deviance", default="auto-certified",
+    ap.add_argument("--deviance", default="auto-fast",
                     choices=["auto", "exact", "one-log", "auto-one-log", "certified",
-                             "auto-certified"],
+                             "auto-certified", "fast", "auto-fast"],
                     help="block-matching deviance evaluation (integer counts use the "
                          "term table; other inputs the certified one-log form, whose keys "
                          "equal the reference expression's, or with 'auto' the reference's "
@@ -177,9 +177,9 @@ def main() -> int:
         # block matching: the reference's deviance expression per term
         pairs, terms = blockmatch_work(plan, noisy_list[0].shape)
         mode = pipe.last_modes.get(st, 0)
-        if mode == 3:
-            kname = "k_blockmatch_cert<{}, {}, {}, {}, {}, {}, {}>(BMArgs)".format(
-                *plan.block3, *plan.radius3, BM_THREADS[3])
+        if mode in (3, 4):
+            kname = "k_blockmatch_{}<{}, {}, {}, {}, {}, {}, {}>(BMArgs)".format(
+                "cert" if mode == 3 else "fast", *plan.block3, *plan.radius3, BM_THREADS[mode])
         else:
             kname = "k_blockmatch<{}, {}, {}, {}, {}, {}, {}, {}>(BMArgs)".format(
                 *plan.block3, *plan.radius3, mode, BM_THREADS[mode])

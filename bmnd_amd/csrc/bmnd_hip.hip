@@ -308,35 +308,36 @@ __device__ __forceinline__ unsigned long long bm_composite(float key, bool ok, i
          ((unsigned long long)__float_as_uint(key) << 32) | (unsigned long long)cand;
 }
 
-// Selection shared by the block-matching kernels: from the box composites
-// (unique) and the accepted count, keep the K smallest the reference keeps,
-// order them and write the group of reference g: reference first, dedupe,
-// cap, power-of-two truncation (blockmatching.py:913-954, 1045-1205).
-template <int NT, int BOX>
-__device__ __forceinline__ void bm_select_write(const unsigned long long *__restrict__ keys,
-                                                int box, int acc, const BMArgs &a, int64_t g,
-                                                int lo0, int lo1, int lo2, int e1, int e2,
-                                                int center_local) {
-  __shared__ int s_hist[256];
-  __shared__ unsigned long long s_small[kMaxStack];
-  __shared__ unsigned long long s_sorted[kMaxStack];
-  __shared__ unsigned long long s_prefix;
-  __shared__ int s_krem, s_cnt, s_done;
-  const int tid = threadIdx.x;
-  const int req = min(a.min_m, a.max_m);
-  int K;
-  if (acc >= a.max_m)
-    K = a.max_m;
-  else if (acc >= req)
-    K = acc;
-  else
-    K = min(req, box);
+// LDS scratch of the selection routines (declared by each kernel)
+struct BMSelScratch {
+  int *hist;                    // 256 bins
+  unsigned long long *small;    // kMaxStack
+  unsigned long long *sorted;   // kMaxStack
+  unsigned long long *prefix;   // 1
+  int *krem, *cnt, *done;       // 1 each
+};
 
-  // Radix select of the K smallest composites.  The candidate index is below
-  // 2^16, so bits 16..31 are zero in every key and their passes are skipped;
-  // once the bin holding the K-th key is needed in full, every key whose
-  // resolved bits are <= the prefix is selected.
+// K as the reference keeps it: the accepted top-max, else every accepted
+// candidate, else the required minimum (blockmatching.py:1045-1205)
+__device__ __forceinline__ int bm_keep_count(int acc, int box, const BMArgs &a) {
+  const int req = min(a.min_m, a.max_m);
+  if (acc >= a.max_m) return a.max_m;
+  if (acc >= req) return acc;
+  return min(req, box);
+}
+
+// Radix select over the unique composites key_of(i), i < box: on return the K
+// smallest are exactly those with (key & mask) <= prefix.  The candidate
+// index is below 2^16, so bits 16..31 are zero in every key and their passes
+// are skipped; once the bin holding the K-th key is needed in full the
+// select stops with the bits resolved so far.
+template <int NT, int BOX, class KeyOf>
+__device__ __forceinline__ void bm_radix_select(KeyOf key_of, int box, int K,
+                                                const BMSelScratch &sc,
+                                                unsigned long long &prefix_out,
+                                                unsigned long long &mask_out) {
   static_assert(BOX < 65536, "candidate index must fit 16 bits");
+  const int tid = threadIdx.x;
   unsigned long long prefix = 0ull, mask = 0ull;
   int krem = K;
   for (int shift = 56; shift >= 0; shift -= 8) {
@@ -344,16 +345,16 @@ __device__ __forceinline__ void bm_select_write(const unsigned long long *__rest
       mask |= 255ull << shift;
       continue;
     }
-    for (int i = tid; i < 256; i += NT) s_hist[i] = 0;
+    for (int i = tid; i < 256; i += NT) sc.hist[i] = 0;
     __syncthreads();
     for (int i = tid; i < box; i += NT) {
-      const unsigned long long k = keys[i];
-      if ((k & mask) == prefix) atomicAdd(&s_hist[(int)((k >> shift) & 255ull)], 1);
+      const unsigned long long k = key_of(i);
+      if ((k & mask) == prefix) atomicAdd(&sc.hist[(int)((k >> shift) & 255ull)], 1);
     }
     __syncthreads();
     if (tid < 64) {
-      const int c0 = s_hist[4 * tid], c1 = s_hist[4 * tid + 1], c2 = s_hist[4 * tid + 2],
-                c3 = s_hist[4 * tid + 3];
+      const int c0 = sc.hist[4 * tid], c1 = sc.hist[4 * tid + 1], c2 = sc.hist[4 * tid + 2],
+                c3 = sc.hist[4 * tid + 3];
       const int tot = c0 + c1 + c2 + c3;
       int incl = tot;
 #pragma unroll
@@ -366,33 +367,49 @@ __device__ __forceinline__ void bm_select_write(const unsigned long long *__rest
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
         if (ex < krem && ex + cs[b] >= krem) {
-          s_prefix = prefix | ((unsigned long long)(4 * tid + b) << shift);
-          s_krem = krem - ex;
-          s_done = (ex + cs[b] == krem) ? 1 : 0;
+          *sc.prefix = prefix | ((unsigned long long)(4 * tid + b) << shift);
+          *sc.krem = krem - ex;
+          *sc.done = (ex + cs[b] == krem) ? 1 : 0;
         }
         ex += cs[b];
       }
     }
     __syncthreads();
-    prefix = s_prefix;
-    krem = s_krem;
+    prefix = *sc.prefix;
+    krem = *sc.krem;
     mask |= 255ull << shift;
-    const int done = s_done;
+    const int done = *sc.done;
     __syncthreads();
     if (done) break;
   }
-  if (tid == 0) s_cnt = 0;
+  prefix_out = prefix;
+  mask_out = mask;
+}
+
+// Selection shared by the block-matching kernels: from the box composites
+// (unique) and the accepted count, keep the K smallest the reference keeps,
+// order them and write the group of reference g: reference first, dedupe,
+// cap, power-of-two truncation (blockmatching.py:913-954, 1045-1205).
+template <int NT, int BOX, class KeyOf>
+__device__ __forceinline__ void bm_select_write(KeyOf key_of, int box, int acc, const BMArgs &a,
+                                                int64_t g, int lo0, int lo1, int lo2, int e1,
+                                                int e2, int center_local, const BMSelScratch &sc) {
+  const int tid = threadIdx.x;
+  const int K = bm_keep_count(acc, box, a);
+  unsigned long long prefix, mask;
+  bm_radix_select<NT, BOX>(key_of, box, K, sc, prefix, mask);
+  if (tid == 0) *sc.cnt = 0;
   __syncthreads();
   for (int i = tid; i < box; i += NT) {
-    const unsigned long long k = keys[i];
-    if ((k & mask) <= prefix) s_small[atomicAdd(&s_cnt, 1)] = k;
+    const unsigned long long k = key_of(i);
+    if ((k & mask) <= prefix) sc.small[atomicAdd(sc.cnt, 1)] = k;
   }
   __syncthreads();
   if (tid < K) {
-    const unsigned long long mine = s_small[tid];
+    const unsigned long long mine = sc.small[tid];
     int rank = 0;
-    for (int j = 0; j < K; ++j) rank += s_small[j] < mine ? 1 : 0;
-    s_sorted[rank] = mine;
+    for (int j = 0; j < K; ++j) rank += sc.small[j] < mine ? 1 : 0;
+    sc.sorted[rank] = mine;
   }
   __syncthreads();
   if (tid < 64) {
@@ -401,7 +418,7 @@ __device__ __forceinline__ void bm_select_write(const unsigned long long *__rest
     // at max, then power-of-two truncation; lane i holds the i-th smallest
     const int lane = tid;
     const bool valid = lane < K;
-    const int loc = valid ? (int)(s_sorted[lane] & 0xffffffffull) : -1;
+    const int loc = valid ? (int)(sc.sorted[lane] & 0xffffffffull) : -1;
     const bool keep = valid && loc != center_local;
     const unsigned long long kept = __ballot(keep);
     const int pos = 1 + __popcll(kept & ((1ull << lane) - 1ull));
@@ -426,6 +443,15 @@ __device__ __forceinline__ void bm_select_write(const unsigned long long *__rest
     }
   }
 }
+
+// declares the selection scratch in the calling kernel
+#define BM_SEL_SCRATCH(name)                                                         \
+  __shared__ int name##_hist[256];                                                   \
+  __shared__ unsigned long long name##_small[kMaxStack], name##_sorted[kMaxStack];   \
+  __shared__ unsigned long long name##_prefix;                                       \
+  __shared__ int name##_krem, name##_cnt, name##_done;                               \
+  const BMSelScratch name{name##_hist,    name##_small, name##_sorted, &name##_prefix, \
+                          &name##_krem, &name##_cnt, &name##_done}
 
 // MODE 0: each term as the reference writes it (two divisions, two logs);
 // MODE 1: one log per term, r log 2r + c log 2c - s log s (values clamped
@@ -556,7 +582,9 @@ __global__ __launch_bounds__(NT) void k_blockmatch(BMArgs a) {
   }
   if (my_acc) atomicAdd(&s_acc, my_acc);
   __syncthreads();
-  bm_select_write<NT, BOX>(s_keys, box, s_acc, a, g, lo0, lo1, lo2, e1, e2, center_local);
+  BM_SEL_SCRATCH(sc);
+  bm_select_write<NT, BOX>([&](int i) { return s_keys[i]; }, box, s_acc, a, g, lo0, lo1, lo2, e1,
+                           e2, center_local, sc);
 }
 
 // Certified block matching (keys equal to MODE 0's).  The distance is
@@ -762,7 +790,322 @@ __global__ __launch_bounds__(NT) void k_blockmatch_cert(BMArgs a) {
   }
   if (my_acc) atomicAdd(&s_acc, my_acc);
   __syncthreads();
-  bm_select_write<NT, BOX>(keys, box, s_acc, a, g, lo0, lo1, lo2, e1, e2, center_local);
+  BM_SEL_SCRATCH(sc);
+  bm_select_write<NT, BOX>([&](int i) { return keys[i]; }, box, s_acc, a, g, lo0, lo1, lo2, e1,
+                           e2, center_local, sc);
+}
+
+// Bound on the hardware base-2 logarithm that the fast matcher relies on:
+// |v_log_f32(x) - log2 x| <= kLog2Tol (|log2 x| + 1) for every normal float
+// x >= kLog2Floor.  k_log2_check measures it on the device over all of them;
+// the fast matcher runs only where the measurement is within the bound.
+constexpr double kLog2Tol = 0x1p-20;
+constexpr float kLog2Floor = 0x1p-100f;
+constexpr int kBMThreadsFast = 512;  // two workgroups per CU
+constexpr int kFastListCap = 512;
+
+// max over x of |v_log_f32(x) - log2 x| / (|log2 x| + 1), x over every normal
+// float in [kLog2Floor, inf), log2 x from the float64 dlog; written to *out as
+// the bits of a non-negative double (ordered like unsigned integers)
+__global__ void k_log2_check(const double *__restrict__ tab, unsigned long long *out) {
+  __shared__ double s_tab[kLogTabDoubles];
+  for (int i = threadIdx.x; i < kLogTabDoubles; i += blockDim.x) s_tab[i] = tab[i];
+  __syncthreads();
+  const double inv_ln2 = 1.0 / (s_tab[3 * kLogTabN] + s_tab[3 * kLogTabN + 1]);
+  const unsigned lo = __float_as_uint(kLog2Floor), hi = 0x7f800000u;
+  double worst = 0.0;
+  for (unsigned b = lo + blockIdx.x * blockDim.x + threadIdx.x; b < hi;
+       b += gridDim.x * blockDim.x) {
+    const float x = __uint_as_float(b);
+    const double ref = dlog((double)x, s_tab) * inv_ln2;
+    const double err = fabs((double)__builtin_amdgcn_logf(x) - ref) / (fabs(ref) + 1.0);
+    worst = fmax(worst, err == err ? err : 1.0);  // a NaN result fails the check
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) worst = fmax(worst, __shfl_xor(worst, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(worst));
+}
+
+// Fast certified block matching (keys equal to MODE 0's).  A float pass gives
+// every candidate an interval [key(d - Ef), key(d + Ef)] from
+//   d = 2 (sum_p r log 2r + sum_p c log 2c - ln2 sum_p s log2 s),  s = r + c,
+// with s log2 s in float32 (hardware log2, rows of B2 terms summed in float
+// and accumulated in float64) and the other sums in float64 (degree-4 log,
+// window sums).  The K_max-th smallest upper-bound composite (K_max the keep
+// count at the largest possible accepted count) bounds the K-th smallest
+// true composite, so only candidates whose lower-bound composite is at most
+// that cutoff, or whose interval straddles the threshold, can change the
+// result; those get their key from the certified float64 evaluation (and the
+// reference expression where its bound leaves the key open).  Every other
+// candidate keeps its lower key, whose threshold decision is already known.
+template <int B0, int B1, int B2, int R0, int R1, int R2, int NT>
+__global__ __launch_bounds__(NT) void k_blockmatch_fast(BMArgs a) {
+  constexpr int S0 = 2 * R0 + B0, S1 = 2 * R1 + B1, S2 = 2 * R2 + B2;
+  constexpr int SLAB = S0 * S1 * S2;
+  constexpr int BOX = (2 * R0 + 1) * (2 * R1 + 1) * (2 * R2 + 1);
+  constexpr int P = B0 * B1 * B2;
+  constexpr int W2 = S2 - B2 + 1;         // window positions along axis 2
+  constexpr int XS = S0 * S1 * W2;        // axis-2 window sums of c log 2c
+  constexpr int KC = (BOX + NT - 1) / NT;  // candidates per thread
+  static_assert(BOX <= XS, "the key pairs reuse the window-sum buffer");
+  static_assert(KC <= 32, "per-thread candidate mask");
+  __shared__ float s_slab[SLAB];           // max(x, 0)
+  __shared__ double s_xsum[XS];            // window sums, then per-candidate sums, then key pairs
+  __shared__ float s_reff[P];              // max(r, 0)
+  __shared__ double s_tab[kLogTabDoubles];
+  __shared__ double2 s_clog[kFastLogN];    // degree-4 log
+  __shared__ double s_rpart[(P + 63) / 64];
+  __shared__ double s_rsum, s_errc, s_errf;
+  __shared__ unsigned s_vmax[2];
+  __shared__ int s_acc, s_maybe;
+  __shared__ int s_scan[NT / 64 + 1];
+  __shared__ unsigned short s_list[kFastListCap];
+  BM_SEL_SCRATCH(sc);
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int64_t g = blockIdx.x;
+  if (g >= a.nrefs) return;
+  const int lin = a.ref_lin[g];
+  const int o2 = lin % a.c2, o1 = (lin / a.c2) % a.c1, o0 = lin / (a.c2 * a.c1);
+  const int lo0 = max(o0 - R0, 0), hi0 = min(o0 + R0, a.c0 - 1);
+  const int lo1 = max(o1 - R1, 0), hi1 = min(o1 + R1, a.c1 - 1);
+  const int lo2 = max(o2 - R2, 0), hi2 = min(o2 + R2, a.c2 - 1);
+  const int e0 = hi0 - lo0 + 1, e1 = hi1 - lo1 + 1, e2 = hi2 - lo2 + 1;
+  const int box = e0 * e1 * e2;
+
+  for (int i = tid; i < kLogTabDoubles; i += NT) s_tab[i] = a.logtab[i];
+  for (int i = tid; i < kFastLogN; i += NT) s_clog[i] = a.certlog[i];
+  if (tid < 2) s_vmax[tid] = 0u;
+  if (tid == 0) {
+    s_acc = 0;
+    s_maybe = 0;
+  }
+  __syncthreads();
+  const int x0 = e0 + B0 - 1, x1 = e1 + B1 - 1, x2 = e2 + B2 - 1;
+  float vmax = 0.0f;
+  {
+    constexpr int IT = (SLAB + NT - 1) / NT;
+    float vv[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int i = tid + it * NT;
+      const int i2 = i % S2, i1 = (i / S2) % S1, i0 = i / (S2 * S1);
+      vv[it] = 0.0f;
+      if (i < SLAB && i0 < x0 && i1 < x1 && i2 < x2)
+        vv[it] = a.vol[((int64_t)(lo0 + i0) * a.n1 + (lo1 + i1)) * a.n2 + (lo2 + i2)];
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int i = tid + it * NT;
+      const float cf = vv[it] > 0.0f ? vv[it] : 0.0f;
+      vmax = fmaxf(vmax, cf);
+      if (i < SLAB) s_slab[i] = cf;
+    }
+  }
+  float rmax = 0.0f;
+  for (int p = tid; p < P; p += NT) {
+    const int p2 = p % B2, p1 = (p / B2) % B1, p0 = p / (B2 * B1);
+    const float v = a.vol[((int64_t)(o0 + p0) * a.n1 + (o1 + p1)) * a.n2 + (o2 + p2)];
+    const float rf = v > 0.0f ? v : 0.0f;
+    rmax = fmaxf(rmax, rf);
+    s_reff[p] = rf;
+  }
+  // non-negative floats order like their bit patterns
+  if (vmax > 0.0f) atomicMax(&s_vmax[1], __float_as_uint(vmax));
+  if (rmax > 0.0f) atomicMax(&s_vmax[0], __float_as_uint(rmax));
+  __syncthreads();
+
+  const double ln2 = s_tab[3 * kLogTabN] + s_tab[3 * kLogTabN + 1];
+  const double c3 = a.cert_c3, c4 = a.cert_c4;
+  // sum of r log 2r over the reference patch (fixed-order wave reduction)
+  if (tid < 64 * ((P + 63) / 64)) {
+    double rl = 0.0;
+    if (tid < P) {
+      const double r = (double)s_reff[tid];
+      rl = r > 0.0 ? r * log_deg4(2.0 * r, s_clog, ln2, c3, c4) : 0.0;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) rl = rl + __shfl_xor(rl, o, 64);
+    if (lane == 0) s_rpart[tid >> 6] = rl;
+  }
+  // axis-2 window sums of c log 2c, one slab row per thread
+  for (int row = tid; row < S0 * S1; row += NT) {
+    const float *src = s_slab + row * S2;
+    double w[B2];
+#pragma unroll
+    for (int t = 0; t < B2; ++t) w[t] = 0.0;
+#pragma unroll
+    for (int i2 = 0; i2 < S2; ++i2) {
+#pragma unroll
+      for (int t = 0; t + 1 < B2; ++t) w[t] = w[t + 1];
+      const double c = (double)src[i2];
+      w[B2 - 1] = c > 0.0 ? c * log_deg4(2.0 * c, s_clog, ln2, c3, c4) : 0.0;
+      if (i2 >= B2 - 1) {
+        double sum = w[0];
+#pragma unroll
+        for (int t = 1; t < B2; ++t) sum = sum + w[t];
+        s_xsum[row * W2 + (i2 - (B2 - 1))] = sum;
+      }
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double rs = 0.0;
+    for (int w = 0; w < (P + 63) / 64; ++w) rs = rs + s_rpart[w];
+    s_rsum = rs;
+    // Bounds on |d - d_ref| for every candidate of this reference, with
+    // S >= s = r + c over the slab.  Float64 evaluation (the terms of
+    // r log 2r, c log 2c and s log s with the degree-4 log, truncation
+    // < 1.9e-13 * x per log): each path's rounding error is at most a few
+    // hundred units of 2^-53 times P*W, W >= s (|ln s| + 1) + 2, for any
+    // summation order; the constants keep margins of about 7x and 2x.
+    // Float pass, added for sum_p s log2 s: per term the hardware log bound
+    // (kLog2Tol, twice), the rounding of s, of the product and of the float
+    // row sums come to at most 2.5e-6 * s (|log2 s| + 2) <= 2.5e-6 * W2,
+    // W2 = S (|log2 S| + 2) + 3; kept with a 2x margin and scaled by 2 ln2.
+    const double S = (double)__uint_as_float(s_vmax[0]) + (double)__uint_as_float(s_vmax[1]);
+    const double lS = S > 0.0 ? fabs(dlog(S, s_tab)) : 0.0;
+    const double Wn = S * (lS + 1.0) + 2.0;
+    const double Wb = S * (lS / ln2 + 2.0) + 3.0;
+    const double ec = 2.0 * (double)P * (S * 8e-13 + 4096.0 * 0x1p-53 * Wn);
+    s_errc = ec;
+    s_errf = ec + 2.0 * ln2 * (double)P * (5e-6 * Wb);
+  }
+  // patch sums of c log 2c per candidate, parked in the slot of its key pair
+  double zc[KC];
+#pragma unroll
+  for (int k = 0; k < KC; ++k) {
+    const int cand = tid + k * NT;
+    zc[k] = 0.0;
+    if (cand < box) {
+      const int j2 = cand % e2, j1 = (cand / e2) % e1, j0 = cand / (e2 * e1);
+      double z = 0.0;
+#pragma unroll
+      for (int b0 = 0; b0 < B0; ++b0)
+#pragma unroll
+        for (int b1 = 0; b1 < B1; ++b1) z = z + s_xsum[((j0 + b0) * S1 + (j1 + b1)) * W2 + j2];
+      zc[k] = z;
+    }
+  }
+  __syncthreads();
+  double *zslot = s_xsum;
+  float2 *kp = reinterpret_cast<float2 *>(s_xsum);  // (lower key, upper key) per candidate
+#pragma unroll
+  for (int k = 0; k < KC; ++k) {
+    const int cand = tid + k * NT;
+    if (cand < box) zslot[cand] = zc[k];
+  }
+  const double rsum = s_rsum, errc = s_errc, errf = s_errf;
+  const float thr = a.stabilize ? a.thr_key : a.threshold;
+  const int rel0 = o0 - lo0, rel1 = o1 - lo1, rel2 = o2 - lo2;
+  const int center_local = (rel0 * e1 + rel1) * e2 + rel2;
+
+  // ---- float pass ------------------------------------------------------------
+  int my_maybe = 0;
+  for (int cand = tid; cand < box; cand += NT) {
+    const int j2 = cand % e2, j1 = (cand / e2) % e1, j0 = cand / (e2 * e1);
+    const int base = (j0 * S1 + j1) * S2 + j2;
+    double F = 0.0;
+#pragma unroll 1
+    for (int p01 = 0; p01 < B0 * B1; ++p01) {
+      const int p0 = p01 / B1, p1 = p01 - p0 * B1;
+      const float *row = s_slab + base + (p0 * S1 + p1) * S2;
+      const float *rrow = s_reff + p01 * B2;
+      float rs = 0.0f;
+#pragma unroll
+      for (int p2 = 0; p2 < B2; ++p2) {
+        const float s = rrow[p2] + row[p2];
+        rs = rs + s * __builtin_amdgcn_logf(fmaxf(s, kLog2Floor));
+      }
+      F = F + (double)rs;
+    }
+    const double df = 2.0 * (rsum + (zslot[cand] - ln2 * F));
+    bool ok_lo, ok_hi;
+    const float klo = bm_key<P>(df - errf, a, ok_lo);
+    const float khi = bm_key<P>(df + errf, a, ok_hi);
+    kp[cand] = make_float2(klo, khi);
+    my_maybe += ok_lo ? 1 : 0;
+  }
+  if (my_maybe) atomicAdd(&s_maybe, my_maybe);
+  __syncthreads();
+
+  // ---- cutoff: the K_max-th smallest upper-bound composite --------------------
+  auto comp = [&](float key, int i) {
+    if (key == 0.0f) key = 0.0f;  // a signed zero sorts as +0
+    return ((unsigned long long)(key < thr ? 0u : 1u) << 63) |
+           ((unsigned long long)__float_as_uint(key) << 32) | (unsigned long long)i;
+  };
+  const int kmax = bm_keep_count(s_maybe, box, a);
+  unsigned long long cpre, cmask;
+  bm_radix_select<NT, BOX>([&](int i) { return comp(kp[i].y, i); }, box, kmax, sc, cpre, cmask);
+  const unsigned long long cut = cpre | ~cmask;
+  unsigned need = 0u;
+#pragma unroll
+  for (int k = 0; k < KC; ++k) {
+    const int cand = tid + k * NT;
+    if (cand < box) {
+      const float2 kk = kp[cand];
+      if (comp(kk.x, cand) <= cut || ((kk.x < thr) != (kk.y < thr))) need |= 1u << k;
+    }
+  }
+  int total;
+  const int off = block_exclusive_scan<NT>(__popc(need), s_scan, &total);
+
+  // ---- exact keys for the candidates that can change the result ---------------
+  for (int lbase = 0; lbase < total; lbase += kFastListCap) {
+    int r = off;
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      if (need & (1u << k)) {
+        if (r >= lbase && r < lbase + kFastListCap) s_list[r - lbase] = (unsigned short)(tid + k * NT);
+        ++r;
+      }
+    }
+    __syncthreads();
+    const int m = min(kFastListCap, total - lbase);
+    for (int e = tid; e < m; e += NT) {
+      const int cand = s_list[e];
+      const int j2 = cand % e2, j1 = (cand / e2) % e1, j0 = cand / (e2 * e1);
+      const int base = (j0 * S1 + j1) * S2 + j2;
+      double t = 0.0;
+      for (int p01 = 0; p01 < B0 * B1; ++p01) {
+        const int p0 = p01 / B1, p1 = p01 - p0 * B1;
+        const float *row = s_slab + base + (p0 * S1 + p1) * S2;
+        for (int p2 = 0; p2 < B2; ++p2) {
+          const double c = (double)row[p2];
+          const double s = (double)s_reff[p01 * B2 + p2] + c;
+          const double lc = c > 0.0 ? c * log_deg4(2.0 * c, s_clog, ln2, c3, c4) : 0.0;
+          t = t + fma(-s, log_deg4(s, s_clog, ln2, c3, c4), lc);
+        }
+      }
+      const double dc = 2.0 * (rsum + t);
+      bool ok_lo, ok_hi, ok;
+      const float k_lo = bm_key<P>(dc - errc, a, ok_lo);
+      const float k_hi = bm_key<P>(dc + errc, a, ok_hi);
+      float key = k_lo;
+      if (k_lo != k_hi) {
+        double d = 0.0;
+        for (int p01 = 0; p01 < B0 * B1; ++p01) {
+          const int p0 = p01 / B1, p1 = p01 - p0 * B1;
+          const float *row = s_slab + base + (p0 * S1 + p1) * S2;
+          for (int p2 = 0; p2 < B2; ++p2)
+            d = bm_reference_term((double)s_reff[p01 * B2 + p2], (double)row[p2], d, s_tab);
+        }
+        key = bm_key<P>(d, a, ok);
+      }
+      kp[cand].x = key;
+    }
+    __syncthreads();
+  }
+  // final keys: exact where computed, else the lower key (its threshold side
+  // is known and its composite is above the cutoff)
+  int my_acc = 0;
+  for (int cand = tid; cand < box; cand += NT) my_acc += kp[cand].x < thr ? 1 : 0;
+  if (my_acc) atomicAdd(&s_acc, my_acc);
+  __syncthreads();
+  bm_select_write<NT, BOX>([&](int i) { return comp(kp[i].x, i); }, box, s_acc, a, g, lo0, lo1,
+                           lo2, e1, e2, center_local, sc);
 }
 
 // ---------------------------------------------------------------------------
@@ -1652,6 +1995,39 @@ int fastlog_for_device(const double2 **out, int certified = 0) {
   return 0;
 }
 
+std::vector<double> g_log2_ratio;  // per device: measured hardware log2 error ratio, -1 unknown
+
+// Measures v_log_f32 against the float64 log over every normal float from
+// kLog2Floor up (about 2e9 values, a few milliseconds) once per device.
+int log2_ratio_for_device(double *ratio) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  {
+    std::lock_guard<std::mutex> lock(g_tab_mutex);
+    if ((int)g_log2_ratio.size() <= dev) g_log2_ratio.resize(dev + 1, -1.0);
+    if (g_log2_ratio[dev] >= 0.0) {
+      *ratio = g_log2_ratio[dev];
+      return 0;
+    }
+  }
+  const double *tab = nullptr;
+  if (int rc = logtab_for_device(&tab)) return rc;
+  unsigned long long *d = nullptr;
+  HIP_TRY(hipMalloc(&d, sizeof(unsigned long long)));
+  HIP_TRY(hipMemset(d, 0, sizeof(unsigned long long)));
+  hipLaunchKernelGGL(k_log2_check, dim3(4096), dim3(256), 0, 0, tab, d);
+  HIP_TRY(hipGetLastError());
+  unsigned long long bits = 0;
+  HIP_TRY(hipMemcpy(&bits, d, sizeof(bits), hipMemcpyDeviceToHost));
+  HIP_TRY(hipFree(d));
+  double r;
+  std::memcpy(&r, &bits, sizeof(r));
+  std::lock_guard<std::mutex> lock(g_tab_mutex);
+  g_log2_ratio[dev] = r;
+  *ratio = r;
+  return 0;
+}
+
 float host_canonical_key(float d) {
   double mag = std::fabs((double)d);
   if (mag < 1.0) mag = 1.0;
@@ -1701,6 +2077,11 @@ int32_t bmnd_hip_variance_volume(const float *vol, float *out, int64_t n, double
   return 0;
 }
 
+int32_t bmnd_hip_log2_check(double *max_ratio) {
+  if (!max_ratio) return fail(BMND_HIP_EINVAL, "max_ratio must not be null");
+  return log2_ratio_for_device(max_ratio);
+}
+
 int32_t bmnd_hip_count_range(const float *vol, int64_t n, int32_t *flags, void *stream) {
   hipStream_t s = (hipStream_t)stream;
   const int32_t init[2] = {1, 0};
@@ -1726,20 +2107,26 @@ int32_t bmnd_hip_blockmatch(const float *vol, const int64_t *shape_h, const int3
     return fail(BMND_HIP_EINVAL, "count_scale must be finite and positive");
   if (nrefs <= 0) return 0;
   if (nrefs > INT32_MAX) return fail(BMND_HIP_EINVAL, "too many references");
-  if (mode < 0 || mode > 3) return fail(BMND_HIP_EINVAL, "mode must be 0, 1, 2 or 3");
+  if (mode < 0 || mode > 4) return fail(BMND_HIP_EINVAL, "mode must be 0, 1, 2, 3 or 4");
   if (mode == 2 && (count_scale != 1.0 || table_max < 0 || table_max > kMaxTableCount))
     return fail(BMND_HIP_EINVAL, "table mode needs count_scale == 1 and 0 <= table_max <= 100");
   const double *tab = nullptr;
   if (int rc = logtab_for_device(&tab)) return rc;
   // the one-log slabs assume unit count scale
-  if ((mode == 1 || mode == 3) && count_scale != 1.0) mode = 0;
+  if ((mode == 1 || mode == 3 || mode == 4) && count_scale != 1.0) mode = 0;
+  if (mode == 4) {
+    // the float pass needs the hardware log2 within its bound on this device
+    double ratio = 1.0;
+    if (int rc = log2_ratio_for_device(&ratio)) return rc;
+    if (!(ratio <= kLog2Tol)) mode = 3;
+  }
   const double *terms = nullptr;
   if (mode == 2)
     if (int rc = term_table_for_device(table_max, &terms)) return rc;
   const double2 *flog = nullptr, *clog = nullptr;
   if (mode == 1 || mode == 3)
     if (int rc = fastlog_for_device(&flog)) return rc;
-  if (mode == 3)
+  if (mode == 3 || mode == 4)
     if (int rc = fastlog_for_device(&clog, 1)) return rc;
   BMArgs a;
   a.vol = vol;
@@ -1782,6 +2169,9 @@ int32_t bmnd_hip_blockmatch(const float *vol, const int64_t *shape_h, const int3
     else if (mode == 3)                                                                          \
       hipLaunchKernelGGL((k_blockmatch_cert<b0, b1, b2, r0, r1, r2, kBMThreadsCert>), grid,      \
                          dim3(kBMThreadsCert), 0, s, a);                                         \
+    else if (mode == 4)                                                                          \
+      hipLaunchKernelGGL((k_blockmatch_fast<b0, b1, b2, r0, r1, r2, kBMThreadsFast>), grid,      \
+                         dim3(kBMThreadsFast), 0, s, a);                                         \
     else                                                                                         \
       hipLaunchKernelGGL((k_blockmatch<b0, b1, b2, r0, r1, r2, 2, kBMThreads>), grid, blk, dyn,  \
                          s, a);                                                                  \

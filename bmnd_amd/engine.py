@@ -261,10 +261,11 @@ class PoissonPipeline:
     """Two-stage Poisson denoiser on one device.  Buffers are torch tensors on
     `device`; kernels run on the current torch stream."""
 
-    DEVIANCE_MODES = ("auto", "exact", "one-log", "auto-one-log", "certified", "auto-certified")
+    DEVIANCE_MODES = ("auto", "exact", "one-log", "auto-one-log", "certified", "auto-certified",
+                      "fast", "auto-fast")
 
     def __init__(self, profile: BMNDProfile, shape: tuple[int, ...], device=None,
-                 deviance: str = "auto-certified"):
+                 deviance: str = "auto-fast"):
         self.profile = profile
         self.shape = tuple(int(s) for s in shape)
         self.ndim = len(self.shape)
@@ -275,11 +276,15 @@ class PoissonPipeline:
         # Integer counts <= 100 always use the float64 term table (the reference
         # expression, evaluated on the host with the C library's log).  Other
         # inputs: "auto" evaluates the reference's two-log expression term by
-        # term; "auto-certified" (default) evaluates the one-log form
+        # term; "auto-certified" evaluates the one-log form
         # r log 2r + c log 2c - s log s with a degree-4 log together with a
         # bound on its distance from the reference expression, and re-evaluates
         # the reference expression for the rare candidates whose tie-grid key
         # the bound does not pin, so its keys equal those of "auto";
+        # "auto-fast" (default) bounds every candidate's key with a float32
+        # pass first and runs the certified evaluation only for the
+        # candidates that can change the selection or the accepted count,
+        # with the same result;
         # "auto-one-log" uses the one-log form alone, which can move a
         # distance by ~1e-14 and so a key across the 32-ulp tie grid in rare
         # cases.
@@ -316,7 +321,10 @@ class PoissonPipeline:
             return 1, 0
         if self.deviance == "certified":
             return 3, 0
-        fallback = {"auto": 0, "auto-one-log": 1, "auto-certified": 3}[self.deviance]
+        if self.deviance == "fast":
+            return 4, 0
+        fallback = {"auto": 0, "auto-one-log": 1, "auto-certified": 3,
+                    "auto-fast": 4}[self.deviance]
         if self.count_scale != 1.0:
             return fallback, 0
         flags = torch.empty(2, dtype=torch.int32, device=self.device)

@@ -65,7 +65,12 @@ int32_t bmnd_hip_count_range(const float *vol, int64_t n, int32_t *flags,
  * mode 3 (certified) evaluates the one-log form with a degree-4 logarithm
  * together with an error bound per reference and re-evaluates the reference
  * expression wherever the bound leaves the tie-grid key open, so its output
- * equals mode 0's.  Modes 1 and 3 fall back to mode 0 when count_scale != 1. */
+ * equals mode 0's; mode 4 (fast certified) first bounds every candidate's
+ * key with a float32 pass (hardware log2) and evaluates mode 3's keys only
+ * for the candidates that can change the selection or the accepted count,
+ * so its output equals mode 0's too.  Modes 1, 3 and 4 fall back to mode 0
+ * when count_scale != 1; mode 4 falls back to mode 3 on a device whose
+ * hardware log2 is not within the bound bmnd_hip_log2_check measures. */
 int32_t bmnd_hip_blockmatch(const float *vol, const int64_t *shape_h,
                             const int32_t *block_h, const int32_t *radius_h,
                             const int32_t *ref_lin, int64_t nrefs,
@@ -74,6 +79,12 @@ int32_t bmnd_hip_blockmatch(const float *vol, const int64_t *shape_h,
                             int32_t stabilize, double count_scale,
                             int32_t mode, int32_t table_max, int32_t *sel,
                             int32_t *counts, int32_t *accepted, void *stream);
+
+/* Accuracy check behind block-matching mode 4 (no reference counterpart):
+ * *max_ratio = max |v_log_f32(x) - log2 x| / (|log2 x| + 1) over every normal
+ * float x >= 2^-100 on the current device (measured once per device and
+ * cached).  Mode 4 runs when it is <= 2^-20. */
+int32_t bmnd_hip_log2_check(double *max_ratio);
 
 /* Matrix table consumed by bmnd_hip_filter_groups, float32, packed as
  *   G_n (n x n, row-major, out index first) for n = 1..max_matches,

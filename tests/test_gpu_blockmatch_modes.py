@@ -1,9 +1,9 @@
 """Block-matching modes through the C-ABI, against each other and the oracle.
 
 Float-valued inputs (a smoothed Poisson volume, like the Wiener stage's pilot)
-take the evaluated-deviance paths.  The certified mode (3) must return the
-same match lists and accepted counts as the reference expression (mode 0) and
-as the oracle.  With unstabilised keys (plain float32 distances, a grid much
+take the evaluated-deviance paths.  The certified modes (3, and 4 with its
+float pre-pass) must return the same match lists and accepted counts as the
+reference expression (mode 0) and as the oracle.  With unstabilised keys (plain float32 distances, a grid much
 finer than the 32-ulp tie grid) a few candidates in a thousand are not pinned
 by the error bound and are re-evaluated with the reference expression, so the
 second case exercises that path."""
@@ -48,8 +48,18 @@ def _gpu_match(vol_t, shape, block, radius, refs_t, thr, max_m, min_m, stabilize
     return sel.view(R, max_m).cpu().numpy(), cnt.cpu().numpy(), acc.cpu().numpy()
 
 
+def test_hardware_log2_within_the_fast_bound():
+    """The fast matcher's float pass relies on |v_log_f32(x) - log2 x| <=
+    2^-20 (|log2 x| + 1); the library measures it over every normal float."""
+    ratio = ctypes.c_double(-1.0)
+    _lib.check(_lib.lib().bmnd_hip_log2_check(ctypes.byref(ratio)))
+    print(f"hardware log2: max |err| / (|log2 x| + 1) = {ratio.value:.3e}", flush=True)
+    assert 0.0 <= ratio.value <= 2.0 ** -20
+
+
+@pytest.mark.parametrize("mode", [3, 4])
 @pytest.mark.parametrize("stabilize", [1, 0])
-def test_certified_matches_reference_expression(stabilize):
+def test_certified_matches_reference_expression(stabilize, mode):
     shape = (34, 34, 34)
     vol = _pilot_like(shape, seed=11)
     plan = make_plan(BM4DProfile(noise_model=NoiseModel.POISSON), "wiener", 3)
@@ -61,10 +71,11 @@ def test_certified_matches_reference_expression(stabilize):
     dev = torch.device("cuda", 0)
     vol_t = torch.from_numpy(vol).to(dev)
     refs_t = torch.from_numpy(lin.astype(np.int32)).to(dev)
-    print(f"case: stabilize={stabilize} refs={len(lin)} block={block} radius={radius}", flush=True)
+    print(f"case: mode={mode} stabilize={stabilize} refs={len(lin)} block={block} "
+          f"radius={radius}", flush=True)
     args = (vol_t, shape, block, radius, refs_t, plan.threshold, plan.max_m, plan.min_m, stabilize)
     s0, c0, a0 = _gpu_match(*args, mode=0)
-    s3, c3, a3 = _gpu_match(*args, mode=3)
+    s3, c3, a3 = _gpu_match(*args, mode=mode)
     o = oracle.blockmatch(refs, counts, radius, plan.max_m, plan.min_m, plan.threshold,
                           volume=vol, block=block, stabilize=bool(stabilize))
     np.testing.assert_array_equal(c3, c0)

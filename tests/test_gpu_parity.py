@@ -42,7 +42,7 @@ def _compare(noisy, profile, params, deviance="auto"):
     return stats, out, o_out
 
 
-@pytest.mark.parametrize("deviance", ["auto", "auto-certified"])
+@pytest.mark.parametrize("deviance", ["auto", "auto-certified", "auto-fast"])
 def test_readme_square_bm3d_matches_oracle(deviance):
     _, noisy = phantoms.square_image(0)
     prof = BM3DProfile(noise_model=NoiseModel.POISSON)
@@ -53,7 +53,7 @@ def test_readme_square_bm3d_matches_oracle(deviance):
     assert stats["out_bits_differing"] == 0
 
 
-@pytest.mark.parametrize("deviance", ["auto", "auto-certified"])
+@pytest.mark.parametrize("deviance", ["auto", "auto-certified", "auto-fast"])
 def test_small_volume_bm4d_matches_oracle(deviance):
     _, noisy = phantoms.ellipsoid_volume((28, 32, 24), seed=3, peak=10.0, n_ellipsoids=6,
                                          axes_range=(3.0, 12.0))
@@ -65,16 +65,16 @@ def test_small_volume_bm4d_matches_oracle(deviance):
     assert stats["out_bits_differing"] == 0
 
 
-@pytest.mark.parametrize("deviance", ["exact", "certified", "one-log"])
+@pytest.mark.parametrize("deviance", ["exact", "certified", "fast", "one-log"])
 def test_deviance_modes_on_crop(deviance):
-    """The two-log and certified modes must reproduce the oracle; the one-log
-    mode is reported (match-list differences come only from last-bit distance
+    """The two-log and both certified modes must reproduce the oracle; the
+    one-log mode is reported (match-list differences come only from last-bit distance
     changes that move a key across a tie-grid boundary)."""
     _, noisy = phantoms.ellipsoid_volume((40, 40, 40), seed=5, peak=10.0, n_ellipsoids=10,
                                          axes_range=(4.0, 16.0))
     prof = BM4DProfile(noise_model=NoiseModel.POISSON)
     stats, out, o_out = _compare(noisy, prof, oracle.default_params(3, noisy.shape), deviance)
-    if deviance in ("exact", "certified"):
+    if deviance in ("exact", "certified", "fast"):
         assert stats["ht_groups_differing"] == 0
         assert stats["wiener_groups_differing"] == 0
         assert stats["out_bits_differing"] == 0
@@ -90,7 +90,7 @@ def test_blob_image_bm3d_matches_oracle(peak):
     _, noisy = phantoms.blob_image(144, seed=7, peak=peak)
     prof = BM3DProfile(noise_model=NoiseModel.POISSON)
     stats, out, o_out = _compare(noisy, prof, oracle.default_params(2, noisy.shape),
-                                 "auto-certified")
+                                 "auto-fast")
     assert stats["ht_groups_differing"] == 0
     assert stats["wiener_groups_differing"] == 0
     assert stats["pilot_bits_differing"] == 0
