@@ -260,7 +260,8 @@ struct BMArgs {
   const double *logtab;
   const double2 *fastlog;    // MODES 1, 3: {1/c_j, log c_j}, j < kFastLogN
   const double2 *certlog;    // MODE 3: {2/c_j, log c_j - ln2}, j < kFastLogN
-  double cert_c3, cert_c4;   // MODE 3: 1/3 and -1/4, the degree-4 log's coefficients
+  double cert_c3, cert_c4;   // MODES 3, 4: 1/3 and -1/4, the degree-4 log's coefficients
+  double log2_tol;           // MODE 4: bound on the hardware log2 error ratio
   const double *term_table;  // MODE 2: (tmax+1)^2 terms, row = reference count
   int tmax;
   int32_t *sel, *counts, *accepted;
@@ -386,6 +387,10 @@ __device__ __forceinline__ void bm_radix_select(KeyOf key_of, int box, int K,
   mask_out = mask;
 }
 
+__device__ __forceinline__ void bm_finalize(int K, int acc, const BMArgs &a, int64_t g, int lo0,
+                                            int lo1, int lo2, int e1, int e2, int center_local,
+                                            const BMSelScratch &sc);
+
 // Selection shared by the block-matching kernels: from the box composites
 // (unique) and the accepted count, keep the K smallest the reference keeps,
 // order them and write the group of reference g: reference first, dedupe,
@@ -412,6 +417,16 @@ __device__ __forceinline__ void bm_select_write(KeyOf key_of, int box, int acc, 
     sc.sorted[rank] = mine;
   }
   __syncthreads();
+  bm_finalize(K, acc, a, g, lo0, lo1, lo2, e1, e2, center_local, sc);
+}
+
+// Writes the group of reference g from the K smallest composites in
+// sc.sorted (ascending): reference first, dedupe, cap, power-of-two
+// truncation (blockmatching.py:913-954).
+__device__ __forceinline__ void bm_finalize(int K, int acc, const BMArgs &a, int64_t g, int lo0,
+                                            int lo1, int lo2, int e1, int e2, int center_local,
+                                            const BMSelScratch &sc) {
+  const int tid = threadIdx.x;
   if (tid < 64) {
     // reference first, then the sorted candidates without it (composites are
     // unique, so the reference patch is the only possible duplicate), capped
@@ -802,7 +817,6 @@ __global__ __launch_bounds__(NT) void k_blockmatch_cert(BMArgs a) {
 constexpr double kLog2Tol = 0x1p-20;
 constexpr float kLog2Floor = 0x1p-100f;
 constexpr int kBMThreadsFast = 512;  // two workgroups per CU
-constexpr int kFastListCap = 512;
 
 // max over x of |v_log_f32(x) - log2 x| / (|log2 x| + 1), x over every normal
 // float in [kLog2Floor, inf), log2 x from the float64 dlog; written to *out as
@@ -844,22 +858,24 @@ __global__ __launch_bounds__(NT) void k_blockmatch_fast(BMArgs a) {
   constexpr int SLAB = S0 * S1 * S2;
   constexpr int BOX = (2 * R0 + 1) * (2 * R1 + 1) * (2 * R2 + 1);
   constexpr int P = B0 * B1 * B2;
-  constexpr int W2 = S2 - B2 + 1;         // window positions along axis 2
-  constexpr int XS = S0 * S1 * W2;        // axis-2 window sums of c log 2c
+  constexpr int W2 = S2 - B2 + 1;          // window positions along axis 2
+  constexpr int XS = S0 * S1 * W2;         // axis-2 window sums of c log 2c
   constexpr int KC = (BOX + NT - 1) / NT;  // candidates per thread
+  constexpr int XCAP = XS - BOX < 256 ? XS - BOX : 256;  // exact-set composites kept for the final rank
+  constexpr int LCAP = (XS - BOX) / 2 < 256 ? (XS - BOX) / 2 : 256;  // exact-evaluation list
   static_assert(BOX <= XS, "the key pairs reuse the window-sum buffer");
   static_assert(KC <= 32, "per-thread candidate mask");
   __shared__ float s_slab[SLAB];           // max(x, 0)
-  __shared__ double s_xsum[XS];            // window sums, then per-candidate sums, then key pairs
+  __shared__ double s_xsum[XS];            // window sums; then key pairs [BOX], exact-set composites
   __shared__ float s_reff[P];              // max(r, 0)
+  __shared__ float s_reft[P];              // max(r, 2^-126): keeps every float-pass sum normal
   __shared__ double s_tab[kLogTabDoubles];
   __shared__ double2 s_clog[kFastLogN];    // degree-4 log
   __shared__ double s_rpart[(P + 63) / 64];
+  __shared__ float s_wmax[2][NT / 64];     // per-wave maxima of slab and reference values
   __shared__ double s_rsum, s_errc, s_errf;
-  __shared__ unsigned s_vmax[2];
-  __shared__ int s_acc, s_maybe;
+  __shared__ int s_acc, s_maybe, s_nex;
   __shared__ int s_scan[NT / 64 + 1];
-  __shared__ unsigned short s_list[kFastListCap];
   BM_SEL_SCRATCH(sc);
 
   const int tid = threadIdx.x;
@@ -876,15 +892,10 @@ __global__ __launch_bounds__(NT) void k_blockmatch_fast(BMArgs a) {
 
   for (int i = tid; i < kLogTabDoubles; i += NT) s_tab[i] = a.logtab[i];
   for (int i = tid; i < kFastLogN; i += NT) s_clog[i] = a.certlog[i];
-  if (tid < 2) s_vmax[tid] = 0u;
-  if (tid == 0) {
-    s_acc = 0;
-    s_maybe = 0;
-  }
-  __syncthreads();
   const int x0 = e0 + B0 - 1, x1 = e1 + B1 - 1, x2 = e2 + B2 - 1;
   float vmax = 0.0f;
   {
+    // every load in flight before the first store
     constexpr int IT = (SLAB + NT - 1) / NT;
     float vv[IT];
 #pragma unroll
@@ -910,10 +921,22 @@ __global__ __launch_bounds__(NT) void k_blockmatch_fast(BMArgs a) {
     const float rf = v > 0.0f ? v : 0.0f;
     rmax = fmaxf(rmax, rf);
     s_reff[p] = rf;
+    s_reft[p] = fmaxf(rf, 0x1p-126f);
   }
-  // non-negative floats order like their bit patterns
-  if (vmax > 0.0f) atomicMax(&s_vmax[1], __float_as_uint(vmax));
-  if (rmax > 0.0f) atomicMax(&s_vmax[0], __float_as_uint(rmax));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    vmax = fmaxf(vmax, __shfl_xor(vmax, o, 64));
+    rmax = fmaxf(rmax, __shfl_xor(rmax, o, 64));
+  }
+  if (lane == 0) {
+    s_wmax[0][tid >> 6] = rmax;
+    s_wmax[1][tid >> 6] = vmax;
+  }
+  if (tid == 0) {
+    s_acc = 0;
+    s_maybe = 0;
+    s_nex = 0;
+  }
   __syncthreads();
 
   const double ln2 = s_tab[3 * kLogTabN] + s_tab[3 * kLogTabN + 1];
@@ -954,25 +977,32 @@ __global__ __launch_bounds__(NT) void k_blockmatch_fast(BMArgs a) {
     double rs = 0.0;
     for (int w = 0; w < (P + 63) / 64; ++w) rs = rs + s_rpart[w];
     s_rsum = rs;
+    float rm = 0.0f, vm = 0.0f;
+    for (int w = 0; w < NT / 64; ++w) {
+      rm = fmaxf(rm, s_wmax[0][w]);
+      vm = fmaxf(vm, s_wmax[1][w]);
+    }
     // Bounds on |d - d_ref| for every candidate of this reference, with
     // S >= s = r + c over the slab.  Float64 evaluation (the terms of
     // r log 2r, c log 2c and s log s with the degree-4 log, truncation
     // < 1.9e-13 * x per log): each path's rounding error is at most a few
     // hundred units of 2^-53 times P*W, W >= s (|ln s| + 1) + 2, for any
     // summation order; the constants keep margins of about 7x and 2x.
-    // Float pass, added for sum_p s log2 s: per term the hardware log bound
-    // (kLog2Tol, twice), the rounding of s, of the product and of the float
-    // row sums come to at most 2.5e-6 * s (|log2 s| + 2) <= 2.5e-6 * W2,
-    // W2 = S (|log2 S| + 2) + 3; kept with a 2x margin and scaled by 2 ln2.
-    const double S = (double)__uint_as_float(s_vmax[0]) + (double)__uint_as_float(s_vmax[1]);
+    // Float pass, added for sum_p s log2 s: per term the hardware log error
+    // (a.log2_tol, the measured worst case with a 2x margin, for both the
+    // relative and the absolute part) plus the rounding of s and of the fused
+    // row sums (10 units of 2^-24) times s (|log2 s| + 2) <= W2 =
+    // S (|log2 S| + 2) + 3, kept with a 2x margin and scaled by 2 ln2; and
+    // the reference floor 2^-126 moves a term by at most 2^-118.
+    const double S = (double)rm + (double)vm;
     const double lS = S > 0.0 ? fabs(dlog(S, s_tab)) : 0.0;
     const double Wn = S * (lS + 1.0) + 2.0;
     const double Wb = S * (lS / ln2 + 2.0) + 3.0;
     const double ec = 2.0 * (double)P * (S * 8e-13 + 4096.0 * 0x1p-53 * Wn);
     s_errc = ec;
-    s_errf = ec + 2.0 * ln2 * (double)P * (5e-6 * Wb);
+    s_errf = ec + 2.0 * ln2 * (double)P * (2.0 * (2.0 * a.log2_tol + 10.0 * 0x1p-24) * Wb + 0x1p-118);
   }
-  // patch sums of c log 2c per candidate, parked in the slot of its key pair
+  // patch sums of c log 2c per candidate, kept in registers
   double zc[KC];
 #pragma unroll
   for (int k = 0; k < KC; ++k) {
@@ -989,13 +1019,8 @@ __global__ __launch_bounds__(NT) void k_blockmatch_fast(BMArgs a) {
     }
   }
   __syncthreads();
-  double *zslot = s_xsum;
-  float2 *kp = reinterpret_cast<float2 *>(s_xsum);  // (lower key, upper key) per candidate
-#pragma unroll
-  for (int k = 0; k < KC; ++k) {
-    const int cand = tid + k * NT;
-    if (cand < box) zslot[cand] = zc[k];
-  }
+  float2 *kp = reinterpret_cast<float2 *>(s_xsum);  // (lower key, upper key), then final key
+  unsigned long long *exset = reinterpret_cast<unsigned long long *>(s_xsum) + BOX;
   const double rsum = s_rsum, errc = s_errc, errf = s_errf;
   const float thr = a.stabilize ? a.thr_key : a.threshold;
   const int rel0 = o0 - lo0, rel1 = o1 - lo1, rel2 = o2 - lo2;
@@ -1003,29 +1028,33 @@ __global__ __launch_bounds__(NT) void k_blockmatch_fast(BMArgs a) {
 
   // ---- float pass ------------------------------------------------------------
   int my_maybe = 0;
-  for (int cand = tid; cand < box; cand += NT) {
-    const int j2 = cand % e2, j1 = (cand / e2) % e1, j0 = cand / (e2 * e1);
-    const int base = (j0 * S1 + j1) * S2 + j2;
-    double F = 0.0;
-#pragma unroll 1
-    for (int p01 = 0; p01 < B0 * B1; ++p01) {
-      const int p0 = p01 / B1, p1 = p01 - p0 * B1;
-      const float *row = s_slab + base + (p0 * S1 + p1) * S2;
-      const float *rrow = s_reff + p01 * B2;
-      float rs = 0.0f;
 #pragma unroll
-      for (int p2 = 0; p2 < B2; ++p2) {
-        const float s = rrow[p2] + row[p2];
-        rs = rs + s * __builtin_amdgcn_logf(fmaxf(s, kLog2Floor));
+  for (int k = 0; k < KC; ++k) {
+    const int cand = tid + k * NT;
+    if (cand < box) {
+      const int j2 = cand % e2, j1 = (cand / e2) % e1, j0 = cand / (e2 * e1);
+      const int base = (j0 * S1 + j1) * S2 + j2;
+      double F = 0.0;
+#pragma unroll 1
+      for (int p01 = 0; p01 < B0 * B1; ++p01) {
+        const int p0 = p01 / B1, p1 = p01 - p0 * B1;
+        const float *row = s_slab + base + (p0 * S1 + p1) * S2;
+        const float *rrow = s_reft + p01 * B2;
+        float rs = 0.0f;
+#pragma unroll
+        for (int p2 = 0; p2 < B2; ++p2) {
+          const float s = rrow[p2] + row[p2];  // >= 2^-126, a normal float
+          rs = fmaf(s, __builtin_amdgcn_logf(s), rs);
+        }
+        F = F + (double)rs;
       }
-      F = F + (double)rs;
+      const double df = 2.0 * (rsum + (zc[k] - ln2 * F));
+      bool ok_lo, ok_hi;
+      const float klo = bm_key<P>(df - errf, a, ok_lo);
+      const float khi = bm_key<P>(df + errf, a, ok_hi);
+      kp[cand] = make_float2(klo, khi);
+      my_maybe += ok_lo ? 1 : 0;
     }
-    const double df = 2.0 * (rsum + (zslot[cand] - ln2 * F));
-    bool ok_lo, ok_hi;
-    const float klo = bm_key<P>(df - errf, a, ok_lo);
-    const float khi = bm_key<P>(df + errf, a, ok_hi);
-    kp[cand] = make_float2(klo, khi);
-    my_maybe += ok_lo ? 1 : 0;
   }
   if (my_maybe) atomicAdd(&s_maybe, my_maybe);
   __syncthreads();
@@ -1040,6 +1069,10 @@ __global__ __launch_bounds__(NT) void k_blockmatch_fast(BMArgs a) {
   unsigned long long cpre, cmask;
   bm_radix_select<NT, BOX>([&](int i) { return comp(kp[i].y, i); }, box, kmax, sc, cpre, cmask);
   const unsigned long long cut = cpre | ~cmask;
+
+  // ---- exact keys for the candidates that can change the result ---------------
+  // (lower composite at most the cutoff, or the threshold undecided); each
+  // thread evaluates its own, with its patch sum of c log 2c
   unsigned need = 0u;
 #pragma unroll
   for (int k = 0; k < KC; ++k) {
@@ -1049,23 +1082,28 @@ __global__ __launch_bounds__(NT) void k_blockmatch_fast(BMArgs a) {
       if (comp(kk.x, cand) <= cut || ((kk.x < thr) != (kk.y < thr))) need |= 1u << k;
     }
   }
+  // compacted list of (candidate, patch sum) in the free part of the buffer,
+  // evaluated by consecutive lanes; in rounds of LCAP if it is longer
+  double *lz = reinterpret_cast<double *>(s_xsum) + BOX;
+  int *lc = reinterpret_cast<int *>(lz + LCAP);
   int total;
   const int off = block_exclusive_scan<NT>(__popc(need), s_scan, &total);
-
-  // ---- exact keys for the candidates that can change the result ---------------
-  for (int lbase = 0; lbase < total; lbase += kFastListCap) {
+  for (int lbase = 0; lbase < total; lbase += LCAP) {
     int r = off;
 #pragma unroll
     for (int k = 0; k < KC; ++k) {
       if (need & (1u << k)) {
-        if (r >= lbase && r < lbase + kFastListCap) s_list[r - lbase] = (unsigned short)(tid + k * NT);
+        if (r >= lbase && r < lbase + LCAP) {
+          lz[r - lbase] = zc[k];
+          lc[r - lbase] = tid + k * NT;
+        }
         ++r;
       }
     }
     __syncthreads();
-    const int m = min(kFastListCap, total - lbase);
+    const int m = min(LCAP, total - lbase);
     for (int e = tid; e < m; e += NT) {
-      const int cand = s_list[e];
+      const int cand = lc[e];
       const int j2 = cand % e2, j1 = (cand / e2) % e1, j0 = cand / (e2 * e1);
       const int base = (j0 * S1 + j1) * S2 + j2;
       double t = 0.0;
@@ -1073,13 +1111,11 @@ __global__ __launch_bounds__(NT) void k_blockmatch_fast(BMArgs a) {
         const int p0 = p01 / B1, p1 = p01 - p0 * B1;
         const float *row = s_slab + base + (p0 * S1 + p1) * S2;
         for (int p2 = 0; p2 < B2; ++p2) {
-          const double c = (double)row[p2];
-          const double s = (double)s_reff[p01 * B2 + p2] + c;
-          const double lc = c > 0.0 ? c * log_deg4(2.0 * c, s_clog, ln2, c3, c4) : 0.0;
-          t = t + fma(-s, log_deg4(s, s_clog, ln2, c3, c4), lc);
+          const double s = (double)s_reff[p01 * B2 + p2] + (double)row[p2];
+          t = fma(-s, log_deg4(s, s_clog, ln2, c3, c4), t);
         }
       }
-      const double dc = 2.0 * (rsum + t);
+      const double dc = 2.0 * (rsum + (lz[e] + t));
       bool ok_lo, ok_hi, ok;
       const float k_lo = bm_key<P>(dc - errc, a, ok_lo);
       const float k_hi = bm_key<P>(dc + errc, a, ok_hi);
@@ -1098,14 +1134,39 @@ __global__ __launch_bounds__(NT) void k_blockmatch_fast(BMArgs a) {
     }
     __syncthreads();
   }
-  // final keys: exact where computed, else the lower key (its threshold side
-  // is known and its composite is above the cutoff)
+  // Final keys: exact where evaluated, else the lower key (its threshold side
+  // is known and its composite lies above the cutoff).  The K smallest true
+  // composites are all in the evaluated set, which is kept for the final rank.
   int my_acc = 0;
-  for (int cand = tid; cand < box; cand += NT) my_acc += kp[cand].x < thr ? 1 : 0;
+#pragma unroll
+  for (int k = 0; k < KC; ++k) {
+    const int cand = tid + k * NT;
+    if (cand < box) {
+      const float key = kp[cand].x;
+      my_acc += key < thr ? 1 : 0;
+      if (need & (1u << k)) {
+        const int slot = atomicAdd(&s_nex, 1);
+        if (slot < XCAP) exset[slot] = comp(key, cand);
+      }
+    }
+  }
   if (my_acc) atomicAdd(&s_acc, my_acc);
   __syncthreads();
-  bm_select_write<NT, BOX>([&](int i) { return comp(kp[i].x, i); }, box, s_acc, a, g, lo0, lo1,
-                           lo2, e1, e2, center_local, sc);
+  const int acc = s_acc, nex = s_nex;
+  if (nex <= XCAP) {
+    const int K = bm_keep_count(acc, box, a);
+    for (int e = tid; e < nex; e += NT) {
+      const unsigned long long mine = exset[e];
+      int rank = 0;
+      for (int j = 0; j < nex; ++j) rank += exset[j] < mine ? 1 : 0;
+      if (rank < K) sc.sorted[rank] = mine;
+    }
+    __syncthreads();
+    bm_finalize(K, acc, a, g, lo0, lo1, lo2, e1, e2, center_local, sc);
+  } else {
+    bm_select_write<NT, BOX>([&](int i) { return comp(kp[i].x, i); }, box, acc, a, g, lo0, lo1,
+                             lo2, e1, e2, center_local, sc);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -2114,11 +2175,14 @@ int32_t bmnd_hip_blockmatch(const float *vol, const int64_t *shape_h, const int3
   if (int rc = logtab_for_device(&tab)) return rc;
   // the one-log slabs assume unit count scale
   if ((mode == 1 || mode == 3 || mode == 4) && count_scale != 1.0) mode = 0;
+  double log2_tol = kLog2Tol;
   if (mode == 4) {
-    // the float pass needs the hardware log2 within its bound on this device
+    // the float pass needs the hardware log2 within its bound on this device;
+    // it is run with twice the measured worst case
     double ratio = 1.0;
     if (int rc = log2_ratio_for_device(&ratio)) return rc;
     if (!(ratio <= kLog2Tol)) mode = 3;
+    log2_tol = std::min(kLog2Tol, 2.0 * ratio);
   }
   const double *terms = nullptr;
   if (mode == 2)
@@ -2150,6 +2214,7 @@ int32_t bmnd_hip_blockmatch(const float *vol, const int64_t *shape_h, const int3
   a.certlog = clog;
   a.cert_c3 = 1.0 / 3.0;
   a.cert_c4 = -0.25;
+  a.log2_tol = log2_tol;
   a.term_table = terms;
   a.tmax = table_max;
   a.sel = sel;
