@@ -1319,8 +1319,8 @@ __global__ __launch_bounds__(64) void k_exact_std(ExactArgs a) {
   }
   const int ub = ue0 * ue1 * ue2;
   for (int q0 = 0; q0 < planes; q0 += 4) {
+    float4 gv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (lane < n) {
-      float4 gv;
       gv.x = gf[q0 * n + lane];
       gv.y = (q0 + 1 < planes) ? gf[(q0 + 1) * n + lane] : 0.0f;
       gv.z = (q0 + 2 < planes) ? gf[(q0 + 2) * n + lane] : 0.0f;
@@ -1474,9 +1474,14 @@ __device__ __forceinline__ void member_transform(float *__restrict__ data, int n
 // In-place transform along one spatial axis of an LDS array viewed as
 // (nouter, B, INNER): each lane owns whole lines, reads the B inputs of a line
 // into registers and then writes its B outputs (lines are disjoint).
-template <int NT, int B, int INNER>
+// epilogue of a transform's last pass: value stored at element e
+struct EpiStore {
+  __device__ __forceinline__ float operator()(int, float v) const { return v; }
+};
+
+template <int NT, int B, int INNER, class Epi = EpiStore>
 __device__ __forceinline__ void line_transform(float *__restrict__ data, int nouter,
-                                               const float *__restrict__ M) {
+                                               const float *__restrict__ M, Epi epi = Epi()) {
   float m[B * B];
 #pragma unroll
   for (int i = 0; i < B * B; ++i) m[i] = M[i];
@@ -1492,22 +1497,25 @@ __device__ __forceinline__ void line_transform(float *__restrict__ data, int nou
       float acc = 0.0f;
 #pragma unroll
       for (int k = 0; k < B; ++k) acc = fmaf(m[r * B + k], x[k], acc);
-      base[r * INNER] = acc;
+      base[r * INNER] = epi(o * B * INNER + in_i + r * INNER, acc);
     }
   }
   __syncthreads();
 }
 
-// separable transform of a group (n, B0, B1, B2) in LDS: member axis, then the
-// spatial axes in order (an axis of length 1 has the matrix [1] and is skipped)
-template <int NT, int B0, int B1, int B2, int NMAX>
+// separable transform of a group (n, B0, B1, B2) in LDS: member axis, then the (an axis of length 1 has the matrix [1] and is skipped)
+// spatial axes in order (an axis of length 1 has the matrix [1] and is
+// skipped); the epilogue runs on the outputs of the last pass
+template <int NT, int B0, int B1, int B2, int NMAX, class Epi = EpiStore>
 __device__ __forceinline__ void group_transform(float *__restrict__ data, int n,
                                                 const float *__restrict__ G,
-                                                const float (*__restrict__ S)[64]) {
+                                                const float (*__restrict__ S)[64],
+                                                Epi epi = Epi()) {
+  static_assert(B2 > 1, "the last spatial axis carries the epilogue");
   member_transform<NT, B0 * B1 * B2, NMAX>(data, n, G);
   if constexpr (B0 > 1) line_transform<NT, B0, B1 * B2>(data, n, S[0]);
   if constexpr (B1 > 1) line_transform<NT, B1, B2>(data, n * B0, S[1]);
-  if constexpr (B2 > 1) line_transform<NT, B2, 1>(data, n * B0 * B1, S[2]);
+  line_transform<NT, B2, 1>(data, n * B0 * B1, S[2], epi);
 }
 
 template <int B0, int B1, int B2, int R0, int R1, int R2, int NMAX, int NT, bool WIENER>
@@ -1636,12 +1644,8 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
   }
   __syncthreads();
   // ---- approximate std: squared transforms along every axis (variance.py:613-623)
-  group_transform<NT, B0, B1, B2, NMAX>(s_sig, n, s_g2, s_sf2);
-  for (int e = tid; e < np; e += NT) {
-    const float v = s_sig[e];
-    s_sig[e] = sqrtf(v > 0.0f ? v : 0.0f);
-  }
-  __syncthreads();
+  group_transform<NT, B0, B1, B2, NMAX>(s_sig, n, s_g2, s_sf2,
+                                        [](int, float v) { return sqrtf(v > 0.0f ? v : 0.0f); });
 
   // ---- exact overlap-aware planes (k_exact_std) and plane limiting ----------
   // the first KEX planes are in LDS; further ones are read from the buffer
@@ -1656,12 +1660,16 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
   } else {
     // variance.py:1572-1631, float32 sequential plane sums
     for (int s = tid; s < P; s += NT) {
+      float av[NMAX];  // every plane's approximate std, loaded at once
+#pragma unroll
+      for (int q = 0; q < NMAX; ++q) av[q] = q < n ? s_sig[q * P + s] : 0.0f;
       float tot_a = 0.0f, tot_e = 0.0f, rem_a = 0.0f;
-#pragma unroll 8
-      for (int q = 0; q < n; ++q) {
-        const float av = s_sig[q * P + s];
-        tot_a = tot_a + av * av;
-        if (q >= planes) rem_a = rem_a + av * av;
+#pragma unroll
+      for (int q = 0; q < NMAX; ++q) {
+        if (q < n) {
+          tot_a = tot_a + av[q] * av[q];
+          if (q >= planes) rem_a = rem_a + av[q] * av[q];
+        }
       }
       for (int q = 0; q < planes; ++q) {
         const float ev = exv(q, s);
@@ -1671,34 +1679,32 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
       if (target < 0.0f) target = 0.0f;
       const float scale = rem_a > 0.0f ? target / rem_a : 0.0f;
       const float fallback = target / (float)(n - planes);
-#pragma unroll 8
-      for (int q = 0; q < n; ++q) {
-        float v;
-        if (q < planes) {
-          const float ev = exv(q, s);
-          v = ev * ev;
-        } else {
-          const float av = s_sig[q * P + s];
-          v = (av * av) * scale;
-          if (!(rem_a > 0.0f)) v = fallback;
+#pragma unroll
+      for (int q = 0; q < NMAX; ++q) {
+        if (q < n) {
+          float v;
+          if (q < planes) {
+            const float ev = exv(q, s);
+            v = ev * ev;
+          } else {
+            v = (av[q] * av[q]) * scale;
+            if (!(rem_a > 0.0f)) v = fallback;
+          }
+          s_sig[q * P + s] = sqrtf(v > 0.0f ? v : 0.0f);
         }
-        s_sig[q * P + s] = sqrtf(v > 0.0f ? v : 0.0f);
       }
     }
   }
   __syncthreads();
 
-  // ---- coefficients ----------------------------------------------------------
-  group_transform<NT, B0, B1, B2, NMAX>(s_x, n, s_g, s_sf);
-  if (WIENER) group_transform<NT, B0, B1, B2, NMAX>(s_r, n, s_g, s_sf);
+  // ---- coefficients, with the shrinkage fused into the last transform pass ----
   float *atten = s_r;
-  if (!WIENER) {
+  if constexpr (!WIENER) {
     // Poisson hard threshold (poisson.py:30-46, bmndalgo.py:460-485)
     const float lam = a.shrink;
     const double cs = a.count_scale;
-    for (int e = tid; e < np; e += NT) {
+    group_transform<NT, B0, B1, B2, NMAX>(s_x, n, s_g, s_sf, [&](int e, float c) {
       const float thr = lam * s_sig[e];
-      const float c = s_x[e];
       const double cc = (double)fabsf(c) / cs;
       const double tc = (double)thr / cs;
       double sc = cc > tc ? cc : tc;
@@ -1706,37 +1712,49 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
       const double tol = 0x1p-16 * sc;
       const float keep = (cc > tc + tol) ? 1.0f : 0.0f;
       atten[e] = keep;
-      s_x[e] = c * keep;
-    }
+      return c * keep;
+    });
   } else {
-    // Poisson Wiener gain, variance-scaled (wiener.py:24-53, 78-95)
+    // Poisson Wiener gain, variance-scaled (wiener.py:24-53, 78-95), applied
+    // as the pilot's coefficients come out of its last pass
+    group_transform<NT, B0, B1, B2, NMAX>(s_x, n, s_g, s_sf);
     const float mu = a.shrink;
     const float eps32 = 1e-10f;
-    for (int e = tid; e < np; e += NT) {
+    group_transform<NT, B0, B1, B2, NMAX>(s_r, n, s_g, s_sf, [&](int e, float r) {
       const float sg = s_sig[e];
       const float nv = sg * sg;
       const float eff = mu * nv;
-      const float rc = fabsf(s_r[e]);
+      const float rc = fabsf(r);
       const float pr = rc * rc;
       float sp_ = pr - eff;
       if (sp_ < 0.0f) sp_ = 0.0f;
       float gain = sp_ / ((sp_ + eff) + eps32);
       gain = gain < 0.0f ? 0.0f : (gain > 1.0f ? 1.0f : gain);
-      atten[e] = gain;
       s_x[e] = s_x[e] * gain;
-    }
+      return gain;  // the pilot coefficient's slot now holds the attenuation
+    });
   }
-  __syncthreads();
   // ---- per-patch weights (aggregationweights.py:213-259, 616-653) ------------
   if (tid < n) {
     double risk = 0.0;
-    for (int c = 0; c < P; ++c) {
-      const double at = (double)atten[tid * P + c];
-      const float sg = s_sig[tid * P + c];
-      const float nv = sg * sg;
-      double err = at * at;
-      err = err * (double)nv;
-      risk = risk + err;
+    constexpr int CH = 25;  // loads of a chunk in flight together
+    for (int c0 = 0; c0 < P; c0 += CH) {
+      float atv[CH], sgv[CH];
+#pragma unroll
+      for (int j = 0; j < CH; ++j) {
+        atv[j] = c0 + j < P ? atten[tid * P + c0 + j] : 0.0f;
+        sgv[j] = c0 + j < P ? s_sig[tid * P + c0 + j] : 0.0f;
+      }
+#pragma unroll
+      for (int j = 0; j < CH; ++j) {
+        if (c0 + j < P) {
+          const double at = (double)atv[j];
+          const float nv = sgv[j] * sgv[j];
+          double err = at * at;
+          err = err * (double)nv;
+          risk = risk + err;
+        }
+      }
     }
     s_plane[tid] = risk;
   }
@@ -1751,10 +1769,12 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
     if (isfinite(risk) && risk >= 0.0) w = (float)(1.0 / (risk + 1e-10));
     a.weights[g * a.max_m + tid] = w;
   }
-  // ---- inverse transform and store -------------------------------------------
-  group_transform<NT, B0, B1, B2, NMAX>(s_x, n, s_gi, s_si);
+  // ---- inverse transform, stored from its last pass --------------------------
   float *dst = a.filt + (g * a.max_m) * (int64_t)P;
-  for (int e = tid; e < np; e += NT) dst[e] = s_x[e];
+  group_transform<NT, B0, B1, B2, NMAX>(s_x, n, s_gi, s_si, [&](int e, float v) {
+    dst[e] = v;
+    return v;
+  });
 }
 
 // ---------------------------------------------------------------------------
