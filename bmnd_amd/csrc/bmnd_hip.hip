@@ -1813,6 +1813,8 @@ __global__ __launch_bounds__(kAggThreads) void k_aggregate(AggArgs a) {
   __shared__ float s_k[P];
   __shared__ int s_gcount, s_take, s_nent;
   __shared__ int s_scan[kAggThreads / 64 + 1];
+  constexpr int kSub = 256;                      // entries per wave-filtering pass
+  __shared__ short s_wl[kAggThreads / 64][kSub];  // per wave: entries reaching its voxels
 
   const int tid = threadIdx.x;
   const int nt2 = (a.n2 + T2 - 1) / T2, nt1 = (a.n1 + T1 - 1) / T1;
@@ -1869,6 +1871,18 @@ __global__ __launch_bounds__(kAggThreads) void k_aggregate(AggArgs a) {
   const int x0 = t_lo0 + v0, x1 = t_lo1 + v1, x2 = t_lo2 + v2;
   const bool inside = x0 <= t_hi0 && x1 <= t_hi1 && x2 <= t_hi2;
   float num = 0.0f, den = 0.0f;
+  // box of this wave's voxels (tile coordinates)
+  const int lane = tid & 63, wave = tid >> 6;
+  int wlo0 = v0, whi0 = v0, wlo1 = v1, whi1 = v1, wlo2 = v2, whi2 = v2;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    wlo0 = min(wlo0, __shfl_xor(wlo0, o, 64));
+    whi0 = max(whi0, __shfl_xor(whi0, o, 64));
+    wlo1 = min(wlo1, __shfl_xor(wlo1, o, 64));
+    whi1 = max(whi1, __shfl_xor(whi1, o, 64));
+    wlo2 = min(wlo2, __shfl_xor(wlo2, o, 64));
+    whi2 = max(whi2, __shfl_xor(whi2, o, 64));
+  }
   int j0 = 0;
   while (j0 < gcount) {
     // members of groups j0.. that intersect the tile, in (group, member) order
@@ -1917,36 +1931,56 @@ __global__ __launch_bounds__(kAggThreads) void k_aggregate(AggArgs a) {
     if (tid == take - 1) s_nent = ex + mcount;  // entries produced
     __syncthreads();
     nent = s_nent;
-    if (inside) {
-      // chunks of entries: every load of a chunk is issued before the
-      // contributions are added, in entry order
-      constexpr int CH = 16;
-      for (int e0 = 0; e0 < nent; e0 += CH) {
-        float fv[CH], wv[CH];
-        unsigned hits = 0u;
+    // Each wave walks only the entries whose patch reaches its voxels' box,
+    // compacted in order (ballot) in passes of kSub entries; a walk takes
+    // chunks of entries whose loads are all issued before the contributions
+    // are added, in entry order.
+    for (int sb = 0; sb < nent; sb += kSub) {
+      const int sbn = min(kSub, nent - sb);
+      int wcount = 0;
+      for (int c0 = 0; c0 < sbn; c0 += 64) {
+        bool reach = false;
+        const int e = sb + c0 + lane;
+        if (c0 + lane < sbn) {
+          const int pk = s_ent_pos[e];
+          const int q0 = ((pk >> 20) & 1023) - 512, q1 = ((pk >> 10) & 1023) - 512,
+                    q2 = (pk & 1023) - 512;
+          reach = q0 <= whi0 && q0 + B0 - 1 >= wlo0 && q1 <= whi1 && q1 + B1 - 1 >= wlo1 &&
+                  q2 <= whi2 && q2 + B2 - 1 >= wlo2;
+        }
+        const unsigned long long m = __ballot(reach);
+        if (reach) s_wl[wave][wcount + __popcll(m & ((1ull << lane) - 1ull))] = (short)e;
+        wcount += __popcll(m);
+      }
+      if (inside) {
+        constexpr int CH = 16;
+        for (int i0 = 0; i0 < wcount; i0 += CH) {
+          float fv[CH], wv[CH];
+          unsigned hits = 0u;
 #pragma unroll
-        for (int k = 0; k < CH; ++k) {
-          const int e = e0 + k;
-          fv[k] = 0.0f;
-          wv[k] = 0.0f;
-          if (e < nent) {
-            const int pk = s_ent_pos[e];
-            const int p0 = v0 - (((pk >> 20) & 1023) - 512);
-            const int p1 = v1 - (((pk >> 10) & 1023) - 512);
-            const int p2 = v2 - ((pk & 1023) - 512);
-            if (p0 >= 0 && p0 < B0 && p1 >= 0 && p1 < B1 && p2 >= 0 && p2 < B2) {
-              const int p = (p0 * B1 + p1) * B2 + p2;
-              wv[k] = s_ent_w[e] * s_k[p];
-              fv[k] = a.filt[(int64_t)s_ent_idx[e] * P + p];
-              hits |= 1u << k;
+          for (int k = 0; k < CH; ++k) {
+            fv[k] = 0.0f;
+            wv[k] = 0.0f;
+            if (i0 + k < wcount) {
+              const int e = s_wl[wave][i0 + k];
+              const int pk = s_ent_pos[e];
+              const int p0 = v0 - (((pk >> 20) & 1023) - 512);
+              const int p1 = v1 - (((pk >> 10) & 1023) - 512);
+              const int p2 = v2 - ((pk & 1023) - 512);
+              if (p0 >= 0 && p0 < B0 && p1 >= 0 && p1 < B1 && p2 >= 0 && p2 < B2) {
+                const int p = (p0 * B1 + p1) * B2 + p2;
+                wv[k] = s_ent_w[e] * s_k[p];
+                fv[k] = a.filt[(int64_t)s_ent_idx[e] * P + p];
+                hits |= 1u << k;
+              }
             }
           }
-        }
 #pragma unroll
-        for (int k = 0; k < CH; ++k) {
-          if (hits & (1u << k)) {
-            num = num + fv[k] * wv[k];
-            den = den + wv[k];
+          for (int k = 0; k < CH; ++k) {
+            if (hits & (1u << k)) {
+              num = num + fv[k] * wv[k];
+              den = den + wv[k];
+            }
           }
         }
       }
