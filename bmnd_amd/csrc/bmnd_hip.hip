@@ -177,6 +177,7 @@ __device__ __forceinline__ float canonical_key(float d) {
 // ---------------------------------------------------------------------------
 // small block-level helpers
 // ---------------------------------------------------------------------------
+
 template <int NT>
 __device__ int block_exclusive_scan(int v, int *s_tmp, int *total) {
   // s_tmp needs NT/64 + 1 ints
@@ -1817,8 +1818,15 @@ __global__ __launch_bounds__(kAggThreads) void k_aggregate(AggArgs a) {
   __shared__ short s_wl[kAggThreads / 64][kSub];  // per wave: entries reaching its voxels
 
   const int tid = threadIdx.x;
-  const int nt2 = (a.n2 + T2 - 1) / T2, nt1 = (a.n1 + T1 - 1) / T1;
-  const int tb = blockIdx.x;
+  const int nt2 = (a.n2 + T2 - 1) / T2, nt1 = (a.n1 + T1 - 1) / T1,
+            nt0 = (a.n0 + T0 - 1) / T0;
+  // XCD-aware order: workgroups are dealt to the 8 XCDs in turn, so logical
+  // tile (x % 8) * per + x / 8 puts consecutive tiles, which share most of
+  // their groups' filtered patches, on one XCD and its L2
+  const int ntiles = nt0 * nt1 * nt2;
+  const int per = (ntiles + 7) / 8;
+  const int tb = (int)(blockIdx.x % 8u) * per + (int)(blockIdx.x / 8u);
+  if (tb >= ntiles) return;
   const int tx2 = tb % nt2, tx1 = (tb / nt2) % nt1, tx0 = tb / (nt2 * nt1);
   const int t_lo0 = tx0 * T0, t_lo1 = tx1 * T1, t_lo2 = tx2 * T2;
   const int t_hi0 = min(t_lo0 + T0, a.n0) - 1, t_hi1 = min(t_lo1 + T1, a.n1) - 1,
@@ -2451,7 +2459,7 @@ int32_t bmnd_hip_aggregate(const float *filt, const float *weights, const int32_
     const int64_t tiles = ((shape_h[0] + t0 - 1) / t0) * ((shape_h[1] + t1 - 1) / t1) *     \
                           ((shape_h[2] + t2 - 1) / t2);                                     \
     hipLaunchKernelGGL((k_aggregate<b0, b1, b2, r0, r1, r2, t0, t1, t2>),                    \
-                       dim3((unsigned)tiles), dim3(kAggThreads), 0, s, a);                  \
+                       dim3((unsigned)(8 * ((tiles + 7) / 8))), dim3(kAggThreads), 0, s, a); \
     launched = true;                                                                        \
   }
   A_CASE(1, 8, 8, 0, 19, 19, 1, 16, 32)
