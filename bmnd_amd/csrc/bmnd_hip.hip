@@ -1239,7 +1239,8 @@ __global__ __launch_bounds__(64) void k_exact_std(ExactArgs a) {
   __shared__ float4 s_g4[NMAX];
   __shared__ int s_pos[NMAX][3];      // member origins in union-box coordinates
   __shared__ unsigned s_mask0[U0], s_mask1[U1], s_mask2[U2];
-  __shared__ unsigned s_list[64 * NMAX];
+  constexpr int LCAP = 512;  // entries of one chunk held at a time (more take extra rounds)
+  __shared__ unsigned s_list[LCAP];
 
   const int lane = threadIdx.x;
   const int64_t g = blockIdx.x;
@@ -1356,23 +1357,29 @@ __global__ __launch_bounds__(64) void k_exact_std(ExactArgs a) {
       }
       const int total = __builtin_amdgcn_readlane(incl, 63);
       // this voxel's entries, members in increasing order; bit 31 marks the
-      // voxel's first entry, bits 21..26 its lane (for its variance)
+      // voxel's first entry, bits 21..26 its lane (for its variance).  The
+      // chunk's entries are listed LCAP at a time, in order.
+      for (int lb = 0; lb < total; lb += LCAP) {
       int w = incl - cnt;
       unsigned first = 0x80000000u;
       unsigned mm = m;
       while (mm) {
         const int i = __ffs(mm) - 1;
         mm &= mm - 1u;
-        const int q01 = (v0 - s_pos[i][0]) * B1 + (v1 - s_pos[i][1]);
-        const int q2 = v2 - s_pos[i][2];
-        s_list[w++] = (unsigned)(q01 * B01) | ((unsigned)(q2 * B2) << 10) | ((unsigned)i << 16) |
-                      ((unsigned)lane << 21) | first;
+        if (w >= lb && w < lb + LCAP) {
+          const int q01 = (v0 - s_pos[i][0]) * B1 + (v1 - s_pos[i][1]);
+          const int q2 = v2 - s_pos[i][2];
+          s_list[w - lb] = (unsigned)(q01 * B01) | ((unsigned)(q2 * B2) << 10) |
+                           ((unsigned)i << 16) | ((unsigned)lane << 21) | first;
+        }
+        ++w;
         first = 0u;
       }
       __syncthreads();
-      for (int e0 = 0; e0 < total; e0 += 64) {
-        const unsigned code = (e0 + lane < total) ? s_list[e0 + lane] : 0u;
-        const int cn = min(64, total - e0);
+      const int ltot = min(LCAP, total - lb);
+      for (int e0 = 0; e0 < ltot; e0 += 64) {
+        const unsigned code = (e0 + lane < ltot) ? s_list[e0 + lane] : 0u;
+        const int cn = min(64, ltot - e0);
         for (int j = 0; j < cn; ++j) {
           const unsigned cd = (unsigned)__builtin_amdgcn_readlane((int)code, j);
           const unsigned ot = cd & 1023u, oa = (cd >> 10) & 63u;
@@ -1402,6 +1409,7 @@ __global__ __launch_bounds__(64) void k_exact_std(ExactArgs a) {
         }
       }
       __syncthreads();
+      }
     }
 #pragma unroll
     for (int k = 0; k < SPL; ++k) {
