@@ -263,6 +263,7 @@ struct BMArgs {
   const double2 *certlog;    // MODE 3: {2/c_j, log c_j - ln2}, j < kFastLogN
   double cert_c3, cert_c4;   // MODES 3, 4: 1/3 and -1/4, the degree-4 log's coefficients
   double log2_tol;           // MODE 4: bound on the hardware log2 error ratio
+  const double *zsum;        // MODE 4: patch sums of c log 2c per patch origin (k_clog_patch)
   const double *term_table;  // MODE 2: (tmax+1)^2 terms, row = reference count
   int tmax;
   int32_t *sel, *counts, *accepted;
@@ -813,10 +814,11 @@ __global__ __launch_bounds__(NT) void k_blockmatch_cert(BMArgs a) {
 
 // Bound on the hardware base-2 logarithm that the fast matcher relies on:
 // |v_log_f32(x) - log2 x| <= kLog2Tol (|log2 x| + 1) for every normal float
-// x >= kLog2Floor.  k_log2_check measures it on the device over all of them;
-// the fast matcher runs only where the measurement is within the bound.
+// x >= kLog2Floor, the smallest value its float pass takes the log of.
+// k_log2_check measures it on the device over all of them; the fast matcher
+// runs only where the measurement is within the bound.
 constexpr double kLog2Tol = 0x1p-20;
-constexpr float kLog2Floor = 0x1p-100f;
+constexpr float kLog2Floor = 0x1p-126f;
 constexpr int kBMThreadsFast = 512;  // two workgroups per CU
 
 // max over x of |v_log_f32(x) - log2 x| / (|log2 x| + 1), x over every normal
@@ -841,12 +843,60 @@ __global__ void k_log2_check(const double *__restrict__ tab, unsigned long long 
   if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(worst));
 }
 
+// Patch sums of c log 2c (c = max(v, 0), float64 with the degree-4 log) for
+// every patch origin: the candidate-only part of the fast matcher's distance.
+// k_clog_rows sums B2 consecutive voxels along axis 2 (rows of n0*n1 by c2),
+// k_clog_patch sums B0 x B1 of those; the certified bound holds for any
+// summation order.
+template <int B2>
+__global__ void k_clog_rows(const float *__restrict__ vol, int64_t rows, int n2, int c2,
+                            const double *__restrict__ logtab,
+                            const double2 *__restrict__ certlog, double c3, double c4,
+                            double *__restrict__ out) {
+  __shared__ double2 s_clog[kFastLogN];
+  for (int i = threadIdx.x; i < kFastLogN; i += blockDim.x) s_clog[i] = certlog[i];
+  __syncthreads();
+  const double ln2 = logtab[3 * kLogTabN] + logtab[3 * kLogTabN + 1];
+  const int64_t total = rows * c2;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t row = i / c2;
+    const float *src = vol + row * n2 + (i - row * c2);
+    double sum = 0.0;
+#pragma unroll
+    for (int t = 0; t < B2; ++t) {
+      const float v = src[t];
+      const double c = v > 0.0f ? (double)v : 0.0;
+      sum = sum + (c > 0.0 ? c * log_deg4(2.0 * c, s_clog, ln2, c3, c4) : 0.0);
+    }
+    out[i] = sum;
+  }
+}
+
+template <int B0, int B1>
+__global__ void k_clog_patch(const double *__restrict__ rows, int n1, int c0, int c1, int c2,
+                             double *__restrict__ z) {
+  const int64_t total = (int64_t)c0 * c1 * c2;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int j2 = (int)(i % c2);
+    const int64_t r = i / c2;
+    const int j1 = (int)(r % c1), j0 = (int)(r / c1);
+    double s = 0.0;
+#pragma unroll
+    for (int t0 = 0; t0 < B0; ++t0)
+#pragma unroll
+      for (int t1 = 0; t1 < B1; ++t1) s = s + rows[((int64_t)(j0 + t0) * n1 + (j1 + t1)) * c2 + j2];
+    z[i] = s;
+  }
+}
+
 // Fast certified block matching (keys equal to MODE 0's).  A float pass gives
 // every candidate an interval [key(d - Ef), key(d + Ef)] from
 //   d = 2 (sum_p r log 2r + sum_p c log 2c - ln2 sum_p s log2 s),  s = r + c,
 // with s log2 s in float32 (hardware log2, rows of B2 terms summed in float
 // and accumulated in float64) and the other sums in float64 (degree-4 log,
-// window sums).  The K_max-th smallest upper-bound composite (K_max the keep
+// the patch sums of k_clog_patch).  The K_max-th smallest upper-bound composite (K_max the keep
 // count at the largest possible accepted count) bounds the K-th smallest
 // true composite, so only candidates whose lower-bound composite is at most
 // that cutoff, or whose interval straddles the threshold, can change the
@@ -859,22 +909,18 @@ __global__ __launch_bounds__(NT) void k_blockmatch_fast(BMArgs a) {
   constexpr int SLAB = S0 * S1 * S2;
   constexpr int BOX = (2 * R0 + 1) * (2 * R1 + 1) * (2 * R2 + 1);
   constexpr int P = B0 * B1 * B2;
-  constexpr int W2 = S2 - B2 + 1;          // window positions along axis 2
-  constexpr int XS = S0 * S1 * W2;         // axis-2 window sums of c log 2c
   constexpr int KC = (BOX + NT - 1) / NT;  // candidates per thread
-  constexpr int XCAP = XS - BOX < 256 ? XS - BOX : 256;  // exact-set composites kept for the final rank
-  constexpr int LCAP = (XS - BOX) / 2 < 256 ? (XS - BOX) / 2 : 256;  // exact-evaluation list
-  static_assert(BOX <= XS, "the key pairs reuse the window-sum buffer");
+  constexpr int XCAP = 256;                // exact-set composites kept for the final rank
+  constexpr int LCAP = 256;                // exact-evaluation list (candidate, patch sum)
+  constexpr int KX = XCAP > LCAP + LCAP / 2 ? XCAP : LCAP + LCAP / 2;
   static_assert(KC <= 32, "per-thread candidate mask");
   __shared__ float s_slab[SLAB];           // max(x, 0)
-  __shared__ double s_xsum[XS];            // window sums; then key pairs [BOX], exact-set composites
+  __shared__ double s_kbuf[BOX + KX];      // key pairs [BOX]; then the list or the exact set
   __shared__ float s_reff[P];              // max(r, 0)
-  __shared__ float s_reft[P];              // max(r, 2^-126): keeps every float-pass sum normal
   __shared__ double s_tab[kLogTabDoubles];
   __shared__ double2 s_clog[kFastLogN];    // degree-4 log
-  __shared__ double s_rpart[(P + 63) / 64];
   __shared__ float s_wmax[2][NT / 64];     // per-wave maxima of slab and reference values
-  __shared__ double s_rsum, s_errc, s_errf;
+  __shared__ double s_errc, s_errf;
   __shared__ int s_acc, s_maybe, s_nex;
   __shared__ int s_scan[NT / 64 + 1];
   BM_SEL_SCRATCH(sc);
@@ -922,7 +968,6 @@ __global__ __launch_bounds__(NT) void k_blockmatch_fast(BMArgs a) {
     const float rf = v > 0.0f ? v : 0.0f;
     rmax = fmaxf(rmax, rf);
     s_reff[p] = rf;
-    s_reft[p] = fmaxf(rf, 0x1p-126f);
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
@@ -942,42 +987,7 @@ __global__ __launch_bounds__(NT) void k_blockmatch_fast(BMArgs a) {
 
   const double ln2 = s_tab[3 * kLogTabN] + s_tab[3 * kLogTabN + 1];
   const double c3 = a.cert_c3, c4 = a.cert_c4;
-  // sum of r log 2r over the reference patch (fixed-order wave reduction)
-  if (tid < 64 * ((P + 63) / 64)) {
-    double rl = 0.0;
-    if (tid < P) {
-      const double r = (double)s_reff[tid];
-      rl = r > 0.0 ? r * log_deg4(2.0 * r, s_clog, ln2, c3, c4) : 0.0;
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) rl = rl + __shfl_xor(rl, o, 64);
-    if (lane == 0) s_rpart[tid >> 6] = rl;
-  }
-  // axis-2 window sums of c log 2c, one slab row per thread
-  for (int row = tid; row < S0 * S1; row += NT) {
-    const float *src = s_slab + row * S2;
-    double w[B2];
-#pragma unroll
-    for (int t = 0; t < B2; ++t) w[t] = 0.0;
-#pragma unroll
-    for (int i2 = 0; i2 < S2; ++i2) {
-#pragma unroll
-      for (int t = 0; t + 1 < B2; ++t) w[t] = w[t + 1];
-      const double c = (double)src[i2];
-      w[B2 - 1] = c > 0.0 ? c * log_deg4(2.0 * c, s_clog, ln2, c3, c4) : 0.0;
-      if (i2 >= B2 - 1) {
-        double sum = w[0];
-#pragma unroll
-        for (int t = 1; t < B2; ++t) sum = sum + w[t];
-        s_xsum[row * W2 + (i2 - (B2 - 1))] = sum;
-      }
-    }
-  }
-  __syncthreads();
   if (tid == 0) {
-    double rs = 0.0;
-    for (int w = 0; w < (P + 63) / 64; ++w) rs = rs + s_rpart[w];
-    s_rsum = rs;
     float rm = 0.0f, vm = 0.0f;
     for (int w = 0; w < NT / 64; ++w) {
       rm = fmaxf(rm, s_wmax[0][w]);
@@ -1003,7 +1013,8 @@ __global__ __launch_bounds__(NT) void k_blockmatch_fast(BMArgs a) {
     s_errc = ec;
     s_errf = ec + 2.0 * ln2 * (double)P * (2.0 * (2.0 * a.log2_tol + 10.0 * 0x1p-24) * Wb + 0x1p-118);
   }
-  // patch sums of c log 2c per candidate, kept in registers
+  // patch sums of c log 2c of the candidates (kept in registers) and of the
+  // reference (sum of r log 2r)
   double zc[KC];
 #pragma unroll
   for (int k = 0; k < KC; ++k) {
@@ -1011,45 +1022,75 @@ __global__ __launch_bounds__(NT) void k_blockmatch_fast(BMArgs a) {
     zc[k] = 0.0;
     if (cand < box) {
       const int j2 = cand % e2, j1 = (cand / e2) % e1, j0 = cand / (e2 * e1);
-      double z = 0.0;
-#pragma unroll
-      for (int b0 = 0; b0 < B0; ++b0)
-#pragma unroll
-        for (int b1 = 0; b1 < B1; ++b1) z = z + s_xsum[((j0 + b0) * S1 + (j1 + b1)) * W2 + j2];
-      zc[k] = z;
+      zc[k] = a.zsum[((int64_t)(lo0 + j0) * a.c1 + (lo1 + j1)) * a.c2 + (lo2 + j2)];
     }
   }
+  const double rsum = a.zsum[lin];
   __syncthreads();
-  float2 *kp = reinterpret_cast<float2 *>(s_xsum);  // (lower key, upper key), then final key
-  unsigned long long *exset = reinterpret_cast<unsigned long long *>(s_xsum) + BOX;
-  const double rsum = s_rsum, errc = s_errc, errf = s_errf;
+  float2 *kp = reinterpret_cast<float2 *>(s_kbuf);  // (lower key, upper key), then final key
+  unsigned long long *exset = reinterpret_cast<unsigned long long *>(s_kbuf) + BOX;
+  const double errc = s_errc, errf = s_errf;
   const float thr = a.stabilize ? a.thr_key : a.threshold;
   const int rel0 = o0 - lo0, rel1 = o1 - lo1, rel2 = o2 - lo2;
   const int center_local = (rel0 * e1 + rel1) * e2 + rel2;
 
   // ---- float pass ------------------------------------------------------------
+  // Patch rows outer, the thread's candidates inner: each reference row is
+  // read once per row into scalar registers (scalar loads of the read-only
+  // volume) and the candidates' row sums are independent chains.
+  int basek[KC];
+  double F[KC];
+#pragma unroll
+  for (int k = 0; k < KC; ++k) {
+    const int cand = min(tid + k * NT, box - 1);  // past the box: any valid row, result unused
+    const int j2 = cand % e2, j1 = (cand / e2) % e1, j0 = cand / (e2 * e1);
+    basek[k] = (j0 * S1 + j1) * S2 + j2;
+    F[k] = 0.0;
+  }
+  const int wave_first = __builtin_amdgcn_readfirstlane(tid & ~63);
+  using cfloat = __attribute__((address_space(4))) const float;
+  const cfloat *vref = (const cfloat *)a.vol + ((int64_t)o0 * a.n1 + o1) * a.n2 + o2;
+  // FULL: the whole search box, where every round but the last has
+  // candidates in every wave (no per-round branch)
+  auto float_rows = [&](auto full) {
+    constexpr bool FULL = decltype(full)::value;
+#pragma unroll 1
+    for (int p01 = 0; p01 < B0 * B1; ++p01) {
+      const int p0 = p01 / B1, p1 = p01 - p0 * B1;
+      const cfloat *vrow = vref + ((int64_t)p0 * a.n1 + p1) * a.n2;
+      float rt[B2];
+#pragma unroll
+      for (int p2 = 0; p2 < B2; ++p2) {
+        // max(max(v, 0), 2^-126) as a signed max of the bits (a NaN counts as 0):
+      // keeps every float-pass sum normal
+        int vb = __float_as_int(vrow[p2]);
+        if ((vb & 0x7fffffff) > 0x7f800000) vb = 0;
+        rt[p2] = __int_as_float(max(vb, (int)__float_as_uint(kLog2Floor)));
+      }
+      const int roff = (p0 * S1 + p1) * S2;
+#pragma unroll
+      for (int k = 0; k < KC; ++k) {
+        if ((FULL && (k + 1) * NT <= BOX) || k * NT + wave_first < box) {
+          const float *row = s_slab + basek[k] + roff;
+          float rs = 0.0f;
+#pragma unroll
+          for (int p2 = 0; p2 < B2; ++p2) {
+            const float s = rt[p2] + row[p2];  // >= 2^-126, a normal float
+            rs = fmaf(s, __builtin_amdgcn_logf(s), rs);
+          }
+          F[k] = F[k] + (double)rs;
+        }
+      }
+    }
+  };
+  if (box == BOX) float_rows(std::true_type{});
+  else float_rows(std::false_type{});
   int my_maybe = 0;
 #pragma unroll
   for (int k = 0; k < KC; ++k) {
     const int cand = tid + k * NT;
     if (cand < box) {
-      const int j2 = cand % e2, j1 = (cand / e2) % e1, j0 = cand / (e2 * e1);
-      const int base = (j0 * S1 + j1) * S2 + j2;
-      double F = 0.0;
-#pragma unroll 1
-      for (int p01 = 0; p01 < B0 * B1; ++p01) {
-        const int p0 = p01 / B1, p1 = p01 - p0 * B1;
-        const float *row = s_slab + base + (p0 * S1 + p1) * S2;
-        const float *rrow = s_reft + p01 * B2;
-        float rs = 0.0f;
-#pragma unroll
-        for (int p2 = 0; p2 < B2; ++p2) {
-          const float s = rrow[p2] + row[p2];  // >= 2^-126, a normal float
-          rs = fmaf(s, __builtin_amdgcn_logf(s), rs);
-        }
-        F = F + (double)rs;
-      }
-      const double df = 2.0 * (rsum + (zc[k] - ln2 * F));
+      const double df = 2.0 * (rsum + (zc[k] - ln2 * F[k]));
       bool ok_lo, ok_hi;
       const float klo = bm_key<P>(df - errf, a, ok_lo);
       const float khi = bm_key<P>(df + errf, a, ok_hi);
@@ -1085,7 +1126,7 @@ __global__ __launch_bounds__(NT) void k_blockmatch_fast(BMArgs a) {
   }
   // compacted list of (candidate, patch sum) in the free part of the buffer,
   // evaluated by consecutive lanes; in rounds of LCAP if it is longer
-  double *lz = reinterpret_cast<double *>(s_xsum) + BOX;
+  double *lz = s_kbuf + BOX;
   int *lc = reinterpret_cast<int *>(lz + LCAP);
   int total;
   const int off = block_exclusive_scan<NT>(__popc(need), s_scan, &total);
@@ -1103,35 +1144,41 @@ __global__ __launch_bounds__(NT) void k_blockmatch_fast(BMArgs a) {
     }
     __syncthreads();
     const int m = min(LCAP, total - lbase);
-    for (int e = tid; e < m; e += NT) {
+    // one wave per listed candidate: lanes take the patch terms of s log s
+    // (the certified bound holds for any summation order) and a butterfly sum
+    // leaves the same total in every lane, so the key test below is uniform
+    for (int e = tid >> 6; e < m; e += NT / 64) {
       const int cand = lc[e];
       const int j2 = cand % e2, j1 = (cand / e2) % e1, j0 = cand / (e2 * e1);
       const int base = (j0 * S1 + j1) * S2 + j2;
       double t = 0.0;
-      for (int p01 = 0; p01 < B0 * B1; ++p01) {
+      for (int p = lane; p < P; p += 64) {
+        const int p01 = p / B2, p2 = p - p01 * B2;
         const int p0 = p01 / B1, p1 = p01 - p0 * B1;
-        const float *row = s_slab + base + (p0 * S1 + p1) * S2;
-        for (int p2 = 0; p2 < B2; ++p2) {
-          const double s = (double)s_reff[p01 * B2 + p2] + (double)row[p2];
-          t = fma(-s, log_deg4(s, s_clog, ln2, c3, c4), t);
-        }
+        const double s = (double)s_reff[p] + (double)s_slab[base + (p0 * S1 + p1) * S2 + p2];
+        t = fma(-s, log_deg4(s, s_clog, ln2, c3, c4), t);
       }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) t = t + __shfl_xor(t, o, 64);
       const double dc = 2.0 * (rsum + (lz[e] + t));
       bool ok_lo, ok_hi, ok;
       const float k_lo = bm_key<P>(dc - errc, a, ok_lo);
       const float k_hi = bm_key<P>(dc + errc, a, ok_hi);
-      float key = k_lo;
-      if (k_lo != k_hi) {
-        double d = 0.0;
-        for (int p01 = 0; p01 < B0 * B1; ++p01) {
-          const int p0 = p01 / B1, p1 = p01 - p0 * B1;
-          const float *row = s_slab + base + (p0 * S1 + p1) * S2;
-          for (int p2 = 0; p2 < B2; ++p2)
-            d = bm_reference_term((double)s_reff[p01 * B2 + p2], (double)row[p2], d, s_tab);
+      if (lane == 0) {
+        float key = k_lo;
+        if (k_lo != k_hi) {
+          // the reference expression, in the reference's serial order
+          double d = 0.0;
+          for (int p01 = 0; p01 < B0 * B1; ++p01) {
+            const int p0 = p01 / B1, p1 = p01 - p0 * B1;
+            const float *row = s_slab + base + (p0 * S1 + p1) * S2;
+            for (int p2 = 0; p2 < B2; ++p2)
+              d = bm_reference_term((double)s_reff[p01 * B2 + p2], (double)row[p2], d, s_tab);
+          }
+          key = bm_key<P>(d, a, ok);
         }
-        key = bm_key<P>(d, a, ok);
+        kp[cand].x = key;
       }
-      kp[cand].x = key;
     }
     __syncthreads();
   }
@@ -1379,7 +1426,7 @@ __global__ __launch_bounds__(64) void k_exact_std(ExactArgs a) {
       const int ltot = min(LCAP, total - lb);
       for (int e0 = 0; e0 < ltot; e0 += 64) {
         const unsigned code = (e0 + lane < ltot) ? s_list[e0 + lane] : 0u;
-        const int cn = min(64, ltot - e0);
+        const int cn = __builtin_amdgcn_readfirstlane(min(64, ltot - e0));
         for (int j = 0; j < cn; ++j) {
           const unsigned cd = (unsigned)__builtin_amdgcn_readlane((int)code, j);
           const unsigned ot = cd & 1023u, oa = (cd >> 10) & 63u;
@@ -2126,10 +2173,29 @@ int fastlog_for_device(const double2 **out, int certified = 0) {
   return 0;
 }
 
+std::vector<char> g_pool_kept;  // per device: default pool keeps freed memory
+
+// Stream-ordered scratch freed by a call stays in the device's default pool
+// for the next call instead of going back to the driver at each
+// synchronisation (set once per device).
+int keep_pool_memory() {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_tab_mutex);
+  if ((int)g_pool_kept.size() <= dev) g_pool_kept.resize(dev + 1, 0);
+  if (g_pool_kept[dev]) return 0;
+  hipMemPool_t pool;
+  HIP_TRY(hipDeviceGetDefaultMemPool(&pool, dev));
+  uint64_t threshold = UINT64_MAX;
+  HIP_TRY(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold));
+  g_pool_kept[dev] = 1;
+  return 0;
+}
+
 std::vector<double> g_log2_ratio;  // per device: measured hardware log2 error ratio, -1 unknown
 
 // Measures v_log_f32 against the float64 log over every normal float from
-// kLog2Floor up (about 2e9 values, a few milliseconds) once per device.
+// kLog2Floor up (about 2.1e9 values, a few milliseconds) once per device.
 int log2_ratio_for_device(double *ratio) {
   int dev = 0;
   HIP_TRY(hipGetDevice(&dev));
@@ -2285,6 +2351,7 @@ int32_t bmnd_hip_blockmatch(const float *vol, const int64_t *shape_h, const int3
   a.cert_c3 = 1.0 / 3.0;
   a.cert_c4 = -0.25;
   a.log2_tol = log2_tol;
+  a.zsum = nullptr;
   a.term_table = terms;
   a.tmax = table_max;
   a.sel = sel;
@@ -2293,8 +2360,40 @@ int32_t bmnd_hip_blockmatch(const float *vol, const int64_t *shape_h, const int3
   hipStream_t s = (hipStream_t)stream;
   dim3 grid((unsigned)nrefs), blk(kBMThreads);
   const size_t dyn = mode == 2 ? sizeof(double) * (size_t)(table_max + 1) * (table_max + 1) : 0;
+  // MODE 4: patch sums of c log 2c for every patch origin, in stream-ordered
+  // scratch (rows of B2 voxels, then B0 x B1 of those)
+  double *rowsum = nullptr, *zsum = nullptr;
+  auto release = [&]() {
+    if (rowsum) (void)hipFreeAsync(rowsum, s);
+    if (zsum) (void)hipFreeAsync(zsum, s);
+  };
+  auto patch_sums = [&](auto b0c, auto b1c, auto b2c) -> int32_t {
+    constexpr int B0 = decltype(b0c)::value, B1 = decltype(b1c)::value, B2 = decltype(b2c)::value;
+    if (int rc = keep_pool_memory()) return rc;
+    const int64_t rows = (int64_t)a.n0 * a.n1;
+    HIP_TRY(hipMallocAsync((void **)&rowsum, sizeof(double) * (size_t)(rows * c[2]), s));
+    HIP_TRY(hipMallocAsync((void **)&zsum, sizeof(double) * (size_t)((int64_t)c[0] * c[1] * c[2]), s));
+    const unsigned blocks = 4096;
+    hipLaunchKernelGGL((k_clog_rows<B2>), dim3(blocks), dim3(256), 0, s, vol, rows, a.n2, c[2], tab,
+                       clog, a.cert_c3, a.cert_c4, rowsum);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL((k_clog_patch<B0, B1>), dim3(blocks), dim3(256), 0, s, rowsum, a.n1, c[0],
+                       c[1], c[2], zsum);
+    HIP_TRY(hipGetLastError());
+    a.zsum = zsum;
+    return 0;
+  };
 #define BM_CASE(b0, b1, b2, r0, r1, r2)                                                        \
   if (same3(block_h, b0, b1, b2) && same3(radius_h, r0, r1, r2)) {                               \
+    if (mode == 4) {                                                                             \
+      const int32_t rc = patch_sums(std::integral_constant<int, b0>{},                          \
+                                    std::integral_constant<int, b1>{},                          \
+                                    std::integral_constant<int, b2>{});                         \
+      if (rc) {                                                                                  \
+        release();                                                                               \
+        return rc;                                                                               \
+      }                                                                                          \
+    }                                                                                            \
     if (mode == 0)                                                                               \
       hipLaunchKernelGGL((k_blockmatch<b0, b1, b2, r0, r1, r2, 0, kBMThreads>), grid, blk, 0, s, \
                          a);                                                                     \
@@ -2310,7 +2409,10 @@ int32_t bmnd_hip_blockmatch(const float *vol, const int64_t *shape_h, const int3
     else                                                                                         \
       hipLaunchKernelGGL((k_blockmatch<b0, b1, b2, r0, r1, r2, 2, kBMThreads>), grid, blk, dyn,  \
                          s, a);                                                                  \
-    HIP_TRY(hipGetLastError());                                                                  \
+    const hipError_t launched = hipGetLastError();                                               \
+    release();                                                                                   \
+    if (launched != hipSuccess)                                                                  \
+      return fail(BMND_HIP_ELAUNCH, std::string("launch: ") + hipGetErrorString(launched));      \
     return 0;                                                                                    \
   }
   BM_CASE(1, 8, 8, 0, 19, 19)

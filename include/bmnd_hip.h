@@ -82,7 +82,7 @@ int32_t bmnd_hip_blockmatch(const float *vol, const int64_t *shape_h,
 
 /* Accuracy check behind block-matching mode 4 (no reference counterpart):
  * *max_ratio = max |v_log_f32(x) - log2 x| / (|log2 x| + 1) over every normal
- * float x >= 2^-100 on the current device (measured once per device and
+ * float x >= 2^-126 on the current device (measured once per device and
  * cached).  Mode 4 runs when it is <= 2^-20. */
 int32_t bmnd_hip_log2_check(double *max_ratio);
 
