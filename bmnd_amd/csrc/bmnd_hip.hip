@@ -329,14 +329,23 @@ __device__ __forceinline__ int bm_keep_count(int acc, int box, const BMArgs &a) 
   return min(req, box);
 }
 
-// Radix select over the unique composites key_of(i), i < box: on return the K
-// smallest are exactly those with (key & mask) <= prefix.  The candidate
-// index is below 2^16, so bits 16..31 are zero in every key and their passes
-// are skipped; once the bin holding the K-th key is needed in full the
-// select stops with the bits resolved so far.
-template <int NT, int BOX, class KeyOf>
-__device__ __forceinline__ void bm_radix_select(KeyOf key_of, int box, int K,
-                                                const BMSelScratch &sc,
+// The box composites held by this thread, visited as each(f) -> f(key) for
+// every one: keys in an LDS array indexed by candidate (i = tid, tid + NT, ...)
+template <int NT, class KeyOf>
+__device__ __forceinline__ auto bm_box_keys(KeyOf key_of, int box) {
+  return [=](auto f) {
+    for (int i = threadIdx.x; i < box; i += NT) f(key_of(i));
+  };
+}
+
+// Radix select over the unique box composites (each(f) visits the thread's
+// own): on return the K smallest are exactly those with
+// (key & mask) <= prefix.  The candidate index is below 2^16, so bits 16..31
+// are zero in every key and their passes are skipped; once the bin holding
+// the K-th key is needed in full the select stops with the bits resolved so
+// far.
+template <int NT, int BOX, class Each>
+__device__ __forceinline__ void bm_radix_select(Each each, int K, const BMSelScratch &sc,
                                                 unsigned long long &prefix_out,
                                                 unsigned long long &mask_out) {
   static_assert(BOX < 65536, "candidate index must fit 16 bits");
@@ -350,10 +359,9 @@ __device__ __forceinline__ void bm_radix_select(KeyOf key_of, int box, int K,
     }
     for (int i = tid; i < 256; i += NT) sc.hist[i] = 0;
     __syncthreads();
-    for (int i = tid; i < box; i += NT) {
-      const unsigned long long k = key_of(i);
+    each([&](unsigned long long k) {
       if ((k & mask) == prefix) atomicAdd(&sc.hist[(int)((k >> shift) & 255ull)], 1);
-    }
+    });
     __syncthreads();
     if (tid < 64) {
       const int c0 = sc.hist[4 * tid], c1 = sc.hist[4 * tid + 1], c2 = sc.hist[4 * tid + 2],
@@ -397,20 +405,19 @@ __device__ __forceinline__ void bm_finalize(int K, int acc, const BMArgs &a, int
 // (unique) and the accepted count, keep the K smallest the reference keeps,
 // order them and write the group of reference g: reference first, dedupe,
 // cap, power-of-two truncation (blockmatching.py:913-954, 1045-1205).
-template <int NT, int BOX, class KeyOf>
-__device__ __forceinline__ void bm_select_write(KeyOf key_of, int box, int acc, const BMArgs &a,
+template <int NT, int BOX, class Each>
+__device__ __forceinline__ void bm_select_write(Each each, int box, int acc, const BMArgs &a,
                                                 int64_t g, int lo0, int lo1, int lo2, int e1,
                                                 int e2, int center_local, const BMSelScratch &sc) {
   const int tid = threadIdx.x;
   const int K = bm_keep_count(acc, box, a);
   unsigned long long prefix, mask;
-  bm_radix_select<NT, BOX>(key_of, box, K, sc, prefix, mask);
+  bm_radix_select<NT, BOX>(each, K, sc, prefix, mask);
   if (tid == 0) *sc.cnt = 0;
   __syncthreads();
-  for (int i = tid; i < box; i += NT) {
-    const unsigned long long k = key_of(i);
+  each([&](unsigned long long k) {
     if ((k & mask) <= prefix) sc.small[atomicAdd(sc.cnt, 1)] = k;
-  }
+  });
   __syncthreads();
   if (tid < K) {
     const unsigned long long mine = sc.small[tid];
@@ -600,8 +607,8 @@ __global__ __launch_bounds__(NT) void k_blockmatch(BMArgs a) {
   if (my_acc) atomicAdd(&s_acc, my_acc);
   __syncthreads();
   BM_SEL_SCRATCH(sc);
-  bm_select_write<NT, BOX>([&](int i) { return s_keys[i]; }, box, s_acc, a, g, lo0, lo1, lo2, e1,
-                           e2, center_local, sc);
+  bm_select_write<NT, BOX>(bm_box_keys<NT>([&](int i) { return s_keys[i]; }, box), box, s_acc, a,
+                           g, lo0, lo1, lo2, e1, e2, center_local, sc);
 }
 
 // Certified block matching (keys equal to MODE 0's).  The distance is
@@ -808,8 +815,8 @@ __global__ __launch_bounds__(NT) void k_blockmatch_cert(BMArgs a) {
   if (my_acc) atomicAdd(&s_acc, my_acc);
   __syncthreads();
   BM_SEL_SCRATCH(sc);
-  bm_select_write<NT, BOX>([&](int i) { return keys[i]; }, box, s_acc, a, g, lo0, lo1, lo2, e1,
-                           e2, center_local, sc);
+  bm_select_write<NT, BOX>(bm_box_keys<NT>([&](int i) { return keys[i]; }, box), box, s_acc, a,
+                           g, lo0, lo1, lo2, e1, e2, center_local, sc);
 }
 
 // Bound on the hardware base-2 logarithm that the fast matcher relies on:
@@ -819,7 +826,9 @@ __global__ __launch_bounds__(NT) void k_blockmatch_cert(BMArgs a) {
 // runs only where the measurement is within the bound.
 constexpr double kLog2Tol = 0x1p-20;
 constexpr float kLog2Floor = 0x1p-126f;
-constexpr int kBMThreadsFast = 512;  // two workgroups per CU
+// three workgroups per CU: at most 80 VGPRs (six waves per SIMD) and
+// 51 KB of LDS for 5x5x5 patches
+constexpr int kBMThreadsFast = 512;
 
 // max over x of |v_log_f32(x) - log2 x| / (|log2 x| + 1), x over every normal
 // float in [kLog2Floor, inf), log2 x from the float64 dlog; written to *out as
@@ -904,7 +913,8 @@ __global__ void k_clog_patch(const double *__restrict__ rows, int n1, int c0, in
 // reference expression where its bound leaves the key open).  Every other
 // candidate keeps its lower key, whose threshold decision is already known.
 template <int B0, int B1, int B2, int R0, int R1, int R2, int NT>
-__global__ __launch_bounds__(NT) void k_blockmatch_fast(BMArgs a) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_blockmatch_fast(
+    BMArgs a) {
   constexpr int S0 = 2 * R0 + B0, S1 = 2 * R1 + B1, S2 = 2 * R2 + B2;
   constexpr int SLAB = S0 * S1 * S2;
   constexpr int BOX = (2 * R0 + 1) * (2 * R1 + 1) * (2 * R2 + 1);
@@ -915,7 +925,8 @@ __global__ __launch_bounds__(NT) void k_blockmatch_fast(BMArgs a) {
   constexpr int KX = XCAP > LCAP + LCAP / 2 ? XCAP : LCAP + LCAP / 2;
   static_assert(KC <= 32, "per-thread candidate mask");
   __shared__ float s_slab[SLAB];           // max(x, 0)
-  __shared__ double s_kbuf[BOX + KX];      // key pairs [BOX]; then the list or the exact set
+  __shared__ float s_klo[BOX];             // lower-bound key, then the final key
+  __shared__ double s_kbuf[KX];            // exact-evaluation list, then the exact set
   __shared__ float s_reff[P];              // max(r, 0)
   __shared__ double s_tab[kLogTabDoubles];
   __shared__ double2 s_clog[kFastLogN];    // degree-4 log
@@ -1013,22 +1024,13 @@ __global__ __launch_bounds__(NT) void k_blockmatch_fast(BMArgs a) {
     s_errc = ec;
     s_errf = ec + 2.0 * ln2 * (double)P * (2.0 * (2.0 * a.log2_tol + 10.0 * 0x1p-24) * Wb + 0x1p-118);
   }
-  // patch sums of c log 2c of the candidates (kept in registers) and of the
-  // reference (sum of r log 2r)
-  double zc[KC];
-#pragma unroll
-  for (int k = 0; k < KC; ++k) {
-    const int cand = tid + k * NT;
-    zc[k] = 0.0;
-    if (cand < box) {
-      const int j2 = cand % e2, j1 = (cand / e2) % e1, j0 = cand / (e2 * e1);
-      zc[k] = a.zsum[((int64_t)(lo0 + j0) * a.c1 + (lo1 + j1)) * a.c2 + (lo2 + j2)];
-    }
-  }
+  // patch sums of c log 2c of a candidate and of the reference (sum of r log 2r)
+  auto zsum_of = [&](int cand) {
+    const int j2 = cand % e2, j1 = (cand / e2) % e1, j0 = cand / (e2 * e1);
+    return a.zsum[((int64_t)(lo0 + j0) * a.c1 + (lo1 + j1)) * a.c2 + (lo2 + j2)];
+  };
   const double rsum = a.zsum[lin];
   __syncthreads();
-  float2 *kp = reinterpret_cast<float2 *>(s_kbuf);  // (lower key, upper key), then final key
-  unsigned long long *exset = reinterpret_cast<unsigned long long *>(s_kbuf) + BOX;
   const double errc = s_errc, errf = s_errf;
   const float thr = a.stabilize ? a.thr_key : a.threshold;
   const int rel0 = o0 - lo0, rel1 = o1 - lo1, rel2 = o2 - lo2;
@@ -1062,7 +1064,7 @@ __global__ __launch_bounds__(NT) void k_blockmatch_fast(BMArgs a) {
 #pragma unroll
       for (int p2 = 0; p2 < B2; ++p2) {
         // max(max(v, 0), 2^-126) as a signed max of the bits (a NaN counts as 0):
-      // keeps every float-pass sum normal
+        // keeps every float-pass sum normal
         int vb = __float_as_int(vrow[p2]);
         if ((vb & 0x7fffffff) > 0x7f800000) vb = 0;
         rt[p2] = __int_as_float(max(vb, (int)__float_as_uint(kLog2Floor)));
@@ -1085,48 +1087,59 @@ __global__ __launch_bounds__(NT) void k_blockmatch_fast(BMArgs a) {
   };
   if (box == BOX) float_rows(std::true_type{});
   else float_rows(std::false_type{});
+  // upper-bound keys of the thread's candidates (slot k: candidate tid + k NT)
+  // in registers, lower-bound keys in s_klo
+  float khi[KC];
   int my_maybe = 0;
 #pragma unroll
   for (int k = 0; k < KC; ++k) {
     const int cand = tid + k * NT;
+    khi[k] = 0.0f;
     if (cand < box) {
-      const double df = 2.0 * (rsum + (zc[k] - ln2 * F[k]));
+      const double df = 2.0 * (rsum + (zsum_of(cand) - ln2 * F[k]));
       bool ok_lo, ok_hi;
-      const float klo = bm_key<P>(df - errf, a, ok_lo);
-      const float khi = bm_key<P>(df + errf, a, ok_hi);
-      kp[cand] = make_float2(klo, khi);
+      s_klo[cand] = bm_key<P>(df - errf, a, ok_lo);
+      khi[k] = bm_key<P>(df + errf, a, ok_hi);
       my_maybe += ok_lo ? 1 : 0;
     }
   }
   if (my_maybe) atomicAdd(&s_maybe, my_maybe);
   __syncthreads();
-
-  // ---- cutoff: the K_max-th smallest upper-bound composite --------------------
   auto comp = [&](float key, int i) {
     if (key == 0.0f) key = 0.0f;  // a signed zero sorts as +0
     return ((unsigned long long)(key < thr ? 0u : 1u) << 63) |
            ((unsigned long long)__float_as_uint(key) << 32) | (unsigned long long)i;
   };
+  // each(f) over the composites of the thread's slots, key_of(k) the slot's
+  auto slot_keys = [&](auto key_of) {
+    return [&, key_of](auto f) {
+#pragma unroll
+      for (int k = 0; k < KC; ++k)
+        if (tid + k * NT < box) f(key_of(k));
+    };
+  };
+
+  // ---- cutoff: the K_max-th smallest upper-bound composite --------------------
   const int kmax = bm_keep_count(s_maybe, box, a);
   unsigned long long cpre, cmask;
-  bm_radix_select<NT, BOX>([&](int i) { return comp(kp[i].y, i); }, box, kmax, sc, cpre, cmask);
+  bm_radix_select<NT, BOX>(slot_keys([&](int k) { return comp(khi[k], tid + k * NT); }), kmax, sc,
+                           cpre, cmask);
   const unsigned long long cut = cpre | ~cmask;
 
   // ---- exact keys for the candidates that can change the result ---------------
-  // (lower composite at most the cutoff, or the threshold undecided); each
-  // thread evaluates its own, with its patch sum of c log 2c
+  // (lower composite at most the cutoff, or the threshold undecided)
   unsigned need = 0u;
 #pragma unroll
   for (int k = 0; k < KC; ++k) {
     const int cand = tid + k * NT;
     if (cand < box) {
-      const float2 kk = kp[cand];
-      if (comp(kk.x, cand) <= cut || ((kk.x < thr) != (kk.y < thr))) need |= 1u << k;
+      const float klo = s_klo[cand];
+      if (comp(klo, cand) <= cut || ((klo < thr) != (khi[k] < thr))) need |= 1u << k;
     }
   }
-  // compacted list of (candidate, patch sum) in the free part of the buffer,
-  // evaluated by consecutive lanes; in rounds of LCAP if it is longer
-  double *lz = s_kbuf + BOX;
+  // compacted list of (candidate, patch sum) evaluated one wave per entry, in
+  // rounds of LCAP if it is longer; the exact key replaces the lower one
+  double *lz = s_kbuf;
   int *lc = reinterpret_cast<int *>(lz + LCAP);
   int total;
   const int off = block_exclusive_scan<NT>(__popc(need), s_scan, &total);
@@ -1136,7 +1149,7 @@ __global__ __launch_bounds__(NT) void k_blockmatch_fast(BMArgs a) {
     for (int k = 0; k < KC; ++k) {
       if (need & (1u << k)) {
         if (r >= lbase && r < lbase + LCAP) {
-          lz[r - lbase] = zc[k];
+          lz[r - lbase] = zsum_of(tid + k * NT);
           lc[r - lbase] = tid + k * NT;
         }
         ++r;
@@ -1177,7 +1190,7 @@ __global__ __launch_bounds__(NT) void k_blockmatch_fast(BMArgs a) {
           }
           key = bm_key<P>(d, a, ok);
         }
-        kp[cand].x = key;
+        s_klo[cand] = key;
       }
     }
     __syncthreads();
@@ -1185,12 +1198,13 @@ __global__ __launch_bounds__(NT) void k_blockmatch_fast(BMArgs a) {
   // Final keys: exact where evaluated, else the lower key (its threshold side
   // is known and its composite lies above the cutoff).  The K smallest true
   // composites are all in the evaluated set, which is kept for the final rank.
+  unsigned long long *exset = reinterpret_cast<unsigned long long *>(s_kbuf);
   int my_acc = 0;
 #pragma unroll
   for (int k = 0; k < KC; ++k) {
     const int cand = tid + k * NT;
     if (cand < box) {
-      const float key = kp[cand].x;
+      const float key = s_klo[cand];
       my_acc += key < thr ? 1 : 0;
       if (need & (1u << k)) {
         const int slot = atomicAdd(&s_nex, 1);
@@ -1212,8 +1226,8 @@ __global__ __launch_bounds__(NT) void k_blockmatch_fast(BMArgs a) {
     __syncthreads();
     bm_finalize(K, acc, a, g, lo0, lo1, lo2, e1, e2, center_local, sc);
   } else {
-    bm_select_write<NT, BOX>([&](int i) { return comp(kp[i].x, i); }, box, acc, a, g, lo0, lo1,
-                             lo2, e1, e2, center_local, sc);
+    bm_select_write<NT, BOX>(bm_box_keys<NT>([&](int i) { return comp(s_klo[i], i); }, box), box,
+                             acc, a, g, lo0, lo1, lo2, e1, e2, center_local, sc);
   }
 }
 
