@@ -1943,8 +1943,16 @@ __global__ __launch_bounds__(kAggThreads) void k_aggregate(AggArgs a) {
       __syncthreads();
     }
   }
-  // this thread's voxel
-  const int v2 = tid % T2, v1 = (tid / T2) % T1, v0 = tid / (T2 * T1);
+  // this thread's voxel: each wave covers a compact W0 x W1 x W2 sub-box of
+  // the tile (4x4x4, or 8x8 for a 2-D tile), so fewer patches reach a wave
+  // and more of its lanes take part in each patch than with whole rows
+  constexpr int W0 = T0 == 1 ? 1 : 4, W1 = T0 == 1 ? 8 : 4, W2 = T0 == 1 ? 8 : 4;
+  static_assert(W0 * W1 * W2 == 64 && T0 % W0 == 0 && T1 % W1 == 0 && T2 % W2 == 0,
+                "tile must split into 64-voxel wave boxes");
+  const int wv = tid >> 6, wl = tid & 63;
+  const int wb2 = wv % (T2 / W2), wb1 = (wv / (T2 / W2)) % (T1 / W1),
+            wb0 = wv / ((T2 / W2) * (T1 / W1));
+  const int v2 = wb2 * W2 + wl % W2, v1 = wb1 * W1 + (wl / W2) % W1, v0 = wb0 * W0 + wl / (W2 * W1);
   const int x0 = t_lo0 + v0, x1 = t_lo1 + v1, x2 = t_lo2 + v2;
   const bool inside = x0 <= t_hi0 && x1 <= t_hi1 && x2 <= t_hi2;
   float num = 0.0f, den = 0.0f;
