@@ -2037,25 +2037,36 @@ __global__ __launch_bounds__(kAggThreads) void k_aggregate(AggArgs a) {
         if (reach) s_wl[wave][wcount + __popcll(m & ((1ull << lane) - 1ull))] = (short)e;
         wcount += __popcll(m);
       }
-      if (inside) {
-        constexpr int CH = 16;
-        for (int i0 = 0; i0 < wcount; i0 += CH) {
+      // lane j holds entry i0 + j of the wave's list (origin, filtered row,
+      // weight); each entry is then read with readlane into scalar registers
+      constexpr int CH = 16;
+      for (int i0 = 0; i0 < wcount; i0 += 64) {
+        const int cn = min(64, wcount - i0);
+        int epk = 0, eidx = 0, ewb = 0;
+        if (lane < cn) {
+          const int e = s_wl[wave][i0 + lane];
+          epk = s_ent_pos[e];
+          eidx = s_ent_idx[e];
+          ewb = __float_as_int(s_ent_w[e]);
+        }
+        for (int k0 = 0; k0 < cn; k0 += CH) {
           float fv[CH], wv[CH];
           unsigned hits = 0u;
 #pragma unroll
           for (int k = 0; k < CH; ++k) {
             fv[k] = 0.0f;
             wv[k] = 0.0f;
-            if (i0 + k < wcount) {
-              const int e = s_wl[wave][i0 + k];
-              const int pk = s_ent_pos[e];
+            if (k0 + k < cn) {
+              const int pk = __builtin_amdgcn_readlane(epk, k0 + k);
               const int p0 = v0 - (((pk >> 20) & 1023) - 512);
               const int p1 = v1 - (((pk >> 10) & 1023) - 512);
               const int p2 = v2 - ((pk & 1023) - 512);
-              if (p0 >= 0 && p0 < B0 && p1 >= 0 && p1 < B1 && p2 >= 0 && p2 < B2) {
+              const int idx = __builtin_amdgcn_readlane(eidx, k0 + k);
+              const float w = __int_as_float(__builtin_amdgcn_readlane(ewb, k0 + k));
+              if (inside && p0 >= 0 && p0 < B0 && p1 >= 0 && p1 < B1 && p2 >= 0 && p2 < B2) {
                 const int p = (p0 * B1 + p1) * B2 + p2;
-                wv[k] = s_ent_w[e] * s_k[p];
-                fv[k] = a.filt[(int64_t)s_ent_idx[e] * P + p];
+                wv[k] = w * s_k[p];
+                fv[k] = a.filt[(int64_t)idx * P + p];
                 hits |= 1u << k;
               }
             }
