@@ -1594,9 +1594,12 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
   constexpr int NP = NMAX * P;
   static_assert(NP <= 4096, "group too large");
   static_assert(B0 * B0 <= 64 && B1 * B1 <= 64 && B2 * B2 <= 64, "block too large");
-  __shared__ float s_x[NP];    // coefficients being filtered
   __shared__ float s_sig[NP];  // noise std of every coefficient
   __shared__ float s_r[NP];    // Wiener: pilot coefficients, then the gain; HT: keep mask
+  // coefficients being filtered; the Wiener stage loads its noisy patches
+  // only once the gains and risks are known, into the std buffer
+  __shared__ float s_xh[WIENER ? 1 : NP];
+  float *const s_x = WIENER ? s_sig : s_xh;
   // member matrices, rows with stride NMAX: G, G squared elementwise, G^-1
   __shared__ __align__(16) float s_g[NMAX * NMAX], s_g2[NMAX * NMAX], s_gi[NMAX * NMAX];
   __shared__ float s_sf[3][64], s_si[3][64], s_sf2[3][64];
@@ -1684,22 +1687,23 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
   }
   __syncthreads();
   const int np = n * P;
-  // ---- gather: variance, observation and (Wiener) pilot patches, all loads
-  // in flight at once -------------------------------------------------------
+  // ---- gather: variance and (HT) observation or (Wiener) pilot patches, all
+  // loads in flight at once -------------------------------------------------
+  auto voxel_of = [&](int e) {
+    const int i = e / P, p = e - i * P;
+    const int p2 = p % B2, p1 = (p / B2) % B1, p0 = p / (B2 * B1);
+    return ((int64_t)(s_pos[i][0] + p0) * a.n1 + (s_pos[i][1] + p1)) * a.n2 + (s_pos[i][2] + p2);
+  };
+  constexpr int IT = (NP + NT - 1) / NT;
   {
-    constexpr int IT = (NP + NT - 1) / NT;
-    float xv[IT], xn[IT], xp[WIENER ? IT : 1];
+    float xv[IT], xo[IT];
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
       const int e = tid + it * NT;
       if (e < np) {
-        const int i = e / P, p = e - i * P;
-        const int p2 = p % B2, p1 = (p / B2) % B1, p0 = p / (B2 * B1);
-        const int64_t v = ((int64_t)(s_pos[i][0] + p0) * a.n1 + (s_pos[i][1] + p1)) * a.n2 +
-                          (s_pos[i][2] + p2);
+        const int64_t v = voxel_of(e);
         xv[it] = a.var[v];
-        xn[it] = a.noisy[v];
-        if constexpr (WIENER) xp[it] = a.pilot[v];
+        xo[it] = WIENER ? a.pilot[v] : a.noisy[v];
       }
     }
 #pragma unroll
@@ -1707,8 +1711,8 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
       const int e = tid + it * NT;
       if (e < np) {
         s_sig[e] = xv[it] > 0.0f ? xv[it] : 0.0f;
-        s_x[e] = xn[it];
-        if constexpr (WIENER) s_r[e] = xp[it];
+        if constexpr (WIENER) s_r[e] = xo[it];
+        else s_x[e] = xo[it];
       }
     }
   }
@@ -1785,9 +1789,9 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
       return c * keep;
     });
   } else {
-    // Poisson Wiener gain, variance-scaled (wiener.py:24-53, 78-95), applied
-    // as the pilot's coefficients come out of its last pass
-    group_transform<NT, B0, B1, B2, NMAX>(s_x, n, s_g, s_sf);
+    // Poisson Wiener gain, variance-scaled (wiener.py:24-53, 78-95), computed
+    // as the pilot's coefficients come out of its last pass; the noisy
+    // coefficients take it in theirs, after the risks
     const float mu = a.shrink;
     const float eps32 = 1e-10f;
     group_transform<NT, B0, B1, B2, NMAX>(s_r, n, s_g, s_sf, [&](int e, float r) {
@@ -1800,7 +1804,6 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
       if (sp_ < 0.0f) sp_ = 0.0f;
       float gain = sp_ / ((sp_ + eff) + eps32);
       gain = gain < 0.0f ? 0.0f : (gain > 1.0f ? 1.0f : gain);
-      s_x[e] = s_x[e] * gain;
       return gain;  // the pilot coefficient's slot now holds the attenuation
     });
   }
@@ -1838,6 +1841,24 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
     float w = 0.0f;
     if (isfinite(risk) && risk >= 0.0) w = (float)(1.0 / (risk + 1e-10));
     a.weights[g * a.max_m + tid] = w;
+  }
+  if constexpr (WIENER) {
+    // noisy patches into the std buffer (its last reads were the risks,
+    // before the barrier above), transformed with the gains applied
+    float xn[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int e = tid + it * NT;
+      if (e < np) xn[it] = a.noisy[voxel_of(e)];
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int e = tid + it * NT;
+      if (e < np) s_x[e] = xn[it];
+    }
+    __syncthreads();
+    group_transform<NT, B0, B1, B2, NMAX>(s_x, n, s_g, s_sf,
+                                          [&](int e, float c) { return c * atten[e]; });
   }
   // ---- inverse transform, stored from its last pass --------------------------
   float *dst = a.filt + (g * a.max_m) * (int64_t)P;
