@@ -483,7 +483,8 @@ __device__ __forceinline__ void bm_finalize(int K, int acc, const BMArgs &a, int
 // MODE 2: integer counts; the term is read from a float64 table built on the
 //         host with the same expression and the C library's log.
 template <int B0, int B1, int B2, int R0, int R1, int R2, int MODE, int NT>
-__global__ __launch_bounds__(NT) void k_blockmatch(BMArgs a) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 6 : 1))) void k_blockmatch(
+    BMArgs a) {
   constexpr int S0 = 2 * R0 + B0, S1 = 2 * R1 + B1, S2 = 2 * R2 + B2;
   constexpr int SLAB = S0 * S1 * S2;
   constexpr int BOX = (2 * R0 + 1) * (2 * R1 + 1) * (2 * R2 + 1);
@@ -528,7 +529,11 @@ __global__ __launch_bounds__(NT) void k_blockmatch(BMArgs a) {
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
       const int i = tid + it * NT;
-      if (i < SLAB) s_slab[i] = vv[it];
+      // MODE 2: the integer count as a byte offset into a term-table row
+      if (i < SLAB) {
+        if constexpr (MODE == 2) s_slab[i] = __int_as_float((int)vv[it] * (int)sizeof(double));
+        else s_slab[i] = vv[it];
+      }
     }
   }
   for (int p = tid; p < P; p += NT) {
@@ -537,7 +542,8 @@ __global__ __launch_bounds__(NT) void k_blockmatch(BMArgs a) {
     double r = (double)v * a.inv_scale;
     if (r < 0.0) r = 0.0;
     s_ref[p] = r;
-    if (MODE == 2) s_refrow[p] = (int)v * (a.tmax + 1);
+    // MODE 2: byte offset of the reference count's term-table row
+    if (MODE == 2) s_refrow[p] = (int)v * (a.tmax + 1) * (int)sizeof(double);
   }
   if (MODE == 2) {
     const int tn = (a.tmax + 1) * (a.tmax + 1);
@@ -568,7 +574,57 @@ __global__ __launch_bounds__(NT) void k_blockmatch(BMArgs a) {
   const int rel0 = o0 - lo0, rel1 = o1 - lo1, rel2 = o2 - lo2;
   const int center_local = (rel0 * e1 + rel1) * e2 + rel2;
   int my_acc = 0;
-  for (int cand = tid; cand < box; cand += NT) {
+  if constexpr (MODE == 2) {
+    // Patch rows outer, the thread's candidates inner: a row's table-row
+    // offsets are read once (same-address reads) and each term is one integer
+    // add, one table read and one float64 add, in patch C order per candidate.
+    constexpr int KC = (BOX + NT - 1) / NT;
+    const int *slabi = reinterpret_cast<const int *>(s_slab);
+    const char *terms = reinterpret_cast<const char *>(s_terms);
+    int basek[KC];
+    double dk[KC];
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      const int cand = min(tid + k * NT, box - 1);  // past the box: any valid row, result unused
+      const int j2 = cand % e2, j1 = (cand / e2) % e1, j0 = cand / (e2 * e1);
+      basek[k] = (j0 * S1 + j1) * S2 + j2;
+      dk[k] = 0.0;
+    }
+    const int wave_first = __builtin_amdgcn_readfirstlane(tid & ~63);
+    auto table_rows = [&](auto full) {
+      constexpr bool FULL = decltype(full)::value;
+#pragma unroll 1
+      for (int p01 = 0; p01 < B0 * B1; ++p01) {
+        const int p0 = p01 / B1, p1 = p01 - p0 * B1;
+        int trow[B2];
+#pragma unroll
+        for (int p2 = 0; p2 < B2; ++p2) trow[p2] = s_refrow[p01 * B2 + p2];
+        const int roff = (p0 * S1 + p1) * S2;
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+          if ((FULL && (k + 1) * NT <= BOX) || k * NT + wave_first < box) {
+            const int *row = slabi + basek[k] + roff;
+#pragma unroll
+            for (int p2 = 0; p2 < B2; ++p2)
+              dk[k] = dk[k] + *reinterpret_cast<const double *>(terms + trow[p2] + row[p2]);
+          }
+        }
+      }
+    };
+    if (box == BOX) table_rows(std::true_type{});
+    else table_rows(std::false_type{});
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      const int cand = tid + k * NT;
+      if (cand < box) {
+        bool ok;
+        const float key = bm_key<P>(dk[k], a, ok);
+        my_acc += ok ? 1 : 0;
+        s_keys[cand] = bm_composite(key, ok, cand);
+      }
+    }
+  }
+  for (int cand = tid; MODE != 2 && cand < box; cand += NT) {
     const int j2 = cand % e2, j1 = (cand / e2) % e1, j0 = cand / (e2 * e1);
     const int base = (j0 * S1 + j1) * S2 + j2;
     double d = 0.0;
@@ -580,13 +636,9 @@ __global__ __launch_bounds__(NT) void k_blockmatch(BMArgs a) {
       const double *lcrow = s_lc + (MODE == 1 ? base + (p0 * S1 + p1) * S2 : 0);
       const double *rrow = s_ref + p01 * B2;
       const double *lrow = s_ref_log + p01 * B2;
-      const int *trow = s_refrow + p01 * B2;
 #pragma unroll
       for (int p2 = 0; p2 < B2; ++p2) {
-        if (MODE == 2) {
-          // table entry = 2*(r log(2r/s) + c log(2c/s)), 0 when r = c = 0
-          d = d + s_terms[trow[p2] + (int)row[p2]];
-        } else if (MODE == 1) {
+        if (MODE == 1) {
           // the term is exactly 0 at r = c = 0 (the log of 0 stays finite)
           const double s = rrow[p2] + (double)row[p2];
           const double ls = dlog_bits(s, s_flog, ln2k);
