@@ -28,8 +28,12 @@ EXPORTS = (
     "bmnd_hip_matrix_table_size",
     "bmnd_hip_exact_std",
     "bmnd_hip_filter_groups",
+    "bmnd_hip_filter_groups_sigma",
     "bmnd_hip_reference_map",
     "bmnd_hip_aggregate",
+    "bmnd_hip_aggregate_sigma",
+    "bmnd_hip_poisson_global_sigma",
+    "bmnd_hip_np_sum_host",
 )
 
 
@@ -63,6 +67,16 @@ def _declare(L: ctypes.CDLL) -> None:
     L.bmnd_hip_reference_map.restype = i32
     L.bmnd_hip_aggregate.argtypes = [vp, vp, vp, vp, vp, i64, i32, vp, vp, vp, vp, vp, vp, vp]
     L.bmnd_hip_aggregate.restype = i32
+    L.bmnd_hip_filter_groups_sigma.argtypes = [i32, vp, vp, vp, vp, vp, vp, vp, vp, i64, i32, vp,
+                                               vp, f32, i32, f64, vp, vp, vp, vp]
+    L.bmnd_hip_filter_groups_sigma.restype = i32
+    L.bmnd_hip_aggregate_sigma.argtypes = [vp, vp, vp, vp, vp, i64, i32, vp, vp, vp, vp, vp, vp,
+                                           vp, i32, f32, vp, vp]
+    L.bmnd_hip_aggregate_sigma.restype = i32
+    L.bmnd_hip_poisson_global_sigma.argtypes = [vp, i64, f64, ctypes.POINTER(ctypes.c_double), vp]
+    L.bmnd_hip_poisson_global_sigma.restype = i32
+    L.bmnd_hip_np_sum_host.argtypes = [vp, i64]
+    L.bmnd_hip_np_sum_host.restype = f32
 
 
 def lib() -> ctypes.CDLL:

@@ -68,7 +68,10 @@ def bmnd(volume, profile, sigma: float | None = None, sigma_psd=None,
          max_cache_bytes=CacheMode.AUTO):
     """Denoise an N-D Poisson volume (2-D and 3-D on the MI355X path).
 
-    Returns a float32 array of the input shape.  Callbacks receive copies of
+    Returns a float32 array of the input shape or, with ``return_sigma_map``,
+    ``(array, {"global": g, "ht": map, "wiener": map})`` as the reference
+    does (bmndalgo.py:1048-1062; maps of zeros and no "global" entry for an
+    all-zero input).  Callbacks receive copies of
     the HT pilot / Wiener estimate (``stage_callback``) and the block-match
     results (``blockmatch_callback``), in the reference's order."""
     if sigma is not None and sigma_psd is not None:
@@ -98,13 +101,14 @@ def bmnd(volume, profile, sigma: float | None = None, sigma_psd=None,
             return vol, {"ht": zeros, "wiener": zeros}
         return vol
     check_supported(profile, resolved)
-    if return_sigma_map:
-        raise NotImplementedError("return_sigma_map is not implemented on the MI355X path yet.")
     vol = np.ascontiguousarray(vol)
     device = torch.device("cuda", torch.cuda.current_device())
     pipe = _pipeline(profile, vol.shape, device)
     vol_t = torch.from_numpy(vol).to(device)
-    ht, wi = pipe.run(vol_t)
+    if return_sigma_map:
+        ht, wi, global_sigma = pipe.run(vol_t, sigma_maps=True)
+    else:
+        ht, wi = pipe.run(vol_t)
     if blockmatch_callback is not None:
         counts = tuple(vol.shape[d] - pipe.plans["ht"].block[d] + 1 for d in range(vol.ndim))
         blockmatch_callback("ht", build_result(ht.schedule, counts, ht.sel.cpu().numpy(),
@@ -118,4 +122,9 @@ def bmnd(volume, profile, sigma: float | None = None, sigma_psd=None,
     out = wi.est.cpu().numpy()
     if stage_callback is not None:
         stage_callback("wiener", out.copy())
+    if return_sigma_map:
+        # bmndalgo.py:1048-1062: per-stage maps, global sigma where no patch
+        # reaches a voxel
+        return out, {"global": global_sigma, "ht": ht.sigma_map.cpu().numpy(),
+                     "wiener": wi.sigma_map.cpu().numpy()}
     return out

@@ -6,7 +6,7 @@ import os
 import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = os.path.join(HERE, "csrc", "bmnd_hip.hip")
+SRCS = [os.path.join(HERE, "csrc", "bmnd_hip.hip"), os.path.join(HERE, "csrc", "bmnd_sigma.hip")]
 OUT = os.path.join(HERE, "libbmnd_hip.so")
 
 HIPCC_FLAGS = [
@@ -24,7 +24,7 @@ HIPCC_FLAGS = [
 def build(verbose: bool = False) -> str:
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     tmp = OUT + ".tmp"
-    cmd = [hipcc, *HIPCC_FLAGS, "-o", tmp, SRC]
+    cmd = [hipcc, *HIPCC_FLAGS, "-o", tmp, *SRCS]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed:\n{' '.join(cmd)}\n{res.stderr}")

@@ -11,6 +11,7 @@
 //                       shrinkage, inverse transform, weights       FP32 VALU/LDS
 //   k_reference_map     schedule position -> schedule index          HBM
 //   k_aggregate         ordered per-voxel gather + normalisation     L2/HBM
+// The global noise std of the sigma maps is computed in bmnd_sigma.hip.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -22,6 +23,8 @@
 #include <vector>
 
 #include "../../include/bmnd_hip.h"
+#include "bmnd_internal.h"
+#include "np_pairwise.h"
 
 namespace {
 
@@ -1300,6 +1303,11 @@ struct FilterArgs {
   const float *exact_std;  // (nrefs, kcap, P) exact-plane std from k_exact_std
   int kcap;
   float *filt, *weights;
+  // optional noise std for the sigma maps: HT (nrefs, max_m, P) per
+  // coefficient, Wiener one value per group, the numpy mean of its stds;
+  // pw_plans[n] is the offset of the summation plan of n*P elements
+  float *sigma_out;
+  const int32_t *pw_plans;
 };
 
 // ---------------------------------------------------------------------------
@@ -1640,7 +1648,8 @@ __device__ __forceinline__ void group_transform(float *__restrict__ data, int n,
   line_transform<NT, B2, 1>(data, n * B0 * B1, S[2], epi);
 }
 
-template <int B0, int B1, int B2, int R0, int R1, int R2, int NMAX, int NT, bool WIENER>
+template <int B0, int B1, int B2, int R0, int R1, int R2, int NMAX, int NT, bool WIENER,
+          bool SIGMA>
 __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
   constexpr int P = B0 * B1 * B2;
   constexpr int NP = NMAX * P;
@@ -1656,6 +1665,9 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
   __shared__ __align__(16) float s_g[NMAX * NMAX], s_g2[NMAX * NMAX], s_gi[NMAX * NMAX];
   __shared__ float s_sf[3][64], s_si[3][64], s_sf2[3][64];
   constexpr int KEX = NMAX < 4 ? NMAX : 4;  // exact planes staged in LDS
+  // after the plane limiting s_ex is free; the Wiener sigma-map mean uses it
+  // for the slots of its summation tree (leaves of >= 64 elements)
+  static_assert(KEX * P >= 2 * ((NP + 63) / 64), "summation slots do not fit");
   __shared__ float s_ex[KEX * P];
   __shared__ int s_pos[NMAX][3];
   __shared__ double s_plane[NMAX];
@@ -1822,6 +1834,18 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
     }
   }
   __syncthreads();
+  if constexpr (SIGMA) {
+    // the std the reference hands to its sigma-map scatter: HT the limited std
+    // of every coefficient, Wiener its float32 mean over the group
+    // (bmndalgo.py:557, 874-875)
+    if constexpr (!WIENER) {
+      float *so = a.sigma_out + (g * a.max_m) * (int64_t)P;
+      for (int e = tid; e < np; e += NT) so[e] = s_sig[e];
+    } else {
+      const float sum = np_block_sum<NT>(s_sig, a.pw_plans + a.pw_plans[n], s_ex);
+      if (tid == 0) a.sigma_out[g] = (float)((double)sum / (double)np);
+    }
+  }
 
   // ---- coefficients, with the shrinkage fused into the last transform pass ----
   float *atten = s_r;
@@ -1939,13 +1963,19 @@ struct AggArgs {
   const float *kaiser, *fallback;
   float *est;
   int *overflow;
+  // sigma map (SIGMA instantiation): sig per member coefficient, or per group
+  // when sig_per_group; voxels no patch reaches take sig_fill
+  const float *sig;
+  int sig_per_group;
+  float sig_fill;
+  float *sig_map;
 };
 
 constexpr int kAggThreads = 512;
 constexpr int kAggGroupCap = 4096;
 constexpr int kAggEntryCap = 2048;
 
-template <int B0, int B1, int B2, int R0, int R1, int R2, int T0, int T1, int T2>
+template <int B0, int B1, int B2, int R0, int R1, int R2, int T0, int T1, int T2, bool SIGMA>
 __global__ __launch_bounds__(kAggThreads) void k_aggregate(AggArgs a) {
   static_assert(T0 * T1 * T2 == kAggThreads, "tile must match the block");
   constexpr int P = B0 * B1 * B2;
@@ -2028,7 +2058,7 @@ __global__ __launch_bounds__(kAggThreads) void k_aggregate(AggArgs a) {
   const int v2 = wb2 * W2 + wl % W2, v1 = wb1 * W1 + (wl / W2) % W1, v0 = wb0 * W0 + wl / (W2 * W1);
   const int x0 = t_lo0 + v0, x1 = t_lo1 + v1, x2 = t_lo2 + v2;
   const bool inside = x0 <= t_hi0 && x1 <= t_hi1 && x2 <= t_hi2;
-  float num = 0.0f, den = 0.0f;
+  float num = 0.0f, den = 0.0f, snum = 0.0f;
   // box of this wave's voxels (tile coordinates)
   const int lane = tid & 63, wave = tid >> 6;
   int wlo0 = v0, whi0 = v0, wlo1 = v1, whi1 = v1, wlo2 = v2, whi2 = v2;
@@ -2123,12 +2153,13 @@ __global__ __launch_bounds__(kAggThreads) void k_aggregate(AggArgs a) {
           ewb = __float_as_int(s_ent_w[e]);
         }
         for (int k0 = 0; k0 < cn; k0 += CH) {
-          float fv[CH], wv[CH];
+          float fv[CH], wv[CH], sv[SIGMA ? CH : 1];
           unsigned hits = 0u;
 #pragma unroll
           for (int k = 0; k < CH; ++k) {
             fv[k] = 0.0f;
             wv[k] = 0.0f;
+            if constexpr (SIGMA) sv[k] = 0.0f;
             if (k0 + k < cn) {
               const int pk = __builtin_amdgcn_readlane(epk, k0 + k);
               const int p0 = v0 - (((pk >> 20) & 1023) - 512);
@@ -2140,6 +2171,8 @@ __global__ __launch_bounds__(kAggThreads) void k_aggregate(AggArgs a) {
                 const int p = (p0 * B1 + p1) * B2 + p2;
                 wv[k] = w * s_k[p];
                 fv[k] = a.filt[(int64_t)idx * P + p];
+                if constexpr (SIGMA)
+                  sv[k] = a.sig[a.sig_per_group ? (int64_t)(idx / a.max_m) : (int64_t)idx * P + p];
                 hits |= 1u << k;
               }
             }
@@ -2149,6 +2182,7 @@ __global__ __launch_bounds__(kAggThreads) void k_aggregate(AggArgs a) {
             if (hits & (1u << k)) {
               num = num + fv[k] * wv[k];
               den = den + wv[k];
+              if constexpr (SIGMA) snum = snum + sv[k] * wv[k];  // patchaggregation.py:229-294
             }
           }
         }
@@ -2160,6 +2194,7 @@ __global__ __launch_bounds__(kAggThreads) void k_aggregate(AggArgs a) {
   if (inside) {
     const int64_t v = ((int64_t)x0 * a.n1 + x1) * a.n2 + x2;
     a.est[v] = den > 0.0f ? num / (den + 1e-10f) : a.fallback[v];
+    if constexpr (SIGMA) a.sig_map[v] = den > 0.0f ? snum / den : a.sig_fill;  // bmndalgo.py:1048-1057
   }
 }
 
@@ -2353,7 +2388,41 @@ int check_geometry(const int64_t *shape, const int32_t *block, const int32_t *ra
   return 0;
 }
 
+// per device: summation plans of n*P elements for n = 1..max_m, behind a
+// table of their offsets (index n)
+struct PlanTable {
+  int device, P, max_m;
+  int32_t *ptr;
+};
+std::mutex g_plan_mutex;
+std::vector<PlanTable> g_plans;
+
+int group_plans_for_device(int P, int max_m, const int32_t **out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_plan_mutex);
+  for (const PlanTable &t : g_plans)
+    if (t.device == dev && t.P == P && t.max_m == max_m) {
+      *out = t.ptr;
+      return 0;
+    }
+  std::vector<int32_t> h(max_m + 1, 0);
+  for (int n = 1; n <= max_m; ++n) {
+    h[n] = (int32_t)h.size();
+    const std::vector<int32_t> plan = np_pairwise_plan(n * P);
+    h.insert(h.end(), plan.begin(), plan.end());
+  }
+  int32_t *d = nullptr;
+  HIP_TRY(hipMalloc(&d, sizeof(int32_t) * h.size()));
+  HIP_TRY(hipMemcpy(d, h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice));
+  g_plans.push_back({dev, P, max_m, d});
+  *out = d;
+  return 0;
+}
+
 }  // namespace
+
+int bmnd_internal_fail(int code, const char *msg) { return fail(code, msg); }
 
 // ===========================================================================
 // C-ABI
@@ -2581,6 +2650,19 @@ int32_t bmnd_hip_filter_groups(int32_t wiener, const float *noisy, const float *
                                const float *matrices, const float *exact_std, float shrink,
                                int32_t k, double count_scale, float *filt, float *weights,
                                void *stream) {
+  return bmnd_hip_filter_groups_sigma(wiener, noisy, pilot, var, shape_h, block_h, radius_h, sel,
+                                      counts, nrefs, max_matches, matrices, exact_std, shrink, k,
+                                      count_scale, filt, weights, nullptr, stream);
+}
+
+int32_t bmnd_hip_filter_groups_sigma(int32_t wiener, const float *noisy, const float *pilot,
+                                     const float *var, const int64_t *shape_h,
+                                     const int32_t *block_h, const int32_t *radius_h,
+                                     const int32_t *sel, const int32_t *counts, int64_t nrefs,
+                                     int32_t max_matches, const float *matrices,
+                                     const float *exact_std, float shrink, int32_t k,
+                                     double count_scale, float *filt, float *weights,
+                                     float *sigma_out, void *stream) {
   int c[3];
   if (int rc = check_geometry(shape_h, block_h, radius_h, c)) return rc;
   if (k < 0) return fail(BMND_HIP_EINVAL, "k must be non-negative");
@@ -2609,16 +2691,26 @@ int32_t bmnd_hip_filter_groups(int32_t wiener, const float *noisy, const float *
   a.kcap = std::min(k, max_matches);
   a.filt = filt;
   a.weights = weights;
+  a.sigma_out = sigma_out;
+  a.pw_plans = nullptr;
+  if (sigma_out && wiener) {
+    const int P = block_h[0] * block_h[1] * block_h[2];
+    if (int rc = group_plans_for_device(P, max_matches, &a.pw_plans)) return rc;
+  }
   hipStream_t s = (hipStream_t)stream;
   dim3 grid((unsigned)nrefs);
+#define F_LAUNCH(b0, b1, b2, r0, r1, r2, nmax, nt, w, sg)                                    \
+  hipLaunchKernelGGL((k_filter<b0, b1, b2, r0, r1, r2, nmax, nt, w, sg>), grid, dim3(nt), 0, s, a)
 #define F_CASE(b0, b1, b2, r0, r1, r2, nmax, nt)                                            \
   if (same3(block_h, b0, b1, b2) && same3(radius_h, r0, r1, r2) && max_matches <= nmax) {   \
-    if (wiener)                                                                            \
-      hipLaunchKernelGGL((k_filter<b0, b1, b2, r0, r1, r2, nmax, nt, true>), grid, dim3(nt), \
-                         0, s, a);                                                         \
+    if (wiener && sigma_out)                                                               \
+      F_LAUNCH(b0, b1, b2, r0, r1, r2, nmax, nt, true, true);                              \
+    else if (wiener)                                                                       \
+      F_LAUNCH(b0, b1, b2, r0, r1, r2, nmax, nt, true, false);                             \
+    else if (sigma_out)                                                                    \
+      F_LAUNCH(b0, b1, b2, r0, r1, r2, nmax, nt, false, true);                             \
     else                                                                                   \
-      hipLaunchKernelGGL((k_filter<b0, b1, b2, r0, r1, r2, nmax, nt, false>), grid,         \
-                         dim3(nt), 0, s, a);                                               \
+      F_LAUNCH(b0, b1, b2, r0, r1, r2, nmax, nt, false, false);                            \
     HIP_TRY(hipGetLastError());                                                            \
     return 0;                                                                              \
   }
@@ -2626,6 +2718,7 @@ int32_t bmnd_hip_filter_groups(int32_t wiener, const float *noisy, const float *
   F_CASE(4, 4, 4, 7, 7, 7, 16, 256)
   F_CASE(5, 5, 5, 7, 7, 7, 32, 512)
 #undef F_CASE
+#undef F_LAUNCH
   return fail(BMND_HIP_EINVAL, "block/radius/stack configuration not compiled into this library");
 }
 
@@ -2645,6 +2738,18 @@ int32_t bmnd_hip_aggregate(const float *filt, const float *weights, const int32_
                            int32_t max_matches, const int64_t *shape_h, const int32_t *block_h,
                            const int32_t *radius_h, const float *kaiser, const float *fallback,
                            float *est, void *stream) {
+  return bmnd_hip_aggregate_sigma(filt, weights, sel, counts, ref_index, nrefs, max_matches,
+                                  shape_h, block_h, radius_h, kaiser, fallback, est, nullptr, 0,
+                                  0.0f, nullptr, stream);
+}
+
+int32_t bmnd_hip_aggregate_sigma(const float *filt, const float *weights, const int32_t *sel,
+                                 const int32_t *counts, const int32_t *ref_index, int64_t nrefs,
+                                 int32_t max_matches, const int64_t *shape_h,
+                                 const int32_t *block_h, const int32_t *radius_h,
+                                 const float *kaiser, const float *fallback, float *est,
+                                 const float *group_sigma, int32_t per_group, float fill,
+                                 float *sigma_map, void *stream) {
   int c[3];
   if (int rc = check_geometry(shape_h, block_h, radius_h, c)) return rc;
   AggArgs a;
@@ -2664,6 +2769,13 @@ int32_t bmnd_hip_aggregate(const float *filt, const float *weights, const int32_
   a.kaiser = kaiser;
   a.fallback = fallback;
   a.est = est;
+  if ((sigma_map == nullptr) != (group_sigma == nullptr))
+    return fail(BMND_HIP_EINVAL, "the sigma map needs both group_sigma and sigma_map");
+  a.sig = group_sigma;
+  a.sig_per_group = per_group ? 1 : 0;
+  a.sig_fill = fill;
+  a.sig_map = sigma_map;
+  const bool with_sigma = sigma_map != nullptr;
   hipStream_t s = (hipStream_t)stream;
   int *d_over = nullptr;
   HIP_TRY(hipMallocAsync((void **)&d_over, sizeof(int), s));
@@ -2674,8 +2786,13 @@ int32_t bmnd_hip_aggregate(const float *filt, const float *weights, const int32_
   if (!launched && same3(block_h, b0, b1, b2) && same3(radius_h, r0, r1, r2)) {             \
     const int64_t tiles = ((shape_h[0] + t0 - 1) / t0) * ((shape_h[1] + t1 - 1) / t1) *     \
                           ((shape_h[2] + t2 - 1) / t2);                                     \
-    hipLaunchKernelGGL((k_aggregate<b0, b1, b2, r0, r1, r2, t0, t1, t2>),                    \
-                       dim3((unsigned)(8 * ((tiles + 7) / 8))), dim3(kAggThreads), 0, s, a); \
+    const dim3 grid((unsigned)(8 * ((tiles + 7) / 8)));                                     \
+    if (with_sigma)                                                                         \
+      hipLaunchKernelGGL((k_aggregate<b0, b1, b2, r0, r1, r2, t0, t1, t2, true>), grid,      \
+                         dim3(kAggThreads), 0, s, a);                                       \
+    else                                                                                    \
+      hipLaunchKernelGGL((k_aggregate<b0, b1, b2, r0, r1, r2, t0, t1, t2, false>), grid,     \
+                         dim3(kAggThreads), 0, s, a);                                       \
     launched = true;                                                                        \
   }
   A_CASE(1, 8, 8, 0, 19, 19, 1, 16, 32)

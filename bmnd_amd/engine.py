@@ -201,6 +201,7 @@ class StageOutput:
     counts: torch.Tensor
     accepted: torch.Tensor
     schedule: ScheduleArrays
+    sigma_map: torch.Tensor | None = None  # with sigma maps requested
 
 
 def blockmatch_work(plan: StagePlan, shape: tuple[int, ...]) -> tuple[int, int]:
@@ -341,8 +342,18 @@ class PoissonPipeline:
                                                        self.count_scale, self.floor, self._stream()))
         return out
 
+    def global_sigma(self, var: torch.Tensor) -> float:
+        """Global noise std of the Poisson variance model of `var` (the
+        stage-1 variance volume): the sigma maps' "global" value."""
+        g = ctypes.c_double(0.0)
+        _lib.check(_lib.lib().bmnd_hip_poisson_global_sigma(_ptr(var), var.numel(), self.floor,
+                                                            ctypes.byref(g), self._stream()))
+        return float(g.value)
+
     def stage(self, name: str, match_vol: torch.Tensor, noisy: torch.Tensor, var: torch.Tensor,
-              fallback: torch.Tensor, timer=None) -> StageOutput:
+              fallback: torch.Tensor, timer=None, sigma_fill: float | None = None) -> StageOutput:
+        """One stage.  With `sigma_fill` (the global sigma) the stage also
+        builds its sigma map, `sigma_fill` where no patch reaches a voxel."""
         L = _lib.lib()
         timer = timer or _NoTimer()
         plan = self.plans[name]
@@ -376,39 +387,53 @@ class PoissonPipeline:
         filt = torch.empty(R * plan.max_m * plan.P, dtype=torch.float32, device=dev)
         wts = torch.empty(R * plan.max_m, dtype=torch.float32, device=dev)
         wiener = name == "wiener"
+        gsig = None
+        if sigma_fill is not None:
+            # Wiener: one std per group; HT: one per member coefficient
+            gsig = torch.empty(R if wiener else R * plan.max_m * plan.P, dtype=torch.float32,
+                               device=dev)
         tok = timer.start(f"{name}.filter")
-        _lib.check(L.bmnd_hip_filter_groups(int(wiener), _ptr(noisy),
-                                            _ptr(match_vol if wiener else None), _ptr(var), shape,
-                                            block, radius, _ptr(sel), _ptr(cnt), R, plan.max_m,
-                                            _ptr(mats), _ptr(exact), plan.shrink, self.k,
-                                            self.count_scale, _ptr(filt), _ptr(wts), stream))
+        _lib.check(L.bmnd_hip_filter_groups_sigma(int(wiener), _ptr(noisy),
+                                                  _ptr(match_vol if wiener else None), _ptr(var),
+                                                  shape, block, radius, _ptr(sel), _ptr(cnt), R,
+                                                  plan.max_m, _ptr(mats), _ptr(exact),
+                                                  plan.shrink, self.k, self.count_scale,
+                                                  _ptr(filt), _ptr(wts), _ptr(gsig), stream))
         timer.stop(tok)
         del exact
         npatch = int(np.prod([self.shape3[d] - plan.block3[d] + 1 for d in range(3)]))
         ref_index = torch.empty(npatch, dtype=torch.int32, device=dev)
         _lib.check(L.bmnd_hip_reference_map(_ptr(refs), R, _ptr(ref_index), npatch, stream))
         est = torch.empty_like(noisy)
+        smap = torch.empty_like(noisy) if gsig is not None else None
         tok = timer.start(f"{name}.aggregate")
-        _lib.check(L.bmnd_hip_aggregate(_ptr(filt), _ptr(wts), _ptr(sel), _ptr(cnt),
-                                        _ptr(ref_index), R, plan.max_m, shape, block, radius,
-                                        _ptr(kai), _ptr(fallback), _ptr(est), stream))
+        _lib.check(L.bmnd_hip_aggregate_sigma(_ptr(filt), _ptr(wts), _ptr(sel), _ptr(cnt),
+                                              _ptr(ref_index), R, plan.max_m, shape, block,
+                                              radius, _ptr(kai), _ptr(fallback), _ptr(est),
+                                              _ptr(gsig), int(wiener),
+                                              float(sigma_fill or 0.0), _ptr(smap), stream))
         timer.stop(tok)
-        del filt, wts, ref_index
-        return StageOutput(est, sel.view(R, plan.max_m), cnt, acc, plan.schedule)
+        del filt, wts, ref_index, gsig
+        return StageOutput(est, sel.view(R, plan.max_m), cnt, acc, plan.schedule, smap)
 
-    def run(self, vol: torch.Tensor, timer=None) -> tuple[StageOutput, StageOutput]:
+    def run(self, vol: torch.Tensor, timer=None, sigma_maps: bool = False):
         """vol: float32 contiguous device tensor of `shape`.  Returns the HT
-        (pilot) and Wiener stage outputs."""
+        (pilot) and Wiener stage outputs and, with `sigma_maps`, the global
+        sigma as a third value (each stage output then carries its map)."""
         if vol.dtype != torch.float32 or not vol.is_contiguous() or tuple(vol.shape) != self.shape:
             raise ValueError("vol must be a contiguous float32 tensor of the planned shape")
         t = timer or _NoTimer()
         tok = t.start("ht.variance")
         var1 = self.variance(vol)
         t.stop(tok)
-        ht = self.stage("ht", vol, vol, var1, vol, timer)
+        gsig = self.global_sigma(var1) if sigma_maps else None
+        fill = None if gsig is None else float(np.float32(gsig))
+        ht = self.stage("ht", vol, vol, var1, vol, timer, fill)
         del var1
         tok = t.start("wiener.variance")
         var2 = self.variance(ht.est)
         t.stop(tok)
-        wi = self.stage("wiener", ht.est, vol, var2, vol, timer)
+        wi = self.stage("wiener", ht.est, vol, var2, vol, timer, fill)
+        if sigma_maps:
+            return ht, wi, gsig
         return ht, wi

@@ -136,6 +136,22 @@ int32_t bmnd_hip_filter_groups(int32_t wiener, const float *noisy,
                                double count_scale, float *filt, float *weights,
                                void *stream);
 
+/* bmnd_hip_filter_groups plus the noise std that bmnd(return_sigma_map=True)
+ * scatters into its sigma maps (bmndalgo.py:557, 576-595 for HT;
+ * bmndalgo.py:874-875, 979, 1001-1011 for Wiener):
+ *   sigma_out  HT: (nrefs*max_matches, P) limited std of every coefficient,
+ *              row g*max+i; Wiener: (nrefs) float32 mean of each group's
+ *              stds, in numpy's summation order (np.mean).  NULL = none. */
+int32_t bmnd_hip_filter_groups_sigma(int32_t wiener, const float *noisy,
+                                     const float *pilot, const float *var,
+                                     const int64_t *shape_h, const int32_t *block_h,
+                                     const int32_t *radius_h, const int32_t *sel,
+                                     const int32_t *counts, int64_t nrefs,
+                                     int32_t max_matches, const float *matrices,
+                                     const float *exact_std, float shrink, int32_t k,
+                                     double count_scale, float *filt, float *weights,
+                                     float *sigma_out, void *stream);
+
 /* Reference-position map used by the aggregation: ref_index[lin] = g for
  * every scheduled reference g, -1 elsewhere (num_patches entries). */
 int32_t bmnd_hip_reference_map(const int32_t *ref_lin, int64_t nrefs,
@@ -156,6 +172,42 @@ int32_t bmnd_hip_aggregate(const float *filt, const float *weights,
                            const int32_t *block_h, const int32_t *radius_h,
                            const float *kaiser, const float *fallback,
                            float *est, void *stream);
+
+/* bmnd_hip_aggregate plus one stage's sigma map: the same ordered walk sums
+ * snum += s*(w*k) with s = group_sigma[row*P + p] (per_group == 0, the HT
+ * layout of bmnd_hip_filter_groups_sigma) or group_sigma[g] (per_group != 0,
+ * the Wiener layout), and writes sigma_map = den > 0 ? snum / den : fill.
+ * Replaces patchaggregation._scatter_add_patches_with_sigma_numba and
+ * _scatter_add_patches_with_scalar_sigma_numba (patchaggregation.py:229-294,
+ * 400-470) and the sigma-map normalisation of bmndalgo.py:1048-1062; fill is
+ * the global sigma (bmnd_hip_poisson_global_sigma) as float32. */
+int32_t bmnd_hip_aggregate_sigma(const float *filt, const float *weights,
+                                 const int32_t *sel, const int32_t *counts,
+                                 const int32_t *ref_index, int64_t nrefs,
+                                 int32_t max_matches, const int64_t *shape_h,
+                                 const int32_t *block_h, const int32_t *radius_h,
+                                 const float *kaiser, const float *fallback,
+                                 float *est, const float *group_sigma,
+                                 int32_t per_group, float fill, float *sigma_map,
+                                 void *stream);
+
+/* Global noise std of the Poisson variance model, the "global" entry of the
+ * sigma maps: sigmaestimation.estimate_poisson_global_sigma
+ * (sigmaestimation.py:48-93) on the n-voxel variance volume var (device),
+ * with eps = max(variance_floor, 1e-10) as variance.build_poisson_variance_model
+ * passes it (variance.py:581-593): sqrt(max(mean, eps)) of the float32 mean,
+ * in numpy's summation order, of the finite positive values (a 10%-trimmed
+ * mean of the sorted values when they are fewer than a tenth of the finite
+ * ones and at least 3).  Synchronises the stream; *global_sigma is a host
+ * pointer. */
+int32_t bmnd_hip_poisson_global_sigma(const float *var, int64_t n,
+                                      double variance_floor, double *global_sigma,
+                                      void *stream);
+
+/* Host-only check of the summation plans: numpy's float32 order for x[0..n)
+ * (host memory) evaluated on the CPU with the plans and leaf sums the
+ * kernels use.  For tests without a GPU. */
+float bmnd_hip_np_sum_host(const float *x, int64_t n);
 
 #ifdef __cplusplus
 }

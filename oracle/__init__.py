@@ -106,6 +106,15 @@ def _declare(L: ctypes.CDLL) -> None:
     L.oracle_bmnd_poisson.argtypes = [vp, ctypes.POINTER(OracleParams), vp, vp,
                                       vp, vp, i64, vp, vp, i64]
     L.oracle_bmnd_poisson.restype = i32
+    L.oracle_bmnd_poisson_sigma.argtypes = [vp, ctypes.POINTER(OracleParams), vp, vp,
+                                            vp, vp, i64, vp, vp, i64, vp, vp, vp]
+    L.oracle_bmnd_poisson_sigma.restype = i32
+    L.oracle_np_sum_f32.argtypes = [vp, i64]
+    L.oracle_np_sum_f32.restype = f32
+    L.oracle_np_mean_f32.argtypes = [vp, i64]
+    L.oracle_np_mean_f32.restype = f32
+    L.oracle_poisson_global_sigma.argtypes = [vp, i64, f64]
+    L.oracle_poisson_global_sigma.restype = f64
 
 
 # ---------------------------------------------------------------------------
@@ -366,6 +375,43 @@ def default_params(ndim: int, shape, nthreads: int = 0) -> OracleParams:
     p.ht_spatial = p._keep[0].ctypes.data
     p.wi_spatial = p._keep[1].ctypes.data
     return p
+
+
+def np_sum_f32(a: np.ndarray) -> np.float32:
+    """numpy's float32 summation order, restated (np.sum of a contiguous array)."""
+    v = np.ascontiguousarray(a, dtype=np.float32).ravel()
+    return np.float32(lib().oracle_np_sum_f32(_p(v), v.size))
+
+
+def np_mean_f32(a: np.ndarray) -> np.float32:
+    v = np.ascontiguousarray(a, dtype=np.float32).ravel()
+    return np.float32(lib().oracle_np_mean_f32(_p(v), v.size))
+
+
+def poisson_global_sigma(var_volume: np.ndarray, floor: float = 1e-10) -> float:
+    v = np.ascontiguousarray(var_volume, dtype=np.float32).ravel()
+    return float(lib().oracle_poisson_global_sigma(_p(v), v.size, floor))
+
+
+def bmnd_poisson_sigma_maps(vol: np.ndarray, params: OracleParams | None = None, *,
+                            nthreads: int = 0):
+    """The oracle's pipeline with the reference's ``return_sigma_map`` outputs:
+    ``(out, {"global": g, "ht": map, "wiener": map})`` (maps of zeros and no
+    "global" entry for an all-zero input, as the reference returns)."""
+    v = np.ascontiguousarray(vol, dtype=np.float32)
+    if params is None:
+        params = default_params(v.ndim, v.shape, nthreads)
+    out = np.empty_like(v)
+    hts, wis = np.empty_like(v), np.empty_like(v)
+    g = ctypes.c_double(0.0)
+    rc = lib().oracle_bmnd_poisson_sigma(_p(v), ctypes.byref(params), _p(out), None,
+                                         None, None, 0, None, None, 0, _p(hts), _p(wis),
+                                         ctypes.byref(g))
+    if rc != 0:
+        raise RuntimeError(f"oracle_bmnd_poisson_sigma failed with code {rc}")
+    if not np.any(v != 0.0):
+        return out, {"ht": hts, "wiener": wis}
+    return out, {"global": g.value, "ht": hts, "wiener": wis}
 
 
 def bmnd_poisson(vol: np.ndarray, params: OracleParams | None = None, *, nthreads: int = 0,

@@ -942,6 +942,78 @@ void oracle_scatter_add(float *num, float *den, int ndim, const int64_t *vol_sha
   }
 }
 
+/* numpy's float32 summation order for a contiguous array (np.sum / np.mean,
+ * the pairwise sum of numpy's umath loops: blocks of up to 128 elements
+ * summed with 8 interleaved accumulators, larger ranges halved at a multiple
+ * of 8), the order the reference's np.mean calls use
+ * (sigmaestimation.py:62-71, bmndalgo.py:875). */
+static float np_pairwise_f32(const float *a, int64_t n) {
+  if (n < 8) {
+    float r = 0.0f;
+    for (int64_t i = 0; i < n; ++i) r += a[i];
+    return r;
+  }
+  if (n <= 128) {
+    float r[8];
+    for (int j = 0; j < 8; ++j) r[j] = a[j];
+    int64_t i = 8;
+    for (; i < n - (n % 8); i += 8)
+      for (int j = 0; j < 8; ++j) r[j] += a[i + j];
+    float res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+    for (; i < n; ++i) res += a[i];
+    return res;
+  }
+  int64_t n2 = n / 2;
+  n2 -= n2 % 8;
+  return np_pairwise_f32(a, n2) + np_pairwise_f32(a + n2, n - n2);
+}
+
+/* np.sum: the reduction visits the array in buffer-sized chunks of 8192
+ * elements, each summed pairwise and added in turn to the running total. */
+float oracle_np_sum_f32(const float *a, int64_t n) {
+  float r = 0.0f;
+  for (int64_t i = 0; i < n; i += 8192) r += np_pairwise_f32(a + i, n - i < 8192 ? n - i : 8192);
+  return r;
+}
+
+/* np.mean of a float32 array: the float32 sum divided by the count (in
+ * float64 and rounded, which for a float32 sum equals the correctly rounded
+ * float32 quotient) */
+float oracle_np_mean_f32(const float *a, int64_t n) {
+  return (float)((double)oracle_np_sum_f32(a, n) / (double)n);
+}
+
+/* oracle_scatter_add with the sigma-weighted sum of the reference's sigma map
+ * (patchaggregation.py:229-294): sigma_num[x_i + p] += s(i,p) * (w_i * k[p]),
+ * s(i,p) = group_sigma[i,p], or the scalar sigma when group_sigma is NULL. */
+void oracle_scatter_add_sigma(float *num, float *den, float *sigma_num, int ndim,
+                              const int64_t *vol_shape, const int *block, const float *kaiser,
+                              const float *filt, const float *weights, const float *group_sigma,
+                              float sigma, const int64_t *lins, int n) {
+  int64_t counts[ORACLE_MAX_NDIM], vstr[ORACLE_MAX_NDIM];
+  for (int d = 0; d < ndim; ++d) counts[d] = vol_shape[d] - block[d] + 1;
+  c_strides(ndim, vol_shape, vstr);
+  int P = (int)prod_int(block, ndim);
+  for (int i = 0; i < n; ++i) {
+    int64_t lin = lins[i], base = 0;
+    for (int d = ndim - 1; d >= 0; --d) {
+      base += (lin % counts[d]) * vstr[d];
+      lin /= counts[d];
+    }
+    for (int p = 0; p < P; ++p) {
+      int lc[ORACLE_MAX_NDIM];
+      local_coords(ndim, block, p, lc);
+      int64_t v = base;
+      for (int d = 0; d < ndim; ++d) v += lc[d] * vstr[d];
+      float ww = weights[i] * kaiser[p];
+      num[v] += filt[(int64_t)i * P + p] * ww;
+      den[v] += ww;
+      const float sv = group_sigma ? group_sigma[(int64_t)i * P + p] : sigma;
+      sigma_num[v] += sv * ww;
+    }
+  }
+}
+
 /* ======================================================================== */
 /* full two-stage pipeline (bmndalgo.py:71-1063, Poisson branches)            */
 /* ======================================================================== */
@@ -1042,7 +1114,8 @@ static void variance_volume(const float *v, int64_t n, double scale, double floo
 /* one stage: block matching + group filtering + sequential aggregation */
 static int run_stage(const oracle_params *prm, int wiener, const float *match_vol,
                      const float *noisy, const float *var_vol, const float *fallback,
-                     float *est, int64_t *sel_dump, int32_t *cnt_dump, int64_t dump_cap) {
+                     float *est, int64_t *sel_dump, int32_t *cnt_dump, int64_t dump_cap,
+                     float *sigma_map, float sigma_fill) {
   const int *block = wiener ? prm->wi_block : prm->ht_block;
   const int *step = wiener ? prm->wi_step : prm->ht_step;
   const int *radius = wiener ? prm->wi_radius : prm->ht_radius;
@@ -1078,6 +1151,14 @@ static int run_stage(const oracle_params *prm, int wiener, const float *match_vo
   for (int64_t r = 0; r < nrefs; ++r) off[r + 1] = off[r] + cnt[r];
   float *filt = (float *)malloc(sizeof(float) * (size_t)(off[nrefs] * P + 1));
   float *wts = (float *)malloc(sizeof(float) * (size_t)(off[nrefs] + 1));
+  /* sigma map: HT scatters each group's noise std element-wise
+   * (bmndalgo.py:576-598), Wiener the mean of it as one value per group
+   * (bmndalgo.py:874-875, 1001-1012) */
+  float *gsig = NULL, *gsig1 = NULL;
+  if (sigma_map) {
+    if (wiener) gsig1 = (float *)malloc(sizeof(float) * (size_t)(nrefs + 1));
+    else gsig = (float *)malloc(sizeof(float) * (size_t)(off[nrefs] * P + 1));
+  }
 
 #ifdef _OPENMP
 #pragma omp parallel
@@ -1133,6 +1214,8 @@ static int run_stage(const oracle_params *prm, int wiener, const float *match_vo
       float *fo = filt + off[r] * P;
       memcpy(fo, X, sizeof(float) * (size_t)n * P);
       oracle_patch_weights(att, sig, n, P, ginv, 1e-10, wts + off[r]);
+      if (gsig) memcpy(gsig + off[r] * P, sig, sizeof(float) * (size_t)n * P);
+      if (gsig1) gsig1[r] = oracle_np_mean_f32(sig, (int64_t)n * P);
     }
     free(gfwd);
     free(ginv);
@@ -1146,12 +1229,25 @@ static int run_stage(const oracle_params *prm, int wiener, const float *match_vo
   /* sequential aggregation in schedule order (patchaggregation.py:170-199) */
   float *num = (float *)calloc((size_t)nvox, sizeof(float));
   float *den = (float *)calloc((size_t)nvox, sizeof(float));
-  for (int64_t r = 0; r < nrefs; ++r)
-    oracle_scatter_add(num, den, ndim, g.shape, block, g.kaiser, filt + off[r] * P, wts + off[r],
-                       sel + r * max_m, cnt[r]);
+  float *snum = sigma_map ? (float *)calloc((size_t)nvox, sizeof(float)) : NULL;
+  for (int64_t r = 0; r < nrefs; ++r) {
+    if (snum)
+      oracle_scatter_add_sigma(num, den, snum, ndim, g.shape, block, g.kaiser, filt + off[r] * P,
+                               wts + off[r], gsig ? gsig + off[r] * P : NULL,
+                               gsig1 ? gsig1[r] : 0.0f, sel + r * max_m, cnt[r]);
+    else
+      oracle_scatter_add(num, den, ndim, g.shape, block, g.kaiser, filt + off[r] * P,
+                         wts + off[r], sel + r * max_m, cnt[r]);
+  }
   /* normalisation, bmndalgo.py:626-631 / 1039-1044 */
   float eps32 = (float)1e-10;
   for (int64_t v = 0; v < nvox; ++v) est[v] = den[v] > 0.0f ? num[v] / (den[v] + eps32) : fallback[v];
+  /* sigma map, bmndalgo.py:1048-1057: the global sigma where nothing landed */
+  if (snum)
+    for (int64_t v = 0; v < nvox; ++v) sigma_map[v] = den[v] > 0.0f ? snum[v] / den[v] : sigma_fill;
+  free(snum);
+  free(gsig);
+  free(gsig1);
 
   free(num);
   free(den);
@@ -1166,9 +1262,56 @@ static int run_stage(const oracle_params *prm, int wiener, const float *match_vo
   return 0;
 }
 
-int oracle_bmnd_poisson(const float *vol, const oracle_params *prm, float *out,
-                        float *pilot, int64_t *ht_sel, int32_t *ht_cnt, int64_t ht_cap,
-                        int64_t *wi_sel, int32_t *wi_cnt, int64_t wi_cap) {
+/* Poisson global sigma (sigmaestimation.py:48-93) of a variance volume whose
+ * every value is positive (the variance floor is positive): the square root
+ * of its float32 mean */
+static int cmp_f32(const void *a, const void *b) {
+  const float x = *(const float *)a, y = *(const float *)b;
+  return (x > y) - (x < y);
+}
+
+/* estimate_poisson_global_sigma (sigmaestimation.py:48-93) on the variance
+ * volume with eps = max(floor, 1e-10) (variance.py:581-593): the float32 mean
+ * of the finite positive values -- a 10%-trimmed mean of the sorted values
+ * when they are fewer than a tenth of the finite ones (and at least 3) --
+ * then sqrt(max(mean, eps)).  The reference's IQR / MAD fallbacks run only
+ * for a result <= 0, which eps > 0 rules out. */
+double oracle_poisson_global_sigma(const float *var_vol, int64_t n, double floor_) {
+  const double eps = floor_ > 1e-10 ? floor_ : 1e-10;
+  int64_t nfin = 0, npos = 0;
+  for (int64_t i = 0; i < n; ++i)
+    if (isfinite(var_vol[i])) {
+      ++nfin;
+      if (var_vol[i] > 0.0f) ++npos;
+    }
+  double mean = 0.0;  /* no positive value: the mean of zeros */
+  if (npos == n) {
+    mean = (double)oracle_np_mean_f32(var_vol, n);
+  } else if (npos > 0) {
+    float *pos = (float *)malloc(sizeof(float) * (size_t)npos);
+    int64_t j = 0;
+    for (int64_t i = 0; i < n; ++i)
+      if (isfinite(var_vol[i]) && var_vol[i] > 0.0f) pos[j++] = var_vol[i];
+    const double frac = (double)npos / (double)(nfin > 1 ? nfin : 1);
+    if (frac < 0.1 && npos >= 3) {
+      qsort(pos, (size_t)npos, sizeof(float), cmp_f32);
+      int64_t trim = (int64_t)(0.1 * (double)npos);
+      if (trim < 1) trim = 1;
+      mean = npos - 2 * trim > 0 ? (double)oracle_np_mean_f32(pos + trim, npos - 2 * trim)
+                                 : (double)oracle_np_mean_f32(pos, npos);
+    } else {
+      mean = (double)oracle_np_mean_f32(pos, npos);
+    }
+    free(pos);
+  }
+  const double g = sqrt(eps > mean ? eps : mean);
+  return g > 0.0 ? g : 0.0;
+}
+
+int oracle_bmnd_poisson_sigma(const float *vol, const oracle_params *prm, float *out,
+                              float *pilot, int64_t *ht_sel, int32_t *ht_cnt, int64_t ht_cap,
+                              int64_t *wi_sel, int32_t *wi_cnt, int64_t wi_cap,
+                              float *ht_sigma_map, float *wi_sigma_map, double *global_sigma) {
   if (prm->ndim < 1 || prm->ndim > ORACLE_MAX_NDIM) return -1;
   int64_t nvox = 1;
   for (int d = 0; d < prm->ndim; ++d) nvox *= prm->shape[d];
@@ -1184,16 +1327,28 @@ int oracle_bmnd_poisson(const float *vol, const oracle_params *prm, float *out,
   if (!any) { /* all-zero fast path, bmndalgo.py:160-171 */
     memcpy(out, vol, sizeof(float) * (size_t)nvox);
     if (pilot) memcpy(pilot, vol, sizeof(float) * (size_t)nvox);
+    if (ht_sigma_map) memset(ht_sigma_map, 0, sizeof(float) * (size_t)nvox);
+    if (wi_sigma_map) memset(wi_sigma_map, 0, sizeof(float) * (size_t)nvox);
+    if (global_sigma) *global_sigma = 0.0;
     return 0;
   }
   float *var1 = (float *)malloc(sizeof(float) * (size_t)nvox);
   float *est1 = (float *)malloc(sizeof(float) * (size_t)nvox);
   variance_volume(vol, nvox, prm->count_scale, prm->variance_floor, var1);
-  run_stage(prm, 0, vol, vol, var1, vol, est1, ht_sel, ht_cnt, ht_cap);
+  const double gs = oracle_poisson_global_sigma(var1, nvox, prm->variance_floor);
+  if (global_sigma) *global_sigma = gs;
+  run_stage(prm, 0, vol, vol, var1, vol, est1, ht_sel, ht_cnt, ht_cap, ht_sigma_map, (float)gs);
   if (pilot) memcpy(pilot, est1, sizeof(float) * (size_t)nvox);
   variance_volume(est1, nvox, prm->count_scale, prm->variance_floor, var1);
-  run_stage(prm, 1, est1, vol, var1, vol, out, wi_sel, wi_cnt, wi_cap);
+  run_stage(prm, 1, est1, vol, var1, vol, out, wi_sel, wi_cnt, wi_cap, wi_sigma_map, (float)gs);
   free(var1);
   free(est1);
   return 0;
+}
+
+int oracle_bmnd_poisson(const float *vol, const oracle_params *prm, float *out,
+                        float *pilot, int64_t *ht_sel, int32_t *ht_cnt, int64_t ht_cap,
+                        int64_t *wi_sel, int32_t *wi_cnt, int64_t wi_cap) {
+  return oracle_bmnd_poisson_sigma(vol, prm, out, pilot, ht_sel, ht_cnt, ht_cap, wi_sel, wi_cnt,
+                                   wi_cap, NULL, NULL, NULL);
 }

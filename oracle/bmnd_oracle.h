@@ -78,6 +78,19 @@ void oracle_scatter_add(float *num, float *den, int ndim, const int64_t *vol_sha
                         const int *block, const float *kaiser, const float *filt,
                         const float *weights, const int64_t *lins, int n);
 
+/* numpy's float32 pairwise sum / mean of a contiguous array (np.sum, np.mean) */
+float oracle_np_sum_f32(const float *a, int64_t n);
+float oracle_np_mean_f32(const float *a, int64_t n);
+/* scatter with the sigma-weighted sum; group_sigma (n x P) or, when NULL,
+ * the scalar sigma for every element */
+void oracle_scatter_add_sigma(float *num, float *den, float *sigma_num, int ndim,
+                              const int64_t *vol_shape, const int *block,
+                              const float *kaiser, const float *filt,
+                              const float *weights, const float *group_sigma,
+                              float sigma, const int64_t *lins, int n);
+/* Poisson global sigma of a variance volume with a positive floor */
+double oracle_poisson_global_sigma(const float *var_vol, int64_t n, double floor_);
+
 /* ---- full pipeline ----------------------------------------------------- */
 typedef struct {
   int ndim;
@@ -110,6 +123,13 @@ int oracle_bmnd_poisson(const float *vol, const oracle_params *prm,
                         float *out, float *pilot, int64_t *ht_sel,
                         int32_t *ht_cnt, int64_t ht_cap, int64_t *wi_sel,
                         int32_t *wi_cnt, int64_t wi_cap);
+/* As oracle_bmnd_poisson, with the reference's return_sigma_map outputs:
+ * the per-stage sigma maps (NULL to skip) and the global sigma. */
+int oracle_bmnd_poisson_sigma(const float *vol, const oracle_params *prm,
+                              float *out, float *pilot, int64_t *ht_sel,
+                              int32_t *ht_cnt, int64_t ht_cap, int64_t *wi_sel,
+                              int32_t *wi_cnt, int64_t wi_cap, float *ht_sigma_map,
+                              float *wi_sigma_map, double *global_sigma);
 
 #ifdef __cplusplus
 }
