@@ -5,7 +5,7 @@ Drop-in for the reference package's public API (src/bmnd/__init__.py):
 hand-written HIP kernels (libbmnd_hip.so) on one GPU.
 """
 
-from .bmndalgo import bmnd
+from .bmndalgo import bmnd, bmnd_batch
 from .enums import (
     AggregationWeightDomain,
     AggregationWeightModel,
@@ -43,4 +43,5 @@ __all__ = [
     "TransformMode",
     "TransformType",
     "bmnd",
+    "bmnd_batch",
 ]

@@ -34,6 +34,12 @@ EXPORTS = (
     "bmnd_hip_aggregate_sigma",
     "bmnd_hip_poisson_global_sigma",
     "bmnd_hip_np_sum_host",
+    "bmnd_hip_aggregate_tile",
+    "bmnd_hip_blockmatch_batch",
+    "bmnd_hip_exact_std_batch",
+    "bmnd_hip_filter_groups_batch",
+    "bmnd_hip_reference_map_batch",
+    "bmnd_hip_aggregate_batch",
 )
 
 
@@ -77,6 +83,22 @@ def _declare(L: ctypes.CDLL) -> None:
     L.bmnd_hip_poisson_global_sigma.restype = i32
     L.bmnd_hip_np_sum_host.argtypes = [vp, i64]
     L.bmnd_hip_np_sum_host.restype = f32
+    L.bmnd_hip_aggregate_tile.argtypes = [vp, vp]
+    L.bmnd_hip_aggregate_tile.restype = i32
+    L.bmnd_hip_blockmatch_batch.argtypes = [vp, vp, vp, i32, vp, vp, vp, vp, i64, f32, i32, i32,
+                                            i32, i32, f64, i32, i32, vp, vp, vp, vp]
+    L.bmnd_hip_blockmatch_batch.restype = i32
+    L.bmnd_hip_exact_std_batch.argtypes = [vp, vp, vp, i32, vp, vp, vp, vp, vp, i64, i32, vp,
+                                           i32, vp, vp]
+    L.bmnd_hip_exact_std_batch.restype = i32
+    L.bmnd_hip_filter_groups_batch.argtypes = [i32, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp, vp,
+                                               i64, i32, vp, vp, f32, i32, f64, vp, vp, vp]
+    L.bmnd_hip_filter_groups_batch.restype = i32
+    L.bmnd_hip_reference_map_batch.argtypes = [vp, vp, i64, vp, vp, i64, vp]
+    L.bmnd_hip_reference_map_batch.restype = i32
+    L.bmnd_hip_aggregate_batch.argtypes = [vp, vp, vp, vp, vp, i64, i32, vp, vp, i32, vp, vp, vp,
+                                           vp, vp, vp]
+    L.bmnd_hip_aggregate_batch.restype = i32
 
 
 def lib() -> ctypes.CDLL:

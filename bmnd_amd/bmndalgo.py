@@ -16,7 +16,7 @@ import torch
 
 from .blockmatching import BlockMatchResult, build_result
 from .enums import CacheMode, coerce_enum
-from .engine import PoissonPipeline, check_supported
+from .engine import BatchPipeline, PoissonPipeline, check_supported
 from .profilevalidation import resolve_profile
 
 
@@ -61,19 +61,9 @@ def _pipeline(profile, shape, device) -> PoissonPipeline:
     return pipe
 
 
-def bmnd(volume, profile, sigma: float | None = None, sigma_psd=None,
-         return_sigma_map: bool = False,
-         stage_callback: Callable[[str, np.ndarray], None] | None = None,
-         blockmatch_callback: Callable[[str, BlockMatchResult], None] | None = None,
-         max_cache_bytes=CacheMode.AUTO):
-    """Denoise an N-D Poisson volume (2-D and 3-D on the MI355X path).
-
-    Returns a float32 array of the input shape or, with ``return_sigma_map``,
-    ``(array, {"global": g, "ht": map, "wiener": map})`` as the reference
-    does (bmndalgo.py:1048-1062; maps of zeros and no "global" entry for an
-    all-zero input).  Callbacks receive copies of
-    the HT pilot / Wiener estimate (``stage_callback``) and the block-match
-    results (``blockmatch_callback``), in the reference's order."""
+def _validated_input(volume, profile, sigma, sigma_psd, max_cache_bytes):
+    """The reference's argument checks, in its order (bmndalgo.py:124-155):
+    returns the float32 input and the resolved profile."""
     if sigma is not None and sigma_psd is not None:
         raise ValueError("sigma and sigma_psd are mutually exclusive.")
     if sigma is not None:
@@ -92,6 +82,23 @@ def bmnd(volume, profile, sigma: float | None = None, sigma_psd=None,
         raise ValueError("volume values must be representable as finite float32 values.")
     if resolved.is_poisson and np.any(vol < 0.0):
         raise ValueError("Direct Poisson input must contain only non-negative values.")
+    return vol, resolved
+
+
+def bmnd(volume, profile, sigma: float | None = None, sigma_psd=None,
+         return_sigma_map: bool = False,
+         stage_callback: Callable[[str, np.ndarray], None] | None = None,
+         blockmatch_callback: Callable[[str, BlockMatchResult], None] | None = None,
+         max_cache_bytes=CacheMode.AUTO):
+    """Denoise an N-D Poisson volume (2-D and 3-D on the MI355X path).
+
+    Returns a float32 array of the input shape or, with ``return_sigma_map``,
+    ``(array, {"global": g, "ht": map, "wiener": map})`` as the reference
+    does (bmndalgo.py:1048-1062; maps of zeros and no "global" entry for an
+    all-zero input).  Callbacks receive copies of
+    the HT pilot / Wiener estimate (``stage_callback``) and the block-match
+    results (``blockmatch_callback``), in the reference's order."""
+    vol, resolved = _validated_input(volume, profile, sigma, sigma_psd, max_cache_bytes)
     if not np.any(vol != 0.0):
         if stage_callback is not None:
             stage_callback("ht", vol.copy())
@@ -128,3 +135,32 @@ def bmnd(volume, profile, sigma: float | None = None, sigma_psd=None,
         return out, {"global": global_sigma, "ht": ht.sigma_map.cpu().numpy(),
                      "wiener": wi.sigma_map.cpu().numpy()}
     return out
+
+
+def bmnd_batch(volumes, profile, sigma: float | None = None,
+               max_cache_bytes=CacheMode.AUTO) -> list[np.ndarray]:
+    """Denoise several inputs of one dimensionality and one profile, of any
+    shapes, as one batch: every kernel runs once over the references (or
+    aggregation tiles) of all inputs together, which fills the GPU with
+    inputs too small to fill it alone.  No counterpart in the reference; each
+    input is checked as bmnd() checks it and its result is bmnd(input,
+    profile)'s, bit for bit.  Returns float32 arrays in input order."""
+    volumes = list(volumes)
+    if len({np.ndim(v) for v in volumes}) > 1:
+        raise ValueError("every input of a batch must have the same number of dimensions.")
+    items = [_validated_input(v, profile, sigma, None, max_cache_bytes) for v in volumes]
+    outs: list = [None] * len(items)
+    live = []
+    for i, (vol, resolved) in enumerate(items):
+        if not np.any(vol != 0.0):
+            outs[i] = vol  # the reference's all-zero early return
+        else:
+            check_supported(profile, resolved)
+            live.append(i)
+    if live:
+        device = torch.device("cuda", torch.cuda.current_device())
+        pipe = BatchPipeline(profile, [items[i][0].shape for i in live], device)
+        _, wi = pipe.run(pipe.pack([items[i][0] for i in live]))
+        for j, res in enumerate(pipe.unpack(wi.est)):
+            outs[live[j]] = res
+    return outs

@@ -270,7 +270,25 @@ struct BMArgs {
   const double *term_table;  // MODE 2: (tmax+1)^2 terms, row = reference count
   int tmax;
   int32_t *sel, *counts, *accepted;
+  // batch: every reference's input (nullptr for a single input)
+  const bmnd_hip_input *batch;
+  const int32_t *ref_input;
 };
+
+// A batched launch takes the geometry of reference g's input and moves the
+// volume pointers to that input's part of the packed buffers.
+__device__ __forceinline__ void bind_input(BMArgs &a, int64_t g) {
+  if (!a.batch) return;
+  const bmnd_hip_input &d = a.batch[a.ref_input[g]];
+  a.vol += d.vox_off;
+  if (a.zsum) a.zsum += d.grid_off;
+  a.n0 = d.shape[0];
+  a.n1 = d.shape[1];
+  a.n2 = d.shape[2];
+  a.c0 = d.grid[0];
+  a.c1 = d.grid[1];
+  a.c2 = d.grid[2];
+}
 
 constexpr int kBMThreads = 512;
 constexpr int kBMThreadsOneLog = 1024;  // one workgroup per CU fits its LDS
@@ -506,6 +524,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 
   const int tid = threadIdx.x;
   const int64_t g = blockIdx.x;
   if (g >= a.nrefs) return;
+  bind_input(a, g);
   const int lin = a.ref_lin[g];
   const int o2 = lin % a.c2, o1 = (lin / a.c2) % a.c1, o0 = lin / (a.c2 * a.c1);
   const int lo0 = max(o0 - R0, 0), hi0 = min(o0 + R0, a.c0 - 1);
@@ -701,6 +720,7 @@ __global__ __launch_bounds__(NT) void k_blockmatch_cert(BMArgs a) {
   const int lane = tid & 63;
   const int64_t g = blockIdx.x;
   if (g >= a.nrefs) return;
+  bind_input(a, g);
   const int lin = a.ref_lin[g];
   const int o2 = lin % a.c2, o1 = (lin / a.c2) % a.c1, o0 = lin / (a.c2 * a.c1);
   const int lo0 = max(o0 - R0, 0), hi0 = min(o0 + R0, a.c0 - 1);
@@ -995,6 +1015,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
   const int lane = tid & 63;
   const int64_t g = blockIdx.x;
   if (g >= a.nrefs) return;
+  bind_input(a, g);
   const int lin = a.ref_lin[g];
   const int o2 = lin % a.c2, o1 = (lin / a.c2) % a.c1, o0 = lin / (a.c2 * a.c1);
   const int lo0 = max(o0 - R0, 0), hi0 = min(o0 + R0, a.c0 - 1);
@@ -1308,7 +1329,23 @@ struct FilterArgs {
   // pw_plans[n] is the offset of the summation plan of n*P elements
   float *sigma_out;
   const int32_t *pw_plans;
+  const bmnd_hip_input *batch;
+  const int32_t *ref_input;
 };
+
+__device__ __forceinline__ void bind_input(FilterArgs &a, int64_t g) {
+  if (!a.batch) return;
+  const bmnd_hip_input &d = a.batch[a.ref_input[g]];
+  a.noisy += d.vox_off;
+  if (a.pilot) a.pilot += d.vox_off;
+  a.var += d.vox_off;
+  a.n0 = d.shape[0];
+  a.n1 = d.shape[1];
+  a.n2 = d.shape[2];
+  a.c0 = d.grid[0];
+  a.c1 = d.grid[1];
+  a.c2 = d.grid[2];
+}
 
 // ---------------------------------------------------------------------------
 // exact overlap-aware noise std of the first planes (variance.py:652-975)
@@ -1334,7 +1371,21 @@ struct ExactArgs {
   int k;
   int kcap;
   float *exact_std;
+  const bmnd_hip_input *batch;
+  const int32_t *ref_input;
 };
+
+__device__ __forceinline__ void bind_input(ExactArgs &a, int64_t g) {
+  if (!a.batch) return;
+  const bmnd_hip_input &d = a.batch[a.ref_input[g]];
+  a.var += d.vox_off;
+  a.n0 = d.shape[0];
+  a.n1 = d.shape[1];
+  a.n2 = d.shape[2];
+  a.c0 = d.grid[0];
+  a.c1 = d.grid[1];
+  a.c2 = d.grid[2];
+}
 
 // Exact overlap-aware variance of the first min(k, N) group planes
 // (variance.py:652-975).  One wave per group; lane l owns the spatial
@@ -1366,6 +1417,7 @@ __global__ __launch_bounds__(64) void k_exact_std(ExactArgs a) {
   const int lane = threadIdx.x;
   const int64_t g = blockIdx.x;
   if (g >= a.nrefs) return;
+  bind_input(a, g);
   const int n = a.counts[g];
   const int planes = min(a.k, n);
   if (n <= 0 || planes <= 0) return;
@@ -1675,6 +1727,7 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
   const int tid = threadIdx.x;
   const int64_t g = blockIdx.x;
   if (g >= a.nrefs) return;
+  bind_input(a, g);
   const int n = a.counts[g];
   if (n <= 0) return;
   const int planes = min(a.k, n);
@@ -1953,6 +2006,14 @@ __global__ void k_reference_map(const int32_t *__restrict__ ref_lin, int64_t nre
   if (g < nrefs) ref_index[ref_lin[g]] = (int32_t)g;
 }
 
+__global__ void k_reference_map_batch(const int32_t *__restrict__ ref_lin,
+                                      const int32_t *__restrict__ ref_input, int64_t nrefs,
+                                      const bmnd_hip_input *__restrict__ batch,
+                                      int32_t *__restrict__ ref_index) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < nrefs) ref_index[batch[ref_input[g]].grid_off + ref_lin[g]] = (int32_t)g;
+}
+
 struct AggArgs {
   const float *filt, *weights;
   const int32_t *sel, *counts, *ref_index;
@@ -1969,6 +2030,10 @@ struct AggArgs {
   int sig_per_group;
   float sig_fill;
   float *sig_map;
+  // batch: tiles of all inputs in input order (ntiles in all)
+  const bmnd_hip_input *batch;
+  int ninputs;
+  int64_t ntiles;
 };
 
 constexpr int kAggThreads = 512;
@@ -1990,15 +2055,37 @@ __global__ __launch_bounds__(kAggThreads) void k_aggregate(AggArgs a) {
   __shared__ short s_wl[kAggThreads / 64][kSub];  // per wave: entries reaching its voxels
 
   const int tid = threadIdx.x;
-  const int nt2 = (a.n2 + T2 - 1) / T2, nt1 = (a.n1 + T1 - 1) / T1,
-            nt0 = (a.n0 + T0 - 1) / T0;
   // XCD-aware order: workgroups are dealt to the 8 XCDs in turn, so logical
   // tile (x % 8) * per + x / 8 puts consecutive tiles, which share most of
   // their groups' filtered patches, on one XCD and its L2
-  const int ntiles = nt0 * nt1 * nt2;
-  const int per = (ntiles + 7) / 8;
-  const int tb = (int)(blockIdx.x % 8u) * per + (int)(blockIdx.x / 8u);
-  if (tb >= ntiles) return;
+  const int64_t ntiles_all =
+      a.batch ? a.ntiles
+              : (int64_t)((a.n0 + T0 - 1) / T0) * ((a.n1 + T1 - 1) / T1) * ((a.n2 + T2 - 1) / T2);
+  const int64_t per = (ntiles_all + 7) / 8;
+  int64_t tball = (int64_t)(blockIdx.x % 8u) * per + (int64_t)(blockIdx.x / 8u);
+  if (tball >= ntiles_all) return;
+  if (a.batch) {
+    // the input holding this tile: the last one whose first tile is <= tball
+    int lo = 0, hi = a.ninputs - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (a.batch[mid].tile_off <= tball) lo = mid;
+      else hi = mid - 1;
+    }
+    const bmnd_hip_input &d = a.batch[lo];
+    tball -= d.tile_off;
+    a.ref_index += d.grid_off;
+    a.fallback += d.vox_off;
+    a.est += d.vox_off;
+    a.n0 = d.shape[0];
+    a.n1 = d.shape[1];
+    a.n2 = d.shape[2];
+    a.c0 = d.grid[0];
+    a.c1 = d.grid[1];
+    a.c2 = d.grid[2];
+  }
+  const int nt2 = (a.n2 + T2 - 1) / T2, nt1 = (a.n1 + T1 - 1) / T1;
+  const int tb = (int)tball;
   const int tx2 = tb % nt2, tx1 = (tb / nt2) % nt1, tx0 = tb / (nt2 * nt1);
   const int t_lo0 = tx0 * T0, t_lo1 = tx1 * T1, t_lo2 = tx2 * T2;
   const int t_hi0 = min(t_lo0 + T0, a.n0) - 1, t_hi1 = min(t_lo1 + T1, a.n1) - 1,
@@ -2420,6 +2507,42 @@ int group_plans_for_device(int P, int max_m, const int32_t **out) {
   return 0;
 }
 
+// a batch as the host sees it
+struct BatchRef {
+  const bmnd_hip_input *h, *d;
+  int n;
+  const int32_t *ref_input;
+};
+
+// every input valid for the stage geometry, packed in order; *grid_total and
+// *vox_max (may be null) receive the packed patch-grid size and the largest
+// input's voxel count
+int check_batch(const BatchRef &b, const int32_t *block, const int32_t *radius,
+                int64_t *grid_total, int64_t *vox_max) {
+  if (!b.h || !b.d || b.n <= 0) return fail(BMND_HIP_EINVAL, "a batch needs at least one input");
+  int64_t vox = 0, grid = 0, vmax = 0;
+  for (int i = 0; i < b.n; ++i) {
+    const bmnd_hip_input &in = b.h[i];
+    const int64_t shape[3] = {in.shape[0], in.shape[1], in.shape[2]};
+    int c[3];
+    if (int rc = check_geometry(shape, block, radius, c)) return rc;
+    if (c[0] != in.grid[0] || c[1] != in.grid[1] || c[2] != in.grid[2])
+      return fail(BMND_HIP_EINVAL, "batch input " + std::to_string(i) +
+                                       ": grid must be shape - block + 1");
+    if (in.vox_off != vox || in.grid_off != grid)
+      return fail(BMND_HIP_EINVAL, "batch input " + std::to_string(i) +
+                                       ": inputs must be packed in order");
+    const int64_t nv = shape[0] * shape[1] * shape[2];
+    vox += nv;
+    grid += (int64_t)c[0] * c[1] * c[2];
+    vmax = std::max(vmax, nv);
+  }
+  if (grid >= (int64_t)1 << 31) return fail(BMND_HIP_EINVAL, "batch exceeds 2^31 patch origins");
+  if (grid_total) *grid_total = grid;
+  if (vox_max) *vox_max = vmax;
+  return 0;
+}
+
 }  // namespace
 
 int bmnd_internal_fail(int code, const char *msg) { return fail(code, msg); }
@@ -2465,14 +2588,25 @@ int32_t bmnd_hip_count_range(const float *vol, int64_t n, int32_t *flags, void *
   return 0;
 }
 
-int32_t bmnd_hip_blockmatch(const float *vol, const int64_t *shape_h, const int32_t *block_h,
-                            const int32_t *radius_h, const int32_t *ref_lin, int64_t nrefs,
-                            float threshold, int32_t max_matches, int32_t min_matches,
-                            int32_t pow2, int32_t stabilize, double count_scale,
-                            int32_t mode, int32_t table_max, int32_t *sel, int32_t *counts,
-                            int32_t *accepted, void *stream) {
+static int32_t blockmatch_impl(const float *vol, const int64_t *shape_h, const int32_t *block_h,
+                               const int32_t *radius_h, const int32_t *ref_lin, int64_t nrefs,
+                               float threshold, int32_t max_matches, int32_t min_matches,
+                               int32_t pow2, int32_t stabilize, double count_scale,
+                               int32_t mode, int32_t table_max, int32_t *sel, int32_t *counts,
+                               int32_t *accepted, void *stream, const BatchRef *b) {
   int c[3];
-  if (int rc = check_geometry(shape_h, block_h, radius_h, c)) return rc;
+  int64_t shape1[3];
+  int64_t grid_total = 0;
+  if (b) {
+    if (int rc = check_batch(*b, block_h, radius_h, &grid_total, nullptr)) return rc;
+    for (int d = 0; d < 3; ++d) {
+      shape1[d] = b->h[0].shape[d];
+      c[d] = b->h[0].grid[d];
+    }
+    shape_h = shape1;  // the kernels take each reference's input from the batch
+  } else if (int rc = check_geometry(shape_h, block_h, radius_h, c)) {
+    return rc;
+  }
   if (max_matches < 1 || max_matches > kMaxStack || min_matches < 1)
     return fail(BMND_HIP_EINVAL, "stack sizes must satisfy 1 <= min, 1 <= max <= 64");
   if (!(count_scale > 0.0) || !std::isfinite(count_scale))
@@ -2532,6 +2666,8 @@ int32_t bmnd_hip_blockmatch(const float *vol, const int64_t *shape_h, const int3
   a.sel = sel;
   a.counts = counts;
   a.accepted = accepted;
+  a.batch = b ? b->d : nullptr;
+  a.ref_input = b ? b->ref_input : nullptr;
   hipStream_t s = (hipStream_t)stream;
   dim3 grid((unsigned)nrefs), blk(kBMThreads);
   const size_t dyn = mode == 2 ? sizeof(double) * (size_t)(table_max + 1) * (table_max + 1) : 0;
@@ -2545,16 +2681,32 @@ int32_t bmnd_hip_blockmatch(const float *vol, const int64_t *shape_h, const int3
   auto patch_sums = [&](auto b0c, auto b1c, auto b2c) -> int32_t {
     constexpr int B0 = decltype(b0c)::value, B1 = decltype(b1c)::value, B2 = decltype(b2c)::value;
     if (int rc = keep_pool_memory()) return rc;
-    const int64_t rows = (int64_t)a.n0 * a.n1;
-    HIP_TRY(hipMallocAsync((void **)&rowsum, sizeof(double) * (size_t)(rows * c[2]), s));
-    HIP_TRY(hipMallocAsync((void **)&zsum, sizeof(double) * (size_t)((int64_t)c[0] * c[1] * c[2]), s));
+    // one input, or each input of a batch in turn (its grid at grid_off)
+    const int nin = b ? b->n : 1;
+    int64_t rows_max = 0, grid_all = 0;
+    for (int i = 0; i < nin; ++i) {
+      const int64_t sh0 = b ? b->h[i].shape[0] : shape_h[0], sh1 = b ? b->h[i].shape[1] : shape_h[1];
+      const int cc2 = b ? b->h[i].grid[2] : c[2];
+      rows_max = std::max(rows_max, sh0 * sh1 * cc2);
+    }
+    grid_all = b ? grid_total : (int64_t)c[0] * c[1] * c[2];
+    HIP_TRY(hipMallocAsync((void **)&rowsum, sizeof(double) * (size_t)rows_max, s));
+    HIP_TRY(hipMallocAsync((void **)&zsum, sizeof(double) * (size_t)grid_all, s));
     const unsigned blocks = 4096;
-    hipLaunchKernelGGL((k_clog_rows<B2>), dim3(blocks), dim3(256), 0, s, vol, rows, a.n2, c[2], tab,
-                       clog, a.cert_c3, a.cert_c4, rowsum);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL((k_clog_patch<B0, B1>), dim3(blocks), dim3(256), 0, s, rowsum, a.n1, c[0],
-                       c[1], c[2], zsum);
-    HIP_TRY(hipGetLastError());
+    for (int i = 0; i < nin; ++i) {
+      const bmnd_hip_input *in = b ? &b->h[i] : nullptr;
+      const int64_t rows = in ? (int64_t)in->shape[0] * in->shape[1] : (int64_t)a.n0 * a.n1;
+      const int n1 = in ? in->shape[1] : a.n1, n2 = in ? in->shape[2] : a.n2;
+      const int g0 = in ? in->grid[0] : c[0], g1 = in ? in->grid[1] : c[1],
+                g2 = in ? in->grid[2] : c[2];
+      hipLaunchKernelGGL((k_clog_rows<B2>), dim3(blocks), dim3(256), 0, s,
+                         vol + (in ? in->vox_off : 0), rows, n2, g2, tab, clog, a.cert_c3,
+                         a.cert_c4, rowsum);
+      HIP_TRY(hipGetLastError());
+      hipLaunchKernelGGL((k_clog_patch<B0, B1>), dim3(blocks), dim3(256), 0, s, rowsum, n1, g0, g1,
+                         g2, zsum + (in ? in->grid_off : 0));
+      HIP_TRY(hipGetLastError());
+    }
     a.zsum = zsum;
     return 0;
   };
@@ -2597,6 +2749,33 @@ int32_t bmnd_hip_blockmatch(const float *vol, const int64_t *shape_h, const int3
   return fail(BMND_HIP_EINVAL, "block/radius configuration not compiled into this library");
 }
 
+int32_t bmnd_hip_blockmatch(const float *vol, const int64_t *shape_h, const int32_t *block_h,
+                            const int32_t *radius_h, const int32_t *ref_lin, int64_t nrefs,
+                            float threshold, int32_t max_matches, int32_t min_matches,
+                            int32_t pow2, int32_t stabilize, double count_scale,
+                            int32_t mode, int32_t table_max, int32_t *sel, int32_t *counts,
+                            int32_t *accepted, void *stream) {
+  return blockmatch_impl(vol, shape_h, block_h, radius_h, ref_lin, nrefs, threshold,
+                         max_matches, min_matches, pow2, stabilize, count_scale, mode, table_max,
+                         sel, counts, accepted, stream, nullptr);
+}
+
+int32_t bmnd_hip_blockmatch_batch(const float *vol, const bmnd_hip_input *inputs_h,
+                                  const bmnd_hip_input *inputs_d, int32_t ninputs,
+                                  const int32_t *block_h, const int32_t *radius_h,
+                                  const int32_t *ref_lin, const int32_t *ref_input,
+                                  int64_t nrefs, float threshold, int32_t max_matches,
+                                  int32_t min_matches, int32_t pow2, int32_t stabilize,
+                                  double count_scale, int32_t mode, int32_t table_max,
+                                  int32_t *sel, int32_t *counts, int32_t *accepted,
+                                  void *stream) {
+  if (!ref_input) return fail(BMND_HIP_EINVAL, "a batch needs ref_input");
+  const BatchRef b{inputs_h, inputs_d, ninputs, ref_input};
+  return blockmatch_impl(vol, nullptr, block_h, radius_h, ref_lin, nrefs, threshold, max_matches,
+                         min_matches, pow2, stabilize, count_scale, mode, table_max, sel, counts,
+                         accepted, stream, &b);
+}
+
 int64_t bmnd_hip_matrix_table_size(int32_t max_matches, const int32_t *block_h) {
   const int64_t m = max_matches;
   int64_t n = 2 * (m * (m + 1) * (2 * m + 1) / 6);
@@ -2604,12 +2783,22 @@ int64_t bmnd_hip_matrix_table_size(int32_t max_matches, const int32_t *block_h) 
   return n;
 }
 
-int32_t bmnd_hip_exact_std(const float *var, const int64_t *shape_h, const int32_t *block_h,
-                           const int32_t *radius_h, const int32_t *sel, const int32_t *counts,
-                           int64_t nrefs, int32_t max_matches, const float *matrices, int32_t k,
-                           float *exact_std, void *stream) {
+static int32_t exact_std_impl(const float *var, const int64_t *shape_h, const int32_t *block_h,
+                              const int32_t *radius_h, const int32_t *sel, const int32_t *counts,
+                              int64_t nrefs, int32_t max_matches, const float *matrices,
+                              int32_t k, float *exact_std, void *stream, const BatchRef *b) {
   int c[3];
-  if (int rc = check_geometry(shape_h, block_h, radius_h, c)) return rc;
+  int64_t shape1[3];
+  if (b) {
+    if (int rc = check_batch(*b, block_h, radius_h, nullptr, nullptr)) return rc;
+    for (int d = 0; d < 3; ++d) {
+      shape1[d] = b->h[0].shape[d];
+      c[d] = b->h[0].grid[d];
+    }
+    shape_h = shape1;
+  } else if (int rc = check_geometry(shape_h, block_h, radius_h, c)) {
+    return rc;
+  }
   if (k < 0) return fail(BMND_HIP_EINVAL, "k must be non-negative");
   if (nrefs <= 0 || k == 0) return 0;
   ExactArgs a;
@@ -2628,6 +2817,8 @@ int32_t bmnd_hip_exact_std(const float *var, const int64_t *shape_h, const int32
   a.k = k;
   a.kcap = std::min(k, max_matches);
   a.exact_std = exact_std;
+  a.batch = b ? b->d : nullptr;
+  a.ref_input = b ? b->ref_input : nullptr;
   hipStream_t s = (hipStream_t)stream;
   dim3 grid((unsigned)nrefs);
 #define E_CASE(b0, b1, b2, r0, r1, r2, nmax)                                                 \
@@ -2643,6 +2834,27 @@ int32_t bmnd_hip_exact_std(const float *var, const int64_t *shape_h, const int32
   return fail(BMND_HIP_EINVAL, "block/radius/stack configuration not compiled into this library");
 }
 
+int32_t bmnd_hip_exact_std(const float *var, const int64_t *shape_h, const int32_t *block_h,
+                           const int32_t *radius_h, const int32_t *sel, const int32_t *counts,
+                           int64_t nrefs, int32_t max_matches, const float *matrices, int32_t k,
+                           float *exact_std, void *stream) {
+  return exact_std_impl(var, shape_h, block_h, radius_h, sel, counts, nrefs, max_matches,
+                        matrices, k, exact_std, stream, nullptr);
+}
+
+int32_t bmnd_hip_exact_std_batch(const float *var, const bmnd_hip_input *inputs_h,
+                                 const bmnd_hip_input *inputs_d, int32_t ninputs,
+                                 const int32_t *block_h, const int32_t *radius_h,
+                                 const int32_t *sel, const int32_t *counts,
+                                 const int32_t *ref_input, int64_t nrefs,
+                                 int32_t max_matches, const float *matrices, int32_t k,
+                                 float *exact_std, void *stream) {
+  if (!ref_input) return fail(BMND_HIP_EINVAL, "a batch needs ref_input");
+  const BatchRef b{inputs_h, inputs_d, ninputs, ref_input};
+  return exact_std_impl(var, nullptr, block_h, radius_h, sel, counts, nrefs, max_matches,
+                        matrices, k, exact_std, stream, &b);
+}
+
 int32_t bmnd_hip_filter_groups(int32_t wiener, const float *noisy, const float *pilot,
                                const float *var, const int64_t *shape_h, const int32_t *block_h,
                                const int32_t *radius_h, const int32_t *sel,
@@ -2655,16 +2867,25 @@ int32_t bmnd_hip_filter_groups(int32_t wiener, const float *noisy, const float *
                                       count_scale, filt, weights, nullptr, stream);
 }
 
-int32_t bmnd_hip_filter_groups_sigma(int32_t wiener, const float *noisy, const float *pilot,
-                                     const float *var, const int64_t *shape_h,
-                                     const int32_t *block_h, const int32_t *radius_h,
-                                     const int32_t *sel, const int32_t *counts, int64_t nrefs,
-                                     int32_t max_matches, const float *matrices,
-                                     const float *exact_std, float shrink, int32_t k,
-                                     double count_scale, float *filt, float *weights,
-                                     float *sigma_out, void *stream) {
+static int32_t filter_impl(int32_t wiener, const float *noisy, const float *pilot,
+                           const float *var, const int64_t *shape_h, const int32_t *block_h,
+                           const int32_t *radius_h, const int32_t *sel, const int32_t *counts,
+                           int64_t nrefs, int32_t max_matches, const float *matrices,
+                           const float *exact_std, float shrink, int32_t k, double count_scale,
+                           float *filt, float *weights, float *sigma_out, void *stream,
+                           const BatchRef *b) {
   int c[3];
-  if (int rc = check_geometry(shape_h, block_h, radius_h, c)) return rc;
+  int64_t shape1[3];
+  if (b) {
+    if (int rc = check_batch(*b, block_h, radius_h, nullptr, nullptr)) return rc;
+    for (int d = 0; d < 3; ++d) {
+      shape1[d] = b->h[0].shape[d];
+      c[d] = b->h[0].grid[d];
+    }
+    shape_h = shape1;
+  } else if (int rc = check_geometry(shape_h, block_h, radius_h, c)) {
+    return rc;
+  }
   if (k < 0) return fail(BMND_HIP_EINVAL, "k must be non-negative");
   if (nrefs <= 0) return 0;
   if (wiener && !pilot) return fail(BMND_HIP_EINVAL, "Wiener filtering needs the pilot volume");
@@ -2693,6 +2914,8 @@ int32_t bmnd_hip_filter_groups_sigma(int32_t wiener, const float *noisy, const f
   a.weights = weights;
   a.sigma_out = sigma_out;
   a.pw_plans = nullptr;
+  a.batch = b ? b->d : nullptr;
+  a.ref_input = b ? b->ref_input : nullptr;
   if (sigma_out && wiener) {
     const int P = block_h[0] * block_h[1] * block_h[2];
     if (int rc = group_plans_for_device(P, max_matches, &a.pw_plans)) return rc;
@@ -2722,6 +2945,36 @@ int32_t bmnd_hip_filter_groups_sigma(int32_t wiener, const float *noisy, const f
   return fail(BMND_HIP_EINVAL, "block/radius/stack configuration not compiled into this library");
 }
 
+int32_t bmnd_hip_filter_groups_sigma(int32_t wiener, const float *noisy, const float *pilot,
+                                     const float *var, const int64_t *shape_h,
+                                     const int32_t *block_h, const int32_t *radius_h,
+                                     const int32_t *sel, const int32_t *counts, int64_t nrefs,
+                                     int32_t max_matches, const float *matrices,
+                                     const float *exact_std, float shrink, int32_t k,
+                                     double count_scale, float *filt, float *weights,
+                                     float *sigma_out, void *stream) {
+  return filter_impl(wiener, noisy, pilot, var, shape_h, block_h, radius_h, sel, counts, nrefs,
+                     max_matches, matrices, exact_std, shrink, k, count_scale, filt, weights,
+                     sigma_out, stream, nullptr);
+}
+
+int32_t bmnd_hip_filter_groups_batch(int32_t wiener, const float *noisy, const float *pilot,
+                                     const float *var, const bmnd_hip_input *inputs_h,
+                                     const bmnd_hip_input *inputs_d, int32_t ninputs,
+                                     const int32_t *block_h, const int32_t *radius_h,
+                                     const int32_t *sel, const int32_t *counts,
+                                     const int32_t *ref_input, int64_t nrefs,
+                                     int32_t max_matches, const float *matrices,
+                                     const float *exact_std, float shrink, int32_t k,
+                                     double count_scale, float *filt, float *weights,
+                                     void *stream) {
+  if (!ref_input) return fail(BMND_HIP_EINVAL, "a batch needs ref_input");
+  const BatchRef b{inputs_h, inputs_d, ninputs, ref_input};
+  return filter_impl(wiener, noisy, pilot, var, nullptr, block_h, radius_h, sel, counts, nrefs,
+                     max_matches, matrices, exact_std, shrink, k, count_scale, filt, weights,
+                     nullptr, stream, &b);
+}
+
 int32_t bmnd_hip_reference_map(const int32_t *ref_lin, int64_t nrefs, int32_t *ref_index,
                                int64_t num_patches, void *stream) {
   hipStream_t s = (hipStream_t)stream;
@@ -2731,6 +2984,34 @@ int32_t bmnd_hip_reference_map(const int32_t *ref_lin, int64_t nrefs, int32_t *r
                      ref_lin, nrefs, ref_index);
   HIP_TRY(hipGetLastError());
   return 0;
+}
+
+int32_t bmnd_hip_reference_map_batch(const int32_t *ref_lin, const int32_t *ref_input,
+                                     int64_t nrefs, const bmnd_hip_input *inputs_d,
+                                     int32_t *ref_index, int64_t num_patches, void *stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (num_patches > 0) HIP_TRY(hipMemsetAsync(ref_index, 0xff, sizeof(int32_t) * num_patches, s));
+  if (nrefs <= 0) return 0;
+  if (!ref_input || !inputs_d) return fail(BMND_HIP_EINVAL, "a batch needs ref_input and inputs");
+  hipLaunchKernelGGL(k_reference_map_batch, dim3((unsigned)((nrefs + 255) / 256)), dim3(256), 0, s,
+                     ref_lin, ref_input, nrefs, inputs_d, ref_index);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int32_t bmnd_hip_aggregate_tile(const int32_t *block_h, int32_t *tile) {
+  if (!block_h || !tile) return fail(BMND_HIP_EINVAL, "null pointer");
+  if (same3(block_h, 1, 8, 8)) {
+    tile[0] = 1;
+    tile[1] = 16;
+    tile[2] = 32;
+    return 0;
+  }
+  if (same3(block_h, 4, 4, 4) || same3(block_h, 5, 5, 5)) {
+    tile[0] = tile[1] = tile[2] = 8;
+    return 0;
+  }
+  return fail(BMND_HIP_EINVAL, "block configuration not compiled into this library");
 }
 
 int32_t bmnd_hip_aggregate(const float *filt, const float *weights, const int32_t *sel,
@@ -2743,15 +3024,36 @@ int32_t bmnd_hip_aggregate(const float *filt, const float *weights, const int32_
                                   0.0f, nullptr, stream);
 }
 
-int32_t bmnd_hip_aggregate_sigma(const float *filt, const float *weights, const int32_t *sel,
-                                 const int32_t *counts, const int32_t *ref_index, int64_t nrefs,
-                                 int32_t max_matches, const int64_t *shape_h,
-                                 const int32_t *block_h, const int32_t *radius_h,
-                                 const float *kaiser, const float *fallback, float *est,
-                                 const float *group_sigma, int32_t per_group, float fill,
-                                 float *sigma_map, void *stream) {
+static int32_t aggregate_impl(const float *filt, const float *weights, const int32_t *sel,
+                              const int32_t *counts, const int32_t *ref_index, int64_t nrefs,
+                              int32_t max_matches, const int64_t *shape_h,
+                              const int32_t *block_h, const int32_t *radius_h,
+                              const float *kaiser, const float *fallback, float *est,
+                              const float *group_sigma, int32_t per_group, float fill,
+                              float *sigma_map, void *stream, const BatchRef *b) {
   int c[3];
-  if (int rc = check_geometry(shape_h, block_h, radius_h, c)) return rc;
+  int64_t shape1[3];
+  int64_t ntiles = 0;
+  if (b) {
+    if (int rc = check_batch(*b, block_h, radius_h, nullptr, nullptr)) return rc;
+    int32_t tile[3];
+    if (int rc = bmnd_hip_aggregate_tile(block_h, tile)) return rc;
+    for (int i = 0; i < b->n; ++i) {
+      if (b->h[i].tile_off != ntiles)
+        return fail(BMND_HIP_EINVAL, "batch input " + std::to_string(i) +
+                                         ": tile_off must count the earlier inputs' tiles");
+      int64_t t = 1;
+      for (int d = 0; d < 3; ++d) t *= (b->h[i].shape[d] + tile[d] - 1) / tile[d];
+      ntiles += t;
+    }
+    for (int d = 0; d < 3; ++d) {
+      shape1[d] = b->h[0].shape[d];
+      c[d] = b->h[0].grid[d];
+    }
+    shape_h = shape1;
+  } else if (int rc = check_geometry(shape_h, block_h, radius_h, c)) {
+    return rc;
+  }
   AggArgs a;
   a.filt = filt;
   a.weights = weights;
@@ -2775,6 +3077,9 @@ int32_t bmnd_hip_aggregate_sigma(const float *filt, const float *weights, const 
   a.sig_per_group = per_group ? 1 : 0;
   a.sig_fill = fill;
   a.sig_map = sigma_map;
+  a.batch = b ? b->d : nullptr;
+  a.ninputs = b ? b->n : 0;
+  a.ntiles = ntiles;
   const bool with_sigma = sigma_map != nullptr;
   hipStream_t s = (hipStream_t)stream;
   int *d_over = nullptr;
@@ -2784,8 +3089,9 @@ int32_t bmnd_hip_aggregate_sigma(const float *filt, const float *weights, const 
   bool launched = false;
 #define A_CASE(b0, b1, b2, r0, r1, r2, t0, t1, t2)                                           \
   if (!launched && same3(block_h, b0, b1, b2) && same3(radius_h, r0, r1, r2)) {             \
-    const int64_t tiles = ((shape_h[0] + t0 - 1) / t0) * ((shape_h[1] + t1 - 1) / t1) *     \
-                          ((shape_h[2] + t2 - 1) / t2);                                     \
+    const int64_t tiles = b ? ntiles                                                        \
+                            : ((shape_h[0] + t0 - 1) / t0) * ((shape_h[1] + t1 - 1) / t1) * \
+                                  ((shape_h[2] + t2 - 1) / t2);                             \
     const dim3 grid((unsigned)(8 * ((tiles + 7) / 8)));                                     \
     if (with_sigma)                                                                         \
       hipLaunchKernelGGL((k_aggregate<b0, b1, b2, r0, r1, r2, t0, t1, t2, true>), grid,      \
@@ -2810,6 +3116,31 @@ int32_t bmnd_hip_aggregate_sigma(const float *filt, const float *weights, const 
   HIP_TRY(hipStreamSynchronize(s));
   if (h_over) return fail(BMND_HIP_ECAPACITY, "aggregation tile reached too many groups");
   return 0;
+}
+
+int32_t bmnd_hip_aggregate_sigma(const float *filt, const float *weights, const int32_t *sel,
+                                 const int32_t *counts, const int32_t *ref_index, int64_t nrefs,
+                                 int32_t max_matches, const int64_t *shape_h,
+                                 const int32_t *block_h, const int32_t *radius_h,
+                                 const float *kaiser, const float *fallback, float *est,
+                                 const float *group_sigma, int32_t per_group, float fill,
+                                 float *sigma_map, void *stream) {
+  return aggregate_impl(filt, weights, sel, counts, ref_index, nrefs, max_matches, shape_h,
+                        block_h, radius_h, kaiser, fallback, est, group_sigma, per_group, fill,
+                        sigma_map, stream, nullptr);
+}
+
+int32_t bmnd_hip_aggregate_batch(const float *filt, const float *weights, const int32_t *sel,
+                                 const int32_t *counts, const int32_t *ref_index, int64_t nrefs,
+                                 int32_t max_matches, const bmnd_hip_input *inputs_h,
+                                 const bmnd_hip_input *inputs_d, int32_t ninputs,
+                                 const int32_t *block_h, const int32_t *radius_h,
+                                 const float *kaiser, const float *fallback, float *est,
+                                 void *stream) {
+  const BatchRef b{inputs_h, inputs_d, ninputs, nullptr};
+  return aggregate_impl(filt, weights, sel, counts, ref_index, nrefs, max_matches, nullptr,
+                        block_h, radius_h, kaiser, fallback, est, nullptr, 0, 0.0f, nullptr,
+                        stream, &b);
 }
 
 }  // extern "C"

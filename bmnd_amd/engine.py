@@ -204,13 +204,15 @@ class StageOutput:
     sigma_map: torch.Tensor | None = None  # with sigma maps requested
 
 
-def blockmatch_work(plan: StagePlan, shape: tuple[int, ...]) -> tuple[int, int]:
+def blockmatch_work(plan: StagePlan, shape: tuple[int, ...],
+                    schedule: ScheduleArrays | None = None) -> tuple[int, int]:
     """(candidate pairs, deviance terms) of a stage: sum over references of
-    the clipped search-box size, times the patch volume for terms."""
+    the clipped search-box size, times the patch volume for terms (the
+    plan's schedule unless `schedule` is given, as for a batch input)."""
     ndim = len(shape)
     counts = np.array([shape[d] - plan.block[d] + 1 for d in range(ndim)], dtype=np.int64)
     radius = np.array(plan.radius3[3 - ndim:], dtype=np.int64)
-    ref = plan.schedule.reference_abs
+    ref = (schedule if schedule is not None else plan.schedule).reference_abs
     lo = np.maximum(ref - radius, 0)
     hi = np.minimum(ref + radius, counts - 1)
     pairs = int(np.prod(hi - lo + 1, axis=1).sum())
@@ -436,4 +438,188 @@ class PoissonPipeline:
         wi = self.stage("wiener", ht.est, vol, var2, vol, timer, fill)
         if sigma_maps:
             return ht, wi, gsig
+        return ht, wi
+
+
+# bmnd_hip_input (include/bmnd_hip.h): one input of a batch
+INPUT_DTYPE = np.dtype([("vox_off", "<i8"), ("grid_off", "<i8"), ("tile_off", "<i8"),
+                        ("shape", "<i4", (3,)), ("grid", "<i4", (3,))])
+assert INPUT_DTYPE.itemsize == 48
+
+
+@dataclass
+class BatchStage:
+    """One stage's view of a batch: per-input schedules, the flat reference
+    list (patch-grid index within the input, and the input), and the input
+    records on the host and on the device."""
+    plan: StagePlan
+    schedules: list
+    ref_lin: torch.Tensor
+    ref_input: torch.Tensor
+    ref_off: np.ndarray        # first flat reference of every input, plus the total
+    inputs_h: np.ndarray       # INPUT_DTYPE records
+    inputs_d: torch.Tensor     # the same bytes on the device
+    grid_total: int
+
+    @property
+    def nrefs(self) -> int:
+        return int(self.ref_off[-1])
+
+    def args(self):
+        """(inputs_h, inputs_d, ninputs) as the C-ABI takes them."""
+        return (ctypes.c_void_p(self.inputs_h.ctypes.data), _ptr(self.inputs_d),
+                len(self.inputs_h))
+
+
+@dataclass
+class BatchStageOutput:
+    est: torch.Tensor       # packed estimates
+    sel: torch.Tensor       # (nrefs, max_m), patch indices within each input
+    counts: torch.Tensor
+    accepted: torch.Tensor
+    stage: BatchStage
+
+
+class BatchPipeline:
+    """The two-stage Poisson denoiser over a batch of inputs of one profile
+    and one dimensionality but any shapes: each kernel runs once over the
+    flat list of every input's references (or aggregation tiles), reading the
+    inputs from one packed device buffer.  Every input's result is the one
+    PoissonPipeline gives for it alone."""
+
+    def __init__(self, profile: BMNDProfile, shapes, device=None, deviance: str = "auto-fast"):
+        shapes = [tuple(int(s) for s in sh) for sh in shapes]
+        if not shapes:
+            raise ValueError("a batch needs at least one input")
+        ndims = {len(s) for s in shapes}
+        if len(ndims) != 1:
+            raise ValueError("every input of a batch must have the same number of dimensions")
+        self.ndim = ndims.pop()
+        self.shapes = shapes
+        self.shape3s = [_host3(s, self.ndim, 1) for s in shapes]
+        self.vox = np.array([int(np.prod(s)) for s in self.shape3s], dtype=np.int64)
+        self.vox_off = np.concatenate([[0], np.cumsum(self.vox)]).astype(np.int64)
+        self.device = torch.device(device if device is not None else "cuda")
+        # matching-mode choice and the per-call constants are PoissonPipeline's
+        self._single = PoissonPipeline(profile, shapes[0], self.device, deviance)
+        self.stages = {name: self._stage_view(profile, name) for name in ("ht", "wiener")}
+
+    @property
+    def total_voxels(self) -> int:
+        return int(self.vox_off[-1])
+
+    def _stage_view(self, profile: BMNDProfile, name: str) -> BatchStage:
+        plan = make_plan(profile, name, self.ndim)
+        tile = (ctypes.c_int32 * 3)()
+        _lib.check(_lib.lib().bmnd_hip_aggregate_tile((ctypes.c_int32 * 3)(*plan.block3), tile))
+        recs = np.zeros(len(self.shapes), dtype=INPUT_DTYPE)
+        schedules, lins, inputs = [], [], []
+        grid_off = tile_off = 0
+        ref_off = [0]
+        for i, (shape, shape3) in enumerate(zip(self.shapes, self.shape3s)):
+            counts = tuple(shape[d] - plan.block[d] + 1 for d in range(self.ndim))
+            sched = get_reference_schedule_arrays(
+                counts, plan.block, plan.step, mode=profile.reference_schedule_mode,
+                shift_density=profile.reference_shift_density,
+                schedule_density=profile.reference_schedule_density)
+            schedules.append(sched)
+            lin = np.ravel_multi_index(tuple(sched.reference_abs.T), counts).astype(np.int32)
+            lins.append(lin)
+            inputs.append(np.full(lin.size, i, dtype=np.int32))
+            grid3 = tuple(shape3[d] - plan.block3[d] + 1 for d in range(3))
+            recs[i] = (self.vox_off[i], grid_off, tile_off, shape3, grid3)
+            grid_off += int(np.prod(grid3))
+            tile_off += int(np.prod([-(-shape3[d] // tile[d]) for d in range(3)]))
+            ref_off.append(ref_off[-1] + lin.size)
+        dev = self.device
+        return BatchStage(
+            plan=plan, schedules=schedules,
+            ref_lin=torch.from_numpy(np.concatenate(lins)).to(dev),
+            ref_input=torch.from_numpy(np.concatenate(inputs)).to(dev),
+            ref_off=np.asarray(ref_off, dtype=np.int64), inputs_h=recs,
+            inputs_d=torch.from_numpy(recs.view(np.uint8).copy()).to(dev),
+            grid_total=grid_off)
+
+    def pack(self, volumes) -> torch.Tensor:
+        """One float32 device buffer holding every input in order."""
+        flat = np.concatenate([np.ascontiguousarray(v, dtype=np.float32).reshape(-1)
+                               for v in volumes])
+        return torch.from_numpy(flat).to(self.device)
+
+    def unpack(self, packed: torch.Tensor) -> list[np.ndarray]:
+        host = packed.cpu().numpy()
+        return [host[self.vox_off[i]:self.vox_off[i + 1]].reshape(self.shapes[i]).copy()
+                for i in range(len(self.shapes))]
+
+    def stage(self, name: str, match_vol: torch.Tensor, noisy: torch.Tensor, var: torch.Tensor,
+              fallback: torch.Tensor, timer=None) -> BatchStageOutput:
+        L = _lib.lib()
+        timer = timer or _NoTimer()
+        st = self.stages[name]
+        plan = st.plan
+        stream = self._single._stream()
+        block = (ctypes.c_int32 * 3)(*plan.block3)
+        radius = (ctypes.c_int32 * 3)(*plan.radius3)
+        R = st.nrefs
+        dev = self.device
+        ih, idv, nin = st.args()
+        sel = torch.empty(R * plan.max_m, dtype=torch.int32, device=dev)
+        cnt = torch.empty(R, dtype=torch.int32, device=dev)
+        acc = torch.empty(R, dtype=torch.int32, device=dev)
+        mode, tmax = self._single.blockmatch_mode(match_vol)
+        tok = timer.start(f"{name}.blockmatch")
+        _lib.check(L.bmnd_hip_blockmatch_batch(
+            _ptr(match_vol), ih, idv, nin, block, radius, _ptr(st.ref_lin), _ptr(st.ref_input), R,
+            plan.threshold, plan.max_m, plan.min_m, 1, 1, self._single.count_scale, mode, tmax,
+            _ptr(sel), _ptr(cnt), _ptr(acc), stream))
+        timer.stop(tok)
+        mats, kai = plan.device_constants(dev)
+        k = self._single.k
+        kcap = min(k, plan.max_m)
+        exact = None
+        if kcap > 0:
+            exact = torch.empty(R * kcap * plan.P, dtype=torch.float32, device=dev)
+            tok = timer.start(f"{name}.exact_std")
+            _lib.check(L.bmnd_hip_exact_std_batch(
+                _ptr(var), ih, idv, nin, block, radius, _ptr(sel), _ptr(cnt), _ptr(st.ref_input),
+                R, plan.max_m, _ptr(mats), k, _ptr(exact), stream))
+            timer.stop(tok)
+        filt = torch.empty(R * plan.max_m * plan.P, dtype=torch.float32, device=dev)
+        wts = torch.empty(R * plan.max_m, dtype=torch.float32, device=dev)
+        wiener = name == "wiener"
+        tok = timer.start(f"{name}.filter")
+        _lib.check(L.bmnd_hip_filter_groups_batch(
+            int(wiener), _ptr(noisy), _ptr(match_vol if wiener else None), _ptr(var), ih, idv, nin,
+            block, radius, _ptr(sel), _ptr(cnt), _ptr(st.ref_input), R, plan.max_m, _ptr(mats),
+            _ptr(exact), plan.shrink, k, self._single.count_scale, _ptr(filt), _ptr(wts), stream))
+        timer.stop(tok)
+        del exact
+        ref_index = torch.empty(st.grid_total, dtype=torch.int32, device=dev)
+        _lib.check(L.bmnd_hip_reference_map_batch(_ptr(st.ref_lin), _ptr(st.ref_input), R,
+                                                  _ptr(st.inputs_d), _ptr(ref_index),
+                                                  st.grid_total, stream))
+        est = torch.empty_like(noisy)
+        tok = timer.start(f"{name}.aggregate")
+        _lib.check(L.bmnd_hip_aggregate_batch(
+            _ptr(filt), _ptr(wts), _ptr(sel), _ptr(cnt), _ptr(ref_index), R, plan.max_m, ih, idv,
+            nin, block, radius, _ptr(kai), _ptr(fallback), _ptr(est), stream))
+        timer.stop(tok)
+        del filt, wts, ref_index
+        return BatchStageOutput(est, sel.view(R, plan.max_m), cnt, acc, st)
+
+    def run(self, packed: torch.Tensor, timer=None) -> tuple[BatchStageOutput, BatchStageOutput]:
+        """packed: float32 device buffer of `total_voxels` (see pack())."""
+        if (packed.dtype != torch.float32 or not packed.is_contiguous()
+                or packed.numel() != self.total_voxels):
+            raise ValueError("packed must be a contiguous float32 tensor of every input's voxels")
+        t = timer or _NoTimer()
+        tok = t.start("ht.variance")
+        var1 = self._single.variance(packed)
+        t.stop(tok)
+        ht = self.stage("ht", packed, packed, var1, packed, timer)
+        del var1
+        tok = t.start("wiener.variance")
+        var2 = self._single.variance(ht.est)
+        t.stop(tok)
+        wi = self.stage("wiener", ht.est, packed, var2, packed, timer)
         return ht, wi

@@ -72,3 +72,27 @@ def psnr(estimate: np.ndarray, truth: np.ndarray, peak: float | None = None) -> 
     peak = float(np.max(truth)) if peak is None else float(peak)
     mse = float(np.mean((np.asarray(estimate, np.float64) - np.asarray(truth, np.float64)) ** 2))
     return float("inf") if mse == 0 else 10.0 * np.log10(peak * peak / mse)
+
+
+def uneven_batch(seed: int = 0, n_images: int = 32, n_volumes: int = 16,
+                 image_sides=(96, 1024), volume_sides=(48, 128), peaks=(1.0, 50.0)):
+    """Inputs of uneven shapes: images with each side uniform in `image_sides`
+    (blob phantoms cropped from a square one) and volumes with each side
+    uniform in `volume_sides` (ellipsoid phantoms), every input at a peak
+    count uniform in `peaks`.  Returns (images, image_truths, volumes,
+    volume_truths)."""
+    rng = np.random.default_rng(seed)
+    imgs, itruth, vols, vtruth = [], [], [], []
+    for _ in range(n_images):
+        h, w = (int(x) for x in rng.integers(image_sides[0], image_sides[1] + 1, 2))
+        truth, noisy = blob_image(max(h, w), seed=int(rng.integers(1 << 30)),
+                                  peak=float(rng.uniform(*peaks)))
+        itruth.append(np.ascontiguousarray(truth[:h, :w]))
+        imgs.append(np.ascontiguousarray(noisy[:h, :w]))
+    for _ in range(n_volumes):
+        shape = tuple(int(x) for x in rng.integers(volume_sides[0], volume_sides[1] + 1, 3))
+        truth, noisy = ellipsoid_volume(shape, seed=int(rng.integers(1 << 30)),
+                                        peak=float(rng.uniform(*peaks)))
+        vtruth.append(truth)
+        vols.append(noisy)
+    return imgs, itruth, vols, vtruth

@@ -204,6 +204,81 @@ int32_t bmnd_hip_poisson_global_sigma(const float *var, int64_t n,
                                       double variance_floor, double *global_sigma,
                                       void *stream);
 
+/* ---- batches of inputs of different shapes --------------------------------
+ * No counterpart in the reference, which denoises one array per call: a batch
+ * runs every input of one profile (one block / search geometry) through each
+ * kernel in a single launch over the flat list of all inputs' references
+ * (or aggregation tiles).  The inputs' voxels, patch grids and tiles are
+ * packed one after the other in flat buffers, in input order; input i is
+ * described by */
+typedef struct {
+  int64_t vox_off;   /* first voxel in the packed volumes: sum of the earlier inputs' voxels */
+  int64_t grid_off;  /* first patch origin in packed patch-grid buffers: sum of earlier grids */
+  int64_t tile_off;  /* first aggregation tile: sum of earlier inputs' tile counts */
+  int32_t shape[3];  /* voxels per axis; images as (1, H, W) */
+  int32_t grid[3];   /* patch-grid counts shape - block + 1 of the stage */
+} bmnd_hip_input;
+/* Each batched entry point takes the descriptors twice: inputs_h (host, used
+ * for checks and launch sizes) and inputs_d (the same ninputs records in
+ * device memory, read by the kernels).  ref_lin holds every reference's
+ * patch-grid index within its own input, ref_input its input; sel lists,
+ * counts, weights and filtered rows are indexed by the flat reference index
+ * as in the single-input calls, and their patch indices are per input.  Each
+ * input's result equals that of the single-input call on it alone. */
+
+/* Aggregation tile of a block configuration (tile[3], voxels per axis), for
+ * bmnd_hip_input.tile_off: an input has prod(ceil(shape[d] / tile[d])) tiles. */
+int32_t bmnd_hip_aggregate_tile(const int32_t *block_h, int32_t *tile);
+
+/* bmnd_hip_blockmatch over a batch (vol: packed volumes). */
+int32_t bmnd_hip_blockmatch_batch(const float *vol, const bmnd_hip_input *inputs_h,
+                                  const bmnd_hip_input *inputs_d, int32_t ninputs,
+                                  const int32_t *block_h, const int32_t *radius_h,
+                                  const int32_t *ref_lin, const int32_t *ref_input,
+                                  int64_t nrefs, float threshold, int32_t max_matches,
+                                  int32_t min_matches, int32_t pow2, int32_t stabilize,
+                                  double count_scale, int32_t mode, int32_t table_max,
+                                  int32_t *sel, int32_t *counts, int32_t *accepted,
+                                  void *stream);
+
+/* bmnd_hip_exact_std over a batch (var: packed variance volumes). */
+int32_t bmnd_hip_exact_std_batch(const float *var, const bmnd_hip_input *inputs_h,
+                                 const bmnd_hip_input *inputs_d, int32_t ninputs,
+                                 const int32_t *block_h, const int32_t *radius_h,
+                                 const int32_t *sel, const int32_t *counts,
+                                 const int32_t *ref_input, int64_t nrefs,
+                                 int32_t max_matches, const float *matrices, int32_t k,
+                                 float *exact_std, void *stream);
+
+/* bmnd_hip_filter_groups over a batch (noisy, pilot, var: packed volumes). */
+int32_t bmnd_hip_filter_groups_batch(int32_t wiener, const float *noisy, const float *pilot,
+                                     const float *var, const bmnd_hip_input *inputs_h,
+                                     const bmnd_hip_input *inputs_d, int32_t ninputs,
+                                     const int32_t *block_h, const int32_t *radius_h,
+                                     const int32_t *sel, const int32_t *counts,
+                                     const int32_t *ref_input, int64_t nrefs,
+                                     int32_t max_matches, const float *matrices,
+                                     const float *exact_std, float shrink, int32_t k,
+                                     double count_scale, float *filt, float *weights,
+                                     void *stream);
+
+/* bmnd_hip_reference_map over a batch: ref_index (packed patch grids,
+ * num_patches in all) gets the flat reference index at every scheduled
+ * position of every input, -1 elsewhere. */
+int32_t bmnd_hip_reference_map_batch(const int32_t *ref_lin, const int32_t *ref_input,
+                                     int64_t nrefs, const bmnd_hip_input *inputs_d,
+                                     int32_t *ref_index, int64_t num_patches, void *stream);
+
+/* bmnd_hip_aggregate over a batch (fallback, est: packed volumes; ref_index
+ * from bmnd_hip_reference_map_batch). */
+int32_t bmnd_hip_aggregate_batch(const float *filt, const float *weights, const int32_t *sel,
+                                 const int32_t *counts, const int32_t *ref_index, int64_t nrefs,
+                                 int32_t max_matches, const bmnd_hip_input *inputs_h,
+                                 const bmnd_hip_input *inputs_d, int32_t ninputs,
+                                 const int32_t *block_h, const int32_t *radius_h,
+                                 const float *kaiser, const float *fallback, float *est,
+                                 void *stream);
+
 /* Host-only check of the summation plans: numpy's float32 order for x[0..n)
  * (host memory) evaluated on the CPU with the plans and leaf sums the
  * kernels use.  For tests without a GPU. */
