@@ -277,7 +277,8 @@ def run_batch(args) -> int:
     import torch
 
     from bmnd_amd import BM3DProfile, BM4DProfile, NoiseModel, phantoms
-    from bmnd_amd.engine import BatchPipeline, KernelTimer, blockmatch_work, exact_std_work
+    from bmnd_amd.engine import (BatchPipeline, KernelTimer, blockmatch_work, exact_std_work,
+                                 table_first_order)
 
     dev = torch.device("cuda", 0)
     if args.config == "batch-uneven":
@@ -295,6 +296,9 @@ def run_batch(args) -> int:
         cfg = dict(cfg, batched=True)
     runs = []
     for tag, prof, inputs, truths in groups:
+        order = table_first_order(inputs)  # as bmnd_batch orders a batch
+        inputs = [inputs[i] for i in order]
+        truths = [truths[i] for i in order] if truths else truths
         bp = BatchPipeline(prof, [x.shape for x in inputs], dev, deviance=args.deviance)
         runs.append((tag, bp, bp.pack(inputs), inputs, truths))
     voxels = sum(bp.total_voxels for _, bp, _, _, _ in runs)

@@ -16,7 +16,7 @@ import torch
 
 from .blockmatching import BlockMatchResult, build_result
 from .enums import CacheMode, coerce_enum
-from .engine import BatchPipeline, PoissonPipeline, check_supported
+from .engine import BatchPipeline, PoissonPipeline, check_supported, table_first_order
 from .profilevalidation import resolve_profile
 
 
@@ -158,6 +158,7 @@ def bmnd_batch(volumes, profile, sigma: float | None = None,
             check_supported(profile, resolved)
             live.append(i)
     if live:
+        live = [live[j] for j in table_first_order([items[i][0] for i in live])]
         device = torch.device("cuda", torch.cuda.current_device())
         pipe = BatchPipeline(profile, [items[i][0].shape for i in live], device)
         _, wi = pipe.run(pipe.pack([items[i][0] for i in live]))

@@ -227,6 +227,13 @@ __global__ void k_variance_volume(const float *__restrict__ v, float *__restrict
 // ---------------------------------------------------------------------------
 // count check: are all voxels non-negative integers, and what is the largest
 // ---------------------------------------------------------------------------
+__global__ void k_count_range_init(int32_t *flags) {
+  if (threadIdx.x == 0) {
+    flags[0] = 1;
+    flags[1] = 0;
+  }
+}
+
 __global__ void k_count_range(const float *__restrict__ v, int64_t n, int32_t *flags) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -266,7 +273,7 @@ struct BMArgs {
   const double2 *certlog;    // MODE 3: {2/c_j, log c_j - ln2}, j < kFastLogN
   double cert_c3, cert_c4;   // MODES 3, 4: 1/3 and -1/4, the degree-4 log's coefficients
   double log2_tol;           // MODE 4: bound on the hardware log2 error ratio
-  const double *zsum;        // MODE 4: patch sums of c log 2c per patch origin (k_clog_patch)
+  const double2 *zsum;       // MODE 4: per patch origin {sum c log 2c, sum c} (k_clog_patch)
   const double *term_table;  // MODE 2: (tmax+1)^2 terms, row = reference count
   int tmax;
   int32_t *sel, *counts, *accepted;
@@ -927,8 +934,9 @@ __global__ void k_log2_check(const double *__restrict__ tab, unsigned long long 
   if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(worst));
 }
 
-// Patch sums of c log 2c (c = max(v, 0), float64 with the degree-4 log) for
-// every patch origin: the candidate-only part of the fast matcher's distance.
+// Patch sums of c log 2c (c = max(v, 0), float64 with the degree-4 log) and
+// of c for every patch origin: the candidate-only part of the fast matcher's
+// distance and the plain sum its per-candidate error bound uses.
 // k_clog_rows sums B2 consecutive voxels along axis 2 (rows of n0*n1 by c2),
 // k_clog_patch sums B0 x B1 of those; the certified bound holds for any
 // summation order.
@@ -936,7 +944,7 @@ template <int B2>
 __global__ void k_clog_rows(const float *__restrict__ vol, int64_t rows, int n2, int c2,
                             const double *__restrict__ logtab,
                             const double2 *__restrict__ certlog, double c3, double c4,
-                            double *__restrict__ out) {
+                            double2 *__restrict__ out) {
   __shared__ double2 s_clog[kFastLogN];
   for (int i = threadIdx.x; i < kFastLogN; i += blockDim.x) s_clog[i] = certlog[i];
   __syncthreads();
@@ -946,32 +954,37 @@ __global__ void k_clog_rows(const float *__restrict__ vol, int64_t rows, int n2,
        i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t row = i / c2;
     const float *src = vol + row * n2 + (i - row * c2);
-    double sum = 0.0;
+    double sum = 0.0, csum = 0.0;
 #pragma unroll
     for (int t = 0; t < B2; ++t) {
       const float v = src[t];
       const double c = v > 0.0f ? (double)v : 0.0;
       sum = sum + (c > 0.0 ? c * log_deg4(2.0 * c, s_clog, ln2, c3, c4) : 0.0);
+      csum = csum + c;
     }
-    out[i] = sum;
+    out[i] = make_double2(sum, csum);
   }
 }
 
 template <int B0, int B1>
-__global__ void k_clog_patch(const double *__restrict__ rows, int n1, int c0, int c1, int c2,
-                             double *__restrict__ z) {
+__global__ void k_clog_patch(const double2 *__restrict__ rows, int n1, int c0, int c1, int c2,
+                             double2 *__restrict__ z) {
   const int64_t total = (int64_t)c0 * c1 * c2;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int j2 = (int)(i % c2);
     const int64_t r = i / c2;
     const int j1 = (int)(r % c1), j0 = (int)(r / c1);
-    double s = 0.0;
+    double s = 0.0, cs = 0.0;
 #pragma unroll
     for (int t0 = 0; t0 < B0; ++t0)
 #pragma unroll
-      for (int t1 = 0; t1 < B1; ++t1) s = s + rows[((int64_t)(j0 + t0) * n1 + (j1 + t1)) * c2 + j2];
-    z[i] = s;
+      for (int t1 = 0; t1 < B1; ++t1) {
+        const double2 r = rows[((int64_t)(j0 + t0) * n1 + (j1 + t1)) * c2 + j2];
+        s = s + r.x;
+        cs = cs + r.y;
+      }
+    z[i] = make_double2(s, cs);
   }
 }
 
@@ -1006,7 +1019,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
   __shared__ double s_tab[kLogTabDoubles];
   __shared__ double2 s_clog[kFastLogN];    // degree-4 log
   __shared__ float s_wmax[2][NT / 64];     // per-wave maxima of slab and reference values
-  __shared__ double s_errc, s_errf;
+  __shared__ double s_errc, s_errf, s_wb;
   __shared__ int s_acc, s_maybe, s_nex;
   __shared__ int s_scan[NT / 64 + 1];
   BM_SEL_SCRATCH(sc);
@@ -1099,15 +1112,18 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
     const double ec = 2.0 * (double)P * (S * 8e-13 + 4096.0 * 0x1p-53 * Wn);
     s_errc = ec;
     s_errf = ec + 2.0 * ln2 * (double)P * (2.0 * (2.0 * a.log2_tol + 10.0 * 0x1p-24) * Wb + 0x1p-118);
+    s_wb = (double)P * Wb;
   }
   // patch sums of c log 2c of a candidate and of the reference (sum of r log 2r)
-  auto zsum_of = [&](int cand) {
+  auto zsum2_of = [&](int cand) {
     const int j2 = cand % e2, j1 = (cand / e2) % e1, j0 = cand / (e2 * e1);
     return a.zsum[((int64_t)(lo0 + j0) * a.c1 + (lo1 + j1)) * a.c2 + (lo2 + j2)];
   };
-  const double rsum = a.zsum[lin];
+  auto zsum_of = [&](int cand) { return zsum2_of(cand).x; };
+  const double2 rz = a.zsum[lin];
+  const double rsum = rz.x;
   __syncthreads();
-  const double errc = s_errc, errf = s_errf;
+  const double errc = s_errc, pwb = s_wb;
   const float thr = a.stabilize ? a.thr_key : a.threshold;
   const int rel0 = o0 - lo0, rel1 = o1 - lo1, rel2 = o2 - lo2;
   const int center_local = (rel0 * e1 + rel1) * e2 + rel2;
@@ -1167,12 +1183,25 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
   // in registers, lower-bound keys in s_klo
   float khi[KC];
   int my_maybe = 0;
+  // Per-candidate float-pass bound: its error is at most eta * G with
+  // G = sum_p s (|log2 s| + 2) and eta = 2 (2 log2_tol + 10 * 2^-24) (the
+  // per-term analysis above).  With F = sum_p s log2 s, s |log2 s| <=
+  // s log2 s + 2 * 0.53074 for s < 1 and 2 sum s = 2 (sum r + sum c), so
+  // G <= (F_float + 1.0615 P + 2 sum s) / (1 - eta / 2) (eta carries a 2x
+  // margin, and eta / 2 < 2^-18 with log2_tol <= 2^-20), so the factor
+  // 1 + 2^-16 covers the division and the float64 sums.  Both this and
+  // P * W2 bound G; the smaller one is used.
+  const double eta = 2.0 * (2.0 * a.log2_tol + 10.0 * 0x1p-24);
 #pragma unroll
   for (int k = 0; k < KC; ++k) {
     const int cand = tid + k * NT;
     khi[k] = 0.0f;
     if (cand < box) {
-      const double df = 2.0 * (rsum + (zsum_of(cand) - ln2 * F[k]));
+      const double2 cz = zsum2_of(cand);
+      const double df = 2.0 * (rsum + (cz.x - ln2 * F[k]));
+      const double gb = (F[k] + 1.0615 * (double)P + 2.0 * (rz.y + cz.y)) * (1.0 + 0x1p-16);
+      const double errf =
+          errc + 2.0 * ln2 * (eta * fmin(fmax(gb, 0.0), pwb) + (double)P * 0x1p-118);
       bool ok_lo, ok_hi;
       s_klo[cand] = bm_key<P>(df - errf, a, ok_lo);
       khi[k] = bm_key<P>(df + errf, a, ok_hi);
@@ -1233,27 +1262,31 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
     }
     __syncthreads();
     const int m = min(LCAP, total - lbase);
-    // one wave per listed candidate: lanes take the patch terms of s log s
-    // (the certified bound holds for any summation order) and a butterfly sum
-    // leaves the same total in every lane, so the key test below is uniform
-    for (int e = tid >> 6; e < m; e += NT / 64) {
+    // G lanes per listed candidate (G a power of two, the largest with
+    // m * G <= NT, at most a wave): the group's lanes take the patch terms of
+    // s log s (the certified bound holds for any summation order) and a
+    // butterfly sum within the group leaves the same total in each of its
+    // lanes, so the key test below is uniform in the group
+    int G = 64;
+    while (G > 1 && m * G > NT) G >>= 1;
+    const int gl = lane & (G - 1);
+    for (int e = tid / G; e < m; e += NT / G) {
       const int cand = lc[e];
       const int j2 = cand % e2, j1 = (cand / e2) % e1, j0 = cand / (e2 * e1);
       const int base = (j0 * S1 + j1) * S2 + j2;
       double t = 0.0;
-      for (int p = lane; p < P; p += 64) {
+      for (int p = gl; p < P; p += G) {
         const int p01 = p / B2, p2 = p - p01 * B2;
         const int p0 = p01 / B1, p1 = p01 - p0 * B1;
         const double s = (double)s_reff[p] + (double)s_slab[base + (p0 * S1 + p1) * S2 + p2];
         t = fma(-s, log_deg4(s, s_clog, ln2, c3, c4), t);
       }
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) t = t + __shfl_xor(t, o, 64);
+      for (int o = G >> 1; o > 0; o >>= 1) t = t + __shfl_xor(t, o, 64);
       const double dc = 2.0 * (rsum + (lz[e] + t));
       bool ok_lo, ok_hi, ok;
       const float k_lo = bm_key<P>(dc - errc, a, ok_lo);
       const float k_hi = bm_key<P>(dc + errc, a, ok_hi);
-      if (lane == 0) {
+      if (gl == 0) {
         float key = k_lo;
         if (k_lo != k_hi) {
           // the reference expression, in the reference's serial order
@@ -2579,8 +2612,10 @@ int32_t bmnd_hip_log2_check(double *max_ratio) {
 
 int32_t bmnd_hip_count_range(const float *vol, int64_t n, int32_t *flags, void *stream) {
   hipStream_t s = (hipStream_t)stream;
-  const int32_t init[2] = {1, 0};
-  HIP_TRY(hipMemcpyAsync(flags, init, sizeof(init), hipMemcpyHostToDevice, s));
+  // flags start as {all integers, max 0}; a kernel sets them so that no host
+  // buffer has to outlive the call
+  hipLaunchKernelGGL(k_count_range_init, dim3(1), dim3(64), 0, s, flags);
+  HIP_TRY(hipGetLastError());
   if (n <= 0) return 0;
   int blocks = (int)std::min<int64_t>((n + 255) / 256, 4096);
   hipLaunchKernelGGL(k_count_range, dim3(blocks), dim3(256), 0, s, vol, n, flags);
@@ -2673,7 +2708,7 @@ static int32_t blockmatch_impl(const float *vol, const int64_t *shape_h, const i
   const size_t dyn = mode == 2 ? sizeof(double) * (size_t)(table_max + 1) * (table_max + 1) : 0;
   // MODE 4: patch sums of c log 2c for every patch origin, in stream-ordered
   // scratch (rows of B2 voxels, then B0 x B1 of those)
-  double *rowsum = nullptr, *zsum = nullptr;
+  double2 *rowsum = nullptr, *zsum = nullptr;
   auto release = [&]() {
     if (rowsum) (void)hipFreeAsync(rowsum, s);
     if (zsum) (void)hipFreeAsync(zsum, s);
@@ -2690,8 +2725,8 @@ static int32_t blockmatch_impl(const float *vol, const int64_t *shape_h, const i
       rows_max = std::max(rows_max, sh0 * sh1 * cc2);
     }
     grid_all = b ? grid_total : (int64_t)c[0] * c[1] * c[2];
-    HIP_TRY(hipMallocAsync((void **)&rowsum, sizeof(double) * (size_t)rows_max, s));
-    HIP_TRY(hipMallocAsync((void **)&zsum, sizeof(double) * (size_t)grid_all, s));
+    HIP_TRY(hipMallocAsync((void **)&rowsum, sizeof(double2) * (size_t)rows_max, s));
+    HIP_TRY(hipMallocAsync((void **)&zsum, sizeof(double2) * (size_t)grid_all, s));
     const unsigned blocks = 4096;
     for (int i = 0; i < nin; ++i) {
       const bmnd_hip_input *in = b ? &b->h[i] : nullptr;

@@ -480,6 +480,16 @@ class BatchStageOutput:
     stage: BatchStage
 
 
+def table_first_order(arrays) -> list[int]:
+    """Batch order that puts inputs of integer counts <= 100 first (stable):
+    stage-1 matching then takes them in one launch of the count-table kernel
+    and the others in one launch of the float-pass kernel."""
+    def table(v):
+        v = np.asarray(v)
+        return bool(np.all(v == np.rint(v)) and float(v.max()) <= 100.0)
+    return sorted(range(len(arrays)), key=lambda i: not table(arrays[i]))
+
+
 class BatchPipeline:
     """The two-stage Poisson denoiser over a batch of inputs of one profile
     and one dimensionality but any shapes: each kernel runs once over the
@@ -503,6 +513,7 @@ class BatchPipeline:
         # matching-mode choice and the per-call constants are PoissonPipeline's
         self._single = PoissonPipeline(profile, shapes[0], self.device, deviance)
         self.stages = {name: self._stage_view(profile, name) for name in ("ht", "wiener")}
+        self.last_runs: dict[str, list] = {}  # (mode, first input, end) per matching launch
 
     @property
     def total_voxels(self) -> int:
@@ -551,6 +562,28 @@ class BatchPipeline:
         return [host[self.vox_off[i]:self.vox_off[i + 1]].reshape(self.shapes[i]).copy()
                 for i in range(len(self.shapes))]
 
+    def input_modes(self, vol: torch.Tensor) -> list[tuple[int, int]]:
+        """(matching mode, table max) of every input of the packed `vol`, as
+        PoissonPipeline.blockmatch_mode chooses them for the input alone."""
+        single = self._single
+        n = len(self.shapes)
+        fixed = {"exact": 0, "one-log": 1, "certified": 3, "fast": 4}
+        if single.deviance in fixed:
+            return [(fixed[single.deviance], 0)] * n
+        fallback = {"auto": 0, "auto-one-log": 1, "auto-certified": 3,
+                    "auto-fast": 4}[single.deviance]
+        if single.count_scale != 1.0:
+            return [(fallback, 0)] * n
+        L = _lib.lib()
+        flags = torch.empty(2 * n, dtype=torch.int32, device=self.device)
+        base, fbase = vol.data_ptr(), flags.data_ptr()
+        for i in range(n):
+            _lib.check(L.bmnd_hip_count_range(ctypes.c_void_p(base + 4 * int(self.vox_off[i])),
+                                              int(self.vox[i]), ctypes.c_void_p(fbase + 8 * i),
+                                              single._stream()))
+        f = flags.view(n, 2).cpu().tolist()
+        return [(2, vmax) if is_int and vmax <= 100 else (fallback, 0) for is_int, vmax in f]
+
     def stage(self, name: str, match_vol: torch.Tensor, noisy: torch.Tensor, var: torch.Tensor,
               fallback: torch.Tensor, timer=None) -> BatchStageOutput:
         L = _lib.lib()
@@ -566,12 +599,27 @@ class BatchPipeline:
         sel = torch.empty(R * plan.max_m, dtype=torch.int32, device=dev)
         cnt = torch.empty(R, dtype=torch.int32, device=dev)
         acc = torch.empty(R, dtype=torch.int32, device=dev)
-        mode, tmax = self._single.blockmatch_mode(match_vol)
+        # one launch per run of consecutive inputs that match in the same mode
+        modes = self.input_modes(match_vol)
+        runs = []
+        for i, (mode, tmax) in enumerate(modes):
+            if runs and runs[-1][0] == mode:
+                runs[-1][1] = max(runs[-1][1], tmax)
+                runs[-1][3] = i + 1
+            else:
+                runs.append([mode, tmax, i, i + 1])
+        self.last_runs[name] = [(m, a, b) for m, _, a, b in runs]
         tok = timer.start(f"{name}.blockmatch")
-        _lib.check(L.bmnd_hip_blockmatch_batch(
-            _ptr(match_vol), ih, idv, nin, block, radius, _ptr(st.ref_lin), _ptr(st.ref_input), R,
-            plan.threshold, plan.max_m, plan.min_m, 1, 1, self._single.count_scale, mode, tmax,
-            _ptr(sel), _ptr(cnt), _ptr(acc), stream))
+        for mode, tmax, a, b in runs:
+            r0, r1 = int(st.ref_off[a]), int(st.ref_off[b])
+            if r1 == r0:
+                continue
+            off = lambda t, n: ctypes.c_void_p(t.data_ptr() + 4 * n)  # noqa: E731
+            _lib.check(L.bmnd_hip_blockmatch_batch(
+                _ptr(match_vol), ih, idv, nin, block, radius, off(st.ref_lin, r0),
+                off(st.ref_input, r0), r1 - r0, plan.threshold, plan.max_m, plan.min_m, 1, 1,
+                self._single.count_scale, mode, tmax, off(sel, r0 * plan.max_m), off(cnt, r0),
+                off(acc, r0), stream))
         timer.stop(tok)
         mats, kai = plan.device_constants(dev)
         k = self._single.k
