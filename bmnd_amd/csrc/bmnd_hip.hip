@@ -429,6 +429,14 @@ __device__ __forceinline__ void bm_finalize(int K, int acc, const BMArgs &a, int
                                             int lo1, int lo2, int e1, int e2, int center_local,
                                             const BMSelScratch &sc);
 
+// Adds every lane's v to *dst with one LDS atomic per wave (all lanes of
+// the wave must call it).
+__device__ __forceinline__ void wave_atomic_add(int *dst, int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, v);
+}
+
 // Selection shared by the block-matching kernels: from the box composites
 // (unique) and the accepted count, keep the K smallest the reference keeps,
 // order them and write the group of reference g: reference first, dedupe,
@@ -685,7 +693,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 
     my_acc += ok ? 1 : 0;
     s_keys[cand] = bm_composite(key, ok, cand);
   }
-  if (my_acc) atomicAdd(&s_acc, my_acc);
+  wave_atomic_add(&s_acc, my_acc);
   __syncthreads();
   BM_SEL_SCRATCH(sc);
   bm_select_write<NT, BOX>(bm_box_keys<NT>([&](int i) { return s_keys[i]; }, box), box, s_acc, a,
@@ -894,7 +902,7 @@ __global__ __launch_bounds__(NT) void k_blockmatch_cert(BMArgs a) {
     my_acc += ok ? 1 : 0;
     keys[cand] = bm_composite(key, ok, cand);
   }
-  if (my_acc) atomicAdd(&s_acc, my_acc);
+  wave_atomic_add(&s_acc, my_acc);
   __syncthreads();
   BM_SEL_SCRATCH(sc);
   bm_select_write<NT, BOX>(bm_box_keys<NT>([&](int i) { return keys[i]; }, box), box, s_acc, a,
@@ -1208,7 +1216,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
       my_maybe += ok_lo ? 1 : 0;
     }
   }
-  if (my_maybe) atomicAdd(&s_maybe, my_maybe);
+  wave_atomic_add(&s_maybe, my_maybe);
   __syncthreads();
   auto comp = [&](float key, int i) {
     if (key == 0.0f) key = 0.0f;  // a signed zero sorts as +0
@@ -1312,16 +1320,20 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
 #pragma unroll
   for (int k = 0; k < KC; ++k) {
     const int cand = tid + k * NT;
-    if (cand < box) {
-      const float key = s_klo[cand];
-      my_acc += key < thr ? 1 : 0;
-      if (need & (1u << k)) {
-        const int slot = atomicAdd(&s_nex, 1);
-        if (slot < XCAP) exset[slot] = comp(key, cand);
-      }
+    const float key = cand < box ? s_klo[cand] : 0.0f;
+    my_acc += (cand < box && key < thr) ? 1 : 0;
+    // the wave's evaluated candidates of this slot take consecutive entries
+    const bool in = cand < box && (need & (1u << k));
+    const unsigned long long inb = __ballot(in);
+    int base = 0;
+    if (lane == 0 && inb) base = atomicAdd(&s_nex, (int)__popcll(inb));
+    base = __shfl(base, 0, 64);
+    if (in) {
+      const int slot = base + (int)__popcll(inb & ((1ull << lane) - 1ull));
+      if (slot < XCAP) exset[slot] = comp(key, cand);
     }
   }
-  if (my_acc) atomicAdd(&s_acc, my_acc);
+  wave_atomic_add(&s_acc, my_acc);
   __syncthreads();
   const int acc = s_acc, nex = s_nex;
   if (nex <= XCAP) {
