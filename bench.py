@@ -53,8 +53,9 @@ def parse() -> argparse.Namespace:
                     choices=["bm4d-256", "bm3d-512x16", "bm3d-64", "batch-uneven"])
     ap.add_argument("--batch", action="store_true",
                     help="run the inputs of the workload as one batch (bmnd_batch)")
-    ap.add_argument("--cpu-sample", type=int, default=64,
-                    help="side of the cube (3-D) or square (2-D) the CPU oracle is timed on")
+    ap.add_argument("--cpu-sample", type=int, default=0,
+                    help="side of the cube (3-D) or square (2-D) the CPU oracle is timed on "
+                         "(default 64 for volumes, 256 for images)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB mebibytes (for scaling "
@@ -105,6 +106,7 @@ def measured_traffic(config: str, deviance: str, kernel: str):
 def cpu_baseline(config: str, noisy: np.ndarray, side: int) -> dict:
     import oracle
     threads = min(os.cpu_count() or 1, 16)
+    side = min(side or (64 if noisy.ndim == 3 else 256), *noisy.shape)
     if noisy.ndim == 3:
         c = [(s - side) // 2 for s in noisy.shape]
         sample = np.ascontiguousarray(noisy[c[0]:c[0] + side, c[1]:c[1] + side, c[2]:c[2] + side])
@@ -360,7 +362,7 @@ def run_batch(args) -> int:
     cpu = None
     if not args.no_cpu_baseline:
         first = runs[0][3][0]
-        cpu = cpu_baseline(args.config, first, min(args.cpu_sample, *first.shape))
+        cpu = cpu_baseline(args.config, first, args.cpu_sample)
     value = voxels / (ms_per_step * 1e-3)
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "voxels/s", "n_gpus": 1,
