@@ -1894,7 +1894,12 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
       s_sig[e] = exv(q, e - q * P);  // every plane exact: the std itself (variance.py:1698)
     }
   } else {
-    // variance.py:1572-1631, float32 sequential plane sums
+    // variance.py:1572-1631, float32 sequential plane sums: one thread per
+    // spatial coefficient forms the sums and the rescaling, then every thread
+    // takes elements of the group to write their limited std.  The squared
+    // member matrix is no longer needed and holds the per-coefficient values.
+    static_assert(NMAX * NMAX >= 2 * P, "limiting scratch does not fit");
+    float *const s_lim = s_g2;
     for (int s = tid; s < P; s += NT) {
       float av[NMAX];  // every plane's approximate std, loaded at once
 #pragma unroll
@@ -1913,22 +1918,24 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
       }
       float target = tot_a - tot_e;
       if (target < 0.0f) target = 0.0f;
-      const float scale = rem_a > 0.0f ? target / rem_a : 0.0f;
-      const float fallback = target / (float)(n - planes);
-#pragma unroll
-      for (int q = 0; q < NMAX; ++q) {
-        if (q < n) {
-          float v;
-          if (q < planes) {
-            const float ev = exv(q, s);
-            v = ev * ev;
-          } else {
-            v = (av[q] * av[q]) * scale;
-            if (!(rem_a > 0.0f)) v = fallback;
-          }
-          s_sig[q * P + s] = sqrtf(v > 0.0f ? v : 0.0f);
-        }
+      // the scale of the approximate planes, or (marked by a negative scale,
+      // rem_a not positive) the fallback variance shared by them
+      s_lim[s] = rem_a > 0.0f ? target / rem_a : -1.0f;
+      s_lim[P + s] = target / (float)(n - planes);
+    }
+    __syncthreads();
+    for (int e = tid; e < np; e += NT) {
+      const int q = e / P, s = e - q * P;
+      float v;
+      if (q < planes) {
+        const float ev = exv(q, s);
+        v = ev * ev;
+      } else {
+        const float sc = s_lim[s];
+        const float av = s_sig[e];
+        v = sc >= 0.0f ? (av * av) * sc : s_lim[P + s];
       }
+      s_sig[e] = sqrtf(v > 0.0f ? v : 0.0f);
     }
   }
   __syncthreads();
