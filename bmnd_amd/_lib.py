@@ -40,6 +40,8 @@ EXPORTS = (
     "bmnd_hip_filter_groups_batch",
     "bmnd_hip_reference_map_batch",
     "bmnd_hip_aggregate_batch",
+    "bmnd_hip_stage_workspace",
+    "bmnd_hip_stage",
 )
 
 
@@ -99,6 +101,11 @@ def _declare(L: ctypes.CDLL) -> None:
     L.bmnd_hip_aggregate_batch.argtypes = [vp, vp, vp, vp, vp, i64, i32, vp, vp, i32, vp, vp, vp,
                                            vp, vp, vp]
     L.bmnd_hip_aggregate_batch.restype = i32
+    L.bmnd_hip_stage_workspace.argtypes = [vp, vp, i64, i32, i32]
+    L.bmnd_hip_stage_workspace.restype = i64
+    L.bmnd_hip_stage.argtypes = [i32, vp, vp, vp, vp, vp, vp, vp, vp, i64, f32, i32, i32, f64, i32,
+                                 vp, vp, f32, i32, vp, i64, vp, vp, vp, vp, vp]
+    L.bmnd_hip_stage.restype = i32
 
 
 def lib() -> ctypes.CDLL:

@@ -3197,4 +3197,87 @@ int32_t bmnd_hip_aggregate_batch(const float *filt, const float *weights, const 
                         stream, &b);
 }
 
+static int64_t align256(int64_t bytes) { return (bytes + 255) & ~(int64_t)255; }
+
+int64_t bmnd_hip_stage_workspace(const int64_t *shape_h, const int32_t *block_h, int64_t nrefs,
+                                 int32_t max_matches, int32_t k) {
+  if (!shape_h || !block_h || nrefs < 0 || max_matches < 1 || k < 0) return -1;
+  int64_t P = 1, patches = 1;
+  for (int d = 0; d < 3; ++d) {
+    if (block_h[d] < 1 || shape_h[d] < block_h[d]) return -1;
+    P *= block_h[d];
+    patches *= shape_h[d] - block_h[d] + 1;
+  }
+  const int64_t kcap = std::min<int64_t>(k, max_matches);
+  return align256(2 * sizeof(int32_t)) + align256(nrefs * kcap * P * (int64_t)sizeof(float)) +
+         align256(nrefs * max_matches * P * (int64_t)sizeof(float)) +
+         align256(nrefs * max_matches * (int64_t)sizeof(float)) +
+         align256(patches * (int64_t)sizeof(int32_t));
+}
+
+int32_t bmnd_hip_stage(int32_t wiener, const float *match_vol, const float *noisy,
+                       const float *var, const float *fallback, const int64_t *shape_h,
+                       const int32_t *block_h, const int32_t *radius_h, const int32_t *ref_lin,
+                       int64_t nrefs, float threshold, int32_t max_matches, int32_t min_matches,
+                       double count_scale, int32_t mode, const float *matrices,
+                       const float *kaiser, float shrink, int32_t k, void *workspace,
+                       int64_t workspace_bytes, float *est, int32_t *sel, int32_t *counts,
+                       int32_t *accepted, void *stream) {
+  const int64_t need = bmnd_hip_stage_workspace(shape_h, block_h, nrefs, max_matches, k);
+  if (need < 0) return fail(BMND_HIP_EINVAL, "invalid stage geometry");
+  if (!workspace || workspace_bytes < need)
+    return fail(BMND_HIP_EINVAL, "workspace is smaller than bmnd_hip_stage_workspace()");
+  if (mode < -1 || mode > 4) return fail(BMND_HIP_EINVAL, "mode must be -1 (automatic) or 0..4");
+  hipStream_t s = (hipStream_t)stream;
+  int64_t P = 1, patches = 1, vox = 1;
+  for (int d = 0; d < 3; ++d) {
+    P *= block_h[d];
+    patches *= shape_h[d] - block_h[d] + 1;
+    vox *= shape_h[d];
+  }
+  const int64_t kcap = std::min<int64_t>(k, max_matches);
+  char *w = static_cast<char *>(workspace);
+  int32_t *flags = reinterpret_cast<int32_t *>(w);
+  w += align256(2 * sizeof(int32_t));
+  float *exact = kcap > 0 ? reinterpret_cast<float *>(w) : nullptr;
+  w += align256(nrefs * kcap * P * (int64_t)sizeof(float));
+  float *filt = reinterpret_cast<float *>(w);
+  w += align256(nrefs * max_matches * P * (int64_t)sizeof(float));
+  float *weights = reinterpret_cast<float *>(w);
+  w += align256(nrefs * max_matches * (int64_t)sizeof(float));
+  int32_t *ref_index = reinterpret_cast<int32_t *>(w);
+  int32_t table_max = 0;
+  if (mode < 0) {
+    // the automatic choice: the count table for integer counts <= 100 at unit
+    // count scale, the fast certified matcher otherwise
+    mode = 4;
+    if (count_scale == 1.0) {
+      if (int rc = bmnd_hip_count_range(match_vol, vox, flags, stream)) return rc;
+      int32_t h[2] = {0, 0};
+      HIP_TRY(hipMemcpyAsync(h, flags, sizeof(h), hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipStreamSynchronize(s));
+      if (h[0] && h[1] <= kMaxTableCount) {
+        mode = 2;
+        table_max = h[1];
+      }
+    }
+  }
+  if (int rc = bmnd_hip_blockmatch(match_vol, shape_h, block_h, radius_h, ref_lin, nrefs,
+                                   threshold, max_matches, min_matches, 1, 1, count_scale, mode,
+                                   table_max, sel, counts, accepted, stream))
+    return rc;
+  if (kcap > 0)
+    if (int rc = bmnd_hip_exact_std(var, shape_h, block_h, radius_h, sel, counts, nrefs,
+                                    max_matches, matrices, k, exact, stream))
+      return rc;
+  if (int rc = bmnd_hip_filter_groups(wiener, noisy, wiener ? match_vol : nullptr, var, shape_h,
+                                      block_h, radius_h, sel, counts, nrefs, max_matches,
+                                      matrices, exact, shrink, k, count_scale, filt, weights,
+                                      stream))
+    return rc;
+  if (int rc = bmnd_hip_reference_map(ref_lin, nrefs, ref_index, patches, stream)) return rc;
+  return bmnd_hip_aggregate(filt, weights, sel, counts, ref_index, nrefs, max_matches, shape_h,
+                            block_h, radius_h, kaiser, fallback, est, stream);
+}
+
 }  // extern "C"

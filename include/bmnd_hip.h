@@ -204,6 +204,34 @@ int32_t bmnd_hip_poisson_global_sigma(const float *var, int64_t n,
                                       double variance_floor, double *global_sigma,
                                       void *stream);
 
+/* ---- one whole stage ---------------------------------------------------------
+ * Device workspace bytes bmnd_hip_stage needs (the exact-plane std, filtered
+ * patches, weights and reference map of one stage), or -1 for an invalid
+ * geometry.  Host-only. */
+int64_t bmnd_hip_stage_workspace(const int64_t *shape_h, const int32_t *block_h,
+                                 int64_t nrefs, int32_t max_matches, int32_t k);
+
+/* One denoising stage of the Poisson bmnd(), the HT pilot (wiener == 0) or the
+ * Wiener estimate: block matching, exact-plane variance, group filtering,
+ * reference map and ordered aggregation, in one call on the caller's stream
+ * (bmndalgo.py:241-633 for HT, 676-1046 for Wiener).  The caller owns every
+ * buffer: match_vol (the observation for HT, the pilot for Wiener), noisy
+ * (the observation), var (the stage's variance volume, bmnd_hip_variance_volume),
+ * fallback (the observation), the schedule ref_lin, the matrix table
+ * (bmnd_hip_matrix_table_size), the Kaiser window (P floats), the workspace of
+ * bmnd_hip_stage_workspace() bytes, and the outputs est, sel, counts,
+ * accepted.  mode as in bmnd_hip_blockmatch, or -1 for the automatic choice
+ * (the count table for integer counts <= 100 at count_scale 1, which reads
+ * two integers back from the device, else mode 4). */
+int32_t bmnd_hip_stage(int32_t wiener, const float *match_vol, const float *noisy,
+                       const float *var, const float *fallback, const int64_t *shape_h,
+                       const int32_t *block_h, const int32_t *radius_h, const int32_t *ref_lin,
+                       int64_t nrefs, float threshold, int32_t max_matches, int32_t min_matches,
+                       double count_scale, int32_t mode, const float *matrices,
+                       const float *kaiser, float shrink, int32_t k, void *workspace,
+                       int64_t workspace_bytes, float *est, int32_t *sel, int32_t *counts,
+                       int32_t *accepted, void *stream);
+
 /* ---- batches of inputs of different shapes --------------------------------
  * No counterpart in the reference, which denoises one array per call: a batch
  * runs every input of one profile (one block / search geometry) through each

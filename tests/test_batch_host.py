@@ -68,3 +68,23 @@ def test_batch_of_all_zero_inputs_needs_no_device():
     z = [np.zeros((9, 12), np.float32), np.zeros((16, 10), np.float32)]
     outs = bmnd_batch(z, BM3DProfile(noise_model=NoiseModel.POISSON))
     assert [o.shape for o in outs] == [(9, 12), (16, 10)] and not any(np.any(o) for o in outs)
+
+
+def test_stage_workspace_size_and_checks():
+    """bmnd_hip_stage_workspace: exact-plane std, filtered patches, weights and
+    reference map, each 256-byte aligned; the driver rejects a short
+    workspace before any device work."""
+    L = _lib.lib()
+    i64x3, i32x3 = ctypes.c_int64 * 3, ctypes.c_int32 * 3
+    R, M, k = 1000, 32, 4
+    got = L.bmnd_hip_stage_workspace(i64x3(30, 40, 50), i32x3(5, 5, 5), R, M, k)
+    al = lambda b: (b + 255) // 256 * 256  # noqa: E731
+    want = (al(8) + al(R * 4 * 125 * 4) + al(R * M * 125 * 4) + al(R * M * 4)
+            + al(26 * 36 * 46 * 4))
+    assert got == want
+    assert L.bmnd_hip_stage_workspace(i64x3(4, 40, 50), i32x3(5, 5, 5), R, M, k) == -1
+    dummy = ctypes.c_void_p(16)
+    rc = L.bmnd_hip_stage(0, dummy, dummy, dummy, dummy, i64x3(30, 40, 50), i32x3(5, 5, 5),
+                          i32x3(7, 7, 7), dummy, R, 1.0, M, 2, 1.0, -1, dummy, dummy, 3.0, k,
+                          dummy, got - 1, dummy, dummy, dummy, dummy, None)
+    assert rc == -1 and b"workspace" in L.bmnd_hip_last_error()
