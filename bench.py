@@ -103,6 +103,35 @@ def measured_traffic(config: str, deviance: str, kernel: str):
         return None
 
 
+def crop(a: np.ndarray, side: int) -> np.ndarray:
+    c = [(s - side) // 2 for s in a.shape]
+    return np.ascontiguousarray(a[tuple(slice(c[d], c[d] + side) for d in range(a.ndim))])
+
+
+def fp64_check(profile, noisy: np.ndarray, truth: np.ndarray, side: int) -> dict:
+    """On the CPU-baseline sample: the GPU output against the float32 oracle
+    (bits) and against the oracle's float64 run (the error of the float32
+    path), PSNR against the noiseless phantom."""
+    import torch
+
+    import oracle
+    from bmnd_amd.engine import PoissonPipeline
+    from bmnd_amd.phantoms import psnr
+    side = min(side or (64 if noisy.ndim == 3 else 256), *noisy.shape)
+    sample, tsample = crop(noisy, side), crop(truth, side)
+    dev = torch.device("cuda", 0)
+    pipe = PoissonPipeline(profile, sample.shape, dev)
+    gpu = pipe.run(torch.from_numpy(sample).to(dev))[1].est.cpu().numpy()
+    threads = min(os.cpu_count() or 1, 16)
+    o32, _ = oracle.bmnd_poisson(sample, nthreads=threads)
+    o64, _ = oracle.bmnd_poisson_f64(sample, nthreads=threads)
+    return {"sample": f"{side}^{sample.ndim} centre crop",
+            "gpu_vs_oracle_bits_differing": int(np.sum(gpu.view(np.int32) != o32.view(np.int32))),
+            "gpu_vs_fp64_max_rel": float(np.max(np.abs(gpu - o64)) / np.max(np.abs(o64))),
+            "psnr_gpu_db": round(psnr(gpu, tsample), 4),
+            "psnr_fp64_db": round(psnr(o64, tsample), 4)}
+
+
 def cpu_baseline(config: str, noisy: np.ndarray, side: int) -> dict:
     import oracle
     threads = min(os.cpu_count() or 1, 16)
@@ -238,6 +267,8 @@ def main() -> int:
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.config, noisy_list[0], args.cpu_sample)
+        # the same leg checks the GPU on the sample against the oracle's runs
+        cpu["fp64_check"] = fp64_check(profile, noisy_list[0], truth_list[0], args.cpu_sample)
 
     value = voxels / (ms_per_step * 1e-3)
     line = {

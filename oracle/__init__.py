@@ -377,6 +377,83 @@ def default_params(ndim: int, shape, nthreads: int = 0) -> OracleParams:
     return p
 
 
+# ---- the FP64 build (libbmnd_oracle_f64.so): the same algorithm with every
+# float32 quantity in float64, for the error of the float32 path
+_LIB64_PATH = os.path.join(_HERE, "libbmnd_oracle_f64.so")
+_LIB64 = None
+
+OracleParams64 = type("OracleParams64", (ctypes.Structure,), {
+    "_fields_": [(n, ctypes.c_double if t is ctypes.c_float else t)
+                 for n, t in OracleParams._fields_]})
+
+
+def lib64() -> ctypes.CDLL:
+    global _LIB64
+    if _LIB64 is None:
+        if not os.path.exists(_LIB64_PATH):
+            build(force=True)
+        L = ctypes.CDLL(_LIB64_PATH)
+        vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
+        L.oracle_bmnd_poisson.argtypes = [vp, ctypes.POINTER(OracleParams64), vp, vp, vp, vp, i64,
+                                          vp, vp, i64]
+        L.oracle_bmnd_poisson.restype = i32
+        L.oracle_dct_matrix.argtypes = [ctypes.c_int, vp, vp]
+        L.oracle_dct_matrix.restype = None
+        L.oracle_bior15_matrix.argtypes = [ctypes.c_int, vp, vp]
+        L.oracle_bior15_matrix.restype = ctypes.c_int
+        _LIB64 = L
+    return _LIB64
+
+
+def _spatial_matrices_f64(kind: str, block) -> np.ndarray:
+    """Analysis then synthesis matrices of each axis in float64: the DCT-II
+    from float64 cosines, bior1.5 from its decimal table with a float64
+    inverse (the FP64 build's own constructions)."""
+    L = lib64()
+    fwd, inv = [], []
+    for b in block:
+        f = np.empty((b, b), np.float64)
+        i = np.empty((b, b), np.float64)
+        if kind == "dct":
+            L.oracle_dct_matrix(b, f.ctypes.data, i.ctypes.data)
+        elif L.oracle_bior15_matrix(b, f.ctypes.data, i.ctypes.data) != 0:
+            raise ValueError(f"no bior1.5 matrix of size {b}")
+        fwd.append(f)
+        inv.append(i)
+    return np.concatenate([m.reshape(-1) for m in fwd + inv])
+
+
+def default_params_f64(ndim: int, shape, nthreads: int = 0) -> OracleParams64:
+    """default_params() for the FP64 build."""
+    p32 = default_params(ndim, shape, nthreads)
+    p = OracleParams64()
+    for name, _ in OracleParams._fields_:
+        if name not in ("ht_spatial", "wi_spatial"):
+            setattr(p, name, getattr(p32, name))
+    hb = tuple(p32.ht_block[d] for d in range(ndim))
+    wb = tuple(p32.wi_block[d] for d in range(ndim))
+    p._keep = (_spatial_matrices_f64("bior", hb), _spatial_matrices_f64("dct", wb))
+    p.ht_spatial = p._keep[0].ctypes.data
+    p.wi_spatial = p._keep[1].ctypes.data
+    return p
+
+
+def bmnd_poisson_f64(vol: np.ndarray, params: OracleParams64 | None = None, *,
+                     nthreads: int = 0):
+    """The two-stage pipeline computed in float64 throughout: (out, pilot)
+    as float64 arrays."""
+    v = np.ascontiguousarray(vol, dtype=np.float64)
+    if params is None:
+        params = default_params_f64(v.ndim, v.shape, nthreads)
+    out = np.empty_like(v)
+    pilot = np.empty_like(v)
+    rc = lib64().oracle_bmnd_poisson(_p(v), ctypes.byref(params), _p(out), _p(pilot), None, None,
+                                     0, None, None, 0)
+    if rc != 0:
+        raise RuntimeError(f"oracle_bmnd_poisson (float64) failed with code {rc}")
+    return out, pilot
+
+
 def np_sum_f32(a: np.ndarray) -> np.float32:
     """numpy's float32 summation order, restated (np.sum of a contiguous array)."""
     v = np.ascontiguousarray(a, dtype=np.float32).ravel()
