@@ -23,6 +23,7 @@ from .enums import (
     PoissonHTMatchPolicy,
     PoissonVarianceSource,
     PoissonWienerGainMode,
+    coerce_enum,
 )
 from .profiles import BMNDProfile
 from .shifts import ScheduleArrays, get_reference_schedule_arrays
@@ -150,10 +151,6 @@ def check_supported(profile: BMNDProfile, resolved) -> None:
          "poisson_wiener_gain_mode='variance-scaled'"),
         (profile.poisson_group_mass_conservation is PoissonGroupMassConservation.NONE,
          "poisson_group_mass_conservation='none'"),
-        (profile.poisson_variance_source_wiener is PoissonVarianceSource.PILOT,
-         "poisson_variance_source_wiener='pilot'"),
-        (profile.poisson_wiener_match_source is PoissonVarianceSource.PILOT,
-         "poisson_wiener_match_source='pilot'"),
         (not profile.ht_use_soft_thresholding, "ht_use_soft_thresholding=False"),
     ]
     for stage in ("ht", "wiener"):
@@ -306,6 +303,22 @@ class PoissonPipeline:
         self._refs = {}
 
     # -- helpers -----------------------------------------------------------------
+    def _wiener_sources(self, vol: torch.Tensor, pilot: torch.Tensor, var1: torch.Tensor, t):
+        """Volume the Wiener stage matches on and its variance volume: the
+        pilot and the pilot's variance by default, the observation and the
+        stage-1 variance with the OBSERVATION sources (bmndalgo.py:635-646,
+        714-720)."""
+        def source(field):
+            return coerce_enum(getattr(self.profile, field), PoissonVarianceSource, field)
+        on_obs = source("poisson_wiener_match_source") is PoissonVarianceSource.OBSERVATION
+        if source("poisson_variance_source_wiener") is PoissonVarianceSource.OBSERVATION:
+            var2 = var1
+        else:
+            tok = t.start("wiener.variance")
+            var2 = self.variance(pilot)
+            t.stop(tok)
+        return (vol if on_obs else pilot), var2
+
     def _stream(self) -> ctypes.c_void_p:
         return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
 
@@ -353,9 +366,11 @@ class PoissonPipeline:
         return float(g.value)
 
     def stage(self, name: str, match_vol: torch.Tensor, noisy: torch.Tensor, var: torch.Tensor,
-              fallback: torch.Tensor, timer=None, sigma_fill: float | None = None) -> StageOutput:
+              fallback: torch.Tensor, timer=None, sigma_fill: float | None = None,
+              pilot: torch.Tensor | None = None) -> StageOutput:
         """One stage.  With `sigma_fill` (the global sigma) the stage also
-        builds its sigma map, `sigma_fill` where no patch reaches a voxel."""
+        builds its sigma map, `sigma_fill` where no patch reaches a voxel.
+        The Wiener gains use `pilot` (default: the matched volume)."""
         L = _lib.lib()
         timer = timer or _NoTimer()
         plan = self.plans[name]
@@ -396,7 +411,8 @@ class PoissonPipeline:
                                device=dev)
         tok = timer.start(f"{name}.filter")
         _lib.check(L.bmnd_hip_filter_groups_sigma(int(wiener), _ptr(noisy),
-                                                  _ptr(match_vol if wiener else None), _ptr(var),
+                                                  _ptr((match_vol if pilot is None else pilot)
+                                                       if wiener else None), _ptr(var),
                                                   shape, block, radius, _ptr(sel), _ptr(cnt), R,
                                                   plan.max_m, _ptr(mats), _ptr(exact),
                                                   plan.shrink, self.k, self.count_scale,
@@ -431,11 +447,9 @@ class PoissonPipeline:
         gsig = self.global_sigma(var1) if sigma_maps else None
         fill = None if gsig is None else float(np.float32(gsig))
         ht = self.stage("ht", vol, vol, var1, vol, timer, fill)
+        match2, var2 = self._wiener_sources(vol, ht.est, var1, t)
         del var1
-        tok = t.start("wiener.variance")
-        var2 = self.variance(ht.est)
-        t.stop(tok)
-        wi = self.stage("wiener", ht.est, vol, var2, vol, timer, fill)
+        wi = self.stage("wiener", match2, vol, var2, vol, timer, fill, pilot=ht.est)
         if sigma_maps:
             return ht, wi, gsig
         return ht, wi
@@ -585,7 +599,8 @@ class BatchPipeline:
         return [(2, vmax) if is_int and vmax <= 100 else (fallback, 0) for is_int, vmax in f]
 
     def stage(self, name: str, match_vol: torch.Tensor, noisy: torch.Tensor, var: torch.Tensor,
-              fallback: torch.Tensor, timer=None) -> BatchStageOutput:
+              fallback: torch.Tensor, timer=None,
+              pilot: torch.Tensor | None = None) -> BatchStageOutput:
         L = _lib.lib()
         timer = timer or _NoTimer()
         st = self.stages[name]
@@ -637,7 +652,8 @@ class BatchPipeline:
         wiener = name == "wiener"
         tok = timer.start(f"{name}.filter")
         _lib.check(L.bmnd_hip_filter_groups_batch(
-            int(wiener), _ptr(noisy), _ptr(match_vol if wiener else None), _ptr(var), ih, idv, nin,
+            int(wiener), _ptr(noisy),
+            _ptr((match_vol if pilot is None else pilot) if wiener else None), _ptr(var), ih, idv, nin,
             block, radius, _ptr(sel), _ptr(cnt), _ptr(st.ref_input), R, plan.max_m, _ptr(mats),
             _ptr(exact), plan.shrink, k, self._single.count_scale, _ptr(filt), _ptr(wts), stream))
         timer.stop(tok)
@@ -665,9 +681,7 @@ class BatchPipeline:
         var1 = self._single.variance(packed)
         t.stop(tok)
         ht = self.stage("ht", packed, packed, var1, packed, timer)
+        match2, var2 = self._single._wiener_sources(packed, ht.est, var1, t)
         del var1
-        tok = t.start("wiener.variance")
-        var2 = self._single.variance(ht.est)
-        t.stop(tok)
-        wi = self.stage("wiener", ht.est, packed, var2, packed, timer)
+        wi = self.stage("wiener", match2, packed, var2, packed, timer, pilot=ht.est)
         return ht, wi

@@ -59,6 +59,9 @@ class OracleParams(ctypes.Structure):
         ("nthreads", ctypes.c_int),
         ("ht_spatial", ctypes.c_void_p),
         ("wi_spatial", ctypes.c_void_p),
+        ("wi_match_observation", ctypes.c_int),
+        ("wi_var_observation", ctypes.c_int),
+        ("ht_thresholds", ctypes.c_void_p),
     ]
 
 
@@ -428,7 +431,7 @@ def default_params_f64(ndim: int, shape, nthreads: int = 0) -> OracleParams64:
     p32 = default_params(ndim, shape, nthreads)
     p = OracleParams64()
     for name, _ in OracleParams._fields_:
-        if name not in ("ht_spatial", "wi_spatial"):
+        if name not in ("ht_spatial", "wi_spatial", "ht_thresholds"):
             setattr(p, name, getattr(p32, name))
     hb = tuple(p32.ht_block[d] for d in range(ndim))
     wb = tuple(p32.wi_block[d] for d in range(ndim))

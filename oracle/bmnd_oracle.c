@@ -1113,7 +1113,8 @@ static void variance_volume(const float *v, int64_t n, double scale, double floo
 
 /* one stage: block matching + group filtering + sequential aggregation */
 static int run_stage(const oracle_params *prm, int wiener, const float *match_vol,
-                     const float *noisy, const float *var_vol, const float *fallback,
+                     const float *pilot, const float *noisy, const float *var_vol,
+                     const float *fallback,
                      float *est, int64_t *sel_dump, int32_t *cnt_dump, int64_t dump_cap,
                      float *sigma_map, float sigma_fill) {
   const int *block = wiener ? prm->wi_block : prm->ht_block;
@@ -1134,7 +1135,8 @@ static int run_stage(const oracle_params *prm, int wiener, const float *match_vo
   oracle_schedule_generated(ndim, g.counts, block, step, prm->shift_density,
                             prm->schedule_density, refs, nrefs);
   float *thrs = (float *)malloc(sizeof(float) * (size_t)(nrefs + 1));
-  for (int64_t i = 0; i < nrefs; ++i) thrs[i] = thr;
+  for (int64_t i = 0; i < nrefs; ++i)
+    thrs[i] = (!wiener && prm->ht_thresholds) ? prm->ht_thresholds[i] : thr;
   int64_t *sel = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nrefs * max_m + 1));
   int32_t *cnt = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nrefs + 1));
   oracle_blockmatch(NULL, match_vol, ndim, g.shape, g.counts, block, refs, nrefs, radius,
@@ -1200,7 +1202,7 @@ static int run_stage(const oracle_params *prm, int wiener, const float *match_vo
         int64_t base = lin_to_vol_base(&g, lin);
         for (int p = 0; p < P; ++p) X[i * P + p] = noisy[base + g.poff[p]];
         if (wiener)
-          for (int p = 0; p < P; ++p) Y[i * P + p] = match_vol[base + g.poff[p]];
+          for (int p = 0; p < P; ++p) Y[i * P + p] = pilot[base + g.poff[p]];
       }
       oracle_apply_transform(X, ndim + 1, dims, fm);
       oracle_noise_std(var_vol, ndim, g.shape, block, n, apos, gfwd, spf, prm->k, sig);
@@ -1337,10 +1339,16 @@ int oracle_bmnd_poisson_sigma(const float *vol, const oracle_params *prm, float 
   variance_volume(vol, nvox, prm->count_scale, prm->variance_floor, var1);
   const double gs = oracle_poisson_global_sigma(var1, nvox, prm->variance_floor);
   if (global_sigma) *global_sigma = gs;
-  run_stage(prm, 0, vol, vol, var1, vol, est1, ht_sel, ht_cnt, ht_cap, ht_sigma_map, (float)gs);
+  run_stage(prm, 0, vol, NULL, vol, var1, vol, est1, ht_sel, ht_cnt, ht_cap, ht_sigma_map,
+            (float)gs);
   if (pilot) memcpy(pilot, est1, sizeof(float) * (size_t)nvox);
-  variance_volume(est1, nvox, prm->count_scale, prm->variance_floor, var1);
-  run_stage(prm, 1, est1, vol, var1, vol, out, wi_sel, wi_cnt, wi_cap, wi_sigma_map, (float)gs);
+  /* Wiener variance model from the pilot, or the observation's model kept
+   * (bmndalgo.py:635-646); matching on the pilot or the observation
+   * (bmndalgo.py:714-720); the gains always use the pilot */
+  if (!prm->wi_var_observation)
+    variance_volume(est1, nvox, prm->count_scale, prm->variance_floor, var1);
+  run_stage(prm, 1, prm->wi_match_observation ? vol : est1, est1, vol, var1, vol, out, wi_sel,
+            wi_cnt, wi_cap, wi_sigma_map, (float)gs);
   free(var1);
   free(est1);
   return 0;

@@ -114,6 +114,15 @@ typedef struct {
    * them here (bior1.5 for HT, DCT for Wiener) */
   const float *ht_spatial;
   const float *wi_spatial;
+  /* non-default Poisson options (profiles.py:42-72): Wiener matching on the
+   * observation instead of the pilot (poisson_wiener_match_source), the
+   * Wiener variance model from the observation (poisson_variance_source_wiener),
+   * and per-reference HT thresholds (poisson_ht_match_policy
+   * "reference_finite_count", one per HT reference in schedule order; NULL
+   * for the scalar ht_threshold) */
+  int wi_match_observation;
+  int wi_var_observation;
+  const float *ht_thresholds;
 } oracle_params;
 
 /* Runs both stages.  pilot may be NULL.  The match lists of each stage can
