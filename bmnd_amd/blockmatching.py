@@ -10,12 +10,15 @@ the device pipeline keeps match lists as flat int32 arrays.
 
 from __future__ import annotations
 
+import functools
+import math
 from dataclasses import dataclass
 
 import numpy as np
 from numpy.typing import NDArray
 
 from .enums import BlockMatchDistance, coerce_enum
+from .shifts import ScheduleArrays
 from .profilevalidation import resolve_blockmatch_distance  # noqa: F401  (re-export)
 
 
@@ -77,3 +80,89 @@ def build_result(schedule, counts: tuple[int, ...], sel: np.ndarray, final: np.n
             accepted_count=int(accepted[g]),
             final_count=int(final[g])))
     return BlockMatchResult(groups=groups)
+
+
+# ---- per-reference thresholds (poisson_ht_match_policy="reference_finite_count")
+
+@functools.cache
+def poisson_null_moment_table(distance_measure=BlockMatchDistance.POISSON_DEVIANCE,
+                              max_count: int = 512) -> tuple[np.ndarray, np.ndarray]:
+    """Mean and variance of one Poisson-deviance term when the reference and
+    candidate counts are a Binomial(N, 1/2) split of a pooled count N, for N =
+    0..max_count, summed in the reference's order in float64
+    (blockmatching.py:29-92; N = 0 stays 0)."""
+    distance_measure = coerce_enum(distance_measure, BlockMatchDistance, "block-matching distance")
+    if distance_measure is not BlockMatchDistance.POISSON_DEVIANCE:
+        raise NotImplementedError("per-reference thresholds run with 'poisson_deviance' only.")
+    if max_count < 1:
+        raise ValueError(f"max_count must be positive, got {max_count}.")
+    means = np.zeros(max_count + 1, dtype=np.float64)
+    variances = np.zeros(max_count + 1, dtype=np.float64)
+    for pooled in range(1, max_count + 1):
+        expectation = 0.0
+        second = 0.0
+        scale = 2.0 ** (-pooled)
+        for r in range(pooled + 1):
+            c = pooled - r
+            prob = math.comb(pooled, r) * scale
+            term = 0.0
+            if r > 0:
+                term += r * math.log(2.0 * r / pooled)
+            if c > 0:
+                term += c * math.log(2.0 * c / pooled)
+            term *= 2.0
+            expectation += prob * term
+            second += prob * term * term
+        means[pooled] = expectation
+        variances[pooled] = max(second - expectation * expectation, 0.0)
+    return means, variances
+
+
+def reference_patches(volume: np.ndarray, schedule: ScheduleArrays, block, start: int,
+                      stop: int) -> np.ndarray:
+    """float32 patches (rows in C order) of references start..stop-1."""
+    ref = schedule.reference_abs[start:stop]
+    offs = np.stack(np.meshgrid(*[np.arange(b) for b in block], indexing="ij"),
+                    axis=-1).reshape(-1, len(block))
+    idx = ref[:, None, :] + offs[None, :, :]
+    return volume[tuple(idx[..., d] for d in range(len(block)))].astype(np.float32, copy=False)
+
+
+def compute_poisson_reference_thresholds(volume: np.ndarray, schedule: ScheduleArrays, block,
+                                         count_scale: float, ht_match_threshold: float,
+                                         structure_beta: float, intensity_power: float,
+                                         max_count: int = 512, chunk: int = 65536) -> np.ndarray:
+    """Per-reference stage-1 match thresholds of the "reference_finite_count"
+    policy (blockmatching.py:95-148), vectorised over references: the mean
+    of the reference patch's null means plus ht_match_threshold times the
+    standard deviation of the null distance, plus structure_beta times the
+    mean reference count to intensity_power; float32 per reference."""
+    count_scale = float(count_scale)
+    if not np.isfinite(ht_match_threshold) or ht_match_threshold < 0.0:
+        raise ValueError("ht_match_threshold must be finite and non-negative, "
+                         f"got {ht_match_threshold}.")
+    if not np.isfinite(structure_beta) or structure_beta < 0.0:
+        raise ValueError(f"structure_beta must be finite and non-negative, got {structure_beta}.")
+    if not np.isfinite(intensity_power) or intensity_power <= 0.0:
+        raise ValueError(f"intensity_power must be finite and positive, got {intensity_power}.")
+    means, variances = poisson_null_moment_table(BlockMatchDistance.POISSON_DEVIANCE, max_count)
+    P = int(np.prod(block))
+    out = np.empty(len(schedule), dtype=np.float32)
+    for a in range(0, len(schedule), chunk):
+        b = min(a + chunk, len(schedule))
+        counts = np.clip(reference_patches(volume, schedule, block, a, b).astype(np.float64)
+                         / count_scale, 0.0, None)
+        pooled = np.rint(2.0 * counts)
+        idx = np.minimum(pooled, max_count).astype(np.int64)
+        null_means = means[idx]
+        null_vars = variances[idx]
+        above = pooled > max_count
+        null_means[above] = 1.0
+        null_vars[above] = 2.0
+        patch_mean = np.mean(null_means, axis=1)
+        patch_var = np.sum(null_vars, axis=1) / (P * P)
+        intensity = np.mean(counts, axis=1)
+        thr = (patch_mean + float(ht_match_threshold) * np.sqrt(np.maximum(patch_var, 0.0))
+               + float(structure_beta) * np.power(intensity, float(intensity_power)))
+        out[a:b] = thr.astype(np.float32)
+    return out

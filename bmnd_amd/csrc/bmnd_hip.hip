@@ -280,11 +280,19 @@ struct BMArgs {
   // batch: every reference's input (nullptr for a single input)
   const bmnd_hip_input *batch;
   const int32_t *ref_input;
+  // per-reference thresholds (nullptr: the scalar threshold for all)
+  const float *ref_thr;
 };
 
 // A batched launch takes the geometry of reference g's input and moves the
-// volume pointers to that input's part of the packed buffers.
+// volume pointers to that input's part of the packed buffers; with
+// per-reference thresholds, g's threshold and its tie-grid key replace the
+// scalar ones.
 __device__ __forceinline__ void bind_input(BMArgs &a, int64_t g) {
+  if (a.ref_thr) {
+    a.threshold = a.ref_thr[g];
+    a.thr_key = canonical_key(a.threshold);
+  }
   if (!a.batch) return;
   const bmnd_hip_input &d = a.batch[a.ref_input[g]];
   a.vol += d.vox_off;
@@ -2647,7 +2655,8 @@ static int32_t blockmatch_impl(const float *vol, const int64_t *shape_h, const i
                                float threshold, int32_t max_matches, int32_t min_matches,
                                int32_t pow2, int32_t stabilize, double count_scale,
                                int32_t mode, int32_t table_max, int32_t *sel, int32_t *counts,
-                               int32_t *accepted, void *stream, const BatchRef *b) {
+                               int32_t *accepted, void *stream, const BatchRef *b,
+                               const float *ref_thresholds = nullptr) {
   int c[3];
   int64_t shape1[3];
   int64_t grid_total = 0;
@@ -2722,6 +2731,7 @@ static int32_t blockmatch_impl(const float *vol, const int64_t *shape_h, const i
   a.accepted = accepted;
   a.batch = b ? b->d : nullptr;
   a.ref_input = b ? b->ref_input : nullptr;
+  a.ref_thr = ref_thresholds;
   hipStream_t s = (hipStream_t)stream;
   dim3 grid((unsigned)nrefs), blk(kBMThreads);
   const size_t dyn = mode == 2 ? sizeof(double) * (size_t)(table_max + 1) * (table_max + 1) : 0;
@@ -2814,20 +2824,33 @@ int32_t bmnd_hip_blockmatch(const float *vol, const int64_t *shape_h, const int3
                          sel, counts, accepted, stream, nullptr);
 }
 
+int32_t bmnd_hip_blockmatch_thresholds(const float *vol, const int64_t *shape_h,
+                                       const int32_t *block_h, const int32_t *radius_h,
+                                       const int32_t *ref_lin, const float *ref_thresholds,
+                                       int64_t nrefs, int32_t max_matches, int32_t min_matches,
+                                       int32_t pow2, int32_t stabilize, double count_scale,
+                                       int32_t mode, int32_t table_max, int32_t *sel,
+                                       int32_t *counts, int32_t *accepted, void *stream) {
+  if (!ref_thresholds) return fail(BMND_HIP_EINVAL, "ref_thresholds must not be null");
+  return blockmatch_impl(vol, shape_h, block_h, radius_h, ref_lin, nrefs, 0.0f, max_matches,
+                         min_matches, pow2, stabilize, count_scale, mode, table_max, sel, counts,
+                         accepted, stream, nullptr, ref_thresholds);
+}
+
 int32_t bmnd_hip_blockmatch_batch(const float *vol, const bmnd_hip_input *inputs_h,
                                   const bmnd_hip_input *inputs_d, int32_t ninputs,
                                   const int32_t *block_h, const int32_t *radius_h,
                                   const int32_t *ref_lin, const int32_t *ref_input,
-                                  int64_t nrefs, float threshold, int32_t max_matches,
-                                  int32_t min_matches, int32_t pow2, int32_t stabilize,
-                                  double count_scale, int32_t mode, int32_t table_max,
-                                  int32_t *sel, int32_t *counts, int32_t *accepted,
-                                  void *stream) {
+                                  const float *ref_thresholds, int64_t nrefs, float threshold,
+                                  int32_t max_matches, int32_t min_matches, int32_t pow2,
+                                  int32_t stabilize, double count_scale, int32_t mode,
+                                  int32_t table_max, int32_t *sel, int32_t *counts,
+                                  int32_t *accepted, void *stream) {
   if (!ref_input) return fail(BMND_HIP_EINVAL, "a batch needs ref_input");
   const BatchRef b{inputs_h, inputs_d, ninputs, ref_input};
   return blockmatch_impl(vol, nullptr, block_h, radius_h, ref_lin, nrefs, threshold, max_matches,
                          min_matches, pow2, stabilize, count_scale, mode, table_max, sel, counts,
-                         accepted, stream, &b);
+                         accepted, stream, &b, ref_thresholds);
 }
 
 int64_t bmnd_hip_matrix_table_size(int32_t max_matches, const int32_t *block_h) {

@@ -25,6 +25,7 @@ from .enums import (
     PoissonWienerGainMode,
     coerce_enum,
 )
+from .blockmatching import compute_poisson_reference_thresholds
 from .profiles import BMNDProfile
 from .shifts import ScheduleArrays, get_reference_schedule_arrays
 from .transforms import Transform, TransformMode, TransformType, group_matrices, spatial_matrices
@@ -137,6 +138,11 @@ class StagePlan:
         return self._dev[key]
 
 
+def _ht_policy(profile: BMNDProfile) -> PoissonHTMatchPolicy:
+    return coerce_enum(profile.poisson_ht_match_policy, PoissonHTMatchPolicy,
+                       "poisson_ht_match_policy")
+
+
 def check_supported(profile: BMNDProfile, resolved) -> None:
     """Raise NotImplementedError for profile options outside the MI355X path."""
     if not resolved.is_poisson:
@@ -146,7 +152,8 @@ def check_supported(profile: BMNDProfile, resolved) -> None:
         raise NotImplementedError("only the 'poisson_deviance' block-matching distance runs on "
                                   "the MI355X path.")
     checks = [
-        (profile.poisson_ht_match_policy is PoissonHTMatchPolicy.FIXED, "poisson_ht_match_policy='fixed'"),
+        (_ht_policy(profile) is not PoissonHTMatchPolicy.CANDIDATE_STANDARDIZED,
+         "poisson_ht_match_policy='fixed' or 'reference_finite_count'"),
         (profile.poisson_wiener_gain_mode is PoissonWienerGainMode.VARIANCE_SCALED,
          "poisson_wiener_gain_mode='variance-scaled'"),
         (profile.poisson_group_mass_conservation is PoissonGroupMassConservation.NONE,
@@ -257,6 +264,15 @@ def exact_std_work(plan: StagePlan, shape3: tuple[int, int, int], sel: torch.Ten
     return {"entries": entries, "union_voxels": union, "flops": flops}
 
 
+def _threshold_options(profile: BMNDProfile) -> tuple:
+    """The options compute_poisson_reference_thresholds takes after the
+    schedule and block, from the profile (bmndalgo.py:273-283)."""
+    return (float(profile.poisson_count_scale), float(profile.ht_match_threshold),
+            float(profile.poisson_ht_match_structure_beta),
+            float(profile.poisson_ht_match_intensity_power),
+            int(profile.poisson_ht_match_moment_max_count))
+
+
 class PoissonPipeline:
     """Two-stage Poisson denoiser on one device.  Buffers are torch tensors on
     `device`; kernels run on the current torch stream."""
@@ -357,6 +373,22 @@ class PoissonPipeline:
                                                        self.count_scale, self.floor, self._stream()))
         return out
 
+    def ht_thresholds(self, vol: torch.Tensor, timer=None) -> torch.Tensor | None:
+        """Per-reference stage-1 thresholds of the "reference_finite_count"
+        policy (bmndalgo.py:269-284), formed on the host from the input's
+        reference patches and uploaded; None under the fixed policy."""
+        if _ht_policy(self.profile) is not PoissonHTMatchPolicy.REFERENCE_FINITE_COUNT:
+            return None
+        t = timer or _NoTimer()
+        tok = t.start("ht.thresholds")
+        plan = self.plans["ht"]
+        thr = compute_poisson_reference_thresholds(
+            vol.cpu().numpy().reshape(self.shape), plan.schedule, plan.block,
+            *_threshold_options(self.profile))
+        out = torch.from_numpy(thr).to(self.device)
+        t.stop(tok)
+        return out
+
     def global_sigma(self, var: torch.Tensor) -> float:
         """Global noise std of the Poisson variance model of `var` (the
         stage-1 variance volume): the sigma maps' "global" value."""
@@ -367,10 +399,13 @@ class PoissonPipeline:
 
     def stage(self, name: str, match_vol: torch.Tensor, noisy: torch.Tensor, var: torch.Tensor,
               fallback: torch.Tensor, timer=None, sigma_fill: float | None = None,
-              pilot: torch.Tensor | None = None) -> StageOutput:
+              pilot: torch.Tensor | None = None,
+              ref_thresholds: torch.Tensor | None = None) -> StageOutput:
         """One stage.  With `sigma_fill` (the global sigma) the stage also
         builds its sigma map, `sigma_fill` where no patch reaches a voxel.
-        The Wiener gains use `pilot` (default: the matched volume)."""
+        The Wiener gains use `pilot` (default: the matched volume).
+        `ref_thresholds` (float32, one per reference) replaces the plan's
+        threshold."""
         L = _lib.lib()
         timer = timer or _NoTimer()
         plan = self.plans[name]
@@ -387,10 +422,18 @@ class PoissonPipeline:
         mode, tmax = self.blockmatch_mode(match_vol)
         self.last_modes[name] = mode
         tok = timer.start(f"{name}.blockmatch")
-        _lib.check(L.bmnd_hip_blockmatch(_ptr(match_vol), shape, block, radius, _ptr(refs), R,
-                                         plan.threshold, plan.max_m, plan.min_m, 1, 1,
-                                         self.count_scale, mode, tmax, _ptr(sel), _ptr(cnt),
-                                         _ptr(acc), stream))
+        if ref_thresholds is None:
+            _lib.check(L.bmnd_hip_blockmatch(_ptr(match_vol), shape, block, radius, _ptr(refs), R,
+                                             plan.threshold, plan.max_m, plan.min_m, 1, 1,
+                                             self.count_scale, mode, tmax, _ptr(sel), _ptr(cnt),
+                                             _ptr(acc), stream))
+        else:
+            if ref_thresholds.numel() != R or ref_thresholds.dtype != torch.float32:
+                raise ValueError(f"ref_thresholds must hold {R} float32 values")
+            _lib.check(L.bmnd_hip_blockmatch_thresholds(
+                _ptr(match_vol), shape, block, radius, _ptr(refs), _ptr(ref_thresholds), R,
+                plan.max_m, plan.min_m, 1, 1, self.count_scale, mode, tmax, _ptr(sel), _ptr(cnt),
+                _ptr(acc), stream))
         timer.stop(tok)
         mats, kai = plan.device_constants(dev)
         kcap = min(self.k, plan.max_m)
@@ -446,7 +489,8 @@ class PoissonPipeline:
         t.stop(tok)
         gsig = self.global_sigma(var1) if sigma_maps else None
         fill = None if gsig is None else float(np.float32(gsig))
-        ht = self.stage("ht", vol, vol, var1, vol, timer, fill)
+        thr = self.ht_thresholds(vol, t)
+        ht = self.stage("ht", vol, vol, var1, vol, timer, fill, ref_thresholds=thr)
         match2, var2 = self._wiener_sources(vol, ht.est, var1, t)
         del var1
         wi = self.stage("wiener", match2, vol, var2, vol, timer, fill, pilot=ht.est)
@@ -599,8 +643,8 @@ class BatchPipeline:
         return [(2, vmax) if is_int and vmax <= 100 else (fallback, 0) for is_int, vmax in f]
 
     def stage(self, name: str, match_vol: torch.Tensor, noisy: torch.Tensor, var: torch.Tensor,
-              fallback: torch.Tensor, timer=None,
-              pilot: torch.Tensor | None = None) -> BatchStageOutput:
+              fallback: torch.Tensor, timer=None, pilot: torch.Tensor | None = None,
+              ref_thresholds: torch.Tensor | None = None) -> BatchStageOutput:
         L = _lib.lib()
         timer = timer or _NoTimer()
         st = self.stages[name]
@@ -632,7 +676,9 @@ class BatchPipeline:
             off = lambda t, n: ctypes.c_void_p(t.data_ptr() + 4 * n)  # noqa: E731
             _lib.check(L.bmnd_hip_blockmatch_batch(
                 _ptr(match_vol), ih, idv, nin, block, radius, off(st.ref_lin, r0),
-                off(st.ref_input, r0), r1 - r0, plan.threshold, plan.max_m, plan.min_m, 1, 1,
+                off(st.ref_input, r0),
+                None if ref_thresholds is None else off(ref_thresholds, r0),
+                r1 - r0, plan.threshold, plan.max_m, plan.min_m, 1, 1,
                 self._single.count_scale, mode, tmax, off(sel, r0 * plan.max_m), off(cnt, r0),
                 off(acc, r0), stream))
         timer.stop(tok)
@@ -671,6 +717,26 @@ class BatchPipeline:
         del filt, wts, ref_index
         return BatchStageOutput(est, sel.view(R, plan.max_m), cnt, acc, st)
 
+    def ht_thresholds(self, packed: torch.Tensor, timer=None) -> torch.Tensor | None:
+        """PoissonPipeline.ht_thresholds for every input, in the flat
+        reference order."""
+        profile = self._single.profile
+        if _ht_policy(profile) is not PoissonHTMatchPolicy.REFERENCE_FINITE_COUNT:
+            return None
+        t = timer or _NoTimer()
+        tok = t.start("ht.thresholds")
+        st = self.stages["ht"]
+        host = packed.cpu().numpy()
+        opts = _threshold_options(profile)
+        thr = np.concatenate([
+            compute_poisson_reference_thresholds(
+                host[self.vox_off[i]:self.vox_off[i + 1]].reshape(self.shapes[i]),
+                st.schedules[i], st.plan.block, *opts)
+            for i in range(len(self.shapes))])
+        out = torch.from_numpy(thr).to(self.device)
+        t.stop(tok)
+        return out
+
     def run(self, packed: torch.Tensor, timer=None) -> tuple[BatchStageOutput, BatchStageOutput]:
         """packed: float32 device buffer of `total_voxels` (see pack())."""
         if (packed.dtype != torch.float32 or not packed.is_contiguous()
@@ -680,7 +746,8 @@ class BatchPipeline:
         tok = t.start("ht.variance")
         var1 = self._single.variance(packed)
         t.stop(tok)
-        ht = self.stage("ht", packed, packed, var1, packed, timer)
+        ht = self.stage("ht", packed, packed, var1, packed, timer,
+                        ref_thresholds=self.ht_thresholds(packed, t))
         match2, var2 = self._single._wiener_sources(packed, ht.est, var1, t)
         del var1
         wi = self.stage("wiener", match2, packed, var2, packed, timer, pilot=ht.est)

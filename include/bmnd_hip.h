@@ -94,6 +94,19 @@ int32_t bmnd_hip_log2_check(double *max_ratio);
  * bmnd_hip_matrix_table_size() gives its length in floats. */
 int64_t bmnd_hip_matrix_table_size(int32_t max_matches, const int32_t *block_h);
 
+/* bmnd_hip_blockmatch with one threshold per reference (ref_thresholds,
+ * device, nrefs float32): the "reference_finite_count" stage-1 policy, whose
+ * thresholds compute_poisson_reference_thresholds forms on the host
+ * (blockmatching.py:95-148, bmndalgo.py:269-284); each reference's tie-grid
+ * key threshold is its threshold on the 32-ulp grid. */
+int32_t bmnd_hip_blockmatch_thresholds(const float *vol, const int64_t *shape_h,
+                                       const int32_t *block_h, const int32_t *radius_h,
+                                       const int32_t *ref_lin, const float *ref_thresholds,
+                                       int64_t nrefs, int32_t max_matches, int32_t min_matches,
+                                       int32_t pow2, int32_t stabilize, double count_scale,
+                                       int32_t mode, int32_t table_max, int32_t *sel,
+                                       int32_t *counts, int32_t *accepted, void *stream);
+
 /* Exact overlap-aware noise std of the first min(k, N) group planes of every
  * group: var[q,s] = sum over union voxels v (C order) of
  * var_v * (sum over entries (i,p) with x_i + p = v of G[q,i]*S[s,p])^2,
@@ -258,16 +271,17 @@ typedef struct {
  * bmnd_hip_input.tile_off: an input has prod(ceil(shape[d] / tile[d])) tiles. */
 int32_t bmnd_hip_aggregate_tile(const int32_t *block_h, int32_t *tile);
 
-/* bmnd_hip_blockmatch over a batch (vol: packed volumes). */
+/* bmnd_hip_blockmatch over a batch (vol: packed volumes); ref_thresholds as
+ * in bmnd_hip_blockmatch_thresholds, or NULL for the scalar threshold. */
 int32_t bmnd_hip_blockmatch_batch(const float *vol, const bmnd_hip_input *inputs_h,
                                   const bmnd_hip_input *inputs_d, int32_t ninputs,
                                   const int32_t *block_h, const int32_t *radius_h,
                                   const int32_t *ref_lin, const int32_t *ref_input,
-                                  int64_t nrefs, float threshold, int32_t max_matches,
-                                  int32_t min_matches, int32_t pow2, int32_t stabilize,
-                                  double count_scale, int32_t mode, int32_t table_max,
-                                  int32_t *sel, int32_t *counts, int32_t *accepted,
-                                  void *stream);
+                                  const float *ref_thresholds, int64_t nrefs, float threshold,
+                                  int32_t max_matches, int32_t min_matches, int32_t pow2,
+                                  int32_t stabilize, double count_scale, int32_t mode,
+                                  int32_t table_max, int32_t *sel, int32_t *counts,
+                                  int32_t *accepted, void *stream);
 
 /* bmnd_hip_exact_std over a batch (var: packed variance volumes). */
 int32_t bmnd_hip_exact_std_batch(const float *var, const bmnd_hip_input *inputs_h,

@@ -527,3 +527,75 @@ def bmnd_poisson(vol: np.ndarray, params: OracleParams | None = None, *, nthread
     if capture_matches:
         return out, pilot, ht, wi
     return out, pilot
+
+
+# ---- per-reference stage-1 thresholds ("reference_finite_count" policy) ----
+# Pure-Python loops in the reference's literal order, for small inputs.
+
+def poisson_null_moments(max_count: int = 512) -> tuple[list, list]:
+    """Mean and variance (float64) of one Poisson-deviance term 2[r log(2r/N)
+    + c log(2c/N)] when (r, c) is a Binomial(N, 1/2) split of a pooled count
+    N, for N = 0..max_count; N = 0 stays 0 (blockmatching.py:29-87)."""
+    import math
+    means, variances = [0.0] * (max_count + 1), [0.0] * (max_count + 1)
+    for n in range(1, max_count + 1):
+        e = s2 = 0.0
+        half_n = 2.0 ** (-n)
+        for r in range(n + 1):
+            c = n - r
+            p = math.comb(n, r) * half_n
+            t = 0.0
+            if r > 0:
+                t += r * math.log(2.0 * r / n)
+            if c > 0:
+                t += c * math.log(2.0 * c / n)
+            t *= 2.0
+            e += p * t
+            s2 += p * t * t
+        means[n] = e
+        variances[n] = max(s2 - e * e, 0.0)
+    return means, variances
+
+
+def poisson_reference_thresholds(vol: np.ndarray, params, ht_match_threshold: float,
+                                 structure_beta: float, intensity_power: float,
+                                 max_count: int = 512) -> np.ndarray:
+    """float32 stage-1 threshold of every stage-1 reference, in schedule
+    order (blockmatching.py:90-148): the mean of the reference patch's null
+    means, plus ht_match_threshold times sqrt(sum of null variances) / P,
+    plus structure_beta times the mean count to intensity_power; pooled
+    counts above max_count take the chi-square(1) moments 1 and 2."""
+    import math
+    v = np.ascontiguousarray(vol, dtype=np.float32)
+    nd = v.ndim
+    block = tuple(params.ht_block[:nd])
+    counts = tuple(int(v.shape[d]) - block[d] + 1 for d in range(nd))
+    refs = schedule(counts, block, tuple(params.ht_step[:nd]), params.shift_density,
+                    params.schedule_density)
+    means, variances = poisson_null_moments(max_count)
+    P = int(np.prod(block))
+    out = np.empty(len(refs), dtype=np.float32)
+    for i, ref in enumerate(refs):
+        patch = v[tuple(slice(int(ref[d]), int(ref[d]) + block[d]) for d in range(nd))]
+        cnt = np.clip(patch.reshape(-1).astype(np.float64) / float(params.count_scale), 0.0, None)
+        nm = np.empty(P)
+        nv = np.empty(P)
+        for j in range(P):
+            pooled = np.rint(2.0 * cnt[j])
+            if pooled > max_count:
+                nm[j], nv[j] = 1.0, 2.0
+            else:
+                nm[j], nv[j] = means[int(pooled)], variances[int(pooled)]
+        thr = (float(np.mean(nm)) + float(ht_match_threshold)
+               * math.sqrt(max(float(np.sum(nv)) / (P * P), 0.0))
+               + float(structure_beta) * float(np.mean(cnt)) ** float(intensity_power))
+        out[i] = np.float32(thr)
+    return out
+
+
+def set_ht_thresholds(params, thresholds: np.ndarray) -> None:
+    """Point the oracle's stage-1 matching at per-reference thresholds
+    (kept alive on the params object)."""
+    arr = np.ascontiguousarray(thresholds, dtype=np.float32)
+    params._thr_keep = arr
+    params.ht_thresholds = arr.ctypes.data

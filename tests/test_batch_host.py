@@ -28,7 +28,7 @@ def _blockmatch(recs, block=(4, 4, 4), radius=(7, 7, 7)):
     i32x3 = ctypes.c_int32 * 3
     dummy = ctypes.c_void_p(16)  # never dereferenced: the records are rejected first
     return L.bmnd_hip_blockmatch_batch(dummy, ctypes.c_void_p(recs.ctypes.data), dummy, len(recs),
-                                       i32x3(*block), i32x3(*radius), dummy, dummy, 10, 1.0, 16,
+                                       i32x3(*block), i32x3(*radius), dummy, dummy, None, 10, 1.0, 16,
                                        2, 1, 1, 1.0, 4, 0, dummy, dummy, dummy, None)
 
 

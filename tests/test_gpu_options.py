@@ -2,13 +2,19 @@
 bit: the Wiener stage matching on the observation instead of the pilot
 (poisson_wiener_match_source, bmndalgo.py:714-720; its counts then use the
 count-table matcher) and taking the observation's variance model
-(poisson_variance_source_wiener, bmndalgo.py:635-646), alone and together."""
+(poisson_variance_source_wiener, bmndalgo.py:635-646), alone and together;
+and the per-reference stage-1 thresholds of poisson_ht_match_policy=
+"reference_finite_count" (bmndalgo.py:269-284) in every matching mode and
+through bmnd_batch."""
 
+import numpy as np
 import pytest
+import torch
 
 import oracle
-from bmnd_amd import BM3DProfile, BM4DProfile, NoiseModel, phantoms
-from bmnd_amd.enums import PoissonVarianceSource
+from bmnd_amd import BM3DProfile, BM4DProfile, NoiseModel, bmnd, bmnd_batch, phantoms
+from bmnd_amd.engine import PoissonPipeline
+from bmnd_amd.enums import PoissonHTMatchPolicy, PoissonVarianceSource
 
 from test_gpu_parity import _compare
 
@@ -40,3 +46,84 @@ def test_wiener_sources_match_oracle(kind, match_src, var_src):
     assert stats["out_bits_differing"] == 0
     if match_src is OBS:
         assert stats["modes"]["wiener"] == 2  # integer counts: the count table
+
+
+def _policy_input(kind):
+    if kind.endswith("volume"):
+        _, noisy = phantoms.ellipsoid_volume((24, 26, 22), seed=73, peak=10.0, n_ellipsoids=6,
+                                             axes_range=(3.0, 10.0))
+        prof = BM4DProfile(noise_model=NoiseModel.POISSON)
+    else:
+        # piecewise-constant rates 10..50: thresholds bind in most windows
+        rng = np.random.default_rng(74)
+        rate = np.kron(rng.integers(1, 6, (8, 8)), np.ones((8, 8))) * 10.0
+        noisy = np.ascontiguousarray(rng.poisson(rate).astype(np.float32)[:57, :64])
+        prof = BM3DProfile(noise_model=NoiseModel.POISSON)
+    if kind.startswith("scaled"):
+        # non-integer counts: the float-pass / certified / exact matchers
+        noisy = (noisy * np.float32(0.37)).astype(np.float32)
+    return noisy, prof
+
+
+def _with_policy(prof, beta, power):
+    prof.poisson_ht_match_policy = PoissonHTMatchPolicy.REFERENCE_FINITE_COUNT
+    prof.poisson_ht_match_structure_beta = beta
+    prof.poisson_ht_match_intensity_power = power
+    return prof
+
+
+@pytest.mark.parametrize("kind,deviance,lam,beta,power", [
+    ("volume", "auto-fast", 2.9527, 0.1, 1.0), ("volume", "auto-fast", 1.0, 0.4, 0.5),
+    ("image", "auto-fast", 2.9527, 0.4, 0.5), ("scaled-image", "auto", 2.9527, 0.1, 1.0),
+    ("scaled-image", "auto-certified", 1.0, 0.1, 1.0),
+    ("scaled-image", "auto-fast", 1.0, 0.4, 0.5), ("scaled-volume", "auto-fast", 0.5, 0.1, 1.0)])
+def test_reference_finite_count_matches_oracle(kind, deviance, lam, beta, power):
+    """Pilot, output and match lists against the oracle pipeline fed the
+    oracle's thresholds; stage-1 accepted counts (every reference's
+    threshold decision over its whole window) against the oracle matcher."""
+    print(f"reference_finite_count: {kind}, {deviance}, lambda {lam}, beta {beta}, "
+          f"power {power}", flush=True)
+    noisy, prof = _policy_input(kind)
+    prof = _with_policy(prof, beta, power)
+    prof.ht_match_threshold = lam
+    params = oracle.default_params(noisy.ndim, noisy.shape)
+    thr = oracle.poisson_reference_thresholds(noisy, params, lam, beta, power)
+    oracle.set_ht_thresholds(params, thr)
+    stats, _, _ = _compare(noisy, prof, params, deviance)
+    assert stats["ht_groups_differing"] == 0
+    assert stats["wiener_groups_differing"] == 0
+    assert stats["pilot_bits_differing"] == 0
+    assert stats["out_bits_differing"] == 0
+    dev = torch.device("cuda", 0)
+    pipe = PoissonPipeline(prof, noisy.shape, dev, deviance=deviance)
+    ht, _ = pipe.run(torch.from_numpy(noisy).to(dev))
+    plan = pipe.plans["ht"]
+    counts = [noisy.shape[d] - plan.block[d] + 1 for d in range(noisy.ndim)]
+    refs = plan.schedule.reference_abs
+    radius = tuple(int(r) for r in prof.ht_search_window)
+    o = oracle.blockmatch(refs, counts, radius, plan.max_m, plan.min_m, thr, volume=noisy,
+                          block=plan.block)
+    acc = ht.accepted.cpu().numpy()
+    assert np.array_equal(acc, o.accepted)
+    assert np.array_equal(ht.counts.cpu().numpy(), o.counts)
+    print(f"accepted counts: min {acc.min()} median {np.median(acc)} max {acc.max()}", flush=True)
+
+
+def test_reference_finite_count_batch_matches_bmnd():
+    """bmnd_batch under the policy: integer and non-integer images of
+    uneven shapes, each equal to bmnd() alone (and one to the oracle)."""
+    prof = _with_policy(BM3DProfile(noise_model=NoiseModel.POISSON), 0.1, 1.0)
+    imgs = []
+    for side, seed, peak in (((40, 45), 75, 5.0), ((64, 33), 76, 30.0), ((19, 50), 77, 10.0)):
+        _, img = phantoms.blob_image(max(max(side), 32), seed=seed, peak=peak)
+        imgs.append(np.ascontiguousarray(img[:side[0], :side[1]]))
+    imgs[2] = (imgs[2] * np.float32(0.37)).astype(np.float32)
+    print("reference_finite_count: bmnd_batch of 3 images", flush=True)
+    outs = bmnd_batch(imgs, prof)
+    for i, x in enumerate(imgs):
+        assert np.array_equal(outs[i].view(np.int32), bmnd(x, prof).view(np.int32)), f"input {i}"
+    params = oracle.default_params(2, imgs[0].shape)
+    oracle.set_ht_thresholds(params, oracle.poisson_reference_thresholds(
+        imgs[0], params, prof.ht_match_threshold, 0.1, 1.0))
+    ref, _ = oracle.bmnd_poisson(imgs[0], params)
+    assert np.array_equal(outs[0].view(np.int32), ref.view(np.int32))
