@@ -25,6 +25,7 @@ EXPORTS = (
     "bmnd_hip_count_range",
     "bmnd_hip_blockmatch",
     "bmnd_hip_blockmatch_thresholds",
+    "bmnd_hip_blockmatch_standardized",
     "bmnd_hip_log2_check",
     "bmnd_hip_matrix_table_size",
     "bmnd_hip_exact_std",
@@ -66,6 +67,9 @@ def _declare(L: ctypes.CDLL) -> None:
     L.bmnd_hip_blockmatch_thresholds.argtypes = [vp, vp, vp, vp, vp, vp, i64, i32, i32, i32, i32,
                                                  f64, i32, i32, vp, vp, vp, vp]
     L.bmnd_hip_blockmatch_thresholds.restype = i32
+    L.bmnd_hip_blockmatch_standardized.argtypes = [vp, vp, vp, vp, vp, i64, f32, i32, i32, i32,
+                                                   i32, f64, i32, vp, i32, vp, vp, vp, vp]
+    L.bmnd_hip_blockmatch_standardized.restype = i32
     L.bmnd_hip_log2_check.argtypes = [vp]
     L.bmnd_hip_log2_check.restype = i32
     L.bmnd_hip_matrix_table_size.argtypes = [i32, vp]
@@ -92,7 +96,8 @@ def _declare(L: ctypes.CDLL) -> None:
     L.bmnd_hip_aggregate_tile.argtypes = [vp, vp]
     L.bmnd_hip_aggregate_tile.restype = i32
     L.bmnd_hip_blockmatch_batch.argtypes = [vp, vp, vp, i32, vp, vp, vp, vp, vp, i64, f32, i32,
-                                            i32, i32, i32, f64, i32, i32, vp, vp, vp, vp]
+                                            i32, i32, i32, f64, i32, i32, vp, i32, vp, vp, vp,
+                                            vp]
     L.bmnd_hip_blockmatch_batch.restype = i32
     L.bmnd_hip_exact_std_batch.argtypes = [vp, vp, vp, i32, vp, vp, vp, vp, vp, i64, i32, vp,
                                            i32, vp, vp]

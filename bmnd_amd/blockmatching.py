@@ -87,17 +87,26 @@ def build_result(schedule, counts: tuple[int, ...], sel: np.ndarray, final: np.n
 @functools.cache
 def poisson_null_moment_table(distance_measure=BlockMatchDistance.POISSON_DEVIANCE,
                               max_count: int = 512) -> tuple[np.ndarray, np.ndarray]:
-    """Mean and variance of one Poisson-deviance term when the reference and
+    """Mean and variance of one distance term when the reference and
     candidate counts are a Binomial(N, 1/2) split of a pooled count N, for N =
     0..max_count, summed in the reference's order in float64
-    (blockmatching.py:29-92; N = 0 stays 0)."""
+    (blockmatching.py:29-87; N = 0 stays 0).  Pearson has the closed form
+    mean 1, variance 2 - 2/N."""
     distance_measure = coerce_enum(distance_measure, BlockMatchDistance, "block-matching distance")
-    if distance_measure is not BlockMatchDistance.POISSON_DEVIANCE:
-        raise NotImplementedError("per-reference thresholds run with 'poisson_deviance' only.")
+    if distance_measure not in (BlockMatchDistance.POISSON_DEVIANCE, BlockMatchDistance.PEARSON,
+                                BlockMatchDistance.ANSCOMBE_SSD):
+        raise ValueError("Poisson null moments require 'poisson_deviance', 'pearson', "
+                         f"or 'anscombe_ssd', got {distance_measure!r}.")
     if max_count < 1:
         raise ValueError(f"max_count must be positive, got {max_count}.")
     means = np.zeros(max_count + 1, dtype=np.float64)
     variances = np.zeros(max_count + 1, dtype=np.float64)
+    if distance_measure is BlockMatchDistance.PEARSON:
+        n = np.arange(1, max_count + 1, dtype=np.float64)
+        means[1:] = 1.0
+        variances[1:] = 2.0 - 2.0 / n
+        return means, variances
+    deviance = distance_measure is BlockMatchDistance.POISSON_DEVIANCE
     for pooled in range(1, max_count + 1):
         expectation = 0.0
         second = 0.0
@@ -105,12 +114,15 @@ def poisson_null_moment_table(distance_measure=BlockMatchDistance.POISSON_DEVIAN
         for r in range(pooled + 1):
             c = pooled - r
             prob = math.comb(pooled, r) * scale
-            term = 0.0
-            if r > 0:
-                term += r * math.log(2.0 * r / pooled)
-            if c > 0:
-                term += c * math.log(2.0 * c / pooled)
-            term *= 2.0
+            if deviance:
+                term = 0.0
+                if r > 0:
+                    term += r * math.log(2.0 * r / pooled)
+                if c > 0:
+                    term += c * math.log(2.0 * c / pooled)
+                term *= 2.0
+            else:  # Anscombe: squared difference of the stabilised counts / 2
+                term = 0.5 * (2.0 * math.sqrt(c + 0.375) - 2.0 * math.sqrt(r + 0.375)) ** 2
             expectation += prob * term
             second += prob * term * term
         means[pooled] = expectation
@@ -131,7 +143,8 @@ def reference_patches(volume: np.ndarray, schedule: ScheduleArrays, block, start
 def compute_poisson_reference_thresholds(volume: np.ndarray, schedule: ScheduleArrays, block,
                                          count_scale: float, ht_match_threshold: float,
                                          structure_beta: float, intensity_power: float,
-                                         max_count: int = 512, chunk: int = 65536) -> np.ndarray:
+                                         max_count: int = 512, chunk: int = 65536,
+                                         distance=BlockMatchDistance.POISSON_DEVIANCE) -> np.ndarray:
     """Per-reference stage-1 match thresholds of the "reference_finite_count"
     policy (blockmatching.py:95-148), vectorised over references: the mean
     of the reference patch's null means plus ht_match_threshold times the
@@ -145,7 +158,7 @@ def compute_poisson_reference_thresholds(volume: np.ndarray, schedule: ScheduleA
         raise ValueError(f"structure_beta must be finite and non-negative, got {structure_beta}.")
     if not np.isfinite(intensity_power) or intensity_power <= 0.0:
         raise ValueError(f"intensity_power must be finite and positive, got {intensity_power}.")
-    means, variances = poisson_null_moment_table(BlockMatchDistance.POISSON_DEVIANCE, max_count)
+    means, variances = poisson_null_moment_table(distance, max_count)
     P = int(np.prod(block))
     out = np.empty(len(schedule), dtype=np.float32)
     for a in range(0, len(schedule), chunk):
@@ -166,3 +179,15 @@ def compute_poisson_reference_thresholds(volume: np.ndarray, schedule: ScheduleA
                + float(structure_beta) * np.power(intensity, float(intensity_power)))
         out[a:b] = thr.astype(np.float32)
     return out
+
+
+def poisson_null_moment_table_f32(distance_measure, max_count: int = 512) -> np.ndarray:
+    """The null table as the candidate-standardized distance reads it: the
+    float32 means for pooled counts 0..max_count, then the float32 variances
+    (blockmatching.py:288-319)."""
+    if isinstance(max_count, bool) or not isinstance(max_count, (int, np.integer)) \
+            or int(max_count) <= 0:
+        raise ValueError(
+            f"poisson_moment_max_count must be a positive integer, got {max_count!r}.")
+    m, v = poisson_null_moment_table(distance_measure, int(max_count))
+    return np.concatenate([m.astype(np.float32), v.astype(np.float32)])

@@ -282,6 +282,12 @@ struct BMArgs {
   const int32_t *ref_input;
   // per-reference thresholds (nullptr: the scalar threshold for all)
   const float *ref_thr;
+  // MODES 5, 6: distance code (1 deviance, 2 Pearson, 3 Anscombe SSD); MODE 6:
+  // float32 null means and variances of one term for pooled counts
+  // 0..null_max (means, then variances)
+  int dcode;
+  const float *null_mom;
+  int null_max;
 };
 
 // A batched launch takes the geometry of reference g's input and moves the
@@ -326,6 +332,30 @@ __device__ __forceinline__ float bm_key(double d, const BMArgs &a, bool &ok) {
     ok = d32 < a.threshold;
   }
   return key;
+}
+
+// MODE 6: the standardized distance is signed and not divided by the patch
+// size (blockmatching.py:374-426); same tie grid and threshold rule
+__device__ __forceinline__ float bm_key_signed(float d32, const BMArgs &a, bool &ok) {
+  float key;
+  if (a.stabilize) {
+    key = canonical_key(d32);
+    ok = key < a.thr_key;
+  } else {
+    key = d32;
+    ok = d32 < a.threshold;
+  }
+  return key;
+}
+
+// (key, candidate) composite of a signed key: the key's bits mapped to an
+// order-preserving unsigned integer in the upper half (the accepted flag is
+// implied: accepted keys are exactly those below the threshold key)
+__device__ __forceinline__ unsigned long long bm_composite_signed(float key, int cand) {
+  if (key == 0.0f) key = 0.0f;  // a signed zero sorts as +0
+  const unsigned b = __float_as_uint(key);
+  const unsigned m = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+  return ((unsigned long long)m << 32) | (unsigned long long)cand;
 }
 
 // one term of the reference expression (blockmatching.py:347-355) added to d
@@ -525,7 +555,12 @@ __device__ __forceinline__ void bm_finalize(int K, int acc, const BMArgs &a, int
 // MODE 1: one log per term, r log 2r + c log 2c - s log s (values clamped
 //         and scaled when the slab is staged);
 // MODE 2: integer counts; the term is read from a float64 table built on the
-//         host with the same expression and the C library's log.
+//         host with the same expression and the C library's log;
+// MODE 5: the Pearson (a.dcode 2) or Anscombe SSD (3) distance as the
+//         reference writes it (blockmatching.py:356-369);
+// MODE 6: the candidate-standardized distance of code a.dcode
+//         (blockmatching.py:374-426): (sum of terms - sum of null means) /
+//         sqrt(sum of null variances), the null sums in float32.
 template <int B0, int B1, int B2, int R0, int R1, int R2, int MODE, int NT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 6 : 1))) void k_blockmatch(
     BMArgs a) {
@@ -540,7 +575,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 
   __shared__ double s_tab[kLogTabDoubles];
   __shared__ int s_acc;
   __shared__ int s_refrow[P];
-  __shared__ double s_lc[MODE == 1 ? SLAB : 1];  // MODE 1: c*log(2c) per slab voxel
+  // MODE 1: c*log(2c) per slab voxel; MODES 5, 6 with Anscombe: 2 sqrt(c + 3/8)
+  __shared__ double s_lc[MODE == 1 || MODE >= 5 ? SLAB : 1];
   __shared__ double2 s_flog[MODE == 1 ? kFastLogN : 1];
   extern __shared__ double s_terms[];  // MODE 2 only
 
@@ -587,6 +623,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 
     double r = (double)v * a.inv_scale;
     if (r < 0.0) r = 0.0;
     s_ref[p] = r;
+    if (MODE >= 5) s_ref_log[p] = 2.0 * sqrt(r + 0.375);  // Anscombe transform
     // MODE 2: byte offset of the reference count's term-table row
     if (MODE == 2) s_refrow[p] = (int)v * (a.tmax + 1) * (int)sizeof(double);
   }
@@ -596,6 +633,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 
   }
   if (tid == 0) s_acc = 0;
   __syncthreads();
+  if (MODE >= 5 && a.dcode == 3) {
+    for (int i = tid; i < SLAB; i += NT) {
+      double c = (double)s_slab[i] * a.inv_scale;
+      if (c < 0.0) c = 0.0;
+      s_lc[i] = 2.0 * sqrt(c + 0.375);
+    }
+    __syncthreads();
+  }
   if (MODE == 1) {
     // x log 2x of every slab voxel and reference voxel (0 at x = 0); the slab
     // keeps max(x, 0) (count_scale is 1 in this mode)
@@ -673,12 +718,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 
     const int j2 = cand % e2, j1 = (cand / e2) % e1, j0 = cand / (e2 * e1);
     const int base = (j0 * S1 + j1) * S2 + j2;
     double d = 0.0;
+    float nms = 0.0f, nvs = 0.0f;  // MODE 6: null mean and variance sums
     // terms are accumulated in patch C order (p0, p1, p2), as the reference does
 #pragma unroll 1
     for (int p01 = 0; p01 < B0 * B1; ++p01) {
       const int p0 = p01 / B1, p1 = p01 - p0 * B1;
       const float *row = s_slab + base + (p0 * S1 + p1) * S2;
-      const double *lcrow = s_lc + (MODE == 1 ? base + (p0 * S1 + p1) * S2 : 0);
+      const double *lcrow =
+          s_lc + (MODE == 1 || MODE >= 5 ? base + (p0 * S1 + p1) * S2 : 0);
       const double *rrow = s_ref + p01 * B2;
       const double *lrow = s_ref_log + p01 * B2;
 #pragma unroll
@@ -688,6 +735,33 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 
           const double s = rrow[p2] + (double)row[p2];
           const double ls = dlog_bits(s, s_flog, ln2k);
           d = d + fma(-s, ls, lrow[p2] + lcrow[p2]);
+        } else if (MODE >= 5) {
+          const double r = rrow[p2];
+          double c = (double)row[p2] * a.inv_scale;
+          if (c < 0.0) c = 0.0;
+          const double comb = r + c;
+          if (MODE == 6 && !(comb > 0.0)) continue;  // no term, no null moments
+          if (a.dcode == 1) {
+            d = bm_reference_term(r, c, d, s_tab);
+          } else if (a.dcode == 2) {
+            if (comb > 0.0) {
+              const double df = c - r;
+              d = d + df * df / comb;
+            }
+          } else {
+            const double df = lcrow[p2] - lrow[p2];
+            d = d + 0.5 * df * df;
+          }
+          if (MODE == 6) {
+            const double pooled = rint(comb);
+            if (pooled <= (double)a.null_max) {
+              nms = nms + a.null_mom[(int)pooled];
+              nvs = nvs + a.null_mom[a.null_max + 1 + (int)pooled];
+            } else {
+              nms = nms + 1.0f;
+              nvs = nvs + 2.0f;
+            }
+          }
         } else {
           double c = (double)row[p2] * a.inv_scale;
           if (c < 0.0) c = 0.0;
@@ -697,9 +771,17 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 
     }
     if (MODE == 1) d = 2.0 * d;  // terms were accumulated without the factor 2
     bool ok;
-    const float key = bm_key<P>(d, a, ok);
+    if constexpr (MODE == 6) {
+      // float32 sqrt, correctly rounded through float64
+      const float dstd =
+          nvs > 0.0f ? (float)((d - (double)nms) / (double)(float)sqrt((double)nvs)) : 0.0f;
+      const float key = bm_key_signed(dstd, a, ok);
+      s_keys[cand] = bm_composite_signed(key, cand);
+    } else {
+      const float key = bm_key<P>(d, a, ok);
+      s_keys[cand] = bm_composite(key, ok, cand);
+    }
     my_acc += ok ? 1 : 0;
-    s_keys[cand] = bm_composite(key, ok, cand);
   }
   wave_atomic_add(&s_acc, my_acc);
   __syncthreads();
@@ -2656,7 +2738,8 @@ static int32_t blockmatch_impl(const float *vol, const int64_t *shape_h, const i
                                int32_t pow2, int32_t stabilize, double count_scale,
                                int32_t mode, int32_t table_max, int32_t *sel, int32_t *counts,
                                int32_t *accepted, void *stream, const BatchRef *b,
-                               const float *ref_thresholds = nullptr) {
+                               const float *ref_thresholds = nullptr,
+                               const float *null_mom = nullptr, int32_t null_max = 0) {
   int c[3];
   int64_t shape1[3];
   int64_t grid_total = 0;
@@ -2676,7 +2759,9 @@ static int32_t blockmatch_impl(const float *vol, const int64_t *shape_h, const i
     return fail(BMND_HIP_EINVAL, "count_scale must be finite and positive");
   if (nrefs <= 0) return 0;
   if (nrefs > INT32_MAX) return fail(BMND_HIP_EINVAL, "too many references");
-  if (mode < 0 || mode > 4) return fail(BMND_HIP_EINVAL, "mode must be 0, 1, 2, 3 or 4");
+  if (mode < 0 || mode > 9) return fail(BMND_HIP_EINVAL, "mode must be in 0..9");
+  if (mode >= 7 && (!null_mom || null_max < 1))
+    return fail(BMND_HIP_EINVAL, "standardized modes need the null-moment table (max count >= 1)");
   if (mode == 2 && (count_scale != 1.0 || table_max < 0 || table_max > kMaxTableCount))
     return fail(BMND_HIP_EINVAL, "table mode needs count_scale == 1 and 0 <= table_max <= 100");
   const double *tab = nullptr;
@@ -2732,6 +2817,11 @@ static int32_t blockmatch_impl(const float *vol, const int64_t *shape_h, const i
   a.batch = b ? b->d : nullptr;
   a.ref_input = b ? b->ref_input : nullptr;
   a.ref_thr = ref_thresholds;
+  // modes 5, 6: Pearson, Anscombe SSD; 7, 8, 9: standardized deviance,
+  // Pearson, Anscombe SSD
+  a.dcode = mode == 5 ? 2 : mode == 6 ? 3 : mode >= 7 ? mode - 6 : 1;
+  a.null_mom = null_mom;
+  a.null_max = null_max;
   hipStream_t s = (hipStream_t)stream;
   dim3 grid((unsigned)nrefs), blk(kBMThreads);
   const size_t dyn = mode == 2 ? sizeof(double) * (size_t)(table_max + 1) * (table_max + 1) : 0;
@@ -2797,6 +2887,12 @@ static int32_t blockmatch_impl(const float *vol, const int64_t *shape_h, const i
     else if (mode == 4)                                                                          \
       hipLaunchKernelGGL((k_blockmatch_fast<b0, b1, b2, r0, r1, r2, kBMThreadsFast>), grid,      \
                          dim3(kBMThreadsFast), 0, s, a);                                         \
+    else if (mode == 5 || mode == 6)                                                             \
+      hipLaunchKernelGGL((k_blockmatch<b0, b1, b2, r0, r1, r2, 5, kBMThreads>), grid, blk, 0, s, \
+                         a);                                                                     \
+    else if (mode >= 7)                                                                          \
+      hipLaunchKernelGGL((k_blockmatch<b0, b1, b2, r0, r1, r2, 6, kBMThreads>), grid, blk, 0, s, \
+                         a);                                                                     \
     else                                                                                         \
       hipLaunchKernelGGL((k_blockmatch<b0, b1, b2, r0, r1, r2, 2, kBMThreads>), grid, blk, dyn,  \
                          s, a);                                                                  \
@@ -2837,6 +2933,21 @@ int32_t bmnd_hip_blockmatch_thresholds(const float *vol, const int64_t *shape_h,
                          accepted, stream, nullptr, ref_thresholds);
 }
 
+int32_t bmnd_hip_blockmatch_standardized(const float *vol, const int64_t *shape_h,
+                                         const int32_t *block_h, const int32_t *radius_h,
+                                         const int32_t *ref_lin, int64_t nrefs, float threshold,
+                                         int32_t max_matches, int32_t min_matches, int32_t pow2,
+                                         int32_t stabilize, double count_scale, int32_t distance,
+                                         const float *null_moments, int32_t null_max,
+                                         int32_t *sel, int32_t *counts, int32_t *accepted,
+                                         void *stream) {
+  if (distance < 1 || distance > 3)
+    return fail(BMND_HIP_EINVAL, "distance must be 1 (deviance), 2 (Pearson) or 3 (Anscombe SSD)");
+  return blockmatch_impl(vol, shape_h, block_h, radius_h, ref_lin, nrefs, threshold, max_matches,
+                         min_matches, pow2, stabilize, count_scale, 6 + distance, 0, sel, counts,
+                         accepted, stream, nullptr, nullptr, null_moments, null_max);
+}
+
 int32_t bmnd_hip_blockmatch_batch(const float *vol, const bmnd_hip_input *inputs_h,
                                   const bmnd_hip_input *inputs_d, int32_t ninputs,
                                   const int32_t *block_h, const int32_t *radius_h,
@@ -2844,13 +2955,14 @@ int32_t bmnd_hip_blockmatch_batch(const float *vol, const bmnd_hip_input *inputs
                                   const float *ref_thresholds, int64_t nrefs, float threshold,
                                   int32_t max_matches, int32_t min_matches, int32_t pow2,
                                   int32_t stabilize, double count_scale, int32_t mode,
-                                  int32_t table_max, int32_t *sel, int32_t *counts,
-                                  int32_t *accepted, void *stream) {
+                                  int32_t table_max, const float *null_moments, int32_t null_max,
+                                  int32_t *sel, int32_t *counts, int32_t *accepted,
+                                  void *stream) {
   if (!ref_input) return fail(BMND_HIP_EINVAL, "a batch needs ref_input");
   const BatchRef b{inputs_h, inputs_d, ninputs, ref_input};
   return blockmatch_impl(vol, nullptr, block_h, radius_h, ref_lin, nrefs, threshold, max_matches,
                          min_matches, pow2, stabilize, count_scale, mode, table_max, sel, counts,
-                         accepted, stream, &b, ref_thresholds);
+                         accepted, stream, &b, ref_thresholds, null_moments, null_max);
 }
 
 int64_t bmnd_hip_matrix_table_size(int32_t max_matches, const int32_t *block_h) {

@@ -19,13 +19,18 @@ from .enums import (
     AggregationWeightModel,
     AggregationWeightScope,
     BlockMatchDistance,
+    NoiseModel,
     PoissonGroupMassConservation,
     PoissonHTMatchPolicy,
     PoissonVarianceSource,
     PoissonWienerGainMode,
     coerce_enum,
 )
-from .blockmatching import compute_poisson_reference_thresholds
+from .blockmatching import (
+    compute_poisson_reference_thresholds,
+    poisson_null_moment_table_f32,
+    resolve_blockmatch_distance,
+)
 from .profiles import BMNDProfile
 from .shifts import ScheduleArrays, get_reference_schedule_arrays
 from .transforms import Transform, TransformMode, TransformType, group_matrices, spatial_matrices
@@ -138,6 +143,11 @@ class StagePlan:
         return self._dev[key]
 
 
+# reference distance codes (blockmatching.py:13-26) of the Poisson distances
+_DISTANCE_CODES = {BlockMatchDistance.POISSON_DEVIANCE: 1, BlockMatchDistance.PEARSON: 2,
+                   BlockMatchDistance.ANSCOMBE_SSD: 3}
+
+
 def _ht_policy(profile: BMNDProfile) -> PoissonHTMatchPolicy:
     return coerce_enum(profile.poisson_ht_match_policy, PoissonHTMatchPolicy,
                        "poisson_ht_match_policy")
@@ -148,26 +158,27 @@ def check_supported(profile: BMNDProfile, resolved) -> None:
     if not resolved.is_poisson:
         raise NotImplementedError("bmnd_amd implements the Poisson noise model only "
                                   "(noise_model='poisson').")
-    if resolved.blockmatch_distance is not BlockMatchDistance.POISSON_DEVIANCE:
-        raise NotImplementedError("only the 'poisson_deviance' block-matching distance runs on "
-                                  "the MI355X path.")
+    if resolved.blockmatch_distance not in _DISTANCE_CODES:
+        raise NotImplementedError("the 'poisson_deviance', 'pearson' and 'anscombe_ssd' "
+                                  "block-matching distances run on the MI355X path.")
+    def field(name, enum):
+        return coerce_enum(getattr(profile, name), enum, name)
+
     checks = [
-        (_ht_policy(profile) is not PoissonHTMatchPolicy.CANDIDATE_STANDARDIZED,
-         "poisson_ht_match_policy='fixed' or 'reference_finite_count'"),
-        (profile.poisson_wiener_gain_mode is PoissonWienerGainMode.VARIANCE_SCALED,
-         "poisson_wiener_gain_mode='variance-scaled'"),
-        (profile.poisson_group_mass_conservation is PoissonGroupMassConservation.NONE,
-         "poisson_group_mass_conservation='none'"),
+        (field("poisson_wiener_gain_mode", PoissonWienerGainMode)
+         is PoissonWienerGainMode.VARIANCE_SCALED, "poisson_wiener_gain_mode='variance-scaled'"),
+        (field("poisson_group_mass_conservation", PoissonGroupMassConservation)
+         is PoissonGroupMassConservation.NONE, "poisson_group_mass_conservation='none'"),
         (not profile.ht_use_soft_thresholding, "ht_use_soft_thresholding=False"),
     ]
     for stage in ("ht", "wiener"):
         checks += [
-            (getattr(profile, f"{stage}_weight_model") is AggregationWeightModel.VARIANCE,
-             f"{stage}_weight_model='variance'"),
-            (getattr(profile, f"{stage}_weight_domain") is AggregationWeightDomain.COEFFICIENT,
-             f"{stage}_weight_domain='coefficient'"),
-            (getattr(profile, f"{stage}_weight_scope") is AggregationWeightScope.PATCH,
-             f"{stage}_weight_scope='patch'"),
+            (field(f"{stage}_weight_model", AggregationWeightModel)
+             is AggregationWeightModel.VARIANCE, f"{stage}_weight_model='variance'"),
+            (field(f"{stage}_weight_domain", AggregationWeightDomain)
+             is AggregationWeightDomain.COEFFICIENT, f"{stage}_weight_domain='coefficient'"),
+            (field(f"{stage}_weight_scope", AggregationWeightScope)
+             is AggregationWeightScope.PATCH, f"{stage}_weight_scope='patch'"),
         ]
     for ok, what in checks:
         if not ok:
@@ -308,6 +319,10 @@ class PoissonPipeline:
         self.last_modes: dict[str, int] = {}
         self.count_scale = float(profile.poisson_count_scale)
         self.floor = float(profile.poisson_variance_floor)
+        self.distance = resolve_blockmatch_distance(profile.blockmatch_distance, NoiseModel.POISSON)
+        # candidate-standardized stage-1 distance (bmndalgo.py:311-316)
+        self.standardized = _ht_policy(profile) is PoissonHTMatchPolicy.CANDIDATE_STANDARDIZED
+        self._null = None
         self.k = int(profile.k)
         self.plans = {s: make_plan(profile, s, self.ndim) for s in ("ht", "wiener")}
         for plan in self.plans.values():
@@ -345,8 +360,24 @@ class PoissonPipeline:
             self._refs[plan.name] = torch.from_numpy(lin.astype(np.int32)).to(self.device)
         return self._refs[plan.name]
 
-    def blockmatch_mode(self, vol: torch.Tensor) -> tuple[int, int]:
-        """(kernel mode, table max) for matching on `vol`."""
+    def null_moments(self) -> tuple[torch.Tensor, int]:
+        """Device null table of the candidate-standardized distance and its
+        max pooled count."""
+        if self._null is None:
+            mc = self.profile.poisson_ht_match_moment_max_count
+            tab = poisson_null_moment_table_f32(self.distance, mc)
+            self._null = (torch.from_numpy(tab).to(self.device), int(mc))
+        return self._null
+
+    def blockmatch_mode(self, vol: torch.Tensor, name: str = "wiener") -> tuple[int, int]:
+        """(kernel mode, table max) for matching stage `name` on `vol`: the
+        deviance modes below, 5 / 6 for the Pearson / Anscombe SSD distances,
+        7-9 for the standardized stage-1 distances."""
+        code = _DISTANCE_CODES[self.distance]
+        if name == "ht" and self.standardized:
+            return 6 + code, 0
+        if code != 1:
+            return 3 + code, 0
         if self.deviance == "exact":
             return 0, 0
         if self.deviance == "one-log":
@@ -384,7 +415,7 @@ class PoissonPipeline:
         plan = self.plans["ht"]
         thr = compute_poisson_reference_thresholds(
             vol.cpu().numpy().reshape(self.shape), plan.schedule, plan.block,
-            *_threshold_options(self.profile))
+            *_threshold_options(self.profile), distance=self.distance)
         out = torch.from_numpy(thr).to(self.device)
         t.stop(tok)
         return out
@@ -419,10 +450,16 @@ class PoissonPipeline:
         sel = torch.empty(R * plan.max_m, dtype=torch.int32, device=dev)
         cnt = torch.empty(R, dtype=torch.int32, device=dev)
         acc = torch.empty(R, dtype=torch.int32, device=dev)
-        mode, tmax = self.blockmatch_mode(match_vol)
+        mode, tmax = self.blockmatch_mode(match_vol, name)
         self.last_modes[name] = mode
         tok = timer.start(f"{name}.blockmatch")
-        if ref_thresholds is None:
+        if mode >= 7:
+            null, null_max = self.null_moments()
+            _lib.check(L.bmnd_hip_blockmatch_standardized(
+                _ptr(match_vol), shape, block, radius, _ptr(refs), R, plan.threshold, plan.max_m,
+                plan.min_m, 1, 1, self.count_scale, mode - 6, _ptr(null), null_max, _ptr(sel),
+                _ptr(cnt), _ptr(acc), stream))
+        elif ref_thresholds is None:
             _lib.check(L.bmnd_hip_blockmatch(_ptr(match_vol), shape, block, radius, _ptr(refs), R,
                                              plan.threshold, plan.max_m, plan.min_m, 1, 1,
                                              self.count_scale, mode, tmax, _ptr(sel), _ptr(cnt),
@@ -620,11 +657,14 @@ class BatchPipeline:
         return [host[self.vox_off[i]:self.vox_off[i + 1]].reshape(self.shapes[i]).copy()
                 for i in range(len(self.shapes))]
 
-    def input_modes(self, vol: torch.Tensor) -> list[tuple[int, int]]:
+    def input_modes(self, vol: torch.Tensor, name: str = "wiener") -> list[tuple[int, int]]:
         """(matching mode, table max) of every input of the packed `vol`, as
         PoissonPipeline.blockmatch_mode chooses them for the input alone."""
         single = self._single
         n = len(self.shapes)
+        code = _DISTANCE_CODES[single.distance]
+        if code != 1 or (name == "ht" and single.standardized):
+            return [single.blockmatch_mode(vol, name)] * n
         fixed = {"exact": 0, "one-log": 1, "certified": 3, "fast": 4}
         if single.deviance in fixed:
             return [(fixed[single.deviance], 0)] * n
@@ -659,7 +699,7 @@ class BatchPipeline:
         cnt = torch.empty(R, dtype=torch.int32, device=dev)
         acc = torch.empty(R, dtype=torch.int32, device=dev)
         # one launch per run of consecutive inputs that match in the same mode
-        modes = self.input_modes(match_vol)
+        modes = self.input_modes(match_vol, name)
         runs = []
         for i, (mode, tmax) in enumerate(modes):
             if runs and runs[-1][0] == mode:
@@ -668,6 +708,7 @@ class BatchPipeline:
             else:
                 runs.append([mode, tmax, i, i + 1])
         self.last_runs[name] = [(m, a, b) for m, _, a, b in runs]
+        null, null_max = self._single.null_moments() if runs[0][0] >= 7 else (None, 0)
         tok = timer.start(f"{name}.blockmatch")
         for mode, tmax, a, b in runs:
             r0, r1 = int(st.ref_off[a]), int(st.ref_off[b])
@@ -679,8 +720,8 @@ class BatchPipeline:
                 off(st.ref_input, r0),
                 None if ref_thresholds is None else off(ref_thresholds, r0),
                 r1 - r0, plan.threshold, plan.max_m, plan.min_m, 1, 1,
-                self._single.count_scale, mode, tmax, off(sel, r0 * plan.max_m), off(cnt, r0),
-                off(acc, r0), stream))
+                self._single.count_scale, mode, tmax, _ptr(null), null_max,
+                off(sel, r0 * plan.max_m), off(cnt, r0), off(acc, r0), stream))
         timer.stop(tok)
         mats, kai = plan.device_constants(dev)
         k = self._single.k
@@ -731,7 +772,7 @@ class BatchPipeline:
         thr = np.concatenate([
             compute_poisson_reference_thresholds(
                 host[self.vox_off[i]:self.vox_off[i + 1]].reshape(self.shapes[i]),
-                st.schedules[i], st.plan.block, *opts)
+                st.schedules[i], st.plan.block, *opts, distance=self._single.distance)
             for i in range(len(self.shapes))])
         out = torch.from_numpy(thr).to(self.device)
         t.stop(tok)

@@ -70,7 +70,10 @@ int32_t bmnd_hip_count_range(const float *vol, int64_t n, int32_t *flags,
  * for the candidates that can change the selection or the accepted count,
  * so its output equals mode 0's too.  Modes 1, 3 and 4 fall back to mode 0
  * when count_scale != 1; mode 4 falls back to mode 3 on a device whose
- * hardware log2 is not within the bound bmnd_hip_log2_check measures. */
+ * hardware log2 is not within the bound bmnd_hip_log2_check measures.
+ * Modes 5 and 6 match with the Pearson and the Anscombe SSD distances
+ * (blockmatching.py:356-369), evaluated as the reference writes them in
+ * float64.  Modes 7-9 (standardized) need bmnd_hip_blockmatch_standardized. */
 int32_t bmnd_hip_blockmatch(const float *vol, const int64_t *shape_h,
                             const int32_t *block_h, const int32_t *radius_h,
                             const int32_t *ref_lin, int64_t nrefs,
@@ -106,6 +109,25 @@ int32_t bmnd_hip_blockmatch_thresholds(const float *vol, const int64_t *shape_h,
                                        int32_t pow2, int32_t stabilize, double count_scale,
                                        int32_t mode, int32_t table_max, int32_t *sel,
                                        int32_t *counts, int32_t *accepted, void *stream);
+
+/* bmnd_hip_blockmatch with the candidate-standardized distance of the
+ * "candidate_standardized" stage-1 policy (blockmatching.py:374-426,
+ * bmndalgo.py:311-316): per candidate (sum of the distance's terms - sum of
+ * the null means) / sqrt(sum of the null variances) over the patch voxels of
+ * positive pooled count, null sums in float32, signed and not divided by the
+ * patch size.  distance: 1 deviance, 2 Pearson, 3 Anscombe SSD.
+ * null_moments (device float32, 2 * (null_max + 1)): the null means for pooled
+ * counts 0..null_max, then the null variances (poisson_null_moment_table cast
+ * to float32, blockmatching.py:288-319); pooled counts above null_max take 1
+ * and 2. */
+int32_t bmnd_hip_blockmatch_standardized(const float *vol, const int64_t *shape_h,
+                                         const int32_t *block_h, const int32_t *radius_h,
+                                         const int32_t *ref_lin, int64_t nrefs, float threshold,
+                                         int32_t max_matches, int32_t min_matches, int32_t pow2,
+                                         int32_t stabilize, double count_scale, int32_t distance,
+                                         const float *null_moments, int32_t null_max,
+                                         int32_t *sel, int32_t *counts, int32_t *accepted,
+                                         void *stream);
 
 /* Exact overlap-aware noise std of the first min(k, N) group planes of every
  * group: var[q,s] = sum over union voxels v (C order) of
@@ -272,7 +294,9 @@ typedef struct {
 int32_t bmnd_hip_aggregate_tile(const int32_t *block_h, int32_t *tile);
 
 /* bmnd_hip_blockmatch over a batch (vol: packed volumes); ref_thresholds as
- * in bmnd_hip_blockmatch_thresholds, or NULL for the scalar threshold. */
+ * in bmnd_hip_blockmatch_thresholds, or NULL for the scalar threshold; modes
+ * 7-9 are the standardized distances 1-3 of bmnd_hip_blockmatch_standardized
+ * with its null_moments / null_max (else NULL / 0). */
 int32_t bmnd_hip_blockmatch_batch(const float *vol, const bmnd_hip_input *inputs_h,
                                   const bmnd_hip_input *inputs_d, int32_t ninputs,
                                   const int32_t *block_h, const int32_t *radius_h,
@@ -280,8 +304,9 @@ int32_t bmnd_hip_blockmatch_batch(const float *vol, const bmnd_hip_input *inputs
                                   const float *ref_thresholds, int64_t nrefs, float threshold,
                                   int32_t max_matches, int32_t min_matches, int32_t pow2,
                                   int32_t stabilize, double count_scale, int32_t mode,
-                                  int32_t table_max, int32_t *sel, int32_t *counts,
-                                  int32_t *accepted, void *stream);
+                                  int32_t table_max, const float *null_moments, int32_t null_max,
+                                  int32_t *sel, int32_t *counts, int32_t *accepted,
+                                  void *stream);
 
 /* bmnd_hip_exact_std over a batch (var: packed variance volumes). */
 int32_t bmnd_hip_exact_std_batch(const float *var, const bmnd_hip_input *inputs_h,

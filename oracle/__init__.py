@@ -62,6 +62,11 @@ class OracleParams(ctypes.Structure):
         ("wi_match_observation", ctypes.c_int),
         ("wi_var_observation", ctypes.c_int),
         ("ht_thresholds", ctypes.c_void_p),
+        ("distance", ctypes.c_int),
+        ("ht_standardize", ctypes.c_int),
+        ("null_means", ctypes.c_void_p),
+        ("null_vars", ctypes.c_void_p),
+        ("null_max", ctypes.c_int),
     ]
 
 
@@ -96,6 +101,11 @@ def _declare(L: ctypes.CDLL) -> None:
     L.oracle_patch_distance.restype = f32
     L.oracle_blockmatch.argtypes = [vp, vp, i32, vp, vp, vp, vp, i64, vp, vp,
                                     i32, i32, i32, i32, i32, f64, i32, vp, vp, vp]
+    L.oracle_standardized_distance.argtypes = [vp, vp, i32, i32, f64, vp, vp, i32]
+    L.oracle_standardized_distance.restype = f32
+    L.oracle_blockmatch_standardized.argtypes = [vp, vp, i32, vp, vp, vp, vp, i64, vp, vp,
+                                                 i32, i32, i32, i32, i32, f64, vp, vp, i32, i32,
+                                                 vp, vp, vp]
     L.oracle_apply_transform.argtypes = [vp, i32, vp, vp]
     L.oracle_exact_variance.argtypes = [vp, i32, vp, vp, i32, vp, vp, vp, i32, vp]
     L.oracle_exact_variance.restype = i32
@@ -194,7 +204,11 @@ class MatchResult:
 
 def blockmatch(refs, counts, radius, max_matches, min_matches, threshold, *,
                patches_flat=None, volume=None, block=None, pow2=True, stabilize=True,
-               code=DIST_DEVIANCE, count_scale=1.0, nthreads=0) -> MatchResult:
+               code=DIST_DEVIANCE, count_scale=1.0, nthreads=0,
+               null_moments=None) -> MatchResult:
+    """Stage matching of every reference (blockmatching.py:1045-1205); with
+    null_moments = (float32 means, float32 variances) the candidate-
+    standardized distance."""
     refs = np.ascontiguousarray(np.asarray(refs, dtype=np.int64).reshape(-1, len(counts)))
     nrefs, ndim = refs.shape
     cts = np.asarray(counts, dtype=np.int64)
@@ -216,10 +230,18 @@ def blockmatch(refs, counts, radius, max_matches, min_matches, threshold, *,
     sel = np.empty((nrefs, max_matches), dtype=np.int64)
     cnt = np.empty(nrefs, dtype=np.int32)
     acc = np.empty(nrefs, dtype=np.int64)
-    lib().oracle_blockmatch(_p(pf), _p(vol), ndim, _p(vshape), _p(cts), _p(blk), _p(refs),
-                            nrefs, _p(rad), _p(thr), max_matches, min_matches, int(pow2),
-                            int(stabilize), code, float(count_scale), nthreads, _p(sel),
-                            _p(cnt), _p(acc))
+    if null_moments is None:
+        lib().oracle_blockmatch(_p(pf), _p(vol), ndim, _p(vshape), _p(cts), _p(blk), _p(refs),
+                                nrefs, _p(rad), _p(thr), max_matches, min_matches, int(pow2),
+                                int(stabilize), code, float(count_scale), nthreads, _p(sel),
+                                _p(cnt), _p(acc))
+    else:
+        nm = np.ascontiguousarray(null_moments[0], dtype=np.float32)
+        nv = np.ascontiguousarray(null_moments[1], dtype=np.float32)
+        lib().oracle_blockmatch_standardized(
+            _p(pf), _p(vol), ndim, _p(vshape), _p(cts), _p(blk), _p(refs), nrefs, _p(rad),
+            _p(thr), max_matches, min_matches, int(pow2), int(stabilize), code,
+            float(count_scale), _p(nm), _p(nv), len(nm) - 1, nthreads, _p(sel), _p(cnt), _p(acc))
     return MatchResult(sel, cnt, acc)
 
 
@@ -373,6 +395,7 @@ def default_params(ndim: int, shape, nthreads: int = 0) -> OracleParams:
     p.k = 4
     p.count_scale = 1.0
     p.variance_floor = 1e-10
+    p.distance = DIST_DEVIANCE
     p.nthreads = nthreads
     p._keep = (reference_spatial_matrices("bior", hb), reference_spatial_matrices("dct", wb))
     p.ht_spatial = p._keep[0].ctypes.data
@@ -431,7 +454,7 @@ def default_params_f64(ndim: int, shape, nthreads: int = 0) -> OracleParams64:
     p32 = default_params(ndim, shape, nthreads)
     p = OracleParams64()
     for name, _ in OracleParams._fields_:
-        if name not in ("ht_spatial", "wi_spatial", "ht_thresholds"):
+        if name not in ("ht_spatial", "wi_spatial", "ht_thresholds", "null_means", "null_vars"):
             setattr(p, name, getattr(p32, name))
     hb = tuple(p32.ht_block[d] for d in range(ndim))
     wb = tuple(p32.wi_block[d] for d in range(ndim))
@@ -532,24 +555,33 @@ def bmnd_poisson(vol: np.ndarray, params: OracleParams | None = None, *, nthread
 # ---- per-reference stage-1 thresholds ("reference_finite_count" policy) ----
 # Pure-Python loops in the reference's literal order, for small inputs.
 
-def poisson_null_moments(max_count: int = 512) -> tuple[list, list]:
-    """Mean and variance (float64) of one Poisson-deviance term 2[r log(2r/N)
-    + c log(2c/N)] when (r, c) is a Binomial(N, 1/2) split of a pooled count
-    N, for N = 0..max_count; N = 0 stays 0 (blockmatching.py:29-87)."""
+def poisson_null_moments(max_count: int = 512, code: int = DIST_DEVIANCE) -> tuple[list, list]:
+    """Mean and variance (float64) of one distance term when (r, c) is a
+    Binomial(N, 1/2) split of a pooled count N, for N = 0..max_count; N = 0
+    stays 0 (blockmatching.py:29-87).  Deviance term 2[r log(2r/N) +
+    c log(2c/N)]; Anscombe term (2 sqrt(c + 3/8) - 2 sqrt(r + 3/8))^2 / 2;
+    Pearson: mean 1 and variance 2 - 2/N in closed form."""
     import math
     means, variances = [0.0] * (max_count + 1), [0.0] * (max_count + 1)
+    if code == DIST_PEARSON:
+        for n in range(1, max_count + 1):
+            means[n], variances[n] = 1.0, 2.0 - 2.0 / float(n)
+        return means, variances
     for n in range(1, max_count + 1):
         e = s2 = 0.0
         half_n = 2.0 ** (-n)
         for r in range(n + 1):
             c = n - r
             p = math.comb(n, r) * half_n
-            t = 0.0
-            if r > 0:
-                t += r * math.log(2.0 * r / n)
-            if c > 0:
-                t += c * math.log(2.0 * c / n)
-            t *= 2.0
+            if code == DIST_DEVIANCE:
+                t = 0.0
+                if r > 0:
+                    t += r * math.log(2.0 * r / n)
+                if c > 0:
+                    t += c * math.log(2.0 * c / n)
+                t *= 2.0
+            else:
+                t = 0.5 * (2.0 * math.sqrt(c + 0.375) - 2.0 * math.sqrt(r + 0.375)) ** 2
             e += p * t
             s2 += p * t * t
         means[n] = e
@@ -572,7 +604,7 @@ def poisson_reference_thresholds(vol: np.ndarray, params, ht_match_threshold: fl
     counts = tuple(int(v.shape[d]) - block[d] + 1 for d in range(nd))
     refs = schedule(counts, block, tuple(params.ht_step[:nd]), params.shift_density,
                     params.schedule_density)
-    means, variances = poisson_null_moments(max_count)
+    means, variances = poisson_null_moments(max_count, params.distance or DIST_DEVIANCE)
     P = int(np.prod(block))
     out = np.empty(len(refs), dtype=np.float32)
     for i, ref in enumerate(refs):
@@ -599,3 +631,15 @@ def set_ht_thresholds(params, thresholds: np.ndarray) -> None:
     arr = np.ascontiguousarray(thresholds, dtype=np.float32)
     params._thr_keep = arr
     params.ht_thresholds = arr.ctypes.data
+
+
+def set_ht_standardized(params, max_count: int = 512) -> None:
+    """Stage-1 matching with the candidate-standardized form of
+    params.distance, its null tables cast to float32 as the reference does
+    (blockmatching.py:288-319)."""
+    m, v = poisson_null_moments(max_count, params.distance or DIST_DEVIANCE)
+    params._null_keep = (np.asarray(m, dtype=np.float32), np.asarray(v, dtype=np.float32))
+    params.ht_standardize = 1
+    params.null_means = params._null_keep[0].ctypes.data
+    params.null_vars = params._null_keep[1].ctypes.data
+    params.null_max = int(max_count)

@@ -401,6 +401,49 @@ float oracle_patch_distance(const float *ref, const float *cand, int p,
   return (float)(dist / p);
 }
 
+/* Candidate-standardized distance, blockmatching.py:374-426: over the patch
+ * voxels of positive pooled count r + c, (sum of the distance's terms - sum
+ * of the null means of round(r + c)) / sqrt(sum of the null variances); the
+ * null sums are float32, pooled counts above null_max take the chi-square(1)
+ * moments 1 and 2; 0 when the variance sum is not positive. */
+float oracle_standardized_distance(const float *ref, const float *cand, int p, int code,
+                                   double count_scale, const float *null_means,
+                                   const float *null_vars, int null_max) {
+  double inv = 1.0 / count_scale;
+  double dist = 0.0;
+  float nm = 0.0f, nv = 0.0f;
+  for (int i = 0; i < p; ++i) {
+    double r = (double)ref[i] * inv;
+    if (r < 0.0) r = 0.0;
+    double c = (double)cand[i] * inv;
+    if (c < 0.0) c = 0.0;
+    double comb = r + c;
+    if (comb <= 0.0) continue;
+    if (code == ORACLE_DIST_DEVIANCE) {
+      double t = 0.0;
+      if (r > 0.0) t += r * log(2.0 * r / comb);
+      if (c > 0.0) t += c * log(2.0 * c / comb);
+      dist += 2.0 * t;
+    } else if (code == ORACLE_DIST_PEARSON) {
+      double diff = c - r;
+      dist += diff * diff / comb;
+    } else {
+      double diff = 2.0 * sqrt(c + 0.375) - 2.0 * sqrt(r + 0.375);
+      dist += 0.5 * diff * diff;
+    }
+    double pooled = rint(comb);
+    if (pooled <= (double)null_max) {
+      nm += null_means[(int)pooled];
+      nv += null_vars[(int)pooled];
+    } else {
+      nm += 1.0f;
+      nv += 2.0f;
+    }
+  }
+  if (nv <= 0.0f) return 0.0f;
+  return (float)((dist - (double)nm) / (double)sqrtf(nv));
+}
+
 static int precedes(int64_t cl, float cd, int64_t el, float ed, int stabilize) {
   if (stabilize) {
     cd = oracle_canonical_key(cd);
@@ -478,6 +521,23 @@ void oracle_blockmatch(const float *patches_flat, const float *volume,
                        int min_matches, int pow2, int stabilize, int code,
                        double count_scale, int nthreads, int64_t *sel_out,
                        int32_t *count_out, int64_t *accepted_out) {
+  oracle_blockmatch_standardized(patches_flat, volume, ndim, vol_shape, counts, block, refs,
+                                 nrefs, radius, thresholds, max_matches, min_matches, pow2,
+                                 stabilize, code, count_scale, NULL, NULL, 0, nthreads, sel_out,
+                                 count_out, accepted_out);
+}
+
+/* oracle_blockmatch; with null_means the candidate-standardized distance
+ * (blockmatching.py:1128-1145) */
+void oracle_blockmatch_standardized(const float *patches_flat, const float *volume, int ndim,
+                                    const int64_t *vol_shape, const int64_t *counts,
+                                    const int *block, const int64_t *refs, int64_t nrefs,
+                                    const int *radius, const float *thresholds, int max_matches,
+                                    int min_matches, int pow2, int stabilize, int code,
+                                    double count_scale, const float *null_means,
+                                    const float *null_vars, int null_max, int nthreads,
+                                    int64_t *sel_out, int32_t *count_out,
+                                    int64_t *accepted_out) {
   int P = (int)prod_int(block, ndim);
   int64_t cstr[ORACLE_MAX_NDIM], vstr[ORACLE_MAX_NDIM];
   c_strides(ndim, counts, cstr);
@@ -551,7 +611,9 @@ void oracle_blockmatch(const float *patches_flat, const float *volume,
           for (int p = 0; p < P; ++p) candbuf[p] = volume[base + poff[p]];
           cp = candbuf;
         }
-        float dist = oracle_patch_distance(refp, cp, P, code, count_scale);
+        float dist = null_means ? oracle_standardized_distance(refp, cp, P, code, count_scale,
+                                                               null_means, null_vars, null_max)
+                                : oracle_patch_distance(refp, cp, P, code, count_scale);
         int ok = stab ? (oracle_canonical_key(dist) < thr_key) : (dist < thr);
         if (ok) {
           ++accepted;
@@ -1139,9 +1201,12 @@ static int run_stage(const oracle_params *prm, int wiener, const float *match_vo
     thrs[i] = (!wiener && prm->ht_thresholds) ? prm->ht_thresholds[i] : thr;
   int64_t *sel = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nrefs * max_m + 1));
   int32_t *cnt = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nrefs + 1));
-  oracle_blockmatch(NULL, match_vol, ndim, g.shape, g.counts, block, refs, nrefs, radius,
-                    thrs, max_m, min_m, 1, 1, ORACLE_DIST_DEVIANCE, prm->count_scale,
-                    prm->nthreads, sel, cnt, NULL);
+  const int code = prm->distance ? prm->distance : ORACLE_DIST_DEVIANCE;
+  const int std_ht = !wiener && prm->ht_standardize;
+  oracle_blockmatch_standardized(NULL, match_vol, ndim, g.shape, g.counts, block, refs, nrefs,
+                                 radius, thrs, max_m, min_m, 1, 1, code, prm->count_scale,
+                                 std_ht ? prm->null_means : NULL, std_ht ? prm->null_vars : NULL,
+                                 prm->null_max, prm->nthreads, sel, cnt, NULL);
   if (sel_dump && cnt_dump) {
     int64_t lim = nrefs < dump_cap ? nrefs : dump_cap;
     memcpy(sel_dump, sel, sizeof(int64_t) * (size_t)(lim * max_m));

@@ -51,6 +51,19 @@ void oracle_blockmatch(const float *patches_flat, const float *volume,
                        double count_scale, int nthreads, int64_t *sel_out,
                        int32_t *count_out, int64_t *accepted_out);
 
+float oracle_standardized_distance(const float *ref, const float *cand, int p, int code,
+                                   double count_scale, const float *null_means,
+                                   const float *null_vars, int null_max);
+void oracle_blockmatch_standardized(const float *patches_flat, const float *volume, int ndim,
+                                    const int64_t *vol_shape, const int64_t *counts,
+                                    const int *block, const int64_t *refs, int64_t nrefs,
+                                    const int *radius, const float *thresholds, int max_matches,
+                                    int min_matches, int pow2, int stabilize, int code,
+                                    double count_scale, const float *null_means,
+                                    const float *null_vars, int null_max, int nthreads,
+                                    int64_t *sel_out, int32_t *count_out,
+                                    int64_t *accepted_out);
+
 /* ---- group statistics --------------------------------------------------- */
 void oracle_apply_transform(float *data, int ndim_total, const int *dims,
                             const float *const *mats);
@@ -123,6 +136,13 @@ typedef struct {
   int wi_match_observation;
   int wi_var_observation;
   const float *ht_thresholds;
+  /* distance of both stages (ORACLE_DIST_*; 0 means the deviance) and the
+   * candidate-standardized stage-1 distance with its float32 null tables */
+  int distance;
+  int ht_standardize;
+  const float *null_means;
+  const float *null_vars;
+  int null_max;
 } oracle_params;
 
 /* Runs both stages.  pilot may be NULL.  The match lists of each stage can

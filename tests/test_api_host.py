@@ -91,7 +91,7 @@ def test_all_zero_volume_fast_path_runs_no_kernel():
     ("poisson_wiener_gain_mode", "classic"),
     ("ht_weight_scope", "group"),
     ("poisson_group_mass_conservation", "ht"),
-    ("blockmatch_distance", "pearson"),
+    ("blockmatch_distance", "ssd"),
 ])
 def test_unsupported_options_raise_before_any_gpu_work(field, value):
     p = _poisson()

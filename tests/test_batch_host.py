@@ -29,7 +29,7 @@ def _blockmatch(recs, block=(4, 4, 4), radius=(7, 7, 7)):
     dummy = ctypes.c_void_p(16)  # never dereferenced: the records are rejected first
     return L.bmnd_hip_blockmatch_batch(dummy, ctypes.c_void_p(recs.ctypes.data), dummy, len(recs),
                                        i32x3(*block), i32x3(*radius), dummy, dummy, None, 10, 1.0, 16,
-                                       2, 1, 1, 1.0, 4, 0, dummy, dummy, dummy, None)
+                                       2, 1, 1, 1.0, 4, 0, None, 0, dummy, dummy, dummy, None)
 
 
 def test_aggregate_tiles():

@@ -127,3 +127,75 @@ def test_reference_finite_count_batch_matches_bmnd():
         imgs[0], params, prof.ht_match_threshold, 0.1, 1.0))
     ref, _ = oracle.bmnd_poisson(imgs[0], params)
     assert np.array_equal(outs[0].view(np.int32), ref.view(np.int32))
+
+
+_CODES = {"poisson_deviance": oracle.DIST_DEVIANCE, "pearson": oracle.DIST_PEARSON,
+          "anscombe_ssd": oracle.DIST_ANSCOMBE}
+
+
+@pytest.mark.parametrize("kind,dist,policy", [
+    ("volume", "pearson", "fixed"), ("image", "anscombe_ssd", "fixed"),
+    ("scaled-volume", "anscombe_ssd", "fixed"), ("scaled-image", "pearson", "fixed"),
+    ("volume", "poisson_deviance", "candidate_standardized"),
+    ("image", "pearson", "candidate_standardized"),
+    ("scaled-image", "anscombe_ssd", "candidate_standardized"),
+    ("volume", "anscombe_ssd", "reference_finite_count")])
+def test_distances_and_standardized_policy_match_oracle(kind, dist, policy):
+    """The Pearson and Anscombe SSD distances in both stages and the
+    candidate-standardized stage-1 distance (blockmatching.py:322-426):
+    lists, pilot and output bit-identical to the oracle; stage-1 accepted
+    counts equal to the oracle matcher's."""
+    print(f"distance: {kind}, {dist}, policy {policy}", flush=True)
+    noisy, prof = _policy_input(kind)
+    prof.blockmatch_distance = dist
+    prof.poisson_ht_match_policy = policy
+    params = oracle.default_params(noisy.ndim, noisy.shape)
+    params.distance = _CODES[dist]
+    thr = np.float32(prof.ht_match_threshold)
+    null = None
+    if policy == "candidate_standardized":
+        oracle.set_ht_standardized(params, 512)
+        null = params._null_keep
+    elif policy == "reference_finite_count":
+        thr = oracle.poisson_reference_thresholds(noisy, params, prof.ht_match_threshold,
+                                                  prof.poisson_ht_match_structure_beta,
+                                                  prof.poisson_ht_match_intensity_power)
+        oracle.set_ht_thresholds(params, thr)
+    stats, _, _ = _compare(noisy, prof, params, "auto-fast")
+    assert stats["ht_groups_differing"] == 0
+    assert stats["wiener_groups_differing"] == 0
+    assert stats["pilot_bits_differing"] == 0
+    assert stats["out_bits_differing"] == 0
+    dev = torch.device("cuda", 0)
+    pipe = PoissonPipeline(prof, noisy.shape, dev)
+    ht, _ = pipe.run(torch.from_numpy(noisy).to(dev))
+    plan = pipe.plans["ht"]
+    counts = [noisy.shape[d] - plan.block[d] + 1 for d in range(noisy.ndim)]
+    o = oracle.blockmatch(plan.schedule.reference_abs, counts,
+                          tuple(int(r) for r in prof.ht_search_window), plan.max_m, plan.min_m,
+                          thr, volume=noisy, block=plan.block, code=_CODES[dist],
+                          null_moments=null)
+    acc = ht.accepted.cpu().numpy()
+    assert np.array_equal(acc, o.accepted)
+    assert np.array_equal(ht.counts.cpu().numpy(), o.counts)
+    print(f"accepted counts: min {acc.min()} median {np.median(acc)} max {acc.max()}", flush=True)
+
+
+def test_standardized_pearson_batch_matches_bmnd():
+    prof = BM3DProfile(noise_model=NoiseModel.POISSON)
+    prof.blockmatch_distance = "pearson"
+    prof.poisson_ht_match_policy = "candidate_standardized"
+    imgs = []
+    for side, seed, peak in (((40, 45), 78, 5.0), ((64, 33), 79, 30.0), ((19, 50), 80, 10.0)):
+        _, img = phantoms.blob_image(max(max(side), 32), seed=seed, peak=peak)
+        imgs.append(np.ascontiguousarray(img[:side[0], :side[1]]))
+    imgs[1] = (imgs[1] * np.float32(0.37)).astype(np.float32)
+    print("candidate_standardized pearson: bmnd_batch of 3 images", flush=True)
+    outs = bmnd_batch(imgs, prof)
+    for i, x in enumerate(imgs):
+        assert np.array_equal(outs[i].view(np.int32), bmnd(x, prof).view(np.int32)), f"input {i}"
+    params = oracle.default_params(2, imgs[0].shape)
+    params.distance = oracle.DIST_PEARSON
+    oracle.set_ht_standardized(params, 512)
+    ref, _ = oracle.bmnd_poisson(imgs[0], params)
+    assert np.array_equal(outs[0].view(np.int32), ref.view(np.int32))

@@ -120,5 +120,7 @@ def test_policy_support():
     prof.poisson_ht_match_policy = PoissonHTMatchPolicy.REFERENCE_FINITE_COUNT
     check_supported(prof, resolve_profile(prof, (16, 16, 16)))
     prof.poisson_ht_match_policy = PoissonHTMatchPolicy.CANDIDATE_STANDARDIZED
-    with pytest.raises(NotImplementedError, match="poisson_ht_match_policy"):
+    check_supported(prof, resolve_profile(prof, (16, 16, 16)))
+    prof.poisson_group_mass_conservation = "ht"
+    with pytest.raises(NotImplementedError, match="poisson_group_mass_conservation"):
         check_supported(prof, resolve_profile(prof, (16, 16, 16)))
