@@ -1454,6 +1454,7 @@ struct FilterArgs {
   int max_m;
   const float *mats;
   float shrink;
+  int gain_mode;  // Wiener: 0 variance-scaled, 1 classic, 2 noise-floor
   int k;
   double count_scale;
   const float *exact_std;  // (nrefs, kcap, P) exact-plane std from k_exact_std
@@ -2060,10 +2061,12 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
       return c * keep;
     });
   } else {
-    // Poisson Wiener gain, variance-scaled (wiener.py:24-53, 78-95), computed
-    // as the pilot's coefficients come out of its last pass; the noisy
-    // coefficients take it in theirs, after the risks
+    // Poisson Wiener gain (wiener.py:24-53, 78-95), signal power |r|^2 less
+    // mu sigma^2 (variance-scaled), less sigma^2 (noise-floor) or not at all
+    // (classic), computed as the pilot's coefficients come out of its last
+    // pass; the noisy coefficients take it in theirs, after the risks
     const float mu = a.shrink;
+    const int gmode = a.gain_mode;
     const float eps32 = 1e-10f;
     group_transform<NT, B0, B1, B2, NMAX>(s_r, n, s_g, s_sf, [&](int e, float r) {
       const float sg = s_sig[e];
@@ -2071,8 +2074,11 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
       const float eff = mu * nv;
       const float rc = fabsf(r);
       const float pr = rc * rc;
-      float sp_ = pr - eff;
-      if (sp_ < 0.0f) sp_ = 0.0f;
+      float sp_ = pr;
+      if (gmode != 1) {
+        sp_ = pr - (gmode == 2 ? nv : eff);
+        if (sp_ < 0.0f) sp_ = 0.0f;
+      }
       float gain = sp_ / ((sp_ + eff) + eps32);
       gain = gain < 0.0f ? 0.0f : (gain > 1.0f ? 1.0f : gain);
       return gain;  // the pilot coefficient's slot now holds the attenuation
@@ -3076,6 +3082,9 @@ static int32_t filter_impl(int32_t wiener, const float *noisy, const float *pilo
     return rc;
   }
   if (k < 0) return fail(BMND_HIP_EINVAL, "k must be non-negative");
+  if (wiener < 0 || wiener > 3)
+    return fail(BMND_HIP_EINVAL, "wiener must be 0 (hard threshold) or 1, 2, 3 (Wiener gain: "
+                                 "variance-scaled, classic, noise-floor)");
   if (nrefs <= 0) return 0;
   if (wiener && !pilot) return fail(BMND_HIP_EINVAL, "Wiener filtering needs the pilot volume");
   if (k > 0 && !exact_std) return fail(BMND_HIP_EINVAL, "k > 0 needs the exact-plane std buffer");
@@ -3095,6 +3104,7 @@ static int32_t filter_impl(int32_t wiener, const float *noisy, const float *pilo
   a.max_m = max_matches;
   a.mats = matrices;
   a.shrink = shrink;
+  a.gain_mode = wiener > 0 ? wiener - 1 : 0;
   a.k = k;
   a.count_scale = count_scale;
   a.exact_std = exact_std;

@@ -148,6 +148,11 @@ _DISTANCE_CODES = {BlockMatchDistance.POISSON_DEVIANCE: 1, BlockMatchDistance.PE
                    BlockMatchDistance.ANSCOMBE_SSD: 3}
 
 
+# Wiener gain signal power (wiener.py:24-53) -> the filters' wiener argument
+_GAIN_ARG = {PoissonWienerGainMode.VARIANCE_SCALED: 1, PoissonWienerGainMode.CLASSIC: 2,
+             PoissonWienerGainMode.NOISE_FLOOR: 3}
+
+
 def _ht_policy(profile: BMNDProfile) -> PoissonHTMatchPolicy:
     return coerce_enum(profile.poisson_ht_match_policy, PoissonHTMatchPolicy,
                        "poisson_ht_match_policy")
@@ -165,8 +170,6 @@ def check_supported(profile: BMNDProfile, resolved) -> None:
         return coerce_enum(getattr(profile, name), enum, name)
 
     checks = [
-        (field("poisson_wiener_gain_mode", PoissonWienerGainMode)
-         is PoissonWienerGainMode.VARIANCE_SCALED, "poisson_wiener_gain_mode='variance-scaled'"),
         (field("poisson_group_mass_conservation", PoissonGroupMassConservation)
          is PoissonGroupMassConservation.NONE, "poisson_group_mass_conservation='none'"),
         (not profile.ht_use_soft_thresholding, "ht_use_soft_thresholding=False"),
@@ -322,6 +325,9 @@ class PoissonPipeline:
         self.distance = resolve_blockmatch_distance(profile.blockmatch_distance, NoiseModel.POISSON)
         # candidate-standardized stage-1 distance (bmndalgo.py:311-316)
         self.standardized = _ht_policy(profile) is PoissonHTMatchPolicy.CANDIDATE_STANDARDIZED
+        # the filters' stage argument: 0 hard threshold, 1-3 the Wiener gain
+        self.wiener_arg = _GAIN_ARG[coerce_enum(profile.poisson_wiener_gain_mode,
+                                                PoissonWienerGainMode, "poisson_wiener_gain_mode")]
         self._null = None
         self.k = int(profile.k)
         self.plans = {s: make_plan(profile, s, self.ndim) for s in ("ht", "wiener")}
@@ -490,7 +496,7 @@ class PoissonPipeline:
             gsig = torch.empty(R if wiener else R * plan.max_m * plan.P, dtype=torch.float32,
                                device=dev)
         tok = timer.start(f"{name}.filter")
-        _lib.check(L.bmnd_hip_filter_groups_sigma(int(wiener), _ptr(noisy),
+        _lib.check(L.bmnd_hip_filter_groups_sigma(self.wiener_arg if wiener else 0, _ptr(noisy),
                                                   _ptr((match_vol if pilot is None else pilot)
                                                        if wiener else None), _ptr(var),
                                                   shape, block, radius, _ptr(sel), _ptr(cnt), R,
@@ -739,7 +745,7 @@ class BatchPipeline:
         wiener = name == "wiener"
         tok = timer.start(f"{name}.filter")
         _lib.check(L.bmnd_hip_filter_groups_batch(
-            int(wiener), _ptr(noisy),
+            self._single.wiener_arg if wiener else 0, _ptr(noisy),
             _ptr((match_vol if pilot is None else pilot) if wiener else None), _ptr(var), ih, idv, nin,
             block, radius, _ptr(sel), _ptr(cnt), _ptr(st.ref_input), R, plan.max_m, _ptr(mats),
             _ptr(exact), plan.shrink, k, self._single.count_scale, _ptr(filt), _ptr(wts), stream))

@@ -160,7 +160,10 @@ int32_t bmnd_hip_exact_std(const float *var, const int64_t *shape_h,
  *   var        variance volume of the stage
  *   filt       out: (nrefs*max_matches, P) filtered patches, row g*max+i
  *   weights    out: (nrefs*max_matches) float32 weights
- * shrink is ht_lambda_threshold (HT) or wiener_variance_scale (Wiener). */
+ * shrink is ht_lambda_threshold (HT) or wiener_variance_scale (Wiener).
+ * wiener selects the Wiener gain's signal power (wiener.py:24-53): 1
+ * variance-scaled max(r^2 - shrink sigma^2, 0), 2 classic r^2, 3 noise-floor
+ * max(r^2 - sigma^2, 0); the same values in every entry that takes wiener. */
 int32_t bmnd_hip_filter_groups(int32_t wiener, const float *noisy,
                                const float *pilot, const float *var,
                                const int64_t *shape_h, const int32_t *block_h,

@@ -67,6 +67,7 @@ class OracleParams(ctypes.Structure):
         ("null_means", ctypes.c_void_p),
         ("null_vars", ctypes.c_void_p),
         ("null_max", ctypes.c_int),
+        ("wi_gain_mode", ctypes.c_int),
     ]
 
 
@@ -115,6 +116,7 @@ def _declare(L: ctypes.CDLL) -> None:
     L.oracle_patch_weights.argtypes = [vp, vp, i32, i32, vp, f64, vp]
     L.oracle_ht_shrink.argtypes = [vp, vp, i64, f32, f64, vp]
     L.oracle_wiener_shrink.argtypes = [vp, vp, vp, i64, f32, vp]
+    L.oracle_wiener_shrink_mode.argtypes = [vp, vp, vp, i64, f32, i32, vp]
     L.oracle_scatter_add.argtypes = [vp, vp, i32, vp, vp, vp, vp, vp, vp, i32]
     L.oracle_bmnd_poisson.argtypes = [vp, ctypes.POINTER(OracleParams), vp, vp,
                                       vp, vp, i64, vp, vp, i64]
@@ -322,12 +324,15 @@ def ht_shrink(coef, sigma, lam=3.0, count_scale=1.0):
     return c, att
 
 
-def wiener_shrink(coef, ref_coef, sigma, mu=0.4):
+GAIN_MODES = {"variance-scaled": 0, "classic": 1, "noise-floor": 2}
+
+
+def wiener_shrink(coef, ref_coef, sigma, mu=0.4, mode="variance-scaled"):
     c = np.ascontiguousarray(coef, dtype=np.float32).copy()
     r = np.ascontiguousarray(ref_coef, dtype=np.float32)
     s = np.ascontiguousarray(sigma, dtype=np.float32)
     att = np.empty_like(c)
-    lib().oracle_wiener_shrink(_p(c), _p(r), _p(s), c.size, mu, _p(att))
+    lib().oracle_wiener_shrink_mode(_p(c), _p(r), _p(s), c.size, mu, GAIN_MODES[mode], _p(att))
     return c, att
 
 

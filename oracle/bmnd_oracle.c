@@ -962,13 +962,24 @@ void oracle_ht_shrink(float *coef, const float *sigma, int64_t n, float lambda_,
  * all float32; coef (the noisy coefficients) is multiplied by g. */
 void oracle_wiener_shrink(float *coef, const float *ref_coef, const float *sigma,
                           int64_t n, float mu, float *atten) {
+  oracle_wiener_shrink_mode(coef, ref_coef, sigma, n, mu, 0, atten);
+}
+
+/* wiener.py:24-53, 78-95: signal power |r|^2 less mu sigma^2 (mode 0,
+ * variance-scaled), kept whole (1, classic) or less sigma^2 (2, noise-floor),
+ * floored at 0; gain = power / (power + mu sigma^2 + eps) clipped to [0, 1]. */
+void oracle_wiener_shrink_mode(float *coef, const float *ref_coef, const float *sigma,
+                               int64_t n, float mu, int mode, float *atten) {
   const float eps32 = (float)1e-10;
   for (int64_t e = 0; e < n; ++e) {
     float nv = sigma[e] * sigma[e];
     float eff = mu * nv;
     float pr = fabsf(ref_coef[e]) * fabsf(ref_coef[e]);
-    float spow = pr - eff;
-    if (spow < 0.0f) spow = 0.0f;
+    float spow = pr;
+    if (mode != 1) {
+      spow = pr - (mode == 2 ? nv : eff);
+      if (spow < 0.0f) spow = 0.0f;
+    }
     float gain = spow / ((spow + eff) + eps32);
     if (gain < 0.0f) gain = 0.0f;
     if (gain > 1.0f) gain = 1.0f;
@@ -1275,7 +1286,7 @@ static int run_stage(const oracle_params *prm, int wiener, const float *match_vo
         oracle_ht_shrink(X, sig, (int64_t)n * P, prm->ht_lambda, prm->count_scale, att);
       } else {
         oracle_apply_transform(Y, ndim + 1, dims, fm);
-        oracle_wiener_shrink(X, Y, sig, (int64_t)n * P, prm->wi_mu, att);
+        oracle_wiener_shrink_mode(X, Y, sig, (int64_t)n * P, prm->wi_mu, prm->wi_gain_mode, att);
       }
       oracle_apply_transform(X, ndim + 1, dims, im);
       float *fo = filt + off[r] * P;

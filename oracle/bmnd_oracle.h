@@ -87,6 +87,8 @@ void oracle_ht_shrink(float *coef, const float *sigma, int64_t n, float lambda_,
                       double count_scale, float *atten);
 void oracle_wiener_shrink(float *coef, const float *ref_coef, const float *sigma,
                           int64_t n, float mu, float *atten);
+void oracle_wiener_shrink_mode(float *coef, const float *ref_coef, const float *sigma,
+                               int64_t n, float mu, int mode, float *atten);
 void oracle_scatter_add(float *num, float *den, int ndim, const int64_t *vol_shape,
                         const int *block, const float *kaiser, const float *filt,
                         const float *weights, const int64_t *lins, int n);
@@ -143,6 +145,8 @@ typedef struct {
   const float *null_means;
   const float *null_vars;
   int null_max;
+  /* Wiener gain signal power: 0 variance-scaled, 1 classic, 2 noise-floor */
+  int wi_gain_mode;
 } oracle_params;
 
 /* Runs both stages.  pilot may be NULL.  The match lists of each stage can

@@ -146,3 +146,44 @@ def test_engine_modes(dist, policy, ht, wi):
     assert pipe.blockmatch_mode(None, "ht") == (ht, 0)
     if wi is not None:
         assert pipe.blockmatch_mode(None, "wiener") == (wi, 0)
+
+
+# ---- Wiener gain modes (wiener.py:24-53, 78-95) ------------------------------
+
+@pytest.mark.parametrize("mu,ref,checks", [
+    # tests/compat/test_component_semantics.py:54-76 (mu = 1)
+    (1.0, [0.0, 0.1, 0.4, 1.5], (("classic", 0.0), ("variance-scaled", 1.0))),
+    # tests/compat/test_component_semantics.py:79-108 (mu = 2)
+    (2.0, [0.0, 0.3, 0.6, 1.5], (("classic", 0.0), ("noise-floor", 1.0),
+                                 ("variance-scaled", 2.0)))])
+def test_gain_modes_known_answers(mu, ref, checks):
+    """The reference's closed forms: signal = max(r^2 - s sigma^2, 0) with
+    s = 0 (classic), 1 (noise-floor), mu (variance-scaled); gain = signal /
+    (signal + mu sigma^2)."""
+    r = np.asarray(ref, np.float32)
+    sigma = np.full(4, 0.2, np.float32)
+    for mode, sub in checks:
+        coef, gain = oracle.wiener_shrink(np.ones(4, np.float32), r, sigma, mu, mode)
+        signal = np.maximum(r.astype(np.float64) ** 2 - sub * 0.04, 0.0)
+        np.testing.assert_allclose(gain, signal / (signal + mu * 0.04), rtol=1e-6)
+        assert np.array_equal(coef, gain)
+
+
+def test_gain_modes_are_ordered():
+    """Classic >= noise-floor >= variance-scaled at mu >= 1 (the reference's
+    'sits between' property)."""
+    rng = np.random.default_rng(5)
+    r = rng.normal(0.0, 1.0, 4096).astype(np.float32)
+    sigma = rng.uniform(0.05, 1.0, 4096).astype(np.float32)
+    g = {m: oracle.wiener_shrink(np.ones_like(r), r, sigma, 2.0, m)[1]
+         for m in ("classic", "noise-floor", "variance-scaled")}
+    assert np.all(g["classic"] >= g["noise-floor"])
+    assert np.all(g["noise-floor"] >= g["variance-scaled"])
+
+
+@pytest.mark.parametrize("mode,arg", [("variance-scaled", 1), ("classic", 2), ("noise-floor", 3)])
+def test_engine_gain_argument(mode, arg):
+    prof = BM4DProfile(noise_model=NoiseModel.POISSON)
+    prof.poisson_wiener_gain_mode = mode
+    check_supported(prof, resolve_profile(prof, (16, 16, 16)))
+    assert PoissonPipeline(prof, (16, 16, 16), "cpu").wiener_arg == arg

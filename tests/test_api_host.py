@@ -88,7 +88,7 @@ def test_all_zero_volume_fast_path_runs_no_kernel():
 
 
 @pytest.mark.parametrize("field,value", [
-    ("poisson_wiener_gain_mode", "classic"),
+    ("wiener_weight_model", "risk"),
     ("ht_weight_scope", "group"),
     ("poisson_group_mass_conservation", "ht"),
     ("blockmatch_distance", "ssd"),

@@ -199,3 +199,23 @@ def test_standardized_pearson_batch_matches_bmnd():
     oracle.set_ht_standardized(params, 512)
     ref, _ = oracle.bmnd_poisson(imgs[0], params)
     assert np.array_equal(outs[0].view(np.int32), ref.view(np.int32))
+
+
+@pytest.mark.parametrize("kind,mode", [("volume", "classic"), ("image", "noise-floor"),
+                                       ("scaled-volume", "noise-floor"),
+                                       ("scaled-image", "classic")])
+def test_wiener_gain_modes_match_oracle(kind, mode):
+    """poisson_wiener_gain_mode 'classic' and 'noise-floor' (wiener.py:24-53):
+    pilot, output and lists bit-identical to the oracle."""
+    print(f"gain mode: {kind}, {mode}", flush=True)
+    noisy, prof = _policy_input(kind)
+    prof.poisson_wiener_gain_mode = mode
+    params = oracle.default_params(noisy.ndim, noisy.shape)
+    params.wi_gain_mode = oracle.GAIN_MODES[mode]
+    stats, out, _ = _compare(noisy, prof, params, "auto-fast")
+    assert stats["ht_groups_differing"] == 0
+    assert stats["wiener_groups_differing"] == 0
+    assert stats["pilot_bits_differing"] == 0
+    assert stats["out_bits_differing"] == 0
+    default, _ = oracle.bmnd_poisson(noisy, oracle.default_params(noisy.ndim, noisy.shape))
+    assert not np.array_equal(out, default)  # the option changes the estimate
