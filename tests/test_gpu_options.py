@@ -219,3 +219,51 @@ def test_wiener_gain_modes_match_oracle(kind, mode):
     assert stats["out_bits_differing"] == 0
     default, _ = oracle.bmnd_poisson(noisy, oracle.default_params(noisy.ndim, noisy.shape))
     assert not np.array_equal(out, default)  # the option changes the estimate
+
+
+@pytest.mark.parametrize("kind,scale,dist,policy", [
+    ("scaled-volume", 2.5, "pearson", "candidate_standardized"),
+    ("image", 0.5, "poisson_deviance", "reference_finite_count"),
+    ("volume", 1.7, "anscombe_ssd", "fixed")])
+def test_options_with_count_scale_match_oracle(kind, scale, dist, policy):
+    """The new distances and policies at a count scale other than 1 (counts
+    divided by poisson_count_scale in every distance and table lookup)."""
+    print(f"count scale {scale}: {kind}, {dist}, {policy}", flush=True)
+    noisy, prof = _policy_input(kind)
+    prof.blockmatch_distance = dist
+    prof.poisson_ht_match_policy = policy
+    prof.poisson_count_scale = scale
+    params = oracle.default_params(noisy.ndim, noisy.shape)
+    params.distance = _CODES[dist]
+    params.count_scale = scale
+    if policy == "candidate_standardized":
+        oracle.set_ht_standardized(params, 512)
+    elif policy == "reference_finite_count":
+        oracle.set_ht_thresholds(params, oracle.poisson_reference_thresholds(
+            noisy, params, prof.ht_match_threshold, prof.poisson_ht_match_structure_beta,
+            prof.poisson_ht_match_intensity_power))
+    stats, _, _ = _compare(noisy, prof, params, "auto-fast")
+    assert stats["ht_groups_differing"] == 0
+    assert stats["wiener_groups_differing"] == 0
+    assert stats["pilot_bits_differing"] == 0
+    assert stats["out_bits_differing"] == 0
+
+
+def test_sigma_maps_under_options_match_oracle():
+    """return_sigma_map with the standardized Anscombe policy and the
+    noise-floor gain: estimate and both maps bit-identical to the oracle."""
+    noisy, prof = _policy_input("volume")
+    prof.blockmatch_distance = "anscombe_ssd"
+    prof.poisson_ht_match_policy = "candidate_standardized"
+    prof.poisson_wiener_gain_mode = "noise-floor"
+    print("sigma maps under options", flush=True)
+    out, maps = bmnd(noisy, prof, return_sigma_map=True)
+    params = oracle.default_params(3, noisy.shape)
+    params.distance = oracle.DIST_ANSCOMBE
+    params.wi_gain_mode = oracle.GAIN_MODES["noise-floor"]
+    oracle.set_ht_standardized(params, 512)
+    o_out, o_maps = oracle.bmnd_poisson_sigma_maps(noisy, params)
+    assert np.array_equal(out.view(np.int32), o_out.view(np.int32))
+    assert maps["global"] == o_maps["global"]
+    for stage in ("ht", "wiener"):
+        assert np.array_equal(maps[stage].view(np.int32), o_maps[stage].view(np.int32)), stage
