@@ -258,9 +258,11 @@ int64_t bmnd_hip_stage_workspace(const int64_t *shape_h, const int32_t *block_h,
  * fallback (the observation), the schedule ref_lin, the matrix table
  * (bmnd_hip_matrix_table_size), the Kaiser window (P floats), the workspace of
  * bmnd_hip_stage_workspace() bytes, and the outputs est, sel, counts,
- * accepted.  mode as in bmnd_hip_blockmatch, or -1 for the automatic choice
- * (the count table for integer counts <= 100 at count_scale 1, which reads
- * two integers back from the device, else mode 4). */
+ * accepted.  mode as in bmnd_hip_blockmatch (0-6; the standardized modes
+ * need bmnd_hip_blockmatch_standardized's null table and are refused here),
+ * or -1 for the automatic choice (the count table for integer counts <= 100
+ * at count_scale 1, which reads two integers back from the device, else mode
+ * 4).  wiener as in bmnd_hip_filter_groups (0 HT, 1-3 Wiener gain). */
 int32_t bmnd_hip_stage(int32_t wiener, const float *match_vol, const float *noisy,
                        const float *var, const float *fallback, const int64_t *shape_h,
                        const int32_t *block_h, const int32_t *radius_h, const int32_t *ref_lin,
