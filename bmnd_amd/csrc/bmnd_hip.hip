@@ -1547,7 +1547,9 @@ __global__ __launch_bounds__(64) void k_exact_std(ExactArgs a) {
   __shared__ float4 s_g4[NMAX];
   __shared__ int s_pos[NMAX][3];      // member origins in union-box coordinates
   __shared__ unsigned s_mask0[U0], s_mask1[U1], s_mask2[U2];
-  constexpr int LCAP = 512;  // entries of one chunk held at a time (more take extra rounds)
+  // entries of one chunk held at a time (more take extra rounds); 320 keeps
+  // the workgroup's LDS under 5 KB, which allows 8 waves per SIMD
+  constexpr int LCAP = 320;
   __shared__ unsigned s_list[LCAP];
 
   const int lane = threadIdx.x;
