@@ -267,3 +267,21 @@ def test_sigma_maps_under_options_match_oracle():
     assert maps["global"] == o_maps["global"]
     for stage in ("ht", "wiener"):
         assert np.array_equal(maps[stage].view(np.int32), o_maps[stage].view(np.int32)), stage
+
+
+def test_finite_count_anscombe_volume_batch_matches_bmnd():
+    """bmnd_batch over uneven volumes with the Anscombe SSD distance and
+    per-reference thresholds: each input equal to bmnd() alone."""
+    prof = BM4DProfile(noise_model=NoiseModel.POISSON)
+    prof.blockmatch_distance = "anscombe_ssd"
+    prof.poisson_ht_match_policy = "reference_finite_count"
+    vols = []
+    for shape, seed in (((20, 24, 18), 91), ((16, 16, 30), 92), ((9, 28, 22), 93)):
+        _, v = phantoms.ellipsoid_volume(shape, seed=seed, peak=8.0, n_ellipsoids=4,
+                                         axes_range=(2.0, 8.0))
+        vols.append(v)
+    vols[1] = (vols[1] * np.float32(0.37)).astype(np.float32)
+    print("reference_finite_count anscombe: bmnd_batch of 3 volumes", flush=True)
+    outs = bmnd_batch(vols, prof)
+    for i, x in enumerate(vols):
+        assert np.array_equal(outs[i].view(np.int32), bmnd(x, prof).view(np.int32)), f"input {i}"
