@@ -3,7 +3,9 @@
 The oracle (``bmnd_oracle.c``) restates the reference denoiser's Poisson
 branch on the CPU.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
 ``cpu_baseline`` leg of ``bench.py`` import this module; the product package
-``bmnd_amd`` never does.
+``bmnd_amd`` never does.  What pins each function, and what stays unpinned
+(the last bit of ``log``, SSD on non-integer data, the per-voxel sigma-map
+values, end-to-end reference outputs), is stated in ``bmnd_oracle.h``.
 """
 
 from __future__ import annotations

@@ -9,6 +9,20 @@
  *
  * Conventions: arrays are C-contiguous; positions and linear indices are
  * int64; the patch grid has counts[d] = shape[d] - block[d] + 1 origins.
+ *
+ * Pinning (tests/test_oracle_*.py and tests/test_*_cpu.py): every function
+ * is checked against the reference's own known answers or properties, a
+ * closed form, numpy/scipy, or brute force.  Not fully pinned:
+ *   - the last bit of log() in the deviance terms (the reference calls the
+ *     platform log through numba; nothing in its tests fixes its ulps):
+ *     parity unpinned at that bit;
+ *   - oracle_patch_distance with ORACLE_DIST_SSD: its float64 accumulation is
+ *     checked on integer-valued known answers only (parity unpinned for
+ *     non-integer data; the Poisson path never uses SSD);
+ *   - the per-voxel values of the sigma maps (oracle_bmnd_poisson_sigma):
+ *     invariants only, parity partially pinned;
+ *   - end-to-end outputs: the reference cannot run here, so the pipeline
+ *     rests on its components' pins.
  */
 #ifndef BMND_ORACLE_H
 #define BMND_ORACLE_H
