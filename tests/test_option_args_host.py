@@ -1,0 +1,63 @@
+"""Argument checks of the option entry points, on the host: the library
+refuses a matching mode out of range, the standardized modes without their
+null table, an unknown standardized distance, a missing per-reference
+threshold array and an unknown Wiener gain before any device work (dummy
+device pointers are never dereferenced)."""
+
+import ctypes
+
+import pytest
+
+from bmnd_amd import _lib
+
+I32X3, I64X3 = ctypes.c_int32 * 3, ctypes.c_int64 * 3
+DUMMY = ctypes.c_void_p(16)
+SHAPE, BLOCK, RADIUS = I64X3(24, 24, 24), I32X3(4, 4, 4), I32X3(7, 7, 7)
+
+
+def _error(rc):
+    assert rc != 0
+    return _lib.lib().bmnd_hip_last_error().decode()
+
+
+@pytest.mark.parametrize("mode", [-1, 10])
+def test_mode_out_of_range(mode):
+    L = _lib.lib()
+    rc = L.bmnd_hip_blockmatch(DUMMY, SHAPE, BLOCK, RADIUS, DUMMY, 10, 1.0, 16, 2, 1, 1, 1.0,
+                               mode, 0, DUMMY, DUMMY, DUMMY, None)
+    assert "mode must be in 0..9" in _error(rc)
+
+
+@pytest.mark.parametrize("mode", [7, 8, 9])
+def test_standardized_modes_need_the_table(mode):
+    L = _lib.lib()
+    rc = L.bmnd_hip_blockmatch(DUMMY, SHAPE, BLOCK, RADIUS, DUMMY, 10, 1.0, 16, 2, 1, 1, 1.0,
+                               mode, 0, DUMMY, DUMMY, DUMMY, None)
+    assert "null-moment table" in _error(rc)
+    rc = L.bmnd_hip_blockmatch_standardized(DUMMY, SHAPE, BLOCK, RADIUS, DUMMY, 10, 1.0, 16, 2,
+                                            1, 1, 1.0, mode - 6, DUMMY, 0, DUMMY, DUMMY, DUMMY,
+                                            None)
+    assert "null-moment table" in _error(rc)
+
+
+@pytest.mark.parametrize("distance", [0, 4])
+def test_standardized_distance_codes(distance):
+    rc = _lib.lib().bmnd_hip_blockmatch_standardized(
+        DUMMY, SHAPE, BLOCK, RADIUS, DUMMY, 10, 1.0, 16, 2, 1, 1, 1.0, distance, DUMMY, 512,
+        DUMMY, DUMMY, DUMMY, None)
+    assert "distance must be 1" in _error(rc)
+
+
+def test_thresholds_entry_needs_the_array():
+    rc = _lib.lib().bmnd_hip_blockmatch_thresholds(
+        DUMMY, SHAPE, BLOCK, RADIUS, DUMMY, None, 10, 16, 2, 1, 1, 1.0, 4, 0, DUMMY, DUMMY,
+        DUMMY, None)
+    assert "ref_thresholds must not be null" in _error(rc)
+
+
+@pytest.mark.parametrize("wiener", [-1, 4])
+def test_filter_stage_argument(wiener):
+    rc = _lib.lib().bmnd_hip_filter_groups(
+        wiener, DUMMY, DUMMY, DUMMY, SHAPE, BLOCK, RADIUS, DUMMY, DUMMY, 10, 16, DUMMY, DUMMY,
+        3.0, 0, 1.0, DUMMY, DUMMY, None)
+    assert "wiener must be 0" in _error(rc)
