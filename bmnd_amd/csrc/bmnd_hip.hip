@@ -571,7 +571,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 
   __shared__ float s_slab[SLAB];
   __shared__ double s_ref[P];
   __shared__ double s_ref_log[P];
-  __shared__ unsigned long long s_keys[BOX];
+  // tie-grid key of every box candidate; its composite (accepted flag, key,
+  // box index) is formed when read, the flag being key < threshold key
+  __shared__ float s_keys[BOX];
   __shared__ double s_tab[kLogTabDoubles];
   __shared__ int s_acc;
   __shared__ int s_refrow[P];
@@ -710,7 +712,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 
         bool ok;
         const float key = bm_key<P>(dk[k], a, ok);
         my_acc += ok ? 1 : 0;
-        s_keys[cand] = bm_composite(key, ok, cand);
+        s_keys[cand] = key;
       }
     }
   }
@@ -776,17 +778,23 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 
       const float dstd =
           nvs > 0.0f ? (float)((d - (double)nms) / (double)(float)sqrt((double)nvs)) : 0.0f;
       const float key = bm_key_signed(dstd, a, ok);
-      s_keys[cand] = bm_composite_signed(key, cand);
+      s_keys[cand] = key;
     } else {
       const float key = bm_key<P>(d, a, ok);
-      s_keys[cand] = bm_composite(key, ok, cand);
+      s_keys[cand] = key;
     }
     my_acc += ok ? 1 : 0;
   }
   wave_atomic_add(&s_acc, my_acc);
   __syncthreads();
   BM_SEL_SCRATCH(sc);
-  bm_select_write<NT, BOX>(bm_box_keys<NT>([&](int i) { return s_keys[i]; }, box), box, s_acc, a,
+  const float thr_k = a.stabilize ? a.thr_key : a.threshold;
+  auto composite = [&](int i) {
+    const float key = s_keys[i];
+    if constexpr (MODE == 6) return bm_composite_signed(key, i);
+    else return bm_composite(key, key < thr_k, i);
+  };
+  bm_select_write<NT, BOX>(bm_box_keys<NT>(composite, box), box, s_acc, a,
                            g, lo0, lo1, lo2, e1, e2, center_local, sc);
 }
 
