@@ -3382,7 +3382,7 @@ int32_t bmnd_hip_stage(int32_t wiener, const float *match_vol, const float *nois
   if (need < 0) return fail(BMND_HIP_EINVAL, "invalid stage geometry");
   if (!workspace || workspace_bytes < need)
     return fail(BMND_HIP_EINVAL, "workspace is smaller than bmnd_hip_stage_workspace()");
-  if (mode < -1 || mode > 4) return fail(BMND_HIP_EINVAL, "mode must be -1 (automatic) or 0..4");
+  if (mode < -1 || mode > 6) return fail(BMND_HIP_EINVAL, "mode must be -1 (automatic) or 0..6");
   hipStream_t s = (hipStream_t)stream;
   int64_t P = 1, patches = 1, vox = 1;
   for (int d = 0; d < 3; ++d) {
