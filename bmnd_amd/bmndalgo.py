@@ -2,9 +2,11 @@
 
 Same signature, argument validation order, error messages and return
 conventions as the reference's src/bmnd/bmndalgo.py:71-183; the work runs on
-the MI355X device pipeline (engine.PoissonPipeline).  Profile options
-outside the Poisson default path raise NotImplementedError after the
-reference's own validation has passed.
+the MI355X device pipeline (engine.PoissonPipeline).  Profile options the
+device pipeline does not run (engine.check_supported: group-mass
+conservation, risk weighting and the classic/risk weight models, soft
+thresholding, the SSD distance, the Gaussian noise model) raise
+NotImplementedError after the reference's own validation has passed.
 """
 
 from __future__ import annotations
