@@ -187,3 +187,19 @@ def test_engine_gain_argument(mode, arg):
     prof.poisson_wiener_gain_mode = mode
     check_supported(prof, resolve_profile(prof, (16, 16, 16)))
     assert PoissonPipeline(prof, (16, 16, 16), "cpu").wiener_arg == arg
+
+
+@pytest.mark.parametrize("dist,policy,ht,wi", [
+    ("pearson", "fixed", 5, 5), ("anscombe_ssd", "candidate_standardized", 9, 6),
+    ("poisson_deviance", "candidate_standardized", 7, None)])
+def test_batch_modes_follow_the_single_pipeline(dist, policy, ht, wi):
+    """A batch matches every input in the mode the single-input pipeline
+    picks; these options need no look at the data."""
+    from bmnd_amd.engine import BatchPipeline
+    prof = BM3DProfile(noise_model=NoiseModel.POISSON)
+    prof.blockmatch_distance = dist
+    prof.poisson_ht_match_policy = policy
+    bp = BatchPipeline(prof, [(32, 40), (20, 64), (48, 48)], "cpu")
+    assert bp.input_modes(None, "ht") == [(ht, 0)] * 3
+    if wi is not None:
+        assert bp.input_modes(None, "wiener") == [(wi, 0)] * 3
