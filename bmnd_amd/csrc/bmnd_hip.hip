@@ -561,11 +561,14 @@ __device__ __forceinline__ void bm_finalize(int K, int acc, const BMArgs &a, int
 // MODE 6: the candidate-standardized distance of code a.dcode
 //         (blockmatching.py:374-426): (sum of terms - sum of null means) /
 //         sqrt(sum of null variances), the null sums in float32.
-// The count-table kernel targets 6 waves per SIMD (80 VGPRs, no spills for the
-// 4^3 and 8x8 blocks); the 5^3 one needs 100 VGPRs and runs faster at 4 than
-// spilling to reach 6 (103 against 114 ms at 256^3).
+// Occupancy targets of the count-table kernel: 6 waves per SIMD for the 4^3
+// block (80 VGPRs, no spills), 8 for the 8x8 block (64 VGPRs, four workgroups
+// per CU: 21.9 against 22.8 ms on sixteen 512^2 images), 4 for the 5^3 block,
+// which needs 100 VGPRs and runs faster than spilling to reach 6 (103 against
+// 114 ms at 256^3).
 template <int B0, int B1, int B2, int R0, int R1, int R2, int MODE, int NT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MODE == 2 ? (B0 == 5 ? 4 : 6) : 1))) void k_blockmatch(
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(
+    MODE == 2 ? (B0 == 5 ? 4 : (B0 == 1 ? 8 : 6)) : 1))) void k_blockmatch(
     BMArgs a) {
   constexpr int S0 = 2 * R0 + B0, S1 = 2 * R1 + B1, S2 = 2 * R2 + B2;
   constexpr int SLAB = S0 * S1 * S2;
