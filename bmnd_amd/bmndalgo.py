@@ -52,6 +52,11 @@ def _validate_max_cache_bytes(max_cache_bytes):
 _PIPELINES: dict = {}
 
 
+def _device() -> torch.device:
+    """The caller's current HIP device."""
+    return torch.device("cuda", torch.cuda.current_device())
+
+
 def _pipeline(profile, shape, device) -> PoissonPipeline:
     key = (repr(profile), tuple(shape), str(device))
     pipe = _PIPELINES.get(key)
@@ -111,23 +116,32 @@ def bmnd(volume, profile, sigma: float | None = None, sigma_psd=None,
         return vol
     check_supported(profile, resolved)
     vol = np.ascontiguousarray(vol)
-    device = torch.device("cuda", torch.cuda.current_device())
+    device = _device()
     pipe = _pipeline(profile, vol.shape, device)
     vol_t = torch.from_numpy(vol).to(device)
-    if return_sigma_map:
-        ht, wi, global_sigma = pipe.run(vol_t, sigma_maps=True)
-    else:
-        ht, wi = pipe.run(vol_t)
-    if blockmatch_callback is not None:
-        counts = tuple(vol.shape[d] - pipe.plans["ht"].block[d] + 1 for d in range(vol.ndim))
-        blockmatch_callback("ht", build_result(ht.schedule, counts, ht.sel.cpu().numpy(),
-                                               ht.counts.cpu().numpy(), ht.accepted.cpu().numpy()))
+
+    def match_hook(stage: str):
+        # blockmatch_callback(stage, result) right after that stage's matching
+        # and before its filtering, as the reference calls it
+        # (bmndalgo.py:325-326, 742-743)
+        if blockmatch_callback is None:
+            return None
+        plan = pipe.plans[stage]
+        counts = tuple(vol.shape[d] - plan.block[d] + 1 for d in range(vol.ndim))
+
+        def hook(sel, cnt, acc):
+            blockmatch_callback(stage, build_result(plan.schedule, counts, sel.cpu().numpy(),
+                                                    cnt.cpu().numpy(), acc.cpu().numpy()))
+        return hook
+
+    ht, var1, global_sigma = pipe.run_ht(vol_t, sigma_maps=return_sigma_map,
+                                         on_match=match_hook("ht"))
     if stage_callback is not None:
+        # the pilot, before stage 2 starts (bmndalgo.py:632-633)
         stage_callback("ht", ht.est.cpu().numpy())
-    if blockmatch_callback is not None:
-        counts = tuple(vol.shape[d] - pipe.plans["wiener"].block[d] + 1 for d in range(vol.ndim))
-        blockmatch_callback("wiener", build_result(wi.schedule, counts, wi.sel.cpu().numpy(),
-                                                   wi.counts.cpu().numpy(), wi.accepted.cpu().numpy()))
+    wi = pipe.run_wiener(vol_t, ht.est, var1, global_sigma=global_sigma,
+                         on_match=match_hook("wiener"))
+    del var1
     out = wi.est.cpu().numpy()
     if stage_callback is not None:
         stage_callback("wiener", out.copy())

@@ -620,7 +620,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(
       const int i = tid + it * NT;
       // MODE 2: the integer count as a byte offset into a term-table row
       if (i < SLAB) {
-        if constexpr (MODE == 2) s_slab[i] = __int_as_float((int)vv[it] * (int)sizeof(double));
+        // (clamped to the table so that a count outside [0, tmax] cannot read
+        // past it; such inputs break the mode's precondition)
+        if constexpr (MODE == 2)
+          s_slab[i] = __int_as_float((int)fminf(fmaxf(vv[it], 0.0f), (float)a.tmax) *
+                                     (int)sizeof(double));
         else s_slab[i] = vv[it];
       }
     }
@@ -633,7 +637,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(
     s_ref[p] = r;
     if (MODE >= 5) s_ref_log[p] = 2.0 * sqrt(r + 0.375);  // Anscombe transform
     // MODE 2: byte offset of the reference count's term-table row
-    if (MODE == 2) s_refrow[p] = (int)v * (a.tmax + 1) * (int)sizeof(double);
+    if (MODE == 2)
+      s_refrow[p] = (int)fminf(fmaxf(v, 0.0f), (float)a.tmax) * (a.tmax + 1) * (int)sizeof(double);
   }
   if (MODE == 2) {
     const int tn = (a.tmax + 1) * (a.tmax + 1);
@@ -3408,7 +3413,18 @@ int32_t bmnd_hip_stage(int32_t wiener, const float *match_vol, const float *nois
   w += align256(nrefs * max_matches * (int64_t)sizeof(float));
   int32_t *ref_index = reinterpret_cast<int32_t *>(w);
   int32_t table_max = 0;
-  if (mode < 0) {
+  if (mode == 2) {
+    // the count table needs integer counts <= 100 at unit count scale; its
+    // row length is the largest count, read back from the device
+    if (count_scale != 1.0) return fail(BMND_HIP_EINVAL, "mode 2 needs count_scale == 1");
+    if (int rc = bmnd_hip_count_range(match_vol, vox, flags, stream)) return rc;
+    int32_t h[2] = {0, 0};
+    HIP_TRY(hipMemcpyAsync(h, flags, sizeof(h), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (!h[0] || h[1] > kMaxTableCount)
+      return fail(BMND_HIP_EINVAL, "mode 2 needs non-negative integer counts <= 100 in match_vol");
+    table_max = h[1];
+  } else if (mode < 0) {
     // the automatic choice: the count table for integer counts <= 100 at unit
     // count scale, the fast certified matcher otherwise
     mode = 4;

@@ -437,12 +437,14 @@ class PoissonPipeline:
     def stage(self, name: str, match_vol: torch.Tensor, noisy: torch.Tensor, var: torch.Tensor,
               fallback: torch.Tensor, timer=None, sigma_fill: float | None = None,
               pilot: torch.Tensor | None = None,
-              ref_thresholds: torch.Tensor | None = None) -> StageOutput:
+              ref_thresholds: torch.Tensor | None = None, on_match=None) -> StageOutput:
         """One stage.  With `sigma_fill` (the global sigma) the stage also
         builds its sigma map, `sigma_fill` where no patch reaches a voxel.
         The Wiener gains use `pilot` (default: the matched volume).
         `ref_thresholds` (float32, one per reference) replaces the plan's
-        threshold."""
+        threshold.  `on_match(sel, counts, accepted)` runs after the block
+        matching and before the filtering (the point where the reference
+        calls its blockmatch_callback, bmndalgo.py:325-326, 742-743)."""
         L = _lib.lib()
         timer = timer or _NoTimer()
         plan = self.plans[name]
@@ -478,6 +480,8 @@ class PoissonPipeline:
                 plan.max_m, plan.min_m, 1, 1, self.count_scale, mode, tmax, _ptr(sel), _ptr(cnt),
                 _ptr(acc), stream))
         timer.stop(tok)
+        if on_match is not None:
+            on_match(sel.view(R, plan.max_m), cnt, acc)
         mats, kai = plan.device_constants(dev)
         kcap = min(self.k, plan.max_m)
         exact = None
@@ -520,10 +524,11 @@ class PoissonPipeline:
         del filt, wts, ref_index, gsig
         return StageOutput(est, sel.view(R, plan.max_m), cnt, acc, plan.schedule, smap)
 
-    def run(self, vol: torch.Tensor, timer=None, sigma_maps: bool = False):
-        """vol: float32 contiguous device tensor of `shape`.  Returns the HT
-        (pilot) and Wiener stage outputs and, with `sigma_maps`, the global
-        sigma as a third value (each stage output then carries its map)."""
+    def run_ht(self, vol: torch.Tensor, timer=None, sigma_maps: bool = False, on_match=None):
+        """Stage 1 (bmndalgo.py:213-633) on the float32 contiguous device
+        tensor `vol` of `shape`: returns the HT stage output (its `est` is the
+        pilot), the stage-1 variance volume and, with `sigma_maps`, the
+        global sigma (else None)."""
         if vol.dtype != torch.float32 or not vol.is_contiguous() or tuple(vol.shape) != self.shape:
             raise ValueError("vol must be a contiguous float32 tensor of the planned shape")
         t = timer or _NoTimer()
@@ -533,10 +538,28 @@ class PoissonPipeline:
         gsig = self.global_sigma(var1) if sigma_maps else None
         fill = None if gsig is None else float(np.float32(gsig))
         thr = self.ht_thresholds(vol, t)
-        ht = self.stage("ht", vol, vol, var1, vol, timer, fill, ref_thresholds=thr)
-        match2, var2 = self._wiener_sources(vol, ht.est, var1, t)
+        ht = self.stage("ht", vol, vol, var1, vol, timer, fill, ref_thresholds=thr,
+                        on_match=on_match)
+        return ht, var1, gsig
+
+    def run_wiener(self, vol: torch.Tensor, pilot: torch.Tensor, var1: torch.Tensor, timer=None,
+                   global_sigma: float | None = None, on_match=None) -> StageOutput:
+        """Stage 2 (bmndalgo.py:635-1046) from the observation `vol`, the
+        stage-1 pilot and the stage-1 variance volume; with `global_sigma`
+        the stage also builds its sigma map."""
+        t = timer or _NoTimer()
+        fill = None if global_sigma is None else float(np.float32(global_sigma))
+        match2, var2 = self._wiener_sources(vol, pilot, var1, t)
+        return self.stage("wiener", match2, vol, var2, vol, timer, fill, pilot=pilot,
+                          on_match=on_match)
+
+    def run(self, vol: torch.Tensor, timer=None, sigma_maps: bool = False):
+        """vol: float32 contiguous device tensor of `shape`.  Returns the HT
+        (pilot) and Wiener stage outputs and, with `sigma_maps`, the global
+        sigma as a third value (each stage output then carries its map)."""
+        ht, var1, gsig = self.run_ht(vol, timer, sigma_maps)
+        wi = self.run_wiener(vol, ht.est, var1, timer, gsig)
         del var1
-        wi = self.stage("wiener", match2, vol, var2, vol, timer, fill, pilot=ht.est)
         if sigma_maps:
             return ht, wi, gsig
         return ht, wi

@@ -61,7 +61,10 @@ int32_t bmnd_hip_count_range(const float *vol, int64_t n, int32_t *flags,
  * divisions, two logarithms in float64); mode 1 uses the one-log form
  * r log 2r + c log 2c - s log s; mode 2 requires integer counts in
  * [0, table_max] (table_max <= 100, count_scale == 1) and reads each term
- * from a float64 table built on the host with the reference's expression;
+ * from a float64 table built on the host with the reference's expression
+ * (the caller checks the precondition, e.g. with bmnd_hip_count_range; the
+ * kernel clamps counts to [0, table_max] so that no other input reads past
+ * the table, but the lists of such an input are meaningless);
  * mode 3 (certified) evaluates the one-log form with a degree-4 logarithm
  * together with an error bound per reference and re-evaluates the reference
  * expression wherever the bound leaves the tie-grid key open, so its output
@@ -262,7 +265,14 @@ int64_t bmnd_hip_stage_workspace(const int64_t *shape_h, const int32_t *block_h,
  * need bmnd_hip_blockmatch_standardized's null table and are refused here),
  * or -1 for the automatic choice (the count table for integer counts <= 100
  * at count_scale 1, which reads two integers back from the device, else mode
- * 4).  wiener as in bmnd_hip_filter_groups (0 HT, 1-3 Wiener gain). */
+ * 4).  An explicit mode 2 reads the count range back too, takes the table
+ * size from it and returns BMND_HIP_EINVAL unless match_vol holds
+ * non-negative integers <= 100.  wiener as in bmnd_hip_filter_groups (0 HT,
+ * 1-3 Wiener gain).  The Wiener gains come from match_vol's coefficients, so
+ * the stage driver covers the reference's default Wiener match source (the
+ * pilot, bmndalgo.py:714-720); a caller matching on the observation
+ * (poisson_wiener_match_source = observation) calls the per-kernel entry
+ * points and passes the pilot to bmnd_hip_filter_groups. */
 int32_t bmnd_hip_stage(int32_t wiener, const float *match_vol, const float *noisy,
                        const float *var, const float *fallback, const int64_t *shape_h,
                        const int32_t *block_h, const int32_t *radius_h, const int32_t *ref_lin,

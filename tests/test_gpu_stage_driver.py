@@ -90,3 +90,24 @@ def test_stage_driver_with_options():
                                            stage_arg=pipe.wiener_arg)
     assert torch.equal(w_sel, wi.sel) and torch.equal(w_cnt, wi.counts)
     assert torch.equal(w_est.view(torch.int32), wi.est.view(torch.int32))
+
+
+def test_stage_driver_explicit_count_table_mode():
+    """Mode 2 given explicitly: the driver reads the count range itself and
+    matches like the automatic choice; a non-integer input is refused."""
+    print("stage driver: explicit count-table mode", flush=True)
+    _, noisy = phantoms.ellipsoid_volume((30, 28, 26), seed=64, peak=10.0, n_ellipsoids=6,
+                                         axes_range=(3.0, 10.0))
+    prof = BM4DProfile(noise_model=NoiseModel.POISSON)
+    dev = torch.device("cuda", 0)
+    pipe = PoissonPipeline(prof, noisy.shape, dev)
+    v = torch.from_numpy(np.ascontiguousarray(noisy)).to(dev)
+    ht, _ = pipe.run(v)
+    assert pipe.last_modes["ht"] == 2
+    var1 = pipe.variance(v)
+    d_est, d_sel, d_cnt, d_acc = _driver_stage(pipe, "ht", v, v, var1, v, mode=2, stage_arg=0)
+    assert torch.equal(d_sel, ht.sel) and torch.equal(d_cnt, ht.counts)
+    assert torch.equal(d_acc, ht.accepted)
+    assert torch.equal(d_est.view(torch.int32), ht.est.view(torch.int32))
+    with pytest.raises(_lib.HipError, match="integer counts <= 100"):
+        _driver_stage(pipe, "ht", v + 0.5, v, var1, v, mode=2, stage_arg=0)

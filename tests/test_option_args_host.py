@@ -61,3 +61,22 @@ def test_filter_stage_argument(wiener):
         wiener, DUMMY, DUMMY, DUMMY, SHAPE, BLOCK, RADIUS, DUMMY, DUMMY, 10, 16, DUMMY, DUMMY,
         3.0, 0, 1.0, DUMMY, DUMMY, None)
     assert "wiener must be 0" in _error(rc)
+
+
+@pytest.mark.parametrize("table_max,scale", [(101, 1.0), (-1, 1.0), (50, 2.0)])
+def test_count_table_mode_refuses_what_its_table_cannot_hold(table_max, scale):
+    # mode 2 reads terms from a (table_max+1)^2 table of integer counts at
+    # unit count scale; a table past 100 counts or another scale is refused
+    L = _lib.lib()
+    rc = L.bmnd_hip_blockmatch(DUMMY, SHAPE, BLOCK, RADIUS, DUMMY, 10, 1.0, 16, 2, 1, 1, scale,
+                               2, table_max, DUMMY, DUMMY, DUMMY, None)
+    assert "table mode needs count_scale == 1 and 0 <= table_max <= 100" in _error(rc)
+
+
+def test_stage_driver_count_table_mode_needs_unit_scale():
+    L = _lib.lib()
+    ws = L.bmnd_hip_stage_workspace(SHAPE, BLOCK, 10, 16, 4)
+    rc = L.bmnd_hip_stage(0, DUMMY, DUMMY, DUMMY, DUMMY, SHAPE, BLOCK, RADIUS, DUMMY, 10, 1.0,
+                          16, 2, 2.0, 2, DUMMY, DUMMY, 3.0, 4, DUMMY, ws, DUMMY, DUMMY, DUMMY,
+                          DUMMY, None)
+    assert "mode 2 needs count_scale == 1" in _error(rc)
