@@ -44,6 +44,8 @@ EXPORTS = (
     "bmnd_hip_aggregate_batch",
     "bmnd_hip_stage_workspace",
     "bmnd_hip_stage",
+    "bmnd_hip_init",
+    "bmnd_hip_stage_noalloc",
 )
 
 
@@ -115,6 +117,11 @@ def _declare(L: ctypes.CDLL) -> None:
     L.bmnd_hip_stage.argtypes = [i32, vp, vp, vp, vp, vp, vp, vp, vp, i64, f32, i32, i32, f64, i32,
                                  vp, vp, f32, i32, vp, i64, vp, vp, vp, vp, vp]
     L.bmnd_hip_stage.restype = i32
+    L.bmnd_hip_init.argtypes = []
+    L.bmnd_hip_init.restype = i32
+    L.bmnd_hip_stage_noalloc.argtypes = [i32, vp, vp, vp, vp, vp, vp, vp, vp, i64, f32, i32, i32,
+                                         f64, i32, vp, vp, f32, i32, vp, vp, vp, vp, vp]
+    L.bmnd_hip_stage_noalloc.restype = i32
 
 
 def lib() -> ctypes.CDLL:

@@ -2169,6 +2169,13 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
 // ---------------------------------------------------------------------------
 // aggregation
 // ---------------------------------------------------------------------------
+// p[0..n) = v: clears flags and maps with a kernel instead of a memset node
+__global__ void k_fill_i32(int32_t *__restrict__ p, int64_t n, int32_t v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
 __global__ void k_reference_map(const int32_t *__restrict__ ref_lin, int64_t nrefs,
                                 int32_t *__restrict__ ref_index) {
   int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2467,6 +2474,9 @@ bool same3(const int32_t *x, int a0, int a1, int a2) {
 
 std::mutex g_tab_mutex;
 std::vector<double *> g_logtab;  // per device
+// per device: the address of that device's copy of `scratch`, set by
+// bmnd_hip_init (null until then)
+std::vector<unsigned char *> g_scratch_ptr;
 
 int logtab_for_device(const double **out) {
   int dev = 0;
@@ -2716,6 +2726,15 @@ int check_batch(const BatchRef &b, const int32_t *block, const int32_t *radius,
 
 int bmnd_internal_fail(int code, const char *msg) { return fail(code, msg); }
 
+// ---------------------------------------------------------------------------
+// scratch of the no-allocation stage entry point: one copy per device, held
+// until the process ends.  Host code uses the device address from
+// hipGetSymbolAddress, never this name.
+// ---------------------------------------------------------------------------
+constexpr int64_t kScratchBytes = (int64_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+
 // ===========================================================================
 // C-ABI
 // ===========================================================================
@@ -2766,7 +2785,8 @@ static int32_t blockmatch_impl(const float *vol, const int64_t *shape_h, const i
                                int32_t mode, int32_t table_max, int32_t *sel, int32_t *counts,
                                int32_t *accepted, void *stream, const BatchRef *b,
                                const float *ref_thresholds = nullptr,
-                               const float *null_mom = nullptr, int32_t null_max = 0) {
+                               const float *null_mom = nullptr, int32_t null_max = 0,
+                               void *ws = nullptr, int64_t ws_bytes = 0) {
   int c[3];
   int64_t shape1[3];
   int64_t grid_total = 0;
@@ -2854,14 +2874,17 @@ static int32_t blockmatch_impl(const float *vol, const int64_t *shape_h, const i
   const size_t dyn = mode == 2 ? sizeof(double) * (size_t)(table_max + 1) * (table_max + 1) : 0;
   // MODE 4: patch sums of c log 2c for every patch origin, in stream-ordered
   // scratch (rows of B2 voxels, then B0 x B1 of those)
+  // (in the caller's scratch `ws` when given, else stream-ordered allocations)
   double2 *rowsum = nullptr, *zsum = nullptr;
   auto release = [&]() {
+    if (ws) return;
     if (rowsum) (void)hipFreeAsync(rowsum, s);
     if (zsum) (void)hipFreeAsync(zsum, s);
   };
   auto patch_sums = [&](auto b0c, auto b1c, auto b2c) -> int32_t {
     constexpr int B0 = decltype(b0c)::value, B1 = decltype(b1c)::value, B2 = decltype(b2c)::value;
-    if (int rc = keep_pool_memory()) return rc;
+    if (!ws)
+      if (int rc = keep_pool_memory()) return rc;
     // one input, or each input of a batch in turn (its grid at grid_off)
     const int nin = b ? b->n : 1;
     int64_t rows_max = 0, grid_all = 0;
@@ -2871,8 +2894,16 @@ static int32_t blockmatch_impl(const float *vol, const int64_t *shape_h, const i
       rows_max = std::max(rows_max, sh0 * sh1 * cc2);
     }
     grid_all = b ? grid_total : (int64_t)c[0] * c[1] * c[2];
-    HIP_TRY(hipMallocAsync((void **)&rowsum, sizeof(double2) * (size_t)rows_max, s));
-    HIP_TRY(hipMallocAsync((void **)&zsum, sizeof(double2) * (size_t)grid_all, s));
+    if (ws) {
+      const int64_t rows_bytes = ((int64_t)sizeof(double2) * rows_max + 255) & ~(int64_t)255;
+      if (rows_bytes + (int64_t)sizeof(double2) * grid_all > ws_bytes)
+        return fail(BMND_HIP_ECAPACITY, "patch sums exceed the scratch buffer");
+      rowsum = reinterpret_cast<double2 *>(ws);
+      zsum = reinterpret_cast<double2 *>(static_cast<char *>(ws) + rows_bytes);
+    } else {
+      HIP_TRY(hipMallocAsync((void **)&rowsum, sizeof(double2) * (size_t)rows_max, s));
+      HIP_TRY(hipMallocAsync((void **)&zsum, sizeof(double2) * (size_t)grid_all, s));
+    }
     const unsigned blocks = 4096;
     for (int i = 0; i < nin; ++i) {
       const bmnd_hip_input *in = b ? &b->h[i] : nullptr;
@@ -3195,6 +3226,25 @@ int32_t bmnd_hip_filter_groups_batch(int32_t wiener, const float *noisy, const f
                      nullptr, stream, &b);
 }
 
+static int32_t reference_map_impl(const int32_t *ref_lin, int64_t nrefs, int32_t *ref_index,
+                                  int64_t num_patches, void *stream, bool fill_kernel) {
+  hipStream_t s = (hipStream_t)stream;
+  if (num_patches > 0) {
+    if (fill_kernel) {
+      const unsigned blocks = (unsigned)std::min<int64_t>((num_patches + 255) / 256, 4096);
+      hipLaunchKernelGGL(k_fill_i32, dim3(blocks), dim3(256), 0, s, ref_index, num_patches, -1);
+      HIP_TRY(hipGetLastError());
+    } else {
+      HIP_TRY(hipMemsetAsync(ref_index, 0xff, sizeof(int32_t) * num_patches, s));
+    }
+  }
+  if (nrefs <= 0) return 0;
+  hipLaunchKernelGGL(k_reference_map, dim3((unsigned)((nrefs + 255) / 256)), dim3(256), 0, s,
+                     ref_lin, nrefs, ref_index);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
 int32_t bmnd_hip_reference_map(const int32_t *ref_lin, int64_t nrefs, int32_t *ref_index,
                                int64_t num_patches, void *stream) {
   hipStream_t s = (hipStream_t)stream;
@@ -3250,7 +3300,8 @@ static int32_t aggregate_impl(const float *filt, const float *weights, const int
                               const int32_t *block_h, const int32_t *radius_h,
                               const float *kaiser, const float *fallback, float *est,
                               const float *group_sigma, int32_t per_group, float fill,
-                              float *sigma_map, void *stream, const BatchRef *b) {
+                              float *sigma_map, void *stream, const BatchRef *b,
+                              int32_t *over_ws = nullptr) {
   int c[3];
   int64_t shape1[3];
   int64_t ntiles = 0;
@@ -3302,9 +3353,16 @@ static int32_t aggregate_impl(const float *filt, const float *weights, const int
   a.ntiles = ntiles;
   const bool with_sigma = sigma_map != nullptr;
   hipStream_t s = (hipStream_t)stream;
-  int *d_over = nullptr;
-  HIP_TRY(hipMallocAsync((void **)&d_over, sizeof(int), s));
-  HIP_TRY(hipMemsetAsync(d_over, 0, sizeof(int), s));
+  // the capacity flag: in the caller's scratch (cleared by a kernel) when
+  // given, else a stream-ordered allocation
+  int *d_over = over_ws;
+  if (over_ws) {
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, over_ws, (int64_t)1, 0);
+    HIP_TRY(hipGetLastError());
+  } else {
+    HIP_TRY(hipMallocAsync((void **)&d_over, sizeof(int), s));
+    HIP_TRY(hipMemsetAsync(d_over, 0, sizeof(int), s));
+  }
   a.overflow = d_over;
   bool launched = false;
 #define A_CASE(b0, b1, b2, r0, r1, r2, t0, t1, t2)                                           \
@@ -3326,13 +3384,13 @@ static int32_t aggregate_impl(const float *filt, const float *weights, const int
   A_CASE(5, 5, 5, 7, 7, 7, 8, 8, 8)
 #undef A_CASE
   if (!launched) {
-    (void)hipFreeAsync(d_over, s);
+    if (!over_ws) (void)hipFreeAsync(d_over, s);
     return fail(BMND_HIP_EINVAL, "block/radius configuration not compiled into this library");
   }
   HIP_TRY(hipGetLastError());
   int h_over = 0;
   HIP_TRY(hipMemcpyAsync(&h_over, d_over, sizeof(int), hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipFreeAsync(d_over, s));
+  if (!over_ws) HIP_TRY(hipFreeAsync(d_over, s));
   HIP_TRY(hipStreamSynchronize(s));
   if (h_over) return fail(BMND_HIP_ECAPACITY, "aggregation tile reached too many groups");
   return 0;
@@ -3381,14 +3439,20 @@ int64_t bmnd_hip_stage_workspace(const int64_t *shape_h, const int32_t *block_h,
          align256(patches * (int64_t)sizeof(int32_t));
 }
 
-int32_t bmnd_hip_stage(int32_t wiener, const float *match_vol, const float *noisy,
-                       const float *var, const float *fallback, const int64_t *shape_h,
-                       const int32_t *block_h, const int32_t *radius_h, const int32_t *ref_lin,
-                       int64_t nrefs, float threshold, int32_t max_matches, int32_t min_matches,
-                       double count_scale, int32_t mode, const float *matrices,
-                       const float *kaiser, float shrink, int32_t k, void *workspace,
-                       int64_t workspace_bytes, float *est, int32_t *sel, int32_t *counts,
-                       int32_t *accepted, void *stream) {
+// One stage over the caller's workspace.  With `extra` (the no-allocation
+// entry point) the matcher's patch sums and the aggregation's capacity flag
+// live in it as well and the reference map is cleared by a kernel: no
+// allocation and no memset is issued.
+static int32_t stage_impl(int32_t wiener, const float *match_vol, const float *noisy,
+                          const float *var, const float *fallback, const int64_t *shape_h,
+                          const int32_t *block_h, const int32_t *radius_h, const int32_t *ref_lin,
+                          int64_t nrefs, float threshold, int32_t max_matches,
+                          int32_t min_matches, double count_scale, int32_t mode,
+                          const float *matrices, const float *kaiser, float shrink, int32_t k,
+                          void *workspace, int64_t workspace_bytes, float *est, int32_t *sel,
+                          int32_t *counts, int32_t *accepted, void *stream,
+                          int32_t *over_ws = nullptr, void *extra = nullptr,
+                          int64_t extra_bytes = 0) {
   const int64_t need = bmnd_hip_stage_workspace(shape_h, block_h, nrefs, max_matches, k);
   if (need < 0) return fail(BMND_HIP_EINVAL, "invalid stage geometry");
   if (!workspace || workspace_bytes < need)
@@ -3439,9 +3503,10 @@ int32_t bmnd_hip_stage(int32_t wiener, const float *match_vol, const float *nois
       }
     }
   }
-  if (int rc = bmnd_hip_blockmatch(match_vol, shape_h, block_h, radius_h, ref_lin, nrefs,
-                                   threshold, max_matches, min_matches, 1, 1, count_scale, mode,
-                                   table_max, sel, counts, accepted, stream))
+  if (int rc = blockmatch_impl(match_vol, shape_h, block_h, radius_h, ref_lin, nrefs, threshold,
+                               max_matches, min_matches, 1, 1, count_scale, mode, table_max, sel,
+                               counts, accepted, stream, nullptr, nullptr, nullptr, 0, extra,
+                               extra_bytes))
     return rc;
   if (kcap > 0)
     if (int rc = bmnd_hip_exact_std(var, shape_h, block_h, radius_h, sel, counts, nrefs,
@@ -3452,9 +3517,98 @@ int32_t bmnd_hip_stage(int32_t wiener, const float *match_vol, const float *nois
                                       matrices, exact, shrink, k, count_scale, filt, weights,
                                       stream))
     return rc;
-  if (int rc = bmnd_hip_reference_map(ref_lin, nrefs, ref_index, patches, stream)) return rc;
-  return bmnd_hip_aggregate(filt, weights, sel, counts, ref_index, nrefs, max_matches, shape_h,
-                            block_h, radius_h, kaiser, fallback, est, stream);
+  if (int rc = reference_map_impl(ref_lin, nrefs, ref_index, patches, stream, over_ws != nullptr))
+    return rc;
+  return aggregate_impl(filt, weights, sel, counts, ref_index, nrefs, max_matches, shape_h,
+                        block_h, radius_h, kaiser, fallback, est, nullptr, 0, 0.0f, nullptr,
+                        stream, nullptr, over_ws);
+}
+
+int32_t bmnd_hip_stage(int32_t wiener, const float *match_vol, const float *noisy,
+                       const float *var, const float *fallback, const int64_t *shape_h,
+                       const int32_t *block_h, const int32_t *radius_h, const int32_t *ref_lin,
+                       int64_t nrefs, float threshold, int32_t max_matches, int32_t min_matches,
+                       double count_scale, int32_t mode, const float *matrices,
+                       const float *kaiser, float shrink, int32_t k, void *workspace,
+                       int64_t workspace_bytes, float *est, int32_t *sel, int32_t *counts,
+                       int32_t *accepted, void *stream) {
+  return stage_impl(wiener, match_vol, noisy, var, fallback, shape_h, block_h, radius_h, ref_lin,
+                    nrefs, threshold, max_matches, min_matches, count_scale, mode, matrices,
+                    kaiser, shrink, k, workspace, workspace_bytes, est, sel, counts, accepted,
+                    stream);
+}
+
+int32_t bmnd_hip_init(void) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  {
+    std::lock_guard<std::mutex> lock(g_tab_mutex);
+    if ((int)g_scratch_ptr.size() > dev && g_scratch_ptr[dev]) return 0;
+  }
+  // the device copy of `scratch`: its address on this device (the array's
+  // name on the host is only a stand-in)
+  void *ptr = nullptr;
+  HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  if (!ptr) return fail(BMND_HIP_ELAUNCH, "hipGetSymbolAddress(scratch) returned null");
+  // every table the stage kernels read, built now so that later calls find
+  // them cached
+  const double *tab = nullptr;
+  if (int rc = logtab_for_device(&tab)) return rc;
+  const double2 *flog = nullptr;
+  if (int rc = fastlog_for_device(&flog, 0)) return rc;
+  if (int rc = fastlog_for_device(&flog, 1)) return rc;
+  double ratio = 0.0;
+  if (int rc = log2_ratio_for_device(&ratio)) return rc;
+  for (int t = 0; t <= kMaxTableCount; ++t) {
+    const double *terms = nullptr;
+    if (int rc = term_table_for_device(t, &terms)) return rc;
+  }
+  std::lock_guard<std::mutex> lock(g_tab_mutex);
+  if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+  g_scratch_ptr[dev] = static_cast<unsigned char *>(ptr);
+  return 0;
+}
+
+int32_t bmnd_hip_stage_noalloc(int32_t wiener, const float *match_vol, const float *noisy,
+                               const float *var, const float *fallback, const int64_t *shape_h,
+                               const int32_t *block_h, const int32_t *radius_h,
+                               const int32_t *ref_lin, int64_t nrefs, float threshold,
+                               int32_t max_matches, int32_t min_matches, double count_scale,
+                               int32_t mode, const float *matrices, const float *kaiser,
+                               float shrink, int32_t k, float *est, int32_t *sel,
+                               int32_t *counts, int32_t *accepted, void *stream) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  unsigned char *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_tab_mutex);
+    if ((int)g_scratch_ptr.size() > dev) base = g_scratch_ptr[dev];
+  }
+  if (!base)
+    return fail(BMND_HIP_EINVAL, "bmnd_hip_stage_noalloc: call bmnd_hip_init() on this device first");
+  const int64_t need = bmnd_hip_stage_workspace(shape_h, block_h, nrefs, max_matches, k);
+  if (need < 0) return fail(BMND_HIP_EINVAL, "invalid stage geometry");
+  // the fast matcher's patch sums (modes 4 and automatic): rows of the last
+  // axis, then every patch origin, two doubles each
+  int64_t sums = 0;
+  if (mode == -1 || mode == 4) {
+    const int64_t c2 = shape_h[2] - block_h[2] + 1;
+    int64_t grid = 1;
+    for (int d = 0; d < 3; ++d) grid *= shape_h[d] - block_h[d] + 1;
+    sums = align256(16 * shape_h[0] * shape_h[1] * c2) + 16 * grid;
+  }
+  const int64_t total = align256(need) + 256 + sums;
+  if (total > kScratchBytes)
+    return fail(BMND_HIP_ECAPACITY,
+                "bmnd_hip_stage_noalloc: this stage needs " + std::to_string(total) +
+                    " bytes of scratch, more than the " + std::to_string(kScratchBytes) +
+                    " the no-allocation entry point holds; use bmnd_hip_stage");
+  int32_t *over = reinterpret_cast<int32_t *>(base + align256(need));
+  void *extra = sums ? base + align256(need) + 256 : nullptr;
+  return stage_impl(wiener, match_vol, noisy, var, fallback, shape_h, block_h, radius_h, ref_lin,
+                    nrefs, threshold, max_matches, min_matches, count_scale, mode, matrices,
+                    kaiser, shrink, k, base, need, est, sel, counts, accepted, stream, over,
+                    extra, sums);
 }
 
 }  // extern "C"

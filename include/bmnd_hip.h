@@ -282,6 +282,36 @@ int32_t bmnd_hip_stage(int32_t wiener, const float *match_vol, const float *nois
                        int64_t workspace_bytes, float *est, int32_t *sel, int32_t *counts,
                        int32_t *accepted, void *stream);
 
+/* Library set-up for the current device (no counterpart in the reference):
+ * builds every table the kernels read (float64 log tables, the count-table
+ * terms for every table size 0..100, the hardware-log2 accuracy check) and
+ * looks up the device address of the no-allocation scratch.  Allocates; call
+ * it once per device before bmnd_hip_stage_noalloc, outside any region that
+ * must not allocate.  Repeated calls return at once. */
+int32_t bmnd_hip_init(void);
+
+/* bmnd_hip_stage without the workspace argument and without any allocation
+ * (no counterpart in the reference; for callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The stage's workspace, the fast matcher's patch sums
+ * and the aggregation's capacity flag live in one static device array of
+ * 64 MiB per device, held until the process ends; a stage needing more
+ * (bmnd_hip_stage_workspace() + 256 + its patch sums) returns
+ * BMND_HIP_ECAPACITY, and a device on which bmnd_hip_init() has not run
+ * returns BMND_HIP_EINVAL.  Flags and maps are cleared by kernels (no memset).
+ * Every call on one device shares that array and leaves its contents behind:
+ * calls that overlap on one device (other streams, other threads) race.
+ * Otherwise identical to bmnd_hip_stage, bit for bit; like it, it reads the
+ * count range and the capacity flag back with a stream synchronisation. */
+int32_t bmnd_hip_stage_noalloc(int32_t wiener, const float *match_vol, const float *noisy,
+                               const float *var, const float *fallback, const int64_t *shape_h,
+                               const int32_t *block_h, const int32_t *radius_h,
+                               const int32_t *ref_lin, int64_t nrefs, float threshold,
+                               int32_t max_matches, int32_t min_matches, double count_scale,
+                               int32_t mode, const float *matrices, const float *kaiser,
+                               float shrink, int32_t k, float *est, int32_t *sel,
+                               int32_t *counts, int32_t *accepted, void *stream);
+
 /* ---- batches of inputs of different shapes --------------------------------
  * No counterpart in the reference, which denoises one array per call: a batch
  * runs every input of one profile (one block / search geometry) through each
