@@ -132,6 +132,8 @@ def _declare(L: ctypes.CDLL) -> None:
     L.oracle_np_mean_f32.restype = f32
     L.oracle_poisson_global_sigma.argtypes = [vp, i64, f64]
     L.oracle_poisson_global_sigma.restype = f64
+    L.oracle_set_exact_order.argtypes = [i32]
+    L.oracle_numba_argsort.argtypes = [vp, i64, vp]
 
 
 # ---------------------------------------------------------------------------
@@ -184,6 +186,20 @@ def schedule(counts, block, step, shift_density=2.0, schedule_density=2) -> np.n
     lib().oracle_schedule_generated(ndim, _p(c), _p(b), _p(s), float(shift_density),
                                     int(schedule_density), _p(out), n)
     return out
+
+
+def numba_argsort(a) -> np.ndarray:
+    """argsort in numba's default (quicksort) order, from the C oracle."""
+    arr = np.ascontiguousarray(a, dtype=np.int64)
+    out = np.empty(arr.size, dtype=np.int32)
+    lib().oracle_numba_argsort(_p(arr), arr.size, _p(out))
+    return out
+
+
+def set_exact_order(quicksort: bool) -> None:
+    """Union-voxel entry order of the exact variance: stable (default) or
+    numba's quicksort argsort order as the reference leaves it."""
+    lib().oracle_set_exact_order(int(bool(quicksort)))
 
 
 def canonical_key(d: float) -> float:

@@ -710,6 +710,88 @@ static float *spatial_kron(int ndim, const int *block, const float *const *mats)
   return full;
 }
 
+/* Order of the entries inside one union voxel.  0 (default): increasing
+ * entry index, a stable sort.  1: the order numba's np.argsort leaves them in
+ * (its default kind, quicksort: numba/misc/quicksort.py, median-of-three
+ * partition, insertion sort below 15 elements, larger partition pushed on
+ * the stack), which is what the reference's union segmentation uses
+ * (variance.py:804).  Used to measure how far the two orders move the
+ * float32 sums. */
+static int g_exact_order = 0;
+
+void oracle_set_exact_order(int quicksort) { g_exact_order = quicksort ? 1 : 0; }
+
+static void numba_insertion_argsort(const int64_t *A, int32_t *R, int64_t low, int64_t high) {
+  if (high <= low) return;
+  for (int64_t i = low + 1; i <= high; ++i) {
+    int32_t k = R[i];
+    int64_t v = A[k];
+    int64_t j = i;
+    while (j > low && v < A[R[j - 1]]) {
+      R[j] = R[j - 1];
+      --j;
+    }
+    R[j] = k;
+  }
+}
+
+static int64_t numba_partition_argsort(const int64_t *A, int32_t *R, int64_t low, int64_t high) {
+  int64_t mid = (low + high) >> 1;
+  int32_t t;
+#define SWAPR(x, y) (t = R[x], R[x] = R[y], R[y] = t)
+  if (A[R[mid]] < A[R[low]]) SWAPR(low, mid);
+  if (A[R[high]] < A[R[mid]]) SWAPR(high, mid);
+  if (A[R[mid]] < A[R[low]]) SWAPR(low, mid);
+  int64_t pivot = A[R[mid]];
+  SWAPR(high, mid);
+  int64_t i = low, j = high - 1;
+  for (;;) {
+    while (i < high && A[R[i]] < pivot) ++i;
+    while (j >= low && pivot < A[R[j]]) --j;
+    if (i >= j) break;
+    SWAPR(i, j);
+    ++i;
+    --j;
+  }
+  SWAPR(i, high);
+#undef SWAPR
+  return i;
+}
+
+/* R = argsort(A) in numba's quicksort order (R initialised here) */
+void oracle_numba_argsort(const int64_t *A, int64_t n, int32_t *R) {
+  for (int64_t i = 0; i < n; ++i) R[i] = (int32_t)i;
+  if (n < 2) return;
+  int64_t stack_lo[100], stack_hi[100];
+  int top = 0;
+  stack_lo[0] = 0;
+  stack_hi[0] = n - 1;
+  top = 1;
+  while (top > 0) {
+    --top;
+    int64_t low = stack_lo[top], high = stack_hi[top];
+    while (high - low >= 15) {
+      int64_t i = numba_partition_argsort(A, R, low, high);
+      if (high - i > i - low) {
+        if (high > i) {
+          stack_lo[top] = i + 1;
+          stack_hi[top] = high;
+          ++top;
+        }
+        high = i - 1;
+      } else {
+        if (i > low) {
+          stack_lo[top] = low;
+          stack_hi[top] = i - 1;
+          ++top;
+        }
+        low = i + 1;
+      }
+    }
+    numba_insertion_argsort(A, R, low, high);
+  }
+}
+
 int oracle_exact_variance(const float *var_volume, int ndim,
                           const int64_t *vol_shape, const int *block, int n,
                           const int64_t *abs_pos, const float *group_mat,
@@ -752,6 +834,7 @@ int oracle_exact_variance(const float *var_volume, int ndim,
   int32_t *fill = (int32_t *)malloc(sizeof(int32_t) * (size_t)(ucells + 1));
   memcpy(fill, cnt, sizeof(int32_t) * (size_t)(ucells + 1));
   for (int e = 0; e < entries; ++e) order[fill[label[e]]++] = e;
+  if (g_exact_order == 1) oracle_numba_argsort(label, entries, order);
 
   float *S = spatial_kron(ndim, block, spatial_mats);
   int C = planes * P;

@@ -121,6 +121,14 @@ void oracle_scatter_add_sigma(float *num, float *den, float *sigma_num, int ndim
 double oracle_poisson_global_sigma(const float *var_vol, int64_t n, double floor_);
 
 /* ---- full pipeline ----------------------------------------------------- */
+/* Entry order inside one union voxel of the exact variance: 0 (default)
+ * increasing entry index; 1 numba's quicksort argsort order, as the
+ * reference's np.argsort of the union labels leaves it (variance.py:804).
+ * Process-wide; set it before a run. */
+void oracle_set_exact_order(int quicksort);
+/* R[0..n) = argsort(A) in numba's default (quicksort) order. */
+void oracle_numba_argsort(const int64_t *A, int64_t n, int32_t *R);
+
 typedef struct {
   int ndim;
   int64_t shape[ORACLE_MAX_NDIM];
