@@ -46,6 +46,31 @@ EXPORTS = (
     "bmnd_hip_stage",
     "bmnd_hip_init",
     "bmnd_hip_stage_noalloc",
+    "bmnd_hip_schedule_capacity",
+    "bmnd_hip_schedule_generated",
+    "bmnd_hip_schedule_axis_shifts",
+    # the pilot network (include/bmnd_conv.h)
+    "bmnd_conv_kpad",
+    "bmnd_conv3d_forward",
+    "bmnd_conv3d_wgrad_parts",
+    "bmnd_conv3d_wgrad",
+    "bmnd_conv_in_forward",
+    "bmnd_conv_out_forward",
+    "bmnd_conv_out_backward",
+    "bmnd_conv_small_wgrad",
+    "bmnd_conv_bias_grad",
+    "bmnd_conv_pool2",
+    "bmnd_conv_skip_grad",
+    "bmnd_conv_up_grad",
+    "bmnd_conv_mask_grad",
+    "bmnd_conv_flip_weights",
+    "bmnd_conv_mse",
+    "bmnd_adam_step",
+    "bmnd_to_bf16",
+    "bmnd_anscombe",
+    "bmnd_anscombe_moments",
+    "bmnd_pad_edge",
+    "bmnd_compare_crops",
 )
 
 
@@ -122,6 +147,47 @@ def _declare(L: ctypes.CDLL) -> None:
     L.bmnd_hip_stage_noalloc.argtypes = [i32, vp, vp, vp, vp, vp, vp, vp, vp, i64, f32, i32, i32,
                                          f64, i32, vp, vp, f32, i32, vp, vp, vp, vp, vp]
     L.bmnd_hip_stage_noalloc.restype = i32
+    L.bmnd_hip_schedule_capacity.argtypes = [i32, vp, vp, vp, f64, i32]
+    L.bmnd_hip_schedule_capacity.restype = i64
+    L.bmnd_hip_schedule_generated.argtypes = [i32, vp, vp, vp, f64, i32, vp, vp, i64, vp, vp]
+    L.bmnd_hip_schedule_generated.restype = i32
+    L.bmnd_hip_schedule_axis_shifts.argtypes = [i32, i32, i32, f64, vp]
+    L.bmnd_hip_schedule_axis_shifts.restype = i32
+    # the pilot network
+    L.bmnd_conv_kpad.argtypes = [i32]
+    L.bmnd_conv_kpad.restype = i64
+    L.bmnd_conv3d_forward.argtypes = [vp, vp, i32, i32, i32, vp, vp, i32, vp, i32, i32, i32, i32,
+                                      i32, i32, vp, vp]
+    L.bmnd_conv3d_wgrad_parts.argtypes = [i32, i32, i32, i32, i32]
+    L.bmnd_conv3d_wgrad_parts.restype = i64
+    L.bmnd_conv3d_wgrad.argtypes = [vp, vp, i32, i32, i32, vp, i32, vp, i32, vp, i32, i32, i32,
+                                    i32, vp]
+    L.bmnd_conv_in_forward.argtypes = [vp, vp, vp, vp, i32, i32, i32, i32, vp]
+    L.bmnd_conv_out_forward.argtypes = [vp, vp, vp, vp, vp, i32, i32, i32, i32, vp]
+    L.bmnd_conv_out_backward.argtypes = [vp, vp, vp, vp, i32, i32, i32, i32, vp]
+    L.bmnd_conv_small_wgrad.argtypes = [i32, vp, vp, vp, vp, vp, i32, vp, i32, i32, i32, i32, vp]
+    L.bmnd_conv_bias_grad.argtypes = [vp, vp, i32, vp, i32, vp, i64, vp]
+    L.bmnd_conv_pool2.argtypes = [vp, vp, i32, i32, i32, i32, i32, vp]
+    L.bmnd_conv_skip_grad.argtypes = [vp, i32, i32, vp, vp, vp, i32, i32, i32, i32, i32, vp]
+    L.bmnd_conv_up_grad.argtypes = [vp, i32, vp, vp, i32, i32, i32, i32, i32, vp]
+    L.bmnd_conv_mask_grad.argtypes = [vp, vp, vp, i64, vp]
+    L.bmnd_conv_flip_weights.argtypes = [vp, i32, i32, vp, vp]
+    L.bmnd_conv_mse.argtypes = [vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, f32, vp, vp]
+    L.bmnd_adam_step.argtypes = [vp, vp, vp, vp, vp, i64, f32, f32, f32, f32, i32, vp]
+    L.bmnd_to_bf16.argtypes = [vp, vp, i64, vp]
+    L.bmnd_anscombe.argtypes = [vp, vp, i64, i32, f32, f32, vp]
+    L.bmnd_anscombe_moments.argtypes = [vp, i64, vp, vp, vp]
+    L.bmnd_anscombe_moments.restype = i32
+    L.bmnd_pad_edge.argtypes = [vp, vp, i32, i32, i32, i32, i32, i32, i32, vp]
+    L.bmnd_compare_crops.argtypes = [vp, vp, vp, i32, i32, i32, i32, i32, i32, i32, i32, vp, i32,
+                                     vp]
+    for name in ("bmnd_conv3d_forward", "bmnd_conv3d_wgrad", "bmnd_conv_in_forward",
+                 "bmnd_conv_out_forward", "bmnd_conv_out_backward", "bmnd_conv_small_wgrad",
+                 "bmnd_conv_bias_grad", "bmnd_conv_pool2", "bmnd_conv_skip_grad",
+                 "bmnd_conv_up_grad", "bmnd_conv_mask_grad", "bmnd_conv_flip_weights",
+                 "bmnd_conv_mse", "bmnd_adam_step", "bmnd_to_bf16", "bmnd_anscombe",
+                 "bmnd_pad_edge", "bmnd_compare_crops"):
+        getattr(L, name).restype = i32
 
 
 def lib() -> ctypes.CDLL:

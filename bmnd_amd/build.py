@@ -1,12 +1,15 @@
-"""Build libbmnd_hip.so in-tree with hipcc for gfx950."""
+"""Build libbmnd_hip.so in-tree with hipcc for gfx950 (sources compiled in
+parallel, then linked)."""
 
 from __future__ import annotations
 
 import os
 import subprocess
+import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRCS = [os.path.join(HERE, "csrc", "bmnd_hip.hip"), os.path.join(HERE, "csrc", "bmnd_sigma.hip")]
+SRCS = [os.path.join(HERE, "csrc", "bmnd_hip.hip"), os.path.join(HERE, "csrc", "bmnd_sigma.hip"),
+        os.path.join(HERE, "csrc", "bmnd_conv.hip"), os.path.join(HERE, "csrc", "bmnd_schedule.hip")]
 OUT = os.path.join(HERE, "libbmnd_hip.so")
 
 HIPCC_FLAGS = [
@@ -14,7 +17,6 @@ HIPCC_FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     "-ffp-contract=off",
     "-mcode-object-version=5",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -23,13 +25,25 @@ HIPCC_FLAGS = [
 
 def build(verbose: bool = False) -> str:
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    tmp = OUT + ".tmp"
-    cmd = [hipcc, *HIPCC_FLAGS, "-o", tmp, *SRCS]
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        raise RuntimeError(f"hipcc failed:\n{' '.join(cmd)}\n{res.stderr}")
-    if verbose and res.stderr:
-        print(res.stderr)
+    with tempfile.TemporaryDirectory(prefix="bmnd_build_") as tmpdir:
+        objs, procs = [], []
+        for src in SRCS:
+            obj = os.path.join(tmpdir, os.path.basename(src) + ".o")
+            cmd = [hipcc, *HIPCC_FLAGS, "-c", "-o", obj, src]
+            procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE,
+                                                stderr=subprocess.PIPE, text=True)))
+            objs.append(obj)
+        for cmd, proc in procs:
+            _, err = proc.communicate()
+            if proc.returncode != 0:
+                raise RuntimeError(f"hipcc failed:\n{' '.join(cmd)}\n{err}")
+            if verbose and err:
+                print(err)
+        tmp = OUT + ".tmp"
+        cmd = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"hipcc link failed:\n{' '.join(cmd)}\n{res.stderr}")
     os.replace(tmp, OUT)
     return OUT
 

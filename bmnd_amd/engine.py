@@ -1,6 +1,6 @@
 """Device pipeline of the Poisson bmnd(): both stages on one GPU.
 
-Per stage: schedule (host, cached per shape) -> fused block matching ->
+Per stage: schedule (built on the device, kept per shape) -> fused block matching ->
 group filtering -> ordered aggregation, all on the caller's HIP stream over
 torch-allocated device buffers.  2-D inputs run as (1, H, W) volumes.
 """
@@ -32,7 +32,7 @@ from .blockmatching import (
     resolve_blockmatch_distance,
 )
 from .profiles import BMNDProfile
-from .shifts import ScheduleArrays, get_reference_schedule_arrays
+from .shifts import LazySchedule, ScheduleArrays
 from .transforms import Transform, TransformMode, TransformType, group_matrices, spatial_matrices
 from .utils import kaiser_window_nd
 
@@ -108,8 +108,13 @@ class StagePlan:
     spatial: TransformType
     beta: float
     kaiser: np.ndarray
-    schedule: ScheduleArrays | None = None
+    lazy_schedule: LazySchedule | None = None
     _dev: dict = field(default_factory=dict)
+
+    @property
+    def schedule(self) -> ScheduleArrays:
+        """The host arrays of the stage's schedule (built on first use)."""
+        return self.lazy_schedule.arrays
 
     @property
     def P(self) -> int:
@@ -213,12 +218,18 @@ def make_plan(profile: BMNDProfile, stage: str, ndim: int) -> StagePlan:
 
 
 @dataclass
+class SurrogateStages:
+    pilot: torch.Tensor       # the stage-1 pilot stage 2 ran on
+    wiener: "StageOutput"
+
+
+@dataclass
 class StageOutput:
     est: torch.Tensor
     sel: torch.Tensor
     counts: torch.Tensor
     accepted: torch.Tensor
-    schedule: ScheduleArrays
+    schedule: LazySchedule
     sigma_map: torch.Tensor | None = None  # with sigma maps requested
 
 
@@ -333,7 +344,7 @@ class PoissonPipeline:
         self.plans = {s: make_plan(profile, s, self.ndim) for s in ("ht", "wiener")}
         for plan in self.plans.values():
             counts = tuple(self.shape[d] - plan.block[d] + 1 for d in range(self.ndim))
-            plan.schedule = get_reference_schedule_arrays(
+            plan.lazy_schedule = LazySchedule(
                 counts, plan.block, plan.step, mode=profile.reference_schedule_mode,
                 shift_density=profile.reference_shift_density,
                 schedule_density=profile.reference_schedule_density)
@@ -360,10 +371,10 @@ class PoissonPipeline:
         return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
 
     def _ref_lin(self, plan: StagePlan) -> torch.Tensor:
+        """The stage's references (patch-grid linear indices), built on the
+        device on first use and kept."""
         if plan.name not in self._refs:
-            counts = tuple(self.shape[d] - plan.block[d] + 1 for d in range(self.ndim))
-            lin = np.ravel_multi_index(tuple(plan.schedule.reference_abs.T), counts)
-            self._refs[plan.name] = torch.from_numpy(lin.astype(np.int32)).to(self.device)
+            self._refs[plan.name] = plan.lazy_schedule.device_lin(self.device)
         return self._refs[plan.name]
 
     def null_moments(self) -> tuple[torch.Tensor, int]:
@@ -522,7 +533,7 @@ class PoissonPipeline:
                                               float(sigma_fill or 0.0), _ptr(smap), stream))
         timer.stop(tok)
         del filt, wts, ref_index, gsig
-        return StageOutput(est, sel.view(R, plan.max_m), cnt, acc, plan.schedule, smap)
+        return StageOutput(est, sel.view(R, plan.max_m), cnt, acc, plan.lazy_schedule, smap)
 
     def run_ht(self, vol: torch.Tensor, timer=None, sigma_maps: bool = False, on_match=None):
         """Stage 1 (bmndalgo.py:213-633) on the float32 contiguous device
@@ -552,6 +563,39 @@ class PoissonPipeline:
         match2, var2 = self._wiener_sources(vol, pilot, var1, t)
         return self.stage("wiener", match2, vol, var2, vol, timer, fill, pilot=pilot,
                           on_match=on_match)
+
+    def run_surrogate(self, vol: torch.Tensor, net, check, timer=None):
+        """Stage 2 on a network pilot (bmnd_amd.surrogate): the network's
+        pilot of `vol`, its agreement check against the plain stage 1 on
+        sub-volumes, the plain stage 1 on `vol` instead when it fails, then
+        the reference's stage 2 on the pilot kept.  Returns
+        (SurrogateStages, SurrogateInfo)."""
+        from .surrogate import SurrogateInfo, anscombe_stats
+        t = timer or _NoTimer()
+        # the count level first: E[A(x)^2] / 4 - 3/8 = E[x] for A(x) =
+        # 2 sqrt(x + 3/8); above the network's range it is not run at all
+        tok = t.start("surrogate.check")
+        a_mean, a_std = anscombe_stats(vol)
+        mean_count = (a_std * a_std + a_mean * a_mean) / 4.0 - 0.375
+        t.stop(tok)
+        pilot, dist = None, float("nan")
+        if mean_count <= check.max_mean_count:
+            tok = t.start("surrogate.network")
+            pilot = net.pilot(vol)
+            t.stop(tok)
+            tok = t.start("surrogate.check")
+            dist = check.distance(self.profile, vol, pilot)
+            t.stop(tok)
+        if pilot is None or not check.accepts(dist, mean_count):
+            ht, var1, _ = self.run_ht(vol, timer)
+            pilot, used = ht.est, "plain"
+        else:
+            tok = t.start("ht.variance")
+            var1 = self.variance(vol)
+            t.stop(tok)
+            used = "network"
+        wi = self.run_wiener(vol, pilot, var1, timer)
+        return SurrogateStages(pilot, wi), SurrogateInfo(used, dist, check.threshold, mean_count)
 
     def run(self, vol: torch.Tensor, timer=None, sigma_maps: bool = False):
         """vol: float32 contiguous device tensor of `shape`.  Returns the HT
@@ -651,26 +695,25 @@ class BatchPipeline:
         schedules, lins, inputs = [], [], []
         grid_off = tile_off = 0
         ref_off = [0]
+        dev = self.device
         for i, (shape, shape3) in enumerate(zip(self.shapes, self.shape3s)):
             counts = tuple(shape[d] - plan.block[d] + 1 for d in range(self.ndim))
-            sched = get_reference_schedule_arrays(
+            sched = LazySchedule(
                 counts, plan.block, plan.step, mode=profile.reference_schedule_mode,
                 shift_density=profile.reference_shift_density,
                 schedule_density=profile.reference_schedule_density)
             schedules.append(sched)
-            lin = np.ravel_multi_index(tuple(sched.reference_abs.T), counts).astype(np.int32)
+            lin = sched.device_lin(dev)
             lins.append(lin)
-            inputs.append(np.full(lin.size, i, dtype=np.int32))
+            inputs.append(torch.full((lin.numel(),), i, dtype=torch.int32, device=dev))
             grid3 = tuple(shape3[d] - plan.block3[d] + 1 for d in range(3))
             recs[i] = (self.vox_off[i], grid_off, tile_off, shape3, grid3)
             grid_off += int(np.prod(grid3))
             tile_off += int(np.prod([-(-shape3[d] // tile[d]) for d in range(3)]))
-            ref_off.append(ref_off[-1] + lin.size)
-        dev = self.device
+            ref_off.append(ref_off[-1] + lin.numel())
         return BatchStage(
             plan=plan, schedules=schedules,
-            ref_lin=torch.from_numpy(np.concatenate(lins)).to(dev),
-            ref_input=torch.from_numpy(np.concatenate(inputs)).to(dev),
+            ref_lin=torch.cat(lins), ref_input=torch.cat(inputs),
             ref_off=np.asarray(ref_off, dtype=np.int64), inputs_h=recs,
             inputs_d=torch.from_numpy(recs.view(np.uint8).copy()).to(dev),
             grid_total=grid_off)
@@ -807,8 +850,9 @@ class BatchPipeline:
         t.stop(tok)
         return out
 
-    def run(self, packed: torch.Tensor, timer=None) -> tuple[BatchStageOutput, BatchStageOutput]:
-        """packed: float32 device buffer of `total_voxels` (see pack())."""
+    def run_ht(self, packed: torch.Tensor, timer=None):
+        """Stage 1 of every input: (HT stage output, stage-1 variance volume,
+        None), as PoissonPipeline.run_ht."""
         if (packed.dtype != torch.float32 or not packed.is_contiguous()
                 or packed.numel() != self.total_voxels):
             raise ValueError("packed must be a contiguous float32 tensor of every input's voxels")
@@ -818,6 +862,12 @@ class BatchPipeline:
         t.stop(tok)
         ht = self.stage("ht", packed, packed, var1, packed, timer,
                         ref_thresholds=self.ht_thresholds(packed, t))
+        return ht, var1, None
+
+    def run(self, packed: torch.Tensor, timer=None) -> tuple[BatchStageOutput, BatchStageOutput]:
+        """packed: float32 device buffer of `total_voxels` (see pack())."""
+        t = timer or _NoTimer()
+        ht, var1, _ = self.run_ht(packed, timer)
         match2, var2 = self._single._wiener_sources(packed, ht.est, var1, t)
         del var1
         wi = self.stage("wiener", match2, packed, var2, packed, timer, pilot=ht.est)

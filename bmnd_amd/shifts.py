@@ -130,6 +130,82 @@ def get_reference_schedule_arrays(counts, block_size, step_size,
         "(supported: 'generated', 'off').")
 
 
+class LazySchedule:
+    """A stage schedule whose host arrays are built on first use only (for
+    callbacks, per-reference thresholds and work counts); the device
+    pipeline takes its reference list from device_schedule."""
+
+    def __init__(self, counts, block_size, step_size, mode=ReferenceScheduleMode.GENERATED,
+                 shift_density: float = 1.0, schedule_density: int = 1):
+        self.spec = (tuple(int(c) for c in counts), tuple(int(b) for b in block_size),
+                     tuple(int(x) for x in step_size), mode, float(shift_density),
+                     int(schedule_density))
+        self._arrays: ScheduleArrays | None = None
+
+    @property
+    def generated(self) -> bool:
+        return coerce_enum(self.spec[3], ReferenceScheduleMode,
+                           "reference_schedule_mode") is ReferenceScheduleMode.GENERATED
+
+    @property
+    def arrays(self) -> ScheduleArrays:
+        if self._arrays is None:
+            counts, block, step, mode, sd, dens = self.spec
+            self._arrays = get_reference_schedule_arrays(counts, block, step, mode, sd, dens)
+        return self._arrays
+
+    def __getattr__(self, name):
+        if name in ("reference_abs", "reference_shifted", "shift", "items"):
+            return getattr(self.arrays, name)
+        raise AttributeError(name)
+
+    def __len__(self) -> int:
+        return len(self.arrays)
+
+    def device_lin(self, device):
+        """The schedule's reference origins as patch-grid linear indices,
+        an int32 device tensor: built on the device in GENERATED mode
+        (bmnd_hip_schedule_generated), uploaded from the host arrays
+        otherwise."""
+        import torch
+        counts, block, step, _, sd, dens = self.spec
+        if self.generated and torch.device(device).type == "cuda":
+            return device_schedule(counts, block, step, sd, dens, device)
+        lin = np.ravel_multi_index(tuple(self.arrays.reference_abs.T), counts)
+        return torch.from_numpy(lin.astype(np.int32)).to(device)
+
+
+def device_schedule(counts, block_size, step_size, shift_density: float = 1.0,
+                    schedule_density: int = 1, device=None, with_slots: bool = False):
+    """The GENERATED schedule built on the device: int32 tensor of patch-grid
+    linear indices in the reference's order (and, with_slots, each
+    reference's candidate index slot * schedule_density + phase)."""
+    import ctypes
+
+    import torch
+
+    from . import _lib
+    nd = len(counts)
+    c = (ctypes.c_int64 * nd)(*[int(x) for x in counts])
+    b = (ctypes.c_int32 * nd)(*[int(x) for x in block_size])
+    st = (ctypes.c_int32 * nd)(*[int(x) for x in step_size])
+    L = _lib.lib()
+    cap = L.bmnd_hip_schedule_capacity(nd, c, b, st, float(shift_density), int(schedule_density))
+    if cap < 0:
+        _lib.check(-1)
+    dev = torch.device(device if device is not None else "cuda")
+    lin = torch.empty(cap, dtype=torch.int32, device=dev)
+    slot = torch.empty(cap, dtype=torch.int32, device=dev) if with_slots else None
+    n = ctypes.c_int64(0)
+    stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    _lib.check(L.bmnd_hip_schedule_generated(
+        nd, c, b, st, float(shift_density), int(schedule_density),
+        ctypes.c_void_p(lin.data_ptr()), ctypes.c_void_p(0 if slot is None else slot.data_ptr()),
+        cap, ctypes.byref(n), stream))
+    lin = lin[:n.value]
+    return (lin, slot[:n.value]) if with_slots else lin
+
+
 def get_reference_schedule(counts, block_size, step_size, mode=ReferenceScheduleMode.OFFICIAL,
                            shift_density: float = 1.0,
                            schedule_density: int = 1) -> list[ReferenceScheduleItem]:

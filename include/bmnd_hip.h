@@ -312,6 +312,27 @@ int32_t bmnd_hip_stage_noalloc(int32_t wiener, const float *match_vol, const flo
                                float shrink, int32_t k, float *est, int32_t *sel,
                                int32_t *counts, int32_t *accepted, void *stream);
 
+/* ---- the reference schedule on the device ----------------------------------
+ * Replaces shifts.get_reference_schedule in GENERATED mode
+ * (shifts.py:405-423, 519-576, with the shift tables of 229-266): the
+ * stage's reference patch origins as patch-grid linear indices, in the
+ * reference's order, built on the device (ndim <= 4; counts = shape - block
+ * + 1 per axis).  ref_lin (and, unless null, ref_slot: each reference's
+ * candidate index slot * schedule_density + phase, from which the host can
+ * rebuild its shifted position) need bmnd_hip_schedule_capacity() entries;
+ * *nrefs receives the count (the call synchronises the stream to read it). */
+int64_t bmnd_hip_schedule_capacity(int32_t ndim, const int64_t *counts_h, const int32_t *block_h,
+                                   const int32_t *step_h, double shift_density,
+                                   int32_t schedule_density);
+int32_t bmnd_hip_schedule_generated(int32_t ndim, const int64_t *counts_h, const int32_t *block_h,
+                                    const int32_t *step_h, double shift_density,
+                                    int32_t schedule_density, int32_t *ref_lin, int32_t *ref_slot,
+                                    int64_t capacity, int64_t *nrefs, void *stream);
+/* The shift table of one axis (shifts.py:229-247) into out[64]; returns its
+ * length.  Host-only. */
+int32_t bmnd_hip_schedule_axis_shifts(int32_t block, int32_t step, int32_t allow_multiple,
+                                      double density, int32_t *out);
+
 /* ---- batches of inputs of different shapes --------------------------------
  * No counterpart in the reference, which denoises one array per call: a batch
  * runs every input of one profile (one block / search geometry) through each

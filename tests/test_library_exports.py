@@ -23,9 +23,14 @@ def test_hip_library_exports_every_declared_symbol():
     assert len(names) >= 10
     for name in names:
         assert hasattr(L, name), name
-    assert set(_lib.EXPORTS) <= set(names)
+    conv = _declared(os.path.join(ROOT, "include", "bmnd_conv.h"))
+    assert len(conv) >= 15
+    for name in conv:
+        assert hasattr(L, name), name
+    assert set(_lib.EXPORTS) <= set(names) | set(conv)
     assert L.bmnd_hip_version() >= 100
     assert b"block=(4,4,4)" in L.bmnd_hip_configs()
+    assert L.bmnd_conv_kpad(16) == 448 and L.bmnd_conv_kpad(96) == 2592
 
 
 def test_oracle_library_exports_every_declared_symbol():
