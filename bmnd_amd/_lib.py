@@ -46,6 +46,10 @@ EXPORTS = (
     "bmnd_hip_stage",
     "bmnd_hip_init",
     "bmnd_hip_stage_noalloc",
+    "bmnd_hip_pipeline_create",
+    "bmnd_hip_pipeline_launch",
+    "bmnd_hip_pipeline_status",
+    "bmnd_hip_pipeline_destroy",
     "bmnd_hip_schedule_capacity",
     "bmnd_hip_schedule_generated",
     "bmnd_hip_schedule_axis_shifts",
@@ -76,6 +80,37 @@ EXPORTS = (
 
 class HipError(RuntimeError):
     pass
+
+
+class PipelineDesc(ctypes.Structure):
+    """bmnd_hip_pipeline_desc (include/bmnd_hip.h)."""
+    _fields_ = [
+        ("shape", ctypes.c_int64 * 3),
+        ("block", (ctypes.c_int32 * 3) * 2),
+        ("radius", (ctypes.c_int32 * 3) * 2),
+        ("ref_lin", ctypes.c_void_p * 2),
+        ("nrefs", ctypes.c_int64 * 2),
+        ("threshold", ctypes.c_float * 2),
+        ("max_matches", ctypes.c_int32 * 2),
+        ("min_matches", ctypes.c_int32 * 2),
+        ("mode", ctypes.c_int32 * 2),
+        ("table_max", ctypes.c_int32),
+        ("matrices", ctypes.c_void_p * 2),
+        ("kaiser", ctypes.c_void_p * 2),
+        ("shrink", ctypes.c_float * 2),
+        ("k", ctypes.c_int32),
+        ("wiener_gain", ctypes.c_int32),
+        ("count_scale", ctypes.c_double),
+        ("variance_floor", ctypes.c_double),
+        ("match_observation", ctypes.c_int32),
+        ("variance_observation", ctypes.c_int32),
+        ("input", ctypes.c_void_p),
+        ("pilot", ctypes.c_void_p),
+        ("output", ctypes.c_void_p),
+        ("sel", ctypes.c_void_p * 2),
+        ("counts", ctypes.c_void_p * 2),
+        ("accepted", ctypes.c_void_p * 2),
+    ]
 
 
 def _declare(L: ctypes.CDLL) -> None:
@@ -147,6 +182,15 @@ def _declare(L: ctypes.CDLL) -> None:
     L.bmnd_hip_stage_noalloc.argtypes = [i32, vp, vp, vp, vp, vp, vp, vp, vp, i64, f32, i32, i32,
                                          f64, i32, vp, vp, f32, i32, vp, vp, vp, vp, vp]
     L.bmnd_hip_stage_noalloc.restype = i32
+    L.bmnd_hip_pipeline_create.argtypes = [ctypes.POINTER(PipelineDesc),
+                                           ctypes.POINTER(ctypes.c_void_p)]
+    L.bmnd_hip_pipeline_create.restype = i32
+    L.bmnd_hip_pipeline_launch.argtypes = [vp, vp]
+    L.bmnd_hip_pipeline_launch.restype = i32
+    L.bmnd_hip_pipeline_status.argtypes = [vp, vp]
+    L.bmnd_hip_pipeline_status.restype = i32
+    L.bmnd_hip_pipeline_destroy.argtypes = [vp]
+    L.bmnd_hip_pipeline_destroy.restype = None
     L.bmnd_hip_schedule_capacity.argtypes = [i32, vp, vp, vp, f64, i32]
     L.bmnd_hip_schedule_capacity.restype = i64
     L.bmnd_hip_schedule_generated.argtypes = [i32, vp, vp, vp, f64, i32, vp, vp, i64, vp, vp]

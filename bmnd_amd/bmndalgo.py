@@ -51,6 +51,12 @@ def _validate_max_cache_bytes(max_cache_bytes):
 
 _PIPELINES: dict = {}
 
+# bmnd() runs both stages as one captured HIP graph (engine.StageGraph) for
+# inputs up to this many voxels, where launches and read-backs dominate, when
+# no callback or sigma map is asked for; the result is the same bit for bit.
+USE_GRAPHS = True
+GRAPH_MAX_VOXELS = 1 << 22
+
 
 def _device() -> torch.device:
     """The caller's current HIP device."""
@@ -118,6 +124,9 @@ def bmnd(volume, profile, sigma: float | None = None, sigma_psd=None,
     vol = np.ascontiguousarray(vol)
     device = _device()
     pipe = _pipeline(profile, vol.shape, device)
+    if (USE_GRAPHS and vol.size <= GRAPH_MAX_VOXELS and stage_callback is None
+            and blockmatch_callback is None and not return_sigma_map and pipe.graph_supported()):
+        return pipe.run_graph(torch.from_numpy(vol)).cpu().numpy()
     vol_t = torch.from_numpy(vol).to(device)
 
     def match_hook(stage: str):

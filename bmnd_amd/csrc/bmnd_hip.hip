@@ -3301,7 +3301,7 @@ static int32_t aggregate_impl(const float *filt, const float *weights, const int
                               const float *kaiser, const float *fallback, float *est,
                               const float *group_sigma, int32_t per_group, float fill,
                               float *sigma_map, void *stream, const BatchRef *b,
-                              int32_t *over_ws = nullptr) {
+                              int32_t *over_ws = nullptr, bool defer_check = false) {
   int c[3];
   int64_t shape1[3];
   int64_t ntiles = 0;
@@ -3388,6 +3388,8 @@ static int32_t aggregate_impl(const float *filt, const float *weights, const int
     return fail(BMND_HIP_EINVAL, "block/radius configuration not compiled into this library");
   }
   HIP_TRY(hipGetLastError());
+  // a captured pipeline reads the flag after its launch instead
+  if (over_ws && defer_check) return 0;
   int h_over = 0;
   HIP_TRY(hipMemcpyAsync(&h_over, d_over, sizeof(int), hipMemcpyDeviceToHost, s));
   if (!over_ws) HIP_TRY(hipFreeAsync(d_over, s));
@@ -3609,6 +3611,186 @@ int32_t bmnd_hip_stage_noalloc(int32_t wiener, const float *match_vol, const flo
                     nrefs, threshold, max_matches, min_matches, count_scale, mode, matrices,
                     kaiser, shrink, k, base, need, est, sel, counts, accepted, stream, over,
                     extra, sums);
+}
+
+
+// ---- the two-stage pipeline as one HIP graph --------------------------------
+struct bmnd_hip_pipeline {
+  int device = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void *ws = nullptr;          // library-owned device workspace
+  int32_t *flags = nullptr;    // [is integer, max count, HT capacity, Wiener capacity]
+  int32_t table_mode = 0;      // stage 1 runs the count table: check its precondition
+  int32_t table_max = 0;
+};
+
+int32_t bmnd_hip_pipeline_create(const bmnd_hip_pipeline_desc *d, bmnd_hip_pipeline **out) {
+  if (!d || !out) return fail(BMND_HIP_EINVAL, "pipeline: null argument");
+  *out = nullptr;
+  if (!d->input || !d->pilot || !d->output) return fail(BMND_HIP_EINVAL, "pipeline: null volume");
+  for (int st = 0; st < 2; ++st) {
+    if (d->mode[st] < 0 || d->mode[st] > 6)
+      return fail(BMND_HIP_EINVAL, "pipeline: modes must be fixed (0..6)");
+    if (!d->ref_lin[st] || d->nrefs[st] <= 0 || !d->sel[st] || !d->counts[st] ||
+        !d->accepted[st] || !d->matrices[st] || !d->kaiser[st])
+      return fail(BMND_HIP_EINVAL, "pipeline: every stage needs its schedule, outputs and tables");
+  }
+  if (d->mode[0] == 2 && (d->table_max < 0 || d->table_max > kMaxTableCount))
+    return fail(BMND_HIP_EINVAL, "pipeline: 0 <= table_max <= 100");
+  if (d->wiener_gain < 1 || d->wiener_gain > 3)
+    return fail(BMND_HIP_EINVAL, "pipeline: wiener_gain must be 1, 2 or 3");
+  if (int rc = bmnd_hip_init()) return rc;  // every table built before the capture
+  int64_t vox = 1;
+  for (int a = 0; a < 3; ++a) vox *= d->shape[a];
+  // workspace: two variance volumes, the larger stage's workspace, the fast
+  // matcher's patch sums and the flags
+  int64_t stage_bytes = 0, sums = 0;
+  for (int st = 0; st < 2; ++st) {
+    const int64_t need = bmnd_hip_stage_workspace(d->shape, d->block[st], d->nrefs[st],
+                                                  d->max_matches[st], d->k);
+    if (need < 0) return fail(BMND_HIP_EINVAL, "pipeline: invalid stage geometry");
+    stage_bytes = std::max(stage_bytes, need);
+    const int64_t c2 = d->shape[2] - d->block[st][2] + 1;
+    int64_t grid = 1;
+    for (int a = 0; a < 3; ++a) grid *= d->shape[a] - d->block[st][a] + 1;
+    sums = std::max(sums, align256(16 * d->shape[0] * d->shape[1] * c2) + 16 * grid);
+  }
+  const int64_t vbytes = align256(vox * (int64_t)sizeof(float));
+  const int64_t total = 2 * vbytes + align256(stage_bytes) + align256(sums) + 256;
+  auto *p = new bmnd_hip_pipeline();
+  auto cleanup = [&](int32_t rc) {
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->ws) (void)hipFree(p->ws);
+    delete p;
+    return rc;
+  };
+  if (hipGetDevice(&p->device) != hipSuccess || hipMalloc(&p->ws, total) != hipSuccess)
+    return cleanup(fail(BMND_HIP_ELAUNCH, "pipeline: workspace allocation failed"));
+  char *w = static_cast<char *>(p->ws);
+  float *var1 = reinterpret_cast<float *>(w);
+  float *var2 = reinterpret_cast<float *>(w + vbytes);
+  void *stage_ws = w + 2 * vbytes;
+  void *sums_ws = w + 2 * vbytes + align256(stage_bytes);
+  p->flags = reinterpret_cast<int32_t *>(w + 2 * vbytes + align256(stage_bytes) + align256(sums));
+  p->table_mode = d->mode[0] == 2;
+  p->table_max = d->table_max;
+
+  hipStream_t cs = nullptr;
+  if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess)
+    return cleanup(fail(BMND_HIP_ELAUNCH, "pipeline: stream creation failed"));
+  if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    (void)hipStreamDestroy(cs);
+    return cleanup(fail(BMND_HIP_ELAUNCH, "pipeline: stream capture failed to start"));
+  }
+  auto captured = [&]() -> int32_t {
+    // the capacity flags start at zero on every launch; the count range of
+    // the input is recorded for the table mode's precondition
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, cs, p->flags, (int64_t)4, 0);
+    if (p->table_mode)
+      if (int rc = bmnd_hip_count_range(d->input, vox, p->flags, cs)) return rc;
+    if (int rc = bmnd_hip_variance_volume(d->input, var1, vox, d->count_scale, d->variance_floor,
+                                          cs))
+      return rc;
+    for (int st = 0; st < 2; ++st) {
+      const bool wiener = st == 1;
+      const float *match = wiener && !d->match_observation ? d->pilot : d->input;
+      const float *var = wiener && !d->variance_observation ? var2 : var1;
+      float *est = wiener ? d->output : d->pilot;
+      if (wiener && !d->variance_observation)
+        if (int rc = bmnd_hip_variance_volume(d->pilot, var2, vox, d->count_scale,
+                                              d->variance_floor, cs))
+          return rc;
+      int64_t P = 1, patches = 1;
+      for (int a = 0; a < 3; ++a) {
+        P *= d->block[st][a];
+        patches *= d->shape[a] - d->block[st][a] + 1;
+      }
+      const int64_t R = d->nrefs[st], M = d->max_matches[st];
+      const int64_t kcap = std::min<int64_t>(d->k, M);
+      char *sw = static_cast<char *>(stage_ws) + align256(2 * sizeof(int32_t));
+      float *exact = kcap > 0 ? reinterpret_cast<float *>(sw) : nullptr;
+      sw += align256(R * kcap * P * (int64_t)sizeof(float));
+      float *filt = reinterpret_cast<float *>(sw);
+      sw += align256(R * M * P * (int64_t)sizeof(float));
+      float *weights = reinterpret_cast<float *>(sw);
+      sw += align256(R * M * (int64_t)sizeof(float));
+      int32_t *ref_index = reinterpret_cast<int32_t *>(sw);
+      if (int rc = blockmatch_impl(match, d->shape, d->block[st], d->radius[st], d->ref_lin[st],
+                                   R, d->threshold[st], (int32_t)M, d->min_matches[st], 1, 1,
+                                   d->count_scale, d->mode[st], st == 0 ? d->table_max : 0,
+                                   d->sel[st], d->counts[st], d->accepted[st], cs, nullptr,
+                                   nullptr, nullptr, 0, sums_ws, sums))
+        return rc;
+      if (kcap > 0)
+        if (int rc = exact_std_impl(var, d->shape, d->block[st], d->radius[st], d->sel[st],
+                                    d->counts[st], R, (int32_t)M, d->matrices[st], d->k, exact,
+                                    cs, nullptr))
+          return rc;
+      if (int rc = filter_impl(wiener ? d->wiener_gain : 0, d->input, wiener ? d->pilot : nullptr,
+                               var, d->shape, d->block[st], d->radius[st], d->sel[st],
+                               d->counts[st], R, (int32_t)M, d->matrices[st], exact,
+                               d->shrink[st], d->k, d->count_scale, filt, weights, nullptr, cs,
+                               nullptr))
+        return rc;
+      if (int rc = reference_map_impl(d->ref_lin[st], R, ref_index, patches, cs, true)) return rc;
+      if (int rc = aggregate_impl(filt, weights, d->sel[st], d->counts[st], ref_index, R,
+                                  (int32_t)M, d->shape, d->block[st], d->radius[st],
+                                  d->kaiser[st], d->input, est, nullptr, 0, 0.0f, nullptr, cs,
+                                  nullptr, p->flags + 2 + st, true))
+        return rc;
+    }
+    return 0;
+  };
+  const int32_t rc = captured();
+  hipGraph_t graph = nullptr;
+  const hipError_t ended = hipStreamEndCapture(cs, &graph);
+  (void)hipStreamDestroy(cs);
+  p->graph = graph;
+  if (rc) return cleanup(rc);
+  if (ended != hipSuccess || !graph)
+    return cleanup(fail(BMND_HIP_ELAUNCH, std::string("pipeline: capture failed: ") +
+                                              hipGetErrorString(ended)));
+  if (hipGraphInstantiate(&p->exec, graph, nullptr, nullptr, 0) != hipSuccess)
+    return cleanup(fail(BMND_HIP_ELAUNCH, "pipeline: graph instantiation failed"));
+  *out = p;
+  return 0;
+}
+
+int32_t bmnd_hip_pipeline_launch(bmnd_hip_pipeline *p, void *stream) {
+  if (!p) return fail(BMND_HIP_EINVAL, "pipeline: null handle");
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev != p->device) return fail(BMND_HIP_EINVAL, "pipeline: launched on another device");
+  HIP_TRY(hipGraphLaunch(p->exec, (hipStream_t)stream));
+  return 0;
+}
+
+int32_t bmnd_hip_pipeline_status(bmnd_hip_pipeline *p, void *stream) {
+  if (!p) return fail(BMND_HIP_EINVAL, "pipeline: null handle");
+  int32_t h[4] = {0, 0, 0, 0};
+  hipStream_t s = (hipStream_t)stream;
+  HIP_TRY(hipMemcpyAsync(h, p->flags, sizeof(h), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (p->table_mode && (!h[0] || h[1] > p->table_max))
+    return fail(BMND_HIP_EINVAL, "pipeline: the input is not integer counts <= " +
+                                     std::to_string(p->table_max) +
+                                     " (the count table it was captured with); results invalid");
+  if (h[2] || h[3]) return fail(BMND_HIP_ECAPACITY, "aggregation tile reached too many groups");
+  return 0;
+}
+
+void bmnd_hip_pipeline_destroy(bmnd_hip_pipeline *p) {
+  if (!p) return;
+  int prev = 0;
+  (void)hipGetDevice(&prev);
+  (void)hipSetDevice(p->device);
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  if (p->ws) (void)hipFree(p->ws);
+  (void)hipSetDevice(prev);
+  delete p;
 }
 
 }  // extern "C"

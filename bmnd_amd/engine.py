@@ -597,6 +597,49 @@ class PoissonPipeline:
         wi = self.run_wiener(vol, pilot, var1, timer)
         return SurrogateStages(pilot, wi), SurrogateInfo(used, dist, check.threshold, mean_count)
 
+    # ---- both stages as one HIP graph ------------------------------------------
+    def graph_supported(self) -> bool:
+        """Whether the pipeline's options run in a captured graph: the fixed
+        stage-1 threshold and the plain (not standardized) distances; sigma
+        maps and callbacks are served by the call-by-call path."""
+        return (_ht_policy(self.profile) is PoissonHTMatchPolicy.FIXED and not self.standardized)
+
+    def _graph_modes(self, vol: torch.Tensor) -> tuple[int, int, int]:
+        """(stage-1 mode, its table size, stage-2 mode) for a capture: stage 1
+        as the call-by-call path picks it for `vol`; stage 2 the mode that
+        path takes for a non-integer pilot (every mode it could take returns
+        the same lists)."""
+        m1, tmax = self.blockmatch_mode(vol, "ht")
+        code = _DISTANCE_CODES[self.distance]
+        if code != 1:
+            m2 = 3 + code
+        else:
+            m2 = {"auto": 0, "exact": 0, "one-log": 1, "auto-one-log": 1, "certified": 3,
+                  "auto-certified": 3, "fast": 4, "auto-fast": 4}[self.deviance]
+        return m1, tmax, m2
+
+    def run_graph(self, vol) -> torch.Tensor:
+        """Both stages by one launch of the captured graph (StageGraph) on
+        `vol` (a float32 array or device tensor of `shape`); returns the
+        Wiener estimate (a buffer of the graph, overwritten by its next
+        launch).  The graph is captured on the first call, and again if an
+        input breaks the count-table precondition it was captured with
+        (that launch is then repeated with the new graph)."""
+        if not self.graph_supported():
+            raise NotImplementedError("this profile's options run on the call-by-call path")
+        g = getattr(self, "_graph", None)
+        if g is None:
+            self._graph = g = StageGraph(self)
+        g.input.copy_(torch.as_tensor(vol).view(self.shape), non_blocking=False)
+        if g.handle is None:
+            g.capture(*self._graph_modes(g.input))
+        rc = g.launch()
+        if rc == _lib_einval():
+            g.capture(*self._graph_modes(g.input))
+            rc = g.launch()
+        _lib.check(rc)
+        return g.output
+
     def run(self, vol: torch.Tensor, timer=None, sigma_maps: bool = False):
         """vol: float32 contiguous device tensor of `shape`.  Returns the HT
         (pilot) and Wiener stage outputs and, with `sigma_maps`, the global
@@ -607,6 +650,86 @@ class PoissonPipeline:
         if sigma_maps:
             return ht, wi, gsig
         return ht, wi
+
+
+def _lib_einval() -> int:
+    return -1  # BMND_HIP_EINVAL
+
+
+class StageGraph:
+    """Both stages of a PoissonPipeline captured as one HIP graph
+    (bmnd_hip_pipeline_*): the graph reads `input` and writes `pilot`,
+    `output` and each stage's match lists at every launch."""
+
+    def __init__(self, pipe: PoissonPipeline):
+        self.pipe = pipe
+        dev = pipe.device
+        self.input = torch.empty(pipe.shape, dtype=torch.float32, device=dev)
+        self.pilot = torch.empty_like(self.input)
+        self.output = torch.empty_like(self.input)
+        self.lists = {}
+        for name, plan in pipe.plans.items():
+            R = pipe._ref_lin(plan).numel()
+            self.lists[name] = (torch.empty(R * plan.max_m, dtype=torch.int32, device=dev),
+                                torch.empty(R, dtype=torch.int32, device=dev),
+                                torch.empty(R, dtype=torch.int32, device=dev))
+        self.handle = None
+        self.modes = None
+
+    def capture(self, mode_ht: int, table_max: int, mode_wi: int) -> None:
+        self.release()
+        pipe, L = self.pipe, _lib.lib()
+        d = _lib.PipelineDesc()
+        d.shape[:] = pipe.shape3
+        for st, name in enumerate(("ht", "wiener")):
+            plan = pipe.plans[name]
+            d.block[st][:] = plan.block3
+            d.radius[st][:] = plan.radius3
+            refs = pipe._ref_lin(plan)
+            d.ref_lin[st] = refs.data_ptr()
+            d.nrefs[st] = refs.numel()
+            d.threshold[st] = plan.threshold
+            d.max_matches[st], d.min_matches[st] = plan.max_m, plan.min_m
+            mats, kai = plan.device_constants(pipe.device)
+            d.matrices[st], d.kaiser[st] = mats.data_ptr(), kai.data_ptr()
+            d.shrink[st] = plan.shrink
+            sel, cnt, acc = self.lists[name]
+            d.sel[st], d.counts[st], d.accepted[st] = sel.data_ptr(), cnt.data_ptr(), acc.data_ptr()
+        d.mode[0], d.mode[1], d.table_max = mode_ht, mode_wi, table_max
+        d.k, d.wiener_gain = pipe.k, pipe.wiener_arg
+        d.count_scale, d.variance_floor = pipe.count_scale, pipe.floor
+        src = lambda f: coerce_enum(getattr(pipe.profile, f), PoissonVarianceSource, f)  # noqa: E731
+        d.match_observation = int(src("poisson_wiener_match_source")
+                                  is PoissonVarianceSource.OBSERVATION)
+        d.variance_observation = int(src("poisson_variance_source_wiener")
+                                     is PoissonVarianceSource.OBSERVATION)
+        d.input, d.pilot, d.output = (self.input.data_ptr(), self.pilot.data_ptr(),
+                                      self.output.data_ptr())
+        h = ctypes.c_void_p()
+        torch.cuda.synchronize(pipe.device)
+        _lib.check(L.bmnd_hip_pipeline_create(ctypes.byref(d), ctypes.byref(h)))
+        self.handle, self.modes = h, (mode_ht, table_max, mode_wi)
+
+    def launch(self) -> int:
+        """One launch on the current stream, then its status (synchronises)."""
+        L = _lib.lib()
+        stream = self.pipe._stream()
+        rc = L.bmnd_hip_pipeline_launch(self.handle, stream)
+        if rc:
+            return rc
+        return L.bmnd_hip_pipeline_status(self.handle, stream)
+
+    def release(self) -> None:
+        if self.handle is not None:
+            torch.cuda.synchronize(self.pipe.device)
+            _lib.lib().bmnd_hip_pipeline_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
 
 
 # bmnd_hip_input (include/bmnd_hip.h): one input of a batch

@@ -312,6 +312,48 @@ int32_t bmnd_hip_stage_noalloc(int32_t wiener, const float *match_vol, const flo
                                float shrink, int32_t k, float *est, int32_t *sel,
                                int32_t *counts, int32_t *accepted, void *stream);
 
+/* ---- both stages captured as one HIP graph ---------------------------------
+ * No counterpart in the reference (which has no device): the whole two-stage
+ * Poisson pipeline of bmnd() (bmndalgo.py:213-1046) recorded once as a HIP
+ * graph and relaunched with one call, for inputs small enough that kernel
+ * launches and the per-stage read-backs dominate.  The matching modes are
+ * fixed at creation (0..6 as bmnd_hip_blockmatch; mode 2 with table_max);
+ * the library allocates the pipeline's workspace and builds its tables at
+ * creation; a launch reads desc.input and writes desc.pilot, desc.output
+ * and each stage's sel / counts / accepted, with no allocation, memset or
+ * host synchronisation.  After a launch, bmnd_hip_pipeline_status (which
+ * synchronises the stream) returns BMND_HIP_EINVAL if stage 1 ran the count
+ * table on an input it does not hold (not integer counts <= table_max: the
+ * results are not valid and the caller reruns or recreates), or
+ * BMND_HIP_ECAPACITY as bmnd_hip_aggregate. */
+typedef struct {
+  int64_t shape[3];             /* voxels per axis; images as (1, H, W) */
+  int32_t block[2][3];          /* [stage][axis], stage 0 HT, 1 Wiener */
+  int32_t radius[2][3];
+  const int32_t *ref_lin[2];    /* device: each stage's references */
+  int64_t nrefs[2];
+  float threshold[2];
+  int32_t max_matches[2], min_matches[2];
+  int32_t mode[2];              /* matching modes 0..6 */
+  int32_t table_max;            /* stage 1 in mode 2: the count table's size */
+  const float *matrices[2];     /* device, bmnd_hip_matrix_table_size layout */
+  const float *kaiser[2];       /* device, P floats */
+  float shrink[2];              /* lambda (HT), mu (Wiener) */
+  int32_t k;
+  int32_t wiener_gain;          /* 1 variance-scaled, 2 classic, 3 noise-floor */
+  double count_scale, variance_floor;
+  int32_t match_observation;    /* Wiener matching on the observation */
+  int32_t variance_observation; /* Wiener variance model from the observation */
+  const float *input;           /* device volumes */
+  float *pilot, *output;
+  int32_t *sel[2], *counts[2], *accepted[2];
+} bmnd_hip_pipeline_desc;
+typedef struct bmnd_hip_pipeline bmnd_hip_pipeline;
+int32_t bmnd_hip_pipeline_create(const bmnd_hip_pipeline_desc *desc, bmnd_hip_pipeline **out);
+int32_t bmnd_hip_pipeline_launch(bmnd_hip_pipeline *pipeline, void *stream);
+int32_t bmnd_hip_pipeline_status(bmnd_hip_pipeline *pipeline, void *stream);
+void bmnd_hip_pipeline_destroy(bmnd_hip_pipeline *pipeline);
+
 /* ---- the reference schedule on the device ----------------------------------
  * Replaces shifts.get_reference_schedule in GENERATED mode
  * (shifts.py:405-423, 519-576, with the shift tables of 229-266): the
