@@ -35,8 +35,12 @@ METRIC = "voxels/s of bmnd() on 1 MI355X, plain vs surrogate pilot, at fixed PSN
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
-# algorithmic operations per Poisson-deviance term: 2 logarithms + 8 arithmetic
-FLOPS_PER_TERM = 10
+# block matching's roof: the reference's 2 logarithms + 8 arithmetic
+# operations per deviance term, at the transcendental rate (quarter rate of
+# the 64-lane issue, 256 CUs x 16 x 2.4 GHz) and the non-packed FP32 VALU rate
+R_TRANS = 9.8e12
+R_VALU = 3.9e13
+TERM_RATE = 1.0 / max(2.0 / R_TRANS, 8.0 / R_VALU)  # deviance terms per second
 # workgroup size of the block-matching kernel per deviance mode (its last
 # template argument, part of the kernel name rocprofv3 reports)
 BM_THREADS = {0: 512, 1: 1024, 2: 512, 3: 512, 4: 512}
@@ -57,6 +61,8 @@ def parse() -> argparse.Namespace:
                     help="side of the cube (3-D) or square (2-D) the CPU oracle is timed on "
                          "(default 64 for volumes, 256 for images)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="skip the learned-pilot leg (3-D workloads)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB mebibytes (for scaling "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE readings) and exit")
@@ -122,7 +128,7 @@ def fp64_check(profile, noisy: np.ndarray, truth: np.ndarray, side: int) -> dict
     dev = torch.device("cuda", 0)
     pipe = PoissonPipeline(profile, sample.shape, dev)
     gpu = pipe.run(torch.from_numpy(sample).to(dev))[1].est.cpu().numpy()
-    threads = min(os.cpu_count() or 1, 16)
+    threads = host_threads()
     o32, _ = oracle.bmnd_poisson(sample, nthreads=threads)
     o64, _ = oracle.bmnd_poisson_f64(sample, nthreads=threads)
     return {"sample": f"{side}^{sample.ndim} centre crop",
@@ -132,9 +138,37 @@ def fp64_check(profile, noisy: np.ndarray, truth: np.ndarray, side: int) -> dict
             "psnr_fp64_db": round(psnr(o64, tsample), 4)}
 
 
+def host_threads() -> int:
+    """The host cores this process may use, at most the 16 a one-GPU share
+    of the machine provides."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def gpu_on_sample(profile, sample: np.ndarray, reps: int = 3) -> dict:
+    """The GPU timed on the CPU leg's own sample (warm, input in HBM)."""
+    import torch
+
+    from bmnd_amd.engine import PoissonPipeline
+    dev = torch.device("cuda", 0)
+    pipe = PoissonPipeline(profile, sample.shape, dev)
+    v = torch.from_numpy(np.ascontiguousarray(sample)).to(dev)
+    pipe.run(v)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        pipe.run(v)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    return {"value": float(sample.size / dt), "ms": round(1e3 * dt, 3)}
+
+
 def cpu_baseline(config: str, noisy: np.ndarray, side: int) -> dict:
     import oracle
-    threads = min(os.cpu_count() or 1, 16)
+    threads = host_threads()
     side = min(side or (64 if noisy.ndim == 3 else 256), *noisy.shape)
     if noisy.ndim == 3:
         c = [(s - side) // 2 for s in noisy.shape]
@@ -149,7 +183,7 @@ def cpu_baseline(config: str, noisy: np.ndarray, side: int) -> dict:
     oracle.bmnd_poisson(sample, nthreads=threads)
     dt = time.perf_counter() - t0
     return {"value": float(sample.size / dt), "unit": "voxels/s", "cores": threads,
-            "kind": "port", "sample": desc, "seconds": round(dt, 3)}
+            "kind": "port", "sample": desc, "seconds": round(dt, 3), "_sample": sample}
 
 
 def main() -> int:
@@ -177,8 +211,15 @@ def main() -> int:
     profile, noisy_list, truth_list, cfg = make_inputs(args.config)
     dev = torch.device("cuda", 0)
     vols = [torch.from_numpy(np.ascontiguousarray(v)).to(dev) for v in noisy_list]
-    pipe = PoissonPipeline(profile, noisy_list[0].shape, dev, deviance=args.deviance)
     voxels = sum(int(v.numel()) for v in vols)
+    # cold call: a pipeline for a shape not seen before (schedules built on
+    # the device, tables and plans uploaded) and its first two-stage call
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pipe = PoissonPipeline(profile, noisy_list[0].shape, dev, deviance=args.deviance)
+    pipe.run(vols[0])
+    torch.cuda.synchronize()
+    cold_ms = 1e3 * (time.perf_counter() - t0)
 
     last_outs = {}
 
@@ -216,45 +257,19 @@ def main() -> int:
     rooflines = {}
     for st in ("ht", "wiener"):
         plan = pipe.plans[st]
-        # block matching: the reference's deviance expression per term
-        pairs, terms = blockmatch_work(plan, noisy_list[0].shape)
-        mode = pipe.last_modes.get(st, 0)
-        if mode in (3, 4):
-            kname = "k_blockmatch_{}<{}, {}, {}, {}, {}, {}, {}>(BMArgs)".format(
-                "cert" if mode == 3 else "fast", *plan.block3, *plan.radius3, BM_THREADS[mode])
-        else:
-            kname = "k_blockmatch<{}, {}, {}, {}, {}, {}, {}, {}>(BMArgs)".format(
-                *plan.block3, *plan.radius3, mode, BM_THREADS[mode])
-        ms = per_kernel[f"{st}.blockmatch"]
-        ach = FLOPS_PER_TERM * terms / (ms * 1e-3) / 1e12
-        rooflines[f"{st}.blockmatch"] = {
-            "kernel": kname, "bound": "mfma", "achieved": round(ach, 3),
-            "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / PEAK_FP64_TFLOPS, 4),
-            "traffic": measured_traffic(args.config, args.deviance, kname),
-            "algorithmic": f"{FLOPS_PER_TERM} float64 ops x {terms} deviance terms per launch",
-            "avg_ms": round(ms, 3)}
-        # exact overlap-aware variance: float32 product-sums per entry and union voxel
+        rooflines[f"{st}.blockmatch"] = matcher_roofline(
+            plan, pipe.last_modes.get(st, 0), noisy_list[0].shape, per_kernel[f"{st}.blockmatch"],
+            measured_traffic(args.config, args.deviance, matcher_kernel(plan, pipe.last_modes.get(st, 0))))
         if f"{st}.exact_std" in per_kernel:
             out = last_outs[st]
             w = exact_std_work(plan, shape3, out.sel, out.counts, pipe.k)
-            nmax = EXACT_CONFIG[tuple(plan.block3)]
-            kname = "k_exact_std<{}, {}, {}, {}, {}, {}, {}>(ExactArgs)".format(
-                *plan.block3, *plan.radius3, nmax)
-            ms = per_kernel[f"{st}.exact_std"]
-            ach = w["flops"] / (ms * 1e-3) / 1e12
-            rooflines[f"{st}.exact_std"] = {
-                "kernel": kname, "bound": "mfma", "achieved": round(ach, 3),
-                "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(ach / PEAK_FP32_TFLOPS, 4),
-                "traffic": measured_traffic(args.config, args.deviance, kname),
-                "algorithmic": "float32 q*P*(2*n*P + 3*U) per group: {} entries, {} union "
-                               "voxels, {} ops per launch".format(
-                                   w["entries"], w["union_voxels"], w["flops"]),
-                "avg_ms": round(ms, 3)}
+            rooflines[f"{st}.exact_std"] = exact_roofline(
+                plan, w, per_kernel[f"{st}.exact_std"],
+                measured_traffic(args.config, args.deviance, exact_kernel(plan)))
     dominant = max(per_kernel, key=lambda k: per_kernel[k])
     roof = rooflines.get(dominant) or {
-        "kernel": dominant, "bound": "mfma", "achieved": None, "peak": PEAK_FP32_TFLOPS,
-        "unit": "TFLOP/s", "frac": None, "traffic": None, "avg_ms": round(per_kernel[dominant], 3)}
+        "kernel": dominant, "bound": None, "achieved": None, "peak": None, "unit": None,
+        "frac": None, "traffic": None, "avg_ms": round(per_kernel[dominant], 3)}
     work = {s: blockmatch_work(pipe.plans[s], noisy_list[0].shape) for s in ("ht", "wiener")}
 
     psnr = None
@@ -267,6 +282,10 @@ def main() -> int:
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.config, noisy_list[0], args.cpu_sample)
+        sample = cpu.pop("_sample")
+        # like for like: the GPU on the CPU leg's own sample
+        cpu["gpu_on_sample"] = gpu_on_sample(profile, sample)
+        cpu["gpu_over_cpu_on_sample"] = round(cpu["gpu_on_sample"]["value"] / cpu["value"], 1)
         # the same leg checks the GPU on the sample against the oracle's runs
         cpu["fp64_check"] = fp64_check(profile, noisy_list[0], truth_list[0], args.cpu_sample)
 
@@ -281,14 +300,135 @@ def main() -> int:
         "roofline": roof, "rooflines": rooflines, "cpu_baseline": cpu,
         "host_to_host": {"ms": round(h2h_ms, 3), "value": round(voxels / (h2h_ms * 1e-3), 1),
                          "unit": "voxels/s"},
+        "cold_call_ms": round(cold_ms, 1),
         "kernels_ms": {k: round(v, 3) for k, v in sorted(per_kernel.items())},
         "blockmatch_terms": {s: work[s][1] for s in work},
         "psnr": psnr,
     }
-    if cpu:
-        line["gpu_over_cpu"] = round(value / cpu["value"], 1)
+    if len(vols) == 1 and noisy_list[0].size <= (1 << 22):
+        line["graph"] = graph_leg(pipe, vols[0], args.steps)
+    if noisy_list[0].ndim == 3 and not args.no_surrogate:
+        line["surrogate"] = surrogate_leg(profile, noisy_list[0].shape, dev)
     print(json.dumps(line), flush=True)
     return 0
+
+
+def matcher_kernel(plan, mode) -> str:
+    if mode in (3, 4):
+        return "k_blockmatch_{}<{}, {}, {}, {}, {}, {}, {}>(BMArgs)".format(
+            "cert" if mode == 3 else "fast", *plan.block3, *plan.radius3, BM_THREADS[mode])
+    return "k_blockmatch<{}, {}, {}, {}, {}, {}, {}, {}>(BMArgs)".format(
+        *plan.block3, *plan.radius3, min(mode, 6) if mode < 5 else (5 if mode < 7 else 6),
+        BM_THREADS.get(mode, 512))
+
+
+def exact_kernel(plan) -> str:
+    return "k_exact_std<{}, {}, {}, {}, {}, {}, {}>(ExactArgs)".format(
+        *plan.block3, *plan.radius3, EXACT_CONFIG[tuple(plan.block3)])
+
+
+def matcher_roofline(plan, mode, shape, ms, traffic) -> dict:
+    """Block matching against its transcendental roof: the reference's
+    deviance (2 logarithms + 8 FP32-class operations per term) at
+    TERM_RATE, and the executed floor of the fast matcher's float pass (one
+    hardware log per term) at R_TRANS."""
+    from bmnd_amd.engine import blockmatch_work
+    _, terms = blockmatch_work(plan, shape)
+    ach = terms / (ms * 1e-3) / 1e12
+    return {"kernel": matcher_kernel(plan, mode), "bound": "transcendental",
+            "achieved": round(ach, 4), "peak": round(TERM_RATE / 1e12, 4),
+            "unit": "Tterm/s", "frac": round(ach * 1e12 / TERM_RATE, 4),
+            "one_log_floor_frac": round(terms / R_TRANS / (ms * 1e-3), 4),
+            "traffic": traffic, "algorithmic": f"{terms} deviance terms per launch "
+            "(sum over references of clipped search box x patch volume)",
+            "avg_ms": round(ms, 3)}
+
+
+def exact_roofline(plan, w, ms, traffic) -> dict:
+    """Exact-plane variance against the FP32 VALU roof (no MFMA: the
+    reference's serial float32 order fixes every multiply and add):
+    achieved from the multiply-adds 2*q*P*n*P per group, the builder's
+    count with the per-union-voxel work beside it."""
+    ach = w["flops_mac"] / (ms * 1e-3) / 1e12
+    return {"kernel": exact_kernel(plan), "bound": "valu-fp32", "achieved": round(ach, 3),
+            "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32_TFLOPS, 4),
+            "achieved_incl_union_ops": round(w["flops"] / (ms * 1e-3) / 1e12, 3),
+            "traffic": traffic,
+            "algorithmic": "float32 2*q*P*n*P per group (q = min(k, n) planes): {} flops per "
+                           "launch; with 3*q*P*U per union voxel {} ({} entries, {} union "
+                           "voxels)".format(w["flops_mac"], w["flops"], w["entries"],
+                                            w["union_voxels"]),
+            "avg_ms": round(ms, 3)}
+
+
+def graph_leg(pipe, vol, steps: int) -> dict:
+    """Both stages as one captured HIP graph against call-by-call launches
+    (same kernels, bit-identical output), per two-stage call."""
+    import torch
+    n = max(steps, 20)
+    pipe.run_graph(vol)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(n):
+        pipe.run_graph(vol)
+    torch.cuda.synchronize()
+    g_ms = 1e3 * (time.perf_counter() - t0) / n
+    t0 = time.perf_counter()
+    for _ in range(n):
+        pipe.run(vol)
+    torch.cuda.synchronize()
+    e_ms = 1e3 * (time.perf_counter() - t0) / n
+    same = bool(torch.equal(pipe.run_graph(vol).view(torch.int32),
+                            pipe.run(vol)[1].est.view(torch.int32)))
+    return {"graph_ms": round(g_ms, 3), "call_by_call_ms": round(e_ms, 3),
+            "value": round(vol.numel() / (g_ms * 1e-3), 1), "speedup": round(e_ms / g_ms, 3),
+            "bit_identical": same}
+
+
+def surrogate_leg(profile, shape, dev, peaks=(2.0, 10.0, 50.0), reps: int = 2) -> dict:
+    """The learned pilot against the plain stage 1 on the bench phantom
+    family at several count levels: per peak the two-stage time of each,
+    their ratio, the output PSNRs against the noiseless phantom, and which
+    pilot the surrogate's check kept."""
+    import torch
+
+    from bmnd_amd import surrogate as sg
+    from bmnd_amd.engine import PoissonPipeline
+    from bmnd_amd.phantoms import ellipsoid_volume, psnr
+    if not os.path.exists(sg.DEFAULT_WEIGHTS):
+        return {"skipped": "no trained network"}
+    net = sg.PilotNet.load(sg.DEFAULT_WEIGHTS, dev)
+    check = sg.PilotCheck()
+    pipe = PoissonPipeline(profile, shape, dev)
+    res = {}
+    for peak in peaks:
+        truth, noisy = ellipsoid_volume(shape, seed=0, peak=peak)
+        v = torch.from_numpy(noisy).to(dev)
+        pipe.run(v)
+        pipe.run_surrogate(v, net, check)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            _, wi = pipe.run(v)
+        torch.cuda.synchronize()
+        plain_ms = 1e3 * (time.perf_counter() - t0) / reps
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            st, info = pipe.run_surrogate(v, net, check)
+        torch.cuda.synchronize()
+        surr_ms = 1e3 * (time.perf_counter() - t0) / reps
+        p_plain = psnr(wi.est.cpu().numpy(), truth)
+        p_surr = psnr(st.wiener.est.cpu().numpy(), truth)
+        res[f"{peak:g}"] = {
+            "plain_ms": round(plain_ms, 2), "surrogate_ms": round(surr_ms, 2),
+            "value": round(truth.size / (surr_ms * 1e-3), 1),
+            "ratio": round(plain_ms / surr_ms, 4), "psnr_plain_db": round(p_plain, 3),
+            "psnr_surrogate_db": round(p_surr, 3), "psnr_loss_db": round(p_plain - p_surr, 3),
+            "pilot": info.used, "check_distance": round(info.distance, 4),
+            "mean_count": round(info.mean_count, 4)}
+    return {"network": os.path.basename(sg.DEFAULT_WEIGHTS), "params": net.num_parameters,
+            "peaks": res, "threshold": check.threshold,
+            "max_mean_count": check.max_mean_count}
 
 
 class _Prefixed:
@@ -361,24 +501,24 @@ def run_batch(args) -> int:
             terms = sum(blockmatch_work(stage.plan, x.shape, sc)[1]
                         for x, sc in zip(inputs, stage.schedules))
             ms = per_kernel[f"{tag}.{st}.blockmatch"]
-            ach = FLOPS_PER_TERM * terms / (ms * 1e-3) / 1e12
+            ach = terms / (ms * 1e-3) / 1e12
             rooflines[f"{tag}.{st}.blockmatch"] = {
-                "bound": "mfma", "achieved": round(ach, 3), "peak": PEAK_FP64_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(ach / PEAK_FP64_TFLOPS, 4), "traffic": None,
-                "algorithmic": f"{FLOPS_PER_TERM} float64 ops x {terms} deviance terms",
-                "avg_ms": round(ms, 3)}
+                "bound": "transcendental", "achieved": round(ach, 4),
+                "peak": round(TERM_RATE / 1e12, 4), "unit": "Tterm/s",
+                "frac": round(ach * 1e12 / TERM_RATE, 4), "traffic": None,
+                "algorithmic": f"{terms} deviance terms", "avg_ms": round(ms, 3)}
             if f"{tag}.{st}.exact_std" in per_kernel:
                 flops = 0
                 for i, x in enumerate(inputs):
                     r0, r1 = stage.ref_off[i], stage.ref_off[i + 1]
                     flops += exact_std_work(stage.plan, bp.shape3s[i], out.sel[r0:r1],
-                                            out.counts[r0:r1], bp._single.k)["flops"]
+                                            out.counts[r0:r1], bp._single.k)["flops_mac"]
                 ms = per_kernel[f"{tag}.{st}.exact_std"]
                 ach = flops / (ms * 1e-3) / 1e12
                 rooflines[f"{tag}.{st}.exact_std"] = {
-                    "bound": "mfma", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS,
+                    "bound": "valu-fp32", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS,
                     "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": None,
-                    "algorithmic": f"float32 q*P*(2*n*P + 3*U): {flops} ops", "avg_ms": round(ms, 3)}
+                    "algorithmic": f"float32 2*q*P*n*P: {flops} flops", "avg_ms": round(ms, 3)}
         if truths:
             from bmnd_amd.phantoms import psnr as _psnr
             outs = bp.unpack(wi.est)
@@ -387,13 +527,16 @@ def run_batch(args) -> int:
                          "out_db": round(float(np.mean([_psnr(o, t) for o, t in
                                                        zip(outs, truths)])), 3)}
     dominant = max(per_kernel, key=lambda k: per_kernel[k])
-    roof = rooflines.get(dominant) or {"bound": "mfma", "achieved": None, "peak": PEAK_FP32_TFLOPS,
-                                       "unit": "TFLOP/s", "frac": None, "traffic": None}
+    roof = rooflines.get(dominant) or {"bound": None, "achieved": None, "peak": None,
+                                       "unit": None, "frac": None, "traffic": None}
     roof = dict(roof, kernel=dominant)
     cpu = None
     if not args.no_cpu_baseline:
         first = runs[0][3][0]
         cpu = cpu_baseline(args.config, first, args.cpu_sample)
+        sample = cpu.pop("_sample")
+        cpu["gpu_on_sample"] = gpu_on_sample(runs[0][1]._single.profile, sample)
+        cpu["gpu_over_cpu_on_sample"] = round(cpu["gpu_on_sample"]["value"] / cpu["value"], 1)
     value = voxels / (ms_per_step * 1e-3)
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "voxels/s", "n_gpus": 1,
@@ -405,8 +548,6 @@ def run_batch(args) -> int:
         "kernels_ms": {k: round(v, 3) for k, v in sorted(per_kernel.items())},
         "psnr": psnr,
     }
-    if cpu:
-        line["gpu_over_cpu"] = round(value / cpu["value"], 1)
     print(json.dumps(line), flush=True)
     return 0
 

@@ -37,6 +37,7 @@ EXPORTS = (
     "bmnd_hip_poisson_global_sigma",
     "bmnd_hip_np_sum_host",
     "bmnd_hip_aggregate_tile",
+    "bmnd_hip_aggregate_force_tile",
     "bmnd_hip_blockmatch_batch",
     "bmnd_hip_exact_std_batch",
     "bmnd_hip_filter_groups_batch",
@@ -157,6 +158,8 @@ def _declare(L: ctypes.CDLL) -> None:
     L.bmnd_hip_np_sum_host.restype = f32
     L.bmnd_hip_aggregate_tile.argtypes = [vp, vp]
     L.bmnd_hip_aggregate_tile.restype = i32
+    L.bmnd_hip_aggregate_force_tile.argtypes = [i32]
+    L.bmnd_hip_aggregate_force_tile.restype = i32
     L.bmnd_hip_blockmatch_batch.argtypes = [vp, vp, vp, i32, vp, vp, vp, vp, vp, i64, f32, i32,
                                             i32, i32, i32, f64, i32, i32, vp, i32, vp, vp, vp,
                                             vp]

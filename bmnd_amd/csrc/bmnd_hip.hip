@@ -18,7 +18,9 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -2217,8 +2219,10 @@ constexpr int kAggGroupCap = 4096;
 constexpr int kAggEntryCap = 2048;
 
 template <int B0, int B1, int B2, int R0, int R1, int R2, int T0, int T1, int T2, bool SIGMA>
-__global__ __launch_bounds__(kAggThreads) void k_aggregate(AggArgs a) {
-  static_assert(T0 * T1 * T2 == kAggThreads, "tile must match the block");
+__global__ __launch_bounds__(T0 * T1 * T2) void k_aggregate(AggArgs a) {
+  // one thread per voxel of the tile
+  constexpr int NT = T0 * T1 * T2;
+  static_assert(NT % 64 == 0 && NT <= 1024, "tile must fill whole waves");
   constexpr int P = B0 * B1 * B2;
   __shared__ int s_groups[kAggGroupCap];
   __shared__ int s_ent_idx[kAggEntryCap];
@@ -2226,9 +2230,9 @@ __global__ __launch_bounds__(kAggThreads) void k_aggregate(AggArgs a) {
   __shared__ float s_ent_w[kAggEntryCap];
   __shared__ float s_k[P];
   __shared__ int s_gcount, s_take, s_nent;
-  __shared__ int s_scan[kAggThreads / 64 + 1];
+  __shared__ int s_scan[NT / 64 + 1];
   constexpr int kSub = 256;                      // entries per wave-filtering pass
-  __shared__ short s_wl[kAggThreads / 64][kSub];  // per wave: entries reaching its voxels
+  __shared__ short s_wl[NT / 64][kSub];  // per wave: entries reaching its voxels
 
   const int tid = threadIdx.x;
   // XCD-aware order: workgroups are dealt to the 8 XCDs in turn, so logical
@@ -2266,7 +2270,7 @@ __global__ __launch_bounds__(kAggThreads) void k_aggregate(AggArgs a) {
   const int t_lo0 = tx0 * T0, t_lo1 = tx1 * T1, t_lo2 = tx2 * T2;
   const int t_hi0 = min(t_lo0 + T0, a.n0) - 1, t_hi1 = min(t_lo1 + T1, a.n1) - 1,
             t_hi2 = min(t_lo2 + T2, a.n2) - 1;
-  for (int p = tid; p < P; p += kAggThreads) s_k[p] = a.kaiser[p];
+  for (int p = tid; p < P; p += NT) s_k[p] = a.kaiser[p];
   if (tid == 0) s_gcount = 0;
   __syncthreads();
   // references whose group can reach this tile
@@ -2275,7 +2279,7 @@ __global__ __launch_bounds__(kAggThreads) void k_aggregate(AggArgs a) {
   const int r_lo2 = max(t_lo2 - B2 + 1 - R2, 0), r_hi2 = min(t_hi2 + R2, a.c2 - 1);
   const int re0 = r_hi0 - r_lo0 + 1, re1 = r_hi1 - r_lo1 + 1, re2 = r_hi2 - r_lo2 + 1;
   const int reach = (re0 > 0 && re1 > 0 && re2 > 0) ? re0 * re1 * re2 : 0;
-  for (int i = tid; i < reach; i += kAggThreads) {
+  for (int i = tid; i < reach; i += NT) {
     const int i2 = i % re2, i1 = (i / re2) % re1, i0 = i / (re2 * re1);
     const int lin = ((r_lo0 + i0) * a.c1 + (r_lo1 + i1)) * a.c2 + (r_lo2 + i2);
     const int gg = a.ref_index[lin];
@@ -2292,11 +2296,11 @@ __global__ __launch_bounds__(kAggThreads) void k_aggregate(AggArgs a) {
   }
   int sortn = 1;
   while (sortn < gcount) sortn <<= 1;
-  for (int i = gcount + tid; i < sortn; i += kAggThreads) s_groups[i] = 0x7fffffff;
+  for (int i = gcount + tid; i < sortn; i += NT) s_groups[i] = 0x7fffffff;
   __syncthreads();
   for (int size = 2; size <= sortn; size <<= 1) {
     for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int t = tid; t < sortn / 2; t += kAggThreads) {
+      for (int t = tid; t < sortn / 2; t += NT) {
         const int lo = 2 * t - (t & (stride - 1));
         const int hi = lo + stride;
         const bool up = ((lo & size) == 0);
@@ -2352,7 +2356,7 @@ __global__ __launch_bounds__(kAggThreads) void k_aggregate(AggArgs a) {
       }
     }
     int total;
-    const int ex = block_exclusive_scan<kAggThreads>(mcount, s_scan, &total);
+    const int ex = block_exclusive_scan<NT>(mcount, s_scan, &total);
     // largest prefix of groups whose entries fit
     const bool fits = (j < gcount) && (ex + mcount <= kAggEntryCap);
     if (tid == 0) s_take = 0;
@@ -2477,6 +2481,9 @@ std::vector<double *> g_logtab;  // per device
 // per device: the address of that device's copy of `scratch`, set by
 // bmnd_hip_init (null until then)
 std::vector<unsigned char *> g_scratch_ptr;
+// aggregation tile chosen per (device, block, shape, references, sigma map)
+std::map<std::tuple<int, int, int, int, int64_t, int64_t, int64_t, int, int>, size_t> g_agg_tuned;
+int g_agg_forced = -1;  // bmnd_hip_aggregate_force_tile
 
 int logtab_for_device(const double **out) {
   int dev = 0;
@@ -3364,29 +3371,104 @@ static int32_t aggregate_impl(const float *filt, const float *weights, const int
     HIP_TRY(hipMemsetAsync(d_over, 0, sizeof(int), s));
   }
   a.overflow = d_over;
-  bool launched = false;
-#define A_CASE(b0, b1, b2, r0, r1, r2, t0, t1, t2)                                           \
-  if (!launched && same3(block_h, b0, b1, b2) && same3(radius_h, r0, r1, r2)) {             \
-    const int64_t tiles = b ? ntiles                                                        \
-                            : ((shape_h[0] + t0 - 1) / t0) * ((shape_h[1] + t1 - 1) / t1) * \
-                                  ((shape_h[2] + t2 - 1) / t2);                             \
-    const dim3 grid((unsigned)(8 * ((tiles + 7) / 8)));                                     \
-    if (with_sigma)                                                                         \
-      hipLaunchKernelGGL((k_aggregate<b0, b1, b2, r0, r1, r2, t0, t1, t2, true>), grid,      \
-                         dim3(kAggThreads), 0, s, a);                                       \
-    else                                                                                    \
-      hipLaunchKernelGGL((k_aggregate<b0, b1, b2, r0, r1, r2, t0, t1, t2, false>), grid,     \
-                         dim3(kAggThreads), 0, s, a);                                       \
-    launched = true;                                                                        \
+  // tile candidates of the block configuration, largest first: a batch and a
+  // capture take the first (or the tuned choice), a plain call on a new
+  // geometry times them all once on its own data (each writes the same
+  // estimate bit for bit) and keeps the fastest
+  using AggLaunch = void (*)(const AggArgs &, bool, hipStream_t, const int64_t *shape, int64_t ntiles_batch);
+  struct AggTile {
+    int t0, t1, t2;
+    AggLaunch fn;
+  };
+  std::vector<AggTile> tiles;
+#define A_TILE(b0, b1, b2, r0, r1, r2, t0, t1, t2)                                            \
+  tiles.push_back({t0, t1, t2, [](const AggArgs &aa, bool sig, hipStream_t ss, const int64_t *sh, \
+                                  int64_t ntb) {                                               \
+    const int64_t nt = ntb > 0 ? ntb                                                          \
+                               : ((sh[0] + t0 - 1) / t0) * ((sh[1] + t1 - 1) / t1) *          \
+                                     ((sh[2] + t2 - 1) / t2);                                 \
+    const dim3 grid((unsigned)(8 * ((nt + 7) / 8)));                                          \
+    if (sig)                                                                                  \
+      hipLaunchKernelGGL((k_aggregate<b0, b1, b2, r0, r1, r2, t0, t1, t2, true>), grid,       \
+                         dim3(t0 * t1 * t2), 0, ss, aa);                                       \
+    else                                                                                      \
+      hipLaunchKernelGGL((k_aggregate<b0, b1, b2, r0, r1, r2, t0, t1, t2, false>), grid,      \
+                         dim3(t0 * t1 * t2), 0, ss, aa);                                       \
+  }})
+  if (same3(block_h, 1, 8, 8) && same3(radius_h, 0, 19, 19)) {
+    A_TILE(1, 8, 8, 0, 19, 19, 1, 16, 32);
+    A_TILE(1, 8, 8, 0, 19, 19, 1, 16, 16);
+    A_TILE(1, 8, 8, 0, 19, 19, 1, 8, 16);
+  } else if (same3(block_h, 4, 4, 4) && same3(radius_h, 7, 7, 7)) {
+    A_TILE(4, 4, 4, 7, 7, 7, 8, 8, 8);
+    A_TILE(4, 4, 4, 7, 7, 7, 4, 8, 8);
+    A_TILE(4, 4, 4, 7, 7, 7, 4, 4, 8);
+  } else if (same3(block_h, 5, 5, 5) && same3(radius_h, 7, 7, 7)) {
+    A_TILE(5, 5, 5, 7, 7, 7, 8, 8, 8);
+    A_TILE(5, 5, 5, 7, 7, 7, 4, 8, 8);
+    A_TILE(5, 5, 5, 7, 7, 7, 4, 4, 8);
   }
-  A_CASE(1, 8, 8, 0, 19, 19, 1, 16, 32)
-  A_CASE(4, 4, 4, 7, 7, 7, 8, 8, 8)
-  A_CASE(5, 5, 5, 7, 7, 7, 8, 8, 8)
-#undef A_CASE
-  if (!launched) {
+#undef A_TILE
+  if (tiles.empty()) {
     if (!over_ws) (void)hipFreeAsync(d_over, s);
     return fail(BMND_HIP_EINVAL, "block/radius configuration not compiled into this library");
   }
+  size_t pick = 0;
+  int forced = -1;
+  {
+    std::lock_guard<std::mutex> lock(g_tab_mutex);
+    forced = g_agg_forced;
+  }
+  if (forced >= 0 && !b) {
+    pick = std::min<size_t>((size_t)forced, tiles.size() - 1);
+  } else if (!b) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    const auto key = std::make_tuple(dev, block_h[0], block_h[1], block_h[2], shape_h[0],
+                                     shape_h[1], shape_h[2], (int)nrefs, (int)with_sigma);
+    bool known = false;
+    {
+      std::lock_guard<std::mutex> lock(g_tab_mutex);
+      auto it = g_agg_tuned.find(key);
+      if (it != g_agg_tuned.end()) {
+        pick = it->second;
+        known = true;
+      }
+    }
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    HIP_TRY(hipStreamIsCapturing(s, &cap));
+    if (!known && cap == hipStreamCaptureStatusNone && tiles.size() > 1) {
+      hipEvent_t e0, e1;
+      HIP_TRY(hipEventCreate(&e0));
+      HIP_TRY(hipEventCreate(&e1));
+      float best = 1e30f;
+      for (int rep = 0; rep < 2; ++rep)
+        for (size_t t = 0; t < tiles.size(); ++t) {
+          HIP_TRY(hipEventRecord(e0, s));
+          tiles[t].fn(a, with_sigma, s, shape_h, 0);
+          HIP_TRY(hipGetLastError());
+          HIP_TRY(hipEventRecord(e1, s));
+          HIP_TRY(hipEventSynchronize(e1));
+          float ms = 0.f;
+          HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+          if (rep == 1 && ms < best) {
+            best = ms;
+            pick = t;
+          }
+        }
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      {
+        std::lock_guard<std::mutex> lock(g_tab_mutex);
+        g_agg_tuned[key] = pick;
+      }
+      // the capacity flag reports the chosen tile's launch only
+      hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, d_over, (int64_t)1, 0);
+      HIP_TRY(hipGetLastError());
+    }
+  }
+  // (a batch lays its tiles out with the first candidate, bmnd_hip_aggregate_tile)
+  tiles[pick].fn(a, with_sigma, s, shape_h, b ? ntiles : 0);
   HIP_TRY(hipGetLastError());
   // a captured pipeline reads the flag after its launch instead
   if (over_ws && defer_check) return 0;
@@ -3395,6 +3477,13 @@ static int32_t aggregate_impl(const float *filt, const float *weights, const int
   if (!over_ws) HIP_TRY(hipFreeAsync(d_over, s));
   HIP_TRY(hipStreamSynchronize(s));
   if (h_over) return fail(BMND_HIP_ECAPACITY, "aggregation tile reached too many groups");
+  return 0;
+}
+
+int32_t bmnd_hip_aggregate_force_tile(int32_t index) {
+  if (index < -1) return fail(BMND_HIP_EINVAL, "tile index must be -1 (tuned) or >= 0");
+  std::lock_guard<std::mutex> lock(g_tab_mutex);
+  g_agg_forced = index;
   return 0;
 }
 
@@ -3680,10 +3769,8 @@ int32_t bmnd_hip_pipeline_create(const bmnd_hip_pipeline_desc *d, bmnd_hip_pipel
   hipStream_t cs = nullptr;
   if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess)
     return cleanup(fail(BMND_HIP_ELAUNCH, "pipeline: stream creation failed"));
-  if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-    (void)hipStreamDestroy(cs);
-    return cleanup(fail(BMND_HIP_ELAUNCH, "pipeline: stream capture failed to start"));
-  }
+  // the sequence below runs once eagerly first, which settles the tuned
+  // launch shapes, then once under capture
   auto captured = [&]() -> int32_t {
     // the capacity flags start at zero on every launch; the count range of
     // the input is recorded for the table mode's precondition
@@ -3743,6 +3830,16 @@ int32_t bmnd_hip_pipeline_create(const bmnd_hip_pipeline_desc *d, bmnd_hip_pipel
     }
     return 0;
   };
+  if (int32_t rc = captured()) {
+    (void)hipStreamSynchronize(cs);
+    (void)hipStreamDestroy(cs);
+    return cleanup(rc);
+  }
+  if (hipStreamSynchronize(cs) != hipSuccess ||
+      hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    (void)hipStreamDestroy(cs);
+    return cleanup(fail(BMND_HIP_ELAUNCH, "pipeline: stream capture failed to start"));
+  }
   const int32_t rc = captured();
   hipGraph_t graph = nullptr;
   const hipError_t ended = hipStreamEndCapture(cs, &graph);

@@ -255,7 +255,8 @@ def exact_std_work(plan: StagePlan, shape3: tuple[int, int, int], sel: torch.Ten
     by the union of its patches and q = min(k, n) exact planes, the
     overlap-aware variance sum_v var_v (sum_(i,p) G[q,i] S[s,p])^2 costs
     q*P*(2*n*P + 3*U) floating-point operations (a multiply and an add per
-    entry and coefficient, two multiplies and an add per union voxel)."""
+    entry and coefficient, two multiplies and an add per union voxel);
+    `flops_mac` counts the multiply-adds alone, 2*q*P*n*P per group."""
     b0, b1, b2 = plan.block3
     P = b0 * b1 * b2
     n0, n1, n2 = shape3
@@ -270,6 +271,7 @@ def exact_std_work(plan: StagePlan, shape3: tuple[int, int, int], sel: torch.Ten
     entries = 0
     union = 0
     flops = 0
+    flops_pairs = 0
     for a in range(0, sel2.shape[0], chunk):
         s = sel2[a:a + chunk]
         n = cnt[a:a + chunk]
@@ -286,7 +288,9 @@ def exact_std_work(plan: StagePlan, shape3: tuple[int, int, int], sel: torch.Ten
         entries += int((n * P).sum())
         union += int(u.sum())
         flops += int((q * P * (2 * n * P + 3 * u)).sum())
-    return {"entries": entries, "union_voxels": union, "flops": flops}
+        flops_pairs += int((2 * q * P * n * P).sum())
+    return {"entries": entries, "union_voxels": union, "flops": flops,
+            "flops_mac": flops_pairs}
 
 
 def _threshold_options(profile: BMNDProfile) -> tuple:

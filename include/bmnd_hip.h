@@ -398,8 +398,16 @@ typedef struct {
  * input's result equals that of the single-input call on it alone. */
 
 /* Aggregation tile of a block configuration (tile[3], voxels per axis), for
- * bmnd_hip_input.tile_off: an input has prod(ceil(shape[d] / tile[d])) tiles. */
+ * bmnd_hip_input.tile_off: an input has prod(ceil(shape[d] / tile[d])) tiles.
+ * (A single-input aggregation picks among three tiles of 512, 256 and 128
+ * voxels by timing each once per device, block configuration and geometry;
+ * every tile sums each voxel's contributions in the same order, so the
+ * estimate is the same bit for bit.  Batches use the first.) */
 int32_t bmnd_hip_aggregate_tile(const int32_t *block_h, int32_t *tile);
+/* Diagnostic control of that choice, process-wide: index 0..2 forces a tile
+ * of the candidates (512, 256, 128 voxels) for later single-input calls, -1
+ * (the default) lets the tuner choose. */
+int32_t bmnd_hip_aggregate_force_tile(int32_t index);
 
 /* bmnd_hip_blockmatch over a batch (vol: packed volumes); ref_thresholds as
  * in bmnd_hip_blockmatch_thresholds, or NULL for the scalar threshold; modes

@@ -103,3 +103,26 @@ def test_bmnd_through_the_graph_equals_call_by_call(monkeypatch):
     c = bmnd(x, BM3DProfile(noise_model=NoiseModel.POISSON))
     assert np.array_equal(a.view(np.int32), c.view(np.int32))
     assert np.array_equal(b.view(np.int32), c.view(np.int32))
+
+
+@pytest.mark.parametrize("kind", ["image", "volume"])
+def test_every_aggregation_tile_gives_the_same_estimate(kind):
+    """The aggregation tiles the tuner chooses among (512, 256 and 128
+    voxels) sum each voxel in the same order: identical estimates."""
+    print(f"aggregation tiles: {kind}", flush=True)
+    from bmnd_amd import _lib
+    prof, inputs = _inputs(kind)
+    pipe = PoissonPipeline(prof, inputs[0].shape, DEV)
+    v = torch.from_numpy(np.ascontiguousarray(inputs[0])).to(DEV)
+    L = _lib.lib()
+    outs = []
+    try:
+        for t in (0, 1, 2):
+            _lib.check(L.bmnd_hip_aggregate_force_tile(t))
+            ht, wi = pipe.run(v)
+            outs.append((ht.est.clone(), wi.est.clone()))
+    finally:
+        _lib.check(L.bmnd_hip_aggregate_force_tile(-1))
+    ht, wi = pipe.run(v)  # tuned
+    for a, b in outs:
+        assert _same(a, ht.est) and _same(b, wi.est)
