@@ -170,18 +170,52 @@ __device__ __forceinline__ double log_deg4(double x, const double2 *__restrict__
 }
 
 // tie grid of blockmatching.py:957-964: q = 32*eps32*2^floor(log2(max(|d|,1)))
+// = 2^(e-19) with max(|d|,1) = f*2^e, f in [0.5,1), and the key
+// float32(rint(d/q)*q) in float64.  In float32 the same two steps are exact
+// scalings by powers of two: d*2^(19-e) lies in [0, 2^19] (no rounding,
+// and rintf there is the float64 rint), and rint(.)*2^(e-19) is exact or
+// overflows to the infinity the float64 product rounds to -- the same bits
+// for every float32 d (checked on the host over all 2^32 patterns).
 __device__ __forceinline__ float canonical_key(float d) {
-  double mag = fabs((double)d);
-  if (mag < 1.0) mag = 1.0;
-  int e;
-  frexp(mag, &e);  // mag = f*2^e, f in [0.5,1) -> floor(log2 mag) = e-1
-  double q = ldexp(0x1p-18, e - 1);
-  return (float)(rint((double)d / q) * q);
+  float mag = fabsf(d);
+  if (mag < 1.0f) mag = 1.0f;
+  const int e = __builtin_amdgcn_frexp_expf(mag);
+  return ldexpf(rintf(ldexpf(d, 19 - e)), e - 19);
 }
 
 // ---------------------------------------------------------------------------
 // small block-level helpers
 // ---------------------------------------------------------------------------
+
+// Cross-lane reads within a wave64 whose source-lane address is computed at
+// the call: the empty asm makes the address opaque, so the compiler does not
+// keep one address register per shuffle distance alive across a kernel (it
+// spilled them to scratch in the matchers).  wave_shfl_up reads lane
+// (l - o) mod 64; callers ignore what lanes below o read.
+__device__ __forceinline__ int wave_lane() {
+  return (int)__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+__device__ __forceinline__ int wave_addr(int lane) {
+  int addr = (lane & 63) << 2;
+  asm volatile("" : "+v"(addr));
+  return addr;
+}
+__device__ __forceinline__ int wave_shfl_xor(int v, int o) {
+  return __builtin_amdgcn_ds_bpermute(wave_addr(wave_lane() ^ o), v);
+}
+__device__ __forceinline__ float wave_shfl_xor(float v, int o) {
+  return __int_as_float(wave_shfl_xor(__float_as_int(v), o));
+}
+__device__ __forceinline__ double wave_shfl_xor(double v, int o) {
+  const int addr = wave_addr(wave_lane() ^ o);
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_ds_bpermute(addr, (int)(unsigned)b);
+  const unsigned hi = (unsigned)__builtin_amdgcn_ds_bpermute(addr, (int)(unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ int wave_shfl_up(int v, int o) {
+  return __builtin_amdgcn_ds_bpermute(wave_addr(wave_lane() - o), v);
+}
 
 template <int NT>
 __device__ int block_exclusive_scan(int v, int *s_tmp, int *total) {
@@ -190,7 +224,7 @@ __device__ int block_exclusive_scan(int v, int *s_tmp, int *total) {
   int x = v;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
-    int y = __shfl_up(x, o, 64);
+    int y = wave_shfl_up(x, o);
     if (lane >= o) x += y;
   }
   if (lane == 63) s_tmp[wid] = x;
@@ -336,6 +370,28 @@ __device__ __forceinline__ float bm_key(double d, const BMArgs &a, bool &ok) {
   return key;
 }
 
+// A key on a known side of the key of d: up == false gives a key at most
+// bm_key(d), up == true one at least bm_key(d), for the bounds of the
+// certified matchers.  d/P is replaced by the product with a reciprocal moved
+// by 2^-50 to that side: fl(d * fl((1 - 2^-50)/P)) <= (d/P) (1 - 2^-50)
+// (1 + 2^-53)^2 < d/P (and the mirror for up), and the float32 rounding and
+// the tie grid are non-decreasing.
+template <int P>
+__device__ __forceinline__ float bm_key_bound(double d, const BMArgs &a, bool up, bool &ok) {
+  constexpr double c_lo = (1.0 - 0x1p-50) / P, c_hi = (1.0 + 0x1p-50) / P;
+  if (d < 0.0) d = 0.0;
+  const float d32 = (float)(d * (up ? c_hi : c_lo));
+  float key;
+  if (a.stabilize) {
+    key = canonical_key(d32);
+    ok = key < a.thr_key;
+  } else {
+    key = d32;
+    ok = d32 < a.threshold;
+  }
+  return key;
+}
+
 // MODE 6: the standardized distance is signed and not divided by the patch
 // size (blockmatching.py:374-426); same tie grid and threshold rule
 __device__ __forceinline__ float bm_key_signed(float d32, const BMArgs &a, bool &ok) {
@@ -411,8 +467,9 @@ __device__ __forceinline__ auto bm_box_keys(KeyOf key_of, int box) {
 // (key & mask) <= prefix.  The candidate index is below 2^16, so bits 16..31
 // are zero in every key and their passes are skipped; once the bin holding
 // the K-th key is needed in full the select stops with the bits resolved so
-// far.
-template <int NT, int BOX, class Each>
+// far.  STOP > 0 resolves only the bits from STOP up: the K smallest then
+// all satisfy (key & mask) <= prefix, which is what a cutoff needs.
+template <int NT, int BOX, int STOP = 0, class Each>
 __device__ __forceinline__ void bm_radix_select(Each each, int K, const BMSelScratch &sc,
                                                 unsigned long long &prefix_out,
                                                 unsigned long long &mask_out) {
@@ -420,7 +477,7 @@ __device__ __forceinline__ void bm_radix_select(Each each, int K, const BMSelScr
   const int tid = threadIdx.x;
   unsigned long long prefix = 0ull, mask = 0ull;
   int krem = K;
-  for (int shift = 56; shift >= 0; shift -= 8) {
+  for (int shift = 56; shift >= STOP; shift -= 8) {
     if (shift == 24 || shift == 16) {
       mask |= 255ull << shift;
       continue;
@@ -438,7 +495,7 @@ __device__ __forceinline__ void bm_radix_select(Each each, int K, const BMSelScr
       int incl = tot;
 #pragma unroll
       for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(incl, o, 64);
+        const int y = wave_shfl_up(incl, o);
         if (tid >= o) incl += y;
       }
       int ex = incl - tot;
@@ -473,7 +530,7 @@ __device__ __forceinline__ void bm_finalize(int K, int acc, const BMArgs &a, int
 // the wave must call it).
 __device__ __forceinline__ void wave_atomic_add(int *dst, int v) {
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  for (int o = 32; o > 0; o >>= 1) v += wave_shfl_xor(v, o);
   if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, v);
 }
 
@@ -993,8 +1050,8 @@ __global__ __launch_bounds__(NT) void k_blockmatch_cert(BMArgs a) {
     const double zsum = __longlong_as_double((long long)keys[cand]);
     const double df = 2.0 * (rsum + (zsum + acc));
     bool ok_lo, ok_hi, ok;
-    const float k_lo = bm_key<P>(df - err, a, ok_lo);
-    const float k_hi = bm_key<P>(df + err, a, ok_hi);
+    const float k_lo = bm_key_bound<P>(df - err, a, false, ok_lo);
+    const float k_hi = bm_key_bound<P>(df + err, a, true, ok_hi);
     float key;
     if (k_lo == k_hi) {
       // the reference value lies in [df - err, df + err], where the key is constant
@@ -1122,23 +1179,29 @@ __global__ void k_clog_patch(const double2 *__restrict__ rows, int n1, int c0, i
 template <int B0, int B1, int B2, int R0, int R1, int R2, int NT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_blockmatch_fast(
     BMArgs a) {
-  constexpr int S0 = 2 * R0 + B0, S1 = 2 * R1 + B1, S2 = 2 * R2 + B2;
+  // Candidates are indexed over the unclipped search box W0 x W1 x W2
+  // anchored at o - R (the C order of that index is the C order of the
+  // clipped box, so the composites sort alike) and those outside the patch
+  // grid are masked; every index decode is by a compile-time extent.
+  constexpr int W0 = 2 * R0 + 1, W1 = 2 * R1 + 1, W2 = 2 * R2 + 1;
+  constexpr int S0 = W0 + B0 - 1, S1 = W1 + B1 - 1, S2 = W2 + B2 - 1;
   constexpr int SLAB = S0 * S1 * S2;
-  constexpr int BOX = (2 * R0 + 1) * (2 * R1 + 1) * (2 * R2 + 1);
+  constexpr int BOX = W0 * W1 * W2;
   constexpr int P = B0 * B1 * B2;
   constexpr int KC = (BOX + NT - 1) / NT;  // candidates per thread
   constexpr int XCAP = 256;                // exact-set composites kept for the final rank
   constexpr int LCAP = 256;                // exact-evaluation list (candidate, patch sum)
   constexpr int KX = XCAP > LCAP + LCAP / 2 ? XCAP : LCAP + LCAP / 2;
+  constexpr int CENTER = (R0 * W1 + R1) * W2 + R2;
   static_assert(KC <= 32, "per-thread candidate mask");
-  __shared__ float s_slab[SLAB];           // max(x, 0)
+  __shared__ float s_slab[SLAB];           // max(x, 0), zero outside the volume
   __shared__ float s_klo[BOX];             // lower-bound key, then the final key
   __shared__ double s_kbuf[KX];            // exact-evaluation list, then the exact set
   __shared__ float s_reff[P];              // max(r, 0)
   __shared__ double s_tab[kLogTabDoubles];
   __shared__ double2 s_clog[kFastLogN];    // degree-4 log
   __shared__ float s_wmax[2][NT / 64];     // per-wave maxima of slab and reference values
-  __shared__ double s_errc, s_errf, s_wb;
+  __shared__ double s_errc, s_wb;
   __shared__ int s_acc, s_maybe, s_nex;
   __shared__ int s_scan[NT / 64 + 1];
   BM_SEL_SCRATCH(sc);
@@ -1147,21 +1210,57 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
   const int lane = tid & 63;
   const int64_t g = blockIdx.x;
   if (g >= a.nrefs) return;
+  // the thread index through an empty asm: each phase re-forms its slots'
+  // candidate indices from it instead of the compiler keeping KC of them
+  // (as 64-bit composite halves) live through the kernel
+  auto fresh_tid = [&]() {
+    int t = tid;
+    asm volatile("" : "+v"(t));
+    return t;
+  };
   bind_input(a, g);
   const int lin = a.ref_lin[g];
   const int o2 = lin % a.c2, o1 = (lin / a.c2) % a.c1, o0 = lin / (a.c2 * a.c1);
-  const int lo0 = max(o0 - R0, 0), hi0 = min(o0 + R0, a.c0 - 1);
-  const int lo1 = max(o1 - R1, 0), hi1 = min(o1 + R1, a.c1 - 1);
-  const int lo2 = max(o2 - R2, 0), hi2 = min(o2 + R2, a.c2 - 1);
-  const int e0 = hi0 - lo0 + 1, e1 = hi1 - lo1 + 1, e2 = hi2 - lo2 + 1;
-  const int box = e0 * e1 * e2;
+  const int a0 = o0 - R0, a1 = o1 - R1, a2 = o2 - R2;  // box anchor
+  // valid box coordinates j: a + j inside the patch grid
+  const int jl0 = max(-a0, 0), jh0 = min(W0 - 1, a.c0 - 1 - a0);
+  const int jl1 = max(-a1, 0), jh1 = min(W1 - 1, a.c1 - 1 - a1);
+  const int jl2 = max(-a2, 0), jh2 = min(W2 - 1, a.c2 - 1 - a2);
+  const int box = (jh0 - jl0 + 1) * (jh1 - jl1 + 1) * (jh2 - jl2 + 1);
+  auto in_box = [&](int j0, int j1, int j2) {
+    return j0 >= jl0 && j0 <= jh0 && j1 >= jl1 && j1 <= jh1 && j2 >= jl2 && j2 <= jh2;
+  };
+  auto valid_cand = [&](int c) {
+    return c < BOX && in_box(c / (W1 * W2), (c / W2) % W1, c % W2);
+  };
 
   for (int i = tid; i < kLogTabDoubles; i += NT) s_tab[i] = a.logtab[i];
   for (int i = tid; i < kFastLogN; i += NT) s_clog[i] = a.certlog[i];
-  const int x0 = e0 + B0 - 1, x1 = e1 + B1 - 1, x2 = e2 + B2 - 1;
   float vmax = 0.0f;
-  {
-    // every load in flight before the first store
+  // every load in flight before the first store; the in-volume part of the
+  // anchored slab is what the candidates inside the grid read
+  constexpr int PLANE = S1 * S2;
+  if constexpr (PLANE <= NT) {
+    // one thread per (i1, i2) of a plane, down all S0 planes
+    if (tid < PLANE) {
+      const int i1 = tid / S2, i2 = tid - i1 * S2;
+      const bool in12 = (unsigned)(a1 + i1) < (unsigned)a.n1 && (unsigned)(a2 + i2) < (unsigned)a.n2;
+      const int64_t step = (int64_t)a.n1 * a.n2;
+      const int64_t idx = ((int64_t)a0 * a.n1 + (a1 + i1)) * a.n2 + (a2 + i2);
+      float vv[S0];
+#pragma unroll
+      for (int i0 = 0; i0 < S0; ++i0) {
+        vv[i0] = 0.0f;
+        if (in12 && (unsigned)(a0 + i0) < (unsigned)a.n0) vv[i0] = a.vol[idx + i0 * step];
+      }
+#pragma unroll
+      for (int i0 = 0; i0 < S0; ++i0) {
+        const float cf = vv[i0] > 0.0f ? vv[i0] : 0.0f;
+        vmax = fmaxf(vmax, cf);
+        s_slab[i0 * PLANE + tid] = cf;
+      }
+    }
+  } else {
     constexpr int IT = (SLAB + NT - 1) / NT;
     float vv[IT];
 #pragma unroll
@@ -1169,8 +1268,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
       const int i = tid + it * NT;
       const int i2 = i % S2, i1 = (i / S2) % S1, i0 = i / (S2 * S1);
       vv[it] = 0.0f;
-      if (i < SLAB && i0 < x0 && i1 < x1 && i2 < x2)
-        vv[it] = a.vol[((int64_t)(lo0 + i0) * a.n1 + (lo1 + i1)) * a.n2 + (lo2 + i2)];
+      if (i < SLAB && (unsigned)(a0 + i0) < (unsigned)a.n0 &&
+          (unsigned)(a1 + i1) < (unsigned)a.n1 && (unsigned)(a2 + i2) < (unsigned)a.n2)
+        vv[it] = a.vol[((int64_t)(a0 + i0) * a.n1 + (a1 + i1)) * a.n2 + (a2 + i2)];
     }
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
@@ -1190,8 +1290,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
-    vmax = fmaxf(vmax, __shfl_xor(vmax, o, 64));
-    rmax = fmaxf(rmax, __shfl_xor(rmax, o, 64));
+    vmax = fmaxf(vmax, wave_shfl_xor(vmax, o));
+    rmax = fmaxf(rmax, wave_shfl_xor(rmax, o));
   }
   if (lane == 0) {
     s_wmax[0][tid >> 6] = rmax;
@@ -1228,43 +1328,44 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
     const double lS = S > 0.0 ? fabs(dlog(S, s_tab)) : 0.0;
     const double Wn = S * (lS + 1.0) + 2.0;
     const double Wb = S * (lS / ln2 + 2.0) + 3.0;
-    const double ec = 2.0 * (double)P * (S * 8e-13 + 4096.0 * 0x1p-53 * Wn);
-    s_errc = ec;
-    s_errf = ec + 2.0 * ln2 * (double)P * (2.0 * (2.0 * a.log2_tol + 10.0 * 0x1p-24) * Wb + 0x1p-118);
+    s_errc = 2.0 * (double)P * (S * 8e-13 + 4096.0 * 0x1p-53 * Wn);
     s_wb = (double)P * Wb;
   }
-  // patch sums of c log 2c of a candidate and of the reference (sum of r log 2r)
-  auto zsum2_of = [&](int cand) {
-    const int j2 = cand % e2, j1 = (cand / e2) % e1, j0 = cand / (e2 * e1);
-    return a.zsum[((int64_t)(lo0 + j0) * a.c1 + (lo1 + j1)) * a.c2 + (lo2 + j2)];
+  // patch sums {sum c log 2c, sum c} of the candidate at box index j
+  auto zsum2_at = [&](int j0, int j1, int j2) {
+    return a.zsum[((int64_t)(a0 + j0) * a.c1 + (a1 + j1)) * a.c2 + (a2 + j2)];
   };
-  auto zsum_of = [&](int cand) { return zsum2_of(cand).x; };
   const double2 rz = a.zsum[lin];
   const double rsum = rz.x;
+
+  // the thread's slots k (candidate tid + k NT): slab offsets and validity
+  int basek[KC];
+  unsigned vbits = 0u, wbits = 0u;
+#pragma unroll
+  for (int k = 0; k < KC; ++k) {
+    const int c = min(tid + k * NT, BOX - 1);  // past the box: any valid row, result unused
+    const int j2 = c % W2, j1 = (c / W2) % W1, j0 = c / (W2 * W1);
+    basek[k] = (j0 * S1 + j1) * S2 + j2;
+    const bool v = tid + k * NT < BOX && in_box(j0, j1, j2);
+    vbits |= v ? 1u << k : 0u;
+    wbits |= __ballot(v) ? 1u << k : 0u;
+  }
+  wbits = __builtin_amdgcn_readfirstlane(wbits);  // slots with a valid candidate in the wave
   __syncthreads();
   const double errc = s_errc, pwb = s_wb;
   const float thr = a.stabilize ? a.thr_key : a.threshold;
-  const int rel0 = o0 - lo0, rel1 = o1 - lo1, rel2 = o2 - lo2;
-  const int center_local = (rel0 * e1 + rel1) * e2 + rel2;
 
   // ---- float pass ------------------------------------------------------------
   // Patch rows outer, the thread's candidates inner: each reference row is
   // read once per row into scalar registers (scalar loads of the read-only
   // volume) and the candidates' row sums are independent chains.
-  int basek[KC];
   double F[KC];
 #pragma unroll
-  for (int k = 0; k < KC; ++k) {
-    const int cand = min(tid + k * NT, box - 1);  // past the box: any valid row, result unused
-    const int j2 = cand % e2, j1 = (cand / e2) % e1, j0 = cand / (e2 * e1);
-    basek[k] = (j0 * S1 + j1) * S2 + j2;
-    F[k] = 0.0;
-  }
-  const int wave_first = __builtin_amdgcn_readfirstlane(tid & ~63);
+  for (int k = 0; k < KC; ++k) F[k] = 0.0;
   using cfloat = __attribute__((address_space(4))) const float;
   const cfloat *vref = (const cfloat *)a.vol + ((int64_t)o0 * a.n1 + o1) * a.n2 + o2;
-  // FULL: the whole search box, where every round but the last has
-  // candidates in every wave (no per-round branch)
+  // FULL: the whole search box inside the grid, where every slot but the
+  // last is valid in every lane (no per-slot branch)
   auto float_rows = [&](auto full) {
     constexpr bool FULL = decltype(full)::value;
 #pragma unroll 1
@@ -1283,7 +1384,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
       const int roff = (p0 * S1 + p1) * S2;
 #pragma unroll
       for (int k = 0; k < KC; ++k) {
-        if ((FULL && (k + 1) * NT <= BOX) || k * NT + wave_first < box) {
+        if ((FULL && (k + 1) * NT <= BOX) || (wbits & (1u << k))) {
           const float *row = s_slab + basek[k] + roff;
           float rs = 0.0f;
 #pragma unroll
@@ -1311,19 +1412,21 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
   // 1 + 2^-16 covers the division and the float64 sums.  Both this and
   // P * W2 bound G; the smaller one is used.
   const double eta = 2.0 * (2.0 * a.log2_tol + 10.0 * 0x1p-24);
+  const int tk = fresh_tid();
 #pragma unroll
   for (int k = 0; k < KC; ++k) {
-    const int cand = tid + k * NT;
+    const int cand = tk + k * NT;
     khi[k] = 0.0f;
-    if (cand < box) {
-      const double2 cz = zsum2_of(cand);
+    if (vbits & (1u << k)) {
+      const int j2 = cand % W2, j1 = (cand / W2) % W1, j0 = cand / (W2 * W1);
+      const double2 cz = zsum2_at(j0, j1, j2);
       const double df = 2.0 * (rsum + (cz.x - ln2 * F[k]));
       const double gb = (F[k] + 1.0615 * (double)P + 2.0 * (rz.y + cz.y)) * (1.0 + 0x1p-16);
       const double errf =
           errc + 2.0 * ln2 * (eta * fmin(fmax(gb, 0.0), pwb) + (double)P * 0x1p-118);
       bool ok_lo, ok_hi;
-      s_klo[cand] = bm_key<P>(df - errf, a, ok_lo);
-      khi[k] = bm_key<P>(df + errf, a, ok_hi);
+      s_klo[cand] = bm_key_bound<P>(df - errf, a, false, ok_lo);
+      khi[k] = bm_key_bound<P>(df + errf, a, true, ok_hi);
       my_maybe += ok_lo ? 1 : 0;
     }
   }
@@ -1334,29 +1437,35 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
     return ((unsigned long long)(key < thr ? 0u : 1u) << 63) |
            ((unsigned long long)__float_as_uint(key) << 32) | (unsigned long long)i;
   };
-  // each(f) over the composites of the thread's slots, key_of(k) the slot's
-  auto slot_keys = [&](auto key_of) {
-    return [&, key_of](auto f) {
-#pragma unroll
-      for (int k = 0; k < KC; ++k)
-        if (tid + k * NT < box) f(key_of(k));
-    };
-  };
 
-  // ---- cutoff: the K_max-th smallest upper-bound composite --------------------
+  // ---- cutoff: a bound on the K_max-th smallest upper-bound composite --------
+  // (its top 24 bits resolved: every composite of its radix bin is at most
+  // cut, a relative key width of 2^-15)
   const int kmax = bm_keep_count(s_maybe, box, a);
   unsigned long long cpre, cmask;
-  bm_radix_select<NT, BOX>(slot_keys([&](int k) { return comp(khi[k], tid + k * NT); }), kmax, sc,
-                           cpre, cmask);
+  // (the empty asm keeps the compiler from holding all KC composites, two
+  // registers each, live across the passes: it re-forms them from khi)
+  bm_radix_select<NT, BOX, 40>(
+      [&](auto f) {
+        const int t = fresh_tid();
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+          float kh = khi[k];
+          asm volatile("" : "+v"(kh));
+          if (vbits & (1u << k)) f(comp(kh, t + k * NT));
+        }
+      },
+      kmax, sc, cpre, cmask);
   const unsigned long long cut = cpre | ~cmask;
 
   // ---- exact keys for the candidates that can change the result ---------------
   // (lower composite at most the cutoff, or the threshold undecided)
   unsigned need = 0u;
+  const int tn = fresh_tid();
 #pragma unroll
   for (int k = 0; k < KC; ++k) {
-    const int cand = tid + k * NT;
-    if (cand < box) {
+    const int cand = tn + k * NT;
+    if (vbits & (1u << k)) {
       const float klo = s_klo[cand];
       if (comp(klo, cand) <= cut || ((klo < thr) != (khi[k] < thr))) need |= 1u << k;
     }
@@ -1369,12 +1478,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
   const int off = block_exclusive_scan<NT>(__popc(need), s_scan, &total);
   for (int lbase = 0; lbase < total; lbase += LCAP) {
     int r = off;
+    const int tl = fresh_tid();
 #pragma unroll
     for (int k = 0; k < KC; ++k) {
       if (need & (1u << k)) {
         if (r >= lbase && r < lbase + LCAP) {
-          lz[r - lbase] = zsum_of(tid + k * NT);
-          lc[r - lbase] = tid + k * NT;
+          const int cand = tl + k * NT;
+          lz[r - lbase] = zsum2_at(cand / (W2 * W1), (cand / W2) % W1, cand % W2).x;
+          lc[r - lbase] = cand;
         }
         ++r;
       }
@@ -1391,7 +1502,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
     const int gl = lane & (G - 1);
     for (int e = tid / G; e < m; e += NT / G) {
       const int cand = lc[e];
-      const int j2 = cand % e2, j1 = (cand / e2) % e1, j0 = cand / (e2 * e1);
+      const int j2 = cand % W2, j1 = (cand / W2) % W1, j0 = cand / (W2 * W1);
       const int base = (j0 * S1 + j1) * S2 + j2;
       double t = 0.0;
       for (int p = gl; p < P; p += G) {
@@ -1400,11 +1511,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
         const double s = (double)s_reff[p] + (double)s_slab[base + (p0 * S1 + p1) * S2 + p2];
         t = fma(-s, log_deg4(s, s_clog, ln2, c3, c4), t);
       }
-      for (int o = G >> 1; o > 0; o >>= 1) t = t + __shfl_xor(t, o, 64);
+      for (int o = G >> 1; o > 0; o >>= 1) t = t + wave_shfl_xor(t, o);
       const double dc = 2.0 * (rsum + (lz[e] + t));
       bool ok_lo, ok_hi, ok;
-      const float k_lo = bm_key<P>(dc - errc, a, ok_lo);
-      const float k_hi = bm_key<P>(dc + errc, a, ok_hi);
+      const float k_lo = bm_key_bound<P>(dc - errc, a, false, ok_lo);
+      const float k_hi = bm_key_bound<P>(dc + errc, a, true, ok_hi);
       if (gl == 0) {
         float key = k_lo;
         if (k_lo != k_hi) {
@@ -1428,17 +1539,19 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
   // composites are all in the evaluated set, which is kept for the final rank.
   unsigned long long *exset = reinterpret_cast<unsigned long long *>(s_kbuf);
   int my_acc = 0;
+  const int tf = fresh_tid();
 #pragma unroll
   for (int k = 0; k < KC; ++k) {
-    const int cand = tid + k * NT;
-    const float key = cand < box ? s_klo[cand] : 0.0f;
-    my_acc += (cand < box && key < thr) ? 1 : 0;
+    const int cand = tf + k * NT;
+    const bool v = vbits & (1u << k);
+    const float key = v ? s_klo[cand] : 0.0f;
+    my_acc += (v && key < thr) ? 1 : 0;
     // the wave's evaluated candidates of this slot take consecutive entries
-    const bool in = cand < box && (need & (1u << k));
+    const bool in = need & (1u << k);
     const unsigned long long inb = __ballot(in);
     int base = 0;
     if (lane == 0 && inb) base = atomicAdd(&s_nex, (int)__popcll(inb));
-    base = __shfl(base, 0, 64);
+    base = __builtin_amdgcn_readlane(base, 0);
     if (in) {
       const int slot = base + (int)__popcll(inb & ((1ull << lane) - 1ull));
       if (slot < XCAP) exset[slot] = comp(key, cand);
@@ -1456,10 +1569,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
       if (rank < K) sc.sorted[rank] = mine;
     }
     __syncthreads();
-    bm_finalize(K, acc, a, g, lo0, lo1, lo2, e1, e2, center_local, sc);
+    bm_finalize(K, acc, a, g, a0, a1, a2, W1, W2, CENTER, sc);
   } else {
-    bm_select_write<NT, BOX>(bm_box_keys<NT>([&](int i) { return comp(s_klo[i], i); }, box), box,
-                             acc, a, g, lo0, lo1, lo2, e1, e2, center_local, sc);
+    bm_select_write<NT, BOX>(
+        [&](auto f) {
+          for (int i = tid; i < BOX; i += NT)
+            if (valid_cand(i)) f(comp(s_klo[i], i));
+        },
+        box, acc, a, g, a0, a1, a2, W1, W2, CENTER, sc);
   }
 }
 

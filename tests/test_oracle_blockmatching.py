@@ -109,6 +109,36 @@ def test_canonical_key_grid():
     assert oracle.canonical_key(d) == np.float32(round(3.0 / q1) * q1)
 
 
+def test_canonical_key_float32_scaling_form():
+    """The matchers' device key computes the tie grid in float32 as two
+    power-of-two scalings, ldexp(rint(ldexp(d, 19 - e)), e - 19) with e the
+    frexp exponent of max(|d|, 1); it must give the bits of the reference's
+    float64 form float32(rint(d / q) * q) (blockmatching.py:957-964) for
+    every float32 d.  Checked on 2^24 random bit patterns, every exponent's
+    grid boundaries and the halfway points, whose ties round to even."""
+    rng = np.random.default_rng(5)
+    bits = [rng.integers(0, 1 << 32, 1 << 24, dtype=np.uint64).astype(np.uint32)]
+    for ex in range(0, 255):
+        base = np.uint32(ex << 23)
+        bits.append(base + np.arange(0, 64, dtype=np.uint32))
+        bits.append(base + np.uint32((1 << 23) - 64) + np.arange(64, dtype=np.uint32))
+    d = np.concatenate(bits).view(np.float32)
+    d = np.concatenate([d, -d])
+    q0 = np.float32(32 * 2.0 ** -23)
+    halves = (np.arange(1, 4096, dtype=np.float32) + np.float32(0.5)) * q0
+    d = np.concatenate([d, halves, halves * np.float32(2.0 ** 40)])
+    d = d[np.isfinite(d)]
+    mag = np.maximum(np.abs(d.astype(np.float64)), 1.0)
+    e = np.frexp(mag)[1]
+    q = np.ldexp(2.0 ** -18, e - 1)
+    with np.errstate(over="ignore"):
+        want = (np.rint(d.astype(np.float64) / q) * q).astype(np.float32)
+        e32 = np.frexp(np.maximum(np.abs(d), np.float32(1.0)))[1]
+        got = np.ldexp(np.rint(np.ldexp(d, 19 - e32)), e32 - 19).astype(np.float32)
+    assert got.dtype == np.float32
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
 def _deviance_f64(r, c):
     """Term-by-term restatement of blockmatching.py:338-371 in Python."""
     d = 0.0
