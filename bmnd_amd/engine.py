@@ -568,12 +568,13 @@ class PoissonPipeline:
         return self.stage("wiener", match2, vol, var2, vol, timer, fill, pilot=pilot,
                           on_match=on_match)
 
-    def run_surrogate(self, vol: torch.Tensor, net, check, timer=None):
+    def run_surrogate(self, vol: torch.Tensor, net, check, timer=None, on_pilot=None):
         """Stage 2 on a network pilot (bmnd_amd.surrogate): the network's
         pilot of `vol`, its agreement check against the plain stage 1 on
         sub-volumes, the plain stage 1 on `vol` instead when it fails, then
-        the reference's stage 2 on the pilot kept.  Returns
-        (SurrogateStages, SurrogateInfo)."""
+        the reference's stage 2 on the pilot kept; on_pilot(pilot) runs
+        between the two (the place of the reference's "ht" callbacks).
+        Returns (SurrogateStages, SurrogateInfo)."""
         from .surrogate import SurrogateInfo, anscombe_stats
         t = timer or _NoTimer()
         # the count level first: E[A(x)^2] / 4 - 3/8 = E[x] for A(x) =
@@ -598,6 +599,8 @@ class PoissonPipeline:
             var1 = self.variance(vol)
             t.stop(tok)
             used = "network"
+        if on_pilot is not None:
+            on_pilot(pilot)
         wi = self.run_wiener(vol, pilot, var1, timer)
         return SurrogateStages(pilot, wi), SurrogateInfo(used, dist, check.threshold, mean_count)
 

@@ -530,9 +530,11 @@ def bmnd_surrogate(volume, profile, network: PilotNet | None = None, *,
     net = network if network is not None else default_network(device)
     pipe = _pipeline(profile, vol.shape, device)
     vol_t = torch.from_numpy(vol).to(device)
-    est, info = pipe.run_surrogate(vol_t, net, check or PilotCheck())
+    on_pilot = None
     if stage_callback is not None:
-        stage_callback("ht", est.pilot.cpu().numpy())
+        def on_pilot(pilot):  # before stage 2, as the reference orders it (bmndalgo.py:632-633)
+            stage_callback("ht", pilot.cpu().numpy())
+    est, info = pipe.run_surrogate(vol_t, net, check or PilotCheck(), on_pilot=on_pilot)
     out = est.wiener.est.cpu().numpy()
     if stage_callback is not None:
         stage_callback("wiener", out.copy())
