@@ -1605,6 +1605,14 @@ struct FilterArgs {
   const int32_t *pw_plans;
   const bmnd_hip_input *batch;
   const int32_t *ref_input;
+  // non-default options of the filter code (bmnd_hip_filter_groups):
+  int soft;          // HT soft thresholding (bmndalgo.py:475-483)
+  int wmodel;        // weight model: 0 variance, 1 classic, 2 risk (Wiener)
+  int windowed;      // windowed-synthesis weight domain
+  int group_scope;   // one risk per group (aggregationweights.py:175-211)
+  int coef_out;      // filt gets the shrunk coefficients, not the patches
+  const double *smd;  // spatial metric diagonal [P] (matrix table)
+  const double *gmd;  // group metric diagonals, n entries at n(n-1)/2
 };
 
 __device__ __forceinline__ void bind_input(FilterArgs &a, int64_t g) {
@@ -1976,6 +1984,26 @@ __device__ __forceinline__ void group_transform(float *__restrict__ data, int n,
   line_transform<NT, B2, 1>(data, n * B0 * B1, S[2], epi);
 }
 
+// Poisson Wiener gain from the signal power sp and mu sigma^2 (wiener.py:78-95)
+__device__ __forceinline__ float wiener_gain(float sp, float eff) {
+  float gain = sp / ((sp + eff) + 1e-10f);
+  return gain < 0.0f ? 0.0f : (gain > 1.0f ? 1.0f : gain);
+}
+
+// signal power of the Wiener gain (wiener.py:24-53): |r|^2 less mu sigma^2
+// (mode 0, variance-scaled), less sigma^2 (2, noise-floor) or whole (1,
+// classic), floored at 0
+__device__ __forceinline__ float wiener_signal_power(float r, float nv, float eff, int gmode) {
+  const float rc = fabsf(r);
+  const float pr = rc * rc;
+  float sp = pr;
+  if (gmode != 1) {
+    sp = pr - (gmode == 2 ? nv : eff);
+    if (sp < 0.0f) sp = 0.0f;
+  }
+  return sp;
+}
+
 template <int B0, int B1, int B2, int R0, int R1, int R2, int NMAX, int NT, bool WIENER,
           bool SIGMA>
 __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
@@ -1996,7 +2024,7 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
   // after the plane limiting s_ex is free; the Wiener sigma-map mean uses it
   // for the slots of its summation tree (leaves of >= 64 elements)
   static_assert(KEX * P >= 2 * ((NP + 63) / 64), "summation slots do not fit");
-  __shared__ float s_ex[KEX * P];
+  __shared__ __align__(8) float s_ex[KEX * P];
   __shared__ int s_pos[NMAX][3];
   __shared__ double s_plane[NMAX];
 
@@ -2185,79 +2213,125 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
 
   // ---- coefficients, with the shrinkage fused into the last transform pass ----
   float *atten = s_r;
+  const bool risk_model = WIENER && a.wmodel == 2;
   if constexpr (!WIENER) {
-    // Poisson hard threshold (poisson.py:30-46, bmndalgo.py:460-485)
     const float lam = a.shrink;
-    const double cs = a.count_scale;
-    group_transform<NT, B0, B1, B2, NMAX>(s_x, n, s_g, s_sf, [&](int e, float c) {
-      const float thr = lam * s_sig[e];
-      const double cc = (double)fabsf(c) / cs;
-      const double tc = (double)thr / cs;
-      double sc = cc > tc ? cc : tc;
-      if (sc < 1.0) sc = 1.0;
-      const double tol = 0x1p-16 * sc;
-      const float keep = (cc > tc + tol) ? 1.0f : 0.0f;
-      atten[e] = keep;
-      return c * keep;
-    });
+    if (a.soft) {
+      // soft threshold (bmndalgo.py:475-483): |C| shrunk by thr = lambda sigma,
+      // the attenuation shrunk / |C| (0 where C = 0), sign(C) * shrunk with
+      // numpy's sign (0 at +-0)
+      group_transform<NT, B0, B1, B2, NMAX>(s_x, n, s_g, s_sf, [&](int e, float c) {
+        const float thr = lam * s_sig[e];
+        const float mag = fabsf(c);
+        float shr = mag - thr;
+        shr = shr > 0.0f ? shr : 0.0f;
+        atten[e] = mag > 0.0f ? shr / mag : 0.0f;
+        return c > 0.0f ? shr : (c < 0.0f ? -shr : 0.0f);
+      });
+    } else {
+      // Poisson hard threshold (poisson.py:30-46, bmndalgo.py:460-485)
+      const double cs = a.count_scale;
+      group_transform<NT, B0, B1, B2, NMAX>(s_x, n, s_g, s_sf, [&](int e, float c) {
+        const float thr = lam * s_sig[e];
+        const double cc = (double)fabsf(c) / cs;
+        const double tc = (double)thr / cs;
+        double sc = cc > tc ? cc : tc;
+        if (sc < 1.0) sc = 1.0;
+        const double tol = 0x1p-16 * sc;
+        const float keep = (cc > tc + tol) ? 1.0f : 0.0f;
+        atten[e] = keep;
+        return c * keep;
+      });
+    }
   } else {
-    // Poisson Wiener gain (wiener.py:24-53, 78-95), signal power |r|^2 less
-    // mu sigma^2 (variance-scaled), less sigma^2 (noise-floor) or not at all
-    // (classic), computed as the pilot's coefficients come out of its last
-    // pass; the noisy coefficients take it in theirs, after the risks
+    // Poisson Wiener gain (wiener.py:24-53, 78-95), computed as the pilot's
+    // coefficients come out of its last pass; the noisy coefficients take it
+    // in theirs, after the risks.  The risk model also needs the signal
+    // power: its slot keeps the power until the risks are formed, and the
+    // gain replaces it after them.
     const float mu = a.shrink;
     const int gmode = a.gain_mode;
-    const float eps32 = 1e-10f;
     group_transform<NT, B0, B1, B2, NMAX>(s_r, n, s_g, s_sf, [&](int e, float r) {
       const float sg = s_sig[e];
       const float nv = sg * sg;
       const float eff = mu * nv;
-      const float rc = fabsf(r);
-      const float pr = rc * rc;
-      float sp_ = pr;
-      if (gmode != 1) {
-        sp_ = pr - (gmode == 2 ? nv : eff);
-        if (sp_ < 0.0f) sp_ = 0.0f;
-      }
-      float gain = sp_ / ((sp_ + eff) + eps32);
-      gain = gain < 0.0f ? 0.0f : (gain > 1.0f ? 1.0f : gain);
-      return gain;  // the pilot coefficient's slot now holds the attenuation
+      const float sp = wiener_signal_power(r, nv, eff, gmode);
+      return risk_model ? sp : wiener_gain(sp, eff);
     });
   }
-  // ---- per-patch weights (aggregationweights.py:213-259, 616-653) ------------
-  if (tid < n) {
-    double risk = 0.0;
-    constexpr int CH = 25;  // loads of a chunk in flight together
-    for (int c0 = 0; c0 < P; c0 += CH) {
-      float atv[CH], sgv[CH];
-#pragma unroll
-      for (int j = 0; j < CH; ++j) {
-        atv[j] = c0 + j < P ? atten[tid * P + c0 + j] : 0.0f;
-        sgv[j] = c0 + j < P ? s_sig[tid * P + c0 + j] : 0.0f;
-      }
-#pragma unroll
-      for (int j = 0; j < CH; ++j) {
-        if (c0 + j < P) {
-          const double at = (double)atv[j];
-          const float nv = sgv[j] * sgv[j];
-          double err = at * at;
-          err = err * (double)nv;
-          risk = risk + err;
-        }
-      }
+  // ---- aggregation weights (aggregationweights.py:175-259, 616-653, 656-687):
+  // per coefficient the error a^2 (classic), a^2 sigma^2 (variance) or
+  // a^2 sigma^2 + (1 - a)^2 S (risk), times the spatial metric diagonal in the
+  // windowed-synthesis domain; patch scope mixes each member's sum over the
+  // group with |Ginv|^2, group scope sums the whole group in numpy's order
+  const int wmodel = a.wmodel;
+  const bool windowed = a.windowed != 0;
+  auto coef_error = [&](int e) {
+    const int i = e / P, c = e - i * P;
+    const float sg = s_sig[e];
+    const float nv = sg * sg;
+    float at = atten[e];
+    float sp = 0.0f;
+    if (risk_model) {
+      sp = at;
+      at = wiener_gain(sp, a.shrink * nv);
     }
-    s_plane[tid] = risk;
+    const double ad = (double)at;
+    double err = ad * ad;
+    if (wmodel != 1) err = err * (double)nv;
+    if (risk_model) {
+      const double rej = 1.0 - ad;
+      err = err + (rej * rej) * (double)sp;
+    }
+    if (windowed) {
+      if (a.group_scope) err = err * a.gmd[n * (n - 1) / 2 + i];
+      err = err * a.smd[c];
+    }
+    return err;
+  };
+  if (a.group_scope) {
+    static_assert(KEX * P >= 4 * ((NP + 63) / 64), "float64 summation slots do not fit");
+    const double risk = np_block_sum_fn<NT, double>(coef_error, a.pw_plans + a.pw_plans[n],
+                                                    reinterpret_cast<double *>(s_ex));
+    if (tid < n) {
+      float w = 0.0f;
+      if (isfinite(risk) && risk >= 0.0) w = (float)(1.0 / (risk + 1e-10));
+      a.weights[g * a.max_m + tid] = w;
+    }
+  } else {
+    if (tid < n) {
+      double risk = 0.0;
+      constexpr int CH = 25;  // a chunk's LDS reads in flight together
+      for (int c0 = 0; c0 < P; c0 += CH) {
+        double ev[CH];
+#pragma unroll
+        for (int j = 0; j < CH; ++j) ev[j] = c0 + j < P ? coef_error(tid * P + c0 + j) : 0.0;
+#pragma unroll
+        for (int j = 0; j < CH; ++j)
+          if (c0 + j < P) risk = risk + ev[j];
+      }
+      s_plane[tid] = risk;
+    }
+    __syncthreads();
+    if (tid < n) {
+      double risk = 0.0;
+      for (int j = 0; j < n; ++j) {
+        const double m = (double)s_gi[tid * NMAX + j];
+        risk = risk + (m * m) * s_plane[j];
+      }
+      float w = 0.0f;
+      if (isfinite(risk) && risk >= 0.0) w = (float)(1.0 / (risk + 1e-10));
+      a.weights[g * a.max_m + tid] = w;
+    }
   }
-  __syncthreads();
-  if (tid < n) {
-    double risk = 0.0;
-    for (int j = 0; j < n; ++j) {
-      const double m = (double)s_gi[tid * NMAX + j];
-      risk = risk + (m * m) * s_plane[j];
+  if (risk_model) {
+    // the signal powers become the gains the noisy coefficients take
+    __syncthreads();
+    for (int e = tid; e < np; e += NT) {
+      const float sg = s_sig[e];
+      atten[e] = wiener_gain(atten[e], a.shrink * (sg * sg));
     }
-    float w = 0.0f;
-    if (isfinite(risk) && risk >= 0.0) w = (float)(1.0 / (risk + 1e-10));
-    a.weights[g * a.max_m + tid] = w;
+    __syncthreads();  // the noisy patches overwrite the std buffer next
   }
   if constexpr (WIENER) {
     // noisy patches into the std buffer (its last reads were the risks,
@@ -2279,6 +2353,13 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
   }
   // ---- inverse transform, stored from its last pass --------------------------
   float *dst = a.filt + (g * a.max_m) * (int64_t)P;
+  if (a.coef_out) {
+    // the group-mass correction takes the shrunk coefficients
+    // (bmndalgo.py:537-548, 959-970) and runs the inverse transform itself
+    __syncthreads();
+    for (int e = tid; e < np; e += NT) dst[e] = s_x[e];
+    return;
+  }
   group_transform<NT, B0, B1, B2, NMAX>(s_x, n, s_gi, s_si, [&](int e, float v) {
     dst[e] = v;
     return v;
@@ -3147,11 +3228,23 @@ int32_t bmnd_hip_blockmatch_batch(const float *vol, const bmnd_hip_input *inputs
                          accepted, stream, &b, ref_thresholds, null_moments, null_max);
 }
 
-int64_t bmnd_hip_matrix_table_size(int32_t max_matches, const int32_t *block_h) {
-  const int64_t m = max_matches;
+// The matrix table: float32 member matrices G_n and their inverses
+// (n = 1..max), the spatial analysis and synthesis matrices, then, from the
+// next even float offset, float64 tables: the spatial metric diagonal [P],
+// the group metric diagonals (n entries for each n) and the constant-synthesis
+// directions of the group-mass correction (n entries for each n, then each
+// spatial axis's).  Returns the float offset of the float64 part.
+static int64_t matrix_table_f64_offset(int64_t m, const int32_t *block_h) {
   int64_t n = 2 * (m * (m + 1) * (2 * m + 1) / 6);
   for (int d = 0; d < 3; ++d) n += 2 * (int64_t)block_h[d] * block_h[d];
-  return n;
+  return (n + 1) & ~(int64_t)1;
+}
+
+int64_t bmnd_hip_matrix_table_size(int32_t max_matches, const int32_t *block_h) {
+  const int64_t m = max_matches;
+  const int64_t P = (int64_t)block_h[0] * block_h[1] * block_h[2];
+  const int64_t doubles = P + m * (m + 1) + block_h[0] + block_h[1] + block_h[2];
+  return matrix_table_f64_offset(m, block_h) + 2 * doubles;
 }
 
 static int32_t exact_std_impl(const float *var, const int64_t *shape_h, const int32_t *block_h,
@@ -3258,9 +3351,16 @@ static int32_t filter_impl(int32_t wiener, const float *noisy, const float *pilo
     return rc;
   }
   if (k < 0) return fail(BMND_HIP_EINVAL, "k must be non-negative");
-  if (wiener < 0 || wiener > 3)
-    return fail(BMND_HIP_EINVAL, "wiener must be 0 (hard threshold) or 1, 2, 3 (Wiener gain: "
-                                 "variance-scaled, classic, noise-floor)");
+  if (wiener < 0 || (wiener & ~0xff))
+    return fail(BMND_HIP_EINVAL, "unknown bits in the filter code");
+  const int code = wiener;
+  wiener = code & 3;
+  const int wmodel = (code >> 3) & 3;
+  if (wmodel == 3) return fail(BMND_HIP_EINVAL, "weight model must be 0, 1 or 2");
+  if (wmodel == 2 && !wiener)
+    return fail(BMND_HIP_EINVAL, "the risk weight model is a Wiener-stage option");
+  if ((code & 4) && wiener)
+    return fail(BMND_HIP_EINVAL, "soft thresholding is a hard-threshold-stage option");
   if (nrefs <= 0) return 0;
   if (wiener && !pilot) return fail(BMND_HIP_EINVAL, "Wiener filtering needs the pilot volume");
   if (k > 0 && !exact_std) return fail(BMND_HIP_EINVAL, "k > 0 needs the exact-plane std buffer");
@@ -3291,9 +3391,19 @@ static int32_t filter_impl(int32_t wiener, const float *noisy, const float *pilo
   a.pw_plans = nullptr;
   a.batch = b ? b->d : nullptr;
   a.ref_input = b ? b->ref_input : nullptr;
-  if (sigma_out && wiener) {
+  a.soft = (code & 4) != 0;
+  a.wmodel = wmodel;
+  a.windowed = (code & 32) != 0;
+  a.group_scope = (code & 64) != 0;
+  a.coef_out = (code & 128) != 0;
+  {
     const int P = block_h[0] * block_h[1] * block_h[2];
-    if (int rc = group_plans_for_device(P, max_matches, &a.pw_plans)) return rc;
+    const double *t64 = reinterpret_cast<const double *>(
+        matrices + matrix_table_f64_offset(max_matches, block_h));
+    a.smd = t64;
+    a.gmd = t64 + P;
+    if ((sigma_out && wiener) || a.group_scope)
+      if (int rc = group_plans_for_device(P, max_matches, &a.pw_plans)) return rc;
   }
   hipStream_t s = (hipStream_t)stream;
   dim3 grid((unsigned)nrefs);

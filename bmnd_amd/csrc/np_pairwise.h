@@ -82,6 +82,55 @@ __device__ __forceinline__ float np_block_sum(const float *__restrict__ x,
   return r;
 }
 
+// The same two routines over values produced on the fly: val(i) gives
+// element i of the range; T is the element and accumulator type (numpy's
+// float64 sum follows the same tree as its float32 one).
+template <class T, class F>
+__host__ __device__ __forceinline__ T np_leaf_sum_fn(F val, int lo, int m) {
+  if (m < 8) {
+    T r = T(0);
+    for (int i = 0; i < m; ++i) r = r + val(lo + i);
+    return r;
+  }
+  T r[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) r[j] = val(lo + j);
+  int i = 8;
+  const int m8 = m - (m % 8);
+  for (; i < m8; i += 8) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) r[j] = r[j] + val(lo + i + j);
+  }
+  T res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+  for (; i < m; ++i) res = res + val(lo + i);
+  return res;
+}
+
+template <int NT, class T, class F>
+__device__ __forceinline__ T np_block_sum_fn(F val, const int32_t *__restrict__ plan,
+                                             T *__restrict__ s_slot) {
+  const int L = plan[0], H = plan[1];
+  const int32_t *cnt = plan + 2;
+  const int32_t *lstart = cnt + H;
+  const int32_t *llen = lstart + L;
+  const int32_t *ops = llen + L;
+  for (int l = threadIdx.x; l < L; l += NT) s_slot[l] = np_leaf_sum_fn<T>(val, lstart[l], llen[l]);
+  __syncthreads();
+  int base = 0;
+  for (int h = 0; h < H; ++h) {
+    const int c = cnt[h];
+    for (int j = threadIdx.x; j < c; j += NT) {
+      const int o = base + j;
+      s_slot[L + o] = s_slot[ops[2 * o]] + s_slot[ops[2 * o + 1]];
+    }
+    __syncthreads();
+    base += c;
+  }
+  const T r = s_slot[base == 0 ? 0 : L + base - 1];
+  __syncthreads();
+  return r;
+}
+
 // Host: the plan for a range of m elements (1 <= m <= 8192).
 inline std::vector<int32_t> np_pairwise_plan(int m) {
   struct Node {

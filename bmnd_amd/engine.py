@@ -14,6 +14,7 @@ import numpy as np
 import torch
 
 from . import _lib
+from .aggregationweights import constant_synthesis_directions, synthesis_metric_diagonals
 from .enums import (
     AggregationWeightDomain,
     AggregationWeightModel,
@@ -135,6 +136,18 @@ class StagePlan:
             fwd.append(f.reshape(-1))
             inv.append(i.reshape(-1))
         table = np.concatenate(parts + fwd + inv).astype(np.float32)
+        # float64 part from the next even offset: the synthesis metric
+        # diagonals of the windowed-synthesis weights and the constant-synthesis
+        # directions of the group-mass correction
+        ginv = [group_matrices(n)[1] for n in range(1, self.max_m + 1)]
+        sinv = [i.reshape(b, b) for i, b in zip(inv, self.block3)]
+        smd, gmd = synthesis_metric_diagonals(sinv, self.kaiser, ginv)
+        mdir = np.concatenate(constant_synthesis_directions(ginv))
+        sdir = np.concatenate(constant_synthesis_directions(sinv))
+        f64 = np.concatenate([smd, gmd, mdir, sdir]).astype(np.float64)
+        if table.size % 2:
+            table = np.concatenate([table, np.zeros(1, np.float32)])
+        table = np.concatenate([table, f64.view(np.float32)])
         blk = (ctypes.c_int32 * 3)(*self.block3)
         assert table.size == _lib.lib().bmnd_hip_matrix_table_size(self.max_m, blk)
         return table
@@ -177,20 +190,50 @@ def check_supported(profile: BMNDProfile, resolved) -> None:
     checks = [
         (field("poisson_group_mass_conservation", PoissonGroupMassConservation)
          is PoissonGroupMassConservation.NONE, "poisson_group_mass_conservation='none'"),
-        (not profile.ht_use_soft_thresholding, "ht_use_soft_thresholding=False"),
     ]
-    for stage in ("ht", "wiener"):
-        checks += [
-            (field(f"{stage}_weight_model", AggregationWeightModel)
-             is AggregationWeightModel.VARIANCE, f"{stage}_weight_model='variance'"),
-            (field(f"{stage}_weight_domain", AggregationWeightDomain)
-             is AggregationWeightDomain.COEFFICIENT, f"{stage}_weight_domain='coefficient'"),
-            (field(f"{stage}_weight_scope", AggregationWeightScope)
-             is AggregationWeightScope.PATCH, f"{stage}_weight_scope='patch'"),
-        ]
     for ok, what in checks:
         if not ok:
             raise NotImplementedError(f"the MI355X path requires {what}.")
+    for stage in ("ht", "wiener"):
+        # the covariance-aware exact group risk (bmndalgo.py:359-377, 752-758;
+        # wiener.py:638-844) replaces the diagonal weights for these choices
+        models = ((AggregationWeightModel.VARIANCE,) if stage == "ht" else
+                  (AggregationWeightModel.VARIANCE, AggregationWeightModel.RISK))
+        if (field(f"{stage}_weight_model", AggregationWeightModel) in models
+                and field(f"{stage}_weight_domain", AggregationWeightDomain)
+                is AggregationWeightDomain.WINDOWED_SYNTHESIS
+                and field(f"{stage}_weight_scope", AggregationWeightScope)
+                is AggregationWeightScope.GROUP and int(profile.k) > 0):
+            raise NotImplementedError(
+                f"the exact covariance-aware {stage} group risk (windowed-synthesis domain, "
+                "group scope, k > 0) is not part of the MI355X path.")
+
+
+# the filters' weight-model codes (aggregationweights.py:19-21 order differs:
+# 0 here is the default variance model)
+_WEIGHT_MODEL_CODE = {AggregationWeightModel.VARIANCE: 0, AggregationWeightModel.CLASSIC: 1,
+                      AggregationWeightModel.RISK: 2}
+
+
+def filter_code(profile: BMNDProfile, stage: str, gain_arg: int, coefficients: bool = False) -> int:
+    """The filter code of bmnd_hip_filter_groups for one stage: the stage
+    and Wiener gain in bits 0-1, soft thresholding (bit 2), the weight model
+    (bits 3-4), the windowed-synthesis domain (bit 5), the group scope (bit 6)
+    and, for the group-mass correction, coefficient output (bit 7)."""
+    def field(name, enum):
+        return coerce_enum(getattr(profile, name), enum, name)
+    code = gain_arg if stage == "wiener" else 0
+    if stage == "ht" and profile.ht_use_soft_thresholding:
+        code |= 4
+    code |= _WEIGHT_MODEL_CODE[field(f"{stage}_weight_model", AggregationWeightModel)] << 3
+    if field(f"{stage}_weight_domain", AggregationWeightDomain) \
+            is AggregationWeightDomain.WINDOWED_SYNTHESIS:
+        code |= 32
+    if field(f"{stage}_weight_scope", AggregationWeightScope) is AggregationWeightScope.GROUP:
+        code |= 64
+    if coefficients:
+        code |= 128
+    return code
 
 
 def make_plan(profile: BMNDProfile, stage: str, ndim: int) -> StagePlan:
@@ -343,6 +386,8 @@ class PoissonPipeline:
         # the filters' stage argument: 0 hard threshold, 1-3 the Wiener gain
         self.wiener_arg = _GAIN_ARG[coerce_enum(profile.poisson_wiener_gain_mode,
                                                 PoissonWienerGainMode, "poisson_wiener_gain_mode")]
+        # each stage's full filter code (bmnd_hip_filter_groups)
+        self.filter_codes = {st: filter_code(profile, st, self.wiener_arg) for st in ("ht", "wiener")}
         self._null = None
         self.k = int(profile.k)
         self.plans = {s: make_plan(profile, s, self.ndim) for s in ("ht", "wiener")}
@@ -515,7 +560,7 @@ class PoissonPipeline:
             gsig = torch.empty(R if wiener else R * plan.max_m * plan.P, dtype=torch.float32,
                                device=dev)
         tok = timer.start(f"{name}.filter")
-        _lib.check(L.bmnd_hip_filter_groups_sigma(self.wiener_arg if wiener else 0, _ptr(noisy),
+        _lib.check(L.bmnd_hip_filter_groups_sigma(self.filter_codes[name], _ptr(noisy),
                                                   _ptr((match_vol if pilot is None else pilot)
                                                        if wiener else None), _ptr(var),
                                                   shape, block, radius, _ptr(sel), _ptr(cnt), R,
@@ -608,8 +653,10 @@ class PoissonPipeline:
     def graph_supported(self) -> bool:
         """Whether the pipeline's options run in a captured graph: the fixed
         stage-1 threshold and the plain (not standardized) distances; sigma
-        maps and callbacks are served by the call-by-call path."""
-        return (_ht_policy(self.profile) is PoissonHTMatchPolicy.FIXED and not self.standardized)
+        maps, callbacks and the non-default thresholding and weights are served
+        by the call-by-call path."""
+        return (_ht_policy(self.profile) is PoissonHTMatchPolicy.FIXED and not self.standardized
+                and self.filter_codes == {"ht": 0, "wiener": self.wiener_arg})
 
     def _graph_modes(self, vol: torch.Tensor) -> tuple[int, int, int]:
         """(stage-1 mode, its table size, stage-2 mode) for a capture: stage 1
@@ -941,7 +988,7 @@ class BatchPipeline:
         wiener = name == "wiener"
         tok = timer.start(f"{name}.filter")
         _lib.check(L.bmnd_hip_filter_groups_batch(
-            self._single.wiener_arg if wiener else 0, _ptr(noisy),
+            self._single.filter_codes[name], _ptr(noisy),
             _ptr((match_vol if pilot is None else pilot) if wiener else None), _ptr(var), ih, idv, nin,
             block, radius, _ptr(sel), _ptr(cnt), _ptr(st.ref_input), R, plan.max_m, _ptr(mats),
             _ptr(exact), plan.shrink, k, self._single.count_scale, _ptr(filt), _ptr(wts), stream))

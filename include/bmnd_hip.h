@@ -96,7 +96,14 @@ int32_t bmnd_hip_log2_check(double *max_ratio);
  *   G_n (n x n, row-major, out index first) for n = 1..max_matches,
  *   then Ginv_n for n = 1..max_matches,
  *   then the spatial analysis matrices of axes 0,1,2 (b_d x b_d),
- *   then the spatial synthesis matrices of axes 0,1,2.
+ *   then the spatial synthesis matrices of axes 0,1,2,
+ * then, from the next even float offset, float64 values
+ * (aggregationweights.py:79-120, poisson.py:120-160):
+ *   the spatial metric diagonal diag(S^T diag(w^2) S) [P] (S the Kronecker
+ *   product of the synthesis matrices, w the float32 Kaiser window),
+ *   the group metric diagonals diag(Ginv_n^T Ginv_n) for n = 1..max (n each),
+ *   the constant-synthesis directions solve(Ginv_n, 1) for n = 1..max,
+ *   then solve(S_d, 1) of each spatial axis d.
  * bmnd_hip_matrix_table_size() gives its length in floats. */
 int64_t bmnd_hip_matrix_table_size(int32_t max_matches, const int32_t *block_h);
 
@@ -164,9 +171,18 @@ int32_t bmnd_hip_exact_std(const float *var, const int64_t *shape_h,
  *   filt       out: (nrefs*max_matches, P) filtered patches, row g*max+i
  *   weights    out: (nrefs*max_matches) float32 weights
  * shrink is ht_lambda_threshold (HT) or wiener_variance_scale (Wiener).
- * wiener selects the Wiener gain's signal power (wiener.py:24-53): 1
- * variance-scaled max(r^2 - shrink sigma^2, 0), 2 classic r^2, 3 noise-floor
- * max(r^2 - sigma^2, 0); the same values in every entry that takes wiener. */
+ * wiener is the filter code: bits 0-1 select the stage and the Wiener gain's
+ * signal power (wiener.py:24-53): 0 hard threshold, 1 variance-scaled
+ * max(r^2 - shrink sigma^2, 0), 2 classic r^2, 3 noise-floor
+ * max(r^2 - sigma^2, 0); bit 2 (HT only) soft thresholding
+ * (bmndalgo.py:475-483); bits 3-4 the aggregation weight model
+ * (aggregationweights.py:214-259): 0 variance, 1 classic, 2 risk (Wiener
+ * only; the gain's signal power as S); bit 5 the windowed-synthesis weight
+ * domain; bit 6 the group weight scope (one risk per group, numpy's float64
+ * sum; aggregationweights.py:175-211); bit 7 writes the shrunk coefficients
+ * to filt instead of the inverse-transformed patches (the input of
+ * bmnd_hip_mass_correct).  The same values in every entry that takes a
+ * filter code. */
 int32_t bmnd_hip_filter_groups(int32_t wiener, const float *noisy,
                                const float *pilot, const float *var,
                                const int64_t *shape_h, const int32_t *block_h,

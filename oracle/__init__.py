@@ -70,6 +70,17 @@ class OracleParams(ctypes.Structure):
         ("null_vars", ctypes.c_void_p),
         ("null_max", ctypes.c_int),
         ("wi_gain_mode", ctypes.c_int),
+        ("ht_soft", ctypes.c_int),
+        ("ht_wmodel", ctypes.c_int),
+        ("wi_wmodel", ctypes.c_int),
+        ("ht_windowed", ctypes.c_int),
+        ("wi_windowed", ctypes.c_int),
+        ("ht_group_scope", ctypes.c_int),
+        ("wi_group_scope", ctypes.c_int),
+        ("ht_smd", ctypes.c_void_p),
+        ("wi_smd", ctypes.c_void_p),
+        ("ht_gmd", ctypes.c_void_p),
+        ("wi_gmd", ctypes.c_void_p),
     ]
 
 
@@ -117,6 +128,12 @@ def _declare(L: ctypes.CDLL) -> None:
     L.oracle_limit_planes.argtypes = [vp, vp, i32, i32, i32, vp]
     L.oracle_patch_weights.argtypes = [vp, vp, i32, i32, vp, f64, vp]
     L.oracle_ht_shrink.argtypes = [vp, vp, i64, f32, f64, vp]
+    L.oracle_ht_soft.argtypes = [vp, vp, i64, f32, vp]
+    L.oracle_np_sum_f64.argtypes = [vp, i64]
+    L.oracle_np_sum_f64.restype = f64
+    L.oracle_weight_risks.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, i32, i32, i32, vp]
+    L.oracle_weight_risks.restype = i32
+    L.oracle_weights_ex.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, i32, i32, i32, f64, vp]
     L.oracle_wiener_shrink.argtypes = [vp, vp, vp, i64, f32, vp]
     L.oracle_wiener_shrink_mode.argtypes = [vp, vp, vp, i64, f32, i32, vp]
     L.oracle_scatter_add.argtypes = [vp, vp, i32, vp, vp, vp, vp, vp, vp, i32]
@@ -340,6 +357,41 @@ def ht_shrink(coef, sigma, lam=3.0, count_scale=1.0):
     att = np.empty_like(c)
     lib().oracle_ht_shrink(_p(c), _p(s), c.size, lam, count_scale, _p(att))
     return c, att
+
+
+def ht_soft(coef, sigma, lam=3.0):
+    """Soft threshold: (coefficients, attenuation)."""
+    c = np.ascontiguousarray(coef, dtype=np.float32).copy()
+    s = np.ascontiguousarray(sigma, dtype=np.float32)
+    att = np.empty_like(c)
+    lib().oracle_ht_soft(_p(c), _p(s), c.size, lam, _p(att))
+    return c, att
+
+
+def np_sum_f64(a) -> float:
+    x = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+    return float(lib().oracle_np_sum_f64(_p(x), x.size))
+
+
+WEIGHT_MODELS = {"variance": 0, "classic": 1, "risk": 2}
+
+
+def weight_risks(atten, sigma, group_inv, *, model="variance", signal=None, smd=None, gmd=None,
+                 windowed=False, group_scope=False) -> np.ndarray:
+    """The diagonal-filter aggregation risks (n for patch scope, 1 for group
+    scope) of an (n, P) group; gmd is the group metric diagonal of this n."""
+    a = np.ascontiguousarray(atten, dtype=np.float32)
+    n = a.shape[0]
+    p = int(a.size // n)
+    s = np.ascontiguousarray(sigma, dtype=np.float32)
+    sp = np.ascontiguousarray(signal if signal is not None else np.zeros_like(a), dtype=np.float32)
+    gi = np.ascontiguousarray(group_inv, dtype=np.float32)
+    sm = None if smd is None else np.ascontiguousarray(smd, dtype=np.float64)
+    gm = None if gmd is None else np.ascontiguousarray(gmd, dtype=np.float64)
+    out = np.empty(n, dtype=np.float64)
+    k = lib().oracle_weight_risks(_p(a), _p(s), _p(sp), n, p, _p(gi), _p(sm), _p(gm),
+                                  WEIGHT_MODELS[model], int(windowed), int(group_scope), _p(out))
+    return out[:k]
 
 
 GAIN_MODES = {"variance-scaled": 0, "classic": 1, "noise-floor": 2}
@@ -646,6 +698,62 @@ def poisson_reference_thresholds(vol: np.ndarray, params, ht_match_threshold: fl
                + float(structure_beta) * float(np.mean(cnt)) ** float(intensity_power))
         out[i] = np.float32(thr)
     return out
+
+
+def group_inverse(n: int) -> np.ndarray:
+    """Synthesis matrix of the member axis: the transpose of the Haar
+    matrix for powers of two, of the DCT-II otherwise (transforms.py:411-440)."""
+    f = haar(n) if n & (n - 1) == 0 else dct(n)[0]
+    return np.ascontiguousarray(f.T)
+
+
+def _stage_synthesis(params, stage):
+    nd = params.ndim
+    block = tuple(params.ht_block[:nd] if stage == "ht" else params.wi_block[:nd])
+    mats = params._keep[0 if stage == "ht" else 1]
+    o = sum(b * b for b in block)
+    inv = []
+    for b in block:
+        inv.append(np.asarray(mats[o:o + b * b], np.float32).reshape(b, b))
+        o += b * b
+    beta = params.ht_beta if stage == "ht" else params.wi_beta
+    return block, inv, kaiser_nd(block, beta)
+
+
+def synthesis_metric_tables(params, stage: str):
+    """A stage's spatial metric diagonal [P] and its group metric diagonals
+    (n entries for n = 1..max, concatenated), as
+    aggregationweights.py:79-120 builds them: the Kronecker product of the
+    spatial synthesis matrices S in float64, diag(S^T diag(w^2) S) with the
+    float32 Kaiser window w, and diag(Ginv^T Ginv)."""
+    _, inv, window = _stage_synthesis(params, stage)
+    synth = np.array([[1.0]], dtype=np.float64)
+    for m in inv:
+        synth = np.kron(synth, m.astype(np.float64))
+    w = window.astype(np.float64).reshape(-1)
+    smd = np.diag(synth.T @ ((w ** 2)[:, None] * synth)).copy()
+    max_m = params.ht_max if stage == "ht" else params.wi_max
+    gmd = []
+    for n in range(1, max_m + 1):
+        gi = group_inverse(n).astype(np.float64)
+        gmd.append(np.diag(gi.T @ gi).copy())
+    return smd, np.concatenate(gmd)
+
+
+def set_weight_options(params, stage: str, model: str = "variance", windowed: bool = False,
+                       group_scope: bool = False) -> None:
+    """A stage's aggregation weight model, domain and scope
+    (aggregationweights.py:123-259), with the metric tables it reads."""
+    pre = "ht_" if stage == "ht" else "wi_"
+    setattr(params, pre + "wmodel", WEIGHT_MODELS[model])
+    setattr(params, pre + "windowed", int(windowed))
+    setattr(params, pre + "group_scope", int(group_scope))
+    smd, gmd = synthesis_metric_tables(params, stage)
+    keep = getattr(params, "_w_keep", {})
+    keep[stage] = (smd, gmd)
+    params._w_keep = keep
+    setattr(params, pre + "smd", smd.ctypes.data)
+    setattr(params, pre + "gmd", gmd.ctypes.data)
 
 
 def set_ht_thresholds(params, thresholds: np.ndarray) -> None:

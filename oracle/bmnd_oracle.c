@@ -1017,9 +1017,123 @@ void oracle_patch_weights(const float *atten, const float *sigma, int n,
   }
 }
 
+/* numpy's float64 pairwise sum of a contiguous array (np.sum(x,
+ * dtype=np.float64)): the float32 routine's tree in float64 */
+static double np_pairwise_f64(const double *a, int64_t n) {
+  if (n < 8) {
+    double r = 0.0;
+    for (int64_t i = 0; i < n; ++i) r += a[i];
+    return r;
+  }
+  if (n <= 128) {
+    double r[8];
+    for (int j = 0; j < 8; ++j) r[j] = a[j];
+    int64_t i = 8;
+    for (; i < n - (n % 8); i += 8)
+      for (int j = 0; j < 8; ++j) r[j] += a[i + j];
+    double res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+    for (; i < n; ++i) res += a[i];
+    return res;
+  }
+  int64_t n2 = n / 2;
+  n2 -= n2 % 8;
+  return np_pairwise_f64(a, n2) + np_pairwise_f64(a + n2, n - n2);
+}
+
+double oracle_np_sum_f64(const double *a, int64_t n) {
+  double r = 0.0;
+  for (int64_t i = 0; i < n; i += 8192) r += np_pairwise_f64(a + i, n - i < 8192 ? n - i : 8192);
+  return r;
+}
+
+/* Aggregation risks of the diagonal filter models (aggregationweights.py:
+ * 123-211 group scope, 214-259 patch scope): per coefficient the error
+ * a^2 (model 1, classic), a^2 sigma^2 (0, variance) or a^2 sigma^2 +
+ * (1 - a)^2 S (2, risk; S the signal power), in float64 from the float32
+ * inputs; in the windowed-synthesis domain times the spatial metric
+ * diagonal smd[c] (and, group scope, the group metric diagonal gmd[i] first).
+ * Patch scope: risk_i = sum_j Ginv[i,j]^2 * sum_c error[j,c] (sequential
+ * sums); group scope: one risk, numpy's float64 sum of all errors.  Returns
+ * the number of risks written (n or 1). */
+int oracle_weight_risks(const float *atten, const float *sigma, const float *spow, int n, int p,
+                        const float *group_inv, const double *smd, const double *gmd,
+                        int model, int windowed, int group_scope, double *risks) {
+  double *err = (double *)malloc(sizeof(double) * (size_t)n * p);
+  for (int j = 0; j < n; ++j) {
+    for (int c = 0; c < p; ++c) {
+      const int e = j * p + c;
+      const double a = (double)atten[e];
+      const float nv = sigma[e] * sigma[e];
+      double x = a * a;
+      if (model != 1) x *= (double)nv;
+      if (model == 2) {
+        const double rej = 1.0 - a;
+        x += (rej * rej) * (double)spow[e];
+      }
+      if (windowed) {
+        if (group_scope) x *= gmd[j];
+        x *= smd[c];
+      }
+      err[e] = x;
+    }
+  }
+  int out = 1;
+  if (group_scope) {
+    risks[0] = oracle_np_sum_f64(err, (int64_t)n * p);
+  } else {
+    double plane[64];
+    for (int j = 0; j < n; ++j) {
+      double r = 0.0;
+      for (int c = 0; c < p; ++c) r += err[j * p + c];
+      plane[j] = r;
+    }
+    for (int i = 0; i < n; ++i) {
+      double r = 0.0;
+      for (int j = 0; j < n; ++j) {
+        const double m = (double)group_inv[i * n + j];
+        r += (m * m) * plane[j];
+      }
+      risks[i] = r;
+    }
+    out = n;
+  }
+  free(err);
+  return out;
+}
+
+/* weights from oracle_weight_risks: f32(1/(risk + eps)) per member, the
+ * group's one risk for every member in group scope, 0 for a risk that is
+ * not finite or negative (aggregationweights.py:617-687) */
+void oracle_weights_ex(const float *atten, const float *sigma, const float *spow, int n, int p,
+                       const float *group_inv, const double *smd, const double *gmd, int model,
+                       int windowed, int group_scope, double eps, float *w_out) {
+  double risks[64];
+  const int nr = oracle_weight_risks(atten, sigma, spow, n, p, group_inv, smd, gmd, model,
+                                     windowed, group_scope, risks);
+  for (int i = 0; i < n; ++i) {
+    const double r = risks[nr == 1 ? 0 : i];
+    w_out[i] = (!isfinite(r) || r < 0.0) ? 0.0f : (float)(1.0 / (r + eps));
+  }
+}
+
 /* ======================================================================== */
 /* shrinkage and scatter components                                          */
 /* ======================================================================== */
+
+/* soft threshold (bmndalgo.py:475-483): s = max(|C| - thr, 0) with
+ * thr = float32(lambda * sigma), atten = s / |C| (0 where |C| = 0), C =
+ * sign(C) * s with numpy's sign (0 at +-0), all float32 */
+void oracle_ht_soft(float *coef, const float *sigma, int64_t n, float lambda_, float *atten) {
+  for (int64_t e = 0; e < n; ++e) {
+    const float thr = lambda_ * sigma[e];
+    const float mag = fabsf(coef[e]);
+    float shr = mag - thr;
+    shr = shr > 0.0f ? shr : 0.0f;
+    atten[e] = mag > 0.0f ? shr / mag : 0.0f;
+    const float c = coef[e];
+    coef[e] = c > 0.0f ? shr : (c < 0.0f ? -shr : 0.0f);
+  }
+}
 
 /* Poisson hard threshold (poisson.py:30-46 with bmndalgo.py:460-485): keep a
  * coefficient iff |C|/cs > thr/cs + 128*eps32*max(|C|/cs, thr/cs, 1), compared
@@ -1051,8 +1165,18 @@ void oracle_wiener_shrink(float *coef, const float *ref_coef, const float *sigma
 /* wiener.py:24-53, 78-95: signal power |r|^2 less mu sigma^2 (mode 0,
  * variance-scaled), kept whole (1, classic) or less sigma^2 (2, noise-floor),
  * floored at 0; gain = power / (power + mu sigma^2 + eps) clipped to [0, 1]. */
+static void wiener_shrink_power(float *coef, const float *ref_coef, const float *sigma,
+                                int64_t n, float mu, int mode, float *atten, float *spow_out);
+
 void oracle_wiener_shrink_mode(float *coef, const float *ref_coef, const float *sigma,
                                int64_t n, float mu, int mode, float *atten) {
+  wiener_shrink_power(coef, ref_coef, sigma, n, mu, mode, atten, NULL);
+}
+
+/* as oracle_wiener_shrink_mode, also writing the signal power (the risk
+ * weight model's S, wiener.py:24-53) into spow_out unless NULL */
+static void wiener_shrink_power(float *coef, const float *ref_coef, const float *sigma,
+                                int64_t n, float mu, int mode, float *atten, float *spow_out) {
   const float eps32 = (float)1e-10;
   for (int64_t e = 0; e < n; ++e) {
     float nv = sigma[e] * sigma[e];
@@ -1063,6 +1187,7 @@ void oracle_wiener_shrink_mode(float *coef, const float *ref_coef, const float *
       spow = pr - (mode == 2 ? nv : eff);
       if (spow < 0.0f) spow = 0.0f;
     }
+    if (spow_out) spow_out[e] = spow;
     float gain = spow / ((spow + eff) + eps32);
     if (gain < 0.0f) gain = 0.0f;
     if (gain > 1.0f) gain = 1.0f;
@@ -1332,6 +1457,7 @@ static int run_stage(const oracle_params *prm, int wiener, const float *match_vo
     float *Y = (float *)malloc(sizeof(float) * (size_t)nmax * P);
     float *sig = (float *)malloc(sizeof(float) * (size_t)nmax * P);
     float *att = (float *)malloc(sizeof(float) * (size_t)nmax * P);
+    float *spw = (float *)malloc(sizeof(float) * (size_t)nmax * P);
     int64_t *apos = (int64_t *)malloc(sizeof(int64_t) * (size_t)nmax * ndim);
     int last_n = -1;
 #ifdef _OPENMP
@@ -1366,15 +1492,26 @@ static int run_stage(const oracle_params *prm, int wiener, const float *match_vo
       oracle_apply_transform(X, ndim + 1, dims, fm);
       oracle_noise_std(var_vol, ndim, g.shape, block, n, apos, gfwd, spf, prm->k, sig);
       if (!wiener) {
-        oracle_ht_shrink(X, sig, (int64_t)n * P, prm->ht_lambda, prm->count_scale, att);
+        if (prm->ht_soft)
+          oracle_ht_soft(X, sig, (int64_t)n * P, prm->ht_lambda, att);
+        else
+          oracle_ht_shrink(X, sig, (int64_t)n * P, prm->ht_lambda, prm->count_scale, att);
       } else {
         oracle_apply_transform(Y, ndim + 1, dims, fm);
-        oracle_wiener_shrink_mode(X, Y, sig, (int64_t)n * P, prm->wi_mu, prm->wi_gain_mode, att);
+        wiener_shrink_power(X, Y, sig, (int64_t)n * P, prm->wi_mu, prm->wi_gain_mode, att, spw);
       }
       oracle_apply_transform(X, ndim + 1, dims, im);
       float *fo = filt + off[r] * P;
       memcpy(fo, X, sizeof(float) * (size_t)n * P);
-      oracle_patch_weights(att, sig, n, P, ginv, 1e-10, wts + off[r]);
+      {
+        const int wm = wiener ? prm->wi_wmodel : prm->ht_wmodel;
+        const int wd = wiener ? prm->wi_windowed : prm->ht_windowed;
+        const int ws = wiener ? prm->wi_group_scope : prm->ht_group_scope;
+        const double *smd = wiener ? prm->wi_smd : prm->ht_smd;
+        const double *gmd = wiener ? prm->wi_gmd : prm->ht_gmd;
+        oracle_weights_ex(att, sig, spw, n, P, ginv, smd, gmd ? gmd + n * (n - 1) / 2 : NULL, wm,
+                          wd, ws, 1e-10, wts + off[r]);
+      }
       if (gsig) memcpy(gsig + off[r] * P, sig, sizeof(float) * (size_t)n * P);
       if (gsig1) gsig1[r] = oracle_np_mean_f32(sig, (int64_t)n * P);
     }
@@ -1384,6 +1521,7 @@ static int run_stage(const oracle_params *prm, int wiener, const float *match_vo
     free(Y);
     free(sig);
     free(att);
+    free(spw);
     free(apos);
   }
 

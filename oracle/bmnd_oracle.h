@@ -99,6 +99,14 @@ void oracle_patch_weights(const float *atten, const float *sigma, int n,
 
 void oracle_ht_shrink(float *coef, const float *sigma, int64_t n, float lambda_,
                       double count_scale, float *atten);
+void oracle_ht_soft(float *coef, const float *sigma, int64_t n, float lambda_, float *atten);
+double oracle_np_sum_f64(const double *a, int64_t n);
+int oracle_weight_risks(const float *atten, const float *sigma, const float *spow, int n, int p,
+                        const float *group_inv, const double *smd, const double *gmd,
+                        int model, int windowed, int group_scope, double *risks);
+void oracle_weights_ex(const float *atten, const float *sigma, const float *spow, int n, int p,
+                       const float *group_inv, const double *smd, const double *gmd, int model,
+                       int windowed, int group_scope, double eps, float *w_out);
 void oracle_wiener_shrink(float *coef, const float *ref_coef, const float *sigma,
                           int64_t n, float mu, float *atten);
 void oracle_wiener_shrink_mode(float *coef, const float *ref_coef, const float *sigma,
@@ -169,6 +177,17 @@ typedef struct {
   int null_max;
   /* Wiener gain signal power: 0 variance-scaled, 1 classic, 2 noise-floor */
   int wi_gain_mode;
+  /* HT soft thresholding (ht_use_soft_thresholding) and each stage's
+   * aggregation weights: model 0 variance, 1 classic, 2 risk (Wiener);
+   * the windowed-synthesis domain with its spatial metric diagonal smd [P]
+   * and group metric diagonals gmd (n entries at offset n(n-1)/2); the group
+   * scope */
+  int ht_soft;
+  int ht_wmodel, wi_wmodel;
+  int ht_windowed, wi_windowed;
+  int ht_group_scope, wi_group_scope;
+  const double *ht_smd, *wi_smd;
+  const double *ht_gmd, *wi_gmd;
 } oracle_params;
 
 /* Runs both stages.  pilot may be NULL.  The match lists of each stage can

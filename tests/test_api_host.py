@@ -87,15 +87,16 @@ def test_all_zero_volume_fast_path_runs_no_kernel():
     assert set(maps) == {"ht", "wiener"}
 
 
-@pytest.mark.parametrize("field,value", [
-    ("wiener_weight_model", "risk"),
-    ("ht_weight_scope", "group"),
-    ("poisson_group_mass_conservation", "ht"),
-    ("blockmatch_distance", "ssd"),
+@pytest.mark.parametrize("fields", [
+    {"wiener_weight_domain": "windowed_synthesis", "wiener_weight_scope": "group"},
+    {"ht_weight_domain": "windowed_synthesis", "ht_weight_scope": "group"},
+    {"poisson_group_mass_conservation": "ht"},
+    {"blockmatch_distance": "ssd"},
 ])
-def test_unsupported_options_raise_before_any_gpu_work(field, value):
+def test_unsupported_options_raise_before_any_gpu_work(fields):
     p = _poisson()
-    setattr(p, field, value)
+    for field, value in fields.items():
+        setattr(p, field, value)
     with pytest.raises(NotImplementedError):
         bmnd(np.ones((32, 32)), p)
 

@@ -55,12 +55,15 @@ def test_thresholds_entry_needs_the_array():
     assert "ref_thresholds must not be null" in _error(rc)
 
 
-@pytest.mark.parametrize("wiener", [-1, 4])
-def test_filter_stage_argument(wiener):
+@pytest.mark.parametrize("code,msg", [
+    (-1, "unknown bits"), (256, "unknown bits"), (3 << 3, "weight model must be"),
+    (2 << 3, "risk weight model is a Wiener-stage option"),
+    (1 | 4, "soft thresholding is a hard-threshold-stage option")])
+def test_filter_code_argument(code, msg):
     rc = _lib.lib().bmnd_hip_filter_groups(
-        wiener, DUMMY, DUMMY, DUMMY, SHAPE, BLOCK, RADIUS, DUMMY, DUMMY, 10, 16, DUMMY, DUMMY,
+        code, DUMMY, DUMMY, DUMMY, SHAPE, BLOCK, RADIUS, DUMMY, DUMMY, 10, 16, DUMMY, DUMMY,
         3.0, 0, 1.0, DUMMY, DUMMY, None)
-    assert "wiener must be 0" in _error(rc)
+    assert msg in _error(rc)
 
 
 @pytest.mark.parametrize("table_max,scale", [(101, 1.0), (-1, 1.0), (50, 2.0)])
