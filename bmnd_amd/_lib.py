@@ -34,6 +34,8 @@ EXPORTS = (
     "bmnd_hip_reference_map",
     "bmnd_hip_aggregate",
     "bmnd_hip_aggregate_sigma",
+    "bmnd_hip_aggregate_norm",
+    "bmnd_hip_mass_correct",
     "bmnd_hip_poisson_global_sigma",
     "bmnd_hip_np_sum_host",
     "bmnd_hip_aggregate_tile",
@@ -152,6 +154,11 @@ def _declare(L: ctypes.CDLL) -> None:
     L.bmnd_hip_aggregate_sigma.argtypes = [vp, vp, vp, vp, vp, i64, i32, vp, vp, vp, vp, vp, vp,
                                            vp, i32, f32, vp, vp]
     L.bmnd_hip_aggregate_sigma.restype = i32
+    L.bmnd_hip_aggregate_norm.argtypes = [i32, vp, vp, vp, vp, vp, i64, i32, vp, vp, vp, vp, vp,
+                                          vp, vp, i32, f32, vp, vp]
+    L.bmnd_hip_aggregate_norm.restype = i32
+    L.bmnd_hip_mass_correct.argtypes = [vp, vp, vp, vp, i64, i32, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.bmnd_hip_mass_correct.restype = i32
     L.bmnd_hip_poisson_global_sigma.argtypes = [vp, i64, f64, ctypes.POINTER(ctypes.c_double), vp]
     L.bmnd_hip_poisson_global_sigma.restype = i32
     L.bmnd_hip_np_sum_host.argtypes = [vp, i64]

@@ -2367,6 +2367,211 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// group-mass correction (poisson.py:166-265)
+// ---------------------------------------------------------------------------
+struct MassArgs {
+  float *filt;  // in: each group's shrunk coefficients; out: its patches
+  const float *weights;
+  const int32_t *sel, *counts;
+  int64_t nrefs;
+  int max_m;
+  int n0, n1, n2;
+  int c0, c1, c2;
+  const float *den, *src, *kaiser, *mats;
+  const double *mdir, *sdir;  // constant-synthesis directions (matrix table)
+  const bmnd_hip_input *batch;
+  const int32_t *ref_input;
+};
+
+__device__ __forceinline__ void bind_input(MassArgs &a, int64_t g) {
+  if (!a.batch) return;
+  const bmnd_hip_input &d = a.batch[a.ref_input[g]];
+  a.den += d.vox_off;
+  a.src += d.vox_off;
+  a.n0 = d.shape[0];
+  a.n1 = d.shape[1];
+  a.n2 = d.shape[2];
+  a.c0 = d.grid[0];
+  a.c1 = d.grid[1];
+  a.c2 = d.grid[2];
+}
+
+// sum of one value per thread, returned to every thread (s_red: NT/64 + 1)
+template <int NT>
+__device__ __forceinline__ double block_sum_f64(double v, double *s_red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = v + wave_shfl_xor(v, o);
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < NT / 64; ++w) t = t + s_red[w];
+    s_red[NT / 64] = t;
+  }
+  __syncthreads();
+  const double r = s_red[NT / 64];
+  __syncthreads();
+  return r;
+}
+
+// One workgroup per group: the output functional O = f32(w_i k_p) / den at
+// every member voxel (patchaggregation.py:71-98), its image F under the
+// adjoint of the synthesis (M^T along the member axis and each spatial
+// axis), the target sum O * source and the correction of the coefficients
+// along the constant-synthesis direction D (the outer product of the
+// per-axis solutions of M d = 1, scaled by its largest magnitude) that makes
+// sum F * C' equal the target; then the inverse transform of C' into the
+// group's patches.  Float64 throughout, block reductions in a fixed order.
+template <int B0, int B1, int B2, int NMAX, int NT>
+__global__ __launch_bounds__(NT) void k_mass_correct(MassArgs a) {
+  constexpr int P = B0 * B1 * B2;
+  constexpr int NP = NMAX * P;
+  __shared__ float s_c[NP];
+  __shared__ double s_f[NP];
+  __shared__ __align__(16) float s_gi[NMAX * NMAX];
+  __shared__ float s_si[3][64];
+  __shared__ double s_red[NT / 64 + 1];
+  __shared__ int s_pos[NMAX][3];
+  const int tid = threadIdx.x;
+  const int64_t g = blockIdx.x;
+  if (g >= a.nrefs) return;
+  bind_input(a, g);
+  const int n = a.counts[g];
+  if (n <= 0) return;
+  const int np = n * P;
+  const int64_t goff = (int64_t)(n - 1) * n * (2 * n - 1) / 6;
+  const int64_t gtot = (int64_t)a.max_m * (a.max_m + 1) * (2 * a.max_m + 1) / 6;
+  const float *gi = a.mats + gtot + goff;
+  constexpr int SF = B0 * B0 + B1 * B1 + B2 * B2;
+  const float *si = a.mats + 2 * gtot + SF;  // synthesis matrices follow the analysis ones
+  for (int i = tid; i < n * n; i += NT) s_gi[(i / n) * NMAX + i % n] = gi[i];
+  for (int j = tid; j < SF; j += NT) {
+    const int d = j < B0 * B0 ? 0 : (j < B0 * B0 + B1 * B1 ? 1 : 2);
+    const int e = j - (d > 0 ? B0 * B0 : 0) - (d > 1 ? B1 * B1 : 0);
+    s_si[d][e] = si[j];
+  }
+  if (tid < n) {
+    const int lin = a.sel[g * a.max_m + tid];
+    s_pos[tid][2] = lin % a.c2;
+    s_pos[tid][1] = (lin / a.c2) % a.c1;
+    s_pos[tid][0] = lin / (a.c2 * a.c1);
+  }
+  __syncthreads();
+  float *row = a.filt + (g * a.max_m) * (int64_t)P;
+  double tpart = 0.0;
+  for (int e = tid; e < np; e += NT) {
+    const int i = e / P, p = e - i * P;
+    const int p2 = p % B2, p1 = (p / B2) % B1, p0 = p / (B2 * B1);
+    const int64_t v =
+        ((int64_t)(s_pos[i][0] + p0) * a.n1 + (s_pos[i][1] + p1)) * a.n2 + (s_pos[i][2] + p2);
+    const float ww = a.weights[g * a.max_m + i] * a.kaiser[p];
+    const double o = (double)(ww / a.den[v]);
+    s_f[e] = o;
+    s_c[e] = row[e];
+    tpart = tpart + o * (double)a.src[v];
+  }
+  double target = block_sum_f64<NT>(tpart, s_red);
+  // adjoint synthesis, member axis then spatial axes 0, 1, 2 (lines in place)
+  for (int p = tid; p < P; p += NT) {
+    double x[NMAX];
+#pragma unroll
+    for (int k = 0; k < NMAX; ++k) x[k] = k < n ? s_f[k * P + p] : 0.0;
+    for (int j = 0; j < n; ++j) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < NMAX; ++k)
+        if (k < n) acc = acc + (double)s_gi[k * NMAX + j] * x[k];
+      s_f[j * P + p] = acc;
+    }
+  }
+  __syncthreads();
+  auto spatial_adjoint = [&](auto bsz, int d, int stride) {
+    constexpr int B = decltype(bsz)::value;
+    if constexpr (B > 1) {
+      const int lines = np / B;
+      for (int l = tid; l < lines; l += NT) {
+        // the line's first element: split l into the index above the axis
+        // (hi) and below it (lo, < stride)
+        const int hi = l / stride, lo = l - hi * stride;
+        const int base = hi * stride * B + lo;
+        double x[B];
+#pragma unroll
+        for (int k = 0; k < B; ++k) x[k] = s_f[base + k * stride];
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+          double acc = 0.0;
+#pragma unroll
+          for (int k = 0; k < B; ++k) acc = acc + (double)s_si[d][k * B + j] * x[k];
+          s_f[base + j * stride] = acc;
+        }
+      }
+      __syncthreads();
+    }
+  };
+  spatial_adjoint(std::integral_constant<int, B0>{}, 0, B1 * B2);
+  spatial_adjoint(std::integral_constant<int, B1>{}, 1, B2);
+  spatial_adjoint(std::integral_constant<int, B2>{}, 2, 1);
+  double dpart = 0.0;
+  for (int e = tid; e < np; e += NT) dpart = dpart + s_f[e] * s_f[e];
+  const double denom = block_sum_f64<NT>(dpart, s_red);
+  double fscale = 1.0;
+  bool ok = true;
+  if (denom <= 1e-10) {
+    double mx = 0.0;
+    for (int e = tid; e < np; e += NT) mx = fmax(mx, fabs(s_f[e]));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, wave_shfl_xor(mx, o));
+    if ((tid & 63) == 0) s_red[tid >> 6] = mx;
+    __syncthreads();
+    if (tid == 0) {
+      double t = 0.0;
+      for (int w = 0; w < NT / 64; ++w) t = fmax(t, s_red[w]);
+      s_red[NT / 64] = t;
+    }
+    __syncthreads();
+    fscale = s_red[NT / 64];
+    __syncthreads();
+    ok = fscale > 0.0 && isfinite(fscale);
+    if (ok) target = target / fscale;
+  }
+  const double *dm = a.mdir + n * (n - 1) / 2;
+  const double *d0 = a.sdir, *d1 = a.sdir + B0, *d2 = a.sdir + B0 + B1;
+  double dscale = 0.0;
+  {
+    double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+    for (int k = 0; k < n; ++k) m0 = fmax(m0, fabs(dm[k]));
+    for (int k = 0; k < B0; ++k) m1 = fmax(m1, fabs(d0[k]));
+    for (int k = 0; k < B1; ++k) m2 = fmax(m2, fabs(d1[k]));
+    for (int k = 0; k < B2; ++k) m3 = fmax(m3, fabs(d2[k]));
+    dscale = ((m0 * m1) * m2) * m3;
+  }
+  ok = ok && dscale > 0.0 && isfinite(dscale);
+  auto dir = [&](int e) {
+    const int i = e / P, p = e - i * P;
+    const int p2 = p % B2, p1 = (p / B2) % B1, p0 = p / (B2 * B1);
+    return (((dm[i] * d0[p0]) * d1[p1]) * d2[p2]) / dscale;
+  };
+  double cpart = 0.0, rpart = 0.0;
+  for (int e = tid; e < np; e += NT) {
+    const double f = s_f[e] / fscale;
+    cpart = cpart + f * (double)s_c[e];
+    rpart = rpart + f * dir(e);
+  }
+  const double current = block_sum_f64<NT>(cpart, s_red);
+  const double response = block_sum_f64<NT>(rpart, s_red);
+  const double residual = target - current;
+  if (ok && residual != 0.0 && isfinite(response) && response != 0.0) {
+    const double corr = residual / response;
+    for (int e = tid; e < np; e += NT) s_c[e] = (float)((double)s_c[e] + corr * dir(e));
+  }
+  __syncthreads();
+  group_transform<NT, B0, B1, B2, NMAX>(s_c, n, s_gi, s_si, [&](int e, float v) {
+    row[e] = v;
+    return v;
+  });
+}
+
+// ---------------------------------------------------------------------------
 // aggregation
 // ---------------------------------------------------------------------------
 // p[0..n) = v: clears flags and maps with a kernel instead of a memset node
@@ -2410,6 +2615,10 @@ struct AggArgs {
   const bmnd_hip_input *batch;
   int ninputs;
   int64_t ntiles;
+  // 0: num / (den + eps) where den > 0 (bmndalgo.py:626-631, 1039-1044);
+  // 1: num / den where den > 0 (the group-mass correction's np.divide,
+  // bmndalgo.py:618-626, 1031-1038); 2: den itself
+  int norm_mode;
 };
 
 constexpr int kAggThreads = 512;
@@ -2658,7 +2867,12 @@ __global__ __launch_bounds__(T0 * T1 * T2) void k_aggregate(AggArgs a) {
   }
   if (inside) {
     const int64_t v = ((int64_t)x0 * a.n1 + x1) * a.n2 + x2;
-    a.est[v] = den > 0.0f ? num / (den + 1e-10f) : a.fallback[v];
+    if (a.norm_mode == 2)
+      a.est[v] = den;
+    else if (a.norm_mode == 1)
+      a.est[v] = den > 0.0f ? num / den : a.fallback[v];
+    else
+      a.est[v] = den > 0.0f ? num / (den + 1e-10f) : a.fallback[v];
     if constexpr (SIGMA) a.sig_map[v] = den > 0.0f ? snum / den : a.sig_fill;  // bmndalgo.py:1048-1057
   }
 }
@@ -3535,7 +3749,8 @@ static int32_t aggregate_impl(const float *filt, const float *weights, const int
                               const float *kaiser, const float *fallback, float *est,
                               const float *group_sigma, int32_t per_group, float fill,
                               float *sigma_map, void *stream, const BatchRef *b,
-                              int32_t *over_ws = nullptr, bool defer_check = false) {
+                              int32_t *over_ws = nullptr, bool defer_check = false,
+                              int32_t norm_mode = 0) {
   int c[3];
   int64_t shape1[3];
   int64_t ntiles = 0;
@@ -3585,6 +3800,8 @@ static int32_t aggregate_impl(const float *filt, const float *weights, const int
   a.batch = b ? b->d : nullptr;
   a.ninputs = b ? b->n : 0;
   a.ntiles = ntiles;
+  if (norm_mode < 0 || norm_mode > 2) return fail(BMND_HIP_EINVAL, "norm_mode must be 0, 1 or 2");
+  a.norm_mode = norm_mode;
   const bool with_sigma = sigma_map != nullptr;
   hipStream_t s = (hipStream_t)stream;
   // the capacity flag: in the caller's scratch (cleared by a kernel) when
@@ -3724,6 +3941,70 @@ int32_t bmnd_hip_aggregate_sigma(const float *filt, const float *weights, const 
   return aggregate_impl(filt, weights, sel, counts, ref_index, nrefs, max_matches, shape_h,
                         block_h, radius_h, kaiser, fallback, est, group_sigma, per_group, fill,
                         sigma_map, stream, nullptr);
+}
+
+int32_t bmnd_hip_aggregate_norm(int32_t norm_mode, const float *filt, const float *weights,
+                                const int32_t *sel, const int32_t *counts,
+                                const int32_t *ref_index, int64_t nrefs, int32_t max_matches,
+                                const int64_t *shape_h, const int32_t *block_h,
+                                const int32_t *radius_h, const float *kaiser,
+                                const float *fallback, float *est, const float *group_sigma,
+                                int32_t per_group, float fill, float *sigma_map, void *stream) {
+  return aggregate_impl(filt, weights, sel, counts, ref_index, nrefs, max_matches, shape_h,
+                        block_h, radius_h, kaiser, fallback, est, group_sigma, per_group, fill,
+                        sigma_map, stream, nullptr, nullptr, false, norm_mode);
+}
+
+int32_t bmnd_hip_mass_correct(float *filt, const float *weights, const int32_t *sel,
+                              const int32_t *counts, int64_t nrefs, int32_t max_matches,
+                              const int64_t *shape_h, const int32_t *block_h,
+                              const int32_t *radius_h, const float *den, const float *source,
+                              const float *kaiser, const float *matrices, void *stream) {
+  int c[3];
+  if (int rc = check_geometry(shape_h, block_h, radius_h, c)) return rc;
+  if (nrefs <= 0) return 0;
+  if (!filt || !weights || !sel || !counts || !den || !source || !kaiser || !matrices)
+    return fail(BMND_HIP_EINVAL, "bmnd_hip_mass_correct: null buffer");
+  MassArgs a;
+  a.filt = filt;
+  a.weights = weights;
+  a.sel = sel;
+  a.counts = counts;
+  a.nrefs = nrefs;
+  a.max_m = max_matches;
+  a.n0 = (int)shape_h[0];
+  a.n1 = (int)shape_h[1];
+  a.n2 = (int)shape_h[2];
+  a.c0 = c[0];
+  a.c1 = c[1];
+  a.c2 = c[2];
+  a.den = den;
+  a.src = source;
+  a.kaiser = kaiser;
+  a.mats = matrices;
+  {
+    const int64_t P = (int64_t)block_h[0] * block_h[1] * block_h[2];
+    const int64_t m = max_matches;
+    const double *t64 = reinterpret_cast<const double *>(
+        matrices + matrix_table_f64_offset(max_matches, block_h));
+    a.mdir = t64 + P + m * (m + 1) / 2;
+    a.sdir = t64 + P + m * (m + 1);
+  }
+  a.batch = nullptr;
+  a.ref_input = nullptr;
+  hipStream_t s = (hipStream_t)stream;
+  dim3 grid((unsigned)nrefs);
+#define M_CASE(b0, b1, b2, r0, r1, r2, nmax, nt)                                            \
+  if (same3(block_h, b0, b1, b2) && same3(radius_h, r0, r1, r2) && max_matches <= nmax) {   \
+    hipLaunchKernelGGL((k_mass_correct<b0, b1, b2, nmax, nt>), grid, dim3(nt), 0, s, a);   \
+    HIP_TRY(hipGetLastError());                                                            \
+    return 0;                                                                              \
+  }
+  M_CASE(1, 8, 8, 0, 19, 19, 32, 256)
+  M_CASE(4, 4, 4, 7, 7, 7, 16, 256)
+  M_CASE(5, 5, 5, 7, 7, 7, 32, 512)
+#undef M_CASE
+  return fail(BMND_HIP_EINVAL, "block/radius/stack configuration not compiled into this library");
 }
 
 int32_t bmnd_hip_aggregate_batch(const float *filt, const float *weights, const int32_t *sel,

@@ -187,13 +187,6 @@ def check_supported(profile: BMNDProfile, resolved) -> None:
     def field(name, enum):
         return coerce_enum(getattr(profile, name), enum, name)
 
-    checks = [
-        (field("poisson_group_mass_conservation", PoissonGroupMassConservation)
-         is PoissonGroupMassConservation.NONE, "poisson_group_mass_conservation='none'"),
-    ]
-    for ok, what in checks:
-        if not ok:
-            raise NotImplementedError(f"the MI355X path requires {what}.")
     for stage in ("ht", "wiener"):
         # the covariance-aware exact group risk (bmndalgo.py:359-377, 752-758;
         # wiener.py:638-844) replaces the diagonal weights for these choices
@@ -386,8 +379,15 @@ class PoissonPipeline:
         # the filters' stage argument: 0 hard threshold, 1-3 the Wiener gain
         self.wiener_arg = _GAIN_ARG[coerce_enum(profile.poisson_wiener_gain_mode,
                                                 PoissonWienerGainMode, "poisson_wiener_gain_mode")]
+        # group-mass conservation per stage (bmndalgo.py:263-266, 704-711)
+        mass = coerce_enum(profile.poisson_group_mass_conservation, PoissonGroupMassConservation,
+                           "poisson_group_mass_conservation")
+        self.mass = {"ht": mass in (PoissonGroupMassConservation.HT, PoissonGroupMassConservation.BOTH),
+                     "wiener": mass in (PoissonGroupMassConservation.WIENER,
+                                        PoissonGroupMassConservation.BOTH)}
         # each stage's full filter code (bmnd_hip_filter_groups)
-        self.filter_codes = {st: filter_code(profile, st, self.wiener_arg) for st in ("ht", "wiener")}
+        self.filter_codes = {st: filter_code(profile, st, self.wiener_arg, self.mass[st])
+                             for st in ("ht", "wiener")}
         self._null = None
         self.k = int(profile.k)
         self.plans = {s: make_plan(profile, s, self.ndim) for s in ("ht", "wiener")}
@@ -575,11 +575,31 @@ class PoissonPipeline:
         est = torch.empty_like(noisy)
         smap = torch.empty_like(noisy) if gsig is not None else None
         tok = timer.start(f"{name}.aggregate")
-        _lib.check(L.bmnd_hip_aggregate_sigma(_ptr(filt), _ptr(wts), _ptr(sel), _ptr(cnt),
-                                              _ptr(ref_index), R, plan.max_m, shape, block,
-                                              radius, _ptr(kai), _ptr(fallback), _ptr(est),
-                                              _ptr(gsig), int(wiener),
-                                              float(sigma_fill or 0.0), _ptr(smap), stream))
+        if self.mass[name]:
+            # group-mass conservation (poisson.py:235-265): the denominator
+            # (and sigma map) of the plain aggregation, every group's
+            # coefficients corrected against it and synthesised, then the
+            # numerator over the unchanged denominator without eps
+            den = torch.empty_like(noisy)
+            _lib.check(L.bmnd_hip_aggregate_norm(2, _ptr(filt), _ptr(wts), _ptr(sel), _ptr(cnt),
+                                                 _ptr(ref_index), R, plan.max_m, shape, block,
+                                                 radius, _ptr(kai), _ptr(fallback), _ptr(den),
+                                                 _ptr(gsig), int(wiener),
+                                                 float(sigma_fill or 0.0), _ptr(smap), stream))
+            _lib.check(L.bmnd_hip_mass_correct(_ptr(filt), _ptr(wts), _ptr(sel), _ptr(cnt), R,
+                                               plan.max_m, shape, block, radius, _ptr(den),
+                                               _ptr(noisy), _ptr(kai), _ptr(mats), stream))
+            _lib.check(L.bmnd_hip_aggregate_norm(1, _ptr(filt), _ptr(wts), _ptr(sel), _ptr(cnt),
+                                                 _ptr(ref_index), R, plan.max_m, shape, block,
+                                                 radius, _ptr(kai), _ptr(fallback), _ptr(est),
+                                                 None, 0, 0.0, None, stream))
+            del den
+        else:
+            _lib.check(L.bmnd_hip_aggregate_sigma(_ptr(filt), _ptr(wts), _ptr(sel), _ptr(cnt),
+                                                  _ptr(ref_index), R, plan.max_m, shape, block,
+                                                  radius, _ptr(kai), _ptr(fallback), _ptr(est),
+                                                  _ptr(gsig), int(wiener),
+                                                  float(sigma_fill or 0.0), _ptr(smap), stream))
         timer.stop(tok)
         del filt, wts, ref_index, gsig
         return StageOutput(est, sel.view(R, plan.max_m), cnt, acc, plan.lazy_schedule, smap)
@@ -857,6 +877,9 @@ class BatchPipeline:
         self.device = torch.device(device if device is not None else "cuda")
         # matching-mode choice and the per-call constants are PoissonPipeline's
         self._single = PoissonPipeline(profile, shapes[0], self.device, deviance)
+        if any(self._single.mass.values()):
+            raise NotImplementedError("group-mass conservation runs input by input (bmnd()); "
+                                      "bmnd_batch does not take it.")
         self.stages = {name: self._stage_view(profile, name) for name in ("ht", "wiener")}
         self.last_runs: dict[str, list] = {}  # (mode, first input, end) per matching launch
 

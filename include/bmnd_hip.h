@@ -248,6 +248,37 @@ int32_t bmnd_hip_aggregate_sigma(const float *filt, const float *weights,
                                  int32_t per_group, float fill, float *sigma_map,
                                  void *stream);
 
+/* bmnd_hip_aggregate_sigma with a choice of normalisation: norm_mode 0 is
+ * num / (den + 1e-10) where den > 0 (the others' rule, bmndalgo.py:626-631,
+ * 1039-1044); 1 is num / den where den > 0, the fallback elsewhere (the
+ * np.divide of a group-mass-conserving stage, bmndalgo.py:618-626,
+ * 1031-1038); 2 writes den itself into est (the denominator the mass
+ * correction reads).  group_sigma / sigma_map may be NULL. */
+int32_t bmnd_hip_aggregate_norm(int32_t norm_mode, const float *filt, const float *weights,
+                                const int32_t *sel, const int32_t *counts,
+                                const int32_t *ref_index, int64_t nrefs, int32_t max_matches,
+                                const int64_t *shape_h, const int32_t *block_h,
+                                const int32_t *radius_h, const float *kaiser,
+                                const float *fallback, float *est, const float *group_sigma,
+                                int32_t per_group, float fill, float *sigma_map, void *stream);
+
+/* Group-mass correction of every group (poisson.py:166-265,
+ * patchaggregation.py:71-98), in place on filt, which holds each group's
+ * shrunk coefficients (bmnd_hip_filter_groups with filter code bit 7): the
+ * output functional O = f32(w_i k_p) / den at every member voxel, the target
+ * sum O * source, and the correction of the coefficients along the
+ * constant-synthesis direction (matrix table) that makes the synthesised
+ * group carry it; filt then holds the inverse-transformed patches for
+ * bmnd_hip_aggregate_norm(1, ...).  den is the stage's aggregation
+ * denominator (bmnd_hip_aggregate_norm(2, ...)), source the observation.
+ * Float64 sums in a fixed order (a tolerance restatement of the reference's
+ * numpy sums). */
+int32_t bmnd_hip_mass_correct(float *filt, const float *weights, const int32_t *sel,
+                              const int32_t *counts, int64_t nrefs, int32_t max_matches,
+                              const int64_t *shape_h, const int32_t *block_h,
+                              const int32_t *radius_h, const float *den, const float *source,
+                              const float *kaiser, const float *matrices, void *stream);
+
 /* Global noise std of the Poisson variance model, the "global" entry of the
  * sigma maps: sigmaestimation.estimate_poisson_global_sigma
  * (sigmaestimation.py:48-93) on the n-voxel variance volume var (device),

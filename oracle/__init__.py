@@ -81,6 +81,11 @@ class OracleParams(ctypes.Structure):
         ("wi_smd", ctypes.c_void_p),
         ("ht_gmd", ctypes.c_void_p),
         ("wi_gmd", ctypes.c_void_p),
+        ("mass_mode", ctypes.c_int),
+        ("ht_mdir", ctypes.c_void_p),
+        ("wi_mdir", ctypes.c_void_p),
+        ("ht_sdir", ctypes.c_void_p),
+        ("wi_sdir", ctypes.c_void_p),
     ]
 
 
@@ -134,6 +139,7 @@ def _declare(L: ctypes.CDLL) -> None:
     L.oracle_weight_risks.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, i32, i32, i32, vp]
     L.oracle_weight_risks.restype = i32
     L.oracle_weights_ex.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, i32, i32, i32, f64, vp]
+    L.oracle_mass_correct.argtypes = [i32, vp, vp, vp, vp, vp, vp, f64]
     L.oracle_wiener_shrink.argtypes = [vp, vp, vp, i64, f32, vp]
     L.oracle_wiener_shrink_mode.argtypes = [vp, vp, vp, i64, f32, i32, vp]
     L.oracle_scatter_add.argtypes = [vp, vp, i32, vp, vp, vp, vp, vp, vp, i32]
@@ -754,6 +760,46 @@ def set_weight_options(params, stage: str, model: str = "variance", windowed: bo
     params._w_keep = keep
     setattr(params, pre + "smd", smd.ctypes.data)
     setattr(params, pre + "gmd", gmd.ctypes.data)
+
+
+def mass_correct(coef, source, functional, inverse_mats, directions, eps=1e-10) -> np.ndarray:
+    """One group's mass correction (poisson.py:166-232): the coefficients
+    corrected along the constant-synthesis direction (the outer product of
+    the per-axis `directions`) so that the synthesis by `inverse_mats` (one
+    per axis, None for identity) carries sum(functional * source)."""
+    c = np.ascontiguousarray(coef, dtype=np.float32).copy()
+    src = np.ascontiguousarray(source, dtype=np.float32)
+    O = np.ascontiguousarray(functional, dtype=np.float64)
+    dims = np.asarray(c.shape, dtype=np.int32)
+    mats = [None if m is None else np.ascontiguousarray(m, dtype=np.float32) for m in inverse_mats]
+    dirs = [np.ascontiguousarray(d, dtype=np.float64) for d in directions]
+    mp = (ctypes.c_void_p * len(mats))(*[None if m is None else m.ctypes.data for m in mats])
+    dp = (ctypes.c_void_p * len(dirs))(*[d.ctypes.data for d in dirs])
+    lib().oracle_mass_correct(c.ndim, _p(dims), mp, dp, _p(O), _p(src), _p(c), eps)
+    return c
+
+
+MASS_MODES = {"none": 0, "ht": 1, "wiener": 2, "both": 3}
+
+
+def set_mass_conservation(params, mode: str) -> None:
+    """poisson_group_mass_conservation with each stage's constant-synthesis
+    directions: per axis the solution of M d = 1 for its synthesis matrix M
+    in float64 (poisson.py:120-160), the member axis for every n."""
+    params.mass_mode = MASS_MODES[mode]
+    keep = {}
+    for stage in ("ht", "wiener"):
+        _, inv, _ = _stage_synthesis(params, stage)
+        max_m = params.ht_max if stage == "ht" else params.wi_max
+        mdir = np.concatenate([np.linalg.solve(group_inverse(n).astype(np.float64), np.ones(n))
+                               for n in range(1, max_m + 1)])
+        sdir = np.concatenate([np.linalg.solve(m.astype(np.float64), np.ones(m.shape[0]))
+                               for m in inv])
+        keep[stage] = (mdir, sdir)
+        pre = "ht_" if stage == "ht" else "wi_"
+        setattr(params, pre + "mdir", mdir.ctypes.data)
+        setattr(params, pre + "sdir", sdir.ctypes.data)
+    params._mass_keep = keep
 
 
 def set_ht_thresholds(params, thresholds: np.ndarray) -> None:

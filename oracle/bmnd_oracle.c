@@ -1393,6 +1393,107 @@ static void variance_volume(const float *v, int64_t n, double scale, double floo
 }
 
 /* one stage: block matching + group filtering + sequential aggregation */
+/* Group-mass correction of one group (poisson.py:166-232 with
+ * patchaggregation.py:71-98): the output functional O = f32(w_i k_p) / den
+ * at each member voxel (float32, as float64), its image under the adjoint of
+ * the synthesis (M^T along every axis, float64), the target sum O * source,
+ * and the correction of the coefficients along the constant-synthesis
+ * direction D (the outer product of the per-axis directions dirs[a], scaled
+ * by its largest magnitude) that makes the synthesised group carry the
+ * target.  coef (float32, n x P) is corrected in place; sums are sequential
+ * float64.  dims[0] = n, dims[1..ndim] the block; mats[a] the synthesis
+ * matrix of axis a (NULL: identity). */
+/* The correction itself for a given output functional O (float64, the
+ * group's shape): oracle_mass_correct over nax axes. */
+void oracle_mass_correct(int nax, const int *dims, const float *const *mats,
+                         const double *const *dirs, const double *O, const float *src_vox,
+                         float *coef, double eps) {
+  int64_t total = prod_int(dims, nax);
+  double *cf = (double *)malloc(sizeof(double) * (size_t)total);
+  double *tmp = (double *)malloc(sizeof(double) * (size_t)total);
+  double target = 0.0;
+  for (int64_t e = 0; e < total; ++e) {
+    cf[e] = O[e];
+    target += cf[e] * (double)src_vox[e];
+  }
+  for (int a = 0; a < nax; ++a) {
+    const float *m = mats[a];
+    if (!m) continue;
+    const int n = dims[a];
+    int64_t inner = 1;
+    for (int d = a + 1; d < nax; ++d) inner *= dims[d];
+    const int64_t outer = total / (inner * n);
+    for (int64_t o = 0; o < outer; ++o)
+      for (int j = 0; j < n; ++j)
+        for (int64_t in = 0; in < inner; ++in) {
+          double acc = 0.0;
+          const double *x = cf + o * n * inner + in;
+          for (int k = 0; k < n; ++k) acc += (double)m[k * n + j] * x[k * inner];
+          tmp[o * n * inner + j * inner + in] = acc;
+        }
+    memcpy(cf, tmp, sizeof(double) * (size_t)total);
+  }
+  double denom = 0.0;
+  for (int64_t e = 0; e < total; ++e) denom += cf[e] * cf[e];
+  if (denom <= eps) {
+    double sc = 0.0;
+    for (int64_t e = 0; e < total; ++e) sc = fabs(cf[e]) > sc ? fabs(cf[e]) : sc;
+    if (!(sc > 0.0) || !isfinite(sc)) goto done;
+    for (int64_t e = 0; e < total; ++e) cf[e] /= sc;
+    target /= sc;
+  }
+  {
+    double current = 0.0;
+    for (int64_t e = 0; e < total; ++e) current += cf[e] * (double)coef[e];
+    const double residual = target - current;
+    if (residual == 0.0) goto done;
+    double dscale = 1.0;
+    for (int a = 0; a < nax; ++a) {
+      double mx = 0.0;
+      for (int k = 0; k < dims[a]; ++k) mx = fabs(dirs[a][k]) > mx ? fabs(dirs[a][k]) : mx;
+      dscale *= mx;
+    }
+    if (!(dscale > 0.0) || !isfinite(dscale)) goto done;
+    double response = 0.0;
+    for (int64_t e = 0; e < total; ++e) {
+      int64_t r = e;
+      double dv = 1.0;
+      int idx[ORACLE_MAX_NDIM + 1];
+      for (int a = nax - 1; a >= 0; --a) {
+        idx[a] = (int)(r % dims[a]);
+        r /= dims[a];
+      }
+      dv = dirs[0][idx[0]];
+      for (int a = 1; a < nax; ++a) dv *= dirs[a][idx[a]];
+      tmp[e] = dv / dscale;
+      response += cf[e] * tmp[e];
+    }
+    if (!isfinite(response) || response == 0.0) goto done;
+    const double corr = residual / response;
+    for (int64_t e = 0; e < total; ++e) coef[e] = (float)((double)coef[e] + corr * tmp[e]);
+  }
+done:
+  free(cf);
+  free(tmp);
+}
+
+static void mass_correct_group(int ndim, const int *dims, const float *const *mats,
+                               const double *const *dirs, const float *w, const float *kaiser,
+                               const float *den_vox, const float *src_vox, float *coef,
+                               double eps) {
+  const int nax = ndim + 1;
+  const int64_t total = prod_int(dims, nax);
+  const int P = (int)(total / dims[0]);
+  double *O = (double *)malloc(sizeof(double) * (size_t)total);
+  for (int64_t e = 0; e < total; ++e) {
+    const int i = (int)(e / P), p = (int)(e % P);
+    const float ww = w[i] * kaiser[p];
+    O[e] = (double)(ww / den_vox[e]);
+  }
+  oracle_mass_correct(nax, dims, mats, dirs, O, src_vox, coef, eps);
+  free(O);
+}
+
 static int run_stage(const oracle_params *prm, int wiener, const float *match_vol,
                      const float *pilot, const float *noisy, const float *var_vol,
                      const float *fallback,
@@ -1406,6 +1507,9 @@ static int run_stage(const oracle_params *prm, int wiener, const float *match_vo
   float thr = wiener ? prm->wi_threshold : prm->ht_threshold;
   stage_geom g;
   geom_init(&g, prm, block, wiener ? prm->wi_beta : prm->ht_beta, wiener);
+  /* group-mass conservation of this stage (poisson.py:49-265, bmndalgo.py:
+   * 263-266, 537-548, 610-626, 704-711, 959-970, 1023-1038) */
+  const int mass = wiener ? (prm->mass_mode & 2) != 0 : (prm->mass_mode & 1) != 0;
   int ndim = g.ndim, P = g.P;
   int64_t nvox = 1;
   for (int d = 0; d < ndim; ++d) nvox *= g.shape[d];
@@ -1500,9 +1604,9 @@ static int run_stage(const oracle_params *prm, int wiener, const float *match_vo
         oracle_apply_transform(Y, ndim + 1, dims, fm);
         wiener_shrink_power(X, Y, sig, (int64_t)n * P, prm->wi_mu, prm->wi_gain_mode, att, spw);
       }
-      oracle_apply_transform(X, ndim + 1, dims, im);
+      if (!mass) oracle_apply_transform(X, ndim + 1, dims, im);
       float *fo = filt + off[r] * P;
-      memcpy(fo, X, sizeof(float) * (size_t)n * P);
+      memcpy(fo, X, sizeof(float) * (size_t)n * P);  /* mass: the shrunk coefficients */
       {
         const int wm = wiener ? prm->wi_wmodel : prm->ht_wmodel;
         const int wd = wiener ? prm->wi_windowed : prm->ht_windowed;
@@ -1538,9 +1642,71 @@ static int run_stage(const oracle_params *prm, int wiener, const float *match_vo
       oracle_scatter_add(num, den, ndim, g.shape, block, g.kaiser, filt + off[r] * P,
                          wts + off[r], sel + r * max_m, cnt[r]);
   }
-  /* normalisation, bmndalgo.py:626-631 / 1039-1044 */
-  float eps32 = (float)1e-10;
-  for (int64_t v = 0; v < nvox; ++v) est[v] = den[v] > 0.0f ? num[v] / (den[v] + eps32) : fallback[v];
+  if (mass) {
+    /* the denominator stands; every group's coefficients are corrected
+     * against it, synthesised and scattered into a fresh numerator
+     * (poisson.py:235-265) */
+    const double *mdir = wiener ? prm->wi_mdir : prm->ht_mdir;
+    const double *sdir = wiener ? prm->wi_sdir : prm->ht_sdir;
+#ifdef _OPENMP
+#pragma omp parallel
+#endif
+    {
+      float *dv = (float *)malloc(sizeof(float) * (size_t)max_m * P);
+      float *sv = (float *)malloc(sizeof(float) * (size_t)max_m * P);
+      float *gfwd = (float *)malloc(sizeof(float) * (size_t)max_m * max_m);
+      float *ginv = (float *)malloc(sizeof(float) * (size_t)max_m * max_m);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 8)
+#endif
+      for (int64_t r = 0; r < nrefs; ++r) {
+        const int n = cnt[r];
+        if (n == 0) continue;
+        group_matrix(n, gfwd, ginv);
+        int dims[ORACLE_MAX_NDIM + 1];
+        const float *im[ORACLE_MAX_NDIM + 1];
+        const double *dirs[ORACLE_MAX_NDIM + 1];
+        dims[0] = n;
+        im[0] = ginv;
+        dirs[0] = mdir + n * (n - 1) / 2;
+        int so = 0;
+        for (int d = 0; d < ndim; ++d) {
+          dims[d + 1] = block[d];
+          im[d + 1] = g.sp_inv[d];
+          dirs[d + 1] = sdir + so;
+          so += block[d];
+        }
+        for (int i = 0; i < n; ++i) {
+          const int64_t base = lin_to_vol_base(&g, sel[r * max_m + i]);
+          for (int p = 0; p < P; ++p) {
+            dv[i * P + p] = den[base + g.poff[p]];
+            sv[i * P + p] = noisy[base + g.poff[p]];
+          }
+        }
+        float *c = filt + off[r] * P;
+        mass_correct_group(ndim, dims, im, dirs, wts + off[r], g.kaiser, dv, sv, c, 1e-10);
+        oracle_apply_transform(c, ndim + 1, dims, im);
+      }
+      free(dv);
+      free(sv);
+      free(gfwd);
+      free(ginv);
+    }
+    memset(num, 0, sizeof(float) * (size_t)nvox);
+    float *den2 = (float *)calloc((size_t)nvox, sizeof(float));
+    for (int64_t r = 0; r < nrefs; ++r)
+      oracle_scatter_add(num, den2, ndim, g.shape, block, g.kaiser, filt + off[r] * P,
+                         wts + off[r], sel + r * max_m, cnt[r]);
+    free(den2);
+    /* np.divide(num, den, where=den > 0) over a copy of the input
+     * (bmndalgo.py:618-626, 1031-1038): no eps */
+    for (int64_t v = 0; v < nvox; ++v) est[v] = den[v] > 0.0f ? num[v] / den[v] : fallback[v];
+  } else {
+    /* normalisation, bmndalgo.py:626-631 / 1039-1044 */
+    float eps32 = (float)1e-10;
+    for (int64_t v = 0; v < nvox; ++v)
+      est[v] = den[v] > 0.0f ? num[v] / (den[v] + eps32) : fallback[v];
+  }
   /* sigma map, bmndalgo.py:1048-1057: the global sigma where nothing landed */
   if (snum)
     for (int64_t v = 0; v < nvox; ++v) sigma_map[v] = den[v] > 0.0f ? snum[v] / den[v] : sigma_fill;

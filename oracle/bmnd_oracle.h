@@ -104,6 +104,9 @@ double oracle_np_sum_f64(const double *a, int64_t n);
 int oracle_weight_risks(const float *atten, const float *sigma, const float *spow, int n, int p,
                         const float *group_inv, const double *smd, const double *gmd,
                         int model, int windowed, int group_scope, double *risks);
+void oracle_mass_correct(int nax, const int *dims, const float *const *mats,
+                         const double *const *dirs, const double *O, const float *src_vox,
+                         float *coef, double eps);
 void oracle_weights_ex(const float *atten, const float *sigma, const float *spow, int n, int p,
                        const float *group_inv, const double *smd, const double *gmd, int model,
                        int windowed, int group_scope, double eps, float *w_out);
@@ -188,6 +191,12 @@ typedef struct {
   int ht_group_scope, wi_group_scope;
   const double *ht_smd, *wi_smd;
   const double *ht_gmd, *wi_gmd;
+  /* group-mass conservation (poisson_group_mass_conservation): bit 0 HT,
+   * bit 1 Wiener, with each stage's constant-synthesis directions: mdir n
+   * entries for each n at n(n-1)/2, sdir the block's axes one after another */
+  int mass_mode;
+  const double *ht_mdir, *wi_mdir;
+  const double *ht_sdir, *wi_sdir;
 } oracle_params;
 
 /* Runs both stages.  pilot may be NULL.  The match lists of each stage can

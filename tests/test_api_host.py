@@ -90,7 +90,6 @@ def test_all_zero_volume_fast_path_runs_no_kernel():
 @pytest.mark.parametrize("fields", [
     {"wiener_weight_domain": "windowed_synthesis", "wiener_weight_scope": "group"},
     {"ht_weight_domain": "windowed_synthesis", "ht_weight_scope": "group"},
-    {"poisson_group_mass_conservation": "ht"},
     {"blockmatch_distance": "ssd"},
 ])
 def test_unsupported_options_raise_before_any_gpu_work(fields):

@@ -122,5 +122,8 @@ def test_policy_support():
     prof.poisson_ht_match_policy = PoissonHTMatchPolicy.CANDIDATE_STANDARDIZED
     check_supported(prof, resolve_profile(prof, (16, 16, 16)))
     prof.poisson_group_mass_conservation = "ht"
-    with pytest.raises(NotImplementedError, match="poisson_group_mass_conservation"):
+    check_supported(prof, resolve_profile(prof, (16, 16, 16)))
+    prof.wiener_weight_domain = "windowed_synthesis"
+    prof.wiener_weight_scope = "group"
+    with pytest.raises(NotImplementedError, match="exact covariance-aware"):
         check_supported(prof, resolve_profile(prof, (16, 16, 16)))
