@@ -2,10 +2,11 @@
 
 Same signature, argument validation order, error messages and return
 conventions as the reference's src/bmnd/bmndalgo.py:71-183; the work runs on
-the MI355X device pipeline (engine.PoissonPipeline).  Profile options the
-device pipeline does not run (engine.check_supported: group-mass
-conservation, risk weighting and the classic/risk weight models, soft
-thresholding, the SSD distance, the Gaussian noise model) raise
+the MI355X device pipelines (engine.PoissonPipeline for Poisson noise,
+gaussian.GaussianPipeline for stationary Gaussian noise).  Profile options
+the device pipelines do not run (engine.check_supported: the exact
+covariance-aware group risk, the Poisson SSD distance, non-SSD distances
+under Gaussian noise, transforms and geometries not compiled in) raise
 NotImplementedError after the reference's own validation has passed.
 """
 
@@ -19,7 +20,9 @@ import torch
 from .blockmatching import BlockMatchResult, build_result
 from .enums import CacheMode, coerce_enum
 from .engine import BatchPipeline, PoissonPipeline, check_supported, table_first_order
+from .gaussian import GaussianPipeline
 from .profilevalidation import resolve_profile
+from .psd import validate_sigma_psd
 
 
 def _validate_sigma(sigma: object) -> float:
@@ -74,6 +77,17 @@ def _pipeline(profile, shape, device) -> PoissonPipeline:
     return pipe
 
 
+def _gaussian_pipeline(profile, shape, device, resolved) -> GaussianPipeline:
+    key = ("gaussian", repr(profile), tuple(shape), str(device))
+    pipe = _PIPELINES.get(key)
+    if pipe is None:
+        if len(_PIPELINES) > 8:
+            _PIPELINES.clear()
+        pipe = GaussianPipeline(profile, shape, device, resolved)
+        _PIPELINES[key] = pipe
+    return pipe
+
+
 def _validated_input(volume, profile, sigma, sigma_psd, max_cache_bytes):
     """The reference's argument checks, in its order (bmndalgo.py:124-155):
     returns the float32 input and the resolved profile."""
@@ -95,6 +109,8 @@ def _validated_input(volume, profile, sigma, sigma_psd, max_cache_bytes):
         raise ValueError("volume values must be representable as finite float32 values.")
     if resolved.is_poisson and np.any(vol < 0.0):
         raise ValueError("Direct Poisson input must contain only non-negative values.")
+    if sigma_psd is not None:
+        validate_sigma_psd(tuple(int(s) for s in vol.shape), sigma_psd)
     return vol, resolved
 
 
@@ -103,7 +119,9 @@ def bmnd(volume, profile, sigma: float | None = None, sigma_psd=None,
          stage_callback: Callable[[str, np.ndarray], None] | None = None,
          blockmatch_callback: Callable[[str, BlockMatchResult], None] | None = None,
          max_cache_bytes=CacheMode.AUTO):
-    """Denoise an N-D Poisson volume (2-D and 3-D on the MI355X path).
+    """Denoise an N-D Poisson or stationary-Gaussian volume (2-D and 3-D on
+    the MI355X path); for Gaussian noise `sigma` or `sigma_psd` gives the
+    noise level, else it is estimated from the volume.
 
     Returns a float32 array of the input shape or, with ``return_sigma_map``,
     ``(array, {"global": g, "ht": map, "wiener": map})`` as the reference
@@ -123,6 +141,9 @@ def bmnd(volume, profile, sigma: float | None = None, sigma_psd=None,
     check_supported(profile, resolved)
     vol = np.ascontiguousarray(vol)
     device = _device()
+    if not resolved.is_poisson:
+        return _bmnd_gaussian(vol, profile, resolved, device, sigma, sigma_psd,
+                              return_sigma_map, stage_callback, blockmatch_callback)
     pipe = _pipeline(profile, vol.shape, device)
     if (USE_GRAPHS and vol.size <= GRAPH_MAX_VOXELS and stage_callback is None
             and blockmatch_callback is None and not return_sigma_map and pipe.graph_supported()):
@@ -157,6 +178,44 @@ def bmnd(volume, profile, sigma: float | None = None, sigma_psd=None,
     if return_sigma_map:
         # bmndalgo.py:1048-1062: per-stage maps, global sigma where no patch
         # reaches a voxel
+        return out, {"global": global_sigma, "ht": ht.sigma_map.cpu().numpy(),
+                     "wiener": wi.sigma_map.cpu().numpy()}
+    return out
+
+
+def _match_hook(pipe, shape, blockmatch_callback):
+    """stage -> hook(sel, counts, accepted) calling blockmatch_callback(stage,
+    result) after that stage's matching (bmndalgo.py:325-326, 742-743)."""
+    def make(stage: str):
+        if blockmatch_callback is None:
+            return None
+        plan = pipe.plans[stage]
+        counts = tuple(shape[d] - plan.block[d] + 1 for d in range(len(shape)))
+
+        def hook(sel, cnt, acc):
+            blockmatch_callback(stage, build_result(plan.schedule, counts, sel.cpu().numpy(),
+                                                    cnt.cpu().numpy(), acc.cpu().numpy()))
+        return hook
+    return make
+
+
+def _bmnd_gaussian(vol, profile, resolved, device, sigma, sigma_psd, return_sigma_map,
+                   stage_callback, blockmatch_callback):
+    """The stationary Gaussian branches of bmndalgo.py:203-1063 on the device."""
+    pipe = _gaussian_pipeline(profile, vol.shape, device, resolved)
+    vol_t = torch.from_numpy(vol).to(device)
+
+    def on_pilot(pilot):
+        if stage_callback is not None:
+            stage_callback("ht", pilot.cpu().numpy())
+    ht, wi, global_sigma = pipe.run(vol_t, sigma=sigma, sigma_psd=sigma_psd,
+                                    sigma_maps=return_sigma_map,
+                                    on_match=_match_hook(pipe, vol.shape, blockmatch_callback),
+                                    on_pilot=on_pilot, vol_host=vol)
+    out = wi.est.cpu().numpy()
+    if stage_callback is not None:
+        stage_callback("wiener", out.copy())
+    if return_sigma_map:
         return out, {"global": global_sigma, "ht": ht.sigma_map.cpu().numpy(),
                      "wiener": wi.sigma_map.cpu().numpy()}
     return out

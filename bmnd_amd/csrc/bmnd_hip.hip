@@ -292,6 +292,50 @@ __global__ void k_count_range(const float *__restrict__ v, int64_t n, int32_t *f
 }
 
 // ---------------------------------------------------------------------------
+// value range (min, max) of a volume: the SSD thresholds scale with
+// (max - min)^2.  Each float is mapped to an order-preserving unsigned key
+// for integer atomics; the last kernel maps the two keys back.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned ordered_key(float x) {
+  const unsigned b = __float_as_uint(x);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+__global__ void k_range_init(unsigned *keys) {
+  if (threadIdx.x == 0) {
+    keys[0] = 0xffffffffu;
+    keys[1] = 0u;
+  }
+}
+
+__global__ void k_range(const float *__restrict__ v, int64_t n, unsigned *keys) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  unsigned lo = 0xffffffffu, hi = 0u;
+  for (; i < n; i += stride) {
+    const unsigned k = ordered_key(v[i]);
+    lo = min(lo, k);
+    hi = max(hi, k);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = min(lo, (unsigned)__shfl_xor((int)lo, o, 64));
+    hi = max(hi, (unsigned)__shfl_xor((int)hi, o, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMin(&keys[0], lo);
+    atomicMax(&keys[1], hi);
+  }
+}
+
+__global__ void k_range_done(unsigned *keys) {
+  if (threadIdx.x < 2) {
+    const unsigned m = keys[threadIdx.x];
+    const unsigned b = (m & 0x80000000u) ? (m & 0x7fffffffu) : ~m;
+    reinterpret_cast<float *>(keys)[threadIdx.x] = __uint_as_float(b);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // block matching
 // ---------------------------------------------------------------------------
 struct BMArgs {
@@ -324,6 +368,10 @@ struct BMArgs {
   int dcode;
   const float *null_mom;
   int null_max;
+  // MODE 10 (SSD): the noise SSD of every search-box lag ((2R+1) C order) and
+  // its factor gamma, subtracted from each candidate's SSD; nullptr: none
+  const float *noise_ssd;
+  float gamma;
 };
 
 // A batched launch takes the geometry of reference g's input and moves the
@@ -619,7 +667,10 @@ __device__ __forceinline__ void bm_finalize(int K, int acc, const BMArgs &a, int
 //         reference writes it (blockmatching.py:356-369);
 // MODE 6: the candidate-standardized distance of code a.dcode
 //         (blockmatching.py:374-426): (sum of terms - sum of null means) /
-//         sqrt(sum of null variances), the null sums in float32.
+//         sqrt(sum of null variances), the null sums in float32;
+// MODE 10: the SSD of the Gaussian model, a float32 serial sum in patch C
+//         order less gamma times the noise SSD at the candidate's lag
+//         (blockmatching.py:320-334, 1133-1139); signed, no tie grid.
 // Occupancy targets of the count-table kernel: 6 waves per SIMD for the 4^3
 // block (80 VGPRs, no spills), 8 for the 8x8 block (64 VGPRs, four workgroups
 // per CU: 21.9 against 22.8 ms on sixteen 512^2 images), 4 for the 5^3 block,
@@ -643,7 +694,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(
   __shared__ int s_acc;
   __shared__ int s_refrow[P];
   // MODE 1: c*log(2c) per slab voxel; MODES 5, 6 with Anscombe: 2 sqrt(c + 3/8)
-  __shared__ double s_lc[MODE == 1 || MODE >= 5 ? SLAB : 1];
+  constexpr bool POISSON_SSD = MODE == 5 || MODE == 6;
+  __shared__ double s_lc[MODE == 1 || POISSON_SSD ? SLAB : 1];
   __shared__ double2 s_flog[MODE == 1 ? kFastLogN : 1];
   extern __shared__ double s_terms[];  // MODE 2 only
 
@@ -693,8 +745,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(
     float v = a.vol[((int64_t)(o0 + p0) * a.n1 + (o1 + p1)) * a.n2 + (o2 + p2)];
     double r = (double)v * a.inv_scale;
     if (r < 0.0) r = 0.0;
-    s_ref[p] = r;
-    if (MODE >= 5) s_ref_log[p] = 2.0 * sqrt(r + 0.375);  // Anscombe transform
+    s_ref[p] = MODE == 10 ? (double)v : r;  // MODE 10: the value itself
+    if (POISSON_SSD) s_ref_log[p] = 2.0 * sqrt(r + 0.375);  // Anscombe transform
     // MODE 2: byte offset of the reference count's term-table row
     if (MODE == 2)
       s_refrow[p] = (int)fminf(fmaxf(v, 0.0f), (float)a.tmax) * (a.tmax + 1) * (int)sizeof(double);
@@ -705,7 +757,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(
   }
   if (tid == 0) s_acc = 0;
   __syncthreads();
-  if (MODE >= 5 && a.dcode == 3) {
+  if (POISSON_SSD && a.dcode == 3) {
     for (int i = tid; i < SLAB; i += NT) {
       double c = (double)s_slab[i] * a.inv_scale;
       if (c < 0.0) c = 0.0;
@@ -790,6 +842,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(
     const int j2 = cand % e2, j1 = (cand / e2) % e1, j0 = cand / (e2 * e1);
     const int base = (j0 * S1 + j1) * S2 + j2;
     double d = 0.0;
+    float d32 = 0.0f;              // MODE 10
     float nms = 0.0f, nvs = 0.0f;  // MODE 6: null mean and variance sums
     // terms are accumulated in patch C order (p0, p1, p2), as the reference does
 #pragma unroll 1
@@ -797,17 +850,20 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(
       const int p0 = p01 / B1, p1 = p01 - p0 * B1;
       const float *row = s_slab + base + (p0 * S1 + p1) * S2;
       const double *lcrow =
-          s_lc + (MODE == 1 || MODE >= 5 ? base + (p0 * S1 + p1) * S2 : 0);
+          s_lc + (MODE == 1 || POISSON_SSD ? base + (p0 * S1 + p1) * S2 : 0);
       const double *rrow = s_ref + p01 * B2;
       const double *lrow = s_ref_log + p01 * B2;
 #pragma unroll
       for (int p2 = 0; p2 < B2; ++p2) {
-        if (MODE == 1) {
+        if (MODE == 10) {
+          const float df = row[p2] - (float)rrow[p2];
+          d32 = d32 + df * df;
+        } else if (MODE == 1) {
           // the term is exactly 0 at r = c = 0 (the log of 0 stays finite)
           const double s = rrow[p2] + (double)row[p2];
           const double ls = dlog_bits(s, s_flog, ln2k);
           d = d + fma(-s, ls, lrow[p2] + lcrow[p2]);
-        } else if (MODE >= 5) {
+        } else if (POISSON_SSD) {
           const double r = rrow[p2];
           double c = (double)row[p2] * a.inv_scale;
           if (c < 0.0) c = 0.0;
@@ -843,7 +899,16 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(
     }
     if (MODE == 1) d = 2.0 * d;  // terms were accumulated without the factor 2
     bool ok;
-    if constexpr (MODE == 6) {
+    if constexpr (MODE == 10) {
+      float key = d32;
+      if (a.noise_ssd) {
+        constexpr int W1 = 2 * R1 + 1, W2 = 2 * R2 + 1;
+        const int li = ((lo0 + j0 - o0 + R0) * W1 + (lo1 + j1 - o1 + R1)) * W2 +
+                       (lo2 + j2 - o2 + R2);
+        key = key - a.gamma * a.noise_ssd[li];
+      }
+      s_keys[cand] = bm_key_signed(key, a, ok);
+    } else if constexpr (MODE == 6) {
       // float32 sqrt, correctly rounded through float64
       const float dstd =
           nvs > 0.0f ? (float)((d - (double)nms) / (double)(float)sqrt((double)nvs)) : 0.0f;
@@ -861,7 +926,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(
   const float thr_k = a.stabilize ? a.thr_key : a.threshold;
   auto composite = [&](int i) {
     const float key = s_keys[i];
-    if constexpr (MODE == 6) return bm_composite_signed(key, i);
+    if constexpr (MODE == 6 || MODE == 10) return bm_composite_signed(key, i);
     else return bm_composite(key, key < thr_k, i);
   };
   bm_select_write<NT, BOX>(bm_box_keys<NT>(composite, box), box, s_acc, a,
@@ -1613,6 +1678,11 @@ struct FilterArgs {
   int coef_out;      // filt gets the shrunk coefficients, not the patches
   const double *smd;  // spatial metric diagonal [P] (matrix table)
   const double *gmd;  // group metric diagonals, n entries at n(n-1)/2
+  // the stationary Gaussian model (filter code bit 8): exact_std holds the
+  // exact planes' variances (k_gaussian_var) and gpv the P coefficient
+  // variances; the Gaussian hard threshold and Wiener gain
+  int gaussian;
+  const float *gpv;
 };
 
 __device__ __forceinline__ void bind_input(FilterArgs &a, int64_t g) {
@@ -1620,7 +1690,7 @@ __device__ __forceinline__ void bind_input(FilterArgs &a, int64_t g) {
   const bmnd_hip_input &d = a.batch[a.ref_input[g]];
   a.noisy += d.vox_off;
   if (a.pilot) a.pilot += d.vox_off;
-  a.var += d.vox_off;
+  if (!a.gaussian) a.var += d.vox_off;
   a.n0 = d.shape[0];
   a.n1 = d.shape[1];
   a.n2 = d.shape[2];
@@ -1883,6 +1953,150 @@ __global__ __launch_bounds__(64) void k_exact_std(ExactArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// exact-plane noise variance of the stationary Gaussian model
+// (variance.py:1889-2106)
+// ---------------------------------------------------------------------------
+// For a group of n members at patch origins x_i and a member analysis matrix
+// G, plane q and spatial coefficient s:
+//   v[q,s] = sum over the nonzero entries f of row q (increasing) of
+//            (G[q,f] C_s(0)) G[q,f], then of (G[q,f] G[q,t]) (C_s(l) + C_s(l))
+//            for the nonzero t < f, with l = (x_f - x_t) mod nf,
+// every product and sum a separately rounded float32 operation, floored at
+// 0.  C_s is the coefficient's cross-covariance map over the nf-periodic
+// domain; it is even in the lag (symmetrised when built), so
+// C_s(x_t - x_f) = C_s(x_f - x_t) and the reference's two reads give the same
+// value.  The maps are stored lag-major, [lag][s], so the lanes of a wave
+// (one coefficient each) read consecutive addresses.
+struct GaussVarArgs {
+  const float *cov;  // [D][P]
+  int d0, d1, d2;    // nf domain (3-D form)
+  int c0, c1, c2;    // patch grid
+  const int32_t *sel, *counts;
+  int64_t nrefs;
+  int max_m;
+  const float *mats;
+  int k, kcap;
+  float *exact_var;  // (nrefs, kcap, P)
+  const bmnd_hip_input *batch;
+  const int32_t *ref_input;
+};
+
+__device__ __forceinline__ void bind_input(GaussVarArgs &a, int64_t g) {
+  if (!a.batch) return;
+  const bmnd_hip_input &d = a.batch[a.ref_input[g]];
+  a.c0 = d.grid[0];
+  a.c1 = d.grid[1];
+  a.c2 = d.grid[2];
+}
+
+// One workgroup per group, one lane per spatial coefficient.  Per plane the
+// terms are listed in LDS in the reference's order (for the f-th nonzero
+// entry: its diagonal term, then its pairs with the earlier ones), each as
+// the float offset of its map row (-1 for a diagonal term) and its weight
+// product; every lane then walks the list, reading C_s at the listed lag.
+template <int P, int NMAX, int NT>
+__global__ __launch_bounds__(NT) void k_gaussian_var(GaussVarArgs a) {
+  static_assert(NT >= P && NT >= NMAX && NT % 64 == 0, "one lane per coefficient and member");
+  constexpr int TERMS = NMAX * (NMAX + 1) / 2;
+  __shared__ int s_pos[NMAX][3];
+  __shared__ int s_idx[NMAX];
+  __shared__ float s_wv[NMAX];
+  __shared__ int s_nnz;
+  __shared__ int2 s_term[TERMS];
+  const int tid = threadIdx.x;
+  const int64_t g = blockIdx.x;
+  if (g >= a.nrefs) return;
+  bind_input(a, g);
+  const int n = a.counts[g];
+  const int planes = min(a.k, n);
+  if (planes <= 0) return;
+  if (tid < n) {
+    const int lin = a.sel[g * a.max_m + tid];
+    s_pos[tid][2] = lin % a.c2;
+    s_pos[tid][1] = (lin / a.c2) % a.c1;
+    s_pos[tid][0] = lin / (a.c2 * a.c1);
+  }
+  const float *gf = a.mats + (int64_t)(n - 1) * n * (2 * n - 1) / 6;
+  const float pv = tid < P ? a.cov[tid] : 0.0f;  // lag 0: the coefficient's variance
+  for (int q = 0; q < planes; ++q) {
+    // nonzero entries of row q in increasing member order
+    if (tid < 64) {
+      const float w = tid < n ? gf[q * n + tid] : 0.0f;
+      const bool nz = w != 0.0f;
+      const unsigned long long m = __ballot(nz);
+      if (nz) {
+        const int r = __popcll(m & ((1ull << tid) - 1ull));
+        s_idx[r] = tid;
+        s_wv[r] = w;
+      }
+      if (tid == 0) s_nnz = __popcll(m);
+    }
+    __syncthreads();
+    const int nnz = s_nnz;
+    const int T = nnz * (nnz + 1) / 2;
+    for (int e = tid; e < T; e += NT) {
+      // e = fo (fo + 1) / 2 + r: r == 0 the diagonal term of fo, else its pair
+      // with the (r-1)-th nonzero entry
+      int fo = (int)((sqrtf(8.0f * (float)e + 1.0f) - 1.0f) * 0.5f);
+      while (fo * (fo + 1) / 2 > e) --fo;
+      while ((fo + 1) * (fo + 2) / 2 <= e) ++fo;
+      const int r = e - fo * (fo + 1) / 2;
+      const float wf = s_wv[fo];
+      int2 t;
+      if (r == 0) {
+        t = make_int2(-1, __float_as_int(wf));
+      } else {
+        const int so = r - 1;
+        const int f = s_idx[fo], s = s_idx[so];
+        int l0 = (s_pos[f][0] - s_pos[s][0]) % a.d0;
+        int l1 = (s_pos[f][1] - s_pos[s][1]) % a.d1;
+        int l2 = (s_pos[f][2] - s_pos[s][2]) % a.d2;
+        l0 += l0 < 0 ? a.d0 : 0;
+        l1 += l1 < 0 ? a.d1 : 0;
+        l2 += l2 < 0 ? a.d2 : 0;
+        t = make_int2(((l0 * a.d1 + l1) * a.d2 + l2) * P, __float_as_int(wf * s_wv[so]));
+      }
+      s_term[e] = t;
+    }
+    __syncthreads();
+    if (tid < P) {
+      float v = 0.0f;
+      // branch-free term: (a * b) * c with (wf, C(0), wf) on the diagonal and
+      // (wf wt, C(l) + C(l), 1) for a pair; times 1 is exact
+      constexpr int U = 8;
+      int e = 0;
+      for (; e + U <= T; e += U) {
+        int2 tt[U];
+        float cv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) tt[u] = s_term[e + u];
+#pragma unroll
+        for (int u = 0; u < U; ++u) cv[u] = a.cov[(tt[u].x < 0 ? 0 : tt[u].x) + tid];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const bool diag = tt[u].x < 0;
+          const float w = __int_as_float(tt[u].y);
+          const float b = diag ? pv : cv[u] + cv[u];
+          const float c = diag ? w : 1.0f;
+          v = v + (w * b) * c;
+        }
+      }
+      for (; e < T; ++e) {
+        const int2 t = s_term[e];
+        const bool diag = t.x < 0;
+        const float w = __int_as_float(t.y);
+        const float cv = a.cov[(diag ? 0 : t.x) + tid];
+        const float b = diag ? pv : cv + cv;
+        const float c = diag ? w : 1.0f;
+        v = v + (w * b) * c;
+      }
+      a.exact_var[(g * a.kcap + q) * P + tid] = v > 0.0f ? v : 0.0f;
+    }
+    __syncthreads();
+  }
+}
+
 // In-place transform along the member axis of an LDS array (n, P):
 //   data[r][c] <- sum_k M[r][k] * data[k][c], one fmaf chain over k = 0..n-1
 // (M out-major, rows stored with stride NMAX).  Lanes own columns (P padded to
@@ -1982,6 +2196,14 @@ __device__ __forceinline__ void group_transform(float *__restrict__ data, int n,
   if constexpr (B0 > 1) line_transform<NT, B0, B1 * B2>(data, n, S[0]);
   if constexpr (B1 > 1) line_transform<NT, B1, B2>(data, n * B0, S[1]);
   line_transform<NT, B2, 1>(data, n * B0 * B1, S[2], epi);
+}
+
+// Gaussian Wiener gain (wiener.py:56-75): |r|^2 / (|r|^2 + mu max(sigma^2,
+// 1e-8)), 0 where that denominator is not positive, clipped to [0, 1]
+__device__ __forceinline__ float gaussian_gain(float pr, float nv, float mu) {
+  const float den = pr + mu * (nv > 1e-8f ? nv : 1e-8f);
+  const float gain = den > 0.0f ? pr / den : 0.0f;
+  return gain < 0.0f ? 0.0f : (gain > 1.0f ? 1.0f : gain);
 }
 
 // Poisson Wiener gain from the signal power sp and mu sigma^2 (wiener.py:78-95)
@@ -2123,7 +2345,7 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
       const int e = tid + it * NT;
       if (e < np) {
         const int64_t v = voxel_of(e);
-        xv[it] = a.var[v];
+        xv[it] = a.gaussian ? 0.0f : a.var[v];
         xo[it] = WIENER ? a.pilot[v] : a.noisy[v];
       }
     }
@@ -2138,13 +2360,39 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
     }
   }
   __syncthreads();
+  // the first KEX exact planes are in LDS; further ones are read from the buffer
+  auto exv = [&](int q, int s) { return q < KEX ? s_ex[q * P + s] : exact[q * P + s]; };
+  if (a.gaussian) {
+    // stationary Gaussian model (variance.py:2072-2106): the exact planes'
+    // variances, the remaining planes sharing (n pv - their float32 sum in
+    // plane order) / (n - q), pv everywhere for k = 0; then the square root
+    // (bmndalgo.py:443-451)
+    static_assert(NMAX * NMAX >= P, "Gaussian scratch does not fit");
+    float *const s_rem = s_g2;
+    for (int s = tid; s < P; s += NT) {
+      const float pv = a.gpv[s];
+      float rem = pv;
+      if (planes > 0 && planes < n) {
+        float tot = 0.0f;
+        for (int q = 0; q < planes; ++q) tot = tot + exv(q, s);
+        rem = ((float)n * pv - tot) / (float)(n - planes);
+        if (rem < 0.0f) rem = 0.0f;
+      }
+      s_rem[s] = rem;
+    }
+    __syncthreads();
+    for (int e = tid; e < np; e += NT) {
+      const int q = e / P, s = e - q * P;
+      const float v = q < planes ? exv(q, s) : s_rem[s];
+      s_sig[e] = sqrtf(v > 0.0f ? v : 0.0f);
+    }
+    __syncthreads();
+  } else {
   // ---- approximate std: squared transforms along every axis (variance.py:613-623)
   group_transform<NT, B0, B1, B2, NMAX>(s_sig, n, s_g2, s_sf2,
                                         [](int, float v) { return sqrtf(v > 0.0f ? v : 0.0f); });
 
   // ---- exact overlap-aware planes (k_exact_std) and plane limiting ----------
-  // the first KEX planes are in LDS; further ones are read from the buffer
-  auto exv = [&](int q, int s) { return q < KEX ? s_ex[q * P + s] : exact[q * P + s]; };
   if (planes == 0) {
     // k = 0: the approximate std is the answer (variance.py:1647-1653)
   } else if (planes == n) {
@@ -2198,6 +2446,7 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
     }
   }
   __syncthreads();
+  }  // Poisson variance
   if constexpr (SIGMA) {
     // the std the reference hands to its sigma-map scatter: HT the limited std
     // of every coefficient, Wiener its float32 mean over the group
@@ -2228,6 +2477,15 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
         atten[e] = mag > 0.0f ? shr / mag : 0.0f;
         return c > 0.0f ? shr : (c < 0.0f ? -shr : 0.0f);
       });
+    } else if (a.gaussian) {
+      // Gaussian hard threshold (bmndalgo.py:470-474): |C| >= lambda sigma,
+      // the group's first coefficient always kept
+      group_transform<NT, B0, B1, B2, NMAX>(s_x, n, s_g, s_sf, [&](int e, float c) {
+        const float thr = lam * s_sig[e];
+        const float keep = (fabsf(c) >= thr || e == 0) ? 1.0f : 0.0f;
+        atten[e] = keep;
+        return c * keep;
+      });
     } else {
       // Poisson hard threshold (poisson.py:30-46, bmndalgo.py:460-485)
       const double cs = a.count_scale;
@@ -2251,9 +2509,16 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
     // gain replaces it after them.
     const float mu = a.shrink;
     const int gmode = a.gain_mode;
+    const bool gauss = a.gaussian != 0;
     group_transform<NT, B0, B1, B2, NMAX>(s_r, n, s_g, s_sf, [&](int e, float r) {
       const float sg = s_sig[e];
       const float nv = sg * sg;
+      if (gauss) {
+        // Gaussian: the signal power is |r|^2 (bmndalgo.py:889-899)
+        const float rc = fabsf(r);
+        const float pr = rc * rc;
+        return risk_model ? pr : gaussian_gain(pr, nv, mu);
+      }
       const float eff = mu * nv;
       const float sp = wiener_signal_power(r, nv, eff, gmode);
       return risk_model ? sp : wiener_gain(sp, eff);
@@ -2274,7 +2539,7 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
     float sp = 0.0f;
     if (risk_model) {
       sp = at;
-      at = wiener_gain(sp, a.shrink * nv);
+      at = a.gaussian ? gaussian_gain(sp, nv, a.shrink) : wiener_gain(sp, a.shrink * nv);
     }
     const double ad = (double)at;
     double err = ad * ad;
@@ -2329,7 +2594,9 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
     __syncthreads();
     for (int e = tid; e < np; e += NT) {
       const float sg = s_sig[e];
-      atten[e] = wiener_gain(atten[e], a.shrink * (sg * sg));
+      const float nv = sg * sg;
+      atten[e] = a.gaussian ? gaussian_gain(atten[e], nv, a.shrink)
+                            : wiener_gain(atten[e], a.shrink * nv);
     }
     __syncthreads();  // the noisy patches overwrite the std buffer next
   }
@@ -3197,6 +3464,21 @@ int32_t bmnd_hip_count_range(const float *vol, int64_t n, int32_t *flags, void *
   return 0;
 }
 
+int32_t bmnd_hip_value_range(const float *vol, int64_t n, float *range, void *stream) {
+  if (!range) return fail(BMND_HIP_EINVAL, "range must not be null");
+  if (n <= 0) return fail(BMND_HIP_EINVAL, "an empty volume has no value range");
+  hipStream_t s = (hipStream_t)stream;
+  unsigned *keys = reinterpret_cast<unsigned *>(range);
+  hipLaunchKernelGGL(k_range_init, dim3(1), dim3(64), 0, s, keys);
+  HIP_TRY(hipGetLastError());
+  const int blocks = (int)std::min<int64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_range, dim3(blocks), dim3(256), 0, s, vol, n, keys);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_range_done, dim3(1), dim3(64), 0, s, keys);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
 static int32_t blockmatch_impl(const float *vol, const int64_t *shape_h, const int32_t *block_h,
                                const int32_t *radius_h, const int32_t *ref_lin, int64_t nrefs,
                                float threshold, int32_t max_matches, int32_t min_matches,
@@ -3205,7 +3487,9 @@ static int32_t blockmatch_impl(const float *vol, const int64_t *shape_h, const i
                                int32_t *accepted, void *stream, const BatchRef *b,
                                const float *ref_thresholds = nullptr,
                                const float *null_mom = nullptr, int32_t null_max = 0,
-                               void *ws = nullptr, int64_t ws_bytes = 0) {
+                               void *ws = nullptr, int64_t ws_bytes = 0,
+                               const float *noise_ssd = nullptr, float gamma = 0.0f,
+                               bool ssd = false) {
   int c[3];
   int64_t shape1[3];
   int64_t grid_total = 0;
@@ -3230,6 +3514,12 @@ static int32_t blockmatch_impl(const float *vol, const int64_t *shape_h, const i
     return fail(BMND_HIP_EINVAL, "standardized modes need the null-moment table (max count >= 1)");
   if (mode == 2 && (count_scale != 1.0 || table_max < 0 || table_max > kMaxTableCount))
     return fail(BMND_HIP_EINVAL, "table mode needs count_scale == 1 and 0 <= table_max <= 100");
+  if (ssd) {
+    // the Gaussian model's SSD (MODE 10): no tie grid, no count scale
+    mode = 10;
+    stabilize = 0;
+    count_scale = 1.0;
+  }
   const double *tab = nullptr;
   if (int rc = logtab_for_device(&tab)) return rc;
   // the one-log slabs assume unit count scale
@@ -3288,6 +3578,8 @@ static int32_t blockmatch_impl(const float *vol, const int64_t *shape_h, const i
   a.dcode = mode == 5 ? 2 : mode == 6 ? 3 : mode >= 7 ? mode - 6 : 1;
   a.null_mom = null_mom;
   a.null_max = null_max;
+  a.noise_ssd = noise_ssd;
+  a.gamma = gamma;
   hipStream_t s = (hipStream_t)stream;
   dim3 grid((unsigned)nrefs), blk(kBMThreads);
   const size_t dyn = mode == 2 ? sizeof(double) * (size_t)(table_max + 1) * (table_max + 1) : 0;
@@ -3367,6 +3659,9 @@ static int32_t blockmatch_impl(const float *vol, const int64_t *shape_h, const i
     else if (mode == 5 || mode == 6)                                                             \
       hipLaunchKernelGGL((k_blockmatch<b0, b1, b2, r0, r1, r2, 5, kBMThreads>), grid, blk, 0, s, \
                          a);                                                                     \
+    else if (mode == 10)                                                                         \
+      hipLaunchKernelGGL((k_blockmatch<b0, b1, b2, r0, r1, r2, 10, kBMThreads>), grid, blk, 0, \
+                         s, a);                                                                  \
     else if (mode >= 7)                                                                          \
       hipLaunchKernelGGL((k_blockmatch<b0, b1, b2, r0, r1, r2, 6, kBMThreads>), grid, blk, 0, s, \
                          a);                                                                     \
@@ -3408,6 +3703,91 @@ int32_t bmnd_hip_blockmatch_thresholds(const float *vol, const int64_t *shape_h,
   return blockmatch_impl(vol, shape_h, block_h, radius_h, ref_lin, nrefs, 0.0f, max_matches,
                          min_matches, pow2, stabilize, count_scale, mode, table_max, sel, counts,
                          accepted, stream, nullptr, ref_thresholds);
+}
+
+int32_t bmnd_hip_blockmatch_ssd(const float *vol, const int64_t *shape_h, const int32_t *block_h,
+                                const int32_t *radius_h, const int32_t *ref_lin, int64_t nrefs,
+                                float threshold, int32_t max_matches, int32_t min_matches,
+                                int32_t pow2, const float *noise_ssd, float gamma, int32_t *sel,
+                                int32_t *counts, int32_t *accepted, void *stream) {
+  // blockmatching.py:244-286, 1446-1460
+  if (!std::isfinite(gamma) || gamma < 0.0f)
+    return fail(BMND_HIP_EINVAL, "gamma must be finite and non-negative");
+  if (gamma != 0.0f && !noise_ssd)
+    return fail(BMND_HIP_EINVAL, "a nonzero gamma needs the noise SSD of the search box");
+  if (!std::isfinite(threshold) || threshold < 0.0f)
+    return fail(BMND_HIP_EINVAL, "distance_threshold values must be finite and non-negative");
+  return blockmatch_impl(vol, shape_h, block_h, radius_h, ref_lin, nrefs, threshold,
+                         max_matches, min_matches, pow2, 0, 1.0, 0, 0, sel, counts, accepted,
+                         stream, nullptr, nullptr, nullptr, 0, nullptr, 0,
+                         gamma != 0.0f ? noise_ssd : nullptr, gamma, true);
+}
+
+static int32_t gaussian_variance_impl(const float *cov, const int32_t *domain_h,
+                                      const int64_t *shape_h, const int32_t *block_h,
+                                      const int32_t *radius_h, const int32_t *sel,
+                                      const int32_t *counts, int64_t nrefs, int32_t max_matches,
+                                      const float *matrices, int32_t k, float *exact_var,
+                                      void *stream, const BatchRef *b) {
+  int c[3];
+  if (b) {
+    if (int rc = check_batch(*b, block_h, radius_h, nullptr, nullptr)) return rc;
+    for (int d = 0; d < 3; ++d) c[d] = b->h[0].grid[d];
+  } else if (int rc = check_geometry(shape_h, block_h, radius_h, c)) {
+    return rc;
+  }
+  if (k < 0) return fail(BMND_HIP_EINVAL, "k must be non-negative");
+  if (!domain_h) return fail(BMND_HIP_EINVAL, "domain must not be null");
+  int64_t cells = 1;
+  for (int d = 0; d < 3; ++d) {
+    if (domain_h[d] < 1) return fail(BMND_HIP_EINVAL, "domain sizes must be positive");
+    cells *= domain_h[d];
+  }
+  const int64_t P = (int64_t)block_h[0] * block_h[1] * block_h[2];
+  if (cells * P > INT32_MAX) return fail(BMND_HIP_EINVAL, "covariance maps too large");
+  if (nrefs <= 0 || k == 0) return 0;
+  if (!cov || !exact_var) return fail(BMND_HIP_EINVAL, "cov and exact_var must not be null");
+  GaussVarArgs a;
+  a.cov = cov;
+  a.d0 = domain_h[0];
+  a.d1 = domain_h[1];
+  a.d2 = domain_h[2];
+  a.c0 = c[0];
+  a.c1 = c[1];
+  a.c2 = c[2];
+  a.sel = sel;
+  a.counts = counts;
+  a.nrefs = nrefs;
+  a.max_m = max_matches;
+  a.mats = matrices;
+  a.k = k;
+  a.kcap = std::min(k, max_matches);
+  a.exact_var = exact_var;
+  a.batch = b ? b->d : nullptr;
+  a.ref_input = b ? b->ref_input : nullptr;
+  hipStream_t s = (hipStream_t)stream;
+  dim3 grid((unsigned)nrefs);
+#define G_CASE(b0, b1, b2, r0, r1, r2, nmax, nt)                                              \
+  if (same3(block_h, b0, b1, b2) && same3(radius_h, r0, r1, r2) && max_matches <= nmax) {     \
+    hipLaunchKernelGGL((k_gaussian_var<b0 * b1 * b2, nmax, nt>), grid, dim3(nt), 0, s, a);     \
+    HIP_TRY(hipGetLastError());                                                              \
+    return 0;                                                                                \
+  }
+  G_CASE(1, 8, 8, 0, 19, 19, 32, 64)
+  G_CASE(4, 4, 4, 7, 7, 7, 16, 64)
+  G_CASE(5, 5, 5, 7, 7, 7, 32, 128)
+#undef G_CASE
+  return fail(BMND_HIP_EINVAL, "block/radius/stack configuration not compiled into this library");
+}
+
+int32_t bmnd_hip_gaussian_variance(const float *cov, const int32_t *domain_h,
+                                   const int64_t *shape_h, const int32_t *block_h,
+                                   const int32_t *radius_h, const int32_t *sel,
+                                   const int32_t *counts, int64_t nrefs, int32_t max_matches,
+                                   const float *matrices, int32_t k, float *exact_var,
+                                   void *stream) {
+  return gaussian_variance_impl(cov, domain_h, shape_h, block_h, radius_h, sel, counts, nrefs,
+                                max_matches, matrices, k, exact_var, stream, nullptr);
 }
 
 int32_t bmnd_hip_blockmatch_standardized(const float *vol, const int64_t *shape_h,
@@ -3565,7 +3945,7 @@ static int32_t filter_impl(int32_t wiener, const float *noisy, const float *pilo
     return rc;
   }
   if (k < 0) return fail(BMND_HIP_EINVAL, "k must be non-negative");
-  if (wiener < 0 || (wiener & ~0xff))
+  if (wiener < 0 || (wiener & ~0x1ff))
     return fail(BMND_HIP_EINVAL, "unknown bits in the filter code");
   const int code = wiener;
   wiener = code & 3;
@@ -3610,6 +3990,10 @@ static int32_t filter_impl(int32_t wiener, const float *noisy, const float *pilo
   a.windowed = (code & 32) != 0;
   a.group_scope = (code & 64) != 0;
   a.coef_out = (code & 128) != 0;
+  a.gaussian = (code & 256) != 0;
+  a.gpv = a.gaussian ? var : nullptr;
+  if (a.gaussian && !var)
+    return fail(BMND_HIP_EINVAL, "the Gaussian model needs the coefficient variances");
   {
     const int P = block_h[0] * block_h[1] * block_h[2];
     const double *t64 = reinterpret_cast<const double *>(

@@ -179,9 +179,10 @@ def _ht_policy(profile: BMNDProfile) -> PoissonHTMatchPolicy:
 def check_supported(profile: BMNDProfile, resolved) -> None:
     """Raise NotImplementedError for profile options outside the MI355X path."""
     if not resolved.is_poisson:
-        raise NotImplementedError("bmnd_amd implements the Poisson noise model only "
-                                  "(noise_model='poisson').")
-    if resolved.blockmatch_distance not in _DISTANCE_CODES:
+        if resolved.blockmatch_distance is not BlockMatchDistance.SSD:
+            raise NotImplementedError("the Gaussian model runs with the 'ssd' block-matching "
+                                      "distance on the MI355X path.")
+    elif resolved.blockmatch_distance not in _DISTANCE_CODES:
         raise NotImplementedError("the 'poisson_deviance', 'pearson' and 'anscombe_ssd' "
                                   "block-matching distances run on the MI355X path.")
     def field(name, enum):
@@ -196,7 +197,7 @@ def check_supported(profile: BMNDProfile, resolved) -> None:
                 and field(f"{stage}_weight_domain", AggregationWeightDomain)
                 is AggregationWeightDomain.WINDOWED_SYNTHESIS
                 and field(f"{stage}_weight_scope", AggregationWeightScope)
-                is AggregationWeightScope.GROUP and int(profile.k) > 0):
+                is AggregationWeightScope.GROUP and int(resolved.variance_k) > 0):
             raise NotImplementedError(
                 f"the exact covariance-aware {stage} group risk (windowed-synthesis domain, "
                 "group scope, k > 0) is not part of the MI355X path.")
@@ -327,6 +328,32 @@ def exact_std_work(plan: StagePlan, shape3: tuple[int, int, int], sel: torch.Ten
         flops_pairs += int((2 * q * P * n * P).sum())
     return {"entries": entries, "union_voxels": union, "flops": flops,
             "flops_mac": flops_pairs}
+
+
+def aggregate_stage(plan: StagePlan, shape3, refs: torch.Tensor, R: int, filt: torch.Tensor,
+                    wts: torch.Tensor, sel: torch.Tensor, cnt: torch.Tensor, noisy: torch.Tensor,
+                    fallback: torch.Tensor, gsig: torch.Tensor | None, wiener: bool,
+                    sigma_fill: float | None, stream) -> tuple[torch.Tensor, torch.Tensor | None]:
+    """Ordered aggregation of a stage's filtered groups (patchaggregation.py:
+    170-199, bmndalgo.py:626-631, 1039-1044): the estimate, `fallback` where
+    no patch lands, and with `gsig` the sigma map (`sigma_fill` where no
+    patch lands)."""
+    L = _lib.lib()
+    shape = (ctypes.c_int64 * 3)(*shape3)
+    block = (ctypes.c_int32 * 3)(*plan.block3)
+    radius = (ctypes.c_int32 * 3)(*plan.radius3)
+    _, kai = plan.device_constants(noisy.device)
+    npatch = int(np.prod([shape3[d] - plan.block3[d] + 1 for d in range(3)]))
+    ref_index = torch.empty(npatch, dtype=torch.int32, device=noisy.device)
+    _lib.check(L.bmnd_hip_reference_map(_ptr(refs), R, _ptr(ref_index), npatch, stream))
+    est = torch.empty_like(noisy)
+    smap = torch.empty_like(noisy) if gsig is not None else None
+    _lib.check(L.bmnd_hip_aggregate_sigma(_ptr(filt), _ptr(wts), _ptr(sel), _ptr(cnt),
+                                          _ptr(ref_index), R, plan.max_m, shape, block, radius,
+                                          _ptr(kai), _ptr(fallback), _ptr(est), _ptr(gsig),
+                                          int(wiener), float(sigma_fill or 0.0), _ptr(smap),
+                                          stream))
+    return est, smap
 
 
 def _threshold_options(profile: BMNDProfile) -> tuple:

@@ -2,9 +2,8 @@
 
 Mirrors the reference's resolve_profile (src/bmnd/profilevalidation.py:33-456):
 the same checks, in the same order, with the same ValueError messages, and
-the same in-place coercion of string fields to enums.  The Gaussian-only
-`nf` resolution (psd.resolve_nf) is not part of the MI355X path; bmnd()
-rejects Gaussian profiles right after this function returns.
+the same in-place coercion of string fields to enums, and for the Gaussian
+model the nf domain (psd.resolve_nf) with k = 0 on the conventional path.
 """
 
 from __future__ import annotations
@@ -27,6 +26,7 @@ from .enums import (
     coerce_enum,
 )
 from .profiles import BMNDProfile
+from .psd import resolve_nf
 from .transforms import Transform, TransformMode, TransformType
 
 _OPERATOR_TRANSFORMS = {
@@ -243,5 +243,12 @@ def resolve_profile(profile: BMNDProfile, volume_shape: tuple[int, ...]) -> Reso
                 if not np.isfinite(value) or value < 0.0 or (positive and value == 0.0):
                     qualifier = "positive" if positive else "non-negative"
                     raise ValueError(f"{name} must be finite and {qualifier}, got {value}.")
-    return ResolvedProfile(is_poisson=is_poisson, effective_nf=None, conventional_nf=False,
-                           blockmatch_distance=distance, variance_k=variance_k)
+    effective_nf, conventional_nf = None, False
+    if not is_poisson:
+        # profilevalidation.py:445-448
+        effective_nf, conventional_nf = resolve_nf(profile.nf, volume_shape)
+        if conventional_nf:
+            variance_k = 0
+    return ResolvedProfile(is_poisson=is_poisson, effective_nf=effective_nf,
+                           conventional_nf=conventional_nf, blockmatch_distance=distance,
+                           variance_k=variance_k)

@@ -139,6 +139,49 @@ int32_t bmnd_hip_blockmatch_standardized(const float *vol, const int64_t *shape_
                                          int32_t *sel, int32_t *counts, int32_t *accepted,
                                          void *stream);
 
+/* SSD block matching of the stationary Gaussian model (blockmatching.py:
+ * 320-334, 1046-1205 with distance "ssd"): per candidate the float32 SSD
+ * summed in patch C order, less gamma * noise_ssd[lag] when noise_ssd
+ * (device, the (2R+1) search box in C order, zero lag 0) is given
+ * (blockmatching.py:244-286, 1133-1139); accepted below threshold, sorted
+ * by (distance, index), no tie grid; then the same group assembly as
+ * bmnd_hip_blockmatch.  The threshold is the profile's times the squared
+ * value range of vol (blockmatching.py:168-184; bmnd_hip_value_range).
+ * Refuses a negative or non-finite gamma or threshold and a nonzero gamma
+ * without noise_ssd.  Replaces blockmatching.blockmatch_groups
+ * (blockmatching.py:1393-1505) on the Gaussian path (bmndalgo.py:291-326,
+ * 722-743). */
+int32_t bmnd_hip_blockmatch_ssd(const float *vol, const int64_t *shape_h,
+                                const int32_t *block_h, const int32_t *radius_h,
+                                const int32_t *ref_lin, int64_t nrefs, float threshold,
+                                int32_t max_matches, int32_t min_matches, int32_t pow2,
+                                const float *noise_ssd, float gamma, int32_t *sel,
+                                int32_t *counts, int32_t *accepted, void *stream);
+
+/* range (device, 2 floats) <- (min, max) of the n values of vol: the data
+ * scale of compute_blockmatch_threshold (blockmatching.py:168-184). */
+int32_t bmnd_hip_value_range(const float *vol, int64_t n, float *range, void *stream);
+
+/* Exact-plane noise variance of the stationary Gaussian model for the first
+ * min(k, N) planes of every group (variance.py:1889-2033 as called by
+ * get_stationary_gaussian_group_noise_variance, variance.py:2036-2106): for
+ * plane q and spatial coefficient s, over the nonzero entries f of row q of
+ * the member matrix in increasing order, the float32 serial sum of
+ * (G[q,f] C_s(0)) G[q,f] and, for each nonzero t < f, of
+ * (G[q,f] G[q,t]) (C_s(l) + C_s(-l)) with l the members' origin difference
+ * modulo the domain, floored at 0.
+ *   cov        device, [D][P] float32: the cross-covariance maps
+ *              (variance.py:1791-1886) lag-major, D = domain[0]*[1]*[2]
+ *              lags in C order (2-D inputs as (1, d0, d1)); each map even in
+ *              the lag
+ *   exact_var  out: (nrefs, min(k, max_matches), P) */
+int32_t bmnd_hip_gaussian_variance(const float *cov, const int32_t *domain_h,
+                                   const int64_t *shape_h, const int32_t *block_h,
+                                   const int32_t *radius_h, const int32_t *sel,
+                                   const int32_t *counts, int64_t nrefs, int32_t max_matches,
+                                   const float *matrices, int32_t k, float *exact_var,
+                                   void *stream);
+
 /* Exact overlap-aware noise std of the first min(k, N) group planes of every
  * group: var[q,s] = sum over union voxels v (C order) of
  * var_v * (sum over entries (i,p) with x_i + p = v of G[q,i]*S[s,p])^2,
@@ -181,8 +224,15 @@ int32_t bmnd_hip_exact_std(const float *var, const int64_t *shape_h,
  * domain; bit 6 the group weight scope (one risk per group, numpy's float64
  * sum; aggregationweights.py:175-211); bit 7 writes the shrunk coefficients
  * to filt instead of the inverse-transformed patches (the input of
- * bmnd_hip_mass_correct).  The same values in every entry that takes a
- * filter code. */
+ * bmnd_hip_mass_correct); bit 8 the stationary Gaussian model: exact_std
+ * then holds the exact planes' variances from bmnd_hip_gaussian_variance,
+ * var the P coefficient variances (entry 0 of each cross-covariance map),
+ * the remaining planes share (n pv - sum of the exact planes) / (n - k)
+ * (variance.py:2096-2106), the hard threshold keeps |C| >= lambda sigma and
+ * the group's first coefficient (bmndalgo.py:470-474), and the Wiener gain
+ * is |r|^2 / (|r|^2 + shrink max(sigma^2, 1e-8)) (wiener.py:56-75), its
+ * signal power |r|^2 for the risk model.  The same values in every entry
+ * that takes a filter code. */
 int32_t bmnd_hip_filter_groups(int32_t wiener, const float *noisy,
                                const float *pilot, const float *var,
                                const int64_t *shape_h, const int32_t *block_h,

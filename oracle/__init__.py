@@ -86,6 +86,15 @@ class OracleParams(ctypes.Structure):
         ("wi_mdir", ctypes.c_void_p),
         ("ht_sdir", ctypes.c_void_p),
         ("wi_sdir", ctypes.c_void_p),
+        ("gaussian", ctypes.c_int),
+        ("ht_match_d", ctypes.c_double),
+        ("wi_match_d", ctypes.c_double),
+        ("ht_cov", ctypes.c_void_p),
+        ("wi_cov", ctypes.c_void_p),
+        ("cov_domain", ctypes.c_int * MAX_NDIM),
+        ("ht_noise_ssd", ctypes.c_void_p),
+        ("ht_gamma", ctypes.c_float),
+        ("global_sigma", ctypes.c_double),
     ]
 
 
@@ -125,6 +134,11 @@ def _declare(L: ctypes.CDLL) -> None:
     L.oracle_blockmatch_standardized.argtypes = [vp, vp, i32, vp, vp, vp, vp, i64, vp, vp,
                                                  i32, i32, i32, i32, i32, f64, vp, vp, i32, i32,
                                                  vp, vp, vp]
+    L.oracle_blockmatch_ssd.argtypes = [vp, i32, vp, vp, vp, vp, i64, vp, vp, i32, i32, i32,
+                                        vp, f32, i32, vp, vp, vp]
+    L.oracle_gaussian_group_variance.argtypes = [vp, i32, i64, vp, vp, i32, i32, vp]
+    L.oracle_ht_gaussian.argtypes = [vp, vp, i64, f32, vp]
+    L.oracle_wiener_gaussian.argtypes = [vp, vp, vp, i64, f32, vp, vp]
     L.oracle_apply_transform.argtypes = [vp, i32, vp, vp]
     L.oracle_exact_variance.argtypes = [vp, i32, vp, vp, i32, vp, vp, vp, i32, vp]
     L.oracle_exact_variance.restype = i32
@@ -820,3 +834,148 @@ def set_ht_standardized(params, max_count: int = 512) -> None:
     params.null_means = params._null_keep[0].ctypes.data
     params.null_vars = params._null_keep[1].ctypes.data
     params.null_max = int(max_count)
+
+
+# ---- the stationary Gaussian noise model ------------------------------------
+
+def blockmatch_ssd(vol, block, radius, refs, threshold, max_matches, min_matches, *,
+                   noise_ssd=None, gamma=0.0, pow2=True, nthreads=0):
+    """SSD block matching with the optional Gaussian noise-bias correction
+    (blockmatching.py:1046-1205): (sel (nrefs, max) padded with -1, counts,
+    accepted) for the patch-grid references refs (nrefs, ndim)."""
+    v = np.ascontiguousarray(vol, dtype=np.float32)
+    nd = v.ndim
+    shp = np.asarray(v.shape, dtype=np.int64)
+    blk = np.asarray(block, dtype=np.int32)
+    cts = np.asarray([v.shape[d] - block[d] + 1 for d in range(nd)], dtype=np.int64)
+    rad = np.asarray(radius, dtype=np.int32)
+    r = np.ascontiguousarray(refs, dtype=np.int64).reshape(-1, nd)
+    thr = np.asarray(threshold, dtype=np.float32)
+    thr = np.ascontiguousarray(np.broadcast_to(thr, (len(r),)), dtype=np.float32)
+    ns = None if noise_ssd is None else np.ascontiguousarray(noise_ssd, dtype=np.float32)
+    sel = np.empty((len(r), max_matches), np.int64)
+    cnt = np.empty(len(r), np.int32)
+    acc = np.empty(len(r), np.int64)
+    lib().oracle_blockmatch_ssd(_p(v), nd, _p(shp), _p(cts), _p(blk), _p(r), len(r), _p(rad),
+                                _p(thr), max_matches, min_matches, int(pow2), _p(ns),
+                                float(gamma), nthreads, _p(sel), _p(cnt), _p(acc))
+    return sel, cnt, acc
+
+
+def gaussian_group_variance(maps, positions, group_matrix, k: int) -> np.ndarray:
+    """The (n, P) noise variance of one group under the stationary model
+    (variance.py:1889-2106): maps (P,) + domain float32, positions (n, ndim)
+    patch origins, the n x n member analysis matrix."""
+    m = np.ascontiguousarray(maps, dtype=np.float32)
+    P = m.shape[0]
+    domain = m.shape[1:]
+    pos = np.asarray(positions, dtype=np.int64)
+    n = pos.shape[0]
+    lag = np.zeros((n, n), dtype=np.int64)
+    for a, d in enumerate(domain):
+        lag = lag * d + (pos[:, None, a] - pos[None, :, a]) % d
+    g = np.ascontiguousarray(group_matrix, dtype=np.float32)
+    out = np.empty((n, P), np.float32)
+    lib().oracle_gaussian_group_variance(_p(m), P, int(np.prod(domain)), _p(lag), _p(g), n,
+                                         int(k), _p(out))
+    return out
+
+
+def ht_gaussian(coef, sigma, lam=3.0):
+    c = np.ascontiguousarray(coef, dtype=np.float32).copy()
+    s = np.ascontiguousarray(sigma, dtype=np.float32)
+    att = np.empty_like(c)
+    lib().oracle_ht_gaussian(_p(c), _p(s), c.size, lam, _p(att))
+    return c, att
+
+
+def wiener_gaussian(coef, ref_coef, sigma, mu=0.4):
+    c = np.ascontiguousarray(coef, dtype=np.float32).copy()
+    r = np.ascontiguousarray(ref_coef, dtype=np.float32)
+    s = np.ascontiguousarray(sigma, dtype=np.float32)
+    att = np.empty_like(c)
+    sp = np.empty_like(c)
+    lib().oracle_wiener_gaussian(_p(c), _p(r), _p(s), c.size, mu, _p(att), _p(sp))
+    return c, att, sp
+
+
+def set_gaussian(params, *, global_sigma, domain, ht_maps, wi_maps, ht_match, wi_match,
+                 ht_noise_ssd=None, gamma=0.0, k=None) -> None:
+    """Switch the params to the stationary Gaussian model: the tables of
+    each stage (float32 cross-covariance maps (P,) + domain, the HT search-box
+    noise SSD or None), the match thresholds as given (they scale with the
+    matched volume's squared range) and k (None keeps the params' k)."""
+    params.gaussian = 1
+    params.global_sigma = float(global_sigma)
+    for d, v in enumerate(domain):
+        params.cov_domain[d] = int(v)
+    keep = [np.ascontiguousarray(ht_maps, np.float32), np.ascontiguousarray(wi_maps, np.float32)]
+    params.ht_cov, params.wi_cov = keep[0].ctypes.data, keep[1].ctypes.data
+    if ht_noise_ssd is not None:
+        keep.append(np.ascontiguousarray(ht_noise_ssd, np.float32))
+        params.ht_noise_ssd = keep[-1].ctypes.data
+    else:
+        params.ht_noise_ssd = None
+    params.ht_gamma = float(gamma)
+    params.ht_match_d, params.wi_match_d = float(ht_match), float(wi_match)
+    if k is not None:
+        params.k = int(k)
+    params._gauss_keep = keep
+
+
+def bmnd_gaussian_sigma(vol, params, *, capture_matches: bool = False):
+    """The oracle's two-stage Gaussian pipeline with the sigma maps:
+    (out, pilot, ht_map, wi_map[, (ht_sel, ht_cnt), (wi_sel, wi_cnt)])."""
+    v = np.ascontiguousarray(vol, dtype=np.float32)
+    if not params.gaussian:
+        raise ValueError("set_gaussian() first")
+    out, pilot = np.empty_like(v), np.empty_like(v)
+    hm, wm = np.empty_like(v), np.empty_like(v)
+    nd = v.ndim
+    cnt_ht = tuple(int(v.shape[d]) - params.ht_block[d] + 1 for d in range(nd))
+    cnt_wi = tuple(int(v.shape[d]) - params.wi_block[d] + 1 for d in range(nd))
+    n_ht = len(schedule(cnt_ht, tuple(params.ht_block[:nd]), tuple(params.ht_step[:nd]),
+                        params.shift_density, params.schedule_density))
+    n_wi = len(schedule(cnt_wi, tuple(params.wi_block[:nd]), tuple(params.wi_step[:nd]),
+                        params.shift_density, params.schedule_density))
+    ht = (np.empty((n_ht, params.ht_max), np.int64), np.empty(n_ht, np.int32))
+    wi = (np.empty((n_wi, params.wi_max), np.int64), np.empty(n_wi, np.int32))
+    g = ctypes.c_double(0.0)
+    rc = lib().oracle_bmnd_poisson_sigma(_p(v), ctypes.byref(params), _p(out), _p(pilot),
+                                         _p(ht[0]), _p(ht[1]), n_ht, _p(wi[0]), _p(wi[1]), n_wi,
+                                         _p(hm), _p(wm), ctypes.addressof(g))
+    if rc != 0:
+        raise RuntimeError(f"oracle Gaussian pipeline failed with code {rc}")
+    if capture_matches:
+        return out, pilot, hm, wm, ht, wi
+    return out, pilot, hm, wm
+
+
+def bmnd_gaussian(vol, params, *, capture_matches: bool = False):
+    """(out, pilot[, (ht_sel, ht_cnt), (wi_sel, wi_cnt)]) of the Gaussian
+    pipeline."""
+    v = np.ascontiguousarray(vol, dtype=np.float32)
+    if not params.gaussian:
+        raise ValueError("set_gaussian() first")
+    out, pilot = np.empty_like(v), np.empty_like(v)
+    args = [None, None, 0, None, None, 0]
+    ht = wi = None
+    if capture_matches:
+        nd = v.ndim
+        cnt_ht = tuple(int(v.shape[d]) - params.ht_block[d] + 1 for d in range(nd))
+        cnt_wi = tuple(int(v.shape[d]) - params.wi_block[d] + 1 for d in range(nd))
+        n_ht = len(schedule(cnt_ht, tuple(params.ht_block[:nd]), tuple(params.ht_step[:nd]),
+                            params.shift_density, params.schedule_density))
+        n_wi = len(schedule(cnt_wi, tuple(params.wi_block[:nd]), tuple(params.wi_step[:nd]),
+                            params.shift_density, params.schedule_density))
+        ht = (np.empty((n_ht, params.ht_max), np.int64), np.empty(n_ht, np.int32))
+        wi = (np.empty((n_wi, params.wi_max), np.int64), np.empty(n_wi, np.int32))
+        args = [ht[0], ht[1], n_ht, wi[0], wi[1], n_wi]
+    rc = lib().oracle_bmnd_poisson(_p(v), ctypes.byref(params), _p(out), _p(pilot),
+                                   _p(args[0]), _p(args[1]), args[2], _p(args[3]),
+                                   _p(args[4]), args[5])
+    if rc != 0:
+        raise RuntimeError(f"oracle Gaussian pipeline failed with code {rc}")
+    if capture_matches:
+        return out, pilot, ht, wi
+    return out, pilot

@@ -359,17 +359,21 @@ float oracle_canonical_key(float d) {
   return (float)(rint((double)d / q) * q);
 }
 
+/* The SSD branch accumulates in float32: its variable starts as
+ * np.float32(0.0) and takes float32 sums only, which numba's per-assignment
+ * (SSA) typing keeps float32; the other branches widen to float64
+ * (blockmatching.py:320-371). */
 float oracle_patch_distance(const float *ref, const float *cand, int p,
                             int code, double count_scale) {
-  double dist = 0.0;
   if (code == ORACLE_DIST_SSD) {
+    float d32 = 0.0f;
     for (int i = 0; i < p; ++i) {
       float diff = cand[i] - ref[i];
-      float sq = diff * diff;
-      dist += (double)sq;
+      d32 = d32 + diff * diff;
     }
-    return (float)dist;
+    return d32;
   }
+  double dist = 0.0;
   double inv = 1.0 / count_scale;
   for (int i = 0; i < p; ++i) {
     double r = (double)ref[i] * inv;
@@ -527,6 +531,15 @@ void oracle_blockmatch(const float *patches_flat, const float *volume,
                                  count_out, accepted_out);
 }
 
+static void blockmatch_core(const float *patches_flat, const float *volume, int ndim,
+                            const int64_t *vol_shape, const int64_t *counts, const int *block,
+                            const int64_t *refs, int64_t nrefs, const int *radius,
+                            const float *thresholds, int max_matches, int min_matches, int pow2,
+                            int stabilize, int code, double count_scale,
+                            const float *null_means, const float *null_vars, int null_max,
+                            const float *noise_ssd, float gamma, int nthreads, int64_t *sel_out,
+                            int32_t *count_out, int64_t *accepted_out);
+
 /* oracle_blockmatch; with null_means the candidate-standardized distance
  * (blockmatching.py:1128-1145) */
 void oracle_blockmatch_standardized(const float *patches_flat, const float *volume, int ndim,
@@ -538,6 +551,35 @@ void oracle_blockmatch_standardized(const float *patches_flat, const float *volu
                                     const float *null_vars, int null_max, int nthreads,
                                     int64_t *sel_out, int32_t *count_out,
                                     int64_t *accepted_out) {
+  blockmatch_core(patches_flat, volume, ndim, vol_shape, counts, block, refs, nrefs, radius,
+                  thresholds, max_matches, min_matches, pow2, stabilize, code, count_scale,
+                  null_means, null_vars, null_max, NULL, 0.0f, nthreads, sel_out, count_out,
+                  accepted_out);
+}
+
+/* SSD matching with the Gaussian noise-bias correction (blockmatching.py:
+ * 1133-1139): every candidate's float32 SSD less gamma times the expected
+ * noise SSD at its lag, noise_ssd being C-ordered over the (2 r + 1) search
+ * box (NULL: no correction).  No tie grid (blockmatching.py:1108). */
+void oracle_blockmatch_ssd(const float *volume, int ndim, const int64_t *vol_shape,
+                           const int64_t *counts, const int *block, const int64_t *refs,
+                           int64_t nrefs, const int *radius, const float *thresholds,
+                           int max_matches, int min_matches, int pow2, const float *noise_ssd,
+                           float gamma, int nthreads, int64_t *sel_out, int32_t *count_out,
+                           int64_t *accepted_out) {
+  blockmatch_core(NULL, volume, ndim, vol_shape, counts, block, refs, nrefs, radius, thresholds,
+                  max_matches, min_matches, pow2, 0, ORACLE_DIST_SSD, 1.0, NULL, NULL, 0,
+                  noise_ssd, gamma, nthreads, sel_out, count_out, accepted_out);
+}
+
+static void blockmatch_core(const float *patches_flat, const float *volume, int ndim,
+                            const int64_t *vol_shape, const int64_t *counts, const int *block,
+                            const int64_t *refs, int64_t nrefs, const int *radius,
+                            const float *thresholds, int max_matches, int min_matches, int pow2,
+                            int stabilize, int code, double count_scale,
+                            const float *null_means, const float *null_vars, int null_max,
+                            const float *noise_ssd, float gamma, int nthreads, int64_t *sel_out,
+                            int32_t *count_out, int64_t *accepted_out) {
   int P = (int)prod_int(block, ndim);
   int64_t cstr[ORACLE_MAX_NDIM], vstr[ORACLE_MAX_NDIM];
   c_strides(ndim, counts, cstr);
@@ -614,6 +656,12 @@ void oracle_blockmatch_standardized(const float *patches_flat, const float *volu
         float dist = null_means ? oracle_standardized_distance(refp, cp, P, code, count_scale,
                                                                null_means, null_vars, null_max)
                                 : oracle_patch_distance(refp, cp, P, code, count_scale);
+        if (noise_ssd) {
+          int64_t ci = 0;
+          for (int d = 0; d < ndim; ++d)
+            ci = ci * (2 * radius[d] + 1) + (coord[d] - center[d] + radius[d]);
+          dist = dist - gamma * noise_ssd[ci];
+        }
         int ok = stab ? (oracle_canonical_key(dist) < thr_key) : (dist < thr);
         if (ok) {
           ++accepted;
@@ -984,6 +1032,85 @@ int oracle_noise_std(const float *var_volume, int ndim,
   free(exact);
   free(approx);
   return 0;
+}
+
+/* Stationary Gaussian group noise variance (variance.py:1889-2106).
+ * maps: P cross-covariance maps over the nf domain of D cells each (entry 0
+ * the coefficient's variance); pair_lag: n x n flat domain lags of the member
+ * pairs; gmat: the n x n member analysis matrix.  The first q = min(k, n)
+ * planes: for each coefficient the serial float32 sum over the nonzero row
+ * entries f (increasing) of w_f c[lag(f,f)] w_f, then of
+ * w_f w_s (c[lag(f,s)] + c[lag(s,f)]) for the nonzero s < f, floored at 0;
+ * the remaining planes share (n pv - sum of the exact planes) / (n - q),
+ * floored at 0; k = 0 gives pv everywhere.  var_out is n x P. */
+void oracle_gaussian_group_variance(const float *maps, int P, int64_t D, const int64_t *pair_lag,
+                                    const float *gmat, int n, int k, float *var_out) {
+  const int q = k < n ? k : n;
+  for (int s = 0; s < P; ++s) {
+    const float *cm = maps + (int64_t)s * D;
+    const float pv = cm[0];
+    if (q == 0) {
+      for (int i = 0; i < n; ++i) var_out[(int64_t)i * P + s] = pv;
+      continue;
+    }
+    float tot = 0.0f;
+    for (int r = 0; r < q; ++r) {
+      const float *row = gmat + (int64_t)r * n;
+      float v = 0.0f;
+      for (int f = 0; f < n; ++f) {
+        const float wf = row[f];
+        if (wf == 0.0f) continue;
+        v = v + (wf * cm[pair_lag[(int64_t)f * n + f]]) * wf;
+        for (int t = 0; t < f; ++t) {
+          const float wt = row[t];
+          if (wt == 0.0f) continue;
+          const float pair = cm[pair_lag[(int64_t)f * n + t]] + cm[pair_lag[(int64_t)t * n + f]];
+          v = v + (wf * wt) * pair;
+        }
+      }
+      if (v < 0.0f) v = 0.0f;
+      var_out[(int64_t)r * P + s] = v;
+      tot = tot + v;  /* np.sum over the plane axis: sequential in the plane */
+    }
+    if (q < n) {
+      float rem = ((float)n * pv - tot) / (float)(n - q);
+      if (rem < 0.0f) rem = 0.0f;
+      for (int i = q; i < n; ++i) var_out[(int64_t)i * P + s] = rem;
+    }
+  }
+}
+
+/* Gaussian hard threshold (bmndalgo.py:464-474): keep |C| >= lambda sigma,
+ * the group's first coefficient always; atten gets the mask. */
+void oracle_ht_gaussian(float *coef, const float *sigma, int64_t n, float lambda_,
+                        float *atten) {
+  for (int64_t i = 0; i < n; ++i) {
+    const float thr = lambda_ * sigma[i];
+    const float keep = (fabsf(coef[i]) >= thr || i == 0) ? 1.0f : 0.0f;
+    coef[i] = coef[i] * keep;
+    atten[i] = keep;
+  }
+}
+
+/* Gaussian Wiener gain (wiener.py:56-75): |R|^2 / (|R|^2 + mu max(sigma^2,
+ * 1e-8)), 0 where the denominator is not positive, clipped to [0, 1];
+ * coef takes it; spow gets |R|^2, the risk model's signal power
+ * (bmndalgo.py:889-899). */
+void oracle_wiener_gaussian(float *coef, const float *ref_coef, const float *sigma, int64_t n,
+                            float mu, float *atten, float *spow) {
+  const float floor_ = (float)1e-8;
+  for (int64_t i = 0; i < n; ++i) {
+    const float r = fabsf(ref_coef[i]);
+    const float pr = r * r;
+    float nv = sigma[i] * sigma[i];
+    if (nv < floor_) nv = floor_;
+    const float den = pr + mu * nv;
+    float gain = den > 0.0f ? pr / den : 0.0f;
+    gain = gain < 0.0f ? 0.0f : (gain > 1.0f ? 1.0f : gain);
+    coef[i] = coef[i] * gain;
+    atten[i] = gain;
+    if (spow) spow[i] = pr;
+  }
 }
 
 /* per-patch inverse risk weights, VARIANCE/COEFFICIENT/PATCH
@@ -1520,16 +1647,35 @@ static int run_stage(const oracle_params *prm, int wiener, const float *match_vo
   oracle_schedule_generated(ndim, g.counts, block, step, prm->shift_density,
                             prm->schedule_density, refs, nrefs);
   float *thrs = (float *)malloc(sizeof(float) * (size_t)(nrefs + 1));
+  if (prm->gaussian) {
+    /* SSD thresholds scale with the matched volume's squared value range
+     * (blockmatching.py:168-184): float32 max - min, 1 when not positive,
+     * the product in float64, stored as float32 */
+    float mx = match_vol[0], mn = match_vol[0];
+    for (int64_t v = 1; v < nvox; ++v) {
+      if (match_vol[v] > mx) mx = match_vol[v];
+      if (match_vol[v] < mn) mn = match_vol[v];
+    }
+    const float range = mx - mn;
+    const double ds = range > 0.0f ? (double)range : 1.0;
+    thr = (float)(((wiener ? prm->wi_match_d : prm->ht_match_d) * ds) * ds);
+  }
   for (int64_t i = 0; i < nrefs; ++i)
     thrs[i] = (!wiener && prm->ht_thresholds) ? prm->ht_thresholds[i] : thr;
   int64_t *sel = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nrefs * max_m + 1));
   int32_t *cnt = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nrefs + 1));
   const int code = prm->distance ? prm->distance : ORACLE_DIST_DEVIANCE;
   const int std_ht = !wiener && prm->ht_standardize;
-  oracle_blockmatch_standardized(NULL, match_vol, ndim, g.shape, g.counts, block, refs, nrefs,
-                                 radius, thrs, max_m, min_m, 1, 1, code, prm->count_scale,
-                                 std_ht ? prm->null_means : NULL, std_ht ? prm->null_vars : NULL,
-                                 prm->null_max, prm->nthreads, sel, cnt, NULL);
+  if (prm->gaussian)
+    oracle_blockmatch_ssd(match_vol, ndim, g.shape, g.counts, block, refs, nrefs, radius, thrs,
+                          max_m, min_m, 1, wiener ? NULL : prm->ht_noise_ssd, prm->ht_gamma,
+                          prm->nthreads, sel, cnt, NULL);
+  else
+    oracle_blockmatch_standardized(NULL, match_vol, ndim, g.shape, g.counts, block, refs, nrefs,
+                                   radius, thrs, max_m, min_m, 1, 1, code, prm->count_scale,
+                                   std_ht ? prm->null_means : NULL,
+                                   std_ht ? prm->null_vars : NULL, prm->null_max, prm->nthreads,
+                                   sel, cnt, NULL);
   if (sel_dump && cnt_dump) {
     int64_t lim = nrefs < dump_cap ? nrefs : dump_cap;
     memcpy(sel_dump, sel, sizeof(int64_t) * (size_t)(lim * max_m));
@@ -1563,6 +1709,9 @@ static int run_stage(const oracle_params *prm, int wiener, const float *match_vo
     float *att = (float *)malloc(sizeof(float) * (size_t)nmax * P);
     float *spw = (float *)malloc(sizeof(float) * (size_t)nmax * P);
     int64_t *apos = (int64_t *)malloc(sizeof(int64_t) * (size_t)nmax * ndim);
+    int64_t *lags = (int64_t *)malloc(sizeof(int64_t) * (size_t)nmax * nmax);
+    int64_t cov_cells = 1;
+    for (int d = 0; d < ndim; ++d) cov_cells *= prm->gaussian ? prm->cov_domain[d] : 1;
     int last_n = -1;
 #ifdef _OPENMP
 #pragma omp for schedule(dynamic, 8)
@@ -1594,15 +1743,40 @@ static int run_stage(const oracle_params *prm, int wiener, const float *match_vo
           for (int p = 0; p < P; ++p) Y[i * P + p] = pilot[base + g.poff[p]];
       }
       oracle_apply_transform(X, ndim + 1, dims, fm);
-      oracle_noise_std(var_vol, ndim, g.shape, block, n, apos, gfwd, spf, prm->k, sig);
+      if (prm->gaussian) {
+        /* pair lags of the members on the nf domain (variance.py:1889-1906),
+         * the group variance, its square root (bmndalgo.py:443-451) */
+        for (int i = 0; i < n; ++i)
+          for (int j = 0; j < n; ++j) {
+            int64_t l = 0;
+            for (int d = 0; d < ndim; ++d) {
+              const int64_t D = prm->cov_domain[d];
+              int64_t x = (apos[i * ndim + d] - apos[j * ndim + d]) % D;
+              if (x < 0) x += D;
+              l = l * D + x;
+            }
+            lags[i * n + j] = l;
+          }
+        oracle_gaussian_group_variance(wiener ? prm->wi_cov : prm->ht_cov, P, cov_cells, lags,
+                                       gfwd, n, prm->k, sig);
+        for (int e = 0; e < n * P; ++e) sig[e] = sqrtf(sig[e] > 0.0f ? sig[e] : 0.0f);
+      } else {
+        oracle_noise_std(var_vol, ndim, g.shape, block, n, apos, gfwd, spf, prm->k, sig);
+      }
       if (!wiener) {
         if (prm->ht_soft)
           oracle_ht_soft(X, sig, (int64_t)n * P, prm->ht_lambda, att);
+        else if (prm->gaussian)
+          oracle_ht_gaussian(X, sig, (int64_t)n * P, prm->ht_lambda, att);
         else
           oracle_ht_shrink(X, sig, (int64_t)n * P, prm->ht_lambda, prm->count_scale, att);
       } else {
         oracle_apply_transform(Y, ndim + 1, dims, fm);
-        wiener_shrink_power(X, Y, sig, (int64_t)n * P, prm->wi_mu, prm->wi_gain_mode, att, spw);
+        if (prm->gaussian)
+          oracle_wiener_gaussian(X, Y, sig, (int64_t)n * P, prm->wi_mu, att, spw);
+        else
+          wiener_shrink_power(X, Y, sig, (int64_t)n * P, prm->wi_mu, prm->wi_gain_mode, att,
+                              spw);
       }
       if (!mass) oracle_apply_transform(X, ndim + 1, dims, im);
       float *fo = filt + off[r] * P;
@@ -1627,6 +1801,7 @@ static int run_stage(const oracle_params *prm, int wiener, const float *match_vo
     free(att);
     free(spw);
     free(apos);
+    free(lags);
   }
 
   /* sequential aggregation in schedule order (patchaggregation.py:170-199) */
@@ -1795,6 +1970,20 @@ int oracle_bmnd_poisson_sigma(const float *vol, const oracle_params *prm, float 
     if (ht_sigma_map) memset(ht_sigma_map, 0, sizeof(float) * (size_t)nvox);
     if (wi_sigma_map) memset(wi_sigma_map, 0, sizeof(float) * (size_t)nvox);
     if (global_sigma) *global_sigma = 0.0;
+    return 0;
+  }
+  if (prm->gaussian) {
+    /* the stationary model: no variance volume; both stages match with SSD,
+     * stage 2 on the pilot (bmndalgo.py:203-228, 714-743) */
+    float *est1 = (float *)malloc(sizeof(float) * (size_t)nvox);
+    const float fill = (float)prm->global_sigma;
+    if (global_sigma) *global_sigma = prm->global_sigma;
+    run_stage(prm, 0, vol, NULL, vol, NULL, vol, est1, ht_sel, ht_cnt, ht_cap, ht_sigma_map,
+              fill);
+    if (pilot) memcpy(pilot, est1, sizeof(float) * (size_t)nvox);
+    run_stage(prm, 1, est1, est1, vol, NULL, vol, out, wi_sel, wi_cnt, wi_cap, wi_sigma_map,
+              fill);
+    free(est1);
     return 0;
   }
   float *var1 = (float *)malloc(sizeof(float) * (size_t)nvox);

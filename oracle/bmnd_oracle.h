@@ -78,7 +78,20 @@ void oracle_blockmatch_standardized(const float *patches_flat, const float *volu
                                     int64_t *sel_out, int32_t *count_out,
                                     int64_t *accepted_out);
 
+void oracle_blockmatch_ssd(const float *volume, int ndim, const int64_t *vol_shape,
+                           const int64_t *counts, const int *block, const int64_t *refs,
+                           int64_t nrefs, const int *radius, const float *thresholds,
+                           int max_matches, int min_matches, int pow2, const float *noise_ssd,
+                           float gamma, int nthreads, int64_t *sel_out, int32_t *count_out,
+                           int64_t *accepted_out);
+
 /* ---- group statistics --------------------------------------------------- */
+void oracle_gaussian_group_variance(const float *maps, int P, int64_t D, const int64_t *pair_lag,
+                                    const float *gmat, int n, int k, float *var_out);
+void oracle_ht_gaussian(float *coef, const float *sigma, int64_t n, float lambda_,
+                        float *atten);
+void oracle_wiener_gaussian(float *coef, const float *ref_coef, const float *sigma, int64_t n,
+                            float mu, float *atten, float *spow);
 void oracle_apply_transform(float *data, int ndim_total, const int *dims,
                             const float *const *mats);
 int oracle_exact_variance(const float *var_volume, int ndim,
@@ -197,6 +210,21 @@ typedef struct {
   int mass_mode;
   const double *ht_mdir, *wi_mdir;
   const double *ht_sdir, *wi_sdir;
+  /* the stationary Gaussian noise model (noise_model "gaussian",
+   * bmndalgo.py:203-228, 291-316, 404-474, 826-870, variance.py:452-553,
+   * 1791-2106): SSD matching with thresholds ht_match_d / wi_match_d times
+   * the squared value range of the matched volume; the HT search-box noise
+   * SSD scaled by ht_gamma (NULL: no correction); each stage's
+   * cross-covariance maps (P x D over the nf domain cov_domain) for the
+   * group noise variance with k exact planes; global_sigma fills the sigma
+   * maps */
+  int gaussian;
+  double ht_match_d, wi_match_d;
+  const float *ht_cov, *wi_cov;
+  int cov_domain[ORACLE_MAX_NDIM];
+  const float *ht_noise_ssd;
+  float ht_gamma;
+  double global_sigma;
 } oracle_params;
 
 /* Runs both stages.  pilot may be NULL.  The match lists of each stage can

@@ -100,9 +100,23 @@ def test_unsupported_options_raise_before_any_gpu_work(fields):
         bmnd(np.ones((32, 32)), p)
 
 
-def test_gaussian_profiles_are_not_run():
-    with pytest.raises(NotImplementedError, match="Poisson"):
-        bmnd(np.ones((32, 32)), BM3DProfile())
+def test_gaussian_profile_support():
+    # the Gaussian model runs with SSD matching; a Poisson distance, or the
+    # exact covariance-aware group risk, is refused before any device work
+    from bmnd_amd.engine import check_supported
+    from bmnd_amd.profilevalidation import resolve_profile
+    prof = BM3DProfile()
+    check_supported(prof, resolve_profile(prof, (32, 32)))
+    prof.blockmatch_distance = "poisson_deviance"
+    with pytest.raises(NotImplementedError, match="'ssd'"):
+        check_supported(prof, resolve_profile(prof, (32, 32)))
+    prof = BM3DProfile()
+    prof.ht_weight_domain, prof.ht_weight_scope = "windowed_synthesis", "group"
+    with pytest.raises(NotImplementedError, match="exact covariance-aware"):
+        check_supported(prof, resolve_profile(prof, (32, 32)))
+    # the conventional nf path has k = 0, where that risk is the diagonal one
+    prof.nf = 0
+    check_supported(prof, resolve_profile(prof, (32, 32)))
 
 
 def test_exact_std_work_counts_union_voxels():

@@ -56,7 +56,7 @@ def test_thresholds_entry_needs_the_array():
 
 
 @pytest.mark.parametrize("code,msg", [
-    (-1, "unknown bits"), (256, "unknown bits"), (3 << 3, "weight model must be"),
+    (-1, "unknown bits"), (512, "unknown bits"), (3 << 3, "weight model must be"),
     (2 << 3, "risk weight model is a Wiener-stage option"),
     (1 | 4, "soft thresholding is a hard-threshold-stage option")])
 def test_filter_code_argument(code, msg):
@@ -64,6 +64,36 @@ def test_filter_code_argument(code, msg):
         code, DUMMY, DUMMY, DUMMY, SHAPE, BLOCK, RADIUS, DUMMY, DUMMY, 10, 16, DUMMY, DUMMY,
         3.0, 0, 1.0, DUMMY, DUMMY, None)
     assert msg in _error(rc)
+
+
+def test_gaussian_filter_needs_the_coefficient_variances():
+    rc = _lib.lib().bmnd_hip_filter_groups(
+        256, DUMMY, DUMMY, None, SHAPE, BLOCK, RADIUS, DUMMY, DUMMY, 10, 16, DUMMY, DUMMY,
+        3.0, 0, 1.0, DUMMY, DUMMY, None)
+    assert "coefficient variances" in _error(rc)
+
+
+@pytest.mark.parametrize("gamma,noise,thr,msg", [
+    (-1.0, DUMMY, 1.0, "gamma must be finite"), (float("nan"), DUMMY, 1.0, "gamma must be finite"),
+    (3.0, None, 1.0, "needs the noise SSD"), (0.0, None, -1.0, "finite and non-negative"),
+    (0.0, None, float("inf"), "finite and non-negative")])
+def test_ssd_matching_arguments(gamma, noise, thr, msg):
+    rc = _lib.lib().bmnd_hip_blockmatch_ssd(DUMMY, SHAPE, BLOCK, RADIUS, DUMMY, 10, thr, 16, 2, 1,
+                                            noise, gamma, DUMMY, DUMMY, DUMMY, None)
+    assert msg in _error(rc)
+
+
+@pytest.mark.parametrize("domain,msg", [((0, 4, 4), "positive"), ((4096, 4096, 4096), "too large")])
+def test_gaussian_variance_arguments(domain, msg):
+    rc = _lib.lib().bmnd_hip_gaussian_variance(DUMMY, I32X3(*domain), SHAPE, BLOCK, RADIUS, DUMMY,
+                                               DUMMY, 10, 16, DUMMY, 4, DUMMY, None)
+    assert msg in _error(rc)
+
+
+def test_value_range_arguments():
+    L = _lib.lib()
+    assert "must not be null" in _error(L.bmnd_hip_value_range(DUMMY, 10, None, None))
+    assert "empty volume" in _error(L.bmnd_hip_value_range(DUMMY, 0, DUMMY, None))
 
 
 @pytest.mark.parametrize("table_max,scale", [(101, 1.0), (-1, 1.0), (50, 2.0)])

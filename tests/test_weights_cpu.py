@@ -210,10 +210,11 @@ def test_filter_codes_and_option_checks():
         is_poisson = True
         from bmnd_amd.enums import BlockMatchDistance
         blockmatch_distance = BlockMatchDistance.POISSON_DEVIANCE
+        variance_k = 4
     check_supported(prof, R())
     # the exact covariance-aware group risk stays outside the path
     prof.wiener_weight_scope = AggregationWeightScope.GROUP
     with pytest.raises(NotImplementedError, match="exact covariance-aware"):
         check_supported(prof, R())
-    prof.k = 0
+    R.variance_k = 0
     check_supported(prof, R())
