@@ -17,6 +17,11 @@ Workloads (--config):
                with sides uniform in [48, 128], peaks uniform in [1, 50]:
                one batch per dimensionality (BM3D / BM4D defaults), every
                kernel launched once per batch over all its inputs
+  gauss-bm4d-256     the stationary Gaussian model: the 256^3 ellipsoid
+               phantom scaled to [0, 1] plus white noise of std 0.1,
+               BM4DProfile() defaults (nf 16^3, k 4, gamma 3), sigma given
+  gauss-bm3d-512x16  16 blob images of 512^2 scaled to [0, 1] plus white
+               noise of std 0.1, BM3DProfile() defaults (nf 32^2), sigma given
 With --batch, the inputs of a multi-input workload (bm3d-512x16) run as one
 batch instead of one bmnd() call after another.
 """
@@ -54,7 +59,8 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="bm4d-256",
-                    choices=["bm4d-256", "bm3d-512x16", "bm3d-64", "batch-uneven"])
+                    choices=["bm4d-256", "bm3d-512x16", "bm3d-64", "batch-uneven",
+                             "gauss-bm4d-256", "gauss-bm3d-512x16"])
     ap.add_argument("--batch", action="store_true",
                     help="run the inputs of the workload as one batch (bmnd_batch)")
     ap.add_argument("--cpu-sample", type=int, default=0,
@@ -208,6 +214,8 @@ def main() -> int:
 
     if args.batch or args.config == "batch-uneven":
         return run_batch(args)
+    if args.config.startswith("gauss-"):
+        return run_gaussian(args)
     profile, noisy_list, truth_list, cfg = make_inputs(args.config)
     dev = torch.device("cuda", 0)
     vols = [torch.from_numpy(np.ascontiguousarray(v)).to(dev) for v in noisy_list]
@@ -548,6 +556,193 @@ def run_batch(args) -> int:
         "kernels_ms": {k: round(v, 3) for k, v in sorted(per_kernel.items())},
         "psnr": psnr,
     }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+
+
+# ---- the stationary Gaussian model --------------------------------------------
+GAUSS_SIGMA = 0.1
+
+
+def gaussian_inputs(config: str):
+    from bmnd_amd import BM3DProfile, BM4DProfile, phantoms
+    rng = np.random.default_rng(7)
+    if config == "gauss-bm4d-256":
+        truths = [phantoms.ellipsoid_volume((256, 256, 256), seed=0, peak=10.0)[0]]
+        prof, desc = BM4DProfile(), "256^3 BM4D Gaussian, ellipsoid phantom in [0, 1], sigma 0.1"
+    else:
+        truths = [phantoms.blob_image(512, seed=s)[0] for s in range(16)]
+        prof, desc = BM3DProfile(), "16 x 512^2 BM3D Gaussian, blob phantoms in [0, 1], sigma 0.1"
+    truths = [(t / float(t.max())).astype(np.float32) for t in truths]
+    noisy = [(t + GAUSS_SIGMA * rng.standard_normal(t.shape)).astype(np.float32) for t in truths]
+    return prof, noisy, truths, {"workload": desc, "shape": list(noisy[0].shape) if len(noisy) == 1
+                                 else [len(noisy)] + list(noisy[0].shape),
+                                 "profile": type(prof).__name__ + "()", "sigma": GAUSS_SIGMA}
+
+
+def gaussian_oracle_params(profile, shape, model, threads):
+    """The oracle's parameters for the Gaussian profile with the product's
+    noise description (its host tables)."""
+    import oracle
+    from bmnd_amd.engine import make_plan
+    from bmnd_amd.gaussianvariance import blockmatch_noise_ssd, cross_covariance_maps
+    from bmnd_amd.profilevalidation import resolve_profile
+    from bmnd_amd.transforms import spatial_matrices
+    res = resolve_profile(profile, shape)
+    p = oracle.default_params(len(shape), shape, threads)
+    maps, plans = {}, {}
+    for st in ("ht", "wiener"):
+        plans[st] = make_plan(profile, st, len(shape))
+        mats = [spatial_matrices(plans[st].spatial, b)[0] for b in plans[st].block]
+        maps[st] = cross_covariance_maps(model, plans[st].block, mats)
+    gamma = 0.0 if res.conventional_nf else float(profile.gamma)
+    noise = (blockmatch_noise_ssd(model, plans["ht"].block, plans["ht"].radius3[3 - len(shape):])
+             if gamma else None)
+    oracle.set_gaussian(p, global_sigma=model.global_sigma, domain=model.variance_domain_shape,
+                        ht_maps=maps["ht"], wi_maps=maps["wiener"],
+                        ht_match=float(profile.ht_match_threshold),
+                        wi_match=float(profile.wiener_match_threshold), ht_noise_ssd=noise,
+                        gamma=gamma, k=res.variance_k)
+    return p
+
+
+def gaussian_variance_work(plan, counts, k: int) -> int:
+    """Algorithmic float32 operations of a stage's exact-plane variance
+    launch: per group of n members and plane q < min(k, n) with m nonzero
+    row entries, P * (3 m + 4 m (m - 1) / 2) (a diagonal term is two
+    multiplies and an add, a pair term an add, two multiplies and an add)."""
+    from bmnd_amd.transforms import group_matrices
+    hist = np.bincount(counts.cpu().numpy().astype(np.int64), minlength=plan.max_m + 1)
+    total = 0
+    for n in range(1, plan.max_m + 1):
+        if not hist[n]:
+            continue
+        G = group_matrices(n)[0]
+        per = 0
+        for q in range(min(k, n)):
+            m = int(np.count_nonzero(G[q]))
+            per += 3 * m + 2 * m * (m - 1)
+        total += int(hist[n]) * per * plan.P
+    return total
+
+
+def run_gaussian(args) -> int:
+    import torch
+
+    import oracle
+    from bmnd_amd.engine import KernelTimer, blockmatch_work
+    from bmnd_amd.gaussian import GaussianPipeline
+    from bmnd_amd.gaussianvariance import build_variance_model
+    from bmnd_amd.phantoms import psnr as _psnr
+    from bmnd_amd.profilevalidation import resolve_profile
+    profile, noisy_list, truth_list, cfg = gaussian_inputs(args.config)
+    dev = torch.device("cuda", 0)
+    vols = [torch.from_numpy(v).to(dev) for v in noisy_list]
+    voxels = sum(int(v.numel()) for v in vols)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pipe = GaussianPipeline(profile, noisy_list[0].shape, dev)
+    pipe.run(vols[0], sigma=GAUSS_SIGMA)
+    torch.cuda.synchronize()
+    cold_ms = 1e3 * (time.perf_counter() - t0)
+    last = {}
+
+    def step(timer=None):
+        outs = []
+        for v in vols:
+            ht, wi, _ = pipe.run(v, sigma=GAUSS_SIGMA, timer=timer)
+            last["ht"], last["wiener"] = ht, wi
+            outs.append(wi.est)
+        return outs
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    timer = KernelTimer()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        outs = step(timer)
+    torch.cuda.synchronize()
+    ms_per_step = 1e3 * (time.perf_counter() - t0) / args.steps
+    per_kernel = {k: float(np.mean(v)) for k, v in timer.totals_ms().items()}
+    rooflines = {}
+    for st in ("ht", "wiener"):
+        plan = pipe.plans[st]
+        pairs, terms = blockmatch_work(plan, noisy_list[0].shape)
+        ms = per_kernel[f"{st}.blockmatch"]
+        flops = 3 * terms + (2 * pairs if (st == "ht" and pipe.gamma) else 0)
+        ach = flops / (ms * 1e-3) / 1e12
+        rooflines[f"{st}.blockmatch"] = {
+            "kernel": "k_blockmatch<{}, {}, {}, {}, {}, {}, 10, 512>(BMArgs)".format(
+                *plan.block3, *plan.radius3),
+            "bound": "valu-fp32", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": None,
+            "algorithmic": f"float32 SSD: 3 operations per term, {terms} terms ({pairs} "
+                           "candidates) per launch" + (", 2 per candidate for the noise-bias "
+                                                       "correction" if st == "ht" and pipe.gamma
+                                                       else ""),
+            "avg_ms": round(ms, 3)}
+        if f"{st}.gaussian_variance" in per_kernel:
+            ms = per_kernel[f"{st}.gaussian_variance"]
+            w = gaussian_variance_work(plan, last[st].counts, pipe.k)
+            ach = w / (ms * 1e-3) / 1e12
+            rooflines[f"{st}.gaussian_variance"] = {
+                "kernel": "k_gaussian_var<{}, {}, {}>(GaussVarArgs)".format(
+                    plan.P, EXACT_CONFIG[tuple(plan.block3)], 128 if plan.P > 64 else 64),
+                "bound": "valu-fp32", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                "algorithmic": f"{w} float32 operations per launch (pair-lag terms of the "
+                               f"first {pipe.k} planes)", "avg_ms": round(ms, 3)}
+    dominant = max((k for k in per_kernel if k != "gaussian.tables"), key=lambda k: per_kernel[k])
+    roof = rooflines.get(dominant) or {"kernel": dominant, "bound": None, "achieved": None,
+                                      "peak": None, "unit": None, "frac": None, "traffic": None,
+                                      "avg_ms": round(per_kernel[dominant], 3)}
+    out0 = outs[0].cpu().numpy()
+    psnr = {"noisy_db": round(_psnr(noisy_list[0], truth_list[0], 1.0), 3),
+            "out_db": round(_psnr(out0, truth_list[0], 1.0), 3)}
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = host_threads()
+        side = min(args.cpu_sample or (64 if noisy_list[0].ndim == 3 else 256), *noisy_list[0].shape)
+        sample = crop(noisy_list[0], side)
+        res = resolve_profile(profile, sample.shape)
+        model = build_variance_model(sample, sigma=GAUSS_SIGMA, nf=res.effective_nf)
+        params = gaussian_oracle_params(profile, sample.shape, model, threads)
+        oracle.build()
+        t0 = time.perf_counter()
+        o_out, _ = oracle.bmnd_gaussian(sample, params)
+        dt = time.perf_counter() - t0
+        spipe = GaussianPipeline(profile, sample.shape, dev)
+        sv = torch.from_numpy(sample).to(dev)
+        spipe.run(sv, sigma=GAUSS_SIGMA)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(3):
+            g_out = spipe.run(sv, sigma=GAUSS_SIGMA)[1].est
+        torch.cuda.synchronize()
+        g_dt = (time.perf_counter() - t1) / 3
+        g_np = g_out.cpu().numpy()
+        cpu = {"value": float(sample.size / dt), "unit": "voxels/s", "cores": threads,
+               "kind": "port", "seconds": round(dt, 3),
+               "sample": f"{side}^{sample.ndim} centre crop of the first input, oracle "
+                         "Gaussian pipeline (both stages)",
+               "gpu_on_sample": {"value": float(sample.size / g_dt), "ms": round(1e3 * g_dt, 3)},
+               "gpu_over_cpu_on_sample": round(dt / g_dt, 1),
+               "gpu_vs_oracle_bits_differing": int(np.sum(g_np.view(np.int32)
+                                                          != o_out.view(np.int32)))}
+    line = {
+        "metric": "voxels/s of the Gaussian bmnd() on 1 MI355X",
+        "value": round(voxels / (ms_per_step * 1e-3), 1), "unit": "voxels/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic", "config": dict(cfg, k=pipe.k, gamma=pipe.gamma,
+                                            nf=list(pipe.nf) if pipe.nf else None),
+        "roofline": roof, "rooflines": rooflines, "cpu_baseline": cpu,
+        "cold_call_ms": round(cold_ms, 1),
+        "kernels_ms": {k: round(v, 3) for k, v in sorted(per_kernel.items())},
+        "psnr": psnr}
     print(json.dumps(line), flush=True)
     return 0
 
