@@ -667,10 +667,7 @@ __device__ __forceinline__ void bm_finalize(int K, int acc, const BMArgs &a, int
 //         reference writes it (blockmatching.py:356-369);
 // MODE 6: the candidate-standardized distance of code a.dcode
 //         (blockmatching.py:374-426): (sum of terms - sum of null means) /
-//         sqrt(sum of null variances), the null sums in float32;
-// MODE 10: the SSD of the Gaussian model, a float32 serial sum in patch C
-//         order less gamma times the noise SSD at the candidate's lag
-//         (blockmatching.py:320-334, 1133-1139); signed, no tie grid.
+//         sqrt(sum of null variances), the null sums in float32.
 // Occupancy targets of the count-table kernel: 6 waves per SIMD for the 4^3
 // block (80 VGPRs, no spills), 8 for the 8x8 block (64 VGPRs, four workgroups
 // per CU: 21.9 against 22.8 ms on sixteen 512^2 images), 4 for the 5^3 block,
@@ -745,7 +742,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(
     float v = a.vol[((int64_t)(o0 + p0) * a.n1 + (o1 + p1)) * a.n2 + (o2 + p2)];
     double r = (double)v * a.inv_scale;
     if (r < 0.0) r = 0.0;
-    s_ref[p] = MODE == 10 ? (double)v : r;  // MODE 10: the value itself
+    s_ref[p] = r;
     if (POISSON_SSD) s_ref_log[p] = 2.0 * sqrt(r + 0.375);  // Anscombe transform
     // MODE 2: byte offset of the reference count's term-table row
     if (MODE == 2)
@@ -842,7 +839,6 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(
     const int j2 = cand % e2, j1 = (cand / e2) % e1, j0 = cand / (e2 * e1);
     const int base = (j0 * S1 + j1) * S2 + j2;
     double d = 0.0;
-    float d32 = 0.0f;              // MODE 10
     float nms = 0.0f, nvs = 0.0f;  // MODE 6: null mean and variance sums
     // terms are accumulated in patch C order (p0, p1, p2), as the reference does
 #pragma unroll 1
@@ -855,10 +851,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(
       const double *lrow = s_ref_log + p01 * B2;
 #pragma unroll
       for (int p2 = 0; p2 < B2; ++p2) {
-        if (MODE == 10) {
-          const float df = row[p2] - (float)rrow[p2];
-          d32 = d32 + df * df;
-        } else if (MODE == 1) {
+        if (MODE == 1) {
           // the term is exactly 0 at r = c = 0 (the log of 0 stays finite)
           const double s = rrow[p2] + (double)row[p2];
           const double ls = dlog_bits(s, s_flog, ln2k);
@@ -899,16 +892,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(
     }
     if (MODE == 1) d = 2.0 * d;  // terms were accumulated without the factor 2
     bool ok;
-    if constexpr (MODE == 10) {
-      float key = d32;
-      if (a.noise_ssd) {
-        constexpr int W1 = 2 * R1 + 1, W2 = 2 * R2 + 1;
-        const int li = ((lo0 + j0 - o0 + R0) * W1 + (lo1 + j1 - o1 + R1)) * W2 +
-                       (lo2 + j2 - o2 + R2);
-        key = key - a.gamma * a.noise_ssd[li];
-      }
-      s_keys[cand] = bm_key_signed(key, a, ok);
-    } else if constexpr (MODE == 6) {
+    if constexpr (MODE == 6) {
       // float32 sqrt, correctly rounded through float64
       const float dstd =
           nvs > 0.0f ? (float)((d - (double)nms) / (double)(float)sqrt((double)nvs)) : 0.0f;
@@ -926,12 +910,136 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(
   const float thr_k = a.stabilize ? a.thr_key : a.threshold;
   auto composite = [&](int i) {
     const float key = s_keys[i];
-    if constexpr (MODE == 6 || MODE == 10) return bm_composite_signed(key, i);
+    if constexpr (MODE == 6) return bm_composite_signed(key, i);
     else return bm_composite(key, key < thr_k, i);
   };
   bm_select_write<NT, BOX>(bm_box_keys<NT>(composite, box), box, s_acc, a,
                            g, lo0, lo1, lo2, e1, e2, center_local, sc);
 }
+
+// SSD block matching of the stationary Gaussian model (blockmatching.py:
+// 320-334, 1133-1139): per candidate the float32 SSD summed in patch C order,
+// less gamma times the noise SSD at its lag; signed keys, no tie grid.  Each
+// work item is a run of C consecutive candidates along the last axis: per
+// patch row it reads the B2 + C - 1 slab values and the B2 reference values
+// once and forms C * B2 terms, each candidate's sum kept in its own register
+// in the serial order.  The slab is padded so that a run's reads past the
+// box end stay in the array (those candidates are discarded).
+template <int B0, int B1, int B2, int R0, int R1, int R2, int C, int NT>
+__global__ __launch_bounds__(NT) void k_blockmatch_ssd(BMArgs a) {
+  constexpr int S0 = 2 * R0 + B0, S1 = 2 * R1 + B1, S2 = 2 * R2 + B2;
+  constexpr int SLAB = S0 * S1 * S2;
+  constexpr int W0 = 2 * R0 + 1, W1 = 2 * R1 + 1, W2 = 2 * R2 + 1;
+  constexpr int BOX = W0 * W1 * W2;
+  constexpr int P = B0 * B1 * B2;
+  constexpr int RUN = B2 + C - 1;
+  __shared__ float s_slab[SLAB + C];
+  __shared__ float s_ref[P];
+  __shared__ float s_keys[BOX];
+  __shared__ int s_acc;
+
+  const int tid = threadIdx.x;
+  const int64_t g = blockIdx.x;
+  if (g >= a.nrefs) return;
+  bind_input(a, g);
+  const int lin = a.ref_lin[g];
+  const int o2 = lin % a.c2, o1 = (lin / a.c2) % a.c1, o0 = lin / (a.c2 * a.c1);
+  const int lo0 = max(o0 - R0, 0), hi0 = min(o0 + R0, a.c0 - 1);
+  const int lo1 = max(o1 - R1, 0), hi1 = min(o1 + R1, a.c1 - 1);
+  const int lo2 = max(o2 - R2, 0), hi2 = min(o2 + R2, a.c2 - 1);
+  const int e0 = hi0 - lo0 + 1, e1 = hi1 - lo1 + 1, e2 = hi2 - lo2 + 1;
+  const int box = e0 * e1 * e2;
+  // slab: volume region [lo, hi + B - 1] (inside the volume), zeros beyond;
+  // every load in flight before the first store
+  const int x0 = e0 + B0 - 1, x1 = e1 + B1 - 1, x2 = e2 + B2 - 1;
+  {
+    constexpr int IT = (SLAB + C + NT - 1) / NT;
+    float vv[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int i = tid + it * NT;
+      const int i2 = i % S2, i1 = (i / S2) % S1, i0 = i / (S2 * S1);
+      vv[it] = 0.0f;
+      if (i < SLAB && i0 < x0 && i1 < x1 && i2 < x2)
+        vv[it] = a.vol[((int64_t)(lo0 + i0) * a.n1 + (lo1 + i1)) * a.n2 + (lo2 + i2)];
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int i = tid + it * NT;
+      if (i < SLAB + C) s_slab[i] = vv[it];
+    }
+  }
+  for (int p = tid; p < P; p += NT) {
+    const int p2 = p % B2, p1 = (p / B2) % B1, p0 = p / (B2 * B1);
+    s_ref[p] = a.vol[((int64_t)(o0 + p0) * a.n1 + (o1 + p1)) * a.n2 + (o2 + p2)];
+  }
+  if (tid == 0) s_acc = 0;
+  __syncthreads();
+
+  const int rel0 = o0 - lo0, rel1 = o1 - lo1, rel2 = o2 - lo2;
+  const int center_local = (rel0 * e1 + rel1) * e2 + rel2;
+  const int nseg = (e2 + C - 1) / C;
+  const int items = e0 * e1 * nseg;
+  int my_acc = 0;
+  for (int item = tid; item < items; item += NT) {
+    const int seg = item % nseg, line = item / nseg;
+    const int j1 = line % e1, j0 = line / e1;
+    const int j2s = seg * C;
+    const float *base = s_slab + (j0 * S1 + j1) * S2 + j2s;
+    float d[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) d[c] = 0.0f;
+#pragma unroll 1
+    for (int p01 = 0; p01 < B0 * B1; ++p01) {
+      const int p0 = p01 / B1, p1 = p01 - p0 * B1;
+      const float *row = base + (p0 * S1 + p1) * S2;
+      float sv[RUN], rv[B2];
+#pragma unroll
+      for (int i = 0; i < RUN; ++i) sv[i] = row[i];
+#pragma unroll
+      for (int i = 0; i < B2; ++i) rv[i] = s_ref[p01 * B2 + i];
+#pragma unroll
+      for (int p2 = 0; p2 < B2; ++p2) {
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          const float df = sv[c + p2] - rv[p2];
+          d[c] = d[c] + df * df;
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const int j2 = j2s + c;
+      if (j2 < e2) {
+        float key = d[c];
+        if (a.noise_ssd) {
+          const int li = ((lo0 + j0 - o0 + R0) * W1 + (lo1 + j1 - o1 + R1)) * W2 +
+                         (lo2 + j2 - o2 + R2);
+          key = key - a.gamma * a.noise_ssd[li];
+        }
+        bool ok;
+        s_keys[(j0 * e1 + j1) * e2 + j2] = bm_key_signed(key, a, ok);
+        my_acc += ok ? 1 : 0;
+      }
+    }
+  }
+  wave_atomic_add(&s_acc, my_acc);
+  __syncthreads();
+  BM_SEL_SCRATCH(sc);
+  auto composite = [&](int i) { return bm_composite_signed(s_keys[i], i); };
+  bm_select_write<NT, BOX>(bm_box_keys<NT>(composite, box), box, s_acc, a, g, lo0, lo1, lo2, e1,
+                           e2, center_local, sc);
+}
+
+// Run length and workgroup size of the SSD matcher: whole box lines of 15
+// candidates for the 3-D blocks (225 runs per full box), thirds of the 39
+// candidates of the 2-D line (117 runs).
+template <int B2, int R2>
+struct SsdRun {
+  static constexpr int W2 = 2 * R2 + 1;
+  static constexpr int C = W2 <= 16 ? W2 : (W2 + 2) / 3;
+  static constexpr int NT = W2 <= 16 ? 256 : 128;
+};
 
 // Certified block matching (keys equal to MODE 0's).  The distance is
 // evaluated in the one-log form
@@ -3660,8 +3768,9 @@ static int32_t blockmatch_impl(const float *vol, const int64_t *shape_h, const i
       hipLaunchKernelGGL((k_blockmatch<b0, b1, b2, r0, r1, r2, 5, kBMThreads>), grid, blk, 0, s, \
                          a);                                                                     \
     else if (mode == 10)                                                                         \
-      hipLaunchKernelGGL((k_blockmatch<b0, b1, b2, r0, r1, r2, 10, kBMThreads>), grid, blk, 0, \
-                         s, a);                                                                  \
+      hipLaunchKernelGGL((k_blockmatch_ssd<b0, b1, b2, r0, r1, r2, SsdRun<b2, r2>::C,             \
+                                           SsdRun<b2, r2>::NT>),                                  \
+                         grid, dim3(SsdRun<b2, r2>::NT), 0, s, a);                               \
     else if (mode >= 7)                                                                          \
       hipLaunchKernelGGL((k_blockmatch<b0, b1, b2, r0, r1, r2, 6, kBMThreads>), grid, blk, 0, s, \
                          a);                                                                     \
