@@ -29,6 +29,7 @@ EXPORTS = (
     "bmnd_hip_blockmatch_ssd",
     "bmnd_hip_value_range",
     "bmnd_hip_gaussian_variance",
+    "bmnd_hip_median_mad",
     "bmnd_hip_log2_check",
     "bmnd_hip_matrix_table_size",
     "bmnd_hip_exact_std",
@@ -152,6 +153,8 @@ def _declare(L: ctypes.CDLL) -> None:
     L.bmnd_hip_gaussian_variance.argtypes = [vp, vp, vp, vp, vp, vp, vp, i64, i32, vp, i32, vp,
                                              vp]
     L.bmnd_hip_gaussian_variance.restype = i32
+    L.bmnd_hip_median_mad.argtypes = [vp, i64, vp, vp, vp]
+    L.bmnd_hip_median_mad.restype = i32
     L.bmnd_hip_filter_groups.argtypes = [i32, vp, vp, vp, vp, vp, vp, vp, vp, i64, i32, vp, vp,
                                          f32, i32, f64, vp, vp, vp]
     L.bmnd_hip_filter_groups.restype = i32

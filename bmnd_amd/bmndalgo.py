@@ -228,7 +228,10 @@ def bmnd_batch(volumes, profile, sigma: float | None = None,
     aggregation tiles) of all inputs together, which fills the GPU with
     inputs too small to fill it alone.  No counterpart in the reference; each
     input is checked as bmnd() checks it and its result is bmnd(input,
-    profile)'s, bit for bit.  Returns float32 arrays in input order."""
+    profile)'s, bit for bit.  Under the Gaussian model the noise tables
+    depend on each input's shape (and, with sigma None, on its estimated
+    sigma), so the inputs run one after another through the Gaussian
+    pipeline.  Returns float32 arrays in input order."""
     volumes = list(volumes)
     if len({np.ndim(v) for v in volumes}) > 1:
         raise ValueError("every input of a batch must have the same number of dimensions.")
@@ -241,6 +244,15 @@ def bmnd_batch(volumes, profile, sigma: float | None = None,
         else:
             check_supported(profile, resolved)
             live.append(i)
+    if live and not items[live[0]][1].is_poisson:
+        device = _device()
+        for i in live:
+            vol, resolved = items[i]
+            vol = np.ascontiguousarray(vol)
+            pipe = _gaussian_pipeline(profile, vol.shape, device, resolved)
+            _, wi, _ = pipe.run(torch.from_numpy(vol).to(device), sigma=sigma, vol_host=vol)
+            outs[i] = wi.est.cpu().numpy()
+        return outs
     if live:
         live = [live[j] for j in table_first_order([items[i][0] for i in live])]
         device = torch.device("cuda", torch.cuda.current_device())

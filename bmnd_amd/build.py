@@ -9,7 +9,8 @@ import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRCS = [os.path.join(HERE, "csrc", "bmnd_hip.hip"), os.path.join(HERE, "csrc", "bmnd_sigma.hip"),
-        os.path.join(HERE, "csrc", "bmnd_conv.hip"), os.path.join(HERE, "csrc", "bmnd_schedule.hip")]
+        os.path.join(HERE, "csrc", "bmnd_conv.hip"), os.path.join(HERE, "csrc", "bmnd_schedule.hip"),
+        os.path.join(HERE, "csrc", "bmnd_mad.hip")]
 OUT = os.path.join(HERE, "libbmnd_hip.so")
 
 HIPCC_FLAGS = [

@@ -91,13 +91,31 @@ class GaussianPipeline:
         self._tables: dict = {}
 
     # -- noise description ---------------------------------------------------------
-    def tables(self, vol_host: np.ndarray | None, sigma=None, sigma_psd=None) -> GaussianTables:
+    def estimate_sigma(self, vol: torch.Tensor, vol_host: np.ndarray | None = None) -> float:
+        """estimate_gaussian_global_sigma (sigmaestimation.py:7-45) of the
+        device tensor `vol`: the MAD / 0.6745 of its finite nonzero values,
+        median and MAD found on the device (bmnd_hip_median_mad).  When the
+        MAD is 0 the reference falls back to the IQR, a trimmed std and
+        sqrt(eps); those rare cases are evaluated on the host copy."""
+        out = torch.empty(2, dtype=torch.float32, device=self.device)
+        cnt = torch.empty(2, dtype=torch.int64, device=self.device)
+        _lib.check(_lib.lib().bmnd_hip_median_mad(_ptr(vol), vol.numel(), _ptr(out), _ptr(cnt),
+                                                  self._stream()))
+        mad = float(out.cpu().numpy()[1])
+        sigma = mad / 0.6745
+        if sigma > 0.0 and np.isfinite(sigma):
+            return sigma
+        from .psd import estimate_gaussian_global_sigma
+        return estimate_gaussian_global_sigma(vol_host if vol_host is not None
+                                              else vol.cpu().numpy())
+
+    def tables(self, vol: torch.Tensor | None, sigma=None, sigma_psd=None,
+               vol_host: np.ndarray | None = None) -> GaussianTables:
         """The noise description of (sigma, sigma_psd), built on first use:
-        with neither given, sigma is the MAD estimate of `vol_host`
+        with neither given, sigma is the MAD estimate of `vol`
         (variance.py:461-462)."""
         if sigma is None and sigma_psd is None:
-            from .psd import estimate_gaussian_global_sigma
-            sigma = estimate_gaussian_global_sigma(vol_host)
+            sigma = self.estimate_sigma(vol, vol_host)
         if sigma_psd is not None:
             arr = np.asarray(sigma_psd, dtype=np.float32)
             key = ("psd", arr.shape, arr.tobytes())
@@ -223,10 +241,7 @@ class GaussianPipeline:
             raise ValueError("vol must be a contiguous float32 tensor of the planned shape")
         t = timer or _NoTimer()
         tok = t.start("gaussian.tables")
-        host = None
-        if sigma is None and sigma_psd is None:
-            host = vol_host if vol_host is not None else vol.cpu().numpy()
-        tables = self.tables(host, sigma, sigma_psd)
+        tables = self.tables(vol, sigma, sigma_psd, vol_host)
         t.stop(tok)
         gsig = tables.model.global_sigma
         fill = float(np.float32(gsig)) if sigma_maps else None

@@ -158,6 +158,17 @@ int32_t bmnd_hip_blockmatch_ssd(const float *vol, const int64_t *shape_h,
                                 const float *noise_ssd, float gamma, int32_t *sel,
                                 int32_t *counts, int32_t *accepted, void *stream);
 
+/* Median and median absolute deviation behind the Gaussian sigma estimate
+ * (sigmaestimation.py:7-45): of the finite nonzero values of vol (the finite
+ * ones when every finite value is zero), out (device, 2 floats) <- numpy's
+ * median m (the float32 mean of the two central values for an even count)
+ * and the numpy median of |x - m|; counts (device, 2 x uint64) <- the number
+ * of values used and 1 when they are the finite ones.  The estimate is
+ * out[1] / 0.6745 when positive; the reference's IQR and trimmed-std
+ * fallbacks take over otherwise. */
+int32_t bmnd_hip_median_mad(const float *vol, int64_t n, float *out,
+                            unsigned long long *counts, void *stream);
+
 /* range (device, 2 floats) <- (min, max) of the n values of vol: the data
  * scale of compute_blockmatch_threshold (blockmatching.py:168-184). */
 int32_t bmnd_hip_value_range(const float *vol, int64_t n, float *range, void *stream);

@@ -45,7 +45,7 @@ def _run_both(noisy, prof, sigma=0.1, sigma_psd=None, sigma_maps=False):
     ht, wi, g = pipe.run(torch.from_numpy(noisy).to(DEV), sigma=sigma, sigma_psd=sigma_psd,
                          sigma_maps=sigma_maps, vol_host=noisy)
     torch.cuda.synchronize()
-    tables = pipe.tables(noisy, sigma, sigma_psd)
+    tables = pipe.tables(torch.from_numpy(noisy).to(DEV), sigma, sigma_psd)
     params = oracle_params(prof, noisy.shape, tables.model)
     if prof.ht_use_soft_thresholding:
         params.ht_soft = 1
@@ -130,7 +130,7 @@ def test_group_variance_matches_oracle_per_group(kind):
     _, noisy, prof = noisy_input(kind, 0.1, seed=5)
     pipe = GaussianPipeline(prof, noisy.shape, DEV)
     vol = torch.from_numpy(noisy).to(DEV)
-    tables = pipe.tables(noisy, 0.1)
+    tables = pipe.tables(vol, 0.1)
     L = _lib.lib()
     for name in ("ht", "wiener"):
         out = pipe.stage(name, vol, vol, tables, pilot=vol if name == "wiener" else None)
@@ -182,3 +182,55 @@ def test_bmnd_routes_gaussian_profiles_and_callbacks():
     out3 = bmnd(noisy, prof, sigma_psd=0.1)
     out4 = bmnd(noisy, prof, sigma=float(np.float32(0.1)))
     assert np.array_equal(out3.view(np.int32), out4.view(np.int32))
+
+
+@pytest.mark.parametrize("kind,shape", [("image", (256, 256)), ("volume", (48, 44, 40))])
+def test_larger_inputs_match_oracle(kind, shape):
+    print(f"gaussian larger: {kind} {shape}", flush=True)
+    truth, noisy, prof = noisy_input(kind, 0.15, seed=8, shape=shape)
+    stats, out, _ = _run_both(noisy, prof, 0.15)
+    _assert_identical(stats)
+    assert np.mean((out - truth) ** 2) < 0.3 * np.mean((noisy - truth) ** 2)
+
+
+def test_batch_runs_each_input_as_bmnd():
+    from bmnd_amd import bmnd_batch
+    inputs = [noisy_input("image", 0.1, seed=s, shape=(n, n))[1] for s, n in ((9, 48), (10, 72))]
+    prof = noisy_input("image", 0.1, seed=9, shape=(48, 48))[2]
+    outs = bmnd_batch(inputs + [np.zeros((40, 40), np.float32)], prof, sigma=0.1)
+    for x, o in zip(inputs, outs):
+        assert np.array_equal(o.view(np.int32), bmnd(x, prof, sigma=0.1).view(np.int32))
+    assert not np.any(outs[2])
+
+
+def _estimates(vol):
+    from bmnd_amd.psd import estimate_gaussian_global_sigma
+    pipe = GaussianPipeline(noisy_input("volume", 0.1)[2], (16, 16, 16), DEV)
+    dev = pipe.estimate_sigma(torch.from_numpy(np.ascontiguousarray(vol, np.float32)).to(DEV))
+    return dev, estimate_gaussian_global_sigma(vol)
+
+
+@pytest.mark.parametrize("case", ["odd", "even", "zeros", "ties", "constant", "one", "negative"])
+def test_device_sigma_estimate_equals_numpy(case):
+    # the device median / MAD against numpy's on the host
+    # (sigmaestimation.py:7-45), exactly
+    rng = np.random.default_rng(11)
+    if case == "odd":
+        vol = rng.normal(0.3, 0.2, size=(31, 33, 35)).astype(np.float32)
+    elif case == "even":
+        vol = rng.normal(0.3, 0.2, size=(32, 33, 34)).astype(np.float32)
+    elif case == "zeros":
+        vol = rng.normal(0.0, 1.0, size=(40, 40)).astype(np.float32)
+        vol[rng.random(vol.shape) < 0.4] = 0.0
+    elif case == "ties":
+        vol = rng.integers(-3, 5, size=(64, 64)).astype(np.float32)
+    elif case == "constant":
+        vol = np.full((20, 20, 20), 2.5, np.float32)  # MAD 0: the fallbacks
+    elif case == "one":
+        vol = np.zeros((16, 16), np.float32)
+        vol[3, 4] = 7.0
+    else:
+        vol = -np.abs(rng.normal(size=(50, 51))).astype(np.float32)
+    dev, host = _estimates(vol)
+    print(f"sigma estimate {case}: device {dev!r} numpy {host!r}", flush=True)
+    assert dev == host
