@@ -6,6 +6,7 @@ hand-written HIP kernels (libbmnd_hip.so) on one GPU.
 """
 
 from .bmndalgo import bmnd, bmnd_batch
+from .engine import set_exact_order
 from .enums import (
     AggregationWeightDomain,
     AggregationWeightModel,
@@ -44,4 +45,5 @@ __all__ = [
     "TransformType",
     "bmnd",
     "bmnd_batch",
+    "set_exact_order",
 ]

@@ -33,6 +33,7 @@ EXPORTS = (
     "bmnd_hip_log2_check",
     "bmnd_hip_matrix_table_size",
     "bmnd_hip_exact_std",
+    "bmnd_hip_exact_std_pairs",
     "bmnd_hip_filter_groups",
     "bmnd_hip_filter_groups_sigma",
     "bmnd_hip_reference_map",
@@ -46,6 +47,7 @@ EXPORTS = (
     "bmnd_hip_aggregate_force_tile",
     "bmnd_hip_blockmatch_batch",
     "bmnd_hip_exact_std_batch",
+    "bmnd_hip_exact_std_pairs_batch",
     "bmnd_hip_filter_groups_batch",
     "bmnd_hip_reference_map_batch",
     "bmnd_hip_aggregate_batch",
@@ -145,6 +147,8 @@ def _declare(L: ctypes.CDLL) -> None:
     L.bmnd_hip_matrix_table_size.restype = i64
     L.bmnd_hip_exact_std.argtypes = [vp, vp, vp, vp, vp, vp, i64, i32, vp, i32, vp, vp]
     L.bmnd_hip_exact_std.restype = i32
+    L.bmnd_hip_exact_std_pairs.argtypes = L.bmnd_hip_exact_std.argtypes
+    L.bmnd_hip_exact_std_pairs.restype = i32
     L.bmnd_hip_blockmatch_ssd.argtypes = [vp, vp, vp, vp, vp, i64, f32, i32, i32, i32, vp, f32,
                                           vp, vp, vp, vp]
     L.bmnd_hip_blockmatch_ssd.restype = i32
@@ -188,6 +192,8 @@ def _declare(L: ctypes.CDLL) -> None:
     L.bmnd_hip_exact_std_batch.argtypes = [vp, vp, vp, i32, vp, vp, vp, vp, vp, i64, i32, vp,
                                            i32, vp, vp]
     L.bmnd_hip_exact_std_batch.restype = i32
+    L.bmnd_hip_exact_std_pairs_batch.argtypes = L.bmnd_hip_exact_std_batch.argtypes
+    L.bmnd_hip_exact_std_pairs_batch.restype = i32
     L.bmnd_hip_filter_groups_batch.argtypes = [i32, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp, vp,
                                                i64, i32, vp, vp, f32, i32, f64, vp, vp, vp]
     L.bmnd_hip_filter_groups_batch.restype = i32

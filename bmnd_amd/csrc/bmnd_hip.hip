@@ -2062,6 +2062,226 @@ __global__ __launch_bounds__(64) void k_exact_std(ExactArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// the same exact planes by member pairs, separably (variance.py:652-975)
+// ---------------------------------------------------------------------------
+// Expanding the square of the union-voxel sum over the members that cover a
+// voxel gives, for plane q and spatial coefficient s,
+//   var[q,s] = sum_{i <= j} c_q(i,j) K_ij[s],
+//   c_q(i,j) = G[q,i] G[q,j]  (twice that for i < j),
+//   K_ij[s]  = sum_{p in overlap} var(x_i + p) S[s,p] S[s,p - d],  d = x_j - x_i,
+// over the pairs whose patches overlap (i == j included).  S = A0 (x) A1 (x)
+// A2, so S[s,p] S[s,p - d] = prod_a H_a^{d_a}[s_a,p_a] with
+// H_a^{e}[t,p] = A_a[t,p] A_a[t,p - e] (zero where p - e leaves the block),
+// and K_ij is a separable transform of the overlap region: three passes of
+// at most B_a terms per output instead of P terms per union voxel and entry.
+// On the 256^3 Wiener groups (32 members, about 200 overlapping pairs) that
+// is about a sixth of the serial form's multiply-adds.  The sums are float32
+// in another order than the reference's serial one, so the planes agree with
+// it to float32 rounding rather than bit for bit.
+//
+// One workgroup of NW waves per group.  The members' variance patches are
+// staged in LDS; the pairs are listed in (i, j) order by a workgroup scan and
+// dealt to the waves round robin; each wave runs the three passes of its
+// pairs through its own LDS rows (lanes of one wave only, ordered by
+// wave_sync) and adds c_q(i,j) K_ij[s] with v_mfma_f32_4x4x1_16b_f32: lane l
+// gives K[l] (A, rows = coefficients) and c_{l%4} (B, columns = planes), and
+// lane L, register r of the accumulator holds plane L%4 of coefficient
+// 4(L/4) + r.  The waves' partial sums are added in wave order.
+typedef float v4f_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int B0, int B1, int B2, int NMAX, int NW>
+__global__ __launch_bounds__(64 * NW) void k_exact_pairs(ExactArgs a) {
+  constexpr int P = B0 * B1 * B2;
+  constexpr int B12 = B1 * B2;
+  constexpr int SPL = (P + 63) / 64;
+  constexpr int NT = 64 * NW;
+  constexpr int E0 = 2 * B0 - 1, E1 = 2 * B1 - 1, E2 = 2 * B2 - 1;  // offsets per axis
+  constexpr int CPT = NMAX * NMAX / NT;  // pair candidates per thread
+  constexpr int VBUF = NMAX * P > NW * 4 * 64 * SPL ? NMAX * P : NW * 4 * 64 * SPL;
+  static_assert(SPL <= 2, "at most 128 spatial coefficients");
+  static_assert(NMAX <= 32 && (NMAX * NMAX) % NT == 0, "pair candidates per thread");
+  static_assert(E0 <= 16 && E1 <= 16 && E2 <= 16, "pair code fields");
+  __shared__ float s_buf[VBUF];  // member variance patches, then the wave sums
+  __shared__ float s_h0[E0 * B0 * B0], s_h1[E1 * B1 * B1], s_h2[E2 * B2 * B2];  // [e][t][p]
+  __shared__ float s_gq[NMAX][4];
+  __shared__ int s_org[NMAX][3];
+  __shared__ unsigned s_pairs[NMAX * (NMAX + 1) / 2];
+  __shared__ int s_wsum[NW];
+  __shared__ float s_y2[NW][P], s_y1[NW][P];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6;
+  const int64_t g = blockIdx.x;
+  if (g >= a.nrefs) return;
+  bind_input(a, g);
+  const int n = a.counts[g];
+  const int planes = min(a.k, n);
+  if (n <= 0 || planes <= 0) return;
+
+  const int64_t gtot = (int64_t)a.max_m * (a.max_m + 1) * (2 * a.max_m + 1) / 6;
+  const float *gf = a.mats + (int64_t)(n - 1) * n * (2 * n - 1) / 6;
+  const float *A0 = a.mats + 2 * gtot;
+  const float *A1 = A0 + B0 * B0;
+  const float *A2 = A1 + B1 * B1;
+  // H_a^{e - (B_a - 1)}[t][p] for every offset e
+  for (int x = tid; x < E0 * B0 * B0 + E1 * B1 * B1 + E2 * B2 * B2; x += NT) {
+    int y = x, b = B0, ax = 0;
+    const float *A = A0;
+    float *h = s_h0;
+    if (y >= E0 * B0 * B0) {
+      y -= E0 * B0 * B0;
+      b = B1, ax = 1, A = A1, h = s_h1;
+      if (y >= E1 * B1 * B1) {
+        y -= E1 * B1 * B1;
+        b = B2, ax = 2, A = A2, h = s_h2;
+      }
+    }
+    (void)ax;
+    const int e = y / (b * b), t = (y / b) % b, p = y % b;
+    const int q = p - (e - (b - 1));
+    h[y] = (q >= 0 && q < b) ? A[t * b + p] * A[t * b + q] : 0.0f;
+  }
+  if (tid < n) {
+    const int lin = a.sel[g * a.max_m + tid];
+    s_org[tid][2] = lin % a.c2;
+    s_org[tid][1] = (lin / a.c2) % a.c1;
+    s_org[tid][0] = lin / (a.c2 * a.c1);
+  }
+  __syncthreads();
+  // overlapping pairs (i <= j), listed in (i, j) order: CPT consecutive
+  // candidates per thread, then a workgroup exclusive scan of the counts
+  unsigned codes[CPT];
+  int cnt = 0;
+#pragma unroll
+  for (int c = 0; c < CPT; ++c) {
+    const int cand = tid * CPT + c;
+    const int i = cand / NMAX, j = cand % NMAX;
+    codes[c] = 0xffffffffu;
+    if (i <= j && j < n) {
+      const int d0 = s_org[j][0] - s_org[i][0], d1 = s_org[j][1] - s_org[i][1],
+                d2 = s_org[j][2] - s_org[i][2];
+      if (abs(d0) < B0 && abs(d1) < B1 && abs(d2) < B2) {
+        codes[c] = (unsigned)i | ((unsigned)j << 5) | ((unsigned)(d0 + B0 - 1) << 10) |
+                   ((unsigned)(d1 + B1 - 1) << 14) | ((unsigned)(d2 + B2 - 1) << 18);
+        ++cnt;
+      }
+    }
+  }
+  int incl = cnt;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += y;
+  }
+  if (lane == 63) s_wsum[w] = incl;
+  __syncthreads();
+  int base = 0, npairs = 0;
+#pragma unroll
+  for (int v = 0; v < NW; ++v) {
+    base += v < w ? s_wsum[v] : 0;
+    npairs += s_wsum[v];
+  }
+  int pos = base + incl - cnt;
+#pragma unroll
+  for (int c = 0; c < CPT; ++c)
+    if (codes[c] != 0xffffffffu) s_pairs[pos++] = codes[c];
+
+  float *y2 = s_y2[w], *y1 = s_y1[w];
+  for (int q0 = 0; q0 < planes; q0 += 4) {
+    // the members' variance patches, [i][p0][p1][p2] (the buffer holds the
+    // wave sums at the end of every four planes)
+    for (int x = tid; x < n * P; x += NT) {
+      const int i = x / P, p = x - i * P;
+      const int p0 = p / B12, p1 = (p / B2) % B1, p2 = p % B2;
+      s_buf[x] = a.var[((int64_t)(s_org[i][0] + p0) * a.n1 + (s_org[i][1] + p1)) * a.n2 +
+                       (s_org[i][2] + p2)];
+    }
+    for (int x = tid; x < n * 4; x += NT) {
+      const int i = x >> 2, q = x & 3;
+      s_gq[i][q] = q0 + q < planes ? gf[(q0 + q) * n + i] : 0.0f;
+    }
+    __syncthreads();
+    v4f_t acc[SPL];
+#pragma unroll
+    for (int k = 0; k < SPL; ++k) acc[k] = v4f_t{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int pr = w; pr < npairs; pr += NW) {
+      const unsigned code = (unsigned)__builtin_amdgcn_readfirstlane((int)s_pairs[pr]);
+      const int i = code & 31u, j = (code >> 5) & 31u;
+      const int e0 = (code >> 10) & 15u, e1 = (code >> 14) & 15u, e2 = (code >> 18) & 15u;
+      const int d0 = e0 - (B0 - 1), d1 = e1 - (B1 - 1);
+      const int lo0 = max(d0, 0), lo1 = max(d1, 0);
+      const int m0 = B0 - abs(d0), m1 = B1 - abs(d1);
+      const unsigned mg1 = 65536u / (unsigned)m1 + 1u;  // r / m1 == (r * mg1) >> 16 here
+      const float *vi = s_buf + i * P;
+      // axis 2: Y2[u0][u1][t2] = sum_p2 H2[t2][p2] v_i[lo0 + u0][lo1 + u1][p2]
+      for (int t = lane; t < m0 * m1 * B2; t += 64) {
+        const int r = t / B2, t2 = t - r * B2;
+        const int u0 = (int)(((unsigned)r * mg1) >> 16), u1 = r - u0 * m1;
+        const float *row = vi + ((lo0 + u0) * B1 + lo1 + u1) * B2;
+        const float *h = s_h2 + (e2 * B2 + t2) * B2;
+        float y = 0.0f;
+#pragma unroll
+        for (int p2 = 0; p2 < B2; ++p2) y = fmaf(h[p2], row[p2], y);
+        y2[t] = y;
+      }
+      wave_sync();
+      // axis 1: Y1[u0][t1][t2] = sum_u1 H1[t1][lo1 + u1] Y2[u0][u1][t2]
+      for (int t = lane; t < m0 * B12; t += 64) {
+        const int u0 = t / B12, t12 = t - u0 * B12;
+        const int t1 = t12 / B2, t2 = t12 - t1 * B2;
+        const float *h = s_h1 + (e1 * B1 + t1) * B1 + lo1;
+        const float *src = y2 + u0 * m1 * B2 + t2;
+        float y = 0.0f;
+        for (int u1 = 0; u1 < m1; ++u1) y = fmaf(h[u1], src[u1 * B2], y);
+        y1[t] = y;
+      }
+      wave_sync();
+      // axis 0: K[t0][t1][t2] = sum_u0 H0[t0][lo0 + u0] Y1[u0][t1][t2]
+      float kv[SPL];
+#pragma unroll
+      for (int k = 0; k < SPL; ++k) {
+        const int s = lane + 64 * k;
+        float y = 0.0f;
+        if (s < P) {
+          const int t0 = s / B12, t12 = s - t0 * B12;
+          const float *h = s_h0 + (e0 * B0 + t0) * B0 + lo0;
+          for (int u0 = 0; u0 < m0; ++u0) y = fmaf(h[u0], y1[u0 * B12 + t12], y);
+        }
+        kv[k] = y;
+      }
+      wave_sync();
+      const float c = s_gq[i][lane & 3] * s_gq[j][lane & 3] * (i == j ? 1.0f : 2.0f);
+#pragma unroll
+      for (int k = 0; k < SPL; ++k) acc[k] = __builtin_amdgcn_mfma_f32_4x4x1f32(kv[k], c, acc[k], 0, 0, 0);
+    }
+    // the waves' sums, [w][q][s], added in wave order
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < SPL; ++k)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        s_buf[(w * 4 + (lane & 3)) * (64 * SPL) + 64 * k + 4 * (lane >> 2) + r] = acc[k][r];
+    __syncthreads();
+    for (int x = tid; x < 4 * P; x += NT) {
+      const int q = x / P, s = x - q * P;
+      if (q0 + q < planes) {
+        float v = 0.0f;
+#pragma unroll
+        for (int u = 0; u < NW; ++u) v += s_buf[(u * 4 + q) * (64 * SPL) + s];
+        a.exact_std[(g * a.kcap + q0 + q) * P + s] = sqrtf(v > 0.0f ? v : 0.0f);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // exact-plane noise variance of the stationary Gaussian model
 // (variance.py:1889-2106)
 // ---------------------------------------------------------------------------
@@ -2335,8 +2555,19 @@ __device__ __forceinline__ float wiener_signal_power(float r, float nv, float ef
 }
 
 template <int B0, int B1, int B2, int R0, int R1, int R2, int NMAX, int NT, bool WIENER,
-          bool SIGMA>
+          bool SIGMA, bool OPT>
 __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
+  if constexpr (!OPT) {
+    // the default filter (hard threshold or Wiener gain, variance weights,
+    // patch scope, Poisson model): the option branches fold away and leave
+    // the registers to the default path
+    a.soft = 0;
+    a.wmodel = 0;
+    a.windowed = 0;
+    a.group_scope = 0;
+    a.coef_out = 0;
+    a.gaussian = 0;
+  }
   constexpr int P = B0 * B1 * B2;
   constexpr int NP = NMAX * P;
   static_assert(NP <= 4096, "group too large");
@@ -2676,12 +2907,28 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
       double risk = 0.0;
       constexpr int CH = 25;  // a chunk's LDS reads in flight together
       for (int c0 = 0; c0 < P; c0 += CH) {
-        double ev[CH];
+        if constexpr (!OPT) {
+          // the variance model: a^2 sigma^2, from the float values
+          float atv[CH], sgv[CH];
 #pragma unroll
-        for (int j = 0; j < CH; ++j) ev[j] = c0 + j < P ? coef_error(tid * P + c0 + j) : 0.0;
+          for (int j = 0; j < CH; ++j) {
+            atv[j] = c0 + j < P ? atten[tid * P + c0 + j] : 0.0f;
+            sgv[j] = c0 + j < P ? s_sig[tid * P + c0 + j] : 0.0f;
+          }
 #pragma unroll
-        for (int j = 0; j < CH; ++j)
-          if (c0 + j < P) risk = risk + ev[j];
+          for (int j = 0; j < CH; ++j)
+            if (c0 + j < P) {
+              const double ad = (double)atv[j];
+              risk = risk + (ad * ad) * (double)(sgv[j] * sgv[j]);
+            }
+        } else {
+          double ev[CH];
+#pragma unroll
+          for (int j = 0; j < CH; ++j) ev[j] = c0 + j < P ? coef_error(tid * P + c0 + j) : 0.0;
+#pragma unroll
+          for (int j = 0; j < CH; ++j)
+            if (c0 + j < P) risk = risk + ev[j];
+        }
       }
       s_plane[tid] = risk;
     }
@@ -3953,7 +4200,8 @@ int64_t bmnd_hip_matrix_table_size(int32_t max_matches, const int32_t *block_h) 
 static int32_t exact_std_impl(const float *var, const int64_t *shape_h, const int32_t *block_h,
                               const int32_t *radius_h, const int32_t *sel, const int32_t *counts,
                               int64_t nrefs, int32_t max_matches, const float *matrices,
-                              int32_t k, float *exact_std, void *stream, const BatchRef *b) {
+                              int32_t k, float *exact_std, void *stream, const BatchRef *b,
+                              bool pairs = false) {
   int c[3];
   int64_t shape1[3];
   if (b) {
@@ -3994,9 +4242,22 @@ static int32_t exact_std_impl(const float *var, const int64_t *shape_h, const in
     HIP_TRY(hipGetLastError());                                                             \
     return 0;                                                                               \
   }
-  E_CASE(1, 8, 8, 0, 19, 19, 32)
-  E_CASE(4, 4, 4, 7, 7, 7, 16)
-  E_CASE(5, 5, 5, 7, 7, 7, 32)
+#define EP_CASE(b0, b1, b2, r0, r1, r2, nmax, nw)                                            \
+  if (same3(block_h, b0, b1, b2) && same3(radius_h, r0, r1, r2) && max_matches <= nmax) {    \
+    hipLaunchKernelGGL((k_exact_pairs<b0, b1, b2, nmax, nw>), grid, dim3(64 * nw), 0, s, a); \
+    HIP_TRY(hipGetLastError());                                                             \
+    return 0;                                                                               \
+  }
+  if (pairs) {
+    EP_CASE(1, 8, 8, 0, 19, 19, 32, 4)
+    EP_CASE(4, 4, 4, 7, 7, 7, 16, 4)
+    EP_CASE(5, 5, 5, 7, 7, 7, 32, 4)
+  } else {
+    E_CASE(1, 8, 8, 0, 19, 19, 32)
+    E_CASE(4, 4, 4, 7, 7, 7, 16)
+    E_CASE(5, 5, 5, 7, 7, 7, 32)
+  }
+#undef EP_CASE
 #undef E_CASE
   return fail(BMND_HIP_EINVAL, "block/radius/stack configuration not compiled into this library");
 }
@@ -4007,6 +4268,28 @@ int32_t bmnd_hip_exact_std(const float *var, const int64_t *shape_h, const int32
                            float *exact_std, void *stream) {
   return exact_std_impl(var, shape_h, block_h, radius_h, sel, counts, nrefs, max_matches,
                         matrices, k, exact_std, stream, nullptr);
+}
+
+int32_t bmnd_hip_exact_std_pairs(const float *var, const int64_t *shape_h,
+                                 const int32_t *block_h, const int32_t *radius_h,
+                                 const int32_t *sel, const int32_t *counts, int64_t nrefs,
+                                 int32_t max_matches, const float *matrices, int32_t k,
+                                 float *exact_std, void *stream) {
+  return exact_std_impl(var, shape_h, block_h, radius_h, sel, counts, nrefs, max_matches,
+                        matrices, k, exact_std, stream, nullptr, true);
+}
+
+int32_t bmnd_hip_exact_std_pairs_batch(const float *var, const bmnd_hip_input *inputs_h,
+                                       const bmnd_hip_input *inputs_d, int32_t ninputs,
+                                       const int32_t *block_h, const int32_t *radius_h,
+                                       const int32_t *sel, const int32_t *counts,
+                                       const int32_t *ref_input, int64_t nrefs,
+                                       int32_t max_matches, const float *matrices, int32_t k,
+                                       float *exact_std, void *stream) {
+  if (!ref_input) return fail(BMND_HIP_EINVAL, "a batch needs ref_input");
+  const BatchRef b{inputs_h, inputs_d, ninputs, ref_input};
+  return exact_std_impl(var, nullptr, block_h, radius_h, sel, counts, nrefs, max_matches,
+                        matrices, k, exact_std, stream, &b, true);
 }
 
 int32_t bmnd_hip_exact_std_batch(const float *var, const bmnd_hip_input *inputs_h,
@@ -4114,18 +4397,26 @@ static int32_t filter_impl(int32_t wiener, const float *noisy, const float *pilo
   }
   hipStream_t s = (hipStream_t)stream;
   dim3 grid((unsigned)nrefs);
-#define F_LAUNCH(b0, b1, b2, r0, r1, r2, nmax, nt, w, sg)                                    \
-  hipLaunchKernelGGL((k_filter<b0, b1, b2, r0, r1, r2, nmax, nt, w, sg>), grid, dim3(nt), 0, s, a)
+  // the option bits (soft threshold, weight model, windowed domain, group
+  // scope, coefficient output, Gaussian model) select the general instance
+  const bool opt = (code & ~3) != 0;
+#define F_LAUNCH(b0, b1, b2, r0, r1, r2, nmax, nt, w, sg, o)                                  \
+  hipLaunchKernelGGL((k_filter<b0, b1, b2, r0, r1, r2, nmax, nt, w, sg, o>), grid, dim3(nt), 0, \
+                     s, a)
 #define F_CASE(b0, b1, b2, r0, r1, r2, nmax, nt)                                            \
   if (same3(block_h, b0, b1, b2) && same3(radius_h, r0, r1, r2) && max_matches <= nmax) {   \
     if (wiener && sigma_out)                                                               \
-      F_LAUNCH(b0, b1, b2, r0, r1, r2, nmax, nt, true, true);                              \
+      F_LAUNCH(b0, b1, b2, r0, r1, r2, nmax, nt, true, true, true);                        \
+    else if (wiener && opt)                                                                \
+      F_LAUNCH(b0, b1, b2, r0, r1, r2, nmax, nt, true, false, true);                       \
     else if (wiener)                                                                       \
-      F_LAUNCH(b0, b1, b2, r0, r1, r2, nmax, nt, true, false);                             \
+      F_LAUNCH(b0, b1, b2, r0, r1, r2, nmax, nt, true, false, false);                      \
     else if (sigma_out)                                                                    \
-      F_LAUNCH(b0, b1, b2, r0, r1, r2, nmax, nt, false, true);                             \
+      F_LAUNCH(b0, b1, b2, r0, r1, r2, nmax, nt, false, true, true);                       \
+    else if (opt)                                                                          \
+      F_LAUNCH(b0, b1, b2, r0, r1, r2, nmax, nt, false, false, true);                      \
     else                                                                                   \
-      F_LAUNCH(b0, b1, b2, r0, r1, r2, nmax, nt, false, false);                            \
+      F_LAUNCH(b0, b1, b2, r0, r1, r2, nmax, nt, false, false, false);                     \
     HIP_TRY(hipGetLastError());                                                            \
     return 0;                                                                              \
   }
@@ -4517,6 +4808,7 @@ static int64_t align256(int64_t bytes) { return (bytes + 255) & ~(int64_t)255; }
 
 int64_t bmnd_hip_stage_workspace(const int64_t *shape_h, const int32_t *block_h, int64_t nrefs,
                                  int32_t max_matches, int32_t k) {
+  k &= BMND_HIP_K_PLANES;
   if (!shape_h || !block_h || nrefs < 0 || max_matches < 1 || k < 0) return -1;
   int64_t P = 1, patches = 1;
   for (int d = 0; d < 3; ++d) {
@@ -4545,6 +4837,8 @@ static int32_t stage_impl(int32_t wiener, const float *match_vol, const float *n
                           int32_t *counts, int32_t *accepted, void *stream,
                           int32_t *over_ws = nullptr, void *extra = nullptr,
                           int64_t extra_bytes = 0) {
+  const bool pairs = (k & (wiener ? BMND_HIP_EXACT_PAIRS_WIENER : BMND_HIP_EXACT_PAIRS_HT)) != 0;
+  k &= BMND_HIP_K_PLANES;
   const int64_t need = bmnd_hip_stage_workspace(shape_h, block_h, nrefs, max_matches, k);
   if (need < 0) return fail(BMND_HIP_EINVAL, "invalid stage geometry");
   if (!workspace || workspace_bytes < need)
@@ -4601,8 +4895,8 @@ static int32_t stage_impl(int32_t wiener, const float *match_vol, const float *n
                                extra_bytes))
     return rc;
   if (kcap > 0)
-    if (int rc = bmnd_hip_exact_std(var, shape_h, block_h, radius_h, sel, counts, nrefs,
-                                    max_matches, matrices, k, exact, stream))
+    if (int rc = exact_std_impl(var, shape_h, block_h, radius_h, sel, counts, nrefs,
+                                max_matches, matrices, k, exact, stream, nullptr, pairs))
       return rc;
   if (int rc = bmnd_hip_filter_groups(wiener, noisy, wiener ? match_vol : nullptr, var, shape_h,
                                       block_h, radius_h, sel, counts, nrefs, max_matches,
@@ -4796,7 +5090,8 @@ int32_t bmnd_hip_pipeline_create(const bmnd_hip_pipeline_desc *d, bmnd_hip_pipel
         patches *= d->shape[a] - d->block[st][a] + 1;
       }
       const int64_t R = d->nrefs[st], M = d->max_matches[st];
-      const int64_t kcap = std::min<int64_t>(d->k, M);
+      const int32_t k = d->k & BMND_HIP_K_PLANES;
+      const int64_t kcap = std::min<int64_t>(k, M);
       char *sw = static_cast<char *>(stage_ws) + align256(2 * sizeof(int32_t));
       float *exact = kcap > 0 ? reinterpret_cast<float *>(sw) : nullptr;
       sw += align256(R * kcap * P * (int64_t)sizeof(float));
@@ -4813,13 +5108,15 @@ int32_t bmnd_hip_pipeline_create(const bmnd_hip_pipeline_desc *d, bmnd_hip_pipel
         return rc;
       if (kcap > 0)
         if (int rc = exact_std_impl(var, d->shape, d->block[st], d->radius[st], d->sel[st],
-                                    d->counts[st], R, (int32_t)M, d->matrices[st], d->k, exact,
-                                    cs, nullptr))
+                                    d->counts[st], R, (int32_t)M, d->matrices[st], k, exact,
+                                    cs, nullptr,
+                                    (d->k & (st ? BMND_HIP_EXACT_PAIRS_WIENER
+                                                : BMND_HIP_EXACT_PAIRS_HT)) != 0))
           return rc;
       if (int rc = filter_impl(wiener ? d->wiener_gain : 0, d->input, wiener ? d->pilot : nullptr,
                                var, d->shape, d->block[st], d->radius[st], d->sel[st],
                                d->counts[st], R, (int32_t)M, d->matrices[st], exact,
-                               d->shrink[st], d->k, d->count_scale, filt, weights, nullptr, cs,
+                               d->shrink[st], k, d->count_scale, filt, weights, nullptr, cs,
                                nullptr))
         return rc;
       if (int rc = reference_map_impl(d->ref_lin[st], R, ref_index, patches, cs, true)) return rc;

@@ -37,6 +37,51 @@ from .shifts import LazySchedule, ScheduleArrays
 from .transforms import Transform, TransformMode, TransformType, group_matrices, spatial_matrices
 from .utils import kaiser_window_nd
 
+# Summation order of each stage's exact overlap-aware planes:
+#   "serial" -- bmnd_hip_exact_std: the reference's order (union voxels in C
+#               order, members increasing inside a voxel), every product and
+#               sum a separate float32 operation; bit for bit the oracle's.
+#   "pairs"  -- bmnd_hip_exact_std_pairs: the square expanded over member
+#               pairs, each pair's overlap sum a separable transform, the plane
+#               products on the matrix cores; the same quantity to float32
+#               rounding, several times faster.
+# Stage 1's planes decide hard-threshold masks and through them stage 2's
+# lists, so stage 1 keeps the serial order; the Wiener planes enter the gains
+# and weights continuously and take the pair form.
+EXACT_ORDERS = ("serial", "pairs")
+EXACT_ORDER = {"ht": "serial", "wiener": "pairs"}
+K_PAIRS_FLAG = {"ht": 1 << 16, "wiener": 1 << 17}  # BMND_HIP_EXACT_PAIRS_HT / _WIENER
+
+
+def set_exact_order(ht: str | None = None, wiener: str | None = None) -> dict:
+    """Set the exact planes' summation order per stage ("serial" or "pairs")
+    for the calls that follow; returns the previous setting."""
+    prev = dict(EXACT_ORDER)
+    for name, value in (("ht", ht), ("wiener", wiener)):
+        if value is None:
+            continue
+        if value not in EXACT_ORDERS:
+            raise ValueError(f"exact order must be one of {EXACT_ORDERS}, got {value!r}")
+        EXACT_ORDER[name] = value
+    return prev
+
+
+def k_with_order(k: int) -> int:
+    """k with the pair-order flags of the current setting OR'd in (the k of
+    bmnd_hip_stage and bmnd_hip_pipeline_desc)."""
+    return int(k) | sum(K_PAIRS_FLAG[st] for st in ("ht", "wiener") if EXACT_ORDER[st] == "pairs")
+
+
+def exact_std_entry(name: str, batch: bool = False):
+    """The library entry point computing stage `name`'s exact planes in the
+    current order."""
+    L = _lib.lib()
+    pairs = EXACT_ORDER[name] == "pairs"
+    if batch:
+        return L.bmnd_hip_exact_std_pairs_batch if pairs else L.bmnd_hip_exact_std_batch
+    return L.bmnd_hip_exact_std_pairs if pairs else L.bmnd_hip_exact_std
+
+
 # (block, search radius) pairs compiled into libbmnd_hip.so, in 3-D form
 SUPPORTED = {
     ((1, 8, 8), (0, 19, 19)),
@@ -575,8 +620,9 @@ class PoissonPipeline:
         if kcap > 0:
             exact = torch.empty(R * kcap * plan.P, dtype=torch.float32, device=dev)
             tok = timer.start(f"{name}.exact_std")
-            _lib.check(L.bmnd_hip_exact_std(_ptr(var), shape, block, radius, _ptr(sel), _ptr(cnt),
-                                            R, plan.max_m, _ptr(mats), self.k, _ptr(exact), stream))
+            _lib.check(exact_std_entry(name)(_ptr(var), shape, block, radius, _ptr(sel),
+                                             _ptr(cnt), R, plan.max_m, _ptr(mats), self.k,
+                                             _ptr(exact), stream))
             timer.stop(tok)
         filt = torch.empty(R * plan.max_m * plan.P, dtype=torch.float32, device=dev)
         wts = torch.empty(R * plan.max_m, dtype=torch.float32, device=dev)
@@ -732,7 +778,7 @@ class PoissonPipeline:
         if g is None:
             self._graph = g = StageGraph(self)
         g.input.copy_(torch.as_tensor(vol).view(self.shape), non_blocking=False)
-        if g.handle is None:
+        if g.handle is None or g.k != k_with_order(self.k):
             g.capture(*self._graph_modes(g.input))
         rc = g.launch()
         if rc == _lib_einval():
@@ -776,6 +822,7 @@ class StageGraph:
                                 torch.empty(R, dtype=torch.int32, device=dev))
         self.handle = None
         self.modes = None
+        self.k = None
 
     def capture(self, mode_ht: int, table_max: int, mode_wi: int) -> None:
         self.release()
@@ -797,7 +844,7 @@ class StageGraph:
             sel, cnt, acc = self.lists[name]
             d.sel[st], d.counts[st], d.accepted[st] = sel.data_ptr(), cnt.data_ptr(), acc.data_ptr()
         d.mode[0], d.mode[1], d.table_max = mode_ht, mode_wi, table_max
-        d.k, d.wiener_gain = pipe.k, pipe.wiener_arg
+        d.k, d.wiener_gain = k_with_order(pipe.k), pipe.wiener_arg
         d.count_scale, d.variance_floor = pipe.count_scale, pipe.floor
         src = lambda f: coerce_enum(getattr(pipe.profile, f), PoissonVarianceSource, f)  # noqa: E731
         d.match_observation = int(src("poisson_wiener_match_source")
@@ -810,6 +857,7 @@ class StageGraph:
         torch.cuda.synchronize(pipe.device)
         _lib.check(L.bmnd_hip_pipeline_create(ctypes.byref(d), ctypes.byref(h)))
         self.handle, self.modes = h, (mode_ht, table_max, mode_wi)
+        self.k = d.k
 
     def launch(self) -> int:
         """One launch on the current stream, then its status (synchronises)."""
@@ -1029,7 +1077,7 @@ class BatchPipeline:
         if kcap > 0:
             exact = torch.empty(R * kcap * plan.P, dtype=torch.float32, device=dev)
             tok = timer.start(f"{name}.exact_std")
-            _lib.check(L.bmnd_hip_exact_std_batch(
+            _lib.check(exact_std_entry(name, batch=True)(
                 _ptr(var), ih, idv, nin, block, radius, _ptr(sel), _ptr(cnt), _ptr(st.ref_input),
                 R, plan.max_m, _ptr(mats), k, _ptr(exact), stream))
             timer.stop(tok)

@@ -209,6 +209,30 @@ int32_t bmnd_hip_exact_std(const float *var, const int64_t *shape_h,
                            const float *matrices, int32_t k, float *exact_std,
                            void *stream);
 
+/* The same exact-plane std summed in another order, for speed: expanding the
+ * square over the members covering each voxel,
+ *   var[q,s] = sum over member pairs i <= j whose patches overlap of
+ *              G[q,i] G[q,j] (x2 for i < j) *
+ *              sum over the overlap p of var(x_i + p) S[s,p] S[s,p - (x_j - x_i)],
+ * each pair's inner sum a separable transform of its overlap region (three
+ * float32 passes, one per axis) and the plane products added on the matrix
+ * cores.  The same quantity as bmnd_hip_exact_std (variance.py:652-975) to
+ * float32 rounding, not bit for bit: its sums run in a different order.
+ * Same arguments and output. */
+int32_t bmnd_hip_exact_std_pairs(const float *var, const int64_t *shape_h,
+                                 const int32_t *block_h, const int32_t *radius_h,
+                                 const int32_t *sel, const int32_t *counts, int64_t nrefs,
+                                 int32_t max_matches, const float *matrices, int32_t k,
+                                 float *exact_std, void *stream);
+
+/* Flags OR'd into the k argument of bmnd_hip_stage, bmnd_hip_stage_noalloc,
+ * bmnd_hip_stage_workspace and into bmnd_hip_pipeline_desc.k (the exact
+ * planes are k & BMND_HIP_K_PLANES): the HT or the Wiener stage computes its
+ * exact planes with bmnd_hip_exact_std_pairs instead of bmnd_hip_exact_std. */
+#define BMND_HIP_K_PLANES 0xffff
+#define BMND_HIP_EXACT_PAIRS_HT (1 << 16)
+#define BMND_HIP_EXACT_PAIRS_WIENER (1 << 17)
+
 /* Collaborative filtering of every group of a stage: gather, separable
  * forward transform, coefficient noise std (the exact planes q < k from
  * bmnd_hip_exact_std plus the rescaled approximate planes), shrinkage,
@@ -540,6 +564,15 @@ int32_t bmnd_hip_exact_std_batch(const float *var, const bmnd_hip_input *inputs_
                                  const int32_t *ref_input, int64_t nrefs,
                                  int32_t max_matches, const float *matrices, int32_t k,
                                  float *exact_std, void *stream);
+
+/* bmnd_hip_exact_std_pairs over a batch. */
+int32_t bmnd_hip_exact_std_pairs_batch(const float *var, const bmnd_hip_input *inputs_h,
+                                       const bmnd_hip_input *inputs_d, int32_t ninputs,
+                                       const int32_t *block_h, const int32_t *radius_h,
+                                       const int32_t *sel, const int32_t *counts,
+                                       const int32_t *ref_input, int64_t nrefs,
+                                       int32_t max_matches, const float *matrices, int32_t k,
+                                       float *exact_std, void *stream);
 
 /* bmnd_hip_filter_groups over a batch (noisy, pilot, var: packed volumes). */
 int32_t bmnd_hip_filter_groups_batch(int32_t wiener, const float *noisy, const float *pilot,

@@ -10,6 +10,26 @@ if ROOT not in sys.path:
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X GPU and the built HIP library")
+    config.addinivalue_line("markers", "default_exact_order: run with the library's default "
+                            "exact-plane order (the Wiener planes by member pairs)")
+
+
+@pytest.fixture(autouse=True)
+def _serial_exact_order(request):
+    """GPU tests compare bit for bit with the oracle, which sums the exact
+    planes in the reference's serial order: they run both stages in that
+    order unless marked default_exact_order (those hold the pair order to its
+    own tolerance)."""
+    if request.node.get_closest_marker("gpu") is None or \
+            request.node.get_closest_marker("default_exact_order") is not None:
+        yield
+        return
+    from bmnd_amd import engine
+    prev = engine.set_exact_order(ht="serial", wiener="serial")
+    try:
+        yield
+    finally:
+        engine.set_exact_order(**prev)
 
 
 def gpu_available() -> bool:
