@@ -1833,6 +1833,7 @@ struct ExactArgs {
   float *exact_std;
   const bmnd_hip_input *batch;
   const int32_t *ref_input;
+  const float *hpair;  // the matrix table's pair tables (k_exact_pairs_rows)
 };
 
 __device__ __forceinline__ void bind_input(ExactArgs &a, int64_t g) {
@@ -2088,6 +2089,7 @@ __global__ __launch_bounds__(64) void k_exact_std(ExactArgs a) {
 // lane L, register r of the accumulator holds plane L%4 of coefficient
 // 4(L/4) + r.  The waves' partial sums are added in wave order.
 typedef float v4f_t __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(4))) const float cfloat_t;
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -2274,6 +2276,186 @@ __global__ __launch_bounds__(64 * NW) void k_exact_pairs(ExactArgs a) {
         float v = 0.0f;
 #pragma unroll
         for (int u = 0; u < NW; ++u) v += s_buf[(u * 4 + q) * (64 * SPL) + s];
+        a.exact_std[(g * a.kcap + q0 + q) * P + s] = sqrtf(v > 0.0f ? v : 0.0f);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Cubic blocks: the same sum with one pair's passes on B*B lanes, one row
+// each, and the H values uniform across the wave, read from the matrix
+// table's pair tables into scalar registers, so every multiply-add is one
+// VALU instruction and no table is read from LDS:
+//   axis 2: lane (p0,p1) turns its variance row into
+//           Y2[p0][p1][t2] = sum_p2 H2[t2][p2] v_i[p0][p1][p2] for every t2,
+//   axis 1: lane (p0,t2): Y1[p0][t1][t2] = sum_p1 H1[t1][p1] Y2[p0][p1][t2],
+//   axis 0: lane (t1,t2): K[t0][t1][t2] = sum_p0 H0[t0][p0] Y1[p0][t1][t2],
+// the rows changing hands through the wave's LDS rows between the passes.
+// The sums run over whole rows for every pair: H is zero off the overlap and
+// every staged variance is finite.  Each K[t0][.] enters the planes by one
+// v_mfma_f32_4x4x1_16b_f32 (lane l gives K[t0][l] and c_{l%4}); lane L,
+// register r of accumulator t0 then holds plane L%4 of coefficient
+// t0*B*B + 4(L/4) + r.
+template <int B, int NMAX, int NW>
+__global__ __launch_bounds__(64 * NW) void k_exact_pairs_rows(ExactArgs a) {
+  constexpr int BB = B * B;
+  constexpr int P = BB * B;
+  constexpr int NT = 64 * NW;
+  constexpr int E = 2 * B - 1;  // offsets per axis
+  constexpr int CPT = NMAX * NMAX / NT;
+  constexpr int RED = NW * 4 * P;  // the waves' sums, [w][q][t0][t1 t2]
+  constexpr int VBUF = NMAX * P > RED ? NMAX * P : RED;
+  static_assert(BB <= 64, "one lane per row");
+  static_assert(NMAX <= 32 && (NMAX * NMAX) % NT == 0, "pair candidates per thread");
+  static_assert(E <= 16, "pair code fields");
+  __shared__ float s_buf[VBUF];  // member variance patches, then the wave sums
+  __shared__ float s_gq[NMAX][4];
+  __shared__ int s_org[NMAX][3];
+  __shared__ unsigned s_pairs[NMAX * (NMAX + 1) / 2];
+  __shared__ int s_wsum[NW];
+  __shared__ float s_y[NW][2][BB * B];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6;
+  const int64_t g = blockIdx.x;
+  if (g >= a.nrefs) return;
+  bind_input(a, g);
+  const int n = a.counts[g];
+  const int planes = min(a.k, n);
+  if (n <= 0 || planes <= 0) return;
+  const float *gf = a.mats + (int64_t)(n - 1) * n * (2 * n - 1) / 6;
+  if (tid < n) {
+    const int lin = a.sel[g * a.max_m + tid];
+    s_org[tid][2] = lin % a.c2;
+    s_org[tid][1] = (lin / a.c2) % a.c1;
+    s_org[tid][0] = lin / (a.c2 * a.c1);
+  }
+  __syncthreads();
+  // overlapping pairs (i <= j) in (i, j) order, as in k_exact_pairs
+  unsigned codes[CPT];
+  int cnt = 0;
+#pragma unroll
+  for (int c = 0; c < CPT; ++c) {
+    const int cand = tid * CPT + c;
+    const int i = cand / NMAX, j = cand % NMAX;
+    codes[c] = 0xffffffffu;
+    if (i <= j && j < n) {
+      const int d0 = s_org[j][0] - s_org[i][0], d1 = s_org[j][1] - s_org[i][1],
+                d2 = s_org[j][2] - s_org[i][2];
+      if (abs(d0) < B && abs(d1) < B && abs(d2) < B) {
+        codes[c] = (unsigned)i | ((unsigned)j << 5) | ((unsigned)(d0 + B - 1) << 10) |
+                   ((unsigned)(d1 + B - 1) << 14) | ((unsigned)(d2 + B - 1) << 18);
+        ++cnt;
+      }
+    }
+  }
+  int incl = cnt;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += y;
+  }
+  if (lane == 63) s_wsum[w] = incl;
+  __syncthreads();
+  int base = 0, npairs = 0;
+#pragma unroll
+  for (int v = 0; v < NW; ++v) {
+    base += v < w ? s_wsum[v] : 0;
+    npairs += s_wsum[v];
+  }
+  int pos = base + incl - cnt;
+#pragma unroll
+  for (int c = 0; c < CPT; ++c)
+    if (codes[c] != 0xffffffffu) s_pairs[pos++] = codes[c];
+
+  const bool row = lane < BB;
+  const int r0 = lane / B, r1 = lane - (lane / B) * B;
+  float *y2 = s_y[w][0], *y1 = s_y[w][1];
+  for (int q0 = 0; q0 < planes; q0 += 4) {
+    for (int x = tid; x < n * P; x += NT) {
+      const int i = x / P, p = x - i * P;
+      const int p0 = p / BB, p1 = (p / B) % B, p2 = p % B;
+      s_buf[x] = a.var[((int64_t)(s_org[i][0] + p0) * a.n1 + (s_org[i][1] + p1)) * a.n2 +
+                       (s_org[i][2] + p2)];
+    }
+    for (int x = tid; x < n * 4; x += NT) {
+      const int i = x >> 2, q = x & 3;
+      s_gq[i][q] = q0 + q < planes ? gf[(q0 + q) * n + i] : 0.0f;
+    }
+    __syncthreads();
+    v4f_t acc[B];
+#pragma unroll
+    for (int t = 0; t < B; ++t) acc[t] = v4f_t{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int pr = w; pr < npairs; pr += NW) {
+      const unsigned code = (unsigned)__builtin_amdgcn_readfirstlane((int)s_pairs[pr]);
+      const int i = code & 31u, j = (code >> 5) & 31u;
+      const int e0 = (code >> 10) & 15u, e1 = (code >> 14) & 15u, e2 = (code >> 18) & 15u;
+      // the tables are read-only for the whole launch: constant address
+      // space, so the uniform reads go through the scalar cache
+      const cfloat_t *h0 = (const cfloat_t *)(a.hpair + e0 * BB);
+      const cfloat_t *h1 = (const cfloat_t *)(a.hpair + (E + e1) * BB);
+      const cfloat_t *h2 = (const cfloat_t *)(a.hpair + (2 * E + e2) * BB);
+      if (row) {
+        float v[B];
+#pragma unroll
+        for (int p = 0; p < B; ++p) v[p] = s_buf[i * P + lane * B + p];
+#pragma unroll
+        for (int t = 0; t < B; ++t) {
+          float y = 0.0f;
+#pragma unroll
+          for (int p = 0; p < B; ++p) y = fmaf(h2[t * B + p], v[p], y);
+          y2[(r0 * B + t) * B + r1] = y;  // Y2[p0][p1][t2] at [p0][t2][p1]
+        }
+      }
+      wave_sync();
+      if (row) {
+        float x[B];
+#pragma unroll
+        for (int p = 0; p < B; ++p) x[p] = y2[lane * B + p];
+#pragma unroll
+        for (int t = 0; t < B; ++t) {
+          float y = 0.0f;
+#pragma unroll
+          for (int p = 0; p < B; ++p) y = fmaf(h1[t * B + p], x[p], y);
+          y1[(t * B + r1) * B + r0] = y;  // Y1[p0][t1][t2] at [t1][t2][p0]
+        }
+      }
+      wave_sync();
+      float kv[B];
+      {
+        float x[B];
+#pragma unroll
+        for (int p = 0; p < B; ++p) x[p] = row ? y1[lane * B + p] : 0.0f;
+#pragma unroll
+        for (int t = 0; t < B; ++t) {
+          float y = 0.0f;
+#pragma unroll
+          for (int p = 0; p < B; ++p) y = fmaf(h0[t * B + p], x[p], y);
+          kv[t] = y;
+        }
+      }
+      wave_sync();
+      const float c = s_gq[i][lane & 3] * s_gq[j][lane & 3] * (i == j ? 1.0f : 2.0f);
+#pragma unroll
+      for (int t = 0; t < B; ++t)
+        acc[t] = __builtin_amdgcn_mfma_f32_4x4x1f32(kv[t], c, acc[t], 0, 0, 0);
+    }
+    // the waves' sums, [w][q][t0][s12], added in wave order
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < B; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (4 * (lane >> 2) + r < BB)
+          s_buf[((w * 4 + (lane & 3)) * B + t) * BB + 4 * (lane >> 2) + r] = acc[t][r];
+    __syncthreads();
+    for (int x = tid; x < 4 * P; x += NT) {
+      const int q = x / P, s = x - q * P;
+      if (q0 + q < planes) {
+        float v = 0.0f;
+#pragma unroll
+        for (int u = 0; u < NW; ++u) v += s_buf[(u * 4 + q) * P + s];
         a.exact_std[(g * a.kcap + q0 + q) * P + s] = sqrtf(v > 0.0f ? v : 0.0f);
       }
     }
@@ -4190,11 +4372,17 @@ static int64_t matrix_table_f64_offset(int64_t m, const int32_t *block_h) {
   return (n + 1) & ~(int64_t)1;
 }
 
-int64_t bmnd_hip_matrix_table_size(int32_t max_matches, const int32_t *block_h) {
-  const int64_t m = max_matches;
+// the float32 pair tables follow the float64 part
+static int64_t matrix_table_pair_offset(int64_t m, const int32_t *block_h) {
   const int64_t P = (int64_t)block_h[0] * block_h[1] * block_h[2];
   const int64_t doubles = P + m * (m + 1) + block_h[0] + block_h[1] + block_h[2];
   return matrix_table_f64_offset(m, block_h) + 2 * doubles;
+}
+
+int64_t bmnd_hip_matrix_table_size(int32_t max_matches, const int32_t *block_h) {
+  int64_t pairs = 0;
+  for (int d = 0; d < 3; ++d) pairs += (2 * (int64_t)block_h[d] - 1) * block_h[d] * block_h[d];
+  return matrix_table_pair_offset(max_matches, block_h) + pairs;
 }
 
 static int32_t exact_std_impl(const float *var, const int64_t *shape_h, const int32_t *block_h,
@@ -4248,15 +4436,23 @@ static int32_t exact_std_impl(const float *var, const int64_t *shape_h, const in
     HIP_TRY(hipGetLastError());                                                             \
     return 0;                                                                               \
   }
+  a.hpair = matrices + matrix_table_pair_offset(max_matches, block_h);
+#define ER_CASE(b, r, nmax, nw)                                                             \
+  if (same3(block_h, b, b, b) && same3(radius_h, r, r, r) && max_matches <= nmax) {         \
+    hipLaunchKernelGGL((k_exact_pairs_rows<b, nmax, nw>), grid, dim3(64 * nw), 0, s, a);     \
+    HIP_TRY(hipGetLastError());                                                             \
+    return 0;                                                                               \
+  }
   if (pairs) {
     EP_CASE(1, 8, 8, 0, 19, 19, 32, 4)
-    EP_CASE(4, 4, 4, 7, 7, 7, 16, 4)
-    EP_CASE(5, 5, 5, 7, 7, 7, 32, 4)
+    ER_CASE(4, 7, 16, 4)
+    ER_CASE(5, 7, 32, 4)
   } else {
     E_CASE(1, 8, 8, 0, 19, 19, 32)
     E_CASE(4, 4, 4, 7, 7, 7, 16)
     E_CASE(5, 5, 5, 7, 7, 7, 32)
   }
+#undef ER_CASE
 #undef EP_CASE
 #undef E_CASE
   return fail(BMND_HIP_EINVAL, "block/radius/stack configuration not compiled into this library");

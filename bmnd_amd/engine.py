@@ -192,7 +192,19 @@ class StagePlan:
         f64 = np.concatenate([smd, gmd, mdir, sdir]).astype(np.float64)
         if table.size % 2:
             table = np.concatenate([table, np.zeros(1, np.float32)])
-        table = np.concatenate([table, f64.view(np.float32)])
+        # float32 pair tables of the pair-order exact planes: per axis and
+        # offset e, H[t][p] = A[t][p] A[t][p - e + b - 1] (0 off the block)
+        pair_tabs = []
+        for f, b in zip(fwd, self.block3):
+            A = f.reshape(b, b).astype(np.float32)
+            h = np.zeros((2 * b - 1, b, b), np.float32)
+            for e in range(2 * b - 1):
+                for p in range(b):
+                    q = p - (e - (b - 1))
+                    if 0 <= q < b:
+                        h[e, :, p] = A[:, p] * A[:, q]
+            pair_tabs.append(h.reshape(-1))
+        table = np.concatenate([table, f64.view(np.float32)] + pair_tabs)
         blk = (ctypes.c_int32 * 3)(*self.block3)
         assert table.size == _lib.lib().bmnd_hip_matrix_table_size(self.max_m, blk)
         return table

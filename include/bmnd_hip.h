@@ -103,7 +103,11 @@ int32_t bmnd_hip_log2_check(double *max_ratio);
  *   product of the synthesis matrices, w the float32 Kaiser window),
  *   the group metric diagonals diag(Ginv_n^T Ginv_n) for n = 1..max (n each),
  *   the constant-synthesis directions solve(Ginv_n, 1) for n = 1..max,
- *   then solve(S_d, 1) of each spatial axis d.
+ *   then solve(S_d, 1) of each spatial axis d,
+ * then float32 pair tables for bmnd_hip_exact_std_pairs: for each axis
+ *   d = 0,1,2 and offset e = 0..2 b_d - 2, the b_d x b_d matrix
+ *   H[t][p] = f32(A_d[t][p] * A_d[t][p - e + b_d - 1]), 0 where that column
+ *   leaves 0..b_d-1 (A_d the analysis matrix of axis d).
  * bmnd_hip_matrix_table_size() gives its length in floats. */
 int64_t bmnd_hip_matrix_table_size(int32_t max_matches, const int32_t *block_h);
 

@@ -187,10 +187,28 @@ def test_product_metric_tables_equal_the_oracle(kind):
         sf = sum(b * b for b in plan.block3)
         off = 2 * gtot + 2 * sf
         off += off % 2
-        f64 = table[off:].view(np.float64)
+        n64 = P + m * (m + 1) + sum(plan.block3)
+        f64 = table[off:off + 2 * n64].view(np.float64)
         smd, gmd = oracle.synthesis_metric_tables(p, stage)
         assert np.array_equal(f64[:P], smd)
         assert np.array_equal(f64[P:P + gmd.size], gmd)
+        # then the pair tables of the pair-order exact planes, per axis and
+        # offset e: H[t][p] = A[t][p] A[t][p - e + b - 1], 0 off the block
+        o = off + 2 * n64
+        fwd = table[2 * gtot:2 * gtot + sf]
+        fo = 0
+        for b in plan.block3:
+            A = fwd[fo:fo + b * b].reshape(b, b)
+            fo += b * b
+            h = table[o:o + (2 * b - 1) * b * b].reshape(2 * b - 1, b, b)
+            o += (2 * b - 1) * b * b
+            for e in range(2 * b - 1):
+                d = e - (b - 1)
+                for t in range(b):
+                    for q in range(b):
+                        want = A[t, q] * A[t, q - d] if 0 <= q - d < b else np.float32(0)
+                        assert h[e, t, q] == want
+        assert o == table.size
 
 
 def test_filter_codes_and_option_checks():
