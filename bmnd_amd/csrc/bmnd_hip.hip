@@ -2197,13 +2197,24 @@ __global__ __launch_bounds__(64 * NW) void k_exact_pairs(ExactArgs a) {
   float *y2 = s_y2[w], *y1 = s_y1[w];
   for (int q0 = 0; q0 < planes; q0 += 4) {
     // the members' variance patches, [i][p0][p1][p2] (the buffer holds the
-    // wave sums at the end of every four planes)
-    for (int x = tid; x < n * P; x += NT) {
-      const int i = x / P, p = x - i * P;
-      const int p0 = p / B12, p1 = (p / B2) % B1, p2 = p % B2;
-      s_buf[x] = a.var[((int64_t)(s_org[i][0] + p0) * a.n1 + (s_org[i][1] + p1)) * a.n2 +
+    // wave sums at the end of every four planes), every load in flight
+    // before the first store
+    constexpr int IT = (NMAX * P + NT - 1) / NT;
+    float vv[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int x = tid + it * NT;
+      vv[it] = 0.0f;
+      if (x < n * P) {
+        const int i = x / P, p = x - i * P;
+        const int p0 = p / B12, p1 = (p / B2) % B1, p2 = p % B2;
+        vv[it] = a.var[((int64_t)(s_org[i][0] + p0) * a.n1 + (s_org[i][1] + p1)) * a.n2 +
                        (s_org[i][2] + p2)];
+      }
     }
+#pragma unroll
+    for (int it = 0; it < IT; ++it)
+      if (tid + it * NT < n * P) s_buf[tid + it * NT] = vv[it];
     for (int x = tid; x < n * 4; x += NT) {
       const int i = x >> 2, q = x & 3;
       s_gq[i][q] = q0 + q < planes ? gf[(q0 + q) * n + i] : 0.0f;
@@ -2373,12 +2384,30 @@ __global__ __launch_bounds__(64 * NW) void k_exact_pairs_rows(ExactArgs a) {
   const int r0 = lane / B, r1 = lane - (lane / B) * B;
   float *y2 = s_y[w][0], *y1 = s_y[w][1];
   for (int q0 = 0; q0 < planes; q0 += 4) {
-    for (int x = tid; x < n * P; x += NT) {
-      const int i = x / P, p = x - i * P;
-      const int p0 = p / BB, p1 = (p / B) % B, p2 = p % B;
-      s_buf[x] = a.var[((int64_t)(s_org[i][0] + p0) * a.n1 + (s_org[i][1] + p1)) * a.n2 +
-                       (s_org[i][2] + p2)];
+    // one patch row of B values per thread and round, every load of the
+    // staging in flight before the first store
+    constexpr int IT = (NMAX * BB + NT - 1) / NT;
+    float vv[IT][B];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int r = tid + it * NT;
+#pragma unroll
+      for (int p = 0; p < B; ++p) vv[it][p] = 0.0f;
+      if (r < n * BB) {
+        const int i = r / BB, rr = r - i * BB;
+        const int p0 = rr / B, p1 = rr - p0 * B;
+        const float *src = a.var + ((int64_t)(s_org[i][0] + p0) * a.n1 + (s_org[i][1] + p1)) *
+                                       a.n2 + s_org[i][2];
+#pragma unroll
+        for (int p = 0; p < B; ++p) vv[it][p] = src[p];
+      }
     }
+#pragma unroll
+    for (int it = 0; it < IT; ++it)
+      if (tid + it * NT < n * BB) {
+#pragma unroll
+        for (int p = 0; p < B; ++p) s_buf[(tid + it * NT) * B + p] = vv[it][p];
+      }
     for (int x = tid; x < n * 4; x += NT) {
       const int i = x >> 2, q = x & 3;
       s_gq[i][q] = q0 + q < planes ? gf[(q0 + q) * n + i] : 0.0f;
@@ -2387,8 +2416,11 @@ __global__ __launch_bounds__(64 * NW) void k_exact_pairs_rows(ExactArgs a) {
     v4f_t acc[B];
 #pragma unroll
     for (int t = 0; t < B; ++t) acc[t] = v4f_t{0.0f, 0.0f, 0.0f, 0.0f};
+    // the next pair's code is read while the current one runs
+    unsigned next = w < npairs ? s_pairs[w] : 0u;
     for (int pr = w; pr < npairs; pr += NW) {
-      const unsigned code = (unsigned)__builtin_amdgcn_readfirstlane((int)s_pairs[pr]);
+      const unsigned code = (unsigned)__builtin_amdgcn_readfirstlane((int)next);
+      next = pr + NW < npairs ? s_pairs[pr + NW] : 0u;
       const int i = code & 31u, j = (code >> 5) & 31u;
       const int e0 = (code >> 10) & 15u, e1 = (code >> 14) & 15u, e2 = (code >> 18) & 15u;
       // the tables are read-only for the whole launch: constant address

@@ -350,7 +350,12 @@ def exact_std_work(plan: StagePlan, shape3: tuple[int, int, int], sel: torch.Ten
     overlap-aware variance sum_v var_v (sum_(i,p) G[q,i] S[s,p])^2 costs
     q*P*(2*n*P + 3*U) floating-point operations (a multiply and an add per
     entry and coefficient, two multiplies and an add per union voxel);
-    `flops_mac` counts the multiply-adds alone, 2*q*P*n*P per group."""
+    `flops_mac` counts the multiply-adds alone, 2*q*P*n*P per group.
+    `pairs` counts the member pairs i <= j whose patches overlap, the work
+    items of the pair order: each costs three separable passes over whole
+    rows, 2 * P * (b0 + b1 + b2) flops on the vector units (`flops_pairs`),
+    and 2 * 4 * P flops of plane products on the matrix cores
+    (`flops_pairs_mfma`, four planes per product)."""
     b0, b1, b2 = plan.block3
     P = b0 * b1 * b2
     n0, n1, n2 = shape3
@@ -366,6 +371,8 @@ def exact_std_work(plan: StagePlan, shape3: tuple[int, int, int], sel: torch.Ten
     union = 0
     flops = 0
     flops_pairs = 0
+    npairs = 0
+    blk = torch.tensor(plan.block3, device=dev)
     for a in range(0, sel2.shape[0], chunk):
         s = sel2[a:a + chunk]
         n = cnt[a:a + chunk]
@@ -383,8 +390,17 @@ def exact_std_work(plan: StagePlan, shape3: tuple[int, int, int], sel: torch.Ten
         union += int(u.sum())
         flops += int((q * P * (2 * n * P + 3 * u)).sum())
         flops_pairs += int((2 * q * P * n * P).sum())
+        # overlapping member pairs i <= j
+        pos = torch.stack([x0, x1, x2], dim=-1)
+        d = (pos[:, :, None, :] - pos[:, None, :, :]).abs()
+        ov = (d < blk).all(dim=-1)
+        live = torch.arange(max_m, device=dev)[None, :] < n[:, None]
+        ov = ov & live[:, :, None] & live[:, None, :]
+        npairs += int((ov.sum() + ov.diagonal(dim1=1, dim2=2).sum()) // 2)
     return {"entries": entries, "union_voxels": union, "flops": flops,
-            "flops_mac": flops_pairs}
+            "flops_mac": flops_pairs, "pairs": npairs,
+            "flops_pairs": npairs * 2 * P * (b0 + b1 + b2),
+            "flops_pairs_mfma": npairs * 2 * 4 * P}
 
 
 def aggregate_stage(plan: StagePlan, shape3, refs: torch.Tensor, R: int, filt: torch.Tensor,
