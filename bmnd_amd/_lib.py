@@ -36,6 +36,8 @@ EXPORTS = (
     "bmnd_hip_exact_std_pairs",
     "bmnd_hip_filter_groups",
     "bmnd_hip_filter_groups_sigma",
+    "bmnd_hip_exact_risk_workspace",
+    "bmnd_hip_filter_groups_exact",
     "bmnd_hip_reference_map",
     "bmnd_hip_aggregate",
     "bmnd_hip_aggregate_sigma",
@@ -168,6 +170,12 @@ def _declare(L: ctypes.CDLL) -> None:
     L.bmnd_hip_aggregate.restype = i32
     L.bmnd_hip_filter_groups_sigma.argtypes = [i32, vp, vp, vp, vp, vp, vp, vp, vp, i64, i32, vp,
                                                vp, f32, i32, f64, vp, vp, vp, vp]
+    L.bmnd_hip_filter_groups_sigma.restype = i32
+    L.bmnd_hip_exact_risk_workspace.argtypes = [i64, i32, i32, vp]
+    L.bmnd_hip_exact_risk_workspace.restype = i64
+    L.bmnd_hip_filter_groups_exact.argtypes = L.bmnd_hip_filter_groups_sigma.argtypes[:-1] + [
+        vp, i64, vp]
+    L.bmnd_hip_filter_groups_exact.restype = i32
     L.bmnd_hip_filter_groups_sigma.restype = i32
     L.bmnd_hip_aggregate_sigma.argtypes = [vp, vp, vp, vp, vp, i64, i32, vp, vp, vp, vp, vp, vp,
                                            vp, i32, f32, vp, vp]

@@ -254,8 +254,17 @@ def bmnd_batch(volumes, profile, sigma: float | None = None,
             outs[i] = wi.est.cpu().numpy()
         return outs
     if live:
-        live = [live[j] for j in table_first_order([items[i][0] for i in live])]
         device = torch.device("cuda", torch.cuda.current_device())
+        probe = _pipeline(profile, items[live[0]][0].shape, device)
+        if any(probe.mass.values()) or any(probe.exact_risk.values()):
+            # group-mass conservation and the exact group risk run input by input
+            for i in live:
+                vol = np.ascontiguousarray(items[i][0])
+                pipe = _pipeline(profile, vol.shape, device)
+                _, wi = pipe.run(torch.from_numpy(vol).to(device))
+                outs[i] = wi.est.cpu().numpy()
+            return outs
+        live = [live[j] for j in table_first_order([items[i][0] for i in live])]
         pipe = BatchPipeline(profile, [items[i][0].shape for i in live], device)
         _, wi = pipe.run(pipe.pack([items[i][0] for i in live]))
         for j, res in enumerate(pipe.unpack(wi.est)):

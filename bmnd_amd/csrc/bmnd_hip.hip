@@ -1791,6 +1791,13 @@ struct FilterArgs {
   // variances; the Gaussian hard threshold and Wiener gain
   int gaussian;
   const float *gpv;
+  // the exact covariance-aware group risk (filter code bit 9): the filter
+  // writes the gains of the first kcap planes and the risk of the remaining
+  // planes and the bias; k_exact_risk adds the exact planes' risk and writes
+  // the weights
+  int exact_risk;
+  float *egain;    // (nrefs, kcap, P)
+  double *erisk;   // (nrefs)
 };
 
 __device__ __forceinline__ void bind_input(FilterArgs &a, int64_t g) {
@@ -2781,6 +2788,7 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
     a.group_scope = 0;
     a.coef_out = 0;
     a.gaussian = 0;
+    a.exact_risk = 0;
   }
   constexpr int P = B0 * B1 * B2;
   constexpr int NP = NMAX * P;
@@ -3105,13 +3113,28 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
       if (a.group_scope) err = err * a.gmd[n * (n - 1) / 2 + i];
       err = err * a.smd[c];
     }
+    if (a.exact_risk) {
+      // the planes past the exact ones, diagonally (wiener.py:778-787), and
+      // the bias of the risk model (:788-795), with the float32 coefficient
+      // metric diagonal kron(diag(Ginv^T Ginv), diag(SM))
+      const float cm = (float)a.gmd[n * (n - 1) / 2 + i] * (float)a.smd[c];
+      double e = 0.0;
+      if (i >= planes) e = (double)((cm * (at * at)) * nv);
+      if (risk_model) {
+        const float r = 1.0f - at;
+        e = e + (double)(cm * ((r * r) * sp));
+      }
+      return e;
+    }
     return err;
   };
   if (a.group_scope) {
     static_assert(KEX * P >= 4 * ((NP + 63) / 64), "float64 summation slots do not fit");
     const double risk = np_block_sum_fn<NT, double>(coef_error, a.pw_plans + a.pw_plans[n],
                                                     reinterpret_cast<double *>(s_ex));
-    if (tid < n) {
+    if (a.exact_risk) {
+      if (tid == 0) a.erisk[g] = risk;
+    } else if (tid < n) {
       float w = 0.0f;
       if (isfinite(risk) && risk >= 0.0) w = (float)(1.0 / (risk + 1e-10));
       a.weights[g * a.max_m + tid] = w;
@@ -3169,6 +3192,12 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
     }
     __syncthreads();  // the noisy patches overwrite the std buffer next
   }
+  if (a.exact_risk) {
+    // the exact planes' gains for k_exact_risk
+    __syncthreads();
+    for (int e = tid; e < min(planes, a.kcap) * P; e += NT)
+      a.egain[(g * a.kcap) * P + e] = atten[e];
+  }
   if constexpr (WIENER) {
     // noisy patches into the std buffer (its last reads were the risks,
     // before the barrier above), transformed with the gains applied
@@ -3200,6 +3229,240 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
     dst[e] = v;
     return v;
   });
+}
+
+// ---------------------------------------------------------------------------
+// exact covariance-aware group risk (wiener.py:638-844, variance.py:174-450)
+// ---------------------------------------------------------------------------
+// The windowed-synthesis, group-scope risk of the variance (and Wiener risk)
+// weight model with k > 0 exact planes.  For the orthonormal member
+// transforms the group metric Ginv^T Ginv is diagonal, gm[q] on its
+// diagonal, and the exact planes' part is
+//   sum_{q < kq} gm[q] sum_{s,s'} g_q[s] g_q[s'] SM[s][s'] V_q[s][s'],
+// V_q the exact covariance of plane q's coefficients: V_q = S R_q S^T with
+//   R_q[p][p'] = sum over the member pairs i <= j with x_j - x_i = p - p' of
+//                c_q(i,j) var(x_i + p),  c_q = G[q,i] G[q,j] (x2 for i < j),
+// (the union-voxel sum of var_v (sum_i G[q,i] e_{v - x_i})^2 regrouped by
+// pairs; a pair i < j stands for both orders, which the symmetric
+// SM-and-gain form of the trace allows).  One workgroup per group: the
+// members' variance patches and the pair list sorted by offset are staged in
+// LDS; per plane R_q is gathered entry by entry (each entry sums the pairs of
+// its offset, in pair order), S is applied to its rows and then its columns
+// by separable passes in place, and the trace is reduced in float64 in a
+// fixed order.  The rest of the risk (the later planes diagonally and the
+// bias) comes from k_filter (filter code bit 9).
+struct RiskArgs {
+  const float *var;
+  int n0, n1, n2;
+  int c0, c1, c2;
+  const int32_t *sel, *counts;
+  int64_t nrefs;
+  int max_m;
+  const float *mats;   // matrix table: G_n, the analysis matrices
+  const double *gmd;   // diag(Ginv_n^T Ginv_n), n values at n(n-1)/2
+  const float *sm;     // SM [P][P]
+  int k, kcap;
+  const float *egain;  // (nrefs, kcap, P)
+  const double *erisk; // (nrefs)
+  float *weights;      // (nrefs, max_m)
+  const bmnd_hip_input *batch;
+  const int32_t *ref_input;
+};
+
+__device__ __forceinline__ void bind_input(RiskArgs &a, int64_t g) {
+  if (!a.batch) return;
+  const bmnd_hip_input &d = a.batch[a.ref_input[g]];
+  a.var += d.vox_off;
+  a.n0 = d.shape[0];
+  a.n1 = d.shape[1];
+  a.n2 = d.shape[2];
+  a.c0 = d.grid[0];
+  a.c1 = d.grid[1];
+  a.c2 = d.grid[2];
+}
+
+// In-place separable application of the spatial analysis matrices to P
+// vectors of P values held in s_x: element u of vector v at
+// s_x[v * vs + u * us] (rows: vs = P, us = 1; columns: vs = 1, us = P); along
+// axis d every line of B_d values is read, multiplied by A_d and written
+// back.  One barrier after each axis.
+template <int NT, int B0, int B1, int B2>
+__device__ __forceinline__ void apply_spatial(float *s_x, int vs, int us, const float *A0,
+                                              const float *A1, const float *A2) {
+  constexpr int P = B0 * B1 * B2;
+  const int tid = threadIdx.x;
+  auto axis = [&](auto bd, int lines_per_vec, int step, const float *A, auto line_base) {
+    constexpr int BD = decltype(bd)::value;
+    if constexpr (BD > 1) {
+      for (int it = tid; it < P * lines_per_vec; it += NT) {
+        // vectors fastest, so neighbouring threads take neighbouring vectors
+        const int v = it % P, l = it / P;
+        float *base = s_x + v * vs + line_base(l) * us;
+        float x[BD];
+#pragma unroll
+        for (int u = 0; u < BD; ++u) x[u] = base[u * step * us];
+#pragma unroll
+        for (int t = 0; t < BD; ++t) {
+          float y = 0.0f;
+#pragma unroll
+          for (int u = 0; u < BD; ++u) y = fmaf(A[t * BD + u], x[u], y);
+          base[t * step * us] = y;
+        }
+      }
+      __syncthreads();
+    }
+  };
+  axis(std::integral_constant<int, B2>{}, B0 * B1, 1, A2,
+       [&](int l) { return l * B2; });
+  axis(std::integral_constant<int, B1>{}, B0 * B2, B2, A1,
+       [&](int l) { return (l / B2) * B1 * B2 + (l % B2); });
+  axis(std::integral_constant<int, B0>{}, B1 * B2, B1 * B2, A0, [&](int l) { return l; });
+}
+
+template <int B0, int B1, int B2, int NMAX, int NT>
+__global__ __launch_bounds__(NT) void k_exact_risk(RiskArgs a) {
+  constexpr int P = B0 * B1 * B2;
+  constexpr int E0 = 2 * B0 - 1, E1 = 2 * B1 - 1, E2 = 2 * B2 - 1;
+  constexpr int NSLOT = E0 * E1 * E2;
+  constexpr int NPAIR = NMAX * (NMAX + 1) / 2;
+  static_assert(NMAX <= 32 && NT % 64 == 0, "pair codes and waves");
+  __shared__ float s_v[NMAX * P];
+  __shared__ float s_x[P * P];
+  __shared__ float s_gq[NMAX];
+  __shared__ float s_gain[P];
+  __shared__ float s_a[B0 * B0 + B1 * B1 + B2 * B2];
+  __shared__ int s_org[NMAX][3];
+  __shared__ unsigned s_list[NPAIR], s_sorted[NPAIR];
+  __shared__ int s_slot[NSLOT + 1];
+  __shared__ int s_npairs;
+  __shared__ int s_wcnt[NT / 64];
+  __shared__ double s_red[NT / 64];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6;
+  const int64_t g = blockIdx.x;
+  if (g >= a.nrefs) return;
+  bind_input(a, g);
+  const int n = a.counts[g];
+  if (n <= 0) return;
+  const int planes = min(min(a.k, n), a.kcap);
+  const int64_t gtot = (int64_t)a.max_m * (a.max_m + 1) * (2 * a.max_m + 1) / 6;
+  const float *gf = a.mats + (int64_t)(n - 1) * n * (2 * n - 1) / 6;
+  const float *A0g = a.mats + 2 * gtot;
+  for (int x = tid; x < B0 * B0 + B1 * B1 + B2 * B2; x += NT) s_a[x] = A0g[x];
+  if (tid < n) {
+    const int lin = a.sel[g * a.max_m + tid];
+    s_org[tid][2] = lin % a.c2;
+    s_org[tid][1] = (lin / a.c2) % a.c1;
+    s_org[tid][0] = lin / (a.c2 * a.c1);
+  }
+  for (int x = tid; x <= NSLOT; x += NT) s_slot[x] = 0;
+  if (tid == 0) s_npairs = 0;
+  __syncthreads();
+  // the members' variance patches, [i][p]
+  for (int x = tid; x < n * P; x += NT) {
+    const int i = x / P, p = x - i * P;
+    const int p0 = p / (B1 * B2), p1 = (p / B2) % B1, p2 = p % B2;
+    s_v[x] = a.var[((int64_t)(s_org[i][0] + p0) * a.n1 + (s_org[i][1] + p1)) * a.n2 +
+                   (s_org[i][2] + p2)];
+  }
+  // overlapping pairs i <= j in (i, j) order: one candidate per thread and
+  // round, compacted by ballot; then each pair's slot (offset) and its rank
+  // among the earlier pairs of that slot give its place in the sorted list
+  for (int c0 = 0; c0 < NMAX * NMAX; c0 += NT) {
+    const int cand = c0 + tid;
+    const int i = cand / NMAX, j = cand % NMAX;
+    bool keep = false;
+    unsigned code = 0u;
+    if (cand < NMAX * NMAX && i <= j && j < n) {
+      const int d0 = s_org[j][0] - s_org[i][0], d1 = s_org[j][1] - s_org[i][1],
+                d2 = s_org[j][2] - s_org[i][2];
+      if (abs(d0) < B0 && abs(d1) < B1 && abs(d2) < B2) {
+        keep = true;
+        const int slot = ((d0 + B0 - 1) * E1 + (d1 + B1 - 1)) * E2 + (d2 + B2 - 1);
+        code = (unsigned)i | ((unsigned)j << 5) | ((unsigned)slot << 10);
+      }
+    }
+    const unsigned long long bal = __ballot(keep);
+    if (lane == 0) s_wcnt[w] = __popcll(bal);
+    __syncthreads();
+    int base = s_npairs;
+    for (int v = 0; v < w; ++v) base += s_wcnt[v];
+    if (keep) s_list[base + __popcll(bal & ((1ull << lane) - 1ull))] = code;
+    __syncthreads();
+    if (tid == 0) {
+      int t = s_npairs;
+      for (int v = 0; v < NT / 64; ++v) t += s_wcnt[v];
+      s_npairs = t;
+    }
+    __syncthreads();
+  }
+  const int npairs = s_npairs;
+  for (int x = tid; x < npairs; x += NT) atomicAdd(&s_slot[(s_list[x] >> 10) + 1], 1);
+  __syncthreads();
+  if (tid == 0)
+    for (int x = 0; x < NSLOT; ++x) s_slot[x + 1] += s_slot[x];
+  __syncthreads();
+  for (int x = tid; x < npairs; x += NT) {
+    const unsigned c = s_list[x];
+    int rank = 0;
+    for (int y = 0; y < x; ++y) rank += (s_list[y] >> 10) == (c >> 10) ? 1 : 0;
+    s_sorted[s_slot[c >> 10] + rank] = c;
+  }
+  __syncthreads();
+
+  double risk = 0.0;  // thread 0's running total
+  for (int q = 0; q < planes; ++q) {
+    if (tid < n) s_gq[tid] = gf[q * n + tid];
+    for (int x = tid; x < P; x += NT) s_gain[x] = a.egain[(g * a.kcap + q) * P + x];
+    __syncthreads();
+    // R_q, gathered entry by entry
+    for (int x = tid; x < P * P; x += NT) {
+      const int p = x / P, pp = x - p * P;
+      const int d0 = p / (B1 * B2) - pp / (B1 * B2);
+      const int d1 = (p / B2) % B1 - (pp / B2) % B1;
+      const int d2 = p % B2 - pp % B2;
+      const int slot = ((d0 + B0 - 1) * E1 + (d1 + B1 - 1)) * E2 + (d2 + B2 - 1);
+      float acc = 0.0f;
+      for (int m = s_slot[slot]; m < s_slot[slot + 1]; ++m) {
+        const unsigned c = s_sorted[m];
+        const int i = c & 31u, j = (c >> 5) & 31u;
+        const float cq = s_gq[i] * s_gq[j] * (i == j ? 1.0f : 2.0f);
+        acc = fmaf(cq, s_v[i * P + p], acc);
+      }
+      s_x[x] = acc;
+    }
+    __syncthreads();
+    // W = R S^T (rows), then V = S W (columns)
+    apply_spatial<NT, B0, B1, B2>(s_x, P, 1, s_a, s_a + B0 * B0, s_a + B0 * B0 + B1 * B1);
+    apply_spatial<NT, B0, B1, B2>(s_x, 1, P, s_a, s_a + B0 * B0, s_a + B0 * B0 + B1 * B1);
+    // sum_{s,s'} g[s] g[s'] SM[s][s'] V[s][s'], float64 per thread, then the
+    // waves in order
+    double part = 0.0;
+    for (int x = tid; x < P * P; x += NT) {
+      const int s = x / P, t = x - s * P;
+      const float f = (s_gain[s] * s_gain[t]) * a.sm[x];
+      part += (double)f * (double)s_x[x];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o, 64);
+    if (lane == 0) s_red[w] = part;
+    __syncthreads();
+    if (tid == 0) {
+      double t = 0.0;
+      for (int v = 0; v < NT / 64; ++v) t += s_red[v];
+      risk += t * (double)(float)a.gmd[n * (n - 1) / 2 + q];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const double total = risk + a.erisk[g];
+    float wgt = 0.0f;
+    if (isfinite(total) && total >= 0.0) wgt = (float)(1.0 / (total + 1e-10));
+    s_red[0] = (double)wgt;
+  }
+  __syncthreads();
+  if (tid < n) a.weights[g * a.max_m + tid] = (float)s_red[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -4411,10 +4674,16 @@ static int64_t matrix_table_pair_offset(int64_t m, const int32_t *block_h) {
   return matrix_table_f64_offset(m, block_h) + 2 * doubles;
 }
 
-int64_t bmnd_hip_matrix_table_size(int32_t max_matches, const int32_t *block_h) {
+// then the float32 spatial metric SM [P][P] of the exact group risk
+static int64_t matrix_table_sm_offset(int64_t m, const int32_t *block_h) {
   int64_t pairs = 0;
   for (int d = 0; d < 3; ++d) pairs += (2 * (int64_t)block_h[d] - 1) * block_h[d] * block_h[d];
-  return matrix_table_pair_offset(max_matches, block_h) + pairs;
+  return matrix_table_pair_offset(m, block_h) + pairs;
+}
+
+int64_t bmnd_hip_matrix_table_size(int32_t max_matches, const int32_t *block_h) {
+  const int64_t P = (int64_t)block_h[0] * block_h[1] * block_h[2];
+  return matrix_table_sm_offset(max_matches, block_h) + P * P;
 }
 
 static int32_t exact_std_impl(const float *var, const int64_t *shape_h, const int32_t *block_h,
@@ -4551,7 +4820,8 @@ static int32_t filter_impl(int32_t wiener, const float *noisy, const float *pilo
                            int64_t nrefs, int32_t max_matches, const float *matrices,
                            const float *exact_std, float shrink, int32_t k, double count_scale,
                            float *filt, float *weights, float *sigma_out, void *stream,
-                           const BatchRef *b) {
+                           const BatchRef *b, float *egain = nullptr,
+                           double *erisk = nullptr) {
   int c[3];
   int64_t shape1[3];
   if (b) {
@@ -4565,7 +4835,7 @@ static int32_t filter_impl(int32_t wiener, const float *noisy, const float *pilo
     return rc;
   }
   if (k < 0) return fail(BMND_HIP_EINVAL, "k must be non-negative");
-  if (wiener < 0 || (wiener & ~0x1ff))
+  if (wiener < 0 || (wiener & ~0x3ff))
     return fail(BMND_HIP_EINVAL, "unknown bits in the filter code");
   const int code = wiener;
   wiener = code & 3;
@@ -4612,6 +4882,17 @@ static int32_t filter_impl(int32_t wiener, const float *noisy, const float *pilo
   a.coef_out = (code & 128) != 0;
   a.gaussian = (code & 256) != 0;
   a.gpv = a.gaussian ? var : nullptr;
+  a.exact_risk = (code & 512) != 0;
+  a.egain = egain;
+  a.erisk = erisk;
+  if (a.exact_risk) {
+    if (!a.windowed || !a.group_scope || a.kcap < 1 || a.gaussian ||
+        !(wmodel == 0 || (wiener && wmodel == 2)))
+      return fail(BMND_HIP_EINVAL, "the exact group risk needs the variance (or Wiener risk) "
+                                   "model, the windowed domain, group scope, k > 0 and the "
+                                   "Poisson model");
+    if (!egain || !erisk) return fail(BMND_HIP_EINVAL, "the exact group risk needs its workspace");
+  }
   if (a.gaussian && !var)
     return fail(BMND_HIP_EINVAL, "the Gaussian model needs the coefficient variances");
   {
@@ -4667,6 +4948,80 @@ int32_t bmnd_hip_filter_groups_sigma(int32_t wiener, const float *noisy, const f
   return filter_impl(wiener, noisy, pilot, var, shape_h, block_h, radius_h, sel, counts, nrefs,
                      max_matches, matrices, exact_std, shrink, k, count_scale, filt, weights,
                      sigma_out, stream, nullptr);
+}
+
+static int64_t align256(int64_t bytes) { return (bytes + 255) & ~(int64_t)255; }
+
+int64_t bmnd_hip_exact_risk_workspace(int64_t nrefs, int32_t max_matches, int32_t k,
+                                      const int32_t *block_h) {
+  if (nrefs < 0 || max_matches < 1 || k < 1 || !block_h) return -1;
+  const int64_t P = (int64_t)block_h[0] * block_h[1] * block_h[2];
+  const int64_t kcap = std::min<int64_t>(k, max_matches);
+  return align256(nrefs * kcap * P * (int64_t)sizeof(float)) +
+         align256(nrefs * (int64_t)sizeof(double));
+}
+
+int32_t bmnd_hip_filter_groups_exact(int32_t code, const float *noisy, const float *pilot,
+                                     const float *var, const int64_t *shape_h,
+                                     const int32_t *block_h, const int32_t *radius_h,
+                                     const int32_t *sel, const int32_t *counts, int64_t nrefs,
+                                     int32_t max_matches, const float *matrices,
+                                     const float *exact_std, float shrink, int32_t k,
+                                     double count_scale, float *filt, float *weights,
+                                     float *sigma_out, void *workspace, int64_t workspace_bytes,
+                                     void *stream) {
+  if (nrefs <= 0) return 0;
+  const int64_t need = bmnd_hip_exact_risk_workspace(nrefs, max_matches, k, block_h);
+  if (need < 0) return fail(BMND_HIP_EINVAL, "the exact group risk needs k >= 1");
+  if (!workspace || workspace_bytes < need)
+    return fail(BMND_HIP_EINVAL, "workspace is smaller than bmnd_hip_exact_risk_workspace()");
+  const int64_t P = (int64_t)block_h[0] * block_h[1] * block_h[2];
+  const int64_t kcap = std::min<int64_t>(k, max_matches);
+  float *egain = static_cast<float *>(workspace);
+  double *erisk = reinterpret_cast<double *>(static_cast<char *>(workspace) +
+                                             align256(nrefs * kcap * P * (int64_t)sizeof(float)));
+  if (int rc = filter_impl(code | 512, noisy, pilot, var, shape_h, block_h, radius_h, sel, counts,
+                           nrefs, max_matches, matrices, exact_std, shrink, k, count_scale, filt,
+                           weights, sigma_out, stream, nullptr, egain, erisk))
+    return rc;
+  int c[3];
+  if (int rc = check_geometry(shape_h, block_h, radius_h, c)) return rc;
+  RiskArgs a;
+  a.var = var;
+  a.n0 = (int)shape_h[0];
+  a.n1 = (int)shape_h[1];
+  a.n2 = (int)shape_h[2];
+  a.c0 = c[0];
+  a.c1 = c[1];
+  a.c2 = c[2];
+  a.sel = sel;
+  a.counts = counts;
+  a.nrefs = nrefs;
+  a.max_m = max_matches;
+  a.mats = matrices;
+  a.gmd = reinterpret_cast<const double *>(matrices + matrix_table_f64_offset(max_matches,
+                                                                              block_h)) + P;
+  a.sm = matrices + matrix_table_sm_offset(max_matches, block_h);
+  a.k = k;
+  a.kcap = (int)kcap;
+  a.egain = egain;
+  a.erisk = erisk;
+  a.weights = weights;
+  a.batch = nullptr;
+  a.ref_input = nullptr;
+  hipStream_t s = (hipStream_t)stream;
+  dim3 grid((unsigned)nrefs);
+#define R_CASE(b0, b1, b2, r0, r1, r2, nmax, nt)                                             \
+  if (same3(block_h, b0, b1, b2) && same3(radius_h, r0, r1, r2) && max_matches <= nmax) {    \
+    hipLaunchKernelGGL((k_exact_risk<b0, b1, b2, nmax, nt>), grid, dim3(nt), 0, s, a);        \
+    HIP_TRY(hipGetLastError());                                                             \
+    return 0;                                                                               \
+  }
+  R_CASE(1, 8, 8, 0, 19, 19, 32, 256)
+  R_CASE(4, 4, 4, 7, 7, 7, 16, 256)
+  R_CASE(5, 5, 5, 7, 7, 7, 32, 1024)
+#undef R_CASE
+  return fail(BMND_HIP_EINVAL, "block/radius/stack configuration not compiled into this library");
 }
 
 int32_t bmnd_hip_filter_groups_batch(int32_t wiener, const float *noisy, const float *pilot,
@@ -5032,7 +5387,6 @@ int32_t bmnd_hip_aggregate_batch(const float *filt, const float *weights, const 
                         stream, &b);
 }
 
-static int64_t align256(int64_t bytes) { return (bytes + 255) & ~(int64_t)255; }
 
 int64_t bmnd_hip_stage_workspace(const int64_t *shape_h, const int32_t *block_h, int64_t nrefs,
                                  int32_t max_matches, int32_t k) {

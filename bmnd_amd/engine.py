@@ -204,7 +204,14 @@ class StagePlan:
                     if 0 <= q < b:
                         h[e, :, p] = A[:, p] * A[:, q]
             pair_tabs.append(h.reshape(-1))
-        table = np.concatenate([table, f64.view(np.float32)] + pair_tabs)
+        # the spatial metric SM = S^T diag(w^2) S of the exact group risk
+        # (variance.py:1543-1556), float32 from float64 sums
+        S = np.ones((1, 1))
+        for i in sinv:
+            S = np.kron(S, np.asarray(i, np.float64))
+        w2 = (np.asarray(self.kaiser, np.float32).reshape(-1) ** 2).astype(np.float64)
+        sm = (S.T @ (w2[:, None] * S)).astype(np.float32)
+        table = np.concatenate([table, f64.view(np.float32)] + pair_tabs + [sm.reshape(-1)])
         blk = (ctypes.c_int32 * 3)(*self.block3)
         assert table.size == _lib.lib().bmnd_hip_matrix_table_size(self.max_m, blk)
         return table
@@ -245,19 +252,28 @@ def check_supported(profile: BMNDProfile, resolved) -> None:
     def field(name, enum):
         return coerce_enum(getattr(profile, name), enum, name)
 
-    for stage in ("ht", "wiener"):
-        # the covariance-aware exact group risk (bmndalgo.py:359-377, 752-758;
-        # wiener.py:638-844) replaces the diagonal weights for these choices
-        models = ((AggregationWeightModel.VARIANCE,) if stage == "ht" else
-                  (AggregationWeightModel.VARIANCE, AggregationWeightModel.RISK))
-        if (field(f"{stage}_weight_model", AggregationWeightModel) in models
-                and field(f"{stage}_weight_domain", AggregationWeightDomain)
-                is AggregationWeightDomain.WINDOWED_SYNTHESIS
-                and field(f"{stage}_weight_scope", AggregationWeightScope)
-                is AggregationWeightScope.GROUP and int(resolved.variance_k) > 0):
-            raise NotImplementedError(
-                f"the exact covariance-aware {stage} group risk (windowed-synthesis domain, "
-                "group scope, k > 0) is not part of the MI355X path.")
+    if not resolved.is_poisson:
+        for stage in ("ht", "wiener"):
+            if exact_risk_stage(profile, stage, int(resolved.variance_k)):
+                raise NotImplementedError(
+                    f"the exact covariance-aware {stage} group risk (windowed-synthesis domain, "
+                    "group scope, k > 0) runs under the Poisson model on the MI355X path.")
+
+
+def exact_risk_stage(profile: BMNDProfile, stage: str, k: int) -> bool:
+    """Whether stage `stage` weights its groups by the exact covariance-aware
+    group risk (bmndalgo.py:359-377, 752-758; wiener.py:638-844): the
+    variance model (or, in the Wiener stage, the risk model) in the
+    windowed-synthesis domain with group scope and k > 0."""
+    def field(name, enum):
+        return coerce_enum(getattr(profile, name), enum, name)
+    models = ((AggregationWeightModel.VARIANCE,) if stage == "ht" else
+              (AggregationWeightModel.VARIANCE, AggregationWeightModel.RISK))
+    return (field(f"{stage}_weight_model", AggregationWeightModel) in models
+            and field(f"{stage}_weight_domain", AggregationWeightDomain)
+            is AggregationWeightDomain.WINDOWED_SYNTHESIS
+            and field(f"{stage}_weight_scope", AggregationWeightScope)
+            is AggregationWeightScope.GROUP and int(k) > 0)
 
 
 # the filters' weight-model codes (aggregationweights.py:19-21 order differs:
@@ -490,6 +506,8 @@ class PoissonPipeline:
                              for st in ("ht", "wiener")}
         self._null = None
         self.k = int(profile.k)
+        # stages weighted by the exact covariance-aware group risk
+        self.exact_risk = {st: exact_risk_stage(profile, st, self.k) for st in ("ht", "wiener")}
         self.plans = {s: make_plan(profile, s, self.ndim) for s in ("ht", "wiener")}
         for plan in self.plans.values():
             counts = tuple(self.shape[d] - plan.block[d] + 1 for d in range(self.ndim))
@@ -661,13 +679,19 @@ class PoissonPipeline:
             gsig = torch.empty(R if wiener else R * plan.max_m * plan.P, dtype=torch.float32,
                                device=dev)
         tok = timer.start(f"{name}.filter")
-        _lib.check(L.bmnd_hip_filter_groups_sigma(self.filter_codes[name], _ptr(noisy),
-                                                  _ptr((match_vol if pilot is None else pilot)
-                                                       if wiener else None), _ptr(var),
-                                                  shape, block, radius, _ptr(sel), _ptr(cnt), R,
-                                                  plan.max_m, _ptr(mats), _ptr(exact),
-                                                  plan.shrink, self.k, self.count_scale,
-                                                  _ptr(filt), _ptr(wts), _ptr(gsig), stream))
+        filter_args = (self.filter_codes[name], _ptr(noisy),
+                       _ptr((match_vol if pilot is None else pilot) if wiener else None),
+                       _ptr(var), shape, block, radius, _ptr(sel), _ptr(cnt), R, plan.max_m,
+                       _ptr(mats), _ptr(exact), plan.shrink, self.k, self.count_scale,
+                       _ptr(filt), _ptr(wts), _ptr(gsig))
+        if self.exact_risk[name]:
+            # the exact covariance-aware group risk replaces the weights
+            need = L.bmnd_hip_exact_risk_workspace(R, plan.max_m, self.k, block)
+            ws = torch.empty(max(need, 1), dtype=torch.uint8, device=dev)
+            _lib.check(L.bmnd_hip_filter_groups_exact(*filter_args, _ptr(ws), need, stream))
+            del ws
+        else:
+            _lib.check(L.bmnd_hip_filter_groups_sigma(*filter_args, stream))
         timer.stop(tok)
         del exact
         npatch = int(np.prod([self.shape3[d] - plan.block3[d] + 1 for d in range(3)]))
@@ -983,6 +1007,9 @@ class BatchPipeline:
         if any(self._single.mass.values()):
             raise NotImplementedError("group-mass conservation runs input by input (bmnd()); "
                                       "bmnd_batch does not take it.")
+        if any(self._single.exact_risk.values()):
+            raise NotImplementedError("the exact covariance-aware group risk runs input by input "
+                                      "(bmnd()); bmnd_batch does not take it.")
         self.stages = {name: self._stage_view(profile, name) for name in ("ht", "wiener")}
         self.last_runs: dict[str, list] = {}  # (mode, first input, end) per matching launch
 

@@ -107,7 +107,10 @@ int32_t bmnd_hip_log2_check(double *max_ratio);
  * then float32 pair tables for bmnd_hip_exact_std_pairs: for each axis
  *   d = 0,1,2 and offset e = 0..2 b_d - 2, the b_d x b_d matrix
  *   H[t][p] = f32(A_d[t][p] * A_d[t][p - e + b_d - 1]), 0 where that column
- *   leaves 0..b_d-1 (A_d the analysis matrix of axis d).
+ *   leaves 0..b_d-1 (A_d the analysis matrix of axis d),
+ * then the float32 spatial metric of the exact group risk,
+ *   SM = S^T diag(w^2) S [P][P] (S the Kronecker product of the synthesis
+ *   matrices, w the float32 Kaiser window; variance.py:1543-1556).
  * bmnd_hip_matrix_table_size() gives its length in floats. */
 int64_t bmnd_hip_matrix_table_size(int32_t max_matches, const int32_t *block_h);
 
@@ -228,6 +231,33 @@ int32_t bmnd_hip_exact_std_pairs(const float *var, const int64_t *shape_h,
                                  const int32_t *sel, const int32_t *counts, int64_t nrefs,
                                  int32_t max_matches, const float *matrices, int32_t k,
                                  float *exact_std, void *stream);
+
+/* The exact covariance-aware group risk of the windowed-synthesis,
+ * group-scope weights (bmndalgo.py:359-377, 493-505, 752-758, 913-926;
+ * wiener.py:638-844): bmnd_hip_filter_groups_sigma with the weights of the
+ * variance model (HT) or the variance / risk model (Wiener, code bits 3-4)
+ * in the windowed-synthesis domain (bit 5) with group scope (bit 6) and k >
+ * 0 replaced by w = f32(1 / (risk + 1e-10)) for every member, where risk is
+ * the trace of the windowed synthesis of the filtered group's exact noise
+ * covariance over its first min(k, n) planes (the overlap-aware covariance
+ * from var), plus the diagonal variance of the later planes and, for the
+ * risk model, the bias.  The reference builds that covariance per union
+ * voxel; here it is regrouped by overlapping member pairs and transformed
+ * separably, the same quantity to float32 rounding.  The Poisson model only.
+ * Same arguments as bmnd_hip_filter_groups_sigma (code need not carry the
+ * bit this call adds), plus a workspace of bmnd_hip_exact_risk_workspace()
+ * bytes. */
+int64_t bmnd_hip_exact_risk_workspace(int64_t nrefs, int32_t max_matches, int32_t k,
+                                      const int32_t *block_h);
+int32_t bmnd_hip_filter_groups_exact(int32_t code, const float *noisy, const float *pilot,
+                                     const float *var, const int64_t *shape_h,
+                                     const int32_t *block_h, const int32_t *radius_h,
+                                     const int32_t *sel, const int32_t *counts, int64_t nrefs,
+                                     int32_t max_matches, const float *matrices,
+                                     const float *exact_std, float shrink, int32_t k,
+                                     double count_scale, float *filt, float *weights,
+                                     float *sigma_out, void *workspace, int64_t workspace_bytes,
+                                     void *stream);
 
 /* Flags OR'd into the k argument of bmnd_hip_stage, bmnd_hip_stage_noalloc,
  * bmnd_hip_stage_workspace and into bmnd_hip_pipeline_desc.k (the exact

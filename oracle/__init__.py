@@ -153,6 +153,9 @@ def _declare(L: ctypes.CDLL) -> None:
     L.oracle_weight_risks.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, i32, i32, i32, vp]
     L.oracle_weight_risks.restype = i32
     L.oracle_weights_ex.argtypes = [vp, vp, vp, i32, i32, vp, vp, vp, i32, i32, i32, f64, vp]
+    L.oracle_exact_group_risk.argtypes = [vp, i32, vp, vp, i32, vp, vp, vp, vp, vp, vp, i32, vp,
+                                          vp, vp]
+    L.oracle_exact_group_risk.restype = f64
     L.oracle_mass_correct.argtypes = [i32, vp, vp, vp, vp, vp, vp, f64]
     L.oracle_wiener_shrink.argtypes = [vp, vp, vp, i64, f32, vp]
     L.oracle_wiener_shrink_mode.argtypes = [vp, vp, vp, i64, f32, i32, vp]
@@ -412,6 +415,30 @@ def weight_risks(atten, sigma, group_inv, *, model="variance", signal=None, smd=
     k = lib().oracle_weight_risks(_p(a), _p(s), _p(sp), n, p, _p(gi), _p(sm), _p(gm),
                                   WEIGHT_MODELS[model], int(windowed), int(group_scope), _p(out))
     return out[:k]
+
+
+def exact_group_risk(var_volume, block, abs_pos, group_fwd, group_inv, spatial_fwd, spatial_inv,
+                     window, planes, gain, sigma, spow=None) -> float:
+    """The exact covariance-aware group risk (wiener.py:638-844) of one group:
+    exact over the first `planes` planes, diagonal over the rest, plus the
+    bias when `spow` (the signal power) is given.  gain, sigma, spow: (n, P)."""
+    vv = np.ascontiguousarray(var_volume, dtype=np.float32)
+    ndim = vv.ndim
+    shp = np.asarray(vv.shape, dtype=np.int64)
+    blk = np.asarray(block, dtype=np.int32)
+    ap = np.ascontiguousarray(abs_pos, dtype=np.int64)
+    n = ap.shape[0]
+    gf = np.ascontiguousarray(group_fwd, dtype=np.float32)
+    gi = np.ascontiguousarray(group_inv, dtype=np.float32)
+    keep_f, fptrs = _spatial_ptrs(spatial_fwd)
+    keep_i, iptrs = _spatial_ptrs(spatial_inv)
+    w = np.ascontiguousarray(window, dtype=np.float32).reshape(-1)
+    g = np.ascontiguousarray(gain, dtype=np.float32).reshape(n, -1)
+    s = np.ascontiguousarray(sigma, dtype=np.float32).reshape(n, -1)
+    sp = None if spow is None else np.ascontiguousarray(spow, dtype=np.float32).reshape(n, -1)
+    return float(lib().oracle_exact_group_risk(_p(vv), ndim, _p(shp), _p(blk), n, _p(ap), _p(gf),
+                                               _p(gi), fptrs, iptrs, _p(w), int(planes), _p(g),
+                                               _p(s), _p(sp)))
 
 
 GAIN_MODES = {"variance-scaled": 0, "classic": 1, "noise-floor": 2}

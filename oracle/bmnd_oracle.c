@@ -924,6 +924,164 @@ int oracle_exact_variance(const float *var_volume, int ndim,
   return 0;
 }
 
+static double np_pairwise_f64(const double *a, int64_t n);
+
+/* Exact covariance-aware group risk of the windowed-synthesis, group-scope
+ * weights (wiener.py:638-844; the plan of variance.py:174-450, 1413-1449;
+ * the metrics of variance.py:1516-1569), for the diagonal group metric of
+ * the orthonormal Haar / DCT member transforms and no mass functional:
+ *   contributions C_x[q][s] = sum over the entries (i,p) of union voxel x,
+ *     in the exact variance's entry order, of f32(G[q,i] S[s,p])
+ *     (variance.py:1038-1077), q < planes;
+ *   F = C * gain[q][s] (float32, wiener.py:675-680);
+ *   quad[x][q] = sum_s' (double)F[s'] (double)(F @ SM)[s'] with F @ SM a
+ *     float32 product (wiener.py:505-523);
+ *   noise = sum_x var_x sum_q quad[x][q] gm[q] (float64, :526-535, 708-709);
+ *   planes q >= `planes`: + sum f64((cm[q,s] gain^2) sigma^2) (:778-787);
+ *   with the signal power S: + sum f64(cm (1 - gain)^2 S) (:788-795);
+ * where SM = Sinv^T diag(w^2) Sinv (float32; Sinv the Kronecker synthesis
+ * matrix, w the float32 window), gm = diag(Ginv^T Ginv) and cm[q,s] =
+ * gm[q] SM[s][s] (float32).  gain, sigma and spow hold n*P values; spow may
+ * be NULL (no bias term).  Returns the risk. */
+double oracle_exact_group_risk(const float *var_volume, int ndim, const int64_t *vol_shape,
+                               const int *block, int n, const int64_t *abs_pos,
+                               const float *group_mat, const float *group_inv,
+                               const float *const *spatial_mats,
+                               const float *const *spatial_inv, const float *window,
+                               int planes, const float *gain, const float *sigma,
+                               const float *spow) {
+  int P = (int)prod_int(block, ndim);
+  if (planes > n) planes = n;
+  if (planes < 0) planes = 0;
+  /* metrics (variance.py:1543-1561), float64 sums rounded to float32 */
+  float *Sinv = spatial_kron(ndim, block, spatial_inv);
+  float *SM = (float *)malloc(sizeof(float) * (size_t)P * P);
+  for (int a = 0; a < P; ++a)
+    for (int b = 0; b < P; ++b) {
+      double acc = 0.0;
+      for (int x = 0; x < P; ++x) {
+        const float w2 = window[x] * window[x];
+        acc += (double)Sinv[x * P + a] * (double)(w2 * Sinv[x * P + b]);
+      }
+      SM[a * P + b] = (float)acc;
+    }
+  float *gm = (float *)malloc(sizeof(float) * (size_t)n);
+  for (int q = 0; q < n; ++q) {
+    double acc = 0.0;
+    for (int x = 0; x < n; ++x)
+      acc += (double)group_inv[x * n + q] * (double)group_inv[x * n + q];
+    gm[q] = (float)acc;
+  }
+  double noise = 0.0;
+  if (planes > 0) {
+    int64_t mn[ORACLE_MAX_NDIM], ushape[ORACLE_MAX_NDIM], ustr[ORACLE_MAX_NDIM],
+        vstr[ORACLE_MAX_NDIM];
+    for (int d = 0; d < ndim; ++d) {
+      mn[d] = abs_pos[d];
+      for (int i = 1; i < n; ++i)
+        if (abs_pos[i * ndim + d] < mn[d]) mn[d] = abs_pos[i * ndim + d];
+    }
+    for (int d = 0; d < ndim; ++d) {
+      int64_t mx = abs_pos[d] - mn[d];
+      for (int i = 1; i < n; ++i)
+        if (abs_pos[i * ndim + d] - mn[d] > mx) mx = abs_pos[i * ndim + d] - mn[d];
+      ushape[d] = mx + block[d];
+    }
+    c_strides(ndim, ushape, ustr);
+    c_strides(ndim, vol_shape, vstr);
+    int64_t ucells = 1;
+    for (int d = 0; d < ndim; ++d) ucells *= ushape[d];
+    int entries = n * P;
+    int64_t *label = (int64_t *)malloc(sizeof(int64_t) * (size_t)entries);
+    for (int i = 0; i < n; ++i)
+      for (int p = 0; p < P; ++p) {
+        int lc[ORACLE_MAX_NDIM];
+        local_coords(ndim, block, p, lc);
+        int64_t l = 0;
+        for (int d = 0; d < ndim; ++d) l += (abs_pos[i * ndim + d] - mn[d] + lc[d]) * ustr[d];
+        label[i * P + p] = l;
+      }
+    int32_t *cnt = (int32_t *)calloc((size_t)ucells + 1, sizeof(int32_t));
+    for (int e = 0; e < entries; ++e) cnt[label[e] + 1]++;
+    for (int64_t c = 0; c < ucells; ++c) cnt[c + 1] += cnt[c];
+    int32_t *order = (int32_t *)malloc(sizeof(int32_t) * (size_t)entries);
+    int32_t *fill = (int32_t *)malloc(sizeof(int32_t) * (size_t)(ucells + 1));
+    memcpy(fill, cnt, sizeof(int32_t) * (size_t)(ucells + 1));
+    for (int e = 0; e < entries; ++e) order[fill[label[e]]++] = e;
+    if (g_exact_order == 1) oracle_numba_argsort(label, entries, order);
+    float *S = spatial_kron(ndim, block, spatial_mats);
+    float *F = (float *)malloc(sizeof(float) * (size_t)planes * P);
+    float *proj = (float *)malloc(sizeof(float) * (size_t)P);
+    for (int64_t cell = 0; cell < ucells; ++cell) {
+      int beg = cnt[cell], end = cnt[cell + 1];
+      if (beg == end) continue;
+      for (int c = 0; c < planes * P; ++c) F[c] = 0.0f;
+      for (int pos = beg; pos < end; ++pos) {
+        int e = order[pos];
+        int i = e / P, p = e % P;
+        for (int q = 0; q < planes; ++q) {
+          float g = group_mat[q * n + i];
+          for (int s = 0; s < P; ++s) F[q * P + s] += g * S[s * P + p];
+        }
+      }
+      int64_t t = cell, off = 0;
+      for (int d = ndim - 1; d >= 0; --d) {
+        int64_t u = t % ushape[d];
+        t /= ushape[d];
+        off += (u + mn[d]) * vstr[d];
+      }
+      double row = 0.0;
+      for (int q = 0; q < planes; ++q) {
+        float *f = F + q * P;
+        for (int s = 0; s < P; ++s) f[s] = f[s] * gain[q * P + s];
+        for (int b = 0; b < P; ++b) {
+          float acc = 0.0f;
+          for (int a = 0; a < P; ++a) acc += f[a] * SM[a * P + b];
+          proj[b] = acc;
+        }
+        double quad = 0.0;
+        for (int b = 0; b < P; ++b) quad += (double)f[b] * (double)proj[b];
+        row += quad * (double)gm[q];
+      }
+      noise += row * (double)var_volume[off];
+    }
+    free(label);
+    free(cnt);
+    free(order);
+    free(fill);
+    free(S);
+    free(F);
+    free(proj);
+  }
+  const int rest = (n - planes) * P;
+  double *tmp = (double *)malloc(sizeof(double) * (size_t)(n * P + 1));
+  if (planes < n) {
+    for (int e = 0; e < rest; ++e) {
+      const int q = planes + e / P, s = e % P;
+      const float cm = gm[q] * SM[s * P + s];
+      const float g = gain[planes * P + e];
+      const float nv = sigma[planes * P + e] * sigma[planes * P + e];
+      tmp[e] = (double)((cm * (g * g)) * nv);
+    }
+    noise += np_pairwise_f64(tmp, rest);
+  }
+  double bias = 0.0;
+  if (spow) {
+    for (int e = 0; e < n * P; ++e) {
+      const int q = e / P, s = e % P;
+      const float cm = gm[q] * SM[s * P + s];
+      const float r = 1.0f - gain[e];
+      tmp[e] = (double)(cm * ((r * r) * spow[e]));
+    }
+    bias = np_pairwise_f64(tmp, (int64_t)n * P);
+  }
+  free(tmp);
+  free(Sinv);
+  free(SM);
+  free(gm);
+  return noise + bias;
+}
+
 /* plane limiting, variance.py:1572-1631 (float32 sequential plane sums) */
 void oracle_limit_planes(const float *exact_std, const float *approx_std,
                          int n, int p, int k, float *out) {
@@ -1787,8 +1945,24 @@ static int run_stage(const oracle_params *prm, int wiener, const float *match_vo
         const int ws = wiener ? prm->wi_group_scope : prm->ht_group_scope;
         const double *smd = wiener ? prm->wi_smd : prm->ht_smd;
         const double *gmd = wiener ? prm->wi_gmd : prm->ht_gmd;
-        oracle_weights_ex(att, sig, spw, n, P, ginv, smd, gmd ? gmd + n * (n - 1) / 2 : NULL, wm,
-                          wd, ws, 1e-10, wts + off[r]);
+        /* the exact covariance-aware group risk replaces the diagonal one
+         * for the variance (and, in the Wiener stage, risk) model in the
+         * windowed-synthesis domain with group scope and k > 0
+         * (bmndalgo.py:359-377, 493-505, 752-758, 913-926) */
+        const int exact_risk = !prm->gaussian && prm->k > 0 && wd && ws &&
+                               (wm == 0 || (wiener && wm == 2));
+        if (exact_risk) {
+          const double risk = oracle_exact_group_risk(
+              var_vol, ndim, g.shape, block, n, apos, gfwd, ginv, spf,
+              (const float *const *)g.sp_inv, g.kaiser, prm->k < n ? prm->k : n, att, sig,
+              (wiener && wm == 2) ? spw : NULL);
+          float w = 0.0f;
+          if (isfinite(risk) && risk >= 0.0) w = (float)(1.0 / (risk + 1e-10));
+          for (int i = 0; i < n; ++i) wts[off[r] + i] = w;
+        } else {
+          oracle_weights_ex(att, sig, spw, n, P, ginv, smd, gmd ? gmd + n * (n - 1) / 2 : NULL,
+                            wm, wd, ws, 1e-10, wts + off[r]);
+        }
       }
       if (gsig) memcpy(gsig + off[r] * P, sig, sizeof(float) * (size_t)n * P);
       if (gsig1) gsig1[r] = oracle_np_mean_f32(sig, (int64_t)n * P);

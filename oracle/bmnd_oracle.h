@@ -120,6 +120,13 @@ int oracle_weight_risks(const float *atten, const float *sigma, const float *spo
 void oracle_mass_correct(int nax, const int *dims, const float *const *mats,
                          const double *const *dirs, const double *O, const float *src_vox,
                          float *coef, double eps);
+double oracle_exact_group_risk(const float *var_volume, int ndim, const int64_t *vol_shape,
+                               const int *block, int n, const int64_t *abs_pos,
+                               const float *group_mat, const float *group_inv,
+                               const float *const *spatial_mats,
+                               const float *const *spatial_inv, const float *window,
+                               int planes, const float *gain, const float *sigma,
+                               const float *spow);
 void oracle_weights_ex(const float *atten, const float *sigma, const float *spow, int n, int p,
                        const float *group_inv, const double *smd, const double *gmd, int model,
                        int windowed, int group_scope, double eps, float *w_out);

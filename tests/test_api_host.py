@@ -88,8 +88,6 @@ def test_all_zero_volume_fast_path_runs_no_kernel():
 
 
 @pytest.mark.parametrize("fields", [
-    {"wiener_weight_domain": "windowed_synthesis", "wiener_weight_scope": "group"},
-    {"ht_weight_domain": "windowed_synthesis", "ht_weight_scope": "group"},
     {"blockmatch_distance": "ssd"},
 ])
 def test_unsupported_options_raise_before_any_gpu_work(fields):
@@ -98,6 +96,25 @@ def test_unsupported_options_raise_before_any_gpu_work(fields):
         setattr(p, field, value)
     with pytest.raises(NotImplementedError):
         bmnd(np.ones((32, 32)), p)
+
+
+def test_exact_group_risk_is_selected_for_its_options():
+    # the variance (HT) or variance / risk (Wiener) model in the windowed-
+    # synthesis domain with group scope and k > 0 takes the exact
+    # covariance-aware group risk (bmndalgo.py:359-377, 752-758)
+    from bmnd_amd.engine import check_supported, exact_risk_stage
+    from bmnd_amd.profilevalidation import resolve_profile
+    p = _poisson()
+    assert not exact_risk_stage(p, "ht", 4) and not exact_risk_stage(p, "wiener", 4)
+    p.wiener_weight_domain, p.wiener_weight_scope = "windowed_synthesis", "group"
+    p.ht_weight_domain, p.ht_weight_scope = "windowed_synthesis", "group"
+    assert exact_risk_stage(p, "ht", 4) and exact_risk_stage(p, "wiener", 4)
+    assert not exact_risk_stage(p, "wiener", 0)
+    p.wiener_weight_model = "risk"
+    assert exact_risk_stage(p, "wiener", 4)
+    p.ht_weight_model = "classic"
+    assert not exact_risk_stage(p, "ht", 4)
+    check_supported(p, resolve_profile(p, (32, 32)))
 
 
 def test_gaussian_profile_support():

@@ -56,14 +56,33 @@ def test_thresholds_entry_needs_the_array():
 
 
 @pytest.mark.parametrize("code,msg", [
-    (-1, "unknown bits"), (512, "unknown bits"), (3 << 3, "weight model must be"),
+    (-1, "unknown bits"), (1024, "unknown bits"), (3 << 3, "weight model must be"),
     (2 << 3, "risk weight model is a Wiener-stage option"),
-    (1 | 4, "soft thresholding is a hard-threshold-stage option")])
+    (1 | 4, "soft thresholding is a hard-threshold-stage option"),
+    (512, "the exact group risk needs the variance"),
+    (512 | 32 | 64 | (1 << 3), "the exact group risk needs the variance"),
+    (512 | 32 | 64, "k > 0"),
+    (512 | 32 | 64 | 256, "the Poisson model")])
 def test_filter_code_argument(code, msg):
     rc = _lib.lib().bmnd_hip_filter_groups(
         code, DUMMY, DUMMY, DUMMY, SHAPE, BLOCK, RADIUS, DUMMY, DUMMY, 10, 16, DUMMY, DUMMY,
         3.0, 0, 1.0, DUMMY, DUMMY, None)
     assert msg in _error(rc)
+
+
+def test_exact_risk_entry_checks_its_workspace():
+    L = _lib.lib()
+    need = L.bmnd_hip_exact_risk_workspace(10, 16, 4, BLOCK)
+    assert need >= 10 * 4 * 64 * 4 + 10 * 8 and need % 256 == 0
+    assert L.bmnd_hip_exact_risk_workspace(10, 16, 0, BLOCK) == -1
+    rc = L.bmnd_hip_filter_groups_exact(
+        32 | 64, DUMMY, DUMMY, DUMMY, SHAPE, BLOCK, RADIUS, DUMMY, DUMMY, 10, 16, DUMMY, DUMMY,
+        3.0, 4, 1.0, DUMMY, DUMMY, None, DUMMY, need - 1, None)
+    assert "smaller than bmnd_hip_exact_risk_workspace" in _error(rc)
+    rc = L.bmnd_hip_filter_groups_exact(
+        32 | 64, DUMMY, DUMMY, DUMMY, SHAPE, BLOCK, RADIUS, DUMMY, DUMMY, 10, 16, DUMMY, DUMMY,
+        3.0, 0, 1.0, DUMMY, DUMMY, None, DUMMY, need, None)
+    assert "k >= 1" in _error(rc)
 
 
 def test_gaussian_filter_needs_the_coefficient_variances():

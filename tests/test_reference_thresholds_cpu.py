@@ -125,5 +125,4 @@ def test_policy_support():
     check_supported(prof, resolve_profile(prof, (16, 16, 16)))
     prof.wiener_weight_domain = "windowed_synthesis"
     prof.wiener_weight_scope = "group"
-    with pytest.raises(NotImplementedError, match="exact covariance-aware"):
-        check_supported(prof, resolve_profile(prof, (16, 16, 16)))
+    check_supported(prof, resolve_profile(prof, (16, 16, 16)))  # the exact group risk

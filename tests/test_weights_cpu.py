@@ -208,7 +208,18 @@ def test_product_metric_tables_equal_the_oracle(kind):
                     for q in range(b):
                         want = A[t, q] * A[t, q - d] if 0 <= q - d < b else np.float32(0)
                         assert h[e, t, q] == want
-        assert o == table.size
+        # then the spatial metric SM = Sinv^T diag(w^2) Sinv of the exact group
+        # risk (variance.py:1543-1556), from the synthesis matrices and window
+        inv = table[2 * gtot + sf:2 * gtot + 2 * sf]
+        Si, io = np.ones((1, 1)), 0
+        for b in plan.block3:
+            Si = np.kron(Si, inv[io:io + b * b].reshape(b, b).astype(np.float64))
+            io += b * b
+        w2 = (plan.kaiser.reshape(-1).astype(np.float32) ** 2).astype(np.float64)
+        sm = table[o:o + P * P].reshape(P, P)
+        assert np.allclose(sm, Si.T @ (w2[:, None] * Si), rtol=1e-6, atol=1e-7)
+        assert np.array_equal(np.diag(sm), np.diag(sm.astype(np.float64)).astype(np.float32))
+        assert o + P * P == table.size
 
 
 def test_filter_codes_and_option_checks():
@@ -230,8 +241,12 @@ def test_filter_codes_and_option_checks():
         blockmatch_distance = BlockMatchDistance.POISSON_DEVIANCE
         variance_k = 4
     check_supported(prof, R())
-    # the exact covariance-aware group risk stays outside the path
+    # the exact covariance-aware group risk runs under the Poisson model and
+    # is refused under the Gaussian one
     prof.wiener_weight_scope = AggregationWeightScope.GROUP
+    check_supported(prof, R())
+    R.is_poisson = False
+    R.blockmatch_distance = R.BlockMatchDistance.SSD
     with pytest.raises(NotImplementedError, match="exact covariance-aware"):
         check_supported(prof, R())
     R.variance_k = 0
