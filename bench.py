@@ -598,8 +598,36 @@ def run_batch(args) -> int:
         "kernels_ms": {k: round(v, 3) for k, v in sorted(per_kernel.items())},
         "psnr": psnr,
     }
+    line["graph"] = batch_graph_leg(runs, last, args.steps)
     print(json.dumps(line), flush=True)
     return 0
+
+
+def batch_graph_leg(runs, last, steps: int) -> dict:
+    """Every batch of the step captured as ONE HIP graph (engine.BatchGraph)
+    against the call-by-call step (same kernels, bit-identical outputs)."""
+    import torch
+
+    from bmnd_amd.engine import BatchGraph
+    g = BatchGraph([bp for _, bp, _, _, _ in runs])
+    packed = [p for _, _, p, _, _ in runs]
+    n = max(steps, 5)
+    g.run(packed)  # capture
+    g.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(n):
+        g.run()
+    torch.cuda.synchronize()
+    g_ms = 1e3 * (time.perf_counter() - t0) / n
+    same = all(torch.equal(g.outputs[j].view(torch.int32), last[tag][1].est.view(torch.int32))
+               and torch.equal(g.pilots[j].view(torch.int32), last[tag][0].est.view(torch.int32))
+               for j, (tag, _, _, _, _) in enumerate(runs))
+    voxels = sum(bp.total_voxels for _, bp, _, _, _ in runs)
+    out = {"graph_ms": round(g_ms, 3), "value": round(voxels / (g_ms * 1e-3), 1),
+           "stage1_modes": [m[0] for m in g.modes], "bit_identical": same}
+    g.release()
+    return out
 
 
 

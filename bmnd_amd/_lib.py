@@ -58,6 +58,7 @@ EXPORTS = (
     "bmnd_hip_init",
     "bmnd_hip_stage_noalloc",
     "bmnd_hip_pipeline_create",
+    "bmnd_hip_pipeline_create_many",
     "bmnd_hip_pipeline_launch",
     "bmnd_hip_pipeline_status",
     "bmnd_hip_pipeline_destroy",
@@ -121,6 +122,10 @@ class PipelineDesc(ctypes.Structure):
         ("sel", ctypes.c_void_p * 2),
         ("counts", ctypes.c_void_p * 2),
         ("accepted", ctypes.c_void_p * 2),
+        ("ninputs", ctypes.c_int32),
+        ("batch", ctypes.c_void_p * 2),
+        ("batch_d", ctypes.c_void_p * 2),
+        ("ref_input", ctypes.c_void_p * 2),
     ]
 
 
@@ -223,6 +228,9 @@ def _declare(L: ctypes.CDLL) -> None:
     L.bmnd_hip_pipeline_create.argtypes = [ctypes.POINTER(PipelineDesc),
                                            ctypes.POINTER(ctypes.c_void_p)]
     L.bmnd_hip_pipeline_create.restype = i32
+    L.bmnd_hip_pipeline_create_many.argtypes = [ctypes.POINTER(PipelineDesc), i32,
+                                                ctypes.POINTER(vp)]
+    L.bmnd_hip_pipeline_create_many.restype = i32
     L.bmnd_hip_pipeline_launch.argtypes = [vp, vp]
     L.bmnd_hip_pipeline_launch.restype = i32
     L.bmnd_hip_pipeline_status.argtypes = [vp, vp]

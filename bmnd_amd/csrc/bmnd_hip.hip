@@ -5071,6 +5071,25 @@ int32_t bmnd_hip_reference_map(const int32_t *ref_lin, int64_t nrefs, int32_t *r
   return 0;
 }
 
+// the batch reference map with the clear as a fill kernel (no memset node
+// in a captured graph)
+static int32_t reference_map_batch_impl(const int32_t *ref_lin, const int32_t *ref_input,
+                                        int64_t nrefs, const bmnd_hip_input *inputs_d,
+                                        int32_t *ref_index, int64_t num_patches, void *stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (num_patches > 0) {
+    const unsigned blocks = (unsigned)std::min<int64_t>((num_patches + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks), dim3(256), 0, s, ref_index, num_patches, -1);
+    HIP_TRY(hipGetLastError());
+  }
+  if (nrefs <= 0) return 0;
+  if (!ref_input || !inputs_d) return fail(BMND_HIP_EINVAL, "a batch needs ref_input and inputs");
+  hipLaunchKernelGGL(k_reference_map_batch, dim3((unsigned)((nrefs + 255) / 256)), dim3(256), 0, s,
+                     ref_lin, ref_input, nrefs, inputs_d, ref_index);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
 int32_t bmnd_hip_reference_map_batch(const int32_t *ref_lin, const int32_t *ref_input,
                                      int64_t nrefs, const bmnd_hip_input *inputs_d,
                                      int32_t *ref_index, int64_t num_patches, void *stream) {
@@ -5586,15 +5605,21 @@ struct bmnd_hip_pipeline {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   void *ws = nullptr;          // library-owned device workspace
-  int32_t *flags = nullptr;    // [is integer, max count, HT capacity, Wiener capacity]
-  int32_t table_mode = 0;      // stage 1 runs the count table: check its precondition
-  int32_t table_max = 0;
+  // per descriptor: [is integer, max count, HT capacity, Wiener capacity]
+  int32_t *flags = nullptr;
+  std::vector<int32_t> table_mode;  // stage 1 runs the count table: check its precondition
+  std::vector<int32_t> table_max;
 };
 
-int32_t bmnd_hip_pipeline_create(const bmnd_hip_pipeline_desc *d, bmnd_hip_pipeline **out) {
-  if (!d || !out) return fail(BMND_HIP_EINVAL, "pipeline: null argument");
-  *out = nullptr;
-  if (!d->input || !d->pilot || !d->output) return fail(BMND_HIP_EINVAL, "pipeline: null volume");
+namespace {
+// the sizes one descriptor's capture needs: voxels, each stage's patch grid,
+// stage workspace (bmnd_hip_stage_workspace's layout) and the fast matcher's
+// patch sums
+struct PipeSizes {
+  int64_t vox = 0, patches[2] = {0, 0}, stage_bytes = 0, sums = 0;
+};
+
+int32_t pipeline_sizes(const bmnd_hip_pipeline_desc *d, PipeSizes *z) {
   for (int st = 0; st < 2; ++st) {
     if (d->mode[st] < 0 || d->mode[st] > 6)
       return fail(BMND_HIP_EINVAL, "pipeline: modes must be fixed (0..6)");
@@ -5602,28 +5627,71 @@ int32_t bmnd_hip_pipeline_create(const bmnd_hip_pipeline_desc *d, bmnd_hip_pipel
         !d->accepted[st] || !d->matrices[st] || !d->kaiser[st])
       return fail(BMND_HIP_EINVAL, "pipeline: every stage needs its schedule, outputs and tables");
   }
+  if (!d->input || !d->pilot || !d->output) return fail(BMND_HIP_EINVAL, "pipeline: null volume");
   if (d->mode[0] == 2 && (d->table_max < 0 || d->table_max > kMaxTableCount))
     return fail(BMND_HIP_EINVAL, "pipeline: 0 <= table_max <= 100");
   if (d->wiener_gain < 1 || d->wiener_gain > 3)
     return fail(BMND_HIP_EINVAL, "pipeline: wiener_gain must be 1, 2 or 3");
-  if (int rc = bmnd_hip_init()) return rc;  // every table built before the capture
-  int64_t vox = 1;
-  for (int a = 0; a < 3; ++a) vox *= d->shape[a];
-  // workspace: two variance volumes, the larger stage's workspace, the fast
-  // matcher's patch sums and the flags
-  int64_t stage_bytes = 0, sums = 0;
+  const int32_t k = d->k & BMND_HIP_K_PLANES;
+  if (d->ninputs < 0) return fail(BMND_HIP_EINVAL, "pipeline: ninputs must be >= 0");
   for (int st = 0; st < 2; ++st) {
-    const int64_t need = bmnd_hip_stage_workspace(d->shape, d->block[st], d->nrefs[st],
-                                                  d->max_matches[st], d->k);
-    if (need < 0) return fail(BMND_HIP_EINVAL, "pipeline: invalid stage geometry");
-    stage_bytes = std::max(stage_bytes, need);
-    const int64_t c2 = d->shape[2] - d->block[st][2] + 1;
-    int64_t grid = 1;
-    for (int a = 0; a < 3; ++a) grid *= d->shape[a] - d->block[st][a] + 1;
-    sums = std::max(sums, align256(16 * d->shape[0] * d->shape[1] * c2) + 16 * grid);
+    int64_t P = 1;
+    for (int a = 0; a < 3; ++a) P *= d->block[st][a];
+    int64_t rows_max = 0;
+    if (d->ninputs > 0) {
+      if (!d->batch[st] || !d->batch_d[st] || !d->ref_input[st])
+        return fail(BMND_HIP_EINVAL, "pipeline: a batch needs its records and ref_input");
+      const BatchRef b{d->batch[st], d->batch_d[st], d->ninputs, d->ref_input[st]};
+      int64_t grid = 0;
+      if (int rc = check_batch(b, d->block[st], d->radius[st], &grid, nullptr)) return rc;
+      z->patches[st] = grid;
+      const bmnd_hip_input &last = d->batch[st][d->ninputs - 1];
+      z->vox = last.vox_off + (int64_t)last.shape[0] * last.shape[1] * last.shape[2];
+      for (int i = 0; i < d->ninputs; ++i)
+        rows_max = std::max(rows_max,
+                            (int64_t)d->batch[st][i].shape[0] * d->batch[st][i].shape[1] *
+                                d->batch[st][i].grid[2]);
+    } else {
+      int c[3];
+      if (int rc = check_geometry(d->shape, d->block[st], d->radius[st], c)) return rc;
+      z->patches[st] = (int64_t)c[0] * c[1] * c[2];
+      z->vox = d->shape[0] * d->shape[1] * d->shape[2];
+      rows_max = d->shape[0] * d->shape[1] * c[2];
+    }
+    const int64_t R = d->nrefs[st], M = d->max_matches[st];
+    const int64_t kcap = std::min<int64_t>(k, M);
+    const int64_t need = align256(2 * sizeof(int32_t)) + align256(R * kcap * P * 4) +
+                         align256(R * M * P * 4) + align256(R * M * 4) +
+                         align256(z->patches[st] * 4);
+    z->stage_bytes = std::max(z->stage_bytes, need);
+    z->sums = std::max(z->sums, align256(16 * rows_max) + 16 * z->patches[st]);
   }
-  const int64_t vbytes = align256(vox * (int64_t)sizeof(float));
-  const int64_t total = 2 * vbytes + align256(stage_bytes) + align256(sums) + 256;
+  return 0;
+}
+}  // namespace
+
+int32_t bmnd_hip_pipeline_create(const bmnd_hip_pipeline_desc *d, bmnd_hip_pipeline **out) {
+  return bmnd_hip_pipeline_create_many(d, 1, out);
+}
+
+int32_t bmnd_hip_pipeline_create_many(const bmnd_hip_pipeline_desc *descs, int32_t count,
+                                      bmnd_hip_pipeline **out) {
+  if (!descs || !out || count < 1) return fail(BMND_HIP_EINVAL, "pipeline: null argument");
+  *out = nullptr;
+  std::vector<PipeSizes> sz((size_t)count);
+  int64_t vox_max = 0, stage_bytes = 0, sums = 0;
+  for (int j = 0; j < count; ++j) {
+    if (int rc = pipeline_sizes(&descs[j], &sz[j])) return rc;
+    vox_max = std::max(vox_max, sz[j].vox);
+    stage_bytes = std::max(stage_bytes, sz[j].stage_bytes);
+    sums = std::max(sums, sz[j].sums);
+  }
+  if (int rc = bmnd_hip_init()) return rc;  // every table built before the capture
+  // workspace: two variance volumes, the largest stage workspace and patch
+  // sums (the descriptors run one after another and share them), the flags
+  const int64_t vbytes = align256(vox_max * (int64_t)sizeof(float));
+  const int64_t total =
+      2 * vbytes + align256(stage_bytes) + align256(sums) + align256(16 * (int64_t)count);
   auto *p = new bmnd_hip_pipeline();
   auto cleanup = [&](int32_t rc) {
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
@@ -5640,8 +5708,10 @@ int32_t bmnd_hip_pipeline_create(const bmnd_hip_pipeline_desc *d, bmnd_hip_pipel
   void *stage_ws = w + 2 * vbytes;
   void *sums_ws = w + 2 * vbytes + align256(stage_bytes);
   p->flags = reinterpret_cast<int32_t *>(w + 2 * vbytes + align256(stage_bytes) + align256(sums));
-  p->table_mode = d->mode[0] == 2;
-  p->table_max = d->table_max;
+  for (int j = 0; j < count; ++j) {
+    p->table_mode.push_back(descs[j].mode[0] == 2);
+    p->table_max.push_back(descs[j].table_max);
+  }
 
   hipStream_t cs = nullptr;
   if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess)
@@ -5649,64 +5719,77 @@ int32_t bmnd_hip_pipeline_create(const bmnd_hip_pipeline_desc *d, bmnd_hip_pipel
   // the sequence below runs once eagerly first, which settles the tuned
   // launch shapes, then once under capture
   auto captured = [&]() -> int32_t {
-    // the capacity flags start at zero on every launch; the count range of
-    // the input is recorded for the table mode's precondition
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, cs, p->flags, (int64_t)4, 0);
-    if (p->table_mode)
-      if (int rc = bmnd_hip_count_range(d->input, vox, p->flags, cs)) return rc;
-    if (int rc = bmnd_hip_variance_volume(d->input, var1, vox, d->count_scale, d->variance_floor,
-                                          cs))
-      return rc;
-    for (int st = 0; st < 2; ++st) {
-      const bool wiener = st == 1;
-      const float *match = wiener && !d->match_observation ? d->pilot : d->input;
-      const float *var = wiener && !d->variance_observation ? var2 : var1;
-      float *est = wiener ? d->output : d->pilot;
-      if (wiener && !d->variance_observation)
-        if (int rc = bmnd_hip_variance_volume(d->pilot, var2, vox, d->count_scale,
-                                              d->variance_floor, cs))
+    // the flags start at zero on every launch; the count range of each
+    // input buffer is recorded for the table mode's precondition
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, cs, p->flags, (int64_t)4 * count, 0);
+    for (int j = 0; j < count; ++j) {
+      const bmnd_hip_pipeline_desc *d = &descs[j];
+      const int64_t vox = sz[j].vox;
+      int32_t *flags = p->flags + 4 * j;
+      const bool batch = d->ninputs > 0;
+      if (p->table_mode[j])
+        if (int rc = bmnd_hip_count_range(d->input, vox, flags, cs)) return rc;
+      if (int rc = bmnd_hip_variance_volume(d->input, var1, vox, d->count_scale,
+                                            d->variance_floor, cs))
+        return rc;
+      for (int st = 0; st < 2; ++st) {
+        const bool wiener = st == 1;
+        const float *match = wiener && !d->match_observation ? d->pilot : d->input;
+        const float *var = wiener && !d->variance_observation ? var2 : var1;
+        float *est = wiener ? d->output : d->pilot;
+        if (wiener && !d->variance_observation)
+          if (int rc = bmnd_hip_variance_volume(d->pilot, var2, vox, d->count_scale,
+                                                d->variance_floor, cs))
+            return rc;
+        const BatchRef bref{d->batch[st], d->batch_d[st], d->ninputs, d->ref_input[st]};
+        const BatchRef *b = batch ? &bref : nullptr;
+        int64_t P = 1;
+        for (int a = 0; a < 3; ++a) P *= d->block[st][a];
+        const int64_t patches = sz[j].patches[st];
+        const int64_t R = d->nrefs[st], M = d->max_matches[st];
+        const int32_t k = d->k & BMND_HIP_K_PLANES;
+        const int64_t kcap = std::min<int64_t>(k, M);
+        char *sw = static_cast<char *>(stage_ws) + align256(2 * sizeof(int32_t));
+        float *exact = kcap > 0 ? reinterpret_cast<float *>(sw) : nullptr;
+        sw += align256(R * kcap * P * (int64_t)sizeof(float));
+        float *filt = reinterpret_cast<float *>(sw);
+        sw += align256(R * M * P * (int64_t)sizeof(float));
+        float *weights = reinterpret_cast<float *>(sw);
+        sw += align256(R * M * (int64_t)sizeof(float));
+        int32_t *ref_index = reinterpret_cast<int32_t *>(sw);
+        if (int rc = blockmatch_impl(match, d->shape, d->block[st], d->radius[st],
+                                     d->ref_lin[st], R, d->threshold[st], (int32_t)M,
+                                     d->min_matches[st], 1, 1, d->count_scale, d->mode[st],
+                                     st == 0 ? d->table_max : 0, d->sel[st], d->counts[st],
+                                     d->accepted[st], cs, b, nullptr, nullptr, 0, sums_ws, sums))
           return rc;
-      int64_t P = 1, patches = 1;
-      for (int a = 0; a < 3; ++a) {
-        P *= d->block[st][a];
-        patches *= d->shape[a] - d->block[st][a] + 1;
+        if (kcap > 0)
+          if (int rc = exact_std_impl(var, d->shape, d->block[st], d->radius[st], d->sel[st],
+                                      d->counts[st], R, (int32_t)M, d->matrices[st], k, exact,
+                                      cs, b,
+                                      (d->k & (st ? BMND_HIP_EXACT_PAIRS_WIENER
+                                                  : BMND_HIP_EXACT_PAIRS_HT)) != 0))
+            return rc;
+        if (int rc = filter_impl(wiener ? d->wiener_gain : 0, d->input,
+                                 wiener ? d->pilot : nullptr, var, d->shape, d->block[st],
+                                 d->radius[st], d->sel[st], d->counts[st], R, (int32_t)M,
+                                 d->matrices[st], exact, d->shrink[st], k, d->count_scale, filt,
+                                 weights, nullptr, cs, b))
+          return rc;
+        if (batch) {
+          if (int rc = reference_map_batch_impl(d->ref_lin[st], d->ref_input[st], R,
+                                                d->batch_d[st], ref_index, patches, cs))
+            return rc;
+        } else if (int rc = reference_map_impl(d->ref_lin[st], R, ref_index, patches, cs,
+                                               true)) {
+          return rc;
+        }
+        if (int rc = aggregate_impl(filt, weights, d->sel[st], d->counts[st], ref_index, R,
+                                    (int32_t)M, d->shape, d->block[st], d->radius[st],
+                                    d->kaiser[st], d->input, est, nullptr, 0, 0.0f, nullptr, cs,
+                                    b, flags + 2 + st, true))
+          return rc;
       }
-      const int64_t R = d->nrefs[st], M = d->max_matches[st];
-      const int32_t k = d->k & BMND_HIP_K_PLANES;
-      const int64_t kcap = std::min<int64_t>(k, M);
-      char *sw = static_cast<char *>(stage_ws) + align256(2 * sizeof(int32_t));
-      float *exact = kcap > 0 ? reinterpret_cast<float *>(sw) : nullptr;
-      sw += align256(R * kcap * P * (int64_t)sizeof(float));
-      float *filt = reinterpret_cast<float *>(sw);
-      sw += align256(R * M * P * (int64_t)sizeof(float));
-      float *weights = reinterpret_cast<float *>(sw);
-      sw += align256(R * M * (int64_t)sizeof(float));
-      int32_t *ref_index = reinterpret_cast<int32_t *>(sw);
-      if (int rc = blockmatch_impl(match, d->shape, d->block[st], d->radius[st], d->ref_lin[st],
-                                   R, d->threshold[st], (int32_t)M, d->min_matches[st], 1, 1,
-                                   d->count_scale, d->mode[st], st == 0 ? d->table_max : 0,
-                                   d->sel[st], d->counts[st], d->accepted[st], cs, nullptr,
-                                   nullptr, nullptr, 0, sums_ws, sums))
-        return rc;
-      if (kcap > 0)
-        if (int rc = exact_std_impl(var, d->shape, d->block[st], d->radius[st], d->sel[st],
-                                    d->counts[st], R, (int32_t)M, d->matrices[st], k, exact,
-                                    cs, nullptr,
-                                    (d->k & (st ? BMND_HIP_EXACT_PAIRS_WIENER
-                                                : BMND_HIP_EXACT_PAIRS_HT)) != 0))
-          return rc;
-      if (int rc = filter_impl(wiener ? d->wiener_gain : 0, d->input, wiener ? d->pilot : nullptr,
-                               var, d->shape, d->block[st], d->radius[st], d->sel[st],
-                               d->counts[st], R, (int32_t)M, d->matrices[st], exact,
-                               d->shrink[st], k, d->count_scale, filt, weights, nullptr, cs,
-                               nullptr))
-        return rc;
-      if (int rc = reference_map_impl(d->ref_lin[st], R, ref_index, patches, cs, true)) return rc;
-      if (int rc = aggregate_impl(filt, weights, d->sel[st], d->counts[st], ref_index, R,
-                                  (int32_t)M, d->shape, d->block[st], d->radius[st],
-                                  d->kaiser[st], d->input, est, nullptr, 0, 0.0f, nullptr, cs,
-                                  nullptr, p->flags + 2 + st, true))
-        return rc;
     }
     return 0;
   };
@@ -5746,15 +5829,21 @@ int32_t bmnd_hip_pipeline_launch(bmnd_hip_pipeline *p, void *stream) {
 
 int32_t bmnd_hip_pipeline_status(bmnd_hip_pipeline *p, void *stream) {
   if (!p) return fail(BMND_HIP_EINVAL, "pipeline: null handle");
-  int32_t h[4] = {0, 0, 0, 0};
+  const size_t count = p->table_mode.size();
+  std::vector<int32_t> h(4 * count, 0);
   hipStream_t s = (hipStream_t)stream;
-  HIP_TRY(hipMemcpyAsync(h, p->flags, sizeof(h), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(h.data(), p->flags, sizeof(int32_t) * h.size(), hipMemcpyDeviceToHost,
+                         s));
   HIP_TRY(hipStreamSynchronize(s));
-  if (p->table_mode && (!h[0] || h[1] > p->table_max))
-    return fail(BMND_HIP_EINVAL, "pipeline: the input is not integer counts <= " +
-                                     std::to_string(p->table_max) +
-                                     " (the count table it was captured with); results invalid");
-  if (h[2] || h[3]) return fail(BMND_HIP_ECAPACITY, "aggregation tile reached too many groups");
+  for (size_t j = 0; j < count; ++j)
+    if (p->table_mode[j] && (!h[4 * j] || h[4 * j + 1] > p->table_max[j]))
+      return fail(BMND_HIP_EINVAL, "pipeline: input " + std::to_string(j) +
+                                       " is not integer counts <= " +
+                                       std::to_string(p->table_max[j]) +
+                                       " (the count table it was captured with); results invalid");
+  for (size_t j = 0; j < count; ++j)
+    if (h[4 * j + 2] || h[4 * j + 3])
+      return fail(BMND_HIP_ECAPACITY, "aggregation tile reached too many groups");
   return 0;
 }
 

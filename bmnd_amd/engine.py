@@ -809,13 +809,14 @@ class PoissonPipeline:
         path takes for a non-integer pilot (every mode it could take returns
         the same lists)."""
         m1, tmax = self.blockmatch_mode(vol, "ht")
+        return m1, tmax, self._graph_mode_wiener()
+
+    def _graph_mode_wiener(self) -> int:
         code = _DISTANCE_CODES[self.distance]
         if code != 1:
-            m2 = 3 + code
-        else:
-            m2 = {"auto": 0, "exact": 0, "one-log": 1, "auto-one-log": 1, "certified": 3,
-                  "auto-certified": 3, "fast": 4, "auto-fast": 4}[self.deviance]
-        return m1, tmax, m2
+            return 3 + code
+        return {"auto": 0, "exact": 0, "one-log": 1, "auto-one-log": 1, "certified": 3,
+                "auto-certified": 3, "fast": 4, "auto-fast": 4}[self.deviance]
 
     def run_graph(self, vol) -> torch.Tensor:
         """Both stages by one launch of the captured graph (StageGraph) on
@@ -855,6 +856,37 @@ def _lib_einval() -> int:
     return -1  # BMND_HIP_EINVAL
 
 
+def _pipeline_desc(pipe: PoissonPipeline, plans: dict, refs: dict, lists: dict, modes,
+                   vin: torch.Tensor, pilot: torch.Tensor, out: torch.Tensor):
+    """bmnd_hip_pipeline_desc for `pipe`'s options with each stage's plan,
+    reference list (device int32) and match-list buffers (sel, counts,
+    accepted); the caller sets the geometry (shape, or the batch fields)."""
+    d = _lib.PipelineDesc()
+    for st, name in enumerate(("ht", "wiener")):
+        plan = plans[name]
+        d.block[st][:] = plan.block3
+        d.radius[st][:] = plan.radius3
+        d.ref_lin[st] = refs[name].data_ptr()
+        d.nrefs[st] = refs[name].numel()
+        d.threshold[st] = plan.threshold
+        d.max_matches[st], d.min_matches[st] = plan.max_m, plan.min_m
+        mats, kai = plan.device_constants(pipe.device)
+        d.matrices[st], d.kaiser[st] = mats.data_ptr(), kai.data_ptr()
+        d.shrink[st] = plan.shrink
+        sel, cnt, acc = lists[name]
+        d.sel[st], d.counts[st], d.accepted[st] = sel.data_ptr(), cnt.data_ptr(), acc.data_ptr()
+    d.mode[0], d.table_max, d.mode[1] = modes
+    d.k, d.wiener_gain = k_with_order(pipe.k), pipe.wiener_arg
+    d.count_scale, d.variance_floor = pipe.count_scale, pipe.floor
+    src = lambda f: coerce_enum(getattr(pipe.profile, f), PoissonVarianceSource, f)  # noqa: E731
+    d.match_observation = int(src("poisson_wiener_match_source")
+                              is PoissonVarianceSource.OBSERVATION)
+    d.variance_observation = int(src("poisson_variance_source_wiener")
+                                 is PoissonVarianceSource.OBSERVATION)
+    d.input, d.pilot, d.output = vin.data_ptr(), pilot.data_ptr(), out.data_ptr()
+    return d
+
+
 class StageGraph:
     """Both stages of a PoissonPipeline captured as one HIP graph
     (bmnd_hip_pipeline_*): the graph reads `input` and writes `pilot`,
@@ -879,32 +911,10 @@ class StageGraph:
     def capture(self, mode_ht: int, table_max: int, mode_wi: int) -> None:
         self.release()
         pipe, L = self.pipe, _lib.lib()
-        d = _lib.PipelineDesc()
+        refs = {name: pipe._ref_lin(plan) for name, plan in pipe.plans.items()}
+        d = _pipeline_desc(pipe, pipe.plans, refs, self.lists, (mode_ht, table_max, mode_wi),
+                           self.input, self.pilot, self.output)
         d.shape[:] = pipe.shape3
-        for st, name in enumerate(("ht", "wiener")):
-            plan = pipe.plans[name]
-            d.block[st][:] = plan.block3
-            d.radius[st][:] = plan.radius3
-            refs = pipe._ref_lin(plan)
-            d.ref_lin[st] = refs.data_ptr()
-            d.nrefs[st] = refs.numel()
-            d.threshold[st] = plan.threshold
-            d.max_matches[st], d.min_matches[st] = plan.max_m, plan.min_m
-            mats, kai = plan.device_constants(pipe.device)
-            d.matrices[st], d.kaiser[st] = mats.data_ptr(), kai.data_ptr()
-            d.shrink[st] = plan.shrink
-            sel, cnt, acc = self.lists[name]
-            d.sel[st], d.counts[st], d.accepted[st] = sel.data_ptr(), cnt.data_ptr(), acc.data_ptr()
-        d.mode[0], d.mode[1], d.table_max = mode_ht, mode_wi, table_max
-        d.k, d.wiener_gain = k_with_order(pipe.k), pipe.wiener_arg
-        d.count_scale, d.variance_floor = pipe.count_scale, pipe.floor
-        src = lambda f: coerce_enum(getattr(pipe.profile, f), PoissonVarianceSource, f)  # noqa: E731
-        d.match_observation = int(src("poisson_wiener_match_source")
-                                  is PoissonVarianceSource.OBSERVATION)
-        d.variance_observation = int(src("poisson_variance_source_wiener")
-                                     is PoissonVarianceSource.OBSERVATION)
-        d.input, d.pilot, d.output = (self.input.data_ptr(), self.pilot.data_ptr(),
-                                      self.output.data_ptr())
         h = ctypes.c_void_p()
         torch.cuda.synchronize(pipe.device)
         _lib.check(L.bmnd_hip_pipeline_create(ctypes.byref(d), ctypes.byref(h)))
@@ -1202,3 +1212,122 @@ class BatchPipeline:
         del var1
         wi = self.stage("wiener", match2, packed, var2, packed, timer, pilot=ht.est)
         return ht, wi
+
+    def graph_modes(self, packed: torch.Tensor) -> tuple[int, int, int]:
+        """(stage-1 mode, its table size, stage-2 mode) for capturing the
+        batch: the count table when every input holds integer counts <= 100
+        (its size the largest count), else the mode stage 1 takes for the
+        inputs that do not; stage 2 as PoissonPipeline's graph.  Every mode
+        the call-by-call path could take for an input returns the same lists."""
+        modes = self.input_modes(packed, "ht")
+        if all(m == 2 for m, _ in modes):
+            m1, tmax = 2, max(t for _, t in modes)
+        else:
+            m1, tmax = next((m, t) for m, t in modes if m != 2)
+        return m1, tmax, self._single._graph_mode_wiener()
+
+
+class BatchGraph:
+    """Both stages of one or more BatchPipelines (for instance a batch of
+    images and a batch of volumes) captured as ONE HIP graph
+    (bmnd_hip_pipeline_create_many): a launch reads every batch's packed
+    `inputs[j]` and writes `pilots[j]`, `outputs[j]` and each stage's flat
+    match lists `lists[j][stage]`, with no allocation or host
+    synchronisation between the kernels.  Every output equals the
+    BatchPipeline's call-by-call result bit for bit."""
+
+    def __init__(self, pipes):
+        self.pipes = list(pipes)
+        if not self.pipes:
+            raise ValueError("a batch graph needs at least one BatchPipeline")
+        devs = {bp.device for bp in self.pipes}
+        if len(devs) != 1:
+            raise ValueError("every batch of a graph must live on one device")
+        self.device = devs.pop()
+        for bp in self.pipes:
+            if not bp._single.graph_supported():
+                raise NotImplementedError("this profile's options run on the call-by-call path")
+        dev = self.device
+        self.inputs, self.pilots, self.outputs, self.lists = [], [], [], []
+        for bp in self.pipes:
+            self.inputs.append(torch.empty(bp.total_voxels, dtype=torch.float32, device=dev))
+            self.pilots.append(torch.empty_like(self.inputs[-1]))
+            self.outputs.append(torch.empty_like(self.inputs[-1]))
+            lists = {}
+            for name, st in bp.stages.items():
+                R = st.nrefs
+                lists[name] = (torch.empty(R * st.plan.max_m, dtype=torch.int32, device=dev),
+                               torch.empty(R, dtype=torch.int32, device=dev),
+                               torch.empty(R, dtype=torch.int32, device=dev))
+            self.lists.append(lists)
+        self.handle = None
+        self.modes = None
+        self.k = None
+
+    def _modes(self):
+        return [bp.graph_modes(v) for bp, v in zip(self.pipes, self.inputs)]
+
+    def capture(self, modes) -> None:
+        self.release()
+        descs = (_lib.PipelineDesc * len(self.pipes))()
+        for j, bp in enumerate(self.pipes):
+            plans = {name: st.plan for name, st in bp.stages.items()}
+            refs = {name: st.ref_lin for name, st in bp.stages.items()}
+            d = _pipeline_desc(bp._single, plans, refs, self.lists[j], modes[j], self.inputs[j],
+                               self.pilots[j], self.outputs[j])
+            d.shape[:] = bp.shape3s[0]
+            d.ninputs = len(bp.shapes)
+            for s, name in enumerate(("ht", "wiener")):
+                st = bp.stages[name]
+                d.batch[s] = st.inputs_h.ctypes.data
+                d.batch_d[s] = st.inputs_d.data_ptr()
+                d.ref_input[s] = st.ref_input.data_ptr()
+            descs[j] = d
+        h = ctypes.c_void_p()
+        torch.cuda.synchronize(self.device)
+        _lib.check(_lib.lib().bmnd_hip_pipeline_create_many(descs, len(self.pipes),
+                                                            ctypes.byref(h)))
+        self.handle, self.modes = h, [tuple(m) for m in modes]
+        self.k = k_with_order(self.pipes[0]._single.k)
+
+    def launch(self) -> int:
+        """One launch on the current stream, then its status (synchronises)."""
+        L = _lib.lib()
+        stream = self.pipes[0]._single._stream()
+        rc = L.bmnd_hip_pipeline_launch(self.handle, stream)
+        if rc:
+            return rc
+        return L.bmnd_hip_pipeline_status(self.handle, stream)
+
+    def run(self, packed=None) -> list[torch.Tensor]:
+        """One launch over every batch; `packed` (optional) holds each batch's
+        packed input (BatchPipeline.pack), copied into the graph's inputs
+        first.  Returns the Wiener estimates (buffers of the graph,
+        overwritten by its next launch).  The graph is captured on the first
+        call, and again when an input breaks the count-table precondition it
+        was captured with (that launch is then repeated with the new graph)."""
+        if packed is not None:
+            if len(packed) != len(self.pipes):
+                raise ValueError("one packed input per batch")
+            for dst, src in zip(self.inputs, packed):
+                dst.copy_(torch.as_tensor(src).view(-1), non_blocking=False)
+        if self.handle is None or self.k != k_with_order(self.pipes[0]._single.k):
+            self.capture(self._modes())
+        rc = self.launch()
+        if rc == _lib_einval():
+            self.capture(self._modes())
+            rc = self.launch()
+        _lib.check(rc)
+        return self.outputs
+
+    def release(self) -> None:
+        if self.handle is not None:
+            torch.cuda.synchronize(self.device)
+            _lib.lib().bmnd_hip_pipeline_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass

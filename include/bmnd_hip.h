@@ -478,6 +478,21 @@ int32_t bmnd_hip_stage_noalloc(int32_t wiener, const float *match_vol, const flo
                                float shrink, int32_t k, float *est, int32_t *sel,
                                int32_t *counts, int32_t *accepted, void *stream);
 
+/* ---- batches of inputs of different shapes --------------------------------
+ * No counterpart in the reference, which denoises one array per call: a batch
+ * runs every input of one profile (one block / search geometry) through each
+ * kernel in a single launch over the flat list of all inputs' references
+ * (or aggregation tiles).  The inputs' voxels, patch grids and tiles are
+ * packed one after the other in flat buffers, in input order; input i is
+ * described by */
+typedef struct {
+  int64_t vox_off;   /* first voxel in the packed volumes: sum of the earlier inputs' voxels */
+  int64_t grid_off;  /* first patch origin in packed patch-grid buffers: sum of earlier grids */
+  int64_t tile_off;  /* first aggregation tile: sum of earlier inputs' tile counts */
+  int32_t shape[3];  /* voxels per axis; images as (1, H, W) */
+  int32_t grid[3];   /* patch-grid counts shape - block + 1 of the stage */
+} bmnd_hip_input;
+
 /* ---- both stages captured as one HIP graph ---------------------------------
  * No counterpart in the reference (which has no device): the whole two-stage
  * Poisson pipeline of bmnd() (bmndalgo.py:213-1046) recorded once as a HIP
@@ -513,9 +528,26 @@ typedef struct {
   const float *input;           /* device volumes */
   float *pilot, *output;
   int32_t *sel[2], *counts[2], *accepted[2];
+  /* Optional batch.  ninputs = 0: one input of `shape`.  ninputs > 0: input,
+   * pilot and output hold every input packed one after another; batch[st]
+   * (host) and batch_d[st] (device) describe them for stage st as the
+   * bmnd_hip_*_batch calls take them (that stage's grids, offsets and
+   * aggregation tiles); ref_lin, ref_input, sel, counts and accepted are the
+   * stage's flat lists over all inputs (ref_input: each reference's input);
+   * `shape` is not read.  Stage 1's mode applies to every input. */
+  int32_t ninputs;
+  const bmnd_hip_input *batch[2];
+  const bmnd_hip_input *batch_d[2];
+  const int32_t *ref_input[2];
 } bmnd_hip_pipeline_desc;
 typedef struct bmnd_hip_pipeline bmnd_hip_pipeline;
 int32_t bmnd_hip_pipeline_create(const bmnd_hip_pipeline_desc *desc, bmnd_hip_pipeline **out);
+/* One graph over `count` descriptors, run one after another on the launch's
+ * stream (for instance a batch of images and a batch of volumes, whose
+ * geometries differ): one bmnd_hip_pipeline_launch runs them all, and
+ * bmnd_hip_pipeline_status checks every descriptor's flags. */
+int32_t bmnd_hip_pipeline_create_many(const bmnd_hip_pipeline_desc *descs, int32_t count,
+                                      bmnd_hip_pipeline **out);
 int32_t bmnd_hip_pipeline_launch(bmnd_hip_pipeline *pipeline, void *stream);
 int32_t bmnd_hip_pipeline_status(bmnd_hip_pipeline *pipeline, void *stream);
 void bmnd_hip_pipeline_destroy(bmnd_hip_pipeline *pipeline);
@@ -541,20 +573,6 @@ int32_t bmnd_hip_schedule_generated(int32_t ndim, const int64_t *counts_h, const
 int32_t bmnd_hip_schedule_axis_shifts(int32_t block, int32_t step, int32_t allow_multiple,
                                       double density, int32_t *out);
 
-/* ---- batches of inputs of different shapes --------------------------------
- * No counterpart in the reference, which denoises one array per call: a batch
- * runs every input of one profile (one block / search geometry) through each
- * kernel in a single launch over the flat list of all inputs' references
- * (or aggregation tiles).  The inputs' voxels, patch grids and tiles are
- * packed one after the other in flat buffers, in input order; input i is
- * described by */
-typedef struct {
-  int64_t vox_off;   /* first voxel in the packed volumes: sum of the earlier inputs' voxels */
-  int64_t grid_off;  /* first patch origin in packed patch-grid buffers: sum of earlier grids */
-  int64_t tile_off;  /* first aggregation tile: sum of earlier inputs' tile counts */
-  int32_t shape[3];  /* voxels per axis; images as (1, H, W) */
-  int32_t grid[3];   /* patch-grid counts shape - block + 1 of the stage */
-} bmnd_hip_input;
 /* Each batched entry point takes the descriptors twice: inputs_h (host, used
  * for checks and launch sizes) and inputs_d (the same ninputs records in
  * device memory, read by the kernels).  ref_lin holds every reference's
