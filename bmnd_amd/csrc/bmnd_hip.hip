@@ -501,17 +501,46 @@ __device__ __forceinline__ int bm_keep_count(int acc, int box, const BMArgs &a) 
   return min(req, box);
 }
 
-// The box composites held by this thread, visited as each(f) -> f(key) for
-// every one: keys in an LDS array indexed by candidate (i = tid, tid + NT, ...)
+// The box composites held by this thread, visited as each(f) -> f(key, valid)
+// for slots i = tid, tid + NT, ... below box rounded up to NT: every thread of
+// the workgroup makes the same number of calls (the histogram and list
+// appends inside are wave-collective), and valid says whether slot i holds a
+// candidate.  Keys are in an LDS array indexed by candidate.
 template <int NT, class KeyOf>
 __device__ __forceinline__ auto bm_box_keys(KeyOf key_of, int box) {
   return [=](auto f) {
-    for (int i = threadIdx.x; i < box; i += NT) f(key_of(i));
+    for (int base = 0; base < box; base += NT) {
+      const int i = base + (int)threadIdx.x;
+      const bool v = i < box;
+      f(key_of(v ? i : box - 1), v);
+    }
   };
 }
 
+// hist[bin] += 1 for every lane with bin >= 0; every lane of the wave calls
+// it.  The lanes that share the lowest pending lane's bin add once, for two
+// rounds (the leading passes of a radix select put most keys of a reference
+// into one or two bins, where per-lane atomics on one address serialise);
+// the lanes left add one each.
+__device__ __forceinline__ void wave_hist_add(int *hist, int bin) {
+  unsigned long long pend = __ballot(bin >= 0);
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    if (pend) {
+      const int lead = __builtin_ctzll(pend);
+      const int lb = __builtin_amdgcn_readlane(bin, lead);
+      const unsigned long long peers = __ballot(bin == lb);
+      if ((int)(threadIdx.x & 63) == lead) atomicAdd(&hist[lb], (int)__popcll(peers));
+      if (bin == lb) bin = -1;
+      pend &= ~peers;
+    }
+  }
+  if (bin >= 0) atomicAdd(&hist[bin], 1);
+}
+
 // Radix select over the unique box composites (each(f) visits the thread's
-// own): on return the K smallest are exactly those with
+// own as f(key, valid), the same number of calls in every thread): on return
+// the K smallest are exactly those with
 // (key & mask) <= prefix.  The candidate index is below 2^16, so bits 16..31
 // are zero in every key and their passes are skipped; once the bin holding
 // the K-th key is needed in full the select stops with the bits resolved so
@@ -532,8 +561,8 @@ __device__ __forceinline__ void bm_radix_select(Each each, int K, const BMSelScr
     }
     for (int i = tid; i < 256; i += NT) sc.hist[i] = 0;
     __syncthreads();
-    each([&](unsigned long long k) {
-      if ((k & mask) == prefix) atomicAdd(&sc.hist[(int)((k >> shift) & 255ull)], 1);
+    each([&](unsigned long long k, bool v) {
+      wave_hist_add(sc.hist, v && (k & mask) == prefix ? (int)((k >> shift) & 255ull) : -1);
     });
     __syncthreads();
     if (tid < 64) {
@@ -596,8 +625,15 @@ __device__ __forceinline__ void bm_select_write(Each each, int box, int acc, con
   bm_radix_select<NT, BOX>(each, K, sc, prefix, mask);
   if (tid == 0) *sc.cnt = 0;
   __syncthreads();
-  each([&](unsigned long long k) {
-    if ((k & mask) <= prefix) sc.small[atomicAdd(sc.cnt, 1)] = k;
+  each([&](unsigned long long k, bool v) {
+    // the wave's kept composites take consecutive entries (their order is
+    // settled by the rank sort below)
+    const bool in = v && (k & mask) <= prefix;
+    const unsigned long long inb = __ballot(in);
+    int base = 0;
+    if ((tid & 63) == 0 && inb) base = atomicAdd(sc.cnt, (int)__popcll(inb));
+    base = __builtin_amdgcn_readlane(base, 0);
+    if (in) sc.small[base + (int)__popcll(inb & ((1ull << (tid & 63)) - 1ull))] = k;
   });
   __syncthreads();
   if (tid < K) {
@@ -1625,7 +1661,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
         for (int k = 0; k < KC; ++k) {
           float kh = khi[k];
           asm volatile("" : "+v"(kh));
-          if (vbits & (1u << k)) f(comp(kh, t + k * NT));
+          f(comp(kh, t + k * NT), (vbits & (1u << k)) != 0u);
         }
       },
       kmax, sc, cpre, cmask);
@@ -1746,8 +1782,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
   } else {
     bm_select_write<NT, BOX>(
         [&](auto f) {
-          for (int i = tid; i < BOX; i += NT)
-            if (valid_cand(i)) f(comp(s_klo[i], i));
+          const int t = fresh_tid();
+          for (int base = 0; base < BOX; base += NT) {
+            const int i = base + t;
+            f(comp(s_klo[min(i, BOX - 1)], i), valid_cand(i));
+          }
         },
         box, acc, a, g, a0, a1, a2, W1, W2, CENTER, sc);
   }
