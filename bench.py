@@ -138,7 +138,7 @@ def fp64_check(profile, noisy: np.ndarray, truth: np.ndarray, side: int) -> dict
     pipe = PoissonPipeline(profile, sample.shape, dev)
     v = torch.from_numpy(sample).to(dev)
     gpu = pipe.run(v)[1].est.cpu().numpy()
-    prev = engine.set_exact_order(ht="serial", wiener="serial")
+    prev = engine.set_exact_order(ht="serial", wiener="serial", transforms="serial")
     try:
         gpu_serial = pipe.run(v)[1].est.cpu().numpy()
     finally:
@@ -148,7 +148,7 @@ def fp64_check(profile, noisy: np.ndarray, truth: np.ndarray, side: int) -> dict
     o64, _ = oracle.bmnd_poisson_f64(sample, nthreads=threads)
     scale = float(np.max(np.abs(o32)))
     return {"sample": f"{side}^{sample.ndim} centre crop",
-            "exact_order": dict(engine.EXACT_ORDER),
+            "exact_order": dict(engine.EXACT_ORDER, wiener_transforms=engine.WIENER_TRANSFORMS[0]),
             "gpu_vs_oracle_max_rel": float(np.max(np.abs(gpu - o32)) / scale),
             "gpu_vs_oracle_bits_differing": int(np.sum(gpu.view(np.int32) != o32.view(np.int32))),
             "gpu_serial_order_vs_oracle_bits_differing":

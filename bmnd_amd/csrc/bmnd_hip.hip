@@ -1835,6 +1835,10 @@ struct FilterArgs {
   // planes and the bias; k_exact_risk adds the exact planes' risk and writes
   // the weights
   int exact_risk;
+  // the Wiener stage's member-axis products on the matrix cores (filter code
+  // bit 10): equal to the serial fmaf chains up to float32 rounding (beside
+  // exact_risk, in what would be padding)
+  int mfma;
   float *egain;    // (nrefs, kcap, P)
   double *erisk;   // (nrefs)
 };
@@ -2739,6 +2743,96 @@ __device__ __forceinline__ void member_transform(float *__restrict__ data, int n
   member_transform_n<NT, P, NMAX, 0>(data, n, M);
 }
 
+// The member-axis transform on the matrix cores: data (n, P) in LDS <- M data,
+// M (n x n) row-major with row stride n in the matrix table (SQ: its entries
+// squared), by v_mfma_f32_16x16x4f32 tiles (lane l holds A[l % 16][l / 16] and
+// B[l / 16][l % 16]; D[i][j] is lane j + 16 (i / 4), register i % 4).  The
+// products are summed in the hardware's order, which equals member_transform's
+// fmaf chain up to float32 rounding.  Wave w takes the column tiles w, w + NW,
+// ... (16 columns each) and every row tile (16 rows); at K step s lane group
+// lk = l / 16 takes k = lk KS + s (the sum runs over every k whatever the
+// order), so a lane's matrix entries are KS consecutive floats of its row.
+// Rows, columns and k past n or P are zero; nothing past them is read.
+template <int NT, int P, int NMAX, int N, bool SQ>
+__device__ __forceinline__ void member_transform_mfma_n(float *__restrict__ data, int n,
+                                                        const float *__restrict__ M) {
+  constexpr int NW = NT / 64;
+  constexpr int CT = (P + 15) / 16;          // column tiles
+  constexpr int CPW = (CT + NW - 1) / NW;    // column tiles per wave
+  constexpr int NN = N > 0 ? N : NMAX;
+  constexpr int RT = (NN + 15) / 16;         // row tiles (at most)
+  constexpr int KS = (NN + 3) / 4;           // K steps (at most)
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int li = lane & 15, lk = lane >> 4;
+  const int ks = N > 0 ? KS : (n + 3) >> 2;
+  const int rt = N > 0 ? RT : (n + 15) >> 4;
+  float am[RT][KS];
+#pragma unroll
+  for (int t = 0; t < RT; ++t) {
+    const int row = 16 * t + li;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const int k = lk * ks + s;
+      float v = 0.0f;
+      if (t < rt && s < ks && row < n && k < n) v = M[row * n + k];
+      am[t][s] = SQ ? v * v : v;
+    }
+  }
+  v4f_t acc[CPW][RT];
+#pragma unroll
+  for (int c = 0; c < CPW; ++c) {
+    const int ct = w + c * NW;
+    const int col = ct * 16 + li;
+    const bool cv = ct < CT && col < P;
+#pragma unroll
+    for (int t = 0; t < RT; ++t) acc[c][t] = v4f_t{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      if (s < ks) {
+        const int k = lk * ks + s;
+        const float b = (cv && k < n) ? data[k * P + col] : 0.0f;
+#pragma unroll
+        for (int t = 0; t < RT; ++t)
+          if (t < rt) acc[c][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(am[t][s], b, acc[c][t], 0, 0, 0);
+      }
+    }
+  }
+  __syncthreads();  // every input read before the outputs overwrite them
+#pragma unroll
+  for (int c = 0; c < CPW; ++c) {
+    const int ct = w + c * NW;
+    const int col = ct * 16 + li;
+    if (ct < CT && col < P) {
+#pragma unroll
+      for (int t = 0; t < RT; ++t) {
+        if (t < rt) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int row = 16 * t + 4 * lk + v;
+            if (row < n) data[row * P + col] = acc[c][t][v];
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+}
+
+template <int NT, int P, int NMAX, bool SQ>
+__device__ __forceinline__ void member_transform_mfma(float *__restrict__ data, int n,
+                                                      const float *__restrict__ M) {
+  if constexpr (NMAX >= 32) {
+    if (n == 32) return member_transform_mfma_n<NT, P, NMAX, 32, SQ>(data, n, M);
+  }
+  if constexpr (NMAX >= 16) {
+    if (n == 16) return member_transform_mfma_n<NT, P, NMAX, 16, SQ>(data, n, M);
+  }
+  if (n == 8) return member_transform_mfma_n<NT, P, NMAX, 8, SQ>(data, n, M);
+  if (n <= 4) return member_transform_mfma_n<NT, P, NMAX, 4, SQ>(data, n, M);
+  member_transform_mfma_n<NT, P, NMAX, 0, SQ>(data, n, M);
+}
+
 // In-place transform along one spatial axis of an LDS array viewed as
 // (nouter, B, INNER): each lane owns whole lines, reads the B inputs of a line
 // into registers and then writes its B outputs (lines are disjoint).
@@ -2786,6 +2880,20 @@ __device__ __forceinline__ void group_transform(float *__restrict__ data, int n,
   line_transform<NT, B2, 1>(data, n * B0 * B1, S[2], epi);
 }
 
+// group_transform with the member axis on the matrix cores, its matrix read
+// from the matrix table (row stride n; SQ: squared entries)
+template <int NT, int B0, int B1, int B2, int NMAX, bool SQ, class Epi = EpiStore>
+__device__ __forceinline__ void group_transform_mfma(float *__restrict__ data, int n,
+                                                     const float *__restrict__ Gtab,
+                                                     const float (*__restrict__ S)[64],
+                                                     Epi epi = Epi()) {
+  static_assert(B2 > 1, "the last spatial axis carries the epilogue");
+  member_transform_mfma<NT, B0 * B1 * B2, NMAX, SQ>(data, n, Gtab);
+  if constexpr (B0 > 1) line_transform<NT, B0, B1 * B2>(data, n, S[0]);
+  if constexpr (B1 > 1) line_transform<NT, B1, B2>(data, n * B0, S[1]);
+  line_transform<NT, B2, 1>(data, n * B0 * B1, S[2], epi);
+}
+
 // Gaussian Wiener gain (wiener.py:56-75): |r|^2 / (|r|^2 + mu max(sigma^2,
 // 1e-8)), 0 where that denominator is not positive, clipped to [0, 1]
 __device__ __forceinline__ float gaussian_gain(float pr, float nv, float mu) {
@@ -2814,9 +2922,13 @@ __device__ __forceinline__ float wiener_signal_power(float r, float nv, float ef
   return sp;
 }
 
+// MF: the member-axis products on the matrix cores never (0), always (1) or
+// as a.mfma says (2)
 template <int B0, int B1, int B2, int R0, int R1, int R2, int NMAX, int NT, bool WIENER,
-          bool SIGMA, bool OPT>
+          bool SIGMA, bool OPT, int MF>
 __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
+  // the matrix-core member transforms of this instance
+  const bool mfma = MF == 1 || (MF == 2 && a.mfma);
   if constexpr (!OPT) {
     // the default filter (hard threshold or Wiener gain, variance weights,
     // patch scope, Poisson model): the option branches fold away and leave
@@ -2989,8 +3101,17 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
     __syncthreads();
   } else {
   // ---- approximate std: squared transforms along every axis (variance.py:613-623)
-  group_transform<NT, B0, B1, B2, NMAX>(s_sig, n, s_g2, s_sf2,
-                                        [](int, float v) { return sqrtf(v > 0.0f ? v : 0.0f); });
+  {
+    auto root = [](int, float v) { return sqrtf(v > 0.0f ? v : 0.0f); };
+    bool done = false;
+    if constexpr (WIENER && MF != 0) {
+      if (mfma) {
+        group_transform_mfma<NT, B0, B1, B2, NMAX, true>(s_sig, n, gf, s_sf2, root);
+        done = true;
+      }
+    }
+    if (!done) group_transform<NT, B0, B1, B2, NMAX>(s_sig, n, s_g2, s_sf2, root);
+  }
 
   // ---- exact overlap-aware planes (k_exact_std) and plane limiting ----------
   if (planes == 0) {
@@ -3110,7 +3231,7 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
     const float mu = a.shrink;
     const int gmode = a.gain_mode;
     const bool gauss = a.gaussian != 0;
-    group_transform<NT, B0, B1, B2, NMAX>(s_r, n, s_g, s_sf, [&](int e, float r) {
+    auto gain = [&](int e, float r) {
       const float sg = s_sig[e];
       const float nv = sg * sg;
       if (gauss) {
@@ -3122,7 +3243,15 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
       const float eff = mu * nv;
       const float sp = wiener_signal_power(r, nv, eff, gmode);
       return risk_model ? sp : wiener_gain(sp, eff);
-    });
+    };
+    bool done = false;
+    if constexpr (MF != 0) {
+      if (mfma) {
+        group_transform_mfma<NT, B0, B1, B2, NMAX, false>(s_r, n, gf, s_sf, gain);
+        done = true;
+      }
+    }
+    if (!done) group_transform<NT, B0, B1, B2, NMAX>(s_r, n, s_g, s_sf, gain);
   }
   // ---- aggregation weights (aggregationweights.py:175-259, 616-653, 656-687):
   // per coefficient the error a^2 (classic), a^2 sigma^2 (variance) or
@@ -3252,8 +3381,15 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
       if (e < np) s_x[e] = xn[it];
     }
     __syncthreads();
-    group_transform<NT, B0, B1, B2, NMAX>(s_x, n, s_g, s_sf,
-                                          [&](int e, float c) { return c * atten[e]; });
+    auto shrink = [&](int e, float c) { return c * atten[e]; };
+    bool done = false;
+    if constexpr (MF != 0) {
+      if (mfma) {
+        group_transform_mfma<NT, B0, B1, B2, NMAX, false>(s_x, n, gf, s_sf, shrink);
+        done = true;
+      }
+    }
+    if (!done) group_transform<NT, B0, B1, B2, NMAX>(s_x, n, s_g, s_sf, shrink);
   }
   // ---- inverse transform, stored from its last pass --------------------------
   float *dst = a.filt + (g * a.max_m) * (int64_t)P;
@@ -3264,10 +3400,18 @@ __global__ __launch_bounds__(NT) void k_filter(FilterArgs a) {
     for (int e = tid; e < np; e += NT) dst[e] = s_x[e];
     return;
   }
-  group_transform<NT, B0, B1, B2, NMAX>(s_x, n, s_gi, s_si, [&](int e, float v) {
+  auto store = [&](int e, float v) {
     dst[e] = v;
     return v;
-  });
+  };
+  bool done = false;
+  if constexpr (WIENER && MF != 0) {
+    if (mfma) {
+      group_transform_mfma<NT, B0, B1, B2, NMAX, false>(s_x, n, gi, s_si, store);
+      done = true;
+    }
+  }
+  if (!done) group_transform<NT, B0, B1, B2, NMAX>(s_x, n, s_gi, s_si, store);
 }
 
 // ---------------------------------------------------------------------------
@@ -4874,7 +5018,7 @@ static int32_t filter_impl(int32_t wiener, const float *noisy, const float *pilo
     return rc;
   }
   if (k < 0) return fail(BMND_HIP_EINVAL, "k must be non-negative");
-  if (wiener < 0 || (wiener & ~0x3ff))
+  if (wiener < 0 || (wiener & ~0x7ff))
     return fail(BMND_HIP_EINVAL, "unknown bits in the filter code");
   const int code = wiener;
   wiener = code & 3;
@@ -4884,6 +5028,8 @@ static int32_t filter_impl(int32_t wiener, const float *noisy, const float *pilo
     return fail(BMND_HIP_EINVAL, "the risk weight model is a Wiener-stage option");
   if ((code & 4) && wiener)
     return fail(BMND_HIP_EINVAL, "soft thresholding is a hard-threshold-stage option");
+  if ((code & 1024) && !wiener)
+    return fail(BMND_HIP_EINVAL, "the matrix-core member transforms are a Wiener-stage option");
   if (nrefs <= 0) return 0;
   if (wiener && !pilot) return fail(BMND_HIP_EINVAL, "Wiener filtering needs the pilot volume");
   if (k > 0 && !exact_std) return fail(BMND_HIP_EINVAL, "k > 0 needs the exact-plane std buffer");
@@ -4924,6 +5070,7 @@ static int32_t filter_impl(int32_t wiener, const float *noisy, const float *pilo
   a.exact_risk = (code & 512) != 0;
   a.egain = egain;
   a.erisk = erisk;
+  a.mfma = (code & 1024) != 0;
   if (a.exact_risk) {
     if (!a.windowed || !a.group_scope || a.kcap < 1 || a.gaussian ||
         !(wmodel == 0 || (wiener && wmodel == 2)))
@@ -4946,25 +5093,28 @@ static int32_t filter_impl(int32_t wiener, const float *noisy, const float *pilo
   hipStream_t s = (hipStream_t)stream;
   dim3 grid((unsigned)nrefs);
   // the option bits (soft threshold, weight model, windowed domain, group
-  // scope, coefficient output, Gaussian model) select the general instance
-  const bool opt = (code & ~3) != 0;
-#define F_LAUNCH(b0, b1, b2, r0, r1, r2, nmax, nt, w, sg, o)                                  \
-  hipLaunchKernelGGL((k_filter<b0, b1, b2, r0, r1, r2, nmax, nt, w, sg, o>), grid, dim3(nt), 0, \
-                     s, a)
+  // scope, coefficient output, Gaussian model) select the general instance;
+  // the matrix-core bit runs in either
+  const bool opt = (code & ~3 & ~1024) != 0;
+#define F_LAUNCH(b0, b1, b2, r0, r1, r2, nmax, nt, w, sg, o, mf)                              \
+  hipLaunchKernelGGL((k_filter<b0, b1, b2, r0, r1, r2, nmax, nt, w, sg, o, mf>), grid, dim3(nt), \
+                     0, s, a)
 #define F_CASE(b0, b1, b2, r0, r1, r2, nmax, nt)                                            \
   if (same3(block_h, b0, b1, b2) && same3(radius_h, r0, r1, r2) && max_matches <= nmax) {   \
     if (wiener && sigma_out)                                                               \
-      F_LAUNCH(b0, b1, b2, r0, r1, r2, nmax, nt, true, true, true);                        \
+      F_LAUNCH(b0, b1, b2, r0, r1, r2, nmax, nt, true, true, true, 2);                     \
     else if (wiener && opt)                                                                \
-      F_LAUNCH(b0, b1, b2, r0, r1, r2, nmax, nt, true, false, true);                       \
+      F_LAUNCH(b0, b1, b2, r0, r1, r2, nmax, nt, true, false, true, 2);                    \
+    else if (wiener && a.mfma)                                                             \
+      F_LAUNCH(b0, b1, b2, r0, r1, r2, nmax, nt, true, false, false, 1);                   \
     else if (wiener)                                                                       \
-      F_LAUNCH(b0, b1, b2, r0, r1, r2, nmax, nt, true, false, false);                      \
+      F_LAUNCH(b0, b1, b2, r0, r1, r2, nmax, nt, true, false, false, 0);                   \
     else if (sigma_out)                                                                    \
-      F_LAUNCH(b0, b1, b2, r0, r1, r2, nmax, nt, false, true, true);                       \
+      F_LAUNCH(b0, b1, b2, r0, r1, r2, nmax, nt, false, true, true, 0);                    \
     else if (opt)                                                                          \
-      F_LAUNCH(b0, b1, b2, r0, r1, r2, nmax, nt, false, false, true);                      \
+      F_LAUNCH(b0, b1, b2, r0, r1, r2, nmax, nt, false, false, true, 0);                   \
     else                                                                                   \
-      F_LAUNCH(b0, b1, b2, r0, r1, r2, nmax, nt, false, false, false);                     \
+      F_LAUNCH(b0, b1, b2, r0, r1, r2, nmax, nt, false, false, false, 0);                  \
     HIP_TRY(hipGetLastError());                                                            \
     return 0;                                                                              \
   }
@@ -5478,6 +5628,7 @@ static int32_t stage_impl(int32_t wiener, const float *match_vol, const float *n
                           int32_t *over_ws = nullptr, void *extra = nullptr,
                           int64_t extra_bytes = 0) {
   const bool pairs = (k & (wiener ? BMND_HIP_EXACT_PAIRS_WIENER : BMND_HIP_EXACT_PAIRS_HT)) != 0;
+  if (wiener && (k & BMND_HIP_MFMA_WIENER)) wiener |= 1024;
   k &= BMND_HIP_K_PLANES;
   const int64_t need = bmnd_hip_stage_workspace(shape_h, block_h, nrefs, max_matches, k);
   if (need < 0) return fail(BMND_HIP_EINVAL, "invalid stage geometry");
@@ -5809,7 +5960,9 @@ int32_t bmnd_hip_pipeline_create_many(const bmnd_hip_pipeline_desc *descs, int32
                                       (d->k & (st ? BMND_HIP_EXACT_PAIRS_WIENER
                                                   : BMND_HIP_EXACT_PAIRS_HT)) != 0))
             return rc;
-        if (int rc = filter_impl(wiener ? d->wiener_gain : 0, d->input,
+        const int32_t fcode =
+            wiener ? d->wiener_gain | ((d->k & BMND_HIP_MFMA_WIENER) ? 1024 : 0) : 0;
+        if (int rc = filter_impl(fcode, d->input,
                                  wiener ? d->pilot : nullptr, var, d->shape, d->block[st],
                                  d->radius[st], d->sel[st], d->counts[st], R, (int32_t)M,
                                  d->matrices[st], exact, d->shrink[st], k, d->count_scale, filt,

@@ -51,25 +51,50 @@ from .utils import kaiser_window_nd
 EXACT_ORDERS = ("serial", "pairs")
 EXACT_ORDER = {"ht": "serial", "wiener": "pairs"}
 K_PAIRS_FLAG = {"ht": 1 << 16, "wiener": 1 << 17}  # BMND_HIP_EXACT_PAIRS_HT / _WIENER
+# The Wiener filter's member-axis products (the group transforms' n x n
+# matrix products): "mfma" on the matrix cores (v_mfma_f32_16x16x4f32, the
+# products summed in the hardware's order: the same values to float32
+# rounding) or "serial" (the reference's fmaf chains, bit for bit the
+# oracle's).  Stage 1 always runs the serial form: its coefficients meet the
+# hard threshold.
+TRANSFORMS = ("serial", "mfma")
+WIENER_TRANSFORMS = ["mfma"]
+K_MFMA_WIENER = 1 << 18       # BMND_HIP_MFMA_WIENER
+FILTER_MFMA = 1024            # filter code bit 10
 
 
-def set_exact_order(ht: str | None = None, wiener: str | None = None) -> dict:
+def set_exact_order(ht: str | None = None, wiener: str | None = None,
+                    transforms: str | None = None) -> dict:
     """Set the exact planes' summation order per stage ("serial" or "pairs")
-    for the calls that follow; returns the previous setting."""
-    prev = dict(EXACT_ORDER)
+    and the Wiener filter's member transforms ("serial" or "mfma") for the
+    calls that follow; returns the previous setting."""
+    prev = dict(EXACT_ORDER, transforms=WIENER_TRANSFORMS[0])
     for name, value in (("ht", ht), ("wiener", wiener)):
         if value is None:
             continue
         if value not in EXACT_ORDERS:
             raise ValueError(f"exact order must be one of {EXACT_ORDERS}, got {value!r}")
         EXACT_ORDER[name] = value
+    if transforms is not None:
+        if transforms not in TRANSFORMS:
+            raise ValueError(f"transforms must be one of {TRANSFORMS}, got {transforms!r}")
+        WIENER_TRANSFORMS[0] = transforms
     return prev
 
 
 def k_with_order(k: int) -> int:
-    """k with the pair-order flags of the current setting OR'd in (the k of
-    bmnd_hip_stage and bmnd_hip_pipeline_desc)."""
-    return int(k) | sum(K_PAIRS_FLAG[st] for st in ("ht", "wiener") if EXACT_ORDER[st] == "pairs")
+    """k with the pair-order and matrix-core flags of the current setting
+    OR'd in (the k of bmnd_hip_stage and bmnd_hip_pipeline_desc)."""
+    k = int(k) | sum(K_PAIRS_FLAG[st] for st in ("ht", "wiener") if EXACT_ORDER[st] == "pairs")
+    return k | (K_MFMA_WIENER if WIENER_TRANSFORMS[0] == "mfma" else 0)
+
+
+def code_with_transforms(code: int, name: str) -> int:
+    """A Wiener stage's filter code with the matrix-core bit of the current
+    setting."""
+    if name == "wiener" and WIENER_TRANSFORMS[0] == "mfma":
+        return int(code) | FILTER_MFMA
+    return int(code)
 
 
 def exact_std_entry(name: str, batch: bool = False):
@@ -679,7 +704,7 @@ class PoissonPipeline:
             gsig = torch.empty(R if wiener else R * plan.max_m * plan.P, dtype=torch.float32,
                                device=dev)
         tok = timer.start(f"{name}.filter")
-        filter_args = (self.filter_codes[name], _ptr(noisy),
+        filter_args = (code_with_transforms(self.filter_codes[name], name), _ptr(noisy),
                        _ptr((match_vol if pilot is None else pilot) if wiener else None),
                        _ptr(var), shape, block, radius, _ptr(sel), _ptr(cnt), R, plan.max_m,
                        _ptr(mats), _ptr(exact), plan.shrink, self.k, self.count_scale,
@@ -1151,7 +1176,7 @@ class BatchPipeline:
         wiener = name == "wiener"
         tok = timer.start(f"{name}.filter")
         _lib.check(L.bmnd_hip_filter_groups_batch(
-            self._single.filter_codes[name], _ptr(noisy),
+            code_with_transforms(self._single.filter_codes[name], name), _ptr(noisy),
             _ptr((match_vol if pilot is None else pilot) if wiener else None), _ptr(var), ih, idv, nin,
             block, radius, _ptr(sel), _ptr(cnt), _ptr(st.ref_input), R, plan.max_m, _ptr(mats),
             _ptr(exact), plan.shrink, k, self._single.count_scale, _ptr(filt), _ptr(wts), stream))

@@ -262,10 +262,13 @@ int32_t bmnd_hip_filter_groups_exact(int32_t code, const float *noisy, const flo
 /* Flags OR'd into the k argument of bmnd_hip_stage, bmnd_hip_stage_noalloc,
  * bmnd_hip_stage_workspace and into bmnd_hip_pipeline_desc.k (the exact
  * planes are k & BMND_HIP_K_PLANES): the HT or the Wiener stage computes its
- * exact planes with bmnd_hip_exact_std_pairs instead of bmnd_hip_exact_std. */
+ * exact planes with bmnd_hip_exact_std_pairs instead of bmnd_hip_exact_std;
+ * BMND_HIP_MFMA_WIENER sets filter code bit 10 (the matrix-core member
+ * transforms) for the Wiener stage. */
 #define BMND_HIP_K_PLANES 0xffff
 #define BMND_HIP_EXACT_PAIRS_HT (1 << 16)
 #define BMND_HIP_EXACT_PAIRS_WIENER (1 << 17)
+#define BMND_HIP_MFMA_WIENER (1 << 18)
 
 /* Collaborative filtering of every group of a stage: gather, separable
  * forward transform, coefficient noise std (the exact planes q < k from
@@ -300,8 +303,12 @@ int32_t bmnd_hip_filter_groups_exact(int32_t code, const float *noisy, const flo
  * (variance.py:2096-2106), the hard threshold keeps |C| >= lambda sigma and
  * the group's first coefficient (bmndalgo.py:470-474), and the Wiener gain
  * is |r|^2 / (|r|^2 + shrink max(sigma^2, 1e-8)) (wiener.py:56-75), its
- * signal power |r|^2 for the risk model.  The same values in every entry
- * that takes a filter code. */
+ * signal power |r|^2 for the risk model.  Bit 10 (Wiener only) runs the
+ * member-axis products of the four group transforms on the matrix cores
+ * (v_mfma_f32_16x16x4f32): the same products summed in the hardware's
+ * order, so the output equals the serial fmaf chains of the reference's
+ * matrix products up to float32 rounding (the bit is off for the serial
+ * order).  The same values in every entry that takes a filter code. */
 int32_t bmnd_hip_filter_groups(int32_t wiener, const float *noisy,
                                const float *pilot, const float *var,
                                const int64_t *shape_h, const int32_t *block_h,

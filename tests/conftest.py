@@ -11,21 +11,23 @@ if ROOT not in sys.path:
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X GPU and the built HIP library")
     config.addinivalue_line("markers", "default_exact_order: run with the library's default "
-                            "exact-plane order (the Wiener planes by member pairs)")
+                            "exact-plane order (the Wiener planes by member pairs) and "
+                            "the Wiener member transforms on the matrix cores")
 
 
 @pytest.fixture(autouse=True)
 def _serial_exact_order(request):
     """GPU tests compare bit for bit with the oracle, which sums the exact
-    planes in the reference's serial order: they run both stages in that
-    order unless marked default_exact_order (those hold the pair order to its
-    own tolerance)."""
+    planes and the member transforms in the reference's serial order: they
+    run both stages in that order unless marked default_exact_order (those
+    hold the pair order and the matrix-core transforms to their own
+    tolerance)."""
     if request.node.get_closest_marker("gpu") is None or \
             request.node.get_closest_marker("default_exact_order") is not None:
         yield
         return
     from bmnd_amd import engine
-    prev = engine.set_exact_order(ht="serial", wiener="serial")
+    prev = engine.set_exact_order(ht="serial", wiener="serial", transforms="serial")
     try:
         yield
     finally:

@@ -155,6 +155,7 @@ def _oracle_compare(noisy, prof, params):
 @pytest.mark.parametrize("case", ["readme-64", "vol-28x32x24", "vol-40", "blob-144"])
 def test_default_order_pipeline_within_tolerance(case):
     assert engine.EXACT_ORDER == {"ht": "serial", "wiener": "pairs"}
+    assert engine.WIENER_TRANSFORMS == ["mfma"]
     print(f"default exact order end to end: {case}", flush=True)
     if case == "readme-64":
         truth, noisy = phantoms.square_image(0)
@@ -226,7 +227,7 @@ def test_stage_driver_pair_flag():
     refs = pipe._ref_lin(plan)
     R = refs.numel()
     k = engine.k_with_order(pipe.k)
-    assert k == pipe.k | (1 << 17)
+    assert k == pipe.k | (1 << 17) | (1 << 18)
     need = L.bmnd_hip_stage_workspace(shape, block, R, plan.max_m, k)
     assert need == L.bmnd_hip_stage_workspace(shape, block, R, plan.max_m, pipe.k)
     ws = torch.empty(need, dtype=torch.uint8, device=DEV)
@@ -256,7 +257,7 @@ def test_bmnd_default_order_against_serial():
                                              axes_range=(4.0, 20.0))
     prof = BM4DProfile(noise_model=NoiseModel.POISSON)
     out_p = bmnd(noisy, prof)
-    prev = engine.set_exact_order(wiener="serial")
+    prev = engine.set_exact_order(wiener="serial", transforms="serial")
     try:
         out_s = bmnd(noisy, prof)
     finally:

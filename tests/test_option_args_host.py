@@ -56,7 +56,8 @@ def test_thresholds_entry_needs_the_array():
 
 
 @pytest.mark.parametrize("code,msg", [
-    (-1, "unknown bits"), (1024, "unknown bits"), (3 << 3, "weight model must be"),
+    (-1, "unknown bits"), (2048, "unknown bits"), (3 << 3, "weight model must be"),
+    (1024, "matrix-core member transforms are a Wiener-stage option"),
     (2 << 3, "risk weight model is a Wiener-stage option"),
     (1 | 4, "soft thresholding is a hard-threshold-stage option"),
     (512, "the exact group risk needs the variance"),
