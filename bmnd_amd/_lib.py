@@ -27,6 +27,7 @@ EXPORTS = (
     "bmnd_hip_blockmatch_thresholds",
     "bmnd_hip_blockmatch_standardized",
     "bmnd_hip_blockmatch_ssd",
+    "bmnd_hip_blockmatch_ssd_poisson",
     "bmnd_hip_value_range",
     "bmnd_hip_gaussian_variance",
     "bmnd_hip_median_mad",
@@ -159,6 +160,9 @@ def _declare(L: ctypes.CDLL) -> None:
     L.bmnd_hip_blockmatch_ssd.argtypes = [vp, vp, vp, vp, vp, i64, f32, i32, i32, i32, vp, f32,
                                           vp, vp, vp, vp]
     L.bmnd_hip_blockmatch_ssd.restype = i32
+    L.bmnd_hip_blockmatch_ssd_poisson.argtypes = [vp, vp, vp, vp, vp, i64, f32, i32, i32, i32,
+                                                  f32, f64, vp, vp, vp, vp]
+    L.bmnd_hip_blockmatch_ssd_poisson.restype = i32
     L.bmnd_hip_value_range.argtypes = [vp, i64, vp, vp]
     L.bmnd_hip_value_range.restype = i32
     L.bmnd_hip_gaussian_variance.argtypes = [vp, vp, vp, vp, vp, vp, vp, i64, i32, vp, i32, vp,

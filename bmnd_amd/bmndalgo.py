@@ -5,8 +5,8 @@ conventions as the reference's src/bmnd/bmndalgo.py:71-183; the work runs on
 the MI355X device pipelines (engine.PoissonPipeline for Poisson noise,
 gaussian.GaussianPipeline for stationary Gaussian noise).  Profile options
 the device pipelines do not run (engine.check_supported: the exact
-covariance-aware group risk, the Poisson SSD distance, non-SSD distances
-under Gaussian noise, transforms and geometries not compiled in) raise
+covariance-aware group risk under Gaussian noise, non-SSD distances under
+Gaussian noise, transforms and geometries not compiled in) raise
 NotImplementedError after the reference's own validation has passed.
 """
 
@@ -256,8 +256,9 @@ def bmnd_batch(volumes, profile, sigma: float | None = None,
     if live:
         device = torch.device("cuda", torch.cuda.current_device())
         probe = _pipeline(profile, items[live[0]][0].shape, device)
-        if any(probe.mass.values()) or any(probe.exact_risk.values()):
-            # group-mass conservation and the exact group risk run input by input
+        if any(probe.mass.values()) or any(probe.exact_risk.values()) or probe.ssd:
+            # group-mass conservation, the exact group risk and the SSD
+            # distance run input by input
             for i in live:
                 vol = np.ascontiguousarray(items[i][0])
                 pipe = _pipeline(profile, vol.shape, device)

@@ -372,6 +372,9 @@ struct BMArgs {
   // its factor gamma, subtracted from each candidate's SSD; nullptr: none
   const float *noise_ssd;
   float gamma;
+  // MODE 11 (SSD, Poisson gamma correction): gamma times the Poisson noise
+  // bias, pscale times the positive pooled values, off every candidate's SSD
+  double pscale;
 };
 
 // A batched launch takes the geometry of reference g's input and moves the
@@ -542,11 +545,12 @@ __device__ __forceinline__ void wave_hist_add(int *hist, int bin) {
 // own as f(key, valid), the same number of calls in every thread): on return
 // the K smallest are exactly those with
 // (key & mask) <= prefix.  The candidate index is below 2^16, so bits 16..31
-// are zero in every key and their passes are skipped; once the bin holding
+// are zero in every key and their passes are skipped (MID: they hold an
+// order field between the key and the index, and are resolved); once the bin holding
 // the K-th key is needed in full the select stops with the bits resolved so
 // far.  STOP > 0 resolves only the bits from STOP up: the K smallest then
 // all satisfy (key & mask) <= prefix, which is what a cutoff needs.
-template <int NT, int BOX, int STOP = 0, class Each>
+template <int NT, int BOX, int STOP = 0, bool MID = false, class Each>
 __device__ __forceinline__ void bm_radix_select(Each each, int K, const BMSelScratch &sc,
                                                 unsigned long long &prefix_out,
                                                 unsigned long long &mask_out) {
@@ -555,7 +559,7 @@ __device__ __forceinline__ void bm_radix_select(Each each, int K, const BMSelScr
   unsigned long long prefix = 0ull, mask = 0ull;
   int krem = K;
   for (int shift = 56; shift >= STOP; shift -= 8) {
-    if (shift == 24 || shift == 16) {
+    if (!MID && (shift == 24 || shift == 16)) {
       mask |= 255ull << shift;
       continue;
     }
@@ -615,14 +619,14 @@ __device__ __forceinline__ void wave_atomic_add(int *dst, int v) {
 // (unique) and the accepted count, keep the K smallest the reference keeps,
 // order them and write the group of reference g: reference first, dedupe,
 // cap, power-of-two truncation (blockmatching.py:913-954, 1045-1205).
-template <int NT, int BOX, class Each>
+template <int NT, int BOX, bool MID = false, class Each>
 __device__ __forceinline__ void bm_select_write(Each each, int box, int acc, const BMArgs &a,
                                                 int64_t g, int lo0, int lo1, int lo2, int e1,
                                                 int e2, int center_local, const BMSelScratch &sc) {
   const int tid = threadIdx.x;
   const int K = bm_keep_count(acc, box, a);
   unsigned long long prefix, mask;
-  bm_radix_select<NT, BOX>(each, K, sc, prefix, mask);
+  bm_radix_select<NT, BOX, 0, MID>(each, K, sc, prefix, mask);
   if (tid == 0) *sc.cnt = 0;
   __syncthreads();
   each([&](unsigned long long k, bool v) {
@@ -659,7 +663,7 @@ __device__ __forceinline__ void bm_finalize(int K, int acc, const BMArgs &a, int
     // at max, then power-of-two truncation; lane i holds the i-th smallest
     const int lane = tid;
     const bool valid = lane < K;
-    const int loc = valid ? (int)(sc.sorted[lane] & 0xffffffffull) : -1;
+    const int loc = valid ? (int)(sc.sorted[lane] & 0xffffull) : -1;
     const bool keep = valid && loc != center_local;
     const unsigned long long kept = __ballot(keep);
     const int pos = 1 + __popcll(kept & ((1ull << lane) - 1ull));
@@ -961,7 +965,18 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(
 // once and forms C * B2 terms, each candidate's sum kept in its own register
 // in the serial order.  The slab is padded so that a run's reads past the
 // box end stay in the array (those candidates are discarded).
-template <int B0, int B1, int B2, int R0, int R1, int R2, int C, int NT>
+//
+// PB: stage 1 of the Poisson model under the SSD distance (blockmatching.py:
+// 432-443, 1140-1145): every candidate but the reference has gamma times its
+// Poisson noise bias taken off, the bias being the float64 serial sum of
+// pscale * (r + c) over the patch where the float32 r + c is positive.  The
+// corrected distance K is then a float64 that the reference compares with the
+// threshold as such and orders against the float32 values S = f32(K) its
+// buffers keep: a later candidate goes first iff its K is below the earlier
+// one's S.  That is the total order (S, K >= S, index for K >= S, reversed
+// index for K < S), which the composite holds in bits 63..32, 31 and 30..16
+// (the index stays in bits 15..0), so the selection resolves bits 16..31 too.
+template <int B0, int B1, int B2, int R0, int R1, int R2, int C, int NT, bool PB = false>
 __global__ __launch_bounds__(NT) void k_blockmatch_ssd(BMArgs a) {
   constexpr int S0 = 2 * R0 + B0, S1 = 2 * R1 + B1, S2 = 2 * R2 + B2;
   constexpr int SLAB = S0 * S1 * S2;
@@ -972,6 +987,7 @@ __global__ __launch_bounds__(NT) void k_blockmatch_ssd(BMArgs a) {
   __shared__ float s_slab[SLAB + C];
   __shared__ float s_ref[P];
   __shared__ float s_keys[BOX];
+  __shared__ unsigned char s_ge[PB ? BOX : 1];  // PB: K >= S
   __shared__ int s_acc;
 
   const int tid = threadIdx.x;
@@ -1023,8 +1039,13 @@ __global__ __launch_bounds__(NT) void k_blockmatch_ssd(BMArgs a) {
     const int j2s = seg * C;
     const float *base = s_slab + (j0 * S1 + j1) * S2 + j2s;
     float d[C];
+    double bias[PB ? C : 1];
 #pragma unroll
     for (int c = 0; c < C; ++c) d[c] = 0.0f;
+    if constexpr (PB) {
+#pragma unroll
+      for (int c = 0; c < C; ++c) bias[c] = 0.0;
+    }
 #pragma unroll 1
     for (int p01 = 0; p01 < B0 * B1; ++p01) {
       const int p0 = p01 / B1, p1 = p01 - p0 * B1;
@@ -1040,12 +1061,28 @@ __global__ __launch_bounds__(NT) void k_blockmatch_ssd(BMArgs a) {
         for (int c = 0; c < C; ++c) {
           const float df = sv[c + p2] - rv[p2];
           d[c] = d[c] + df * df;
+          if constexpr (PB) {
+            const float pooled = rv[p2] + sv[c + p2];
+            if (pooled > 0.0f) bias[c] = bias[c] + a.pscale * (double)pooled;
+          }
         }
       }
     }
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       const int j2 = j2s + c;
+      if constexpr (PB) {
+        if (j2 < e2) {
+          const int li = (j0 * e1 + j1) * e2 + j2;
+          double k64 = (double)d[c];
+          if (li != center_local) k64 = k64 - (double)a.gamma * bias[c];
+          const float s32 = (float)k64;
+          s_keys[li] = s32;
+          s_ge[li] = k64 >= (double)s32 ? 1 : 0;
+          my_acc += k64 < (double)a.threshold ? 1 : 0;
+        }
+        continue;
+      }
       if (j2 < e2) {
         float key = d[c];
         if (a.noise_ssd) {
@@ -1062,9 +1099,18 @@ __global__ __launch_bounds__(NT) void k_blockmatch_ssd(BMArgs a) {
   wave_atomic_add(&s_acc, my_acc);
   __syncthreads();
   BM_SEL_SCRATCH(sc);
-  auto composite = [&](int i) { return bm_composite_signed(s_keys[i], i); };
-  bm_select_write<NT, BOX>(bm_box_keys<NT>(composite, box), box, s_acc, a, g, lo0, lo1, lo2, e1,
-                           e2, center_local, sc);
+  if constexpr (PB) {
+    auto composite = [&](int i) {
+      const unsigned ord = s_ge[i] ? (0x8000u | (unsigned)i) : (0x7fffu - (unsigned)i);
+      return bm_composite_signed(s_keys[i], i) | ((unsigned long long)ord << 16);
+    };
+    bm_select_write<NT, BOX, true>(bm_box_keys<NT>(composite, box), box, s_acc, a, g, lo0, lo1,
+                                   lo2, e1, e2, center_local, sc);
+  } else {
+    auto composite = [&](int i) { return bm_composite_signed(s_keys[i], i); };
+    bm_select_write<NT, BOX>(bm_box_keys<NT>(composite, box), box, s_acc, a, g, lo0, lo1, lo2, e1,
+                             e2, center_local, sc);
+  }
 }
 
 // Run length and workgroup size of the SSD matcher: whole box lines of 15
@@ -4504,7 +4550,7 @@ static int32_t blockmatch_impl(const float *vol, const int64_t *shape_h, const i
                                const float *null_mom = nullptr, int32_t null_max = 0,
                                void *ws = nullptr, int64_t ws_bytes = 0,
                                const float *noise_ssd = nullptr, float gamma = 0.0f,
-                               bool ssd = false) {
+                               bool ssd = false, bool pbias = false) {
   int c[3];
   int64_t shape1[3];
   int64_t grid_total = 0;
@@ -4529,9 +4575,12 @@ static int32_t blockmatch_impl(const float *vol, const int64_t *shape_h, const i
     return fail(BMND_HIP_EINVAL, "standardized modes need the null-moment table (max count >= 1)");
   if (mode == 2 && (count_scale != 1.0 || table_max < 0 || table_max > kMaxTableCount))
     return fail(BMND_HIP_EINVAL, "table mode needs count_scale == 1 and 0 <= table_max <= 100");
+  double pscale = 1.0;
   if (ssd) {
-    // the Gaussian model's SSD (MODE 10): no tie grid, no count scale
-    mode = 10;
+    // SSD (MODE 10; MODE 11 with the Poisson gamma correction): no tie grid;
+    // the count scale only scales the Poisson noise bias
+    pscale = count_scale;
+    mode = pbias ? 11 : 10;
     stabilize = 0;
     count_scale = 1.0;
   }
@@ -4595,6 +4644,7 @@ static int32_t blockmatch_impl(const float *vol, const int64_t *shape_h, const i
   a.null_max = null_max;
   a.noise_ssd = noise_ssd;
   a.gamma = gamma;
+  a.pscale = pscale;
   hipStream_t s = (hipStream_t)stream;
   dim3 grid((unsigned)nrefs), blk(kBMThreads);
   const size_t dyn = mode == 2 ? sizeof(double) * (size_t)(table_max + 1) * (table_max + 1) : 0;
@@ -4678,6 +4728,10 @@ static int32_t blockmatch_impl(const float *vol, const int64_t *shape_h, const i
       hipLaunchKernelGGL((k_blockmatch_ssd<b0, b1, b2, r0, r1, r2, SsdRun<b2, r2>::C,             \
                                            SsdRun<b2, r2>::NT>),                                  \
                          grid, dim3(SsdRun<b2, r2>::NT), 0, s, a);                               \
+    else if (mode == 11)                                                                         \
+      hipLaunchKernelGGL((k_blockmatch_ssd<b0, b1, b2, r0, r1, r2, SsdRun<b2, r2>::C,             \
+                                           SsdRun<b2, r2>::NT, true>),                            \
+                         grid, dim3(SsdRun<b2, r2>::NT), 0, s, a);                               \
     else if (mode >= 7)                                                                          \
       hipLaunchKernelGGL((k_blockmatch<b0, b1, b2, r0, r1, r2, 6, kBMThreads>), grid, blk, 0, s, \
                          a);                                                                     \
@@ -4737,6 +4791,25 @@ int32_t bmnd_hip_blockmatch_ssd(const float *vol, const int64_t *shape_h, const 
                          max_matches, min_matches, pow2, 0, 1.0, 0, 0, sel, counts, accepted,
                          stream, nullptr, nullptr, nullptr, 0, nullptr, 0,
                          gamma != 0.0f ? noise_ssd : nullptr, gamma, true);
+}
+
+int32_t bmnd_hip_blockmatch_ssd_poisson(const float *vol, const int64_t *shape_h,
+                                        const int32_t *block_h, const int32_t *radius_h,
+                                        const int32_t *ref_lin, int64_t nrefs, float threshold,
+                                        int32_t max_matches, int32_t min_matches, int32_t pow2,
+                                        float gamma, double count_scale, int32_t *sel,
+                                        int32_t *counts, int32_t *accepted, void *stream) {
+  // blockmatching.py:212-257, 432-443, 1140-1145
+  if (!std::isfinite(gamma) || gamma < 0.0f)
+    return fail(BMND_HIP_EINVAL, "gamma must be finite and non-negative");
+  if (!(count_scale > 0.0) || !std::isfinite(count_scale))
+    return fail(BMND_HIP_EINVAL, "poisson_count_scale must be finite and positive");
+  if (!std::isfinite(threshold) || threshold < 0.0f)
+    return fail(BMND_HIP_EINVAL, "distance_threshold values must be finite and non-negative");
+  return blockmatch_impl(vol, shape_h, block_h, radius_h, ref_lin, nrefs, threshold,
+                         max_matches, min_matches, pow2, 0, count_scale, 0, 0, sel, counts,
+                         accepted, stream, nullptr, nullptr, nullptr, 0, nullptr, 0, nullptr,
+                         gamma, true, gamma != 0.0f);
 }
 
 static int32_t gaussian_variance_impl(const float *cov, const int32_t *domain_h,

@@ -271,8 +271,9 @@ def check_supported(profile: BMNDProfile, resolved) -> None:
         if resolved.blockmatch_distance is not BlockMatchDistance.SSD:
             raise NotImplementedError("the Gaussian model runs with the 'ssd' block-matching "
                                       "distance on the MI355X path.")
-    elif resolved.blockmatch_distance not in _DISTANCE_CODES:
-        raise NotImplementedError("the 'poisson_deviance', 'pearson' and 'anscombe_ssd' "
+    elif resolved.blockmatch_distance not in _DISTANCE_CODES \
+            and resolved.blockmatch_distance is not BlockMatchDistance.SSD:
+        raise NotImplementedError("the 'poisson_deviance', 'pearson', 'anscombe_ssd' and 'ssd' "
                                   "block-matching distances run on the MI355X path.")
     def field(name, enum):
         return coerce_enum(getattr(profile, name), enum, name)
@@ -517,6 +518,11 @@ class PoissonPipeline:
         self.distance = resolve_blockmatch_distance(profile.blockmatch_distance, NoiseModel.POISSON)
         # candidate-standardized stage-1 distance (bmndalgo.py:311-316)
         self.standardized = _ht_policy(profile) is PoissonHTMatchPolicy.CANDIDATE_STANDARDIZED
+        # the SSD distance: thresholds scaled by the matched volume's squared
+        # value range, stage 1 with the Poisson gamma correction
+        # (bmndalgo.py:285-294, 321-322)
+        self.ssd = self.distance is BlockMatchDistance.SSD
+        self.gamma = float(profile.gamma) if self.ssd else 0.0
         # the filters' stage argument: 0 hard threshold, 1-3 the Wiener gain
         self.wiener_arg = _GAIN_ARG[coerce_enum(profile.poisson_wiener_gain_mode,
                                                 PoissonWienerGainMode, "poisson_wiener_gain_mode")]
@@ -581,7 +587,10 @@ class PoissonPipeline:
     def blockmatch_mode(self, vol: torch.Tensor, name: str = "wiener") -> tuple[int, int]:
         """(kernel mode, table max) for matching stage `name` on `vol`: the
         deviance modes below, 5 / 6 for the Pearson / Anscombe SSD distances,
-        7-9 for the standardized stage-1 distances."""
+        7-9 for the standardized stage-1 distances, 10 for SSD and 11 for
+        SSD with the Poisson gamma correction (stage 1, gamma != 0)."""
+        if self.ssd:
+            return (11 if name == "ht" and self.gamma != 0.0 else 10), 0
         code = _DISTANCE_CODES[self.distance]
         if name == "ht" and self.standardized:
             return 6 + code, 0
@@ -606,6 +615,19 @@ class PoissonPipeline:
         if is_int and vmax <= 100:
             return 2, vmax
         return fallback, 0
+
+    def ssd_threshold(self, name: str, vol: torch.Tensor) -> float:
+        """The stage's SSD threshold: the profile's times the squared value
+        range of the matched volume, as float32 (blockmatching.py:168-184)."""
+        rng = torch.empty(2, dtype=torch.float32, device=self.device)
+        _lib.check(_lib.lib().bmnd_hip_value_range(_ptr(vol), vol.numel(), _ptr(rng),
+                                                   self._stream()))
+        lo, hi = (np.float32(x) for x in rng.cpu().numpy())
+        scale = float(hi - lo)
+        if scale <= 0.0:
+            scale = 1.0
+        thr = float(getattr(self.profile, f"{name}_match_threshold"))
+        return float(np.float32(thr * scale * scale))
 
     def variance(self, vol: torch.Tensor) -> torch.Tensor:
         out = torch.empty_like(vol)
@@ -664,7 +686,17 @@ class PoissonPipeline:
         mode, tmax = self.blockmatch_mode(match_vol, name)
         self.last_modes[name] = mode
         tok = timer.start(f"{name}.blockmatch")
-        if mode >= 7:
+        if mode == 11:
+            _lib.check(L.bmnd_hip_blockmatch_ssd_poisson(
+                _ptr(match_vol), shape, block, radius, _ptr(refs), R,
+                self.ssd_threshold(name, match_vol), plan.max_m, plan.min_m, 1,
+                self.gamma, self.count_scale, _ptr(sel), _ptr(cnt), _ptr(acc), stream))
+        elif mode == 10:
+            _lib.check(L.bmnd_hip_blockmatch_ssd(
+                _ptr(match_vol), shape, block, radius, _ptr(refs), R,
+                self.ssd_threshold(name, match_vol), plan.max_m, plan.min_m, 1, None, 0.0,
+                _ptr(sel), _ptr(cnt), _ptr(acc), stream))
+        elif mode >= 7:
             null, null_max = self.null_moments()
             _lib.check(L.bmnd_hip_blockmatch_standardized(
                 _ptr(match_vol), shape, block, radius, _ptr(refs), R, plan.threshold, plan.max_m,
@@ -826,6 +858,7 @@ class PoissonPipeline:
         maps, callbacks and the non-default thresholding and weights are served
         by the call-by-call path."""
         return (_ht_policy(self.profile) is PoissonHTMatchPolicy.FIXED and not self.standardized
+                and not self.ssd
                 and self.filter_codes == {"ht": 0, "wiener": self.wiener_arg})
 
     def _graph_modes(self, vol: torch.Tensor) -> tuple[int, int, int]:
@@ -1045,6 +1078,9 @@ class BatchPipeline:
         if any(self._single.exact_risk.values()):
             raise NotImplementedError("the exact covariance-aware group risk runs input by input "
                                       "(bmnd()); bmnd_batch does not take it.")
+        if self._single.ssd:
+            raise NotImplementedError("the SSD distance under the Poisson model runs input by "
+                                      "input (bmnd()); BatchPipeline does not take it.")
         self.stages = {name: self._stage_view(profile, name) for name in ("ht", "wiener")}
         self.last_runs: dict[str, list] = {}  # (mode, first input, end) per matching launch
 

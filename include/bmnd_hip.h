@@ -165,6 +165,29 @@ int32_t bmnd_hip_blockmatch_ssd(const float *vol, const int64_t *shape_h,
                                 const float *noise_ssd, float gamma, int32_t *sel,
                                 int32_t *counts, int32_t *accepted, void *stream);
 
+/* Stage-1 SSD block matching of the Poisson model (blockmatch_distance
+ * "ssd" with noise_model "poisson": bmndalgo.py:292-294, 321-322) with the
+ * Poisson gamma correction (blockmatching.py:432-443, 1140-1145): every
+ * candidate other than the reference has gamma times its Poisson noise bias
+ * (the float64 serial sum over the patch of count_scale * (r + c) where the
+ * float32 r + c is positive) taken off its float32 SSD.  The corrected
+ * distance is a float64, accepted when below threshold and ordered as the
+ * reference's float32 candidate buffers order it (a later candidate goes
+ * first when its float64 distance is below the earlier one's float32 value);
+ * then the same group assembly as bmnd_hip_blockmatch.  gamma == 0 is plain
+ * SSD.  The threshold is the profile's times the squared value range of vol
+ * (blockmatching.py:168-184).  Refuses a negative or non-finite gamma or
+ * threshold and a count_scale that is not finite and positive.  Replaces
+ * blockmatching.blockmatch_groups (blockmatching.py:1393-1505) in stage 1 of
+ * the Poisson path under the SSD distance; stage 2 is bmnd_hip_blockmatch_ssd
+ * without a noise SSD (bmndalgo.py:728-741). */
+int32_t bmnd_hip_blockmatch_ssd_poisson(const float *vol, const int64_t *shape_h,
+                                        const int32_t *block_h, const int32_t *radius_h,
+                                        const int32_t *ref_lin, int64_t nrefs, float threshold,
+                                        int32_t max_matches, int32_t min_matches, int32_t pow2,
+                                        float gamma, double count_scale, int32_t *sel,
+                                        int32_t *counts, int32_t *accepted, void *stream);
+
 /* Median and median absolute deviation behind the Gaussian sigma estimate
  * (sigmaestimation.py:7-45): of the finite nonzero values of vol (the finite
  * ones when every finite value is zero), out (device, 2 floats) <- numpy's

@@ -95,6 +95,7 @@ class OracleParams(ctypes.Structure):
         ("ht_noise_ssd", ctypes.c_void_p),
         ("ht_gamma", ctypes.c_float),
         ("global_sigma", ctypes.c_double),
+        ("poisson_ssd", ctypes.c_int),
     ]
 
 
@@ -136,6 +137,8 @@ def _declare(L: ctypes.CDLL) -> None:
                                                  vp, vp, vp]
     L.oracle_blockmatch_ssd.argtypes = [vp, i32, vp, vp, vp, vp, i64, vp, vp, i32, i32, i32,
                                         vp, f32, i32, vp, vp, vp]
+    L.oracle_blockmatch_ssd_poisson.argtypes = [vp, i32, vp, vp, vp, vp, i64, vp, vp, i32, i32,
+                                                i32, f32, f64, i32, vp, vp, vp]
     L.oracle_gaussian_group_variance.argtypes = [vp, i32, i64, vp, vp, i32, i32, vp]
     L.oracle_ht_gaussian.argtypes = [vp, vp, i64, f32, vp]
     L.oracle_wiener_gaussian.argtypes = [vp, vp, vp, i64, f32, vp, vp]
@@ -887,6 +890,39 @@ def blockmatch_ssd(vol, block, radius, refs, threshold, max_matches, min_matches
                                 _p(thr), max_matches, min_matches, int(pow2), _p(ns),
                                 float(gamma), nthreads, _p(sel), _p(cnt), _p(acc))
     return sel, cnt, acc
+
+
+def blockmatch_ssd_poisson(vol, block, radius, refs, threshold, max_matches, min_matches,
+                           gamma, count_scale=1.0, pow2=True, nthreads=0):
+    """Stage-1 SSD block matching of the Poisson model with the Poisson gamma
+    correction (blockmatching.py:432-443, 1046-1205): (sel (nrefs, max) padded
+    with -1, counts, accepted) for the patch-grid references refs (nrefs, ndim)."""
+    v = np.ascontiguousarray(vol, dtype=np.float32)
+    nd = v.ndim
+    shp = np.asarray(v.shape, dtype=np.int64)
+    blk = np.asarray(block, dtype=np.int32)
+    cts = np.asarray([v.shape[d] - block[d] + 1 for d in range(nd)], dtype=np.int64)
+    rad = np.asarray(radius, dtype=np.int32)
+    r = np.ascontiguousarray(refs, dtype=np.int64).reshape(-1, nd)
+    thr = np.asarray(threshold, dtype=np.float32)
+    thr = np.ascontiguousarray(np.broadcast_to(thr, (len(r),)), dtype=np.float32)
+    sel = np.empty((len(r), max_matches), np.int64)
+    cnt = np.empty(len(r), np.int32)
+    acc = np.empty(len(r), np.int64)
+    lib().oracle_blockmatch_ssd_poisson(_p(v), nd, _p(shp), _p(cts), _p(blk), _p(r), len(r),
+                                        _p(rad), _p(thr), max_matches, min_matches, int(pow2),
+                                        float(np.float32(gamma)), float(count_scale), nthreads,
+                                        _p(sel), _p(cnt), _p(acc))
+    return sel, cnt, acc
+
+
+def set_poisson_ssd(params, *, ht_match, wi_match, gamma) -> None:
+    """Switch the Poisson params to the SSD distance: the profile's match
+    thresholds (they scale with the matched volume's squared range) and the
+    stage-1 Poisson gamma (0: plain SSD)."""
+    params.poisson_ssd = 1
+    params.ht_gamma = float(np.float32(gamma))
+    params.ht_match_d, params.wi_match_d = float(ht_match), float(wi_match)
 
 
 def gaussian_group_variance(maps, positions, group_matrix, k: int) -> np.ndarray:

@@ -482,6 +482,52 @@ static int insert_limited(int64_t *lins, float *dists, int limit, int size,
   return size;
 }
 
+/* insert_limited for a float64 candidate distance against the float32
+ * distances the buffer stores: the Poisson SSD correction makes the
+ * candidate distance a float64 (gamma times the float64 noise bias,
+ * blockmatching.py:432-443, 1140-1145), while the buffers are float32 arrays
+ * (blockmatching.py:1099-1102), so a candidate precedes an entry when its
+ * float64 value is below the entry's stored float32 value, or equal to it
+ * with the smaller index (blockmatching.py:967-1030, no tie grid for SSD). */
+static int mixed_precedes(int64_t cl, double cd, int64_t el, float ed) {
+  if (cd < (double)ed) return 1;
+  if (cd > (double)ed) return 0;
+  return cl < el;
+}
+
+static int insert_limited_mixed(int64_t *lins, float *dists, int limit, int size, int64_t cl,
+                                double cd) {
+  if (limit == 0) return size;
+  int at;
+  if (size < limit) {
+    ++size;
+    at = size - 1;
+  } else {
+    if (!mixed_precedes(cl, cd, lins[limit - 1], dists[limit - 1])) return size;
+    at = limit - 1;
+  }
+  while (at > 0 && mixed_precedes(cl, cd, lins[at - 1], dists[at - 1])) {
+    lins[at] = lins[at - 1];
+    dists[at] = dists[at - 1];
+    --at;
+  }
+  lins[at] = cl;
+  dists[at] = (float)cd;
+  return size;
+}
+
+/* Poisson SSD noise bias of a candidate (blockmatching.py:432-443): over the
+ * patch in C order, the float32 pooled value r + c, and where it is positive
+ * count_scale times it added to a float64 sum. */
+static double poisson_ssd_bias(const float *ref, const float *cand, int p, double scale) {
+  double bias = 0.0;
+  for (int i = 0; i < p; ++i) {
+    const float pooled = ref[i] + cand[i];
+    if (pooled > 0.0f) bias += scale * (double)pooled;
+  }
+  return bias;
+}
+
 static int largest_pow2_at_most(int v) {
   if (v <= 0) return 0;
   int p = 1;
@@ -537,8 +583,9 @@ static void blockmatch_core(const float *patches_flat, const float *volume, int 
                             const float *thresholds, int max_matches, int min_matches, int pow2,
                             int stabilize, int code, double count_scale,
                             const float *null_means, const float *null_vars, int null_max,
-                            const float *noise_ssd, float gamma, int nthreads, int64_t *sel_out,
-                            int32_t *count_out, int64_t *accepted_out);
+                            const float *noise_ssd, float gamma, float pgamma, double pscale,
+                            int nthreads, int64_t *sel_out, int32_t *count_out,
+                            int64_t *accepted_out);
 
 /* oracle_blockmatch; with null_means the candidate-standardized distance
  * (blockmatching.py:1128-1145) */
@@ -553,8 +600,8 @@ void oracle_blockmatch_standardized(const float *patches_flat, const float *volu
                                     int64_t *accepted_out) {
   blockmatch_core(patches_flat, volume, ndim, vol_shape, counts, block, refs, nrefs, radius,
                   thresholds, max_matches, min_matches, pow2, stabilize, code, count_scale,
-                  null_means, null_vars, null_max, NULL, 0.0f, nthreads, sel_out, count_out,
-                  accepted_out);
+                  null_means, null_vars, null_max, NULL, 0.0f, 0.0f, 1.0, nthreads, sel_out,
+                  count_out, accepted_out);
 }
 
 /* SSD matching with the Gaussian noise-bias correction (blockmatching.py:
@@ -569,7 +616,26 @@ void oracle_blockmatch_ssd(const float *volume, int ndim, const int64_t *vol_sha
                            int64_t *accepted_out) {
   blockmatch_core(NULL, volume, ndim, vol_shape, counts, block, refs, nrefs, radius, thresholds,
                   max_matches, min_matches, pow2, 0, ORACLE_DIST_SSD, 1.0, NULL, NULL, 0,
-                  noise_ssd, gamma, nthreads, sel_out, count_out, accepted_out);
+                  noise_ssd, gamma, 0.0f, 1.0, nthreads, sel_out, count_out, accepted_out);
+}
+
+/* SSD matching of the Poisson model's stage 1 with the Poisson gamma
+ * correction (bmndalgo.py:292-294, 321-322; blockmatching.py:1140-1145):
+ * every candidate other than the reference itself has gamma times its
+ * Poisson noise bias (poisson_ssd_bias, count_scale = the profile's
+ * poisson_count_scale) taken off its float32 SSD.  gamma is a float32 and the
+ * bias a float64, so the corrected distance is a float64: it is compared with
+ * the float32 threshold as a float64 and ordered against the float32 values
+ * the buffers hold (insert_limited_mixed).  gamma == 0 is plain SSD. */
+void oracle_blockmatch_ssd_poisson(const float *volume, int ndim, const int64_t *vol_shape,
+                                   const int64_t *counts, const int *block, const int64_t *refs,
+                                   int64_t nrefs, const int *radius, const float *thresholds,
+                                   int max_matches, int min_matches, int pow2, float gamma,
+                                   double count_scale, int nthreads, int64_t *sel_out,
+                                   int32_t *count_out, int64_t *accepted_out) {
+  blockmatch_core(NULL, volume, ndim, vol_shape, counts, block, refs, nrefs, radius, thresholds,
+                  max_matches, min_matches, pow2, 0, ORACLE_DIST_SSD, 1.0, NULL, NULL, 0, NULL,
+                  0.0f, gamma, count_scale, nthreads, sel_out, count_out, accepted_out);
 }
 
 static void blockmatch_core(const float *patches_flat, const float *volume, int ndim,
@@ -578,8 +644,9 @@ static void blockmatch_core(const float *patches_flat, const float *volume, int 
                             const float *thresholds, int max_matches, int min_matches, int pow2,
                             int stabilize, int code, double count_scale,
                             const float *null_means, const float *null_vars, int null_max,
-                            const float *noise_ssd, float gamma, int nthreads, int64_t *sel_out,
-                            int32_t *count_out, int64_t *accepted_out) {
+                            const float *noise_ssd, float gamma, float pgamma, double pscale,
+                            int nthreads, int64_t *sel_out, int32_t *count_out,
+                            int64_t *accepted_out) {
   int P = (int)prod_int(block, ndim);
   int64_t cstr[ORACLE_MAX_NDIM], vstr[ORACLE_MAX_NDIM];
   c_strides(ndim, counts, cstr);
@@ -662,12 +729,25 @@ static void blockmatch_core(const float *patches_flat, const float *volume, int 
             ci = ci * (2 * radius[d] + 1) + (coord[d] - center[d] + radius[d]);
           dist = dist - gamma * noise_ssd[ci];
         }
-        int ok = stab ? (oracle_canonical_key(dist) < thr_key) : (dist < thr);
-        if (ok) {
-          ++accepted;
-          acc_size = insert_limited(acc_l, acc_d, max_matches, acc_size, lin, dist, stab);
+        if (pgamma != 0.0f) {
+          /* Poisson SSD correction: a float64 distance for every candidate
+           * but the reference itself */
+          double dd = (double)dist;
+          if (lin != center_lin) dd = dd - (double)pgamma * poisson_ssd_bias(refp, cp, P, pscale);
+          if (dd < (double)thr) {
+            ++accepted;
+            acc_size = insert_limited_mixed(acc_l, acc_d, max_matches, acc_size, lin, dd);
+          } else {
+            rej_size = insert_limited_mixed(rej_l, rej_d, required, rej_size, lin, dd);
+          }
         } else {
-          rej_size = insert_limited(rej_l, rej_d, required, rej_size, lin, dist, stab);
+          int ok = stab ? (oracle_canonical_key(dist) < thr_key) : (dist < thr);
+          if (ok) {
+            ++accepted;
+            acc_size = insert_limited(acc_l, acc_d, max_matches, acc_size, lin, dist, stab);
+          } else {
+            rej_size = insert_limited(rej_l, rej_d, required, rej_size, lin, dist, stab);
+          }
         }
         for (int d = ndim - 1; d >= 0; --d) {
           coord[d] += 1;
@@ -1805,7 +1885,7 @@ static int run_stage(const oracle_params *prm, int wiener, const float *match_vo
   oracle_schedule_generated(ndim, g.counts, block, step, prm->shift_density,
                             prm->schedule_density, refs, nrefs);
   float *thrs = (float *)malloc(sizeof(float) * (size_t)(nrefs + 1));
-  if (prm->gaussian) {
+  if (prm->gaussian || prm->poisson_ssd) {
     /* SSD thresholds scale with the matched volume's squared value range
      * (blockmatching.py:168-184): float32 max - min, 1 when not positive,
      * the product in float64, stored as float32 */
@@ -1828,6 +1908,12 @@ static int run_stage(const oracle_params *prm, int wiener, const float *match_vo
     oracle_blockmatch_ssd(match_vol, ndim, g.shape, g.counts, block, refs, nrefs, radius, thrs,
                           max_m, min_m, 1, wiener ? NULL : prm->ht_noise_ssd, prm->ht_gamma,
                           prm->nthreads, sel, cnt, NULL);
+  else if (prm->poisson_ssd)
+    /* stage 1 takes the Poisson gamma correction, stage 2 plain SSD
+     * (bmndalgo.py:292-294, 321-322, 728-741) */
+    oracle_blockmatch_ssd_poisson(match_vol, ndim, g.shape, g.counts, block, refs, nrefs, radius,
+                                  thrs, max_m, min_m, 1, wiener ? 0.0f : prm->ht_gamma,
+                                  prm->count_scale, prm->nthreads, sel, cnt, NULL);
   else
     oracle_blockmatch_standardized(NULL, match_vol, ndim, g.shape, g.counts, block, refs, nrefs,
                                    radius, thrs, max_m, min_m, 1, 1, code, prm->count_scale,

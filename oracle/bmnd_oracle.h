@@ -85,6 +85,16 @@ void oracle_blockmatch_ssd(const float *volume, int ndim, const int64_t *vol_sha
                            float gamma, int nthreads, int64_t *sel_out, int32_t *count_out,
                            int64_t *accepted_out);
 
+/* stage-1 SSD matching of the Poisson model with the Poisson gamma
+ * correction (blockmatching.py:432-443, 1140-1145): float64 corrected
+ * distances, ordered against the float32 values the candidate buffers keep */
+void oracle_blockmatch_ssd_poisson(const float *volume, int ndim, const int64_t *vol_shape,
+                                   const int64_t *counts, const int *block, const int64_t *refs,
+                                   int64_t nrefs, const int *radius, const float *thresholds,
+                                   int max_matches, int min_matches, int pow2, float gamma,
+                                   double count_scale, int nthreads, int64_t *sel_out,
+                                   int32_t *count_out, int64_t *accepted_out);
+
 /* ---- group statistics --------------------------------------------------- */
 void oracle_gaussian_group_variance(const float *maps, int P, int64_t D, const int64_t *pair_lag,
                                     const float *gmat, int n, int k, float *var_out);
@@ -232,6 +242,12 @@ typedef struct {
   const float *ht_noise_ssd;
   float ht_gamma;
   double global_sigma;
+  /* the SSD distance under the Poisson model (blockmatch_distance "ssd",
+   * bmndalgo.py:285-322, 721-741): thresholds ht_match_d / wi_match_d times
+   * the squared value range of the matched volume as above, stage 1 with the
+   * Poisson gamma correction (ht_gamma, count_scale; 0: none), stage 2 plain
+   * SSD; everything else is the Poisson path */
+  int poisson_ssd;
 } oracle_params;
 
 /* Runs both stages.  pilot may be NULL.  The match lists of each stage can

@@ -126,10 +126,20 @@ def test_engine_distance_and_policy_support():
             prof.blockmatch_distance = dist
             prof.poisson_ht_match_policy = policy
             check_supported(prof, resolve_profile(prof, (32, 32)))
+    # SSD under the Poisson model: the fixed policy only (the reference's
+    # validation refuses the others, profilevalidation.py:414-425)
     prof.blockmatch_distance = "ssd"
     prof.poisson_ht_match_policy = PoissonHTMatchPolicy.FIXED
-    with pytest.raises(NotImplementedError, match="distances"):
-        check_supported(prof, resolve_profile(prof, (32, 32)))
+    check_supported(prof, resolve_profile(prof, (32, 32)))
+    pipe = PoissonPipeline(prof, (32, 32), "cpu")
+    assert pipe.blockmatch_mode(None, "ht") == (11, 0)  # gamma 3: the Poisson correction
+    assert pipe.blockmatch_mode(None, "wiener") == (10, 0)
+    assert not pipe.graph_supported()
+    prof.gamma = 0.0
+    assert PoissonPipeline(prof, (32, 32), "cpu").blockmatch_mode(None, "ht") == (10, 0)
+    prof.poisson_ht_match_policy = PoissonHTMatchPolicy.REFERENCE_FINITE_COUNT
+    with pytest.raises(ValueError, match="requires"):
+        resolve_profile(prof, (32, 32))
 
 
 @pytest.mark.parametrize("dist,policy,ht,wi", [

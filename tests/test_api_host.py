@@ -88,14 +88,26 @@ def test_all_zero_volume_fast_path_runs_no_kernel():
 
 
 @pytest.mark.parametrize("fields", [
-    {"blockmatch_distance": "ssd"},
+    {"blockmatch_distance": "poisson_deviance"},
+    {"ht_weight_domain": "windowed_synthesis", "ht_weight_scope": "group"},
 ])
 def test_unsupported_options_raise_before_any_gpu_work(fields):
-    p = _poisson()
+    # options the Gaussian model does not run on the device
+    p = BM3DProfile()
     for field, value in fields.items():
         setattr(p, field, value)
     with pytest.raises(NotImplementedError):
-        bmnd(np.ones((32, 32)), p)
+        bmnd(np.ones((32, 32)), p, sigma=0.1)
+
+
+def test_poisson_ssd_is_supported():
+    # the SSD distance under the Poisson model runs on the device (stage 1
+    # with the Poisson gamma correction, bmndalgo.py:292-294, 321-322)
+    from bmnd_amd.engine import check_supported
+    from bmnd_amd.profilevalidation import resolve_profile
+    p = _poisson()
+    p.blockmatch_distance = "ssd"
+    check_supported(p, resolve_profile(p, (32, 32)))
 
 
 def test_exact_group_risk_is_selected_for_its_options():

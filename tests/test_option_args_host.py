@@ -103,6 +103,18 @@ def test_ssd_matching_arguments(gamma, noise, thr, msg):
     assert msg in _error(rc)
 
 
+@pytest.mark.parametrize("gamma,scale,thr,msg", [
+    (-1.0, 1.0, 1.0, "gamma must be finite"), (float("inf"), 1.0, 1.0, "gamma must be finite"),
+    (3.0, 0.0, 1.0, "poisson_count_scale must be finite"),
+    (3.0, float("nan"), 1.0, "poisson_count_scale must be finite"),
+    (3.0, 1.0, -1.0, "finite and non-negative")])
+def test_poisson_ssd_matching_arguments(gamma, scale, thr, msg):
+    rc = _lib.lib().bmnd_hip_blockmatch_ssd_poisson(DUMMY, SHAPE, BLOCK, RADIUS, DUMMY, 10, thr,
+                                                    16, 2, 1, gamma, scale, DUMMY, DUMMY, DUMMY,
+                                                    None)
+    assert msg in _error(rc)
+
+
 @pytest.mark.parametrize("domain,msg", [((0, 4, 4), "positive"), ((4096, 4096, 4096), "too large")])
 def test_gaussian_variance_arguments(domain, msg):
     rc = _lib.lib().bmnd_hip_gaussian_variance(DUMMY, I32X3(*domain), SHAPE, BLOCK, RADIUS, DUMMY,
