@@ -18,7 +18,11 @@
  *    expression through the float64 `1.0/poisson_count_scale`,
  *    blockmatching.py:338-371) with the C library's log();
  *  - risks are float64, weights float32 (aggregationweights.py:213-259,
- *    616-630); scatter, division and variance sums are float32.
+ *    616-630); scatter, division and variance sums are float32;
+ *  - the SSD distance is a float32 sum; under the Poisson gamma correction
+ *    the corrected distance is a float64 (the bias is a float64 sum), ordered
+ *    against the float32 values the candidate buffers keep
+ *    (blockmatching.py:432-443, 1099-1102, 1140-1145).
  *
  * Parity pins: every function here is checked in tests/test_oracle_*.py
  * against the reference's own known answers (blockmatching, shifts,
