@@ -3949,6 +3949,29 @@ struct AggArgs {
   int norm_mode;
 };
 
+// q = x / d and r = x % d for 0 <= x < 2^31 and d >= 1, from the float64
+// reciprocal id = 1 / d: x * id is within (x / d) 2^-52 of x / d, less than
+// the 1 / d that separates x / d from the next whole number above or below
+// it unless x / d is whole, so its truncation is q, or q - 1 when x / d is
+// whole, which one step corrects.
+__device__ __forceinline__ void divmod_small(int x, int d, double id, int &q, int &r) {
+  q = (int)((double)x * id);
+  r = x - q * d;
+  if (r >= d) {
+    ++q;
+    r -= d;
+  }
+}
+
+// patch-grid coordinates (m0, m1, m2) of linear index lin, grid extents c1,
+// c2 along the last two axes (ic1, ic2 their float64 reciprocals)
+__device__ __forceinline__ void agg_decode(int lin, int c1, int c2, double ic1, double ic2,
+                                           int &m0, int &m1, int &m2) {
+  int t;
+  divmod_small(lin, c2, ic2, t, m2);
+  divmod_small(t, c1, ic1, m0, m1);
+}
+
 constexpr int kAggThreads = 512;
 constexpr int kAggGroupCap = 4096;
 constexpr int kAggEntryCap = 2048;
@@ -4001,6 +4024,8 @@ __global__ __launch_bounds__(T0 * T1 * T2) void k_aggregate(AggArgs a) {
   }
   const int nt2 = (a.n2 + T2 - 1) / T2, nt1 = (a.n1 + T1 - 1) / T1;
   const int tb = (int)tball;
+  // reciprocals of the patch-grid extents for the member decodes
+  const double ic1 = 1.0 / (double)a.c1, ic2 = 1.0 / (double)a.c2;
   const int tx2 = tb % nt2, tx1 = (tb / nt2) % nt1, tx0 = tb / (nt2 * nt1);
   const int t_lo0 = tx0 * T0, t_lo1 = tx1 * T1, t_lo2 = tx2 * T2;
   const int t_hi0 = min(t_lo0 + T0, a.n0) - 1, t_hi1 = min(t_lo1 + T1, a.n1) - 1,
@@ -4084,7 +4109,8 @@ __global__ __launch_bounds__(T0 * T1 * T2) void k_aggregate(AggArgs a) {
       const int cnt = a.counts[gsel];
       for (int i = 0; i < cnt; ++i) {
         const int lin = a.sel[(int64_t)gsel * a.max_m + i];
-        const int m2 = lin % a.c2, m1 = (lin / a.c2) % a.c1, m0 = lin / (a.c2 * a.c1);
+        int m0, m1, m2;
+        agg_decode(lin, a.c1, a.c2, ic1, ic2, m0, m1, m2);
         const bool hit = m0 <= t_hi0 && m0 + B0 - 1 >= t_lo0 && m1 <= t_hi1 &&
                          m1 + B1 - 1 >= t_lo1 && m2 <= t_hi2 && m2 + B2 - 1 >= t_lo2;
         mcount += hit ? 1 : 0;
@@ -4105,7 +4131,8 @@ __global__ __launch_bounds__(T0 * T1 * T2) void k_aggregate(AggArgs a) {
       int w = ex;
       for (int i = 0; i < cnt; ++i) {
         const int lin = a.sel[(int64_t)gsel * a.max_m + i];
-        const int m2 = lin % a.c2, m1 = (lin / a.c2) % a.c1, m0 = lin / (a.c2 * a.c1);
+        int m0, m1, m2;
+        agg_decode(lin, a.c1, a.c2, ic1, ic2, m0, m1, m2);
         const bool hit = m0 <= t_hi0 && m0 + B0 - 1 >= t_lo0 && m1 <= t_hi1 &&
                          m1 + B1 - 1 >= t_lo1 && m2 <= t_hi2 && m2 + B2 - 1 >= t_lo2;
         if (hit) {
